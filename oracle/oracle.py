@@ -1,0 +1,360 @@
+"""ctypes wrapper around the FP64 oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; it is the checker, never the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(HERE, "libphaser_oracle.so")
+_lib = None
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+class Config(C.Structure):
+    _fields_ = [("bw", C.c_int), ("n_voxels", C.c_int), ("lower", C.c_int),
+                ("upper", C.c_int), ("lp_lower", C.c_int), ("lp_upper", C.c_int),
+                ("use_kdtree", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("rot_index", C.c_uint32), ("trans_index", C.c_uint32),
+                ("rot_value", C.c_double), ("trans_value", C.c_double),
+                ("zyz", C.c_double * 3), ("quat", C.c_double * 4),
+                ("euler_xyz", C.c_double * 3), ("T_rot", C.c_float * 16),
+                ("translation", C.c_double * 3), ("t_sample_ms", C.c_double),
+                ("t_rot_ms", C.c_double), ("t_trans_ms", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            from oracle import build as _b  # noqa: F401
+            _b.build()
+        L = C.CDLL(_LIB_PATH)
+        sig = {
+            "po_fft": (None, [dp, C.c_int, C.c_int64, C.c_int]),
+            "po_sampler_grid": (None, [C.c_int, fp]),
+            "po_project": (None, [fp, C.c_int64, fp, dp]),
+            "po_nn_brute": (None, [fp, C.c_int64, fp, C.c_int, C.c_float, i32p]),
+            "po_nn_kdtree": (None, [fp, C.c_int64, fp, C.c_int, C.c_float, i32p]),
+            "po_squared_voxel_size": (C.c_float, [C.c_int, C.c_int, C.c_int]),
+            "po_sample_cloud": (None, [C.c_int, fp, C.c_int64, C.c_float, C.c_int, dp]),
+            "po_seanindex": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+            "po_makeweights": (None, [C.c_int, dp]),
+            "po_legendre": (None, [C.c_int, C.c_int, dp]),
+            "po_fst": (None, [C.c_int, dp, dp, dp]),
+            "po_pyramid_struct": (None, [C.c_uint32, C.c_int, C.c_float, u32p, u32p, u32p]),
+            "po_fuse_channels": (None, [dp, C.c_int, C.c_uint32, C.c_int, C.c_float, dp]),
+            "po_fusion_modes": (None, [C.c_uint32, C.c_int, C.c_float, u8p]),
+            "po_fftshift": (None, [dp, C.c_size_t]),
+            "po_ifftshift": (None, [dp, C.c_size_t]),
+            "po_fftshift_real": (None, [dp, C.c_size_t]),
+            "po_ifftshift_real": (None, [dp, C.c_size_t]),
+            "po_wigner_d": (None, [C.c_int, C.c_int, C.c_int, dp]),
+            "po_so3_correlate": (None, [C.c_int, dp, dp, dp, dp, C.c_void_p, dp]),
+            "po_so3_synthesis": (None, [C.c_int, dp, dp, dp, dp, dp]),
+            "po_spherical_correlate_laplace": (None, [C.c_int, C.c_int, dp, dp, dp, C.c_void_p, C.c_void_p]),
+            "po_spherical_correlate_blend": (None, [C.c_int, dp, dp, dp, dp, dp]),
+            "po_linspaced": (None, [C.c_int, C.c_float, C.c_float, fp]),
+            "po_histc": (C.c_int, [C.c_double, fp, C.c_int]),
+            "po_voxelize": (None, [fp, C.c_int64, C.c_int, C.c_float, C.c_float, dp, C.c_void_p]),
+            "po_r2c3": (None, [C.c_int, dp, dp]),
+            "po_c2r3": (None, [C.c_int, dp, dp]),
+            "po_spatial_correlate": (None, [C.c_int, C.c_int, dp, dp, dp]),
+            "po_spatial_correlate_laplace": (None, [C.c_int, C.c_int, dp, dp, dp]),
+            "po_argmax": (C.c_uint32, [dp, C.c_size_t]),
+            "po_zyz_from_index": (None, [C.c_uint32, C.c_int, dp]),
+            "po_quat_from_zyz": (None, [dp, dp]),
+            "po_xyz_from_quat": (None, [dp, dp]),
+            "po_transform_xyz": (None, [C.c_float, C.c_float, C.c_float, fp]),
+            "po_apply_transform": (None, [fp, fp, C.c_int64, fp]),
+            "po_translation_from_index": (C.c_double, [C.c_double, C.c_uint32, C.c_int, C.c_int]),
+            "po_ind2sub": (None, [C.c_uint32, C.c_uint32, C.c_uint32, u32p]),
+            "po_sub2ind": (C.c_uint32, [C.c_uint32] * 5),
+            "po_register": (None, [C.POINTER(Config), fp, C.c_int64, fp, C.c_int64,
+                                   C.POINTER(Result), C.c_void_p, C.c_void_p, C.c_void_p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---- thin numpy-level helpers -------------------------------------------
+def fft(x, sign=-1):
+    a = _c(np.asarray(x, np.complex128).view(np.float64), np.float64).copy()
+    lib().po_fft(a, len(a) // 2, 1, sign)
+    return a.view(np.complex128)
+
+
+def sampler_grid(bw):
+    out = np.zeros((4 * bw * bw, 3), np.float32)
+    lib().po_sampler_grid(bw, out)
+    return out
+
+
+def project(pts):
+    pts = _c(pts, np.float32)
+    P = pts.shape[0]
+    proj = np.zeros((P, 3), np.float32)
+    rng = np.zeros(P, np.float64)
+    lib().po_project(pts, P, proj, rng)
+    return proj, rng
+
+
+def nn(proj, queries, max_sq=1e30, kdtree=False):
+    proj = _c(proj, np.float32)
+    queries = _c(queries, np.float32)
+    out = np.zeros(queries.shape[0], np.int32)
+    f = lib().po_nn_kdtree if kdtree else lib().po_nn_brute
+    f(proj, proj.shape[0], queries, queries.shape[0], max_sq, out)
+    return out
+
+
+def squared_voxel_size(lp_lower=0, lp_upper=100000, n_voxels=100):
+    return lib().po_squared_voxel_size(lp_lower, lp_upper, n_voxels)
+
+
+def sample_cloud(bw, pts, max_sq=None, kdtree=True):
+    pts = _c(pts, np.float32)
+    if max_sq is None:
+        max_sq = squared_voxel_size()
+    out = np.zeros((4, 4 * bw * bw), np.float64)
+    lib().po_sample_cloud(bw, pts, pts.shape[0], max_sq, int(kdtree), out)
+    return out
+
+
+def seanindex(m, l, bw):
+    return lib().po_seanindex(m, l, bw)
+
+
+def makeweights(bw):
+    w = np.zeros(4 * bw, np.float64)
+    lib().po_makeweights(bw, w)
+    return w
+
+
+def legendre(bw, m):
+    out = np.zeros((bw - m, 2 * bw), np.float64)
+    lib().po_legendre(bw, m, out)
+    return out
+
+
+def fst(bw, data):
+    data = _c(data, np.float64).reshape(-1)
+    re = np.zeros(bw * bw)
+    im = np.zeros(bw * bw)
+    lib().po_fst(bw, data, re, im)
+    return re + 1j * im
+
+
+def pyramid_struct(n, levels, div=4.0):
+    a = np.zeros(levels, np.uint32)
+    b = np.zeros(levels, np.uint32)
+    c = np.zeros(levels, np.uint32)
+    lib().po_pyramid_struct(n, levels, div, a, b, c)
+    return a, b, c
+
+
+def fuse_channels(channels, levels=5, div=4.0):
+    ch = _c(np.asarray(channels, np.complex128), np.complex128)
+    Cn, n = ch.shape
+    out = np.zeros(n, np.complex128)
+    lib().po_fuse_channels(ch.view(np.float64), Cn, n, levels, div, out.view(np.float64))
+    return out
+
+
+def fusion_modes(n, levels=5, div=4.0):
+    out = np.zeros(n, np.uint8)
+    lib().po_fusion_modes(n, levels, div, out)
+    return out
+
+
+def fftshift(x):
+    a = _c(np.asarray(x, np.complex128), np.complex128).copy()
+    lib().po_fftshift(a.view(np.float64), len(a))
+    return a
+
+
+def ifftshift(x):
+    a = _c(np.asarray(x, np.complex128), np.complex128).copy()
+    lib().po_ifftshift(a.view(np.float64), len(a))
+    return a
+
+
+def wigner_d(bw, m1, m2):
+    l0 = max(abs(m1), abs(m2))
+    out = np.zeros((bw - l0, 2 * bw), np.float64)
+    lib().po_wigner_d(bw, m1, m2, out)
+    return out
+
+
+def so3_correlate(bw, f, h, want_complex=False):
+    f = _c(f, np.complex128)
+    h = _c(h, np.complex128)
+    n = 2 * bw
+    mag = np.zeros(n ** 3)
+    cp = np.zeros(n ** 3, np.complex128) if want_complex else None
+    lib().po_so3_correlate(bw, _c(f.real, np.float64), _c(f.imag, np.float64),
+                           _c(h.real, np.float64), _c(h.imag, np.float64),
+                           _ptr(cp.view(np.float64)) if cp is not None else None, mag)
+    return (mag, cp) if want_complex else mag
+
+
+def so3_synthesis(bw, f, h):
+    n = 2 * bw
+    S = np.zeros((n, n, n), np.complex128)
+    f = _c(f, np.complex128)
+    h = _c(h, np.complex128)
+    lib().po_so3_synthesis(bw, _c(f.real, np.float64), _c(f.imag, np.float64),
+                           _c(h.real, np.float64), _c(h.imag, np.float64),
+                           S.view(np.float64).reshape(-1))
+    return S
+
+
+def spherical_correlate_laplace(bw, f, h, fused=False):
+    f = _c(f, np.float64)
+    h = _c(h, np.float64)
+    Cn = f.shape[0]
+    mag = np.zeros((2 * bw) ** 3)
+    ff = np.zeros(bw * bw, np.complex128)
+    hf = np.zeros(bw * bw, np.complex128)
+    lib().po_spherical_correlate_laplace(bw, Cn, f, h, mag, _ptr(ff.view(np.float64)),
+                                         _ptr(hf.view(np.float64)))
+    return (mag, ff, hf) if fused else mag
+
+
+def linspaced(N, lo, hi):
+    E = np.zeros(N, np.float32)
+    lib().po_linspaced(N, lo, hi, E)
+    return E
+
+
+def voxelize(pts, N, lo=-50.0, hi=50.0, hist=False):
+    pts = _c(pts, np.float32)
+    ch = np.zeros((4, N ** 3))
+    hs = np.zeros(N ** 3) if hist else None
+    lib().po_voxelize(pts, pts.shape[0], N, lo, hi, ch, _ptr(hs))
+    return (ch, hs) if hist else ch
+
+
+def r2c3(x):
+    x = _c(x, np.float64)
+    N = round(len(x.reshape(-1)) ** (1 / 3))
+    out = np.zeros(N ** 3, np.complex128)
+    lib().po_r2c3(N, x.reshape(-1), out.view(np.float64))
+    return out
+
+
+def c2r3(X, N):
+    X = _c(X, np.complex128)
+    out = np.zeros(N ** 3)
+    lib().po_c2r3(N, X.view(np.float64), out)
+    return out
+
+
+def spatial_correlate(f, g, N, pad=0):
+    out = np.zeros((N + 2 * pad) ** 3)
+    lib().po_spatial_correlate(N, pad, _c(f, np.float64), _c(g, np.float64), out)
+    return out
+
+
+def spatial_correlate_laplace(f, g, N):
+    f = _c(f, np.float64)
+    g = _c(g, np.float64)
+    out = np.zeros(N ** 3)
+    lib().po_spatial_correlate_laplace(N, f.shape[0], f, g, out)
+    return out
+
+
+def zyz_from_index(idx, bw):
+    z = np.zeros(3)
+    lib().po_zyz_from_index(idx, bw, z)
+    return z
+
+
+def quat_from_zyz(zyz):
+    q = np.zeros(4)
+    lib().po_quat_from_zyz(_c(zyz, np.float64), q)
+    return q
+
+
+def xyz_from_quat(q):
+    e = np.zeros(3)
+    lib().po_xyz_from_quat(_c(q, np.float64), e)
+    return e
+
+
+def transform_xyz(a, b, c):
+    T = np.zeros(16, np.float32)
+    lib().po_transform_xyz(a, b, c, T)
+    return T.reshape(4, 4)
+
+
+def apply_transform(T, pts):
+    pts = _c(pts, np.float32)
+    out = np.zeros_like(pts)
+    lib().po_apply_transform(_c(T, np.float32).reshape(-1), pts, pts.shape[0], out)
+    return out
+
+
+def translation_from_index(idx, N, lo=-50, hi=50):
+    return lib().po_translation_from_index(float(idx), N, lo, hi)
+
+
+def ind2sub(idx, rows, cols):
+    out = np.zeros(3, np.uint32)
+    lib().po_ind2sub(idx, rows, cols, out)
+    return out
+
+
+def sub2ind(i, j, k, rows, cols):
+    return lib().po_sub2ind(i, j, k, rows, cols)
+
+
+def register(target, source, bw=32, n_voxels=64, lower=-50, upper=50,
+             kdtree=True, want_grids=False, want_reg=False):
+    cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree))
+    t = _c(target, np.float32)
+    s = _c(source, np.float32)
+    res = Result()
+    rot = np.zeros((2 * bw) ** 3) if want_grids else None
+    tr = np.zeros(n_voxels ** 3) if want_grids else None
+    reg = np.zeros_like(s) if want_reg else None
+    lib().po_register(C.byref(cfg), t, t.shape[0], s, s.shape[0], C.byref(res),
+                      _ptr(reg), _ptr(rot), _ptr(tr))
+    out = {
+        "rot_index": res.rot_index, "trans_index": res.trans_index,
+        "rot_value": res.rot_value, "trans_value": res.trans_value,
+        "zyz": np.array(res.zyz[:]), "quat": np.array(res.quat[:]),
+        "euler_xyz": np.array(res.euler_xyz[:]),
+        "T_rot": np.array(res.T_rot[:], np.float32).reshape(4, 4),
+        "translation": np.array(res.translation[:]),
+        "t_ms": (res.t_sample_ms, res.t_rot_ms, res.t_trans_ms),
+    }
+    if want_grids:
+        out["rot_corr"] = rot
+        out["trans_corr"] = tr
+    if want_reg:
+        out["registered"] = reg
+    return out
