@@ -1,0 +1,131 @@
+/*
+ * phaser_amd.h -- C ABI of the MI355X correlation core of PHASER.
+ *
+ * Plain C types only.  Every call returns 0 on success, a negative status
+ * on error (-1 invalid argument, -2 HIP runtime error, -3 out of capacity);
+ * phz_last_error() returns the message of the calling thread's last error.
+ * A context owns one HIP device, one stream, the resident tables for its
+ * bandwidth and voxel count, and all device workspaces.  A context is not
+ * thread-safe: use one per host thread (the reference's objects are not
+ * thread-safe either, spherical-correlation.h:19-77).
+ *
+ * Point clouds are arrays of float quadruples (x, y, z, intensity), the
+ * fields of the reference's PCL PointXYZI (point-types.h:10-14).
+ * Arguments named d_* are device pointers; all others are host pointers.
+ *
+ * References below are paths in the PHASER repository.
+ */
+#ifndef PHASER_AMD_H_
+#define PHASER_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phz_ctx phz_ctx;
+
+/* Flags of core-gflags.cc:6-39 plus capacities. */
+typedef struct {
+  int32_t bandwidth;              /* phaser_core_spherical_bandwidth */
+  int32_t spherical_zero_padding; /* phaser_core_spherical_zero_padding (0 only) */
+  int32_t n_voxels;               /* phaser_core_spatial_n_voxels */
+  int32_t discretize_lower;       /* phaser_core_spatial_discretize_lower */
+  int32_t discretize_upper;       /* phaser_core_spatial_discretize_upper */
+  int32_t spatial_zero_padding;   /* phaser_core_spatial_zero_padding (0 only) */
+  int32_t low_pass_lower;         /* phaser_core_spatial_low_pass_lower_bound */
+  int32_t low_pass_upper;         /* phaser_core_spatial_low_pass_upper_bound */
+  int64_t max_points;             /* capacity per cloud */
+  int32_t max_batch;              /* pairs per batched call */
+  int32_t device;                 /* HIP device ordinal */
+} phz_config;
+
+/* The pose parts of model::RegistrationResult (registration-result.cc:13-99):
+ * the rotation argmax and its Bingham mode, the translation argmax and its
+ * Gaussian mean (single-sample fits at the default 0 peak neighbours). */
+typedef struct {
+  uint32_t rot_index;     /* argmax of the (2B)^3 SO(3) grid [beta][alpha][gamma] */
+  uint32_t trans_index;   /* argmax of the N^3 grid, x*N + y + N^2*z */
+  float rot_value;        /* |C|^2 at rot_index */
+  float trans_value;      /* correlation at trans_index */
+  double zyz[3];          /* rotation-utils.cc:197-207 */
+  double quat[4];         /* w,x,y,z, rotation-utils.cc:101-107 */
+  double euler_xyz[3];    /* rotation-utils.cc:86-99 */
+  float T_rot[16];        /* row-major, applied to the source (RotateAroundXYZ) */
+  double translation[3];  /* translation-utils.cc:40-52 */
+} phz_result;
+
+/* Defaults of core-gflags.cc (bandwidth 75 there; 64 here so that the
+ * default is the benchmarked configuration), max_points 262144. */
+void phz_default_config(phz_config* cfg);
+/* Replaces the constructors of SphOptRegistration (sph-opt-registration.cc:
+ * 21-31), SphericalCorrelation (spherical-correlation.cc:21-28) and
+ * PhaseAligner (phase-aligner.cc:19-45): builds every table once. */
+int phz_create(const phz_config* cfg, phz_ctx** out);
+void phz_destroy(phz_ctx* ctx);
+const char* phz_last_error(void);
+int phz_get_config(const phz_ctx* ctx, phz_config* out);
+
+/* ---- end to end -------------------------------------------------------- */
+/* SphOptRegistration::registerPointCloud(cloud_prev = target,
+ * cloud_cur = source) (sph-opt-registration.cc:34-47).  rot_corr ((2B)^3
+ * floats, |C|^2), trans_corr (N^3 floats) and registered (Ps x 4 floats,
+ * the registered source) are optional outputs (NULL to skip). */
+int phz_register(phz_ctx* ctx, const float* target, int64_t Pt, const float* source,
+                 int64_t Ps, phz_result* result, float* rot_corr, float* trans_corr,
+                 float* registered);
+/* The same on device-resident clouds and outputs.  stream: a hipStream_t
+ * (NULL = the context's stream).  Returns after the result is on the host. */
+int phz_register_device(phz_ctx* ctx, const float* d_target, int64_t Pt,
+                        const float* d_source, int64_t Ps, phz_result* result,
+                        float* d_rot_corr, float* d_trans_corr, float* d_registered,
+                        void* stream);
+/* n independent pairs in one pass (n <= max_batch).  If shared_target is
+ * nonzero only d_targets[0]/Pt[0] is read and every source registers against
+ * it; the target's spectra are then computed once.  d_rot_corr/d_trans_corr
+ * (optional) hold n consecutive grids. */
+int phz_register_pairs_device(phz_ctx* ctx, int n, int shared_target,
+                              const float* const* d_targets, const int64_t* Pt,
+                              const float* const* d_sources, const int64_t* Ps,
+                              phz_result* results, float* d_rot_corr, float* d_trans_corr,
+                              void* stream);
+
+/* ---- stages (host buffers) ---------------------------------------------- */
+/* SphericalSampler::sampleUniformly + the range and intensity channels of
+ * SphericalCombinedWorker::run (spherical-sampler.cc:20-27,
+ * spherical-combined-worker.cc:19-57).  grids: 2 x (2B)^2 floats
+ * {range, intensity}, theta-major. */
+int phz_sample_s2(phz_ctx* ctx, const float* pts, int64_t P, float* grids);
+/* FST_semi_memo over nsig real grids (s2_semi_memo.c:123-375).
+ * coef: nsig x B^2 complex (interleaved re,im) in seanindex order. */
+int phz_s2_transform(phz_ctx* ctx, int nsig, const float* grids, float* coef);
+/* SphericalCorrelation::correlate + inverseTransform on given coefficients
+ * (spherical-correlation.cc:248-278): so3CombineCoef_fftw +
+ * Inverse_SO3_Naive_fftw + |.|^2.  corr ((2B)^3) and synthesis (the
+ * (m1,m2>=0,beta) half-plane, [m2][m1+B-1][k] complex) are optional. */
+int phz_so3_correlate_coef(phz_ctx* ctx, const float* f_coef, const float* h_coef,
+                           float* corr, float* synthesis, uint32_t* argmax);
+/* SphericalCorrelationLaplace::correlateSampledSignals
+ * (spherical-correlation-laplace.cc:14-40): C channel grids per signal
+ * (2 <= C <= 8).  corr optional. */
+int phz_spherical_correlate(phz_ctx* ctx, int C, const float* f_grids, const float* h_grids,
+                            float* corr, uint32_t* argmax);
+/* PhaseAligner::discretizePointcloud (phase-aligner.cc:71-131): means of
+ * intensity and range per voxel, out: 2 x N^3 {intensity, range}; counts
+ * (N^3 floats) optional. */
+int phz_voxelize(phz_ctx* ctx, const float* pts, int64_t P, float* out, float* counts);
+/* SpatialCorrelationLaplace::correlateSignals (spatial-correlation-laplace.
+ * cc:17-51): C voxel channels per signal (2 <= C <= 8), corr N^3 optional. */
+int phz_spatial_correlate(phz_ctx* ctx, int C, const float* f, const float* g, float* corr,
+                          uint32_t* argmax);
+/* SpatialCorrelation::correlateSignals (spatial-correlation.cc:163-206),
+ * single channel, no padding.  n_voxels may differ from the context's
+ * (any size whose factors are 2, 3 and 5, <= 256). */
+int phz_spatial_correlate_single(phz_ctx* ctx, int n_voxels, const float* f, const float* g,
+                                 float* corr, uint32_t* argmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHASER_AMD_H_ */

@@ -1,0 +1,232 @@
+"""ctypes binding of the C ABI in include/phaser_amd.h.
+
+Loads the in-tree lib/libphaser_amd.so and fails loudly when it is missing:
+there is no CPU fallback on the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libphaser_amd.so")
+_lib = None
+
+
+class Config(C.Structure):
+    _fields_ = [("bandwidth", C.c_int32), ("spherical_zero_padding", C.c_int32),
+                ("n_voxels", C.c_int32), ("discretize_lower", C.c_int32),
+                ("discretize_upper", C.c_int32), ("spatial_zero_padding", C.c_int32),
+                ("low_pass_lower", C.c_int32), ("low_pass_upper", C.c_int32),
+                ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("rot_index", C.c_uint32), ("trans_index", C.c_uint32),
+                ("rot_value", C.c_float), ("trans_value", C.c_float),
+                ("zyz", C.c_double * 3), ("quat", C.c_double * 4),
+                ("euler_xyz", C.c_double * 3), ("T_rot", C.c_float * 16),
+                ("translation", C.c_double * 3)]
+
+    def as_dict(self):
+        return {
+            "rot_index": int(self.rot_index), "trans_index": int(self.trans_index),
+            "rot_value": float(self.rot_value), "trans_value": float(self.trans_value),
+            "zyz": np.array(self.zyz[:]), "quat": np.array(self.quat[:]),
+            "euler_xyz": np.array(self.euler_xyz[:]),
+            "T_rot": np.array(self.T_rot[:], np.float32).reshape(4, 4),
+            "translation": np.array(self.translation[:]),
+        }
+
+
+EXPORTS = {
+    "phz_default_config": (None, [C.POINTER(Config)]),
+    "phz_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "phz_destroy": (None, [C.c_void_p]),
+    "phz_last_error": (C.c_char_p, []),
+    "phz_get_config": (C.c_int, [C.c_void_p, C.POINTER(Config)]),
+    "phz_register": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                               C.POINTER(Result), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "phz_register_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                      C.POINTER(Result), C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]),
+    "phz_register_pairs_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.POINTER(Result),
+                                            C.c_void_p, C.c_void_p, C.c_void_p]),
+    "phz_sample_s2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
+    "phz_s2_transform": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "phz_so3_correlate_coef": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.POINTER(C.c_uint32)]),
+    "phz_spherical_correlate": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.POINTER(C.c_uint32)]),
+    "phz_voxelize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "phz_spatial_correlate": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.POINTER(C.c_uint32)]),
+    "phz_spatial_correlate_single": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.POINTER(C.c_uint32)]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(phaser_amd has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class PhaserError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise PhaserError(f"phaser_amd error {rc}: {lib().phz_last_error().decode()}")
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def default_config(**kw):
+    cfg = Config()
+    lib().phz_default_config(C.byref(cfg))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+class Context:
+    """Owns one phz_ctx (device, stream, resident tables, workspaces)."""
+
+    def __init__(self, **kw):
+        self.cfg = default_config(**kw)
+        h = C.c_void_p()
+        _check(lib().phz_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self.B = self.cfg.bandwidth
+        self.n = 2 * self.B
+        self.N = self.cfg.n_voxels
+
+    def close(self):
+        if self.h:
+            lib().phz_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- end to end
+    def register(self, target, source, grids=False, registered=False):
+        t, s = _f32(target), _f32(source)
+        res = Result()
+        rc = np.zeros(self.n ** 3, np.float32) if grids else None
+        tc = np.zeros(self.N ** 3, np.float32) if grids else None
+        reg = np.zeros_like(s) if registered else None
+        _check(lib().phz_register(self.h, _p(t), t.shape[0], _p(s), s.shape[0], C.byref(res),
+                                  _p(rc), _p(tc), _p(reg)))
+        out = res.as_dict()
+        if grids:
+            out["rot_corr"] = rc
+            out["trans_corr"] = tc
+        if registered:
+            out["registered"] = reg
+        return out
+
+    def register_device(self, d_target, Pt, d_source, Ps, d_rot=0, d_trans=0, d_reg=0, stream=0):
+        res = Result()
+        _check(lib().phz_register_device(self.h, C.c_void_p(d_target), Pt, C.c_void_p(d_source),
+                                         Ps, C.byref(res), C.c_void_p(d_rot or None),
+                                         C.c_void_p(d_trans or None), C.c_void_p(d_reg or None),
+                                         C.c_void_p(stream or None)))
+        return res.as_dict()
+
+    def register_pairs_device(self, d_targets, Pt, d_sources, Ps, shared_target=False,
+                              d_rot=0, d_trans=0, stream=0):
+        n = len(d_sources)
+        T = (C.c_void_p * len(d_targets))(*d_targets)
+        S = (C.c_void_p * n)(*d_sources)
+        PT = (C.c_int64 * len(Pt))(*Pt)
+        PS = (C.c_int64 * n)(*Ps)
+        res = (Result * n)()
+        _check(lib().phz_register_pairs_device(self.h, n, int(shared_target), T, PT, S, PS, res,
+                                               C.c_void_p(d_rot or None),
+                                               C.c_void_p(d_trans or None),
+                                               C.c_void_p(stream or None)))
+        return [r.as_dict() for r in res]
+
+    # ---- stages
+    def sample_s2(self, pts):
+        pts = _f32(pts)
+        out = np.zeros((2, self.n * self.n), np.float32)
+        _check(lib().phz_sample_s2(self.h, _p(pts), pts.shape[0], _p(out)))
+        return out
+
+    def s2_transform(self, grids):
+        g = _f32(grids).reshape(-1, self.n * self.n)
+        out = np.zeros((g.shape[0], self.B * self.B), np.complex64)
+        _check(lib().phz_s2_transform(self.h, g.shape[0], _p(g), _p(out)))
+        return out
+
+    def so3_correlate_coef(self, f, h, synthesis=False):
+        f = np.ascontiguousarray(f, np.complex64)
+        h = np.ascontiguousarray(h, np.complex64)
+        corr = np.zeros(self.n ** 3, np.float32)
+        syn = np.zeros(self.B * (2 * self.B - 1) * self.n, np.complex64) if synthesis else None
+        am = C.c_uint32()
+        _check(lib().phz_so3_correlate_coef(self.h, _p(f), _p(h), _p(corr), _p(syn), C.byref(am)))
+        return (corr, int(am.value), syn) if synthesis else (corr, int(am.value))
+
+    def spherical_correlate(self, f_grids, h_grids):
+        f = _f32(f_grids)
+        h = _f32(h_grids)
+        Cn = f.shape[0]
+        corr = np.zeros(self.n ** 3, np.float32)
+        am = C.c_uint32()
+        _check(lib().phz_spherical_correlate(self.h, Cn, _p(f), _p(h), _p(corr), C.byref(am)))
+        return corr, int(am.value)
+
+    def voxelize(self, pts):
+        pts = _f32(pts)
+        out = np.zeros((2, self.N ** 3), np.float32)
+        cnt = np.zeros(self.N ** 3, np.float32)
+        _check(lib().phz_voxelize(self.h, _p(pts), pts.shape[0], _p(out), _p(cnt)))
+        return out, cnt
+
+    def spatial_correlate(self, f, g):
+        f = _f32(f)
+        g = _f32(g)
+        corr = np.zeros(self.N ** 3, np.float32)
+        am = C.c_uint32()
+        _check(lib().phz_spatial_correlate(self.h, f.shape[0], _p(f), _p(g), _p(corr), C.byref(am)))
+        return corr, int(am.value)
+
+    def spatial_correlate_single(self, f, g, N):
+        f = _f32(f).reshape(-1)
+        g = _f32(g).reshape(-1)
+        corr = np.zeros(N ** 3, np.float32)
+        am = C.c_uint32()
+        _check(lib().phz_spatial_correlate_single(self.h, N, _p(f), _p(g), _p(corr), C.byref(am)))
+        return corr, int(am.value)

@@ -1,0 +1,587 @@
+// capi.hip -- context, resident tables, workspaces and the C ABI entry
+// points of phaser_amd (see include/phaser_amd.h for the reference
+// interfaces each one replaces).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/phaser_amd.h"
+#include "kernels.hpp"
+
+namespace phz {
+
+thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+FftPlan make_plan(int L) {
+  FftPlan p;
+  p.L = L;
+  p.nst = 0;
+  int r = L;
+  while (r % 4 == 0 && p.nst < 10) {
+    p.radix[p.nst++] = 4;
+    r /= 4;
+  }
+  const int others[3] = {2, 3, 5};
+  for (int f : others)
+    while (r % f == 0 && p.nst < 10) {
+      p.radix[p.nst++] = f;
+      r /= f;
+    }
+  if (r != 1) p.nst = -1;
+  return p;
+}
+
+}  // namespace phz
+
+using namespace phz;
+
+struct phz_ctx {
+  phz_config cfg{};
+  int B = 0, n = 0, N = 0, h = 0;
+  int64_t H = 0, T3 = 0, cap = 0;
+  int max_clouds = 0;
+  hipStream_t stream = nullptr;
+  S2Tables s2;
+  VoxelTables vox;
+  FftPlan plan_n{}, plan_N{};
+  float max_sq = 0.f;
+  // resident tables
+  float4* d_grid = nullptr;
+  float* d_theta = nullptr;
+  float* d_leg = nullptr;
+  int* d_leg_off = nullptr;
+  float* d_wig = nullptr;
+  int64_t* d_wig_off = nullptr;
+  float* d_wig_norm = nullptr;
+  float2* d_tw_n = nullptr;
+  float* d_edges = nullptr;
+  float2* d_tw_N = nullptr;
+  // workspaces
+  float4* d_pts = nullptr;
+  int* d_npts = nullptr;
+  CellBuffers cells{};
+  float* d_grids = nullptr;
+  float2* d_F = nullptr;
+  float2* d_coef = nullptr;
+  float2* d_fused = nullptr;
+  float2* d_S = nullptr;
+  float2* d_U = nullptr;
+  float* d_corr_rot = nullptr;
+  unsigned long long* d_keys = nullptr;
+  DevPose* d_pose = nullptr;
+  float* d_sumI = nullptr;
+  float* d_sumR = nullptr;
+  uint32_t* d_cnt = nullptr;
+  float2* d_spec = nullptr;
+  float2* d_work = nullptr;
+  float* d_corr_trans = nullptr;
+  float4* d_reg = nullptr;
+  int* h_npts = nullptr;  // pinned staging for per-cloud point counts
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+template <class T>
+int dalloc(phz_ctx* c, T** p, size_t count) {
+  void* q = nullptr;
+  const size_t bytes = sizeof(T) * (count ? count : 1);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    return -2;
+  }
+  c->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+template <class T>
+int upload(phz_ctx* c, T** p, const std::vector<T>& v) {
+  if (int rc = dalloc(c, p, v.size())) return rc;
+  PHZ_HIP(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+#define TRY(x)                \
+  do {                        \
+    int _rc = (x);            \
+    if (_rc) return _rc;      \
+  } while (0)
+
+int init_ctx(phz_ctx* c) {
+  const phz_config& cf = c->cfg;
+  PHZ_CHECK(cf.bandwidth >= 2 && cf.bandwidth <= 128, "bandwidth must be in [2, 128]");
+  PHZ_CHECK(cf.spherical_zero_padding == 0, "spherical zero padding other than 0 is not supported");
+  PHZ_CHECK(cf.spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
+  PHZ_CHECK(cf.n_voxels >= 2 && cf.n_voxels <= 256, "n_voxels must be in [2, 256]");
+  PHZ_CHECK(cf.discretize_lower < cf.discretize_upper, "discretize bounds must be increasing");
+  PHZ_CHECK(cf.max_points >= 1 && cf.max_points <= (1ll << 26), "max_points out of range");
+  PHZ_CHECK(cf.max_batch >= 1 && cf.max_batch <= 1024, "max_batch out of range");
+  c->B = cf.bandwidth;
+  c->n = 2 * c->B;
+  c->N = cf.n_voxels;
+  c->h = c->N / 2 + 1;
+  c->H = (int64_t)c->N * c->N * c->h;
+  c->T3 = (int64_t)c->N * c->N * c->N;
+  c->cap = cf.max_points;
+  c->max_clouds = 2 * cf.max_batch;
+  c->plan_n = make_plan(c->n);
+  c->plan_N = make_plan(c->N);
+  PHZ_CHECK(c->plan_n.nst > 0, "2*bandwidth must factor into 2, 3 and 5");
+  PHZ_CHECK(c->plan_N.nst > 0, "n_voxels must factor into 2, 3 and 5");
+  {
+    const float v = (float)(std::abs(cf.low_pass_lower) + std::abs(cf.low_pass_upper)) /
+                    (float)cf.n_voxels;  // point-cloud.cc:66-72
+    c->max_sq = v * v;
+  }
+  PHZ_HIP(hipSetDevice(cf.device));
+  PHZ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  build_s2_tables(c->B, &c->s2);
+  build_voxel_tables(c->N, (float)cf.discretize_lower, (float)cf.discretize_upper, &c->vox);
+  PHZ_CHECK(c->s2.runs.count > 0 && c->vox.runs.count > 0, "fusion structure has too many runs");
+  {
+    std::vector<float4> g(c->s2.grid.size() / 4);
+    std::memcpy(g.data(), c->s2.grid.data(), sizeof(float) * c->s2.grid.size());
+    TRY(upload(c, &c->d_grid, g));
+  }
+  TRY(upload(c, &c->d_theta, c->s2.theta));
+  TRY(upload(c, &c->d_leg, c->s2.legendre));
+  TRY(upload(c, &c->d_leg_off, c->s2.leg_off));
+  TRY(upload(c, &c->d_wig, c->s2.wigner));
+  TRY(upload(c, &c->d_wig_off, c->s2.wig_off));
+  TRY(upload(c, &c->d_wig_norm, c->s2.wig_norm));
+  {
+    std::vector<float2> t(c->n), u(c->N);
+    std::memcpy(t.data(), c->s2.tw_n.data(), sizeof(float2) * c->n);
+    std::memcpy(u.data(), c->vox.tw_N.data(), sizeof(float2) * c->N);
+    TRY(upload(c, &c->d_tw_n, t));
+    TRY(upload(c, &c->d_tw_N, u));
+  }
+  TRY(upload(c, &c->d_edges, c->vox.edges));
+  const int M = c->max_clouds, P = cf.max_batch;
+  const int64_t cap = c->cap, nq = (int64_t)c->n * c->n;
+  TRY(dalloc(c, &c->d_pts, (size_t)M * cap));
+  TRY(dalloc(c, &c->d_npts, (size_t)M));
+  TRY(dalloc(c, &c->cells.cell_count, (size_t)M * nq));
+  TRY(dalloc(c, &c->cells.cell_start, (size_t)M * (nq + 1)));
+  TRY(dalloc(c, &c->cells.band_start, (size_t)M * (c->n + 1)));
+  TRY(dalloc(c, &c->cells.next_right, (size_t)M * nq));
+  TRY(dalloc(c, &c->cells.next_left, (size_t)M * nq));
+  TRY(dalloc(c, &c->cells.sorted, (size_t)M * cap));
+  TRY(dalloc(c, &c->cells.sorted_val, (size_t)M * cap));
+  TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
+  TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
+  TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));
+  // stage entry points use up to 16 grids / signals
+  const int G = M * 2 > 16 ? M * 2 : 16;
+  TRY(dalloc(c, &c->d_grids, (size_t)G * nq));
+  TRY(dalloc(c, &c->d_F, (size_t)G * c->B * c->n));
+  TRY(dalloc(c, &c->d_coef, (size_t)G * c->B * c->B));
+  TRY(dalloc(c, &c->d_fused, (size_t)(M > 2 ? M : 2) * c->B * c->B));
+  TRY(dalloc(c, &c->d_S, (size_t)P * c->B * (2 * c->B - 1) * c->n));
+  TRY(dalloc(c, &c->d_U, (size_t)P * c->n * c->B * c->n));
+  TRY(dalloc(c, &c->d_corr_rot, (size_t)nq * c->n));
+  TRY(dalloc(c, &c->d_keys, (size_t)2 * P));
+  TRY(dalloc(c, &c->d_pose, (size_t)P));
+  const int MV = M > 16 ? M : 16;
+  TRY(dalloc(c, &c->d_sumI, (size_t)MV * c->T3));
+  TRY(dalloc(c, &c->d_sumR, (size_t)M * c->T3));
+  TRY(dalloc(c, &c->d_cnt, (size_t)M * c->T3));
+  TRY(dalloc(c, &c->d_spec, (size_t)(2 * M > 16 ? 2 * M : 16) * c->H));
+  TRY(dalloc(c, &c->d_work, (size_t)P * c->H));
+  TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3));
+  TRY(dalloc(c, &c->d_reg, (size_t)cap));
+  PHZ_HIP(hipHostMalloc((void**)&c->h_npts, sizeof(int) * M, hipHostMallocDefault));
+  return 0;
+}
+
+ChannelSet channels(const float2* a, const float2* b, int C) {
+  ChannelSet cs{};
+  cs.C = C;
+  cs.ch[0] = a;
+  cs.ch[1] = b;
+  return cs;
+}
+
+// The registration pipeline on cloud slots [0, nt) targets and
+// [nt, nt+np) sources already resident in ctx->d_pts / d_npts.
+int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
+              float* d_trans_corr) {
+  const int nt = shared ? 1 : np;
+  const int ncl = nt + np;
+  const int B = c->B, n = c->n, N = c->N;
+  const int64_t nq = (int64_t)n * n, BB = (int64_t)B * B;
+  // rotation
+  launch_sample_s2(c->d_pts, c->d_npts, c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq,
+                   c->cells, c->d_grids, s);
+  launch_fst(c->d_grids, 2 * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+  for (int k = 0; k < ncl; ++k)  // spherical order {range, intensity, refl=0, amb=0}
+    launch_fuse(channels(c->d_coef + (2 * k) * BB, c->d_coef + (2 * k + 1) * BB, 4),
+                (uint32_t)BB, c->s2.runs, c->d_fused + k * BB, s);
+  launch_so3_synth(B, c->d_fused, shared ? 0 : BB, c->d_fused + nt * BB, BB, np, c->d_wig,
+                   c->d_wig_off, c->d_wig_norm, c->d_S, s);
+  launch_fill_u32((uint32_t*)c->d_keys, 4 * (size_t)np, 0u, s);
+  launch_so3_fft_alpha(B, np, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
+  launch_so3_fft_gamma(B, np, c->d_U, c->plan_n, c->d_tw_n, d_rot_corr, nq * n, c->d_keys, s);
+  launch_rot_pose(B, np, c->d_keys, c->d_pose, s);
+  // translation
+  launch_fill_u32((uint32_t*)c->d_sumI, (size_t)ncl * c->T3, 0u, s);
+  launch_fill_u32((uint32_t*)c->d_sumR, (size_t)ncl * c->T3, 0u, s);
+  launch_fill_u32(c->d_cnt, (size_t)ncl * c->T3, 0u, s);
+  launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
+                       c->d_sumR, c->d_cnt, s);
+  launch_voxel_scatter(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
+                       c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
+                       c->d_cnt + nt * c->T3, s);
+  launch_spatial_forward(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N, c->d_spec,
+                         s);
+  for (int p = 0; p < np; ++p) {  // spatial order {intensity, range, refl=0, amb=0}
+    const int t = shared ? 0 : p;
+    const float2* F = c->d_spec + (int64_t)(2 * t) * c->H;
+    const float2* G = c->d_spec + (int64_t)(2 * (nt + p)) * c->H;
+    launch_spatial_inverse(N, channels(F, F + c->H, 4), channels(G, G + c->H, 4), c->vox.runs,
+                           true, c->plan_N, c->d_tw_N, c->d_work + p * c->H,
+                           d_trans_corr ? d_trans_corr + p * c->T3 : nullptr,
+                           c->d_keys + np + p, s);
+  }
+  launch_trans_pose(N, c->cfg.discretize_lower, c->cfg.discretize_upper, np, c->d_keys + np,
+                    c->d_pose, s);
+  PHZ_HIP(hipGetLastError());
+  return 0;
+}
+
+void to_result(const DevPose& d, phz_result* r) {
+  r->rot_index = d.rot_index;
+  r->trans_index = d.trans_index;
+  r->rot_value = d.rot_value;
+  r->trans_value = d.trans_value;
+  for (int i = 0; i < 3; ++i) {
+    r->zyz[i] = d.zyz[i];
+    r->euler_xyz[i] = d.euler[i];
+    r->translation[i] = d.translation[i];
+  }
+  for (int i = 0; i < 4; ++i) r->quat[i] = d.quat[i];
+  for (int i = 0; i < 16; ++i) r->T_rot[i] = d.T[i];
+}
+
+int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
+  std::vector<DevPose> hp(np);
+  PHZ_HIP(hipMemcpyAsync(hp.data(), c->d_pose, sizeof(DevPose) * np, hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  for (int p = 0; p < np; ++p) to_result(hp[p], &out[p]);
+  return 0;
+}
+
+int load_cloud(phz_ctx* c, int slot, const float* pts, int64_t P, hipMemcpyKind kind,
+               hipStream_t s) {
+  PHZ_CHECK(P >= 0 && P <= c->cap, "point count exceeds max_points");
+  PHZ_CHECK(P == 0 || pts != nullptr, "null point buffer");
+  if (P > 0)
+    PHZ_HIP(hipMemcpyAsync(c->d_pts + slot * c->cap, pts, sizeof(float4) * P, kind, s));
+  c->h_npts[slot] = (int)P;
+  return 0;
+}
+
+// copies the staged counts of slots [0, ncl) to the device
+int commit_counts(phz_ctx* c, int ncl, hipStream_t s) {
+  PHZ_HIP(hipMemcpyAsync(c->d_npts, c->h_npts, sizeof(int) * ncl, hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void phz_default_config(phz_config* cfg) {
+  if (!cfg) return;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->bandwidth = 64;
+  cfg->spherical_zero_padding = 0;
+  cfg->n_voxels = 128;
+  cfg->discretize_lower = -50;
+  cfg->discretize_upper = 50;
+  cfg->spatial_zero_padding = 0;
+  cfg->low_pass_lower = 0;
+  cfg->low_pass_upper = 100000;
+  cfg->max_points = 262144;
+  cfg->max_batch = 1;
+  cfg->device = 0;
+}
+
+const char* phz_last_error(void) { return g_last_error.c_str(); }
+
+int phz_create(const phz_config* cfg, phz_ctx** out) {
+  PHZ_CHECK(cfg != nullptr && out != nullptr, "null argument");
+  *out = nullptr;
+  phz_ctx* c = new phz_ctx();
+  c->cfg = *cfg;
+  const int rc = init_ctx(c);
+  if (rc) {
+    const std::string msg = g_last_error;
+    phz_destroy(c);
+    set_error(msg);
+    return rc;
+  }
+  *out = c;
+  return 0;
+}
+
+void phz_destroy(phz_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (void* p : c->allocs) hipFree(p);
+  if (c->h_npts) hipHostFree(c->h_npts);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int phz_get_config(const phz_ctx* c, phz_config* out) {
+  PHZ_CHECK(c && out, "null argument");
+  *out = c->cfg;
+  return 0;
+}
+
+int phz_register(phz_ctx* c, const float* target, int64_t Pt, const float* source, int64_t Ps,
+                 phz_result* result, float* rot_corr, float* trans_corr, float* registered) {
+  PHZ_CHECK(c && result, "null argument");
+  hipStream_t s = c->stream;
+  TRY(load_cloud(c, 0, target, Pt, hipMemcpyHostToDevice, s));
+  TRY(load_cloud(c, 1, source, Ps, hipMemcpyHostToDevice, s));
+  TRY(commit_counts(c, 2, s));
+  TRY(run_pairs(c, 1, false, s, rot_corr ? c->d_corr_rot : nullptr,
+                trans_corr ? c->d_corr_trans : nullptr));
+  if (registered && Ps > 0) launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, c->d_reg, s);
+  TRY(fetch_results(c, 1, result, s));
+  const int64_t n3 = (int64_t)c->n * c->n * c->n;
+  if (rot_corr)
+    PHZ_HIP(hipMemcpy(rot_corr, c->d_corr_rot, sizeof(float) * n3, hipMemcpyDeviceToHost));
+  if (trans_corr)
+    PHZ_HIP(hipMemcpy(trans_corr, c->d_corr_trans, sizeof(float) * c->T3, hipMemcpyDeviceToHost));
+  if (registered && Ps > 0)
+    PHZ_HIP(hipMemcpy(registered, c->d_reg, sizeof(float4) * Ps, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int phz_register_pairs_device(phz_ctx* c, int n, int shared_target, const float* const* d_targets,
+                              const int64_t* Pt, const float* const* d_sources, const int64_t* Ps,
+                              phz_result* results, float* d_rot_corr, float* d_trans_corr,
+                              void* stream) {
+  PHZ_CHECK(c && d_targets && Pt && d_sources && Ps && results, "null argument");
+  PHZ_CHECK(n >= 1 && n <= c->cfg.max_batch, "pair count exceeds max_batch");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  const int nt = shared_target ? 1 : n;
+  for (int t = 0; t < nt; ++t) TRY(load_cloud(c, t, d_targets[t], Pt[t], hipMemcpyDeviceToDevice, s));
+  for (int p = 0; p < n; ++p)
+    TRY(load_cloud(c, nt + p, d_sources[p], Ps[p], hipMemcpyDeviceToDevice, s));
+  TRY(commit_counts(c, nt + n, s));
+  TRY(run_pairs(c, n, shared_target != 0, s, d_rot_corr, d_trans_corr));
+  return fetch_results(c, n, results, s);
+}
+
+int phz_register_device(phz_ctx* c, const float* d_target, int64_t Pt, const float* d_source,
+                        int64_t Ps, phz_result* result, float* d_rot_corr, float* d_trans_corr,
+                        float* d_registered, void* stream) {
+  PHZ_CHECK(c && result, "null argument");
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  TRY(load_cloud(c, 0, d_target, Pt, hipMemcpyDeviceToDevice, s));
+  TRY(load_cloud(c, 1, d_source, Ps, hipMemcpyDeviceToDevice, s));
+  TRY(commit_counts(c, 2, s));
+  TRY(run_pairs(c, 1, false, s, d_rot_corr, d_trans_corr));
+  if (d_registered && Ps > 0)
+    launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, (float4*)d_registered, s);
+  return fetch_results(c, 1, result, s);
+}
+
+// ---------------------------------------------------------------- stages
+int phz_sample_s2(phz_ctx* c, const float* pts, int64_t P, float* grids) {
+  PHZ_CHECK(c && grids, "null argument");
+  hipStream_t s = c->stream;
+  TRY(load_cloud(c, 0, pts, P, hipMemcpyHostToDevice, s));
+  TRY(commit_counts(c, 1, s));
+  launch_sample_s2(c->d_pts, c->d_npts, c->cap, 1, c->n, c->d_grid, c->d_theta, c->max_sq,
+                   c->cells, c->d_grids, s);
+  PHZ_HIP(hipGetLastError());
+  PHZ_HIP(hipMemcpyAsync(grids, c->d_grids, sizeof(float) * 2 * c->n * c->n,
+                         hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+int phz_s2_transform(phz_ctx* c, int nsig, const float* grids, float* coef) {
+  PHZ_CHECK(c && grids && coef, "null argument");
+  PHZ_CHECK(nsig >= 1 && nsig <= 16, "nsig must be in [1, 16]");
+  hipStream_t s = c->stream;
+  const int64_t nq = (int64_t)c->n * c->n, BB = (int64_t)c->B * c->B;
+  PHZ_HIP(hipMemcpyAsync(c->d_grids, grids, sizeof(float) * nsig * nq, hipMemcpyHostToDevice, s));
+  launch_fst(c->d_grids, nsig, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+  PHZ_HIP(hipGetLastError());
+  PHZ_HIP(hipMemcpyAsync(coef, c->d_coef, sizeof(float2) * nsig * BB, hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, float* corr,
+                          float* synthesis, uint32_t* argmax, hipStream_t s) {
+  const int B = c->B, n = c->n;
+  launch_so3_synth(B, d_f, 0, d_h, 0, 1, c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S, s);
+  launch_fill_u32((uint32_t*)c->d_keys, 2, 0u, s);
+  launch_so3_fft_alpha(B, 1, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
+  launch_so3_fft_gamma(B, 1, c->d_U, c->plan_n, c->d_tw_n, corr ? c->d_corr_rot : nullptr,
+                       (int64_t)n * n * n, c->d_keys, s);
+  PHZ_HIP(hipGetLastError());
+  unsigned long long key = 0;
+  PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  if (argmax) *argmax = key_index(key);
+  if (corr)
+    PHZ_HIP(hipMemcpy(corr, c->d_corr_rot, sizeof(float) * n * n * n, hipMemcpyDeviceToHost));
+  if (synthesis)
+    PHZ_HIP(hipMemcpy(synthesis, c->d_S, sizeof(float2) * (size_t)B * (2 * B - 1) * n,
+                      hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int phz_so3_correlate_coef(phz_ctx* c, const float* f_coef, const float* h_coef, float* corr,
+                           float* synthesis, uint32_t* argmax) {
+  PHZ_CHECK(c && f_coef && h_coef, "null argument");
+  hipStream_t s = c->stream;
+  const int64_t BB = (int64_t)c->B * c->B;
+  PHZ_HIP(hipMemcpyAsync(c->d_fused, f_coef, sizeof(float2) * BB, hipMemcpyHostToDevice, s));
+  PHZ_HIP(hipMemcpyAsync(c->d_fused + BB, h_coef, sizeof(float2) * BB, hipMemcpyHostToDevice, s));
+  return so3_from_fused(c, c->d_fused, c->d_fused + BB, corr, synthesis, argmax, s);
+}
+
+int phz_spherical_correlate(phz_ctx* c, int C, const float* f_grids, const float* h_grids,
+                            float* corr, uint32_t* argmax) {
+  PHZ_CHECK(c && f_grids && h_grids, "null argument");
+  PHZ_CHECK(C >= 2 && C <= 8, "channel count must be in [2, 8]");
+  hipStream_t s = c->stream;
+  const int64_t nq = (int64_t)c->n * c->n, BB = (int64_t)c->B * c->B;
+  PHZ_HIP(hipMemcpyAsync(c->d_grids, f_grids, sizeof(float) * C * nq, hipMemcpyHostToDevice, s));
+  PHZ_HIP(hipMemcpyAsync(c->d_grids + C * nq, h_grids, sizeof(float) * C * nq,
+                         hipMemcpyHostToDevice, s));
+  launch_fst(c->d_grids, 2 * C, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+  for (int w = 0; w < 2; ++w) {
+    ChannelSet cs{};
+    cs.C = C;
+    for (int k = 0; k < C; ++k) cs.ch[k] = c->d_coef + (int64_t)(w * C + k) * BB;
+    launch_fuse(cs, (uint32_t)BB, c->s2.runs, c->d_fused + w * BB, s);
+  }
+  return so3_from_fused(c, c->d_fused, c->d_fused + BB, corr, nullptr, argmax, s);
+}
+
+int phz_voxelize(phz_ctx* c, const float* pts, int64_t P, float* out, float* counts) {
+  PHZ_CHECK(c && out, "null argument");
+  hipStream_t s = c->stream;
+  TRY(load_cloud(c, 0, pts, P, hipMemcpyHostToDevice, s));
+  TRY(commit_counts(c, 1, s));
+  launch_fill_u32((uint32_t*)c->d_sumI, c->T3, 0u, s);
+  launch_fill_u32((uint32_t*)c->d_sumR, c->T3, 0u, s);
+  launch_fill_u32(c->d_cnt, c->T3, 0u, s);
+  launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, 1, nullptr, 0, c->N, c->d_edges, c->d_sumI,
+                       c->d_sumR, c->d_cnt, s);
+  PHZ_HIP(hipGetLastError());
+  std::vector<float> sI(c->T3), sR(c->T3);
+  std::vector<uint32_t> k(c->T3);
+  PHZ_HIP(hipMemcpyAsync(sI.data(), c->d_sumI, sizeof(float) * c->T3, hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipMemcpyAsync(sR.data(), c->d_sumR, sizeof(float) * c->T3, hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipMemcpyAsync(k.data(), c->d_cnt, sizeof(uint32_t) * c->T3, hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  for (int64_t v = 0; v < c->T3; ++v) {
+    out[v] = k[v] ? sI[v] / (float)k[v] : 0.f;
+    out[c->T3 + v] = k[v] ? sR[v] / (float)k[v] : 0.f;
+    if (counts) counts[v] = (float)k[v];
+  }
+  return 0;
+}
+
+int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, float* corr,
+                          uint32_t* argmax) {
+  PHZ_CHECK(c && f && g, "null argument");
+  PHZ_CHECK(C >= 2 && C <= 8, "channel count must be in [2, 8]");
+  hipStream_t s = c->stream;
+  const int64_t T = c->T3, H = c->H;
+  PHZ_HIP(hipMemcpyAsync(c->d_sumI, f, sizeof(float) * C * T, hipMemcpyHostToDevice, s));
+  PHZ_HIP(hipMemcpyAsync(c->d_sumI + C * T, g, sizeof(float) * C * T, hipMemcpyHostToDevice, s));
+  launch_spatial_forward_dense(c->N, 2 * C, c->d_sumI, c->plan_N, c->d_tw_N, c->d_spec, s);
+  ChannelSet F{}, G{};
+  F.C = G.C = C;
+  for (int k = 0; k < C; ++k) {
+    F.ch[k] = c->d_spec + k * H;
+    G.ch[k] = c->d_spec + (C + k) * H;
+  }
+  launch_fill_u32((uint32_t*)c->d_keys, 2, 0u, s);
+  launch_spatial_inverse(c->N, F, G, c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work,
+                         corr ? c->d_corr_trans : nullptr, c->d_keys, s);
+  PHZ_HIP(hipGetLastError());
+  unsigned long long key = 0;
+  PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  if (argmax) *argmax = key_index(key);
+  if (corr) PHZ_HIP(hipMemcpy(corr, c->d_corr_trans, sizeof(float) * T, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float* g, float* corr,
+                                 uint32_t* argmax) {
+  PHZ_CHECK(c && f && g, "null argument");
+  PHZ_CHECK(N >= 1 && N <= 256, "n_voxels must be in [1, 256]");
+  const FftPlan plan = make_plan(N);
+  PHZ_CHECK(plan.nst > 0 || N == 1, "n_voxels must factor into 2, 3 and 5");
+  hipStream_t s = c->stream;
+  const int64_t T = (int64_t)N * N * N, H = (int64_t)N * N * (N / 2 + 1);
+  std::vector<float> tw = twiddles(N);
+  float *d_in = nullptr, *d_corr = nullptr;
+  float2 *d_spec = nullptr, *d_work = nullptr, *d_tw = nullptr;
+  unsigned long long* d_key = nullptr;
+  int rc = 0;
+  auto fail = [&](const char* what, hipError_t e) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    rc = -2;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&d_in, sizeof(float) * 2 * T)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && (e = hipMalloc(&d_spec, sizeof(float2) * 2 * H)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && (e = hipMalloc(&d_work, sizeof(float2) * H)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && (e = hipMalloc(&d_tw, sizeof(float2) * N)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && (e = hipMalloc(&d_corr, sizeof(float) * T)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && (e = hipMalloc(&d_key, sizeof(unsigned long long))) != hipSuccess) fail("hipMalloc", e);
+  if (!rc) {
+    hipMemcpyAsync(d_in, f, sizeof(float) * T, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(d_in + T, g, sizeof(float) * T, hipMemcpyHostToDevice, s);
+    hipMemcpyAsync(d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s);
+    launch_fill_u32((uint32_t*)d_key, 2, 0u, s);
+    FftPlan p = plan;
+    if (N == 1) {
+      p.L = 1;
+      p.nst = 0;
+    }
+    launch_spatial_forward_dense(N, 2, d_in, p, d_tw, d_spec, s);
+    ChannelSet F{}, G{};
+    F.C = G.C = 1;
+    F.ch[0] = d_spec;
+    G.ch[0] = d_spec + H;
+    FusionRuns none{};
+    launch_spatial_inverse(N, F, G, none, false, p, d_tw, d_work, d_corr, d_key, s);
+    if ((e = hipGetLastError()) != hipSuccess) fail("launch", e);
+  }
+  unsigned long long key = 0;
+  if (!rc && (e = hipMemcpyAsync(&key, d_key, sizeof(key), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    fail("copy", e);
+  if (!rc && (e = hipStreamSynchronize(s)) != hipSuccess) fail("sync", e);
+  if (!rc && corr && (e = hipMemcpy(corr, d_corr, sizeof(float) * T, hipMemcpyDeviceToHost)) != hipSuccess)
+    fail("copy", e);
+  if (!rc && argmax) *argmax = key_index(key);
+  hipFree(d_in);
+  hipFree(d_spec);
+  hipFree(d_work);
+  hipFree(d_tw);
+  hipFree(d_corr);
+  hipFree(d_key);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// common.hpp -- shared device helpers for the phaser_amd HIP kernels (gfx950).
+//
+// * LDS Stockham FFT (radix 4/2/3/5, natural-order output, Govindaraju et al.
+//   2008 formulation) used by every FFT stage;
+// * order-preserving 64-bit argmax keys (first-maximum semantics of
+//   std::max_element, z-score-eval.cc:46) for atomicMax reductions;
+// * status/error plumbing for the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace phz {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+
+#define PHZ_HIP(expr)                                                        \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::phz::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+      return -2;                                                             \
+    }                                                                        \
+  } while (0)
+
+#define PHZ_CHECK(cond, msg)                                                 \
+  do {                                                                       \
+    if (!(cond)) {                                                           \
+      ::phz::set_error(msg);                                                 \
+      return -1;                                                             \
+    }                                                                        \
+  } while (0)
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// ---------------------------------------------------------------- argmax keys
+// key = (ordered value bits << 32) | (0xffffffff - index): atomicMax keeps the
+// largest value and, among equal values, the smallest index.
+__device__ __forceinline__ uint32_t ordered_bits(float v) {
+  const uint32_t u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ inline float unordered_value(uint32_t o) {
+  const uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+#ifdef __HIP_DEVICE_COMPILE__
+  return __uint_as_float(u);
+#else
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+#endif
+}
+__device__ __forceinline__ unsigned long long make_key(float v, uint32_t idx) {
+  return ((unsigned long long)ordered_bits(v) << 32) | (unsigned long long)(0xffffffffu - idx);
+}
+__host__ __device__ inline uint32_t key_index(unsigned long long k) {
+  return 0xffffffffu - (uint32_t)(k & 0xffffffffull);
+}
+
+// wave-level max of keys, then one atomic per wave
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long k) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(k, off, 64);
+    k = o > k ? o : k;
+  }
+  return k;
+}
+
+// ---------------------------------------------------------------- FFT plan
+struct FftPlan {
+  int L;          // transform length (<= 256)
+  int nst;        // stages
+  int radix[10];  // radices in stage order
+};
+
+FftPlan make_plan(int L);  // host: factor L into 4,2,3,5 (error if other)
+
+// One Stockham stage of radix R for nb lines of length L stored line-major
+// in `in`, written to `out`.  tw[e] = exp(-2 pi i e / L).  DIR = -1
+// forward (e^{-i}), +1 backward.
+template <int R, int DIR>
+__device__ __forceinline__ void fft_stage(const float2* __restrict__ in, float2* __restrict__ out,
+                                          int L, int nb, int Ns, const float2* __restrict__ tw) {
+  const int Lr = L / R;
+  const int tstep = L / (Ns * R);
+  for (int w = threadIdx.x; w < nb * Lr; w += blockDim.x) {
+    const int line = w / Lr;
+    const int j = w - line * Lr;
+    const float2* src = in + line * L;
+    float2* dst = out + line * L;
+    float2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[j + r * Lr];
+    const int k = j % Ns;
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 t = tw[(k * r * tstep) % L];
+        if (DIR > 0) t.y = -t.y;
+        v[r] = cmul(v[r], t);
+      }
+    }
+    if (R == 2) {
+      const float2 a = v[0], b = v[1];
+      v[0] = cadd(a, b);
+      v[1] = csub(a, b);
+    } else if (R == 4) {
+      const float2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+      const float2 a2 = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+      // forward: -i*d ; backward: +i*d
+      const float2 a3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+      v[0] = cadd(a0, a2);
+      v[2] = csub(a0, a2);
+      v[1] = cadd(a1, a3);
+      v[3] = csub(a1, a3);
+    } else if (R == 3) {
+      const float c = -0.5f, s = DIR * 0.86602540378443865f;
+      const float2 a = v[0], b = v[1], cc = v[2];
+      const float2 t1 = cadd(b, cc), t2 = csub(b, cc);
+      v[0] = cadd(a, t1);
+      const float2 m1 = make_float2(a.x + c * t1.x, a.y + c * t1.y);
+      const float2 m2 = make_float2(-s * t2.y, s * t2.x);  // i*s*t2
+      v[1] = cadd(m1, m2);
+      v[2] = csub(m1, m2);
+    } else if (R == 5) {
+      const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;
+      const float s1 = DIR * 0.95105651629515357f, s2 = DIR * 0.58778525229247313f;
+      const float2 a = v[0];
+      const float2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+      const float2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+      v[0] = cadd(a, cadd(t1, t2));
+      const float2 b1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+      const float2 b2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+      const float2 e1 = make_float2(-(s1 * t3.y + s2 * t4.y), s1 * t3.x + s2 * t4.x);
+      const float2 e2 = make_float2(-(s2 * t3.y - s1 * t4.y), s2 * t3.x - s1 * t4.x);
+      v[1] = cadd(b1, e1);
+      v[4] = csub(b1, e1);
+      v[2] = cadd(b2, e2);
+      v[3] = csub(b2, e2);
+    }
+    const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[base + r * Ns] = v[r];
+  }
+}
+
+// nb FFTs of length plan.L on line-major LDS data in `a` (scratch `b` of the
+// same size).  Returns the buffer holding the result.  Ends with a barrier.
+template <int DIR>
+__device__ float2* lds_fft(float2* a, float2* b, int nb, const FftPlan& plan,
+                           const float2* __restrict__ tw) {
+  int Ns = 1;
+  for (int s = 0; s < plan.nst; ++s) {
+    const int R = plan.radix[s];
+    if (R == 4)
+      fft_stage<4, DIR>(a, b, plan.L, nb, Ns, tw);
+    else if (R == 2)
+      fft_stage<2, DIR>(a, b, plan.L, nb, Ns, tw);
+    else if (R == 3)
+      fft_stage<3, DIR>(a, b, plan.L, nb, Ns, tw);
+    else
+      fft_stage<5, DIR>(a, b, plan.L, nb, Ns, tw);
+    __syncthreads();
+    float2* t = a;
+    a = b;
+    b = t;
+    Ns *= R;
+  }
+  return a;
+}
+
+}  // namespace phz

@@ -1,0 +1,104 @@
+// kernels.hpp -- launchers for the phaser_amd HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+#include "tables.hpp"
+
+namespace phz {
+
+// ------------------------------------------------------------- S^2 sampling
+// Clouds are float4 (x,y,z,intensity).  `nclouds` clouds of up to `cap`
+// points each, cloud c at pts + c*cap with npts[c] points.  Cells: n bands
+// (theta) x n sectors (phi centred on the query columns).
+struct CellBuffers {
+  int* cell_count;   // nclouds * n*n
+  int* cell_start;   // nclouds * (n*n + 1)
+  int* band_start;   // nclouds * (n + 1)
+  int* next_right;   // nclouds * n*n: nearest non-empty sector to the right
+  int* next_left;    // nclouds * n*n: nearest non-empty sector to the left
+  float4* sorted;    // nclouds * cap: projected x,y,z + original index bits
+  float2* sorted_val;  // nclouds * cap: (range, intensity)
+  float4* proj;      // nclouds * cap
+  float2* val;       // nclouds * cap
+  int* cell;         // nclouds * cap
+};
+
+void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
+                      const float4* grid, const float* theta, float max_sq, CellBuffers cb,
+                      float* grids /* nclouds x 2 x n*n : range, intensity */, hipStream_t s);
+
+// ------------------------------------------------------------- S^2 transform
+// sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
+// complex (seanindex order).
+void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,
+                const float2* tw_n, float2* F, float2* coef, hipStream_t s);
+
+// Laplace fusion of C channels (null = all-zero channel) of `count` complex
+// coefficients, mode runs in unshifted order.  out: count complex.
+struct ChannelSet {
+  const float2* ch[8];
+  int C;
+};
+void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
+                 hipStream_t s);
+
+// ------------------------------------------------------------- SO(3)
+// S buffer: [m2 in 0..B-1][m1+B-1 in 0..2B-2][k in 0..n-1] complex per pair.
+void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
+                      int64_t h_stride, int npairs, const float* wigner,
+                      const int64_t* wig_off_dev, const float* wig_norm, float2* S,
+                      hipStream_t s);
+// FFT over m1 -> alpha: U[k][m2][alpha] per pair
+void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
+                          const float2* tw, float2* U, hipStream_t s);
+// FFT over m2 -> gamma (Hermitian), |C|^2, optional grid write, argmax key
+void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& plan,
+                          const float2* tw, float* corr /*nullable*/, int64_t corr_stride,
+                          unsigned long long* keys, hipStream_t s);
+
+// Device-side pose from the rotation argmax: fills pose[pair] and the float
+// transform T[pair][16] (rotation-utils.cc:17-23,86-143,190-207).
+struct DevPose {
+  uint32_t rot_index, trans_index;
+  float rot_value, trans_value;
+  double zyz[3];
+  double quat[4];
+  double euler[3];
+  double translation[3];
+  float T[16];
+};
+void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose* pose,
+                     hipStream_t s);
+
+// ------------------------------------------------------------- spatial
+// voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
+                          const DevPose* pose /*nullable: no transform*/, int pose_stride,
+                          int N, const float* edges, float* sumI, float* sumR, uint32_t* cnt,
+                          hipStream_t s);
+// forward 3-D r2c of (mean intensity, mean range) per cloud:
+// spec[cloud][ch in {I,R}][z][x][k2], k2 < N/2+1
+void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
+                            const uint32_t* cnt, const FftPlan& plan, const float2* tw,
+                            float2* spec, hipStream_t s);
+// forward from dense real channels (stage API): chans[c][N^3] in order,
+// nch channels (<= 8) -> spec[c][z][x][k2]
+void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
+                                  const float2* tw, float2* spec, hipStream_t s);
+// fused cross spectrum + inverse 3-D c2r + argmax.  F channels and G
+// channels (null = zero channel) in the reference's spatial order.
+void launch_spatial_inverse(int N, ChannelSet F, ChannelSet G, const FusionRuns& runs,
+                            bool fuse, const FftPlan& plan, const float2* tw, float2* work,
+                            float* corr /*nullable*/, unsigned long long* key, hipStream_t s);
+void launch_trans_pose(int N, int lower, int upper, int npairs,
+                       const unsigned long long* keys, DevPose* pose, hipStream_t s);
+// out = T_rot * p + translation (PCL scalar order), intensity kept
+void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,
+                       hipStream_t s);
+
+}  // namespace phz

@@ -1,0 +1,410 @@
+// kernels_s2.hip -- S^2 sampling, forward S^2 transform and coefficient
+// fusion for gfx950.
+//
+// Sampling restates SphericalSampler::sampleUniformly
+// (spherical-sampler.cc:20-27): project every point onto the unit sphere
+// (spherical-projection.cc:19-41), then take the exact nearest projected
+// point of each Driscoll-Healy query direction (point-cloud.cc:93-157, FLANN
+// L2_Simple float distance).  Instead of a kd-tree the points are counting-
+// sorted into a (theta band x phi sector) grid aligned with the queries, and
+// each query searches bands outward from its own with exact lower bounds on
+// the chord distance, so the answer equals a brute-force scan (ties go to the
+// lowest point index).
+#include "kernels.hpp"
+
+namespace phz {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+
+// spherical-projection.cc:19-41 with the reference's float/double mix and no
+// FMA contraction.
+__device__ __forceinline__ void project_point(float x, float y, float z, float* px, float* py,
+                                              float* pz, float* range) {
+#pragma clang fp contract(off)
+  const float dxy = sqrtf(x * x + y * y);
+  const double dxyd = (double)dxy;
+  const double zz = (double)(z * z);
+  const double dist = sqrt(dxyd * dxyd + zz);
+  const float pzf = (float)((double)z / dist);
+  const double tx = (double)acosf(pzf);
+  const double ty = fmod((double)atan2f(y, x) + 2.0 * M_PI, 2.0 * M_PI);
+  const double stx = sin(tx);
+  *px = (float)(stx * cos(ty));
+  *py = (float)(stx * sin(ty));
+  *pz = pzf;
+  *range = (float)dist;
+}
+
+__global__ void k_project(const float4* __restrict__ pts, const int* __restrict__ npts,
+                          int64_t cap, int n, float4* __restrict__ proj, float2* __restrict__ val,
+                          int* __restrict__ cell, int* __restrict__ cell_count) {
+  const int c = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts[c]) return;
+  const float4 p = pts[c * cap + i];
+  float px, py, pz, r;
+  project_point(p.x, p.y, p.z, &px, &py, &pz, &r);
+  // cell of the stored float point: band from its colatitude, sector centred
+  // on the query columns phi_j = 2 pi j / n
+  const float th = atan2f(sqrtf(px * px + py * py), pz);
+  int band = (int)(th * ((float)n / kPi));
+  band = band < 0 ? 0 : (band >= n ? n - 1 : band);
+  float ph = atan2f(py, px);
+  if (ph < 0.f) ph += 2.f * kPi;
+  int sec = (int)floorf(ph * ((float)n / (2.f * kPi)) + 0.5f);
+  sec = sec >= n ? sec - n : (sec < 0 ? 0 : sec);
+  const int cid = band * n + sec;
+  proj[c * cap + i] = make_float4(px, py, pz, __int_as_float(i));
+  val[c * cap + i] = make_float2(r, p.w);
+  cell[c * cap + i] = cid;
+  atomicAdd(&cell_count[(int64_t)c * n * n + cid], 1);
+}
+
+// exclusive scan of n*n cell counts per cloud (one workgroup per cloud)
+__global__ void k_cell_scan(int n, int* __restrict__ cell_count, int* __restrict__ cell_start,
+                            int* __restrict__ band_start) {
+  const int c = blockIdx.x;
+  const int ncell = n * n;
+  int* cnt = cell_count + (int64_t)c * ncell;
+  int* st = cell_start + (int64_t)c * (ncell + 1);
+  int* bs = band_start + (int64_t)c * (n + 1);
+  __shared__ int part[1024];
+  const int T = blockDim.x;
+  const int per = (ncell + T - 1) / T;
+  const int a = threadIdx.x * per, b = min(ncell, a + per);
+  int s = 0;
+  for (int k = a; k < b; ++k) s += cnt[k];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int t = 0; t < T; ++t) {
+      const int v = part[t];
+      part[t] = acc;
+      acc += v;
+    }
+    st[ncell] = acc;
+  }
+  __syncthreads();
+  int acc = part[threadIdx.x];
+  for (int k = a; k < b; ++k) {
+    st[k] = acc;
+    acc += cnt[k];
+    cnt[k] = 0;  // reused as the scatter cursor
+  }
+  __syncthreads();
+  for (int bb = threadIdx.x; bb <= n; bb += T) bs[bb] = st[bb * n];
+}
+
+__global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int n,
+                               const float4* __restrict__ proj, const float2* __restrict__ val,
+                               const int* __restrict__ cell, int* __restrict__ cursor,
+                               const int* __restrict__ cell_start, float4* __restrict__ sorted,
+                               float2* __restrict__ sorted_val) {
+  const int c = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts[c]) return;
+  const int cid = cell[c * cap + i];
+  const int pos = atomicAdd(&cursor[(int64_t)c * n * n + cid], 1);
+  const int dst = cell_start[(int64_t)c * (n * n + 1) + cid] + pos;
+  sorted[c * cap + dst] = proj[c * cap + i];
+  sorted_val[c * cap + dst] = val[c * cap + i];
+}
+
+// per cell: nearest non-empty sector of the same band to the right (>= s)
+// and to the left (<= s), cyclic; -1 in an empty band.  One thread per band.
+__global__ void k_jump_tables(int n, const int* __restrict__ cell_start, int* __restrict__ nr,
+                              int* __restrict__ nl) {
+  const int c = blockIdx.y;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int* st = cell_start + (int64_t)c * (n * n + 1) + (int64_t)b * n;
+  int* r = nr + (int64_t)c * n * n + (int64_t)b * n;
+  int* l = nl + (int64_t)c * n * n + (int64_t)b * n;
+  const bool any = st[n] > st[0];
+  int last = -1;
+  for (int k = 2 * n - 1; k >= 0; --k) {
+    const int s = k % n;
+    if (st[s + 1] > st[s]) last = s;
+    if (k < n) r[s] = any ? last : -1;
+  }
+  last = -1;
+  for (int k = 0; k < 2 * n; ++k) {
+    const int s = k % n;
+    if (st[s + 1] > st[s]) last = s;
+    if (k >= n) l[s] = any ? last : -1;
+  }
+}
+
+__device__ __forceinline__ float l2_simple(float4 q, float4 p) {
+#pragma clang fp contract(off)
+  const float d0 = q.x - p.x, d1 = q.y - p.y, d2 = q.z - p.z;
+  float r = 0.0f;
+  r += d0 * d0;
+  r += d1 * d1;
+  r += d2 * d2;
+  return r;
+}
+
+// Exact nearest neighbour of every query.  Bands are visited in order of
+// colatitude distance (own, +1, -1, +2, ...), sectors of a band outward from
+// the query's own, and a cell is skipped once its lower bound on the squared
+// chord distance,
+//   |q - p|^2 >= 2(1 - cos dth) + 2 sin(th_q) min sin(th_p) (1 - cos dphi),
+// exceeds the best distance found (with a float slack): points outside it are
+// provably farther.  Empty sectors are jumped over with the per-cell tables.
+__global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
+                     float max_sq, const int* __restrict__ cell_start,
+                     const int* __restrict__ band_start, const int* __restrict__ next_right,
+                     const int* __restrict__ next_left, const float4* __restrict__ sorted,
+                     const float2* __restrict__ sorted_val, int64_t cap,
+                     float* __restrict__ grids) {
+  const int c = blockIdx.y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nq = n * n;
+  if (q >= nq) return;
+  const int qi = q / n, qj = q - qi * n;
+  const float4 qv = grid[q];
+  const float thq = theta[qi];
+  const float sq = sinf(thq);
+  const int* st = cell_start + (int64_t)c * (nq + 1);
+  const int* bs = band_start + (int64_t)c * (n + 1);
+  const int* nr = next_right + (int64_t)c * nq;
+  const int* nl = next_left + (int64_t)c * nq;
+  const float4* sp = sorted + c * cap;
+  const float dband = kPi / n, dsec = 2.f * kPi / n;
+  const float eps = 1e-5f, slack = 1e-5f;
+  float best = INFINITY;
+  int bidx = 0x7fffffff, bpos = -1;
+  bool up = true, down = true;
+  auto scan_cell = [&](int cid) {
+    const int e = st[cid + 1];
+    for (int p = st[cid]; p < e; ++p) {
+      const float4 pv = sp[p];
+      const float d = l2_simple(qv, pv);
+      const int oi = __float_as_int(pv.w);
+      if (d < best || (d == best && oi < bidx)) {
+        best = d;
+        bidx = oi;
+        bpos = p;
+      }
+    }
+  };
+  auto thr2 = [&]() {
+    if (bpos < 0) return INFINITY;
+    const float t = sqrtf(best) + slack;
+    return t * t;
+  };
+  for (int t = 0; t < 2 * n && (up || down); ++t) {
+    int b;
+    if (t == 0) {
+      b = qi;
+    } else if (t & 1) {
+      if (!up) continue;
+      b = qi + (t + 1) / 2;
+      if (b >= n) {
+        up = false;
+        continue;
+      }
+    } else {
+      if (!down) continue;
+      b = qi - t / 2;
+      if (b < 0) {
+        down = false;
+        continue;
+      }
+    }
+    const float lo = b * dband - eps, hi = (b + 1) * dband + eps;
+    const float dth = fmaxf(0.f, fmaxf(lo - thq, thq - hi));
+    const float s1 = sinf(0.5f * dth);
+    const float A = 4.f * s1 * s1;
+    if (A > thr2()) {  // every farther band in this direction is farther still
+      if (t == 0)
+        up = down = false;
+      else if (t & 1)
+        up = false;
+      else
+        down = false;
+      continue;
+    }
+    if (bs[b + 1] == bs[b]) continue;
+    const float spm = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
+    const float Bc = 2.f * sq * spm;
+    scan_cell(b * n + qj);
+    // right: sector distances 1..n/2; left: 1..(n-1)/2 (each sector once)
+    for (int side = 0; side < 2; ++side) {
+      int u_prev = 0;
+      while (true) {
+        int sec, u;
+        if (side == 0) {
+          sec = nr[b * n + (qj + u_prev + 1) % n];
+          if (sec < 0) break;
+          u = (sec - qj + n) % n;
+          if (u <= u_prev || u > n / 2) break;
+        } else {
+          sec = nl[b * n + ((qj - u_prev - 1) % n + n) % n];
+          if (sec < 0) break;
+          u = (qj - sec + n) % n;
+          if (u <= u_prev || u > (n - 1) / 2) break;
+        }
+        const float dphi = fmaxf((u - 0.5f) * dsec - eps, 0.f);
+        const float s2 = sinf(0.5f * dphi);
+        if (A + Bc * 2.f * s2 * s2 > thr2()) break;
+        scan_cell(b * n + sec);
+        u_prev = u;
+      }
+    }
+  }
+  float r = 0.f, it = 0.f;
+  if (bpos >= 0 && best <= max_sq) {
+    const float2 v = sorted_val[c * cap + bpos];
+    r = v.x;
+    it = v.y;
+  }
+  grids[(int64_t)c * 2 * nq + q] = r;
+  grids[(int64_t)c * 2 * nq + nq + q] = it;
+}
+
+// ---------------------------------------------------------------- FST
+// F[s][m][i] = (sqrt(2 pi)/n) sum_j x[i][j] e^{-2 pi i m j / n}, m < bw
+// (s2_semi_memo.c:156-177)
+__global__ void k_phi_dft(const float* __restrict__ sig, int n, int bw,
+                          const float2* __restrict__ tw, float2* __restrict__ F) {
+  const int i = blockIdx.x, s = blockIdx.y;
+  extern __shared__ float smem[];
+  float* row = smem;
+  float2* t = (float2*)(smem + n);
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    row[j] = sig[((int64_t)s * n + i) * n + j];
+    t[j] = tw[j];
+  }
+  __syncthreads();
+  const float scale = 2.5066282746310002f / (float)n;  // sqrt(2 pi) / n
+  for (int m = threadIdx.x; m < bw; m += blockDim.x) {
+    float ar = 0.f, ai = 0.f;
+    int e = 0;
+    for (int j = 0; j < n; ++j) {
+      const float2 w = t[e];
+      ar += row[j] * w.x;
+      ai += row[j] * w.y;
+      e += m;
+      if (e >= n) e -= n;
+    }
+    F[((int64_t)s * bw + m) * n + i] = make_float2(ar * scale, ai * scale);
+  }
+}
+
+// coef[s][sean(m,l)] = sum_i (w_i P~_l^m(cos th_i)) F[s][m][i]; negative
+// orders by real-data symmetry (s2_semi_memo.c:349-373)
+__global__ void k_legendre(const float2* __restrict__ F, int S, int n, int bw,
+                           const float* __restrict__ leg, const int* __restrict__ leg_off,
+                           float2* __restrict__ coef) {
+  const int m = blockIdx.x;
+  const int s0 = blockIdx.y * 4;
+  const int ns = min(4, S - s0);
+  extern __shared__ float2 fs[];
+  for (int idx = threadIdx.x; idx < ns * n; idx += blockDim.x) {
+    const int ss = idx / n, i = idx - ss * n;
+    fs[idx] = F[((int64_t)(s0 + ss) * bw + m) * n + i];
+  }
+  __syncthreads();
+  const int L = bw - m;
+  const float* T = leg + leg_off[m];
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    float2 acc[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
+                     make_float2(0.f, 0.f)};
+    for (int i = 0; i < n; ++i) {
+      const float w = T[(int64_t)i * L + t];
+#pragma unroll
+      for (int ss = 0; ss < 4; ++ss)
+        if (ss < ns) {
+          acc[ss].x += w * fs[ss * n + i].x;
+          acc[ss].y += w * fs[ss * n + i].y;
+        }
+    }
+    const int l = m + t;
+    const int L1 = bw - 1;
+    const int ip = m * (L1 + 1) - ((m * (m - 1)) / 2) + (l - m);
+    const int im = ((L1 * (L1 + 3)) / 2) + 1 + ((L1 - m) * (L1 - m + 1) / 2) + (l - m);
+    const float sg = (m & 1) ? -1.f : 1.f;
+    for (int ss = 0; ss < ns; ++ss) {
+      float2* out = coef + (int64_t)(s0 + ss) * bw * bw;
+      out[ip] = acc[ss];
+      if (m > 0) out[im] = make_float2(sg * acc[ss].x, -sg * acc[ss].y);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- fusion
+// LaplacePyramid::fuseChannels (laplace-pyramid.cc:60-151): per index, the
+// channel of largest |.|^2 (first on ties), the mean over all channels in the
+// innermost low-pass band, or zero where the reference's expand leaves one.
+__global__ void k_fuse(ChannelSet cs, uint32_t count, FusionRuns runs, float2* __restrict__ out) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  int mode = 0;
+  for (int r = 0; r < runs.count; ++r)
+    if (q >= runs.start[r]) mode = (int)runs.mode[r];
+  float2 v = make_float2(0.f, 0.f);
+  if (mode == 0) {
+    float be = -1.f;
+    for (int c = 0; c < cs.C; ++c) {
+      const float2 x = cs.ch[c] ? cs.ch[c][q] : make_float2(0.f, 0.f);
+      const float e = x.x * x.x + x.y * x.y;
+      if (e > be) {
+        be = e;
+        v = x;
+      }
+    }
+  } else if (mode == 1) {
+    float ar = 0.f, ai = 0.f;
+    for (int c = 0; c < cs.C; ++c)
+      if (cs.ch[c]) {
+        ar += cs.ch[c][q].x;
+        ai += cs.ch[c][q].y;
+      }
+    v = make_float2(ar / (float)cs.C, ai / (float)cs.C);
+  }
+  out[q] = v;
+}
+
+}  // namespace
+
+void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
+                      const float4* grid, const float* theta, float max_sq, CellBuffers cb,
+                      float* grids, hipStream_t s) {
+  const int ncell = n * n;
+  launch_fill_u32((uint32_t*)cb.cell_count, (size_t)nclouds * ncell, 0u, s);
+  dim3 g((unsigned)((cap + 255) / 256), nclouds);
+  hipLaunchKernelGGL(k_project, g, dim3(256), 0, s, pts, npts, cap, n, cb.proj, cb.val, cb.cell,
+                     cb.cell_count);
+  hipLaunchKernelGGL(k_cell_scan, dim3(nclouds), dim3(1024), 0, s, n, cb.cell_count,
+                     cb.cell_start, cb.band_start);
+  hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, n, cb.proj, cb.val, cb.cell,
+                     cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
+  hipLaunchKernelGGL(k_jump_tables, dim3((n + 63) / 64, nclouds), dim3(64), 0, s, n,
+                     cb.cell_start, cb.next_right, cb.next_left);
+  dim3 gq((unsigned)((ncell + 63) / 64), nclouds);
+  hipLaunchKernelGGL(k_nn, gq, dim3(64), 0, s, grid, theta, n, max_sq, cb.cell_start,
+                     cb.band_start, cb.next_right, cb.next_left, cb.sorted, cb.sorted_val, cap,
+                     grids);
+}
+
+void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,
+                const float2* tw_n, float2* F, float2* coef, hipStream_t s) {
+  const int n = 2 * bw;
+  const int th = ((bw + 63) / 64) * 64;
+  hipLaunchKernelGGL(k_phi_dft, dim3(n, S), dim3(th), n * sizeof(float) + n * sizeof(float2), s,
+                     sig, n, bw, tw_n, F);
+  hipLaunchKernelGGL(k_legendre, dim3(bw, (S + 3) / 4), dim3(th), 4 * n * sizeof(float2), s, F, S,
+                     n, bw, legendre, leg_off_dev, coef);
+}
+
+void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_fuse, dim3((count + 255) / 256), dim3(256), 0, s, cs, count, runs, out);
+}
+
+}  // namespace phz

@@ -1,0 +1,323 @@
+// kernels_so3.hip -- SO(3) correlation for gfx950: coefficient combination
+// fused into the Wigner-d synthesis, the 2-D inverse FFT over (m1, m2) ->
+// (alpha, gamma) with |.|^2 and a first-maximum argmax, and the pose.
+//
+// Restates so3CombineCoef_fftw (so3_correlate_fftw.c:102-161) and
+// Inverse_SO3_Naive_fftw with flag = 1 (soft_fftw.c:1036-1912): the
+// reference synthesises S(m1,m2,beta) for one half of the (m1,m2) plane and
+// fills the other half with conjugates, so the 2-D transform is Hermitian and
+// its result real; here the half m2 >= 0 is synthesised (with the reference's
+// own choice of which entries are direct and which are conjugate copies), the
+// transform over m1 is a complex FFT and the one over m2 a real-output FFT.
+#include "kernels.hpp"
+
+namespace phz {
+
+namespace {
+
+__device__ __forceinline__ int sean(int m, int l, int bw) {
+  const int L = bw - 1;
+  if (m >= 0) return m * (L + 1) - ((m * (m - 1)) / 2) + (l - m);
+  return ((L * (L + 3)) / 2) + 1 + ((L + m) * (L + m + 1) / 2) + (l + m);
+}
+
+// c^l_{m1,m2} = wigNorm (-1)^{m1+m2} f_{l,-m1} conj(h_{l,-m2})
+__device__ __forceinline__ float2 combine(const float2* f, const float2* h, float wn, int l,
+                                          int m1, int m2, int bw) {
+  const float2 s = f[sean(-m1, l, bw)];
+  const float2 p = h[sean(-m2, l, bw)];
+  const float fudge = ((m1 + m2) & 1) ? -1.f : 1.f;
+  const float pr = fudge * p.x, pi = fudge * (-p.y);
+  return make_float2(wn * (s.x * pr - s.y * pi), wn * (s.x * pi + s.y * pr));
+}
+
+// One workgroup per Wigner block (a <= b) and pair; thread k = beta index.
+// Columns: 0 -> (a,b) direct, 1 -> (b,a) direct, 2/3 -> beta-flipped
+// (see the case table in the host launcher).
+__global__ void k_so3_synth(int bw, const float2* __restrict__ fc, int64_t f_stride,
+                            const float2* __restrict__ hc, int64_t h_stride,
+                            const float* __restrict__ wig,
+                            const int64_t* __restrict__ wig_off, const float* __restrict__ wnorm,
+                            float2* __restrict__ S) {
+  const int n = 2 * bw;
+  const int blk = blockIdx.x, pair = blockIdx.y;
+  int b = (int)((sqrtf(8.f * blk + 1.f) - 1.f) * 0.5f);
+  while ((b + 1) * (b + 2) / 2 <= blk) ++b;
+  while (b * (b + 1) / 2 > blk) --b;
+  const int a = blk - b * (b + 1) / 2;
+  const int L = bw - b;
+  const float2* f = fc + pair * f_stride;
+  const float2* h = hc + pair * h_stride;
+  extern __shared__ float2 col[];  // 4 x bw
+  // case: 0 -> (0,0); 1 -> a=0<b; 2 -> 0<a=b; 3 -> 0<a<b
+  const int cs = (b == 0) ? 0 : (a == 0 ? 1 : (a == b ? 2 : 3));
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    const int l = b + t;
+    const float wn = wnorm[l];
+    float2 c0 = combine(f, h, wn, l, a, b, bw);
+    float2 c1 = make_float2(0.f, 0.f), c2 = c1, c3 = c1;
+    if (cs == 1) {
+      c1 = combine(f, h, wn, l, b, 0, bw);
+      if (b & 1) c1 = make_float2(-c1.x, -c1.y);
+    } else if (cs == 2) {
+      c2 = combine(f, h, wn, l, -a, a, bw);
+      const float sg = ((l - a) & 1) ? -1.f : 1.f;
+      c2 = make_float2(sg * c2.x, sg * c2.y);
+    } else if (cs == 3) {
+      c1 = combine(f, h, wn, l, b, a, bw);
+      if ((b - a) & 1) c1 = make_float2(-c1.x, -c1.y);
+      const float sg = ((l + a) & 1) ? -1.f : 1.f;
+      c2 = combine(f, h, wn, l, a, -b, bw);
+      c2 = make_float2(sg * c2.x, sg * c2.y);
+      c3 = combine(f, h, wn, l, b, -a, bw);
+      c3 = make_float2(sg * c3.x, sg * c3.y);
+    }
+    col[t] = c0;
+    col[bw + t] = c1;
+    col[2 * bw + t] = c2;
+    col[3 * bw + t] = c3;
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k >= n) return;
+  const float* T = wig + wig_off[blk];
+  float2 a0 = make_float2(0.f, 0.f), a1 = a0, a2 = a0, a3 = a0;
+  if (cs <= 1) {
+    for (int t = 0; t < L; ++t) {
+      const float d = T[(int64_t)t * n + k];
+      a0.x += col[t].x * d;
+      a0.y += col[t].y * d;
+      a1.x += col[bw + t].x * d;
+      a1.y += col[bw + t].y * d;
+    }
+  } else if (cs == 2) {
+    for (int t = 0; t < L; ++t) {
+      const float d = T[(int64_t)t * n + k];
+      const float df = T[(int64_t)t * n + (n - 1 - k)];
+      a0.x += col[t].x * d;
+      a0.y += col[t].y * d;
+      a2.x += col[2 * bw + t].x * df;
+      a2.y += col[2 * bw + t].y * df;
+    }
+  } else {
+    for (int t = 0; t < L; ++t) {
+      const float d = T[(int64_t)t * n + k];
+      const float df = T[(int64_t)t * n + (n - 1 - k)];
+      a0.x += col[t].x * d;
+      a0.y += col[t].y * d;
+      a1.x += col[bw + t].x * d;
+      a1.y += col[bw + t].y * d;
+      a2.x += col[2 * bw + t].x * df;
+      a2.y += col[2 * bw + t].y * df;
+      a3.x += col[3 * bw + t].x * df;
+      a3.y += col[3 * bw + t].y * df;
+    }
+  }
+  const int w1 = 2 * bw - 1;
+  float2* Sp = S + (int64_t)pair * bw * w1 * n;
+  auto at = [&](int m1, int m2) -> float2* { return Sp + ((int64_t)m2 * w1 + (m1 + bw - 1)) * n + k; };
+  *at(a, b) = a0;  // (a,b); also (0,0) and (a,a)
+  if (cs == 1) {
+    *at(b, 0) = a1;
+    *at(-b, 0) = cconj(a1);
+  } else if (cs == 2) {
+    *at(-a, a) = a2;
+  } else if (cs == 3) {
+    *at(b, a) = a1;
+    *at(-a, b) = cconj(a2);
+    *at(-b, a) = cconj(a3);
+  }
+}
+
+// FFT over m1 (M1 = m1 mod n, Nyquist row zero): U[k][m2][alpha]
+__global__ void k_so3_fft_alpha(int bw, const float2* __restrict__ S, FftPlan plan,
+                                const float2* __restrict__ tw, float2* __restrict__ U) {
+  constexpr int KT = 16;
+  const int n = 2 * bw, w1 = 2 * bw - 1;
+  const int m2 = blockIdx.x, k0 = blockIdx.y * KT, pair = blockIdx.z;
+  extern __shared__ float2 sm[];
+  float2* A = sm;
+  float2* B = sm + KT * n;
+  const float2* Sp = S + ((int64_t)pair * bw + m2) * w1 * n;
+  for (int idx = threadIdx.x; idx < KT * n; idx += blockDim.x) A[idx] = make_float2(0.f, 0.f);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < w1 * KT; idx += blockDim.x) {
+    const int mi = idx / KT, kk = idx - mi * KT;
+    const int m1 = mi - (bw - 1);
+    const int M1 = m1 < 0 ? m1 + n : m1;
+    if (k0 + kk < n) A[kk * n + M1] = Sp[(int64_t)mi * n + k0 + kk];
+  }
+  __syncthreads();
+  float2* R = lds_fft<-1>(A, B, KT, plan, tw);
+  float2* Up = U + (int64_t)pair * n * bw * n;
+  for (int idx = threadIdx.x; idx < KT * n; idx += blockDim.x) {
+    const int kk = idx / n, al = idx - kk * n;
+    if (k0 + kk < n) Up[((int64_t)(k0 + kk) * bw + m2) * n + al] = R[idx];
+  }
+}
+
+// Hermitian FFT over m2 -> gamma for AT alpha values (two per complex
+// line), scale (1/n)(bw/pi) (soft_fftw.c:1905-1911), |C|^2, argmax.
+__global__ void k_so3_fft_gamma(int bw, const float2* __restrict__ U, FftPlan plan,
+                                const float2* __restrict__ tw, float* __restrict__ corr,
+                                int64_t corr_stride, unsigned long long* __restrict__ keys) {
+  constexpr int AT = 16, NL = AT / 2;
+  const int n = 2 * bw;
+  const int a0 = blockIdx.x * AT, k = blockIdx.y, pair = blockIdx.z;
+  extern __shared__ float2 sm[];
+  float2* X = sm;             // AT x bw staging
+  float2* A = sm + AT * bw;   // NL x n
+  float2* B = A + NL * n;
+  const float2* Up = U + ((int64_t)pair * n + k) * bw * n;
+  for (int idx = threadIdx.x; idx < bw * AT; idx += blockDim.x) {
+    const int m2 = idx / AT, aa = idx - m2 * AT;
+    X[aa * bw + m2] = (a0 + aa < n) ? Up[(int64_t)m2 * n + a0 + aa] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < NL * n; idx += blockDim.x) {
+    const int p = idx / n, m = idx - p * n;
+    const float2* xa = X + (2 * p) * bw;
+    const float2* xb = X + (2 * p + 1) * bw;
+    float2 va, vb;
+    if (m == 0) {
+      va = make_float2(xa[0].x, 0.f);
+      vb = make_float2(xb[0].x, 0.f);
+    } else if (m < bw) {
+      va = xa[m];
+      vb = xb[m];
+    } else if (m == bw) {
+      va = vb = make_float2(0.f, 0.f);
+    } else {
+      va = cconj(xa[n - m]);
+      vb = cconj(xb[n - m]);
+    }
+    A[idx] = make_float2(va.x - vb.y, va.y + vb.x);  // va + i vb
+  }
+  __syncthreads();
+  float2* R = lds_fft<-1>(A, B, NL, plan, tw);
+  const float scale = (1.f / (float)n) * ((float)bw / 3.14159265358979323846f);
+  unsigned long long best = 0ull;
+  for (int idx = threadIdx.x; idx < AT * n; idx += blockDim.x) {
+    const int aa = idx / n, g = idx - aa * n;
+    if (a0 + aa >= n) continue;
+    const float2 z = R[(aa >> 1) * n + g];
+    const float c = ((aa & 1) ? z.y : z.x) * scale;
+    const float v = c * c;
+    const uint32_t gi = (uint32_t)(((int64_t)k * n + (a0 + aa)) * n + g);
+    if (corr) corr[pair * corr_stride + gi] = v;
+    const unsigned long long key = make_key(v, gi);
+    best = key > best ? key : best;
+  }
+  best = wave_max_key(best);
+  if ((threadIdx.x & 63) == 0) atomicMax(&keys[pair], best);
+}
+
+__device__ void mat4_mul_nc(const float* A, const float* B, float* R) {
+#pragma clang fp contract(off)
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      float s = A[4 * i] * B[j];
+      s = s + A[4 * i + 1] * B[4 + j];
+      s = s + A[4 * i + 2] * B[8 + j];
+      s = s + A[4 * i + 3] * B[12 + j];
+      R[4 * i + j] = s;
+    }
+}
+
+// rotation-utils.cc:190-207 (index -> ZYZ), :101-107 (quaternion), :86-99,
+// :166-173 (XYZ Euler), :17-23,109-143 (T = Tx Ty Tz', float)
+__global__ void k_rot_pose(int bw, int npairs, const unsigned long long* __restrict__ keys,
+                           DevPose* __restrict__ pose) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const unsigned long long key = keys[p];
+  const uint32_t idx = key_index(key);
+  const uint32_t n = 2 * bw;
+  const uint32_t ii = idx / (n * n), jj = (idx / n) % n, kk = idx % n;
+  DevPose& P = pose[p];
+  P.rot_index = idx;
+  P.rot_value = unordered_value((uint32_t)(key >> 32));
+  const double bd = (double)bw;
+  P.zyz[0] = M_PI * jj / bd;
+  P.zyz[1] = M_PI * (2 * ii + 1) / (4. * bw);
+  P.zyz[2] = M_PI * kk / bd;
+  const double qa[4] = {cos(0.5 * P.zyz[0]), 0, 0, sin(0.5 * P.zyz[0])};
+  const double qb[4] = {cos(0.5 * P.zyz[1]), 0, sin(0.5 * P.zyz[1]), 0};
+  const double qc[4] = {cos(0.5 * P.zyz[2]), 0, 0, sin(0.5 * P.zyz[2])};
+  double t[4], q[4];
+  auto qmul = [](const double* x, const double* y, double* r) {
+    r[0] = x[0] * y[0] - x[1] * y[1] - x[2] * y[2] - x[3] * y[3];
+    r[1] = x[0] * y[1] + x[1] * y[0] + x[2] * y[3] - x[3] * y[2];
+    r[2] = x[0] * y[2] + x[2] * y[0] + x[3] * y[1] - x[1] * y[3];
+    r[3] = x[0] * y[3] + x[3] * y[0] + x[1] * y[2] - x[2] * y[1];
+  };
+  qmul(qa, qb, t);
+  qmul(t, qc, q);
+  for (int d = 0; d < 4; ++d) P.quat[d] = q[d];
+  const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+  const double r11 = -2 * (qy * qz - qw * qx);
+  const double r12 = qw * qw - qx * qx - qy * qy + qz * qz;
+  const double r21 = 2 * (qx * qz + qw * qy);
+  const double r31 = -2 * (qx * qy - qw * qz);
+  const double r32 = qw * qw + qx * qx - qy * qy - qz * qz;
+  P.euler[2] = -atan2(r31, r32);
+  P.euler[1] = asin(r21);
+  P.euler[0] = atan2(r11, r12);
+  const float ea = (float)P.euler[0], eb = (float)P.euler[1], ec = (float)P.euler[2];
+  float X[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+  float Y[16], Z[16], XY[16];
+  for (int e = 0; e < 16; ++e) Y[e] = Z[e] = X[e];
+  X[5] = cosf(ea);
+  X[6] = -sinf(ea);
+  X[9] = sinf(ea);
+  X[10] = cosf(ea);
+  Y[0] = cosf(eb);
+  Y[2] = sinf(eb);
+  Y[8] = -sinf(eb);
+  Y[10] = cosf(eb);
+  Z[0] = cosf(ec);
+  Z[1] = sinf(ec);
+  Z[4] = -sinf(ec);
+  Z[5] = cosf(ec);
+  mat4_mul_nc(X, Y, XY);
+  mat4_mul_nc(XY, Z, P.T);
+}
+
+}  // namespace
+
+void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
+                      int64_t h_stride, int npairs, const float* wigner,
+                      const int64_t* wig_off_dev, const float* wig_norm, float2* S,
+                      hipStream_t s) {
+  const int n = 2 * bw;
+  const int nblk = bw * (bw + 1) / 2;
+  const int th = ((n + 63) / 64) * 64;
+  hipLaunchKernelGGL(k_so3_synth, dim3(nblk, npairs), dim3(th), 4 * bw * sizeof(float2), s, bw,
+                     f_coef, f_stride, h_coef, h_stride, wigner, wig_off_dev, wig_norm, S);
+}
+
+void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
+                          const float2* tw, float2* U, hipStream_t s) {
+  const int n = 2 * bw;
+  dim3 g(bw, (n + 15) / 16, npairs);
+  hipLaunchKernelGGL(k_so3_fft_alpha, g, dim3(256), 2 * 16 * n * sizeof(float2), s, bw, S, plan,
+                     tw, U);
+}
+
+void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& plan,
+                          const float2* tw, float* corr, int64_t corr_stride,
+                          unsigned long long* keys, hipStream_t s) {
+  const int n = 2 * bw;
+  dim3 g((n + 15) / 16, n, npairs);
+  const size_t sh = (16 * bw + 2 * 8 * n) * sizeof(float2);
+  hipLaunchKernelGGL(k_so3_fft_gamma, g, dim3(256), sh, s, bw, U, plan, tw, corr, corr_stride,
+                     keys);
+}
+
+void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose* pose,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_rot_pose, dim3((npairs + 63) / 64), dim3(64), 0, s, bw, npairs, keys,
+                     pose);
+}
+
+}  // namespace phz

@@ -1,0 +1,382 @@
+// kernels_spatial.hip -- translation stage for gfx950: voxelisation,
+// 3-D real FFT of the voxel means, Laplace channel fusion fused with the
+// cross-power spectrum, inverse 3-D FFT and a signed first-maximum argmax.
+//
+// Restates PhaseAligner::discretizePointcloud (phase-aligner.cc:71-131,
+// include/igl/histc.cpp:34-73) and SpatialCorrelationLaplace::
+// correlateSignals (spatial-correlation-laplace.cc:17-89,
+// spatial-correlation.cc:142-161): C = F conj(G) with no normalisation on the
+// r2c half spectrum, c2r of the first N^2(N/2+1) entries.  Grid layout is
+// the reference's: linear index x*N + y + N^2*z, i.e. [z][x][y].
+#include "kernels.hpp"
+
+namespace phz {
+
+namespace {
+
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// igl::histc bin of x against monotone float edges E[0..N)
+__device__ __forceinline__ int histc_bin(float x, const float* E, int N) {
+  if (x < E[0] || x > E[N - 1]) return -1;
+  int l = 0, h = N - 1;
+  while (h - l > 1) {
+    const int k = (h + l) >> 1;
+    if (x < E[k])
+      h = k;
+    else
+      l = k;
+  }
+  return (x == E[h]) ? h : l;
+}
+
+__global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __restrict__ npts,
+                                int64_t cap, const DevPose* __restrict__ pose, int pose_stride,
+                                int N, const float* __restrict__ edges, float* __restrict__ sumI,
+                                float* __restrict__ sumR, uint32_t* __restrict__ cnt) {
+#pragma clang fp contract(off)
+  extern __shared__ float E[];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) E[i] = edges[i];
+  __syncthreads();
+  const int c = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npts[c]) return;
+  float4 p = pts[c * cap + i];
+  if (pose) {
+    const float* T = pose[c * pose_stride].T;
+    const float x = T[0] * p.x + T[1] * p.y + T[2] * p.z + T[3];
+    const float y = T[4] * p.x + T[5] * p.y + T[6] * p.z + T[7];
+    const float z = T[8] * p.x + T[9] * p.y + T[10] * p.z + T[11];
+    p.x = x;
+    p.y = y;
+    p.z = z;
+  }
+  const uint32_t bx = (uint32_t)histc_bin(p.x, E, N);
+  const uint32_t by = (uint32_t)histc_bin(p.y, E, N);
+  const uint32_t bz = (uint32_t)histc_bin(p.z, E, N);
+  const uint32_t NN = (uint32_t)N;
+  const uint32_t lin = (bx * NN + by) + (NN * NN * bz);  // Sub2Ind in uint32
+  const uint32_t nf = NN * NN * NN;
+  if (lin >= nf) return;  // the reference skips lin > nf and overruns at nf
+  const float range = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+  const size_t off = (size_t)c * nf + lin;
+  atomicAdd(&sumI[off], p.w);
+  atomicAdd(&sumR[off], range);
+  atomicAdd(&cnt[off], 1u);
+}
+
+// pass 1: r2c along y of (mean I + i mean R) rows, split into the two
+// half spectra.  spec[cloud][ch][row][k], row = z*N + x.
+template <bool kDense>
+__global__ void k_spat_r2c(int N, const float* __restrict__ a0, const float* __restrict__ a1,
+                           const uint32_t* __restrict__ cnt, int64_t cloud_stride, FftPlan plan,
+                           const float2* __restrict__ tw, float2* __restrict__ spec,
+                           int64_t spec_stride0, int64_t spec_stride1) {
+  constexpr int RT = 16;
+  const int row0 = blockIdx.x * RT, c = blockIdx.y;
+  const int rows = N * N, h = N / 2 + 1;
+  extern __shared__ float2 sm[];
+  float2* A = sm;
+  float2* B = sm + RT * N;
+  for (int idx = threadIdx.x; idx < RT * N; idx += blockDim.x) {
+    const int r = idx / N, y = idx - r * N;
+    float2 v = make_float2(0.f, 0.f);
+    if (row0 + r < rows) {
+      const int64_t off = c * cloud_stride + (int64_t)(row0 + r) * N + y;
+      if (kDense) {
+        v.x = a0[off];
+        v.y = a1 ? a1[off] : 0.f;
+      } else {
+        const uint32_t k = cnt[off];
+        if (k) {
+          v.x = a0[off] / (float)k;
+          v.y = a1[off] / (float)k;
+        }
+      }
+    }
+    A[idx] = v;
+  }
+  __syncthreads();
+  float2* R = lds_fft<-1>(A, B, RT, plan, tw);
+  for (int idx = threadIdx.x; idx < RT * h; idx += blockDim.x) {
+    const int r = idx / h, k = idx - r * h;
+    if (row0 + r >= rows) continue;
+    const float2 zk = R[r * N + k];
+    const float2 zc = cconj(R[r * N + ((N - k) % N)]);
+    const float2 s = cadd(zk, zc), d = csub(zk, zc);
+    const int64_t o = (int64_t)(row0 + r) * h + k;
+    spec[c * spec_stride0 + o] = make_float2(0.5f * s.x, 0.5f * s.y);
+    if (spec_stride1 >= 0) spec[c * spec_stride0 + spec_stride1 + o] = make_float2(0.5f * d.y, -0.5f * d.x);
+  }
+}
+
+// in-place FFT along the middle axis of [outer][L][inner] arrays; one
+// workgroup per (inner tile, outer, batch array)
+template <int DIR>
+__global__ void k_fft_cols(float2* __restrict__ data, int64_t batch_stride, int L, int inner,
+                           FftPlan plan, const float2* __restrict__ tw) {
+  constexpr int IT = 16;
+  const int i0 = blockIdx.x * IT, o = blockIdx.y;
+  float2* base = data + blockIdx.z * batch_stride + (int64_t)o * L * inner;
+  extern __shared__ float2 sm[];
+  float2* A = sm;
+  float2* B = sm + IT * L;
+  for (int idx = threadIdx.x; idx < IT * L; idx += blockDim.x) {
+    const int e = idx / IT, ii = idx - e * IT;
+    A[ii * L + e] = (i0 + ii < inner) ? base[(int64_t)e * inner + i0 + ii] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* R = lds_fft<DIR>(A, B, IT, plan, tw);
+  for (int idx = threadIdx.x; idx < IT * L; idx += blockDim.x) {
+    const int e = idx / IT, ii = idx - e * IT;
+    if (i0 + ii < inner) base[(int64_t)e * inner + i0 + ii] = R[ii * L + e];
+  }
+}
+
+__device__ __forceinline__ float2 fuse_at(const ChannelSet& cs, int mode, int64_t q) {
+  float2 v = make_float2(0.f, 0.f);
+  if (mode == 0) {
+    float be = -1.f;
+    for (int c = 0; c < cs.C; ++c) {
+      const float2 x = cs.ch[c] ? cs.ch[c][q] : make_float2(0.f, 0.f);
+      const float e = x.x * x.x + x.y * x.y;
+      if (e > be) {
+        be = e;
+        v = x;
+      }
+    }
+  } else if (mode == 1) {
+    float ar = 0.f, ai = 0.f;
+    for (int c = 0; c < cs.C; ++c)
+      if (cs.ch[c]) {
+        ar += cs.ch[c][q].x;
+        ai += cs.ch[c][q].y;
+      }
+    v = make_float2(ar / (float)cs.C, ai / (float)cs.C);
+  }
+  return v;
+}
+
+// inverse pass A: fused F, G -> C = F conj(G) (spatial-correlation.cc:158-159)
+// -> inverse FFT along z.  work[z][x][k2].
+__global__ void k_spat_cross_z(int N, ChannelSet F, ChannelSet G, FusionRuns runs, int fuse,
+                               FftPlan plan, const float2* __restrict__ tw,
+                               float2* __restrict__ work) {
+  constexpr int IT = 16;
+  const int h = N / 2 + 1, inner = N * h;
+  const int i0 = blockIdx.x * IT;
+  extern __shared__ float2 sm[];
+  float2* A = sm;
+  float2* B = sm + IT * N;
+  for (int idx = threadIdx.x; idx < IT * N; idx += blockDim.x) {
+    const int z = idx / IT, ii = idx - z * IT;
+    float2 c = make_float2(0.f, 0.f);
+    if (i0 + ii < inner) {
+      const int64_t q = (int64_t)z * inner + i0 + ii;
+      float2 f, g;
+      if (fuse) {
+        int mode = 0;
+        for (int r = 0; r < runs.count; ++r)
+          if ((uint64_t)q >= runs.start[r]) mode = (int)runs.mode[r];
+        f = fuse_at(F, mode, q);
+        g = fuse_at(G, mode, q);
+      } else {
+        f = F.ch[0][q];
+        g = G.ch[0][q];
+      }
+      c.x = f.x * g.x - f.y * (-g.y);
+      c.y = f.x * (-g.y) + f.y * g.x;
+    }
+    A[ii * N + z] = c;
+  }
+  __syncthreads();
+  float2* R = lds_fft<+1>(A, B, IT, plan, tw);
+  for (int idx = threadIdx.x; idx < IT * N; idx += blockDim.x) {
+    const int z = idx / IT, ii = idx - z * IT;
+    if (i0 + ii < inner) work[(int64_t)z * inner + i0 + ii] = R[ii * N + z];
+  }
+}
+
+// last pass: c2r along y (real parts of k=0 and k=N/2, FFTW hc2r), two rows
+// per complex line, signed argmax over x*N + y + N^2*z.
+__global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
+                           const float2* __restrict__ tw, float* __restrict__ corr,
+                           unsigned long long* __restrict__ key) {
+  constexpr int RT = 16, NL = RT / 2;
+  const int h = N / 2 + 1, rows = N * N;
+  const int row0 = blockIdx.x * RT;
+  extern __shared__ float2 sm[];
+  float2* X = sm;  // RT x h
+  float2* A = sm + RT * h;
+  float2* B = A + NL * N;
+  for (int idx = threadIdx.x; idx < RT * h; idx += blockDim.x) {
+    const int r = idx / h, k = idx - r * h;
+    X[idx] = (row0 + r < rows) ? work[(int64_t)(row0 + r) * h + k] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < NL * N; idx += blockDim.x) {
+    const int p = idx / N, m = idx - p * N;
+    const float2* xa = X + (2 * p) * h;
+    const float2* xb = X + (2 * p + 1) * h;
+    float2 va, vb;
+    if (m == 0 || 2 * m == N) {
+      va = make_float2(xa[m].x, 0.f);
+      vb = make_float2(xb[m].x, 0.f);
+    } else if (m < h) {
+      va = xa[m];
+      vb = xb[m];
+    } else {
+      va = cconj(xa[N - m]);
+      vb = cconj(xb[N - m]);
+    }
+    A[idx] = make_float2(va.x - vb.y, va.y + vb.x);
+  }
+  __syncthreads();
+  float2* R = lds_fft<+1>(A, B, NL, plan, tw);
+  unsigned long long best = 0ull;
+  for (int idx = threadIdx.x; idx < RT * N; idx += blockDim.x) {
+    const int r = idx / N, y = idx - r * N;
+    if (row0 + r >= rows) continue;
+    const float2 z = R[(r >> 1) * N + y];
+    const float v = (r & 1) ? z.y : z.x;
+    const uint32_t lin = (uint32_t)((int64_t)(row0 + r) * N + y);
+    if (corr) corr[lin] = v;
+    const unsigned long long kk = make_key(v, lin);
+    best = kk > best ? kk : best;
+  }
+  best = wave_max_key(best);
+  if ((threadIdx.x & 63) == 0) atomicMax(key, best);
+}
+
+// translation-utils.cc:40-52 + signal-utils.cc:11-19
+__global__ void k_trans_pose(int N, int lower, int upper, int npairs,
+                             const unsigned long long* __restrict__ keys,
+                             DevPose* __restrict__ pose) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const unsigned long long key = keys[p];
+  const uint32_t idx = key_index(key);
+  DevPose& P = pose[p];
+  P.trans_index = idx;
+  P.trans_value = unordered_value((uint32_t)(key >> 32));
+  uint32_t xyz[3];
+  xyz[1] = idx % (uint32_t)N;
+  const uint32_t u = idx / (uint32_t)N;
+  xyz[0] = u % (uint32_t)N;
+  xyz[2] = u / (uint32_t)N;
+  const double width = (double)(abs(lower) + abs(upper) + 1);
+  const double half = N / 2.0;
+  for (int d = 0; d < 3; ++d) {
+    const double index = (double)xyz[d];
+    double t;
+    if (index == 0)
+      t = 0.0;
+    else if (index <= half)
+      t = index * width / N;
+    else
+      t = (index - N) * width / N;
+    P.translation[d] = t;
+  }
+}
+
+__global__ void k_apply_pose(const float4* __restrict__ pts, int64_t P,
+                             const DevPose* __restrict__ pose, float4* __restrict__ out) {
+#pragma clang fp contract(off)
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float* T = pose->T;
+  const float4 p = pts[i];
+  const float x = T[0] * p.x + T[1] * p.y + T[2] * p.z + T[3];
+  const float y = T[4] * p.x + T[5] * p.y + T[6] * p.z + T[7];
+  const float z = T[8] * p.x + T[9] * p.y + T[10] * p.z + T[11];
+  out[i] = make_float4(x + (float)pose->translation[0], y + (float)pose->translation[1],
+                       z + (float)pose->translation[2], p.w);
+}
+
+}  // namespace
+
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
+                          const DevPose* pose, int pose_stride, int N, const float* edges,
+                          float* sumI, float* sumR, uint32_t* cnt, hipStream_t s) {
+  dim3 g((unsigned)((cap + 255) / 256), nclouds);
+  hipLaunchKernelGGL(k_voxel_scatter, g, dim3(256), N * sizeof(float), s, pts, npts, cap, pose,
+                     pose_stride, N, edges, sumI, sumR, cnt);
+}
+
+static void fft_cols(float2* data, int nbatch, int64_t batch_stride, int outer, int L, int inner,
+                     int dir, const FftPlan& plan, const float2* tw, hipStream_t s) {
+  dim3 g((inner + 15) / 16, outer, nbatch);
+  const size_t sh = 2 * 16 * L * sizeof(float2);
+  if (dir < 0)
+    hipLaunchKernelGGL(k_fft_cols<-1>, g, dim3(256), sh, s, data, batch_stride, L, inner, plan, tw);
+  else
+    hipLaunchKernelGGL(k_fft_cols<+1>, g, dim3(256), sh, s, data, batch_stride, L, inner, plan, tw);
+}
+
+void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
+                            const uint32_t* cnt, const FftPlan& plan, const float2* tw,
+                            float2* spec, hipStream_t s) {
+  const int h = N / 2 + 1;
+  const int64_t H = (int64_t)N * N * h, T = (int64_t)N * N * N;
+  dim3 g((N * N + 15) / 16, nclouds);
+  hipLaunchKernelGGL(k_spat_r2c<false>, g, dim3(256), 2 * 16 * N * sizeof(float2), s, N, sumI,
+                     sumR, cnt, T, plan, tw, spec, 2 * H, H);
+  // x then z, over all 2*nclouds spectra
+  fft_cols(spec, 2 * nclouds, H, N, N, h, -1, plan, tw, s);
+  fft_cols(spec, 2 * nclouds, H, 1, N, N * h, -1, plan, tw, s);
+}
+
+void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
+                                  const float2* tw, float2* spec, hipStream_t s) {
+  const int h = N / 2 + 1;
+  const int64_t H = (int64_t)N * N * h, T = (int64_t)N * N * N;
+  for (int c = 0; c < nch; c += 2) {
+    const bool pairc = c + 1 < nch;
+    dim3 g((N * N + 15) / 16, 1);
+    hipLaunchKernelGGL(k_spat_r2c<true>, g, dim3(256), 2 * 16 * N * sizeof(float2), s, N,
+                       chans + c * T, pairc ? chans + (c + 1) * T : (const float*)nullptr,
+                       (const uint32_t*)nullptr, T, plan, tw, spec + c * H, 0,
+                       pairc ? H : (int64_t)-1);
+  }
+  fft_cols(spec, nch, H, N, N, h, -1, plan, tw, s);
+  fft_cols(spec, nch, H, 1, N, N * h, -1, plan, tw, s);
+}
+
+void launch_spatial_inverse(int N, ChannelSet F, ChannelSet G, const FusionRuns& runs, bool fuse,
+                            const FftPlan& plan, const float2* tw, float2* work, float* corr,
+                            unsigned long long* key, hipStream_t s) {
+  const int h = N / 2 + 1;
+  const int inner = N * h;
+  hipLaunchKernelGGL(k_spat_cross_z, dim3((inner + 15) / 16), dim3(256),
+                     2 * 16 * N * sizeof(float2), s, N, F, G, runs, (int)fuse, plan, tw, work);
+  fft_cols(work, 1, 0, N, N, h, +1, plan, tw, s);
+  const size_t sh = (16 * h + 2 * 8 * N) * sizeof(float2);
+  hipLaunchKernelGGL(k_spat_c2r, dim3((N * N + 15) / 16), dim3(256), sh, s, N, work, plan, tw,
+                     corr, key);
+}
+
+void launch_trans_pose(int N, int lower, int upper, int npairs, const unsigned long long* keys,
+                       DevPose* pose, hipStream_t s) {
+  hipLaunchKernelGGL(k_trans_pose, dim3((npairs + 63) / 64), dim3(64), 0, s, N, lower, upper,
+                     npairs, keys, pose);
+}
+
+void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,
+                       hipStream_t s) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_apply_pose, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, pts, P,
+                     pose, out);
+}
+
+}  // namespace phz
