@@ -18,6 +18,7 @@
 #ifndef PHASER_AMD_H_
 #define PHASER_AMD_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -124,6 +125,29 @@ int phz_spatial_correlate(phz_ctx* ctx, int C, const float* f, const float* g, f
  * (any size whose factors are 2, 3 and 5, <= 256). */
 int phz_spatial_correlate_single(phz_ctx* ctx, int n_voxels, const float* f, const float* g,
                                  float* corr, uint32_t* argmax);
+
+/* ---- statistics --------------------------------------------------------- */
+/* Per-stage device time measured with HIP events on the context's stream,
+ * the counterpart of the reference's StatisticsManager timings
+ * (sph-registration.cc:142-146,168-189; statistic-utils.h:10-18).
+ * phz_set_profiling resets the accumulators. */
+typedef struct {
+  char name[32];
+  double total_ms;
+  int64_t calls;
+} phz_stage_time;
+int phz_set_profiling(phz_ctx* ctx, int enable);
+int phz_get_stage_times(phz_ctx* ctx, phz_stage_time* out, int max, int* count);
+
+/* ---- device memory -------------------------------------------------------
+ * No counterpart in the reference (a CPU library).  For callers without a
+ * HIP allocator of their own, so that one HIP runtime serves the process. */
+int phz_device_malloc(phz_ctx* ctx, size_t bytes, void** ptr);
+int phz_device_free(phz_ctx* ctx, void* ptr);
+/* kind: 0 host->host, 1 host->device, 2 device->host, 3 device->device;
+ * waits for the context's stream first. */
+int phz_memcpy(phz_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+int phz_synchronize(phz_ctx* ctx);
 
 #ifdef __cplusplus
 }

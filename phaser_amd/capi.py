@@ -64,7 +64,47 @@ EXPORTS = {
                                         C.POINTER(C.c_uint32)]),
     "phz_spatial_correlate_single": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.POINTER(C.c_uint32)]),
+    "phz_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "phz_get_stage_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "phz_device_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    "phz_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phz_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
+    "phz_synchronize": (C.c_int, [C.c_void_p]),
 }
+
+
+class StageTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_double), ("calls", C.c_int64)]
+
+
+class DeviceArray:
+    """A device buffer owned through the context (one HIP runtime per process)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().phz_device_malloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().phz_memcpy(self.ctx.h, C.c_void_p(self.ptr), C.c_void_p(a.ctypes.data),
+                                a.nbytes, 1))
+        return self
+
+    def download(self, shape, dtype):
+        out = np.zeros(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().phz_memcpy(self.ctx.h, C.c_void_p(out.ctypes.data), C.c_void_p(self.ptr),
+                                out.nbytes, 2))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().phz_device_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
 
 
 def lib():
@@ -176,6 +216,26 @@ class Context:
                                                C.c_void_p(d_trans or None),
                                                C.c_void_p(stream or None)))
         return [r.as_dict() for r in res]
+
+    # ---- device memory and statistics
+    def device_array(self, a):
+        a = np.ascontiguousarray(a)
+        return DeviceArray(self, a.nbytes).upload(a)
+
+    def device_empty(self, nbytes):
+        return DeviceArray(self, nbytes)
+
+    def synchronize(self):
+        _check(lib().phz_synchronize(self.h))
+
+    def set_profiling(self, on=True):
+        _check(lib().phz_set_profiling(self.h, int(on)))
+
+    def stage_times(self):
+        n = C.c_int()
+        buf = (StageTime * 32)()
+        _check(lib().phz_get_stage_times(self.h, buf, 32, C.byref(n)))
+        return {buf[i].name.decode(): (buf[i].total_ms, buf[i].calls) for i in range(n.value)}
 
     # ---- stages
     def sample_s2(self, pts):

@@ -70,6 +70,8 @@ struct phz_ctx {
   float2* d_U = nullptr;
   float* d_corr_rot = nullptr;
   unsigned long long* d_keys = nullptr;
+  unsigned long long* d_partials = nullptr;  // per-workgroup argmax keys
+  int64_t partials_per_pair = 0;
   DevPose* d_pose = nullptr;
   float* d_sumI = nullptr;
   float* d_sumR = nullptr;
@@ -80,7 +82,56 @@ struct phz_ctx {
   float4* d_reg = nullptr;
   int* h_npts = nullptr;  // pinned staging for per-cloud point counts
   std::vector<void*> allocs;
+  // per-stage HIP event timing (phz_set_profiling)
+  struct Timing {
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    double total_ms = 0.0;
+    int64_t calls = 0;
+    bool pending = false;
+  };
+  bool profiling = false;
+  std::vector<Timing> timings;
 };
+
+namespace {
+enum Stage {
+  kS2Cells, kS2NN, kS2Transform, kFuse, kSo3Synth, kSo3Alpha, kSo3Gamma, kRotPose,
+  kVoxelScatter, kSpatialForward, kVoxelClear, kSpatialInverse, kTransPose, kNumStages
+};
+const char* kStageNames[kNumStages] = {
+    "s2_cells", "s2_nn", "s2_transform", "fuse", "so3_synth", "so3_fft_alpha", "so3_fft_gamma",
+    "rot_pose", "voxel_scatter", "spatial_forward", "voxel_clear", "spatial_inverse",
+    "trans_pose"};
+
+struct StageScope {
+  phz_ctx* c;
+  int st;
+  hipStream_t s;
+  StageScope(phz_ctx* c_, int st_, hipStream_t s_) : c(c_), st(st_), s(s_) {
+    if (c->profiling) (void)hipEventRecord(c->timings[st].a, s);
+  }
+  ~StageScope() {
+    if (c->profiling) {
+      (void)hipEventRecord(c->timings[st].b, s);
+      c->timings[st].pending = true;
+    }
+  }
+};
+
+void collect_timings(phz_ctx* c) {
+  if (!c->profiling) return;
+  for (auto& t : c->timings)
+    if (t.pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+        t.total_ms += ms;
+        ++t.calls;
+      }
+      t.pending = false;
+    }
+}
+}  // namespace
 
 namespace {
 
@@ -185,6 +236,8 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_U, (size_t)P * c->n * c->B * c->n));
   TRY(dalloc(c, &c->d_corr_rot, (size_t)nq * c->n));
   TRY(dalloc(c, &c->d_keys, (size_t)2 * P));
+  c->partials_per_pair = std::max(so3_gamma_blocks(c->B), spatial_c2r_blocks(c->N));
+  TRY(dalloc(c, &c->d_partials, (size_t)P * c->partials_per_pair));
   TRY(dalloc(c, &c->d_pose, (size_t)P));
   const int MV = M > 16 ? M : 16;
   TRY(dalloc(c, &c->d_sumI, (size_t)MV * c->T3));
@@ -195,6 +248,12 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3));
   TRY(dalloc(c, &c->d_reg, (size_t)cap));
   PHZ_HIP(hipHostMalloc((void**)&c->h_npts, sizeof(int) * M, hipHostMallocDefault));
+  // voxel accumulators stay zero between calls: every pipeline run clears
+  // exactly the voxels it touched
+  launch_fill_u32((uint32_t*)c->d_sumI, (size_t)MV * c->T3, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->d_sumR, (size_t)M * c->T3, 0u, c->stream);
+  launch_fill_u32(c->d_cnt, (size_t)M * c->T3, 0u, c->stream);
+  PHZ_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -215,40 +274,80 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   const int B = c->B, n = c->n, N = c->N;
   const int64_t nq = (int64_t)n * n, BB = (int64_t)B * B;
   // rotation
-  launch_sample_s2(c->d_pts, c->d_npts, c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq,
-                   c->cells, c->d_grids, s);
-  launch_fst(c->d_grids, 2 * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
-  for (int k = 0; k < ncl; ++k)  // spherical order {range, intensity, refl=0, amb=0}
-    launch_fuse(channels(c->d_coef + (2 * k) * BB, c->d_coef + (2 * k + 1) * BB, 4),
-                (uint32_t)BB, c->s2.runs, c->d_fused + k * BB, s);
-  launch_so3_synth(B, c->d_fused, shared ? 0 : BB, c->d_fused + nt * BB, BB, np, c->d_wig,
-                   c->d_wig_off, c->d_wig_norm, c->d_S, s);
-  launch_fill_u32((uint32_t*)c->d_keys, 4 * (size_t)np, 0u, s);
-  launch_so3_fft_alpha(B, np, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
-  launch_so3_fft_gamma(B, np, c->d_U, c->plan_n, c->d_tw_n, d_rot_corr, nq * n, c->d_keys, s);
-  launch_rot_pose(B, np, c->d_keys, c->d_pose, s);
-  // translation
-  launch_fill_u32((uint32_t*)c->d_sumI, (size_t)ncl * c->T3, 0u, s);
-  launch_fill_u32((uint32_t*)c->d_sumR, (size_t)ncl * c->T3, 0u, s);
-  launch_fill_u32(c->d_cnt, (size_t)ncl * c->T3, 0u, s);
-  launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
+  {
+    StageScope t(c, kS2Cells, s);
+    launch_s2_cells(c->d_pts, c->d_npts, c->cap, ncl, n, c->cells, s);
+  }
+  {
+    StageScope t(c, kS2NN, s);
+    launch_s2_nn(c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq, c->cells, c->d_grids, s);
+  }
+  {
+    StageScope t(c, kS2Transform, s);
+    launch_fst(c->d_grids, 2 * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+  }
+  {  // spherical order {range, intensity, refl=0, amb=0}, one launch for all clouds
+    StageScope t(c, kFuse, s);
+    launch_fuse(channels(c->d_coef, c->d_coef + BB, 4), (uint32_t)BB, c->s2.runs, c->d_fused, s,
+                ncl, 2 * BB);
+  }
+  {
+    StageScope t(c, kSo3Synth, s);
+    launch_so3_synth(B, c->d_fused, shared ? 0 : BB, c->d_fused + nt * BB, BB, np, c->d_wig,
+                     c->d_wig_off, c->d_wig_norm, c->d_S, s);
+  }
+  {
+    StageScope t(c, kSo3Alpha, s);
+    launch_so3_fft_alpha(B, np, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
+  }
+  {
+    StageScope t(c, kSo3Gamma, s);
+    launch_so3_fft_gamma(B, np, c->d_U, c->plan_n, c->d_tw_n, d_rot_corr, nq * n, c->d_partials,
+                         c->d_keys, s);
+  }
+  {
+    StageScope t(c, kRotPose, s);
+    launch_rot_pose(B, np, c->d_keys, c->d_pose, s);
+  }
+  // translation (accumulators are zero on entry and cleared again below)
+  {
+    StageScope t(c, kVoxelScatter, s);
+    launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
+                         c->d_sumR, c->d_cnt, s);
+    launch_voxel_scatter(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
+                         c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
+                         c->d_cnt + nt * c->T3, s);
+  }
+  {
+    StageScope t(c, kSpatialForward, s);
+    launch_spatial_forward(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
+                           c->d_spec, s);
+  }
+  {
+    StageScope t(c, kVoxelClear, s);
+    launch_voxel_clear(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
                        c->d_sumR, c->d_cnt, s);
-  launch_voxel_scatter(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
+    launch_voxel_clear(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
                        c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
                        c->d_cnt + nt * c->T3, s);
-  launch_spatial_forward(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N, c->d_spec,
-                         s);
-  for (int p = 0; p < np; ++p) {  // spatial order {intensity, range, refl=0, amb=0}
-    const int t = shared ? 0 : p;
-    const float2* F = c->d_spec + (int64_t)(2 * t) * c->H;
-    const float2* G = c->d_spec + (int64_t)(2 * (nt + p)) * c->H;
-    launch_spatial_inverse(N, channels(F, F + c->H, 4), channels(G, G + c->H, 4), c->vox.runs,
-                           true, c->plan_N, c->d_tw_N, c->d_work + p * c->H,
-                           d_trans_corr ? d_trans_corr + p * c->T3 : nullptr,
-                           c->d_keys + np + p, s);
   }
-  launch_trans_pose(N, c->cfg.discretize_lower, c->cfg.discretize_upper, np, c->d_keys + np,
-                    c->d_pose, s);
+  {
+    StageScope t(c, kSpatialInverse, s);
+    for (int p = 0; p < np; ++p) {  // spatial order {intensity, range, refl=0, amb=0}
+      const int tg = shared ? 0 : p;
+      const float2* F = c->d_spec + (int64_t)(2 * tg) * c->H;
+      const float2* G = c->d_spec + (int64_t)(2 * (nt + p)) * c->H;
+      launch_spatial_inverse(N, channels(F, F + c->H, 4), channels(G, G + c->H, 4), c->vox.runs,
+                             true, c->plan_N, c->d_tw_N, c->d_work + p * c->H,
+                             d_trans_corr ? d_trans_corr + p * c->T3 : nullptr,
+                             c->d_partials + p * c->partials_per_pair, c->d_keys + np + p, s);
+    }
+  }
+  {
+    StageScope t(c, kTransPose, s);
+    launch_trans_pose(N, c->cfg.discretize_lower, c->cfg.discretize_upper, np, c->d_keys + np,
+                      c->d_pose, s);
+  }
   PHZ_HIP(hipGetLastError());
   return 0;
 }
@@ -271,6 +370,7 @@ int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
   std::vector<DevPose> hp(np);
   PHZ_HIP(hipMemcpyAsync(hp.data(), c->d_pose, sizeof(DevPose) * np, hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipStreamSynchronize(s));
+  collect_timings(c);
   for (int p = 0; p < np; ++p) to_result(hp[p], &out[p]);
   return 0;
 }
@@ -332,6 +432,10 @@ int phz_create(const phz_config* cfg, phz_ctx** out) {
 void phz_destroy(phz_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& t : c->timings) {
+    if (t.a) hipEventDestroy(t.a);
+    if (t.b) hipEventDestroy(t.b);
+  }
   for (void* p : c->allocs) hipFree(p);
   if (c->h_npts) hipHostFree(c->h_npts);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -427,10 +531,9 @@ static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, floa
                           float* synthesis, uint32_t* argmax, hipStream_t s) {
   const int B = c->B, n = c->n;
   launch_so3_synth(B, d_f, 0, d_h, 0, 1, c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S, s);
-  launch_fill_u32((uint32_t*)c->d_keys, 2, 0u, s);
   launch_so3_fft_alpha(B, 1, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
   launch_so3_fft_gamma(B, 1, c->d_U, c->plan_n, c->d_tw_n, corr ? c->d_corr_rot : nullptr,
-                       (int64_t)n * n * n, c->d_keys, s);
+                       (int64_t)n * n * n, c->d_partials, c->d_keys, s);
   PHZ_HIP(hipGetLastError());
   unsigned long long key = 0;
   PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
@@ -478,9 +581,6 @@ int phz_voxelize(phz_ctx* c, const float* pts, int64_t P, float* out, float* cou
   hipStream_t s = c->stream;
   TRY(load_cloud(c, 0, pts, P, hipMemcpyHostToDevice, s));
   TRY(commit_counts(c, 1, s));
-  launch_fill_u32((uint32_t*)c->d_sumI, c->T3, 0u, s);
-  launch_fill_u32((uint32_t*)c->d_sumR, c->T3, 0u, s);
-  launch_fill_u32(c->d_cnt, c->T3, 0u, s);
   launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, 1, nullptr, 0, c->N, c->d_edges, c->d_sumI,
                        c->d_sumR, c->d_cnt, s);
   PHZ_HIP(hipGetLastError());
@@ -489,6 +589,8 @@ int phz_voxelize(phz_ctx* c, const float* pts, int64_t P, float* out, float* cou
   PHZ_HIP(hipMemcpyAsync(sI.data(), c->d_sumI, sizeof(float) * c->T3, hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipMemcpyAsync(sR.data(), c->d_sumR, sizeof(float) * c->T3, hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipMemcpyAsync(k.data(), c->d_cnt, sizeof(uint32_t) * c->T3, hipMemcpyDeviceToHost, s));
+  launch_voxel_clear(c->d_pts, c->d_npts, c->cap, 1, nullptr, 0, c->N, c->d_edges, c->d_sumI,
+                     c->d_sumR, c->d_cnt, s);
   PHZ_HIP(hipStreamSynchronize(s));
   for (int64_t v = 0; v < c->T3; ++v) {
     out[v] = k[v] ? sI[v] / (float)k[v] : 0.f;
@@ -513,9 +615,10 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
     F.ch[k] = c->d_spec + k * H;
     G.ch[k] = c->d_spec + (C + k) * H;
   }
-  launch_fill_u32((uint32_t*)c->d_keys, 2, 0u, s);
   launch_spatial_inverse(c->N, F, G, c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work,
-                         corr ? c->d_corr_trans : nullptr, c->d_keys, s);
+                         corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys, s);
+  // the dense channels were staged in the voxel accumulators: zero them again
+  launch_fill_u32((uint32_t*)c->d_sumI, (size_t)2 * C * T, 0u, s);
   PHZ_HIP(hipGetLastError());
   unsigned long long key = 0;
   PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
@@ -548,12 +651,13 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
   if (!rc && (e = hipMalloc(&d_work, sizeof(float2) * H)) != hipSuccess) fail("hipMalloc", e);
   if (!rc && (e = hipMalloc(&d_tw, sizeof(float2) * N)) != hipSuccess) fail("hipMalloc", e);
   if (!rc && (e = hipMalloc(&d_corr, sizeof(float) * T)) != hipSuccess) fail("hipMalloc", e);
-  if (!rc && (e = hipMalloc(&d_key, sizeof(unsigned long long))) != hipSuccess) fail("hipMalloc", e);
+  const int nblk = spatial_c2r_blocks(N);
+  if (!rc && (e = hipMalloc(&d_key, sizeof(unsigned long long) * (1 + nblk))) != hipSuccess)
+    fail("hipMalloc", e);
   if (!rc) {
     hipMemcpyAsync(d_in, f, sizeof(float) * T, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_in + T, g, sizeof(float) * T, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s);
-    launch_fill_u32((uint32_t*)d_key, 2, 0u, s);
     FftPlan p = plan;
     if (N == 1) {
       p.L = 1;
@@ -565,7 +669,7 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
     F.ch[0] = d_spec;
     G.ch[0] = d_spec + H;
     FusionRuns none{};
-    launch_spatial_inverse(N, F, G, none, false, p, d_tw, d_work, d_corr, d_key, s);
+    launch_spatial_inverse(N, F, G, none, false, p, d_tw, d_work, d_corr, d_key + 1, d_key, s);
     if ((e = hipGetLastError()) != hipSuccess) fail("launch", e);
   }
   unsigned long long key = 0;
@@ -582,6 +686,73 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
   hipFree(d_corr);
   hipFree(d_key);
   return rc;
+}
+
+// ---------------------------------------------------------------- statistics
+int phz_set_profiling(phz_ctx* c, int enable) {
+  PHZ_CHECK(c, "null argument");
+  PHZ_HIP(hipStreamSynchronize(c->stream));
+  if (c->timings.empty()) {
+    c->timings.resize(kNumStages);
+    for (int i = 0; i < kNumStages; ++i) {
+      c->timings[i].name = kStageNames[i];
+      PHZ_HIP(hipEventCreate(&c->timings[i].a));
+      PHZ_HIP(hipEventCreate(&c->timings[i].b));
+    }
+  }
+  for (auto& t : c->timings) {
+    t.total_ms = 0.0;
+    t.calls = 0;
+    t.pending = false;
+  }
+  c->profiling = enable != 0;
+  return 0;
+}
+
+int phz_get_stage_times(phz_ctx* c, phz_stage_time* out, int max, int* count) {
+  PHZ_CHECK(c && count, "null argument");
+  int k = 0;
+  for (const auto& t : c->timings) {
+    if (out && k < max) {
+      std::memset(&out[k], 0, sizeof(phz_stage_time));
+      std::strncpy(out[k].name, t.name, sizeof(out[k].name) - 1);
+      out[k].total_ms = t.total_ms;
+      out[k].calls = t.calls;
+    }
+    ++k;
+  }
+  *count = k;
+  return 0;
+}
+
+// ---------------------------------------------------------------- device memory
+int phz_device_malloc(phz_ctx* c, size_t bytes, void** ptr) {
+  PHZ_CHECK(c && ptr, "null argument");
+  PHZ_HIP(hipSetDevice(c->cfg.device));
+  PHZ_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+  return 0;
+}
+
+int phz_device_free(phz_ctx* c, void* ptr) {
+  PHZ_CHECK(c, "null argument");
+  if (ptr) PHZ_HIP(hipFree(ptr));
+  return 0;
+}
+
+int phz_memcpy(phz_ctx* c, void* dst, const void* src, size_t bytes, int kind) {
+  PHZ_CHECK(c && dst && src, "null argument");
+  PHZ_CHECK(kind >= 0 && kind <= 3, "kind must be 0 h2h, 1 h2d, 2 d2h or 3 d2d");
+  const hipMemcpyKind k[4] = {hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                              hipMemcpyDeviceToDevice};
+  PHZ_HIP(hipStreamSynchronize(c->stream));
+  PHZ_HIP(hipMemcpy(dst, src, bytes, k[kind]));
+  return 0;
+}
+
+int phz_synchronize(phz_ctx* c) {
+  PHZ_CHECK(c, "null argument");
+  PHZ_HIP(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 }  // extern "C"

@@ -78,6 +78,22 @@ __device__ __forceinline__ unsigned long long wave_max_key(unsigned long long k)
   return k;
 }
 
+// workgroup-level max of keys (blockDim a multiple of 64, <= 1024); the
+// result is valid in thread 0.  Contains barriers: call from every thread.
+__device__ __forceinline__ unsigned long long block_max_key(unsigned long long k) {
+  __shared__ unsigned long long wk[16];
+  k = wave_max_key(k);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wk[w] = k;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    for (int i = 1; i < nw; ++i) k = wk[i] > k ? wk[i] : k;
+  }
+  __syncthreads();
+  return k;
+}
+
 // ---------------------------------------------------------------- FFT plan
 struct FftPlan {
   int L;          // transform length (<= 256)

@@ -30,6 +30,11 @@ struct CellBuffers {
 void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
                       const float4* grid, const float* theta, float max_sq, CellBuffers cb,
                       float* grids /* nclouds x 2 x n*n : range, intensity */, hipStream_t s);
+// the two halves of launch_sample_s2: projection + cell sort, then the NN search
+void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
+                     CellBuffers cb, hipStream_t s);
+void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
+                  float max_sq, CellBuffers cb, float* grids, hipStream_t s);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
@@ -43,8 +48,13 @@ struct ChannelSet {
   const float2* ch[8];
   int C;
 };
+// nsets channel sets (each channel pointer advanced by set_stride per set)
+// are fused in one launch into out[set][count].
 void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
-                 hipStream_t s);
+                 hipStream_t s, int nsets = 1, int64_t set_stride = 0);
+// keys[pair] = max over partials[pair*count .. +count)
+void launch_reduce_keys(const unsigned long long* partials, int count, int npairs,
+                        unsigned long long* keys, hipStream_t s);
 
 // ------------------------------------------------------------- SO(3)
 // S buffer: [m2 in 0..B-1][m1+B-1 in 0..2B-2][k in 0..n-1] complex per pair.
@@ -56,9 +66,12 @@ void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const floa
 void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
                           const float2* tw, float2* U, hipStream_t s);
 // FFT over m2 -> gamma (Hermitian), |C|^2, optional grid write, argmax key
+// via per-workgroup partials (so3_gamma_blocks(bw) per pair)
+int so3_gamma_blocks(int bw);
 void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& plan,
                           const float2* tw, float* corr /*nullable*/, int64_t corr_stride,
-                          unsigned long long* keys, hipStream_t s);
+                          unsigned long long* partials, unsigned long long* keys,
+                          hipStream_t s);
 
 // Device-side pose from the rotation argmax: fills pose[pair] and the float
 // transform T[pair][16] (rotation-utils.cc:17-23,86-143,190-207).
@@ -92,9 +105,17 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
                                   const float2* tw, float2* spec, hipStream_t s);
 // fused cross spectrum + inverse 3-D c2r + argmax.  F channels and G
 // channels (null = zero channel) in the reference's spatial order.
+// partials: spatial_c2r_blocks(N) entries of scratch
+int spatial_c2r_blocks(int N);
 void launch_spatial_inverse(int N, ChannelSet F, ChannelSet G, const FusionRuns& runs,
                             bool fuse, const FftPlan& plan, const float2* tw, float2* work,
-                            float* corr /*nullable*/, unsigned long long* key, hipStream_t s);
+                            float* corr /*nullable*/, unsigned long long* partials,
+                            unsigned long long* key, hipStream_t s);
+// zero the accumulators of the voxels the points touched (same transform and
+// binning as launch_voxel_scatter), so they need no full clear before reuse
+void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
+                        const DevPose* pose, int pose_stride, int N, const float* edges,
+                        float* sumI, float* sumR, uint32_t* cnt, hipStream_t s);
 void launch_trans_pose(int N, int lower, int upper, int npairs,
                        const unsigned long long* keys, DevPose* pose, hipStream_t s);
 // out = T_rot * p + translation (PCL scalar order), intensity kept

@@ -7,7 +7,7 @@
 // point of each Driscoll-Healy query direction (point-cloud.cc:93-157, FLANN
 // L2_Simple float distance).  Instead of a kd-tree the points are counting-
 // sorted into a (theta band x phi sector) grid aligned with the queries, and
-// each query searches bands outward from its own with exact lower bounds on
+// each query searches cells outward from its own with exact lower bounds on
 // the chord distance, so the answer equals a brute-force scan (ties go to the
 // lowest point index).
 #include "kernels.hpp"
@@ -62,7 +62,8 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
   atomicAdd(&cell_count[(int64_t)c * n * n + cid], 1);
 }
 
-// exclusive scan of n*n cell counts per cloud (one workgroup per cloud)
+// exclusive scan of the n*n cell counts of a cloud (one 1024-thread
+// workgroup per cloud): per-thread serial sums, wave scans, wave totals.
 __global__ void k_cell_scan(int n, int* __restrict__ cell_count, int* __restrict__ cell_start,
                             int* __restrict__ band_start) {
   const int c = blockIdx.x;
@@ -70,25 +71,33 @@ __global__ void k_cell_scan(int n, int* __restrict__ cell_count, int* __restrict
   int* cnt = cell_count + (int64_t)c * ncell;
   int* st = cell_start + (int64_t)c * (ncell + 1);
   int* bs = band_start + (int64_t)c * (n + 1);
-  __shared__ int part[1024];
-  const int T = blockDim.x;
+  __shared__ int wsum[16];
+  const int T = blockDim.x;  // 1024
   const int per = (ncell + T - 1) / T;
   const int a = threadIdx.x * per, b = min(ncell, a + per);
   int s = 0;
   for (int k = a; k < b; ++k) s += cnt[k];
-  part[threadIdx.x] = s;
+  // inclusive scan within the wave
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
     int acc = 0;
-    for (int t = 0; t < T; ++t) {
-      const int v = part[t];
-      part[t] = acc;
+    for (int w = 0; w < T / 64; ++w) {
+      const int v = wsum[w];
+      wsum[w] = acc;
       acc += v;
     }
     st[ncell] = acc;
   }
   __syncthreads();
-  int acc = part[threadIdx.x];
+  int acc = wsum[wv] + x - s;
   for (int k = a; k < b; ++k) {
     st[k] = acc;
     acc += cnt[k];
@@ -114,27 +123,45 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int n,
 }
 
 // per cell: nearest non-empty sector of the same band to the right (>= s)
-// and to the left (<= s), cyclic; -1 in an empty band.  One thread per band.
+// and to the left (<= s), cyclic; -1 in an empty band.  One wave per band:
+// the occupancy of the band's n <= 256 sectors is a 4 x 64-bit ballot mask.
 __global__ void k_jump_tables(int n, const int* __restrict__ cell_start, int* __restrict__ nr,
                               int* __restrict__ nl) {
-  const int c = blockIdx.y;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
+  const int b = blockIdx.x, c = blockIdx.y;
+  const int lane = threadIdx.x;  // 64 lanes
   const int* st = cell_start + (int64_t)c * (n * n + 1) + (int64_t)b * n;
   int* r = nr + (int64_t)c * n * n + (int64_t)b * n;
   int* l = nl + (int64_t)c * n * n + (int64_t)b * n;
-  const bool any = st[n] > st[0];
-  int last = -1;
-  for (int k = 2 * n - 1; k >= 0; --k) {
-    const int s = k % n;
-    if (st[s + 1] > st[s]) last = s;
-    if (k < n) r[s] = any ? last : -1;
+  unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
+  const int nw = (n + 63) / 64;
+  for (int w = 0; w < nw; ++w) {
+    const int s = w * 64 + lane;
+    const bool ne = s < n && st[s + 1] > st[s];
+    m[w] = __ballot(ne);
   }
-  last = -1;
-  for (int k = 0; k < 2 * n; ++k) {
-    const int s = k % n;
-    if (st[s + 1] > st[s]) last = s;
-    if (k >= n) l[s] = any ? last : -1;
+  const bool any = (m[0] | m[1] | m[2] | m[3]) != 0ull;
+  for (int s = lane; s < n; s += 64) {
+    int right = -1, left = -1;
+    if (any) {
+      // first set bit at or after s, cyclically
+      for (int k = 0; k <= nw && right < 0; ++k) {
+        const int w = (s / 64 + k) % nw;
+        unsigned long long mm = m[w];
+        if (k == 0) mm &= (~0ull) << (s % 64);
+        if (k == nw) mm = m[w] & ((s % 64) ? ((1ull << (s % 64)) - 1ull) : 0ull);
+        if (mm) right = w * 64 + __builtin_ctzll(mm);
+      }
+      // last set bit at or before s, cyclically
+      for (int k = 0; k <= nw && left < 0; ++k) {
+        const int w = ((s / 64 - k) % nw + nw) % nw;
+        unsigned long long mm = m[w];
+        if (k == 0) mm &= (s % 64 == 63) ? ~0ull : ((1ull << (s % 64 + 1)) - 1ull);
+        if (k == nw) mm = m[w] & ((s % 64 == 63) ? 0ull : ((~0ull) << (s % 64 + 1)));
+        if (mm) left = w * 64 + 63 - __builtin_clzll(mm);
+      }
+    }
+    r[s] = right;
+    l[s] = left;
   }
 }
 
@@ -148,13 +175,15 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
   return r;
 }
 
-// Exact nearest neighbour of every query.  Bands are visited in order of
-// colatitude distance (own, +1, -1, +2, ...), sectors of a band outward from
-// the query's own, and a cell is skipped once its lower bound on the squared
-// chord distance,
+// Exact nearest neighbour of every query, eight lanes per query.  Bands are
+// visited in order of colatitude distance (own, +1, -1, +2, ...), sectors of
+// a band outward from the query's own, and a cell is skipped once its lower
+// bound on the squared chord distance,
 //   |q - p|^2 >= 2(1 - cos dth) + 2 sin(th_q) min sin(th_p) (1 - cos dphi),
 // exceeds the best distance found (with a float slack): points outside it are
 // provably farther.  Empty sectors are jumped over with the per-cell tables.
+// The eight lanes split each cell's points and agree on the best after it.
+constexpr int kNNLanes = 8;
 __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
                      float max_sq, const int* __restrict__ cell_start,
                      const int* __restrict__ band_start, const int* __restrict__ next_right,
@@ -162,9 +191,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids) {
   const int c = blockIdx.y;
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = threadIdx.x & (kNNLanes - 1);
+  const int q = blockIdx.x * (blockDim.x / kNNLanes) + threadIdx.x / kNNLanes;
   const int nq = n * n;
-  if (q >= nq) return;
+  if (q >= nq) return;  // whole groups leave together
   const int qi = q / n, qj = q - qi * n;
   const float4 qv = grid[q];
   const float thq = theta[qi];
@@ -181,7 +211,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   bool up = true, down = true;
   auto scan_cell = [&](int cid) {
     const int e = st[cid + 1];
-    for (int p = st[cid]; p < e; ++p) {
+    for (int p = st[cid] + sub; p < e; p += kNNLanes) {
       const float4 pv = sp[p];
       const float d = l2_simple(qv, pv);
       const int oi = __float_as_int(pv.w);
@@ -189,6 +219,17 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
         best = d;
         bidx = oi;
         bpos = p;
+      }
+    }
+#pragma unroll
+    for (int off = kNNLanes / 2; off > 0; off >>= 1) {
+      const float ob = __shfl_xor(best, off, kNNLanes);
+      const int oi = __shfl_xor(bidx, off, kNNLanes);
+      const int op = __shfl_xor(bpos, off, kNNLanes);
+      if (ob < best || (ob == best && oi < bidx)) {
+        best = ob;
+        bidx = oi;
+        bpos = op;
       }
     }
   };
@@ -257,14 +298,16 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       }
     }
   }
-  float r = 0.f, it = 0.f;
-  if (bpos >= 0 && best <= max_sq) {
-    const float2 v = sorted_val[c * cap + bpos];
-    r = v.x;
-    it = v.y;
+  if (sub == 0) {
+    float r = 0.f, it = 0.f;
+    if (bpos >= 0 && best <= max_sq) {
+      const float2 v = sorted_val[c * cap + bpos];
+      r = v.x;
+      it = v.y;
+    }
+    grids[(int64_t)c * 2 * nq + q] = r;
+    grids[(int64_t)c * 2 * nq + nq + q] = it;
   }
-  grids[(int64_t)c * 2 * nq + q] = r;
-  grids[(int64_t)c * 2 * nq + nq + q] = it;
 }
 
 // ---------------------------------------------------------------- FST
@@ -297,43 +340,32 @@ __global__ void k_phi_dft(const float* __restrict__ sig, int n, int bw,
 }
 
 // coef[s][sean(m,l)] = sum_i (w_i P~_l^m(cos th_i)) F[s][m][i]; negative
-// orders by real-data symmetry (s2_semi_memo.c:349-373)
-__global__ void k_legendre(const float2* __restrict__ F, int S, int n, int bw,
+// orders by real-data symmetry (s2_semi_memo.c:349-373).  One workgroup per
+// (m, signal), one thread per degree l.
+__global__ void k_legendre(const float2* __restrict__ F, int n, int bw,
                            const float* __restrict__ leg, const int* __restrict__ leg_off,
                            float2* __restrict__ coef) {
-  const int m = blockIdx.x;
-  const int s0 = blockIdx.y * 4;
-  const int ns = min(4, S - s0);
+  const int m = blockIdx.x, s = blockIdx.y;
   extern __shared__ float2 fs[];
-  for (int idx = threadIdx.x; idx < ns * n; idx += blockDim.x) {
-    const int ss = idx / n, i = idx - ss * n;
-    fs[idx] = F[((int64_t)(s0 + ss) * bw + m) * n + i];
-  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) fs[i] = F[((int64_t)s * bw + m) * n + i];
   __syncthreads();
   const int L = bw - m;
   const float* T = leg + leg_off[m];
   for (int t = threadIdx.x; t < L; t += blockDim.x) {
-    float2 acc[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f),
-                     make_float2(0.f, 0.f)};
+    float ar = 0.f, ai = 0.f;
     for (int i = 0; i < n; ++i) {
       const float w = T[(int64_t)i * L + t];
-#pragma unroll
-      for (int ss = 0; ss < 4; ++ss)
-        if (ss < ns) {
-          acc[ss].x += w * fs[ss * n + i].x;
-          acc[ss].y += w * fs[ss * n + i].y;
-        }
+      ar += w * fs[i].x;
+      ai += w * fs[i].y;
     }
     const int l = m + t;
     const int L1 = bw - 1;
     const int ip = m * (L1 + 1) - ((m * (m - 1)) / 2) + (l - m);
     const int im = ((L1 * (L1 + 3)) / 2) + 1 + ((L1 - m) * (L1 - m + 1) / 2) + (l - m);
     const float sg = (m & 1) ? -1.f : 1.f;
-    for (int ss = 0; ss < ns; ++ss) {
-      float2* out = coef + (int64_t)(s0 + ss) * bw * bw;
-      out[ip] = acc[ss];
-      if (m > 0) out[im] = make_float2(sg * acc[ss].x, -sg * acc[ss].y);
-    }
+    float2* out = coef + (int64_t)s * bw * bw;
+    out[ip] = make_float2(ar, ai);
+    if (m > 0) out[im] = make_float2(sg * ar, -sg * ai);
   }
 }
 
@@ -341,9 +373,12 @@ __global__ void k_legendre(const float2* __restrict__ F, int S, int n, int bw,
 // LaplacePyramid::fuseChannels (laplace-pyramid.cc:60-151): per index, the
 // channel of largest |.|^2 (first on ties), the mean over all channels in the
 // innermost low-pass band, or zero where the reference's expand leaves one.
-__global__ void k_fuse(ChannelSet cs, uint32_t count, FusionRuns runs, float2* __restrict__ out) {
+// blockIdx.y selects one of `nsets` channel sets spaced by set_stride.
+__global__ void k_fuse(ChannelSet cs, int64_t set_stride, uint32_t count, FusionRuns runs,
+                       float2* __restrict__ out) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= count) return;
+  const int64_t so = (int64_t)blockIdx.y * set_stride;
   int mode = 0;
   for (int r = 0; r < runs.count; ++r)
     if (q >= runs.start[r]) mode = (int)runs.mode[r];
@@ -351,7 +386,7 @@ __global__ void k_fuse(ChannelSet cs, uint32_t count, FusionRuns runs, float2* _
   if (mode == 0) {
     float be = -1.f;
     for (int c = 0; c < cs.C; ++c) {
-      const float2 x = cs.ch[c] ? cs.ch[c][q] : make_float2(0.f, 0.f);
+      const float2 x = cs.ch[c] ? cs.ch[c][so + q] : make_float2(0.f, 0.f);
       const float e = x.x * x.x + x.y * x.y;
       if (e > be) {
         be = e;
@@ -362,12 +397,12 @@ __global__ void k_fuse(ChannelSet cs, uint32_t count, FusionRuns runs, float2* _
     float ar = 0.f, ai = 0.f;
     for (int c = 0; c < cs.C; ++c)
       if (cs.ch[c]) {
-        ar += cs.ch[c][q].x;
-        ai += cs.ch[c][q].y;
+        ar += cs.ch[c][so + q].x;
+        ai += cs.ch[c][so + q].y;
       }
     v = make_float2(ar / (float)cs.C, ai / (float)cs.C);
   }
-  out[q] = v;
+  out[(int64_t)blockIdx.y * count + q] = v;
 }
 
 }  // namespace
@@ -375,6 +410,12 @@ __global__ void k_fuse(ChannelSet cs, uint32_t count, FusionRuns runs, float2* _
 void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
                       const float4* grid, const float* theta, float max_sq, CellBuffers cb,
                       float* grids, hipStream_t s) {
+  launch_s2_cells(pts, npts, cap, nclouds, n, cb, s);
+  launch_s2_nn(cap, nclouds, n, grid, theta, max_sq, cb, grids, s);
+}
+
+void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
+                     CellBuffers cb, hipStream_t s) {
   const int ncell = n * n;
   launch_fill_u32((uint32_t*)cb.cell_count, (size_t)nclouds * ncell, 0u, s);
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
@@ -384,10 +425,16 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
                      cb.cell_start, cb.band_start);
   hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, n, cb.proj, cb.val, cb.cell,
                      cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
-  hipLaunchKernelGGL(k_jump_tables, dim3((n + 63) / 64, nclouds), dim3(64), 0, s, n,
-                     cb.cell_start, cb.next_right, cb.next_left);
-  dim3 gq((unsigned)((ncell + 63) / 64), nclouds);
-  hipLaunchKernelGGL(k_nn, gq, dim3(64), 0, s, grid, theta, n, max_sq, cb.cell_start,
+  hipLaunchKernelGGL(k_jump_tables, dim3(n, nclouds), dim3(64), 0, s, n, cb.cell_start,
+                     cb.next_right, cb.next_left);
+}
+
+void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
+                  float max_sq, CellBuffers cb, float* grids, hipStream_t s) {
+  const int ncell = n * n;
+  const int qpb = 256 / kNNLanes;
+  dim3 gq((unsigned)((ncell + qpb - 1) / qpb), nclouds);
+  hipLaunchKernelGGL(k_nn, gq, dim3(256), 0, s, grid, theta, n, max_sq, cb.cell_start,
                      cb.band_start, cb.next_right, cb.next_left, cb.sorted, cb.sorted_val, cap,
                      grids);
 }
@@ -398,13 +445,14 @@ void launch_fst(const float* sig, int S, int bw, const float* legendre, const in
   const int th = ((bw + 63) / 64) * 64;
   hipLaunchKernelGGL(k_phi_dft, dim3(n, S), dim3(th), n * sizeof(float) + n * sizeof(float2), s,
                      sig, n, bw, tw_n, F);
-  hipLaunchKernelGGL(k_legendre, dim3(bw, (S + 3) / 4), dim3(th), 4 * n * sizeof(float2), s, F, S,
-                     n, bw, legendre, leg_off_dev, coef);
+  hipLaunchKernelGGL(k_legendre, dim3(bw, S), dim3(th), n * sizeof(float2), s, F, n, bw, legendre,
+                     leg_off_dev, coef);
 }
 
 void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
-                 hipStream_t s) {
-  hipLaunchKernelGGL(k_fuse, dim3((count + 255) / 256), dim3(256), 0, s, cs, count, runs, out);
+                 hipStream_t s, int nsets, int64_t set_stride) {
+  hipLaunchKernelGGL(k_fuse, dim3((count + 255) / 256, nsets), dim3(256), 0, s, cs, set_stride,
+                     count, runs, out);
 }
 
 }  // namespace phz

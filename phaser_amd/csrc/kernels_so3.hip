@@ -160,7 +160,7 @@ __global__ void k_so3_fft_alpha(int bw, const float2* __restrict__ S, FftPlan pl
 // line), scale (1/n)(bw/pi) (soft_fftw.c:1905-1911), |C|^2, argmax.
 __global__ void k_so3_fft_gamma(int bw, const float2* __restrict__ U, FftPlan plan,
                                 const float2* __restrict__ tw, float* __restrict__ corr,
-                                int64_t corr_stride, unsigned long long* __restrict__ keys) {
+                                int64_t corr_stride, unsigned long long* __restrict__ partials) {
   constexpr int AT = 16, NL = AT / 2;
   const int n = 2 * bw;
   const int a0 = blockIdx.x * AT, k = blockIdx.y, pair = blockIdx.z;
@@ -208,8 +208,9 @@ __global__ void k_so3_fft_gamma(int bw, const float2* __restrict__ U, FftPlan pl
     const unsigned long long key = make_key(v, gi);
     best = key > best ? key : best;
   }
-  best = wave_max_key(best);
-  if ((threadIdx.x & 63) == 0) atomicMax(&keys[pair], best);
+  best = block_max_key(best);
+  if (threadIdx.x == 0) partials[(int64_t)pair * gridDim.x * gridDim.y + blockIdx.y * gridDim.x +
+                                 blockIdx.x] = best;
 }
 
 __device__ void mat4_mul_nc(const float* A, const float* B, float* R) {
@@ -304,14 +305,21 @@ void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& pl
                      tw, U);
 }
 
+int so3_gamma_blocks(int bw) {
+  const int n = 2 * bw;
+  return ((n + 15) / 16) * n;
+}
+
 void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& plan,
                           const float2* tw, float* corr, int64_t corr_stride,
-                          unsigned long long* keys, hipStream_t s) {
+                          unsigned long long* partials, unsigned long long* keys,
+                          hipStream_t s) {
   const int n = 2 * bw;
   dim3 g((n + 15) / 16, n, npairs);
   const size_t sh = (16 * bw + 2 * 8 * n) * sizeof(float2);
   hipLaunchKernelGGL(k_so3_fft_gamma, g, dim3(256), sh, s, bw, U, plan, tw, corr, corr_stride,
-                     keys);
+                     partials);
+  launch_reduce_keys(partials, so3_gamma_blocks(bw), npairs, keys, s);
 }
 
 void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose* pose,

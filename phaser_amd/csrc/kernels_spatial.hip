@@ -34,6 +34,9 @@ __device__ __forceinline__ int histc_bin(float x, const float* E, int N) {
   return (x == E[h]) ? h : l;
 }
 
+// kClear = false: accumulate (phase-aligner.cc:98-120); true: zero the same
+// voxels again after use.
+template <bool kClear>
 __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __restrict__ npts,
                                 int64_t cap, const DevPose* __restrict__ pose, int pose_stride,
                                 int N, const float* __restrict__ edges, float* __restrict__ sumI,
@@ -62,11 +65,29 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
   const uint32_t lin = (bx * NN + by) + (NN * NN * bz);  // Sub2Ind in uint32
   const uint32_t nf = NN * NN * NN;
   if (lin >= nf) return;  // the reference skips lin > nf and overruns at nf
-  const float range = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
   const size_t off = (size_t)c * nf + lin;
+  if (kClear) {
+    sumI[off] = 0.f;
+    sumR[off] = 0.f;
+    cnt[off] = 0u;
+    return;
+  }
+  const float range = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
   atomicAdd(&sumI[off], p.w);
   atomicAdd(&sumR[off], range);
   atomicAdd(&cnt[off], 1u);
+}
+
+__global__ void k_reduce_keys(const unsigned long long* __restrict__ partials, int count,
+                              unsigned long long* __restrict__ keys) {
+  const int p = blockIdx.x;
+  unsigned long long best = 0ull;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) {
+    const unsigned long long k = partials[(int64_t)p * count + i];
+    best = k > best ? k : best;
+  }
+  best = block_max_key(best);
+  if (threadIdx.x == 0) keys[p] = best;
 }
 
 // pass 1: r2c along y of (mean I + i mean R) rows, split into the two
@@ -205,7 +226,7 @@ __global__ void k_spat_cross_z(int N, ChannelSet F, ChannelSet G, FusionRuns run
 // per complex line, signed argmax over x*N + y + N^2*z.
 __global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
                            const float2* __restrict__ tw, float* __restrict__ corr,
-                           unsigned long long* __restrict__ key) {
+                           unsigned long long* __restrict__ partials) {
   constexpr int RT = 16, NL = RT / 2;
   const int h = N / 2 + 1, rows = N * N;
   const int row0 = blockIdx.x * RT;
@@ -248,8 +269,8 @@ __global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
     const unsigned long long kk = make_key(v, lin);
     best = kk > best ? kk : best;
   }
-  best = wave_max_key(best);
-  if ((threadIdx.x & 63) == 0) atomicMax(key, best);
+  best = block_max_key(best);
+  if (threadIdx.x == 0) partials[blockIdx.x] = best;
 }
 
 // translation-utils.cc:40-52 + signal-utils.cc:11-19
@@ -310,9 +331,24 @@ void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int n
                           const DevPose* pose, int pose_stride, int N, const float* edges,
                           float* sumI, float* sumR, uint32_t* cnt, hipStream_t s) {
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
-  hipLaunchKernelGGL(k_voxel_scatter, g, dim3(256), N * sizeof(float), s, pts, npts, cap, pose,
-                     pose_stride, N, edges, sumI, sumR, cnt);
+  hipLaunchKernelGGL(k_voxel_scatter<false>, g, dim3(256), N * sizeof(float), s, pts, npts, cap,
+                     pose, pose_stride, N, edges, sumI, sumR, cnt);
 }
+
+void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
+                        const DevPose* pose, int pose_stride, int N, const float* edges,
+                        float* sumI, float* sumR, uint32_t* cnt, hipStream_t s) {
+  dim3 g((unsigned)((cap + 255) / 256), nclouds);
+  hipLaunchKernelGGL(k_voxel_scatter<true>, g, dim3(256), N * sizeof(float), s, pts, npts, cap,
+                     pose, pose_stride, N, edges, sumI, sumR, cnt);
+}
+
+void launch_reduce_keys(const unsigned long long* partials, int count, int npairs,
+                        unsigned long long* keys, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_keys, dim3(npairs), dim3(256), 0, s, partials, count, keys);
+}
+
+int spatial_c2r_blocks(int N) { return (N * N + 15) / 16; }
 
 static void fft_cols(float2* data, int nbatch, int64_t batch_stride, int outer, int L, int inner,
                      int dir, const FftPlan& plan, const float2* tw, hipStream_t s) {
@@ -355,15 +391,17 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
 
 void launch_spatial_inverse(int N, ChannelSet F, ChannelSet G, const FusionRuns& runs, bool fuse,
                             const FftPlan& plan, const float2* tw, float2* work, float* corr,
-                            unsigned long long* key, hipStream_t s) {
+                            unsigned long long* partials, unsigned long long* key,
+                            hipStream_t s) {
   const int h = N / 2 + 1;
   const int inner = N * h;
   hipLaunchKernelGGL(k_spat_cross_z, dim3((inner + 15) / 16), dim3(256),
                      2 * 16 * N * sizeof(float2), s, N, F, G, runs, (int)fuse, plan, tw, work);
   fft_cols(work, 1, 0, N, N, h, +1, plan, tw, s);
   const size_t sh = (16 * h + 2 * 8 * N) * sizeof(float2);
-  hipLaunchKernelGGL(k_spat_c2r, dim3((N * N + 15) / 16), dim3(256), sh, s, N, work, plan, tw,
-                     corr, key);
+  hipLaunchKernelGGL(k_spat_c2r, dim3(spatial_c2r_blocks(N)), dim3(256), sh, s, N, work, plan, tw,
+                     corr, partials);
+  launch_reduce_keys(partials, spatial_c2r_blocks(N), 1, key, s);
 }
 
 void launch_trans_pose(int N, int lower, int upper, int npairs, const unsigned long long* keys,

@@ -102,8 +102,9 @@ def test_voxelize_matches_oracle(ctx32, po):
     v, cnt = ctx32.voxelize(src)
     ch, hist = po.voxelize(src, 64, hist=True)
     np.testing.assert_array_equal(cnt, hist.astype(np.float32))
-    np.testing.assert_allclose(v[0], ch[0], rtol=2e-6, atol=1e-7)
-    np.testing.assert_allclose(v[1], ch[1], rtol=2e-6, atol=1e-6)
+    # FP32 atomic sums in arbitrary order vs ordered FP64 sums
+    np.testing.assert_allclose(v[0], ch[0], rtol=2e-5, atol=1e-7)
+    np.testing.assert_allclose(v[1], ch[1], rtol=2e-5, atol=1e-6)
 
 
 def test_spatial_laplace_matches_oracle(ctx32, po):
@@ -180,16 +181,35 @@ def test_registered_cloud_applies_pose(ctx32):
 
 
 def test_pairs_batch_matches_single(ctx32):
-    import torch
+    """The same pair gives the same result alone and inside a batch."""
     from phaser_amd.capi import Context
     ctx = Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=4)
     pairs = [_pair(s, dict(rings=64, azimuths=1024)) for s in range(4)]
     singles = [ctx32.register(t, s) for s, t, _, _ in pairs]
-    dt = [torch.from_numpy(t).cuda() for _, t, _, _ in pairs]
-    ds = [torch.from_numpy(s).cuda() for s, _, _, _ in pairs]
-    res = ctx.register_pairs_device([t.data_ptr() for t in dt], [t.shape[0] for t in dt],
-                                    [s.data_ptr() for s in ds], [s.shape[0] for s in ds])
+    dt = [ctx.device_array(t) for _, t, _, _ in pairs]
+    ds = [ctx.device_array(s) for s, _, _, _ in pairs]
+    res = ctx.register_pairs_device([a.ptr for a in dt], [len(t) for _, t, _, _ in pairs],
+                                    [a.ptr for a in ds], [len(s) for s, _, _, _ in pairs])
     for a, b in zip(res, singles):
         assert a["rot_index"] == b["rot_index"]
         assert a["trans_index"] == b["trans_index"]
+    # shared target: every source against pair 0's target
+    shared = ctx.register_pairs_device([dt[0].ptr], [len(pairs[0][1])], [a.ptr for a in ds],
+                                       [len(s) for s, _, _, _ in pairs], shared_target=True)
+    for k, (s, _, _, _) in enumerate(pairs):
+        alone = ctx32.register(pairs[0][1], s)
+        assert shared[k]["rot_index"] == alone["rot_index"]
+        assert shared[k]["trans_index"] == alone["trans_index"]
+    for a in dt + ds:
+        a.free()
     ctx.close()
+
+
+def test_profiling_stage_times(ctx32):
+    src, tgt, _, _ = _pair(8, dict(rings=64, azimuths=1024))
+    ctx32.set_profiling(True)
+    ctx32.register(tgt, src)
+    st = ctx32.stage_times()
+    ctx32.set_profiling(False)
+    assert st["s2_nn"][1] == 1 and st["s2_nn"][0] > 0
+    assert st["spatial_inverse"][0] > 0
