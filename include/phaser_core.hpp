@@ -1,0 +1,271 @@
+// phaser_core.hpp -- C++ host mirror of the reference's phaser_core
+// interfaces for the correlation path, implemented over the C ABI of
+// phaser_amd.h (libphaser_host.so).
+//
+// Names, argument meaning and error behaviour follow the reference:
+//   BaseRegistration / SphOptRegistration  base-registration.h:11-27,
+//                                          sph-opt-registration.h
+//   SphericalCorrelation(+Laplace)         spherical-correlation.h:19-33
+//   BaseSpatialCorrelation / SpatialCorrelation(+Laplace)
+//                                          base-spatial-correlation.h:10-15,
+//                                          spatial-correlation.h
+//   PhaseAligner                           phase-aligner.h / base-aligner.h:12-21
+//   common::SphericalSampler               spherical-sampler.h
+//   CloudController                        cloud-controller.h:14-34
+//   model::PointCloud / FunctionValue / RegistrationResult (the fields the
+//   correlation path reads and writes)
+// Flags are the gflags of core-gflags.cc:6-39, as plain globals read by the
+// constructors.  Errors abort the process with a message, as glog CHECK does
+// in the reference.  Eigen types in the reference signatures are replaced by
+// std::vector<double> (Eigen is not a dependency of this build).
+#ifndef PHASER_CORE_HPP_
+#define PHASER_CORE_HPP_
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+struct phz_ctx;
+
+namespace phaser_core {
+// core-gflags.cc:6-39
+extern int32_t FLAGS_phaser_core_spherical_bandwidth;
+extern int32_t FLAGS_phaser_core_spherical_zero_padding;
+extern int32_t FLAGS_phaser_core_spherical_low_pass_lower_bound;
+extern int32_t FLAGS_phaser_core_spherical_low_pass_upper_bound;
+extern int32_t FLAGS_phaser_core_spatial_n_voxels;
+extern int32_t FLAGS_phaser_core_spatial_discretize_lower;
+extern int32_t FLAGS_phaser_core_spatial_discretize_upper;
+extern int32_t FLAGS_phaser_core_spatial_zero_padding;
+extern int32_t FLAGS_phaser_core_spatial_low_pass_lower_bound;
+extern int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound;
+}  // namespace phaser_core
+
+namespace common {
+// pcl::PointXYZI fields used by the path (point-types.h:10-14)
+struct Point_t {
+  float x = 0, y = 0, z = 0, intensity = 0;
+};
+}  // namespace common
+
+namespace model {
+
+class PointCloud {
+ public:
+  PointCloud() = default;
+  explicit PointCloud(std::vector<common::Point_t> points) : points_(std::move(points)) {}
+  std::size_t size() const { return points_.size(); }
+  common::Point_t& pointAt(std::size_t i) { return points_.at(i); }
+  const common::Point_t& pointAt(std::size_t i) const { return points_.at(i); }
+  const float* data() const { return reinterpret_cast<const float*>(points_.data()); }
+  float* data() { return reinterpret_cast<float*>(points_.data()); }
+  std::vector<common::Point_t>& points() { return points_; }
+  // row-major 4x4, PCL scalar order (point-cloud.cc:183-185)
+  void transformPointCloud(const std::array<float, 16>& T);
+  PointCloud clone() const { return *this; }
+
+ private:
+  std::vector<common::Point_t> points_;
+};
+using PointCloudPtr = std::shared_ptr<PointCloud>;
+
+// function-value.h: one sample per grid point (sampling_neighbors = 1)
+class FunctionValue {
+ public:
+  FunctionValue() = default;
+  FunctionValue(double range, double intensity) : range_(range), intensity_(intensity) {}
+  double getAveragedRange() const noexcept { return range_; }
+  double getAveragedIntensity() const noexcept { return intensity_; }
+  double getAveragedReflectivity() const noexcept { return 0.0; }
+  double getAveragedAmbientNoise() const noexcept { return 0.0; }
+
+ private:
+  double range_ = 0.0, intensity_ = 0.0;
+};
+using FunctionValueVec = std::vector<FunctionValue>;
+
+// registration-result.cc:13-99 and state.cc:10-20 (pose as an 8-vector)
+class RegistrationResult {
+ public:
+  RegistrationResult() = default;
+  explicit RegistrationResult(PointCloud&& reg) : reg_cloud_(std::make_shared<PointCloud>(std::move(reg))) {}
+  PointCloudPtr getRegisteredCloud() const { return reg_cloud_; }
+  void setRegisteredCloud(PointCloudPtr c) { reg_cloud_ = std::move(c); }
+  // [qw, qx, qy, qz, 0, tx, ty, tz]
+  std::array<double, 8> getStateAsVec() const;
+  std::array<double, 4> getRotation() const { return quat_; }
+  std::array<double, 3> getTranslation() const { return trans_; }
+  void setRotation(const std::array<double, 4>& q) { quat_ = q; }
+  void setTranslation(const std::array<double, 3>& t) { trans_ = t; }
+  void setRotationCorrelation(std::vector<double> c) { rot_corr_ = std::move(c); }
+  const std::vector<double>& getRotationCorrelation() const { return rot_corr_; }
+  uint32_t rotation_index = 0, translation_index = 0;
+
+ private:
+  PointCloudPtr reg_cloud_;
+  std::array<double, 4> quat_{1, 0, 0, 0};
+  std::array<double, 3> trans_{0, 0, 0};
+  std::vector<double> rot_corr_;
+};
+
+}  // namespace model
+
+namespace phaser_core {
+
+using SampledSignal = std::vector<double>;
+using VectorXd = std::vector<double>;  // stands in for Eigen::VectorXd
+
+// One context per (bandwidth, voxel count, capacity); shared by the objects
+// below and released with the last of them.
+std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points);
+
+class SphericalCorrelation {
+ public:
+  explicit SphericalCorrelation(const uint32_t bw = 100, const uint32_t zero_padding = 0);
+  virtual ~SphericalCorrelation() = default;
+  // single blended channel 0.6 I + 0.4 R (spherical-correlation.cc:54-67,103-112)
+  void correlateSignals(const std::vector<model::FunctionValue>& f1,
+                        const std::vector<model::FunctionValue>& f2);
+  // base class: first channel only (spherical-correlation.cc:69-80)
+  virtual void correlateSampledSignals(const std::vector<SampledSignal>& f1,
+                                       const std::vector<SampledSignal>& f2);
+  std::vector<double> getCorrelation() const noexcept { return corr_; }
+  uint32_t getBandwidth() const noexcept { return bw_ + zero_padding_; }
+  uint32_t getArgmax() const noexcept { return argmax_; }
+  void shutdown() {}
+
+ protected:
+  uint32_t bw_, zero_padding_;
+  std::shared_ptr<phz_ctx> ctx_;
+  std::vector<double> corr_;
+  uint32_t argmax_ = 0;
+};
+
+// spherical-correlation-laplace.cc:14-40 (>= 2 channels, fused)
+class SphericalCorrelationLaplace : public SphericalCorrelation {
+ public:
+  explicit SphericalCorrelationLaplace(const uint32_t bw, const uint32_t zero_padding = 0)
+      : SphericalCorrelation(bw, zero_padding) {}
+  void correlateSampledSignals(const std::vector<SampledSignal>& f,
+                               const std::vector<SampledSignal>& g) override;
+};
+
+// base-spatial-correlation.h:10-15; the returned buffer is owned by the
+// object and valid until the next call (spatial-correlation.cc:47,73)
+class BaseSpatialCorrelation {
+ public:
+  virtual ~BaseSpatialCorrelation() = default;
+  virtual double* correlateSignals(const std::vector<VectorXd*>& f,
+                                   const std::vector<VectorXd*>& g) = 0;
+  virtual uint32_t getZeroPadding() const = 0;
+};
+
+class SpatialCorrelation : public BaseSpatialCorrelation {
+ public:
+  explicit SpatialCorrelation(const uint32_t n_voxels, const uint32_t zero_padding = 0);
+  double* correlateSignals(const std::vector<VectorXd*>& f,
+                           const std::vector<VectorXd*>& g) override;
+  uint32_t getZeroPadding() const override { return zero_padding_; }
+  uint32_t getCorrelationSize() const { return total_; }
+
+ protected:
+  uint32_t n_voxels_, zero_padding_, total_;
+  std::shared_ptr<phz_ctx> ctx_;
+  std::vector<double> c_;
+};
+
+class SpatialCorrelationLaplace : public SpatialCorrelation {
+ public:
+  explicit SpatialCorrelationLaplace(const uint32_t n_voxels, const uint32_t zero_padding = 0)
+      : SpatialCorrelation(n_voxels, zero_padding) {}
+  double* correlateSignals(const std::vector<VectorXd*>& f,
+                           const std::vector<VectorXd*>& g) override;
+};
+
+// base-aligner.h:12-21 / phase-aligner.cc:19-148
+class PhaseAligner {
+ public:
+  PhaseAligner();
+  void alignRegistered(const model::PointCloud& cloud_prev,
+                       const std::vector<model::FunctionValue>& f_prev,
+                       const model::PointCloud& cloud_reg,
+                       const std::vector<model::FunctionValue>& f_reg);
+  std::vector<double> getCorrelation() const { return previous_correlation_; }
+  uint32_t getNumberOfVoxels() const noexcept { return n_voxels_; }
+  int32_t getLowerBound() const noexcept { return lower_bound_; }
+  int32_t getUpperBound() const noexcept { return upper_bound_; }
+  // {intensity, range, reflectivity, ambient} means per voxel
+  void discretizePointcloud(const model::PointCloud& cloud, std::vector<VectorXd>* f) const;
+
+ private:
+  uint32_t n_voxels_;
+  int32_t lower_bound_, upper_bound_;
+  std::shared_ptr<phz_ctx> ctx_;
+  std::unique_ptr<SpatialCorrelationLaplace> spatial_correlation_;
+  std::vector<double> previous_correlation_;
+};
+
+}  // namespace phaser_core
+
+namespace common {
+// spherical-sampler.cc:8-64
+class SphericalSampler {
+ public:
+  explicit SphericalSampler(const int bandwith);
+  void initialize(const int bandwith);
+  void sampleUniformly(const model::PointCloud& cloud, std::vector<model::FunctionValue>* grid);
+  int getInitializedBandwith() const noexcept { return bandwith_; }
+
+ private:
+  int bandwith_;
+  std::shared_ptr<phz_ctx> ctx_;
+};
+}  // namespace common
+
+namespace phaser_core {
+
+class BaseRegistration {
+ public:
+  explicit BaseRegistration(const std::string& name) : name_(name) {}
+  virtual ~BaseRegistration() = default;
+  virtual model::RegistrationResult registerPointCloud(model::PointCloudPtr cloud_prev,
+                                                       model::PointCloudPtr cloud_cur) = 0;
+  const std::string& getName() const { return name_; }
+
+ protected:
+  std::string name_;
+};
+
+// sph-opt-registration.cc:21-146 on the device: one fused GPU pipeline per
+// call (sampling, S^2 transforms, fusion, SO(3) correlation, argmax, pose,
+// voxelisation, spatial correlation, argmax, translation).
+class SphOptRegistration : public BaseRegistration {
+ public:
+  SphOptRegistration();
+  model::RegistrationResult registerPointCloud(model::PointCloudPtr cloud_prev,
+                                               model::PointCloudPtr cloud_cur) override;
+  void setBandwith(const int bandwith);
+  // per-stage device time (ms, calls) since the last call
+  std::vector<std::pair<std::string, double>> getStatistics() const;
+
+ private:
+  int bandwidth_;
+  std::shared_ptr<phz_ctx> ctx_;
+};
+
+// cloud-controller.cc:10-34 (preprocessing is not part of this build)
+class CloudController {
+ public:
+  explicit CloudController(const std::string& method = "sph-opt");
+  model::RegistrationResult registerPointCloud(model::PointCloudPtr cloud_prev,
+                                               model::PointCloudPtr cloud_cur);
+
+ private:
+  std::unique_ptr<BaseRegistration> registrator_;
+};
+
+}  // namespace phaser_core
+
+#endif  // PHASER_CORE_HPP_

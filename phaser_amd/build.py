@@ -68,10 +68,17 @@ def build(verbose=False, jobs=8):
     host_src = [os.path.join(CSRC, s) for s in HOST_SOURCES if os.path.exists(os.path.join(CSRC, s))]
     if host_src:
         hout = os.path.join(LIB, "libphaser_host.so")
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-ffp-contract=off",
               "-I", os.path.join(ROOT, "include"), "-o", hout + ".tmp", *host_src,
               "-L", LIB, "-lphaser_amd", "-Wl,-rpath,$ORIGIN"], verbose)
         os.replace(hout + ".tmp", hout)
+        drv = os.path.join(ROOT, "tests", "cpp", "host_driver.cc")
+        if os.path.exists(drv):
+            dout = os.path.join(LIB, "phaser_host_driver")
+            _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+                  "-o", dout + ".tmp", drv, "-L", LIB, "-lphaser_host", "-lphaser_amd",
+                  "-Wl,-rpath,$ORIGIN"], verbose)
+            os.replace(dout + ".tmp", dout)
     sout = os.path.join(LIB, "libphaser_synth.so")
     _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-o", sout + ".tmp",
           os.path.join(CSRC, "synth.c"), "-lm"], verbose)

@@ -1,0 +1,320 @@
+// phaser_host.cc -- the C++ classes of include/phaser_core.hpp over the C
+// ABI of include/phaser_amd.h.  Host-side orchestration only: every numeric
+// stage runs in libphaser_amd.so on the GPU.
+#include "phaser_core.hpp"
+
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "phaser_amd.h"
+
+namespace phaser_core {
+int32_t FLAGS_phaser_core_spherical_bandwidth = 75;
+int32_t FLAGS_phaser_core_spherical_zero_padding = 0;
+int32_t FLAGS_phaser_core_spherical_low_pass_lower_bound = 0;
+int32_t FLAGS_phaser_core_spherical_low_pass_upper_bound = 100000;
+int32_t FLAGS_phaser_core_spatial_n_voxels = 100;
+int32_t FLAGS_phaser_core_spatial_discretize_lower = -50;
+int32_t FLAGS_phaser_core_spatial_discretize_upper = 50;
+int32_t FLAGS_phaser_core_spatial_zero_padding = 0;
+int32_t FLAGS_phaser_core_spatial_low_pass_lower_bound = 0;
+int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound = 100000;
+}  // namespace phaser_core
+
+namespace {
+
+// glog CHECK equivalent: the reference aborts on every failed precondition
+[[noreturn]] void fatal(const std::string& what) {
+  std::fprintf(stderr, "F phaser_core: %s\n", what.c_str());
+  std::fflush(stderr);
+  std::abort();
+}
+
+void check(int rc, const char* call) {
+  if (rc != 0) fatal(std::string(call) + " failed: " + phz_last_error());
+}
+
+#define CHECK_TRUE(cond, msg) \
+  do {                        \
+    if (!(cond)) fatal(msg);  \
+  } while (0)
+
+constexpr int64_t kDefaultCapacity = 262144;
+
+}  // namespace
+
+namespace phaser_core {
+
+std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int64_t, int, int, int, int>, std::weak_ptr<phz_ctx>> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const int64_t cap = max_points > kDefaultCapacity ? max_points : kDefaultCapacity;
+  const auto key = std::make_tuple(bw, n_voxels, cap, FLAGS_phaser_core_spatial_discretize_lower,
+                                   FLAGS_phaser_core_spatial_discretize_upper,
+                                   FLAGS_phaser_core_spatial_low_pass_lower_bound,
+                                   FLAGS_phaser_core_spatial_low_pass_upper_bound);
+  if (auto p = cache[key].lock()) return p;
+  phz_config cfg;
+  phz_default_config(&cfg);
+  cfg.bandwidth = bw;
+  cfg.n_voxels = n_voxels;
+  cfg.max_points = cap;
+  cfg.discretize_lower = FLAGS_phaser_core_spatial_discretize_lower;
+  cfg.discretize_upper = FLAGS_phaser_core_spatial_discretize_upper;
+  cfg.low_pass_lower = FLAGS_phaser_core_spatial_low_pass_lower_bound;
+  cfg.low_pass_upper = FLAGS_phaser_core_spatial_low_pass_upper_bound;
+  phz_ctx* raw = nullptr;
+  check(phz_create(&cfg, &raw), "phz_create");
+  std::shared_ptr<phz_ctx> p(raw, [](phz_ctx* c) { phz_destroy(c); });
+  cache[key] = p;
+  return p;
+}
+
+// ---------------------------------------------------------------- spherical
+SphericalCorrelation::SphericalCorrelation(const uint32_t bw, const uint32_t zero_padding)
+    : bw_(bw), zero_padding_(zero_padding) {
+  CHECK_TRUE(zero_padding == 0, "spherical zero padding other than 0 is not supported");
+  ctx_ = acquireContext((int)bw, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity);
+}
+
+void SphericalCorrelation::correlateSignals(const std::vector<model::FunctionValue>& f1,
+                                            const std::vector<model::FunctionValue>& f2) {
+  SampledSignal a, b;
+  for (const auto& v : f1) a.push_back(0.6 * v.getAveragedIntensity() + 0.4 * v.getAveragedRange());
+  for (const auto& v : f2) b.push_back(0.6 * v.getAveragedIntensity() + 0.4 * v.getAveragedRange());
+  correlateSampledSignals({a}, {b});
+}
+
+void SphericalCorrelation::correlateSampledSignals(const std::vector<SampledSignal>& f1,
+                                                   const std::vector<SampledSignal>& f2) {
+  CHECK_TRUE(f1.size() == f2.size(), "channel counts differ");
+  CHECK_TRUE(!f1.empty(), "no channels");
+  const size_t nq = 4ull * bw_ * bw_;
+  CHECK_TRUE(f1[0].size() == nq && f2[0].size() == nq, "signal size is not (2 bw)^2");
+  std::vector<float> grids(2 * nq);
+  for (size_t i = 0; i < nq; ++i) {
+    grids[i] = (float)f1[0][i];
+    grids[nq + i] = (float)f2[0][i];
+  }
+  std::vector<float> coef(2 * 2 * (size_t)bw_ * bw_);
+  check(phz_s2_transform(ctx_.get(), 2, grids.data(), coef.data()), "phz_s2_transform");
+  const size_t n3 = 8ull * bw_ * bw_ * bw_;
+  std::vector<float> corr(n3);
+  check(phz_so3_correlate_coef(ctx_.get(), coef.data(), coef.data() + 2 * (size_t)bw_ * bw_,
+                               corr.data(), nullptr, &argmax_),
+        "phz_so3_correlate_coef");
+  corr_.assign(corr.begin(), corr.end());
+}
+
+void SphericalCorrelationLaplace::correlateSampledSignals(const std::vector<SampledSignal>& f,
+                                                          const std::vector<SampledSignal>& g) {
+  CHECK_TRUE(f.size() == g.size(), "channel counts differ");
+  CHECK_TRUE(f.size() > 1u, "the Laplace fusion needs at least two channels");
+  const int C = (int)f.size();
+  const size_t nq = 4ull * bw_ * bw_;
+  std::vector<float> fg(C * nq), gg(C * nq);
+  for (int c = 0; c < C; ++c) {
+    CHECK_TRUE(f[c].size() == nq && g[c].size() == nq, "signal size is not (2 bw)^2");
+    for (size_t i = 0; i < nq; ++i) {
+      fg[c * nq + i] = (float)f[c][i];
+      gg[c * nq + i] = (float)g[c][i];
+    }
+  }
+  std::vector<float> corr(8ull * bw_ * bw_ * bw_);
+  check(phz_spherical_correlate(ctx_.get(), C, fg.data(), gg.data(), corr.data(), &argmax_),
+        "phz_spherical_correlate");
+  corr_.assign(corr.begin(), corr.end());
+}
+
+// ---------------------------------------------------------------- spatial
+SpatialCorrelation::SpatialCorrelation(const uint32_t n_voxels, const uint32_t zero_padding)
+    : n_voxels_(n_voxels), zero_padding_(zero_padding), total_(n_voxels * n_voxels * n_voxels) {
+  CHECK_TRUE(zero_padding == 0, "spatial zero padding other than 0 is not supported");
+  ctx_ = acquireContext(FLAGS_phaser_core_spherical_bandwidth, (int)n_voxels, kDefaultCapacity);
+}
+
+double* SpatialCorrelation::correlateSignals(const std::vector<VectorXd*>& f,
+                                             const std::vector<VectorXd*>& g) {
+  CHECK_TRUE(!f.empty() && !g.empty(), "no channels");
+  CHECK_TRUE(f[0]->size() == total_ && g[0]->size() == total_, "signal size is not N^3");
+  std::vector<float> a(f[0]->begin(), f[0]->end()), b(g[0]->begin(), g[0]->end());
+  std::vector<float> corr(total_);
+  check(phz_spatial_correlate_single(ctx_.get(), (int)n_voxels_, a.data(), b.data(), corr.data(),
+                                     nullptr),
+        "phz_spatial_correlate_single");
+  c_.assign(corr.begin(), corr.end());
+  return c_.data();
+}
+
+double* SpatialCorrelationLaplace::correlateSignals(const std::vector<VectorXd*>& f,
+                                                    const std::vector<VectorXd*>& g) {
+  CHECK_TRUE(f.size() == g.size() && f.size() > 1u, "the Laplace fusion needs >= 2 channels");
+  const int C = (int)f.size();
+  std::vector<float> a((size_t)C * total_), b((size_t)C * total_);
+  for (int c = 0; c < C; ++c) {
+    CHECK_TRUE(f[c]->size() == total_ && g[c]->size() == total_, "signal size is not N^3");
+    for (uint32_t i = 0; i < total_; ++i) {
+      a[(size_t)c * total_ + i] = (float)(*f[c])[i];
+      b[(size_t)c * total_ + i] = (float)(*g[c])[i];
+    }
+  }
+  std::vector<float> corr(total_);
+  check(phz_spatial_correlate(ctx_.get(), C, a.data(), b.data(), corr.data(), nullptr),
+        "phz_spatial_correlate");
+  c_.assign(corr.begin(), corr.end());
+  return c_.data();
+}
+
+PhaseAligner::PhaseAligner()
+    : n_voxels_(FLAGS_phaser_core_spatial_n_voxels),
+      lower_bound_(FLAGS_phaser_core_spatial_discretize_lower),
+      upper_bound_(FLAGS_phaser_core_spatial_discretize_upper) {
+  ctx_ = acquireContext(FLAGS_phaser_core_spherical_bandwidth, (int)n_voxels_, kDefaultCapacity);
+  spatial_correlation_.reset(
+      new SpatialCorrelationLaplace(n_voxels_, FLAGS_phaser_core_spatial_zero_padding));
+}
+
+void PhaseAligner::discretizePointcloud(const model::PointCloud& cloud,
+                                        std::vector<VectorXd>* f) const {
+  const size_t T = (size_t)n_voxels_ * n_voxels_ * n_voxels_;
+  std::shared_ptr<phz_ctx> ctx = ctx_;
+  if ((int64_t)cloud.size() > kDefaultCapacity)
+    ctx = acquireContext(FLAGS_phaser_core_spherical_bandwidth, (int)n_voxels_, cloud.size());
+  std::vector<float> out(2 * T);
+  check(phz_voxelize(ctx.get(), cloud.data(), (int64_t)cloud.size(), out.data(), nullptr),
+        "phz_voxelize");
+  f->assign(4, VectorXd(T, 0.0));  // {intensity, range, reflectivity, ambient}
+  for (size_t v = 0; v < T; ++v) {
+    (*f)[0][v] = out[v];
+    (*f)[1][v] = out[T + v];
+  }
+}
+
+void PhaseAligner::alignRegistered(const model::PointCloud& cloud_prev,
+                                   const std::vector<model::FunctionValue>&,
+                                   const model::PointCloud& cloud_reg,
+                                   const std::vector<model::FunctionValue>&) {
+  std::vector<VectorXd> f, g;
+  discretizePointcloud(cloud_prev, &f);
+  discretizePointcloud(cloud_reg, &g);
+  std::vector<VectorXd*> fp, gp;
+  for (int c = 0; c < 4; ++c) {
+    fp.push_back(&f[c]);
+    gp.push_back(&g[c]);
+  }
+  double* c = spatial_correlation_->correlateSignals(fp, gp);
+  previous_correlation_.assign(c, c + spatial_correlation_->getCorrelationSize());
+}
+
+// ---------------------------------------------------------------- registration
+SphOptRegistration::SphOptRegistration()
+    : BaseRegistration("SphOptRegistration"),
+      bandwidth_(FLAGS_phaser_core_spherical_bandwidth) {
+  CHECK_TRUE(FLAGS_phaser_core_spherical_zero_padding == 0, "spherical zero padding other than 0 is not supported");
+  CHECK_TRUE(FLAGS_phaser_core_spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
+  ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity);
+  check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
+}
+
+void SphOptRegistration::setBandwith(const int bandwith) {
+  bandwidth_ = bandwith;
+  ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity);
+}
+
+model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
+                                                                 model::PointCloudPtr cloud_cur) {
+  CHECK_TRUE(cloud_prev != nullptr, "cloud_prev is null");
+  CHECK_TRUE(cloud_cur != nullptr, "cloud_cur is null");
+  const int64_t need = (int64_t)std::max(cloud_prev->size(), cloud_cur->size());
+  std::shared_ptr<phz_ctx> ctx = ctx_;
+  if (need > kDefaultCapacity)
+    ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need);
+  phz_result r;
+  const size_t n3 = 8ull * bandwidth_ * bandwidth_ * bandwidth_;
+  std::vector<float> rot(n3);
+  model::PointCloud reg = cloud_cur->clone();
+  check(phz_register(ctx.get(), cloud_prev->data(), (int64_t)cloud_prev->size(), cloud_cur->data(),
+                     (int64_t)cloud_cur->size(), &r, rot.data(), nullptr, reg.data()),
+        "phz_register");
+  model::RegistrationResult result(std::move(reg));
+  result.setRotation({r.quat[0], r.quat[1], r.quat[2], r.quat[3]});
+  result.setTranslation({r.translation[0], r.translation[1], r.translation[2]});
+  result.setRotationCorrelation(std::vector<double>(rot.begin(), rot.end()));
+  result.rotation_index = r.rot_index;
+  result.translation_index = r.trans_index;
+  return result;
+}
+
+std::vector<std::pair<std::string, double>> SphOptRegistration::getStatistics() const {
+  phz_stage_time t[32];
+  int n = 0;
+  check(phz_get_stage_times(ctx_.get(), t, 32, &n), "phz_get_stage_times");
+  std::vector<std::pair<std::string, double>> out;
+  for (int i = 0; i < n && i < 32; ++i) out.emplace_back(t[i].name, t[i].total_ms);
+  return out;
+}
+
+CloudController::CloudController(const std::string& method) {
+  if (method == "sph-opt")
+    registrator_.reset(new SphOptRegistration());
+  else
+    fatal("registration method '" + method + "' is not part of this build (use sph-opt)");
+}
+
+model::RegistrationResult CloudController::registerPointCloud(model::PointCloudPtr cloud_prev,
+                                                              model::PointCloudPtr cloud_cur) {
+  return registrator_->registerPointCloud(cloud_prev, cloud_cur);
+}
+
+}  // namespace phaser_core
+
+// ---------------------------------------------------------------- model
+namespace model {
+
+void PointCloud::transformPointCloud(const std::array<float, 16>& T) {
+  for (auto& p : points_) {
+    const float x = T[0] * p.x + T[1] * p.y + T[2] * p.z + T[3];
+    const float y = T[4] * p.x + T[5] * p.y + T[6] * p.z + T[7];
+    const float z = T[8] * p.x + T[9] * p.y + T[10] * p.z + T[11];
+    p.x = x;
+    p.y = y;
+    p.z = z;
+  }
+}
+
+std::array<double, 8> RegistrationResult::getStateAsVec() const {
+  return {quat_[0], quat_[1], quat_[2], quat_[3], 0.0, trans_[0], trans_[1], trans_[2]};
+}
+
+}  // namespace model
+
+namespace common {
+
+SphericalSampler::SphericalSampler(const int bandwith) { initialize(bandwith); }
+
+void SphericalSampler::initialize(const int bandwith) {
+  bandwith_ = bandwith;
+  ctx_ = phaser_core::acquireContext(bandwith, phaser_core::FLAGS_phaser_core_spatial_n_voxels,
+                                     kDefaultCapacity);
+}
+
+void SphericalSampler::sampleUniformly(const model::PointCloud& cloud,
+                                       std::vector<model::FunctionValue>* grid) {
+  CHECK_TRUE(grid != nullptr, "grid is null");
+  std::shared_ptr<phz_ctx> ctx = ctx_;
+  if ((int64_t)cloud.size() > kDefaultCapacity)
+    ctx = phaser_core::acquireContext(bandwith_, phaser_core::FLAGS_phaser_core_spatial_n_voxels,
+                                      cloud.size());
+  const size_t nq = 4ull * bandwith_ * bandwith_;
+  std::vector<float> g(2 * nq);
+  check(phz_sample_s2(ctx.get(), cloud.data(), (int64_t)cloud.size(), g.data()), "phz_sample_s2");
+  grid->clear();
+  grid->reserve(nq);
+  for (size_t i = 0; i < nq; ++i) grid->emplace_back(g[i], g[nq + i]);
+}
+
+}  // namespace common
