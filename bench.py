@@ -8,12 +8,17 @@ One step registers `--batch` independent pairs whose clouds are already in
 HBM (one phz_register_pairs_device call, correlation grids written out), so
 value = batch * steps / time.  Latency p50/p99 comes from single-pair calls.
 The argmax-match rate compares the rotation bin and translation voxel of the
-first --cpu-sample pairs with the FP64 oracle, which is also timed on one
-host core as the CPU baseline.
+first --cpu-sample pairs with the FP64 oracle, which is also timed on up to
+--cpu-threads host threads (one pair per thread) as the CPU baseline.  The
+roofline names the kernel of the stage with the largest device time: its
+algorithmic bytes per launch over its mean launch duration (HIP events on
+the context stream), against the 8 TB/s HBM peak; the streaming copy
+kernel's rate is reported beside it.
 
 Prints one JSON line.
 """
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import sys
@@ -29,25 +34,121 @@ B, N = 64, 128
 WORKLOAD = dict(rings=128, azimuths=1024, noise_sigma=0.02, overlap=0.7)
 
 
-def stage_bytes(B, N, P_mean, pairs):
-    """Algorithmic (compulsory) HBM bytes of one launch of each stage for a
-    batch of `pairs` independent registrations: inputs read once, outputs
-    written once, FP32 / complex64 elements, C = 2 populated channels."""
+def kernel_bytes(B, N, P_mean, pairs, shared=False):
+    """Algorithmic (compulsory) HBM bytes of ONE launch of the kernel that
+    dominates each stage, for a batch of `pairs` registrations: inputs read
+    once, outputs written once, FP32 / complex64 elements, C = 2 populated
+    channels (range/intensity), H = N^2 (N/2+1) half-spectrum entries.
+    Returns {stage: (kernel name, bytes per launch)}."""
     n = 2 * B
     C = 2
     H = N * N * (N // 2 + 1)
-    clouds = 2 * pairs
+    nt = 1 if shared else pairs
+    clouds = nt + pairs
     return {
-        "s2_cells": clouds * (16 * P_mean + 24 * P_mean),
-        "s2_nn": clouds * (24 * P_mean + 16 * n * n + 4 * C * n * n),
-        "s2_transform": clouds * (4 * C * n * n + 8 * C * B * B),
-        "so3_synth": 4 * n * B * (B + 1) * (B + 2) // 6 + pairs * 8 * n * B * (2 * B - 1),
-        "so3_fft_alpha": pairs * (8 * n * B * (2 * B - 1) + 8 * n * B * n),
-        "so3_fft_gamma": pairs * (8 * n * B * n + 4 * n ** 3),
-        "voxel_scatter": clouds * (16 * P_mean + 4 * (C + 1) * N ** 3),
-        "spatial_forward": clouds * C * (4 * N ** 3 + 8 * H),
-        "spatial_inverse": pairs * (8 * H * C * 2 + 4 * N ** 3),
+        # sorted points + values read once, grid queries once, C grids written
+        "s2_nn": ("k_nn", clouds * (24 * P_mean + 4 * C * n * n) + 16 * n * n),
+        # Wigner table once per batch + fused coefficients in, S half-plane out
+        "so3_synth": ("k_so3_synth", 4 * n * B * (B + 1) * (B + 2) // 6
+                      + clouds * 8 * B * B + pairs * 8 * n * B * (2 * B - 1)),
+        "so3_fft_alpha": ("k_so3_fft_alpha", pairs * (8 * n * B * (2 * B - 1) + 8 * n * B * n)),
+        "so3_fft_gamma": ("k_so3_fft_gamma", pairs * (8 * n * B * n + 4 * n ** 3)),
+        # points in; three FP32 accumulators read-modify-written per point
+        "voxel_scatter": ("k_voxel_scatter<false>", clouds * (16 + 24) * P_mean),
+        # three N^3 accumulators in, two half spectra out
+        "spatial_r2c": ("k_spat_r2c", clouds * (12 * N ** 3 + 2 * 8 * H)),
+        # one FFT pass (x or z) over every half spectrum: read + write
+        "spatial_fft_fwd": ("k_fft_cols<-1>", 2 * clouds * 16 * H),
+        # C target + C source spectra in, cross spectrum out
+        "spatial_cross_z": ("k_spat_cross_z", nt * C * 8 * H + pairs * (C * 8 * H + 8 * H)),
+        "spatial_fft_inv": ("k_fft_cols<1>", pairs * 16 * H),
+        "spatial_c2r": ("k_spat_c2r", pairs * (8 * H + 4 * N ** 3)),
     }
+
+
+# launches of the stage's dominant kernel per pipeline pass
+LAUNCHES = {"spatial_fft_fwd": 2, "voxel_scatter": 2}
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on the
+    streaming copy kernel), or None when no summary covers it."""
+    from phaser_amd import profiling
+    path = os.path.join(ROOT, "profiles", "traffic_b64_n128_batch16.json")
+    try:
+        with open(path) as f:
+            return profiling.lookup(json.load(f), kernel)
+    except (OSError, ValueError):
+        return None
+
+
+SERVE_SHAPES = [(32, 64), (64, 128), (96, 192), (128, 256)]
+
+
+def serve_bench(args):
+    """Poisson arrivals of scan pairs with bandwidth B in {32, 64, 96, 128}
+    (voxel grids 64^3 .. 256^3, overlap 40-90%, noise 1-5 cm, 5% outliers)
+    through the continuous-batching request loop, at each offered rate of
+    --rates: throughput and p50/p99 latency (submit -> result)."""
+    from phaser_amd import serve, synth
+    rng = np.random.default_rng(0)
+    pool = {sh: [synth.mixed_pair(seed, sh[0]) for seed in range(args.pool)]
+            for sh in SERVE_SHAPES}
+    max_pts = max(max(len(s), len(t)) for v in pool.values() for s, t, _, _ in v)
+    rates = [float(x) for x in args.rates.split(",")]
+    sweep = []
+    with serve.Server(max_batch=args.serve_batch, contexts_per_shape=2,
+                      hbm_budget=int(args.hbm_budget_gib * (1 << 30)),
+                      max_points=max_pts) as srv:
+        for sh in SERVE_SHAPES:  # build the contexts before timing
+            src, tgt, _, _ = pool[sh][0]
+            srv.wait(srv.submit(tgt, src, bandwidth=sh[0], n_voxels=sh[1]))
+        for lam in rates:
+            n_req = max(int(lam * args.duration), 8)
+            tickets, shapes = [], []
+            t_start = time.perf_counter()
+            next_t = t_start
+            for _ in range(n_req):
+                next_t += rng.exponential(1.0 / lam)
+                delay = next_t - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                sh = SERVE_SHAPES[int(rng.integers(len(SERVE_SHAPES)))]
+                src, tgt, _, _ = pool[sh][int(rng.integers(args.pool))]
+                tickets.append(srv.submit(tgt, src, bandwidth=sh[0], n_voxels=sh[1]))
+                shapes.append(sh[0])
+            got = [srv.wait(t) for t in tickets]
+            t_end = time.perf_counter()
+            lat = np.array([g["latency_ms"] for g in got])
+            per_b = {}
+            for b, g in zip(shapes, got):
+                per_b.setdefault(b, []).append(g["latency_ms"])
+            sweep.append({
+                "rate_offered": lam, "requests": n_req,
+                "throughput": round(n_req / (t_end - t_start), 2),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "mean_batch": round(float(np.mean([g["batch_pairs"] for g in got])), 2),
+                "failed": int(sum(g["status"] != 0 for g in got)),
+                "p50_ms_by_bandwidth": {str(b): round(float(np.percentile(v, 50)), 3)
+                                        for b, v in sorted(per_b.items())}})
+            print(f"serve rate {lam}: {sweep[-1]}", file=sys.stderr, flush=True)
+        st = srv.stats()
+    best = max(sweep, key=lambda r: r["throughput"])
+    out = {
+        "metric": "served registrations/sec, mixed B in {32,64,96,128}, Poisson arrivals",
+        "value": best["throughput"], "unit": "registrations/s", "n_gpus": 1,
+        "steps": len(rates), "warmup": len(SERVE_SHAPES), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "serving mix: B 32/64/96/128 with 64^3/128^3/192^3/256^3 "
+                               "voxels, overlap 40-90%, noise 1-5 cm, 5% outliers",
+                   "max_batch": args.serve_batch, "contexts_per_shape": 2,
+                   "hbm_budget_gib": args.hbm_budget_gib, "form": "plain fp32"},
+        "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "sweep": sweep,
+        "server": {k: int(v) for k, v in st.items()},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def main():
@@ -57,13 +158,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--latency-samples", type=int, default=40)
-    ap.add_argument("--cpu-sample", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--mode", choices=["batch", "serve"], default="batch")
+    ap.add_argument("--rates", default="100,200,400,800", help="serve: offered requests/s")
+    ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds per rate")
+    ap.add_argument("--pool", type=int, default=6, help="serve: pairs per bandwidth")
+    ap.add_argument("--serve-batch", type=int, default=8)
+    ap.add_argument("--hbm-budget-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); registrations are "
               "independent, so N GPUs would each run this same benchmark on their own pairs.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.mode == "serve":
+        return serve_bench(args)
 
     from phaser_amd import synth
     from phaser_amd.capi import Context
@@ -105,28 +216,54 @@ def main():
     lat = np.array(lat)
 
     # per-stage device time over `steps` more batched steps (HIP events on the
-    # context stream), dominant stage -> roofline
+    # context stream); the stage with the largest time is the roofline kernel
     ctx.set_profiling(True)
     for _ in range(args.steps):
         step()
     st = ctx.stage_times()
     ctx.set_profiling(False)
-    alg = stage_bytes(B, N, P_mean, P)
-    dom = max((k for k in alg if st.get(k, (0, 0))[1] > 0), key=lambda k: st[k][0])
-    dom_ms = st[dom][0] / st[dom][1]
-    achieved = alg[dom] / (dom_ms * 1e-3) / 1e9
+    kb = kernel_bytes(B, N, P_mean, P)
+    dom = max((k for k in kb if st.get(k, (0, 0))[1] > 0), key=lambda k: st[k][0])
+    launches = st[dom][1] * LAUNCHES.get(dom, 1)
+    dom_ms = st[dom][0] / launches
+    kname, kbytes = kb[dom]
+    achieved = kbytes / (dom_ms * 1e-3) / 1e9
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in st.items()}
 
-    # oracle: argmax match and CPU baseline on the first pairs
+    # HBM reference: the streaming copy kernel on 1 GiB
+    nb = 1 << 30
+    d_a, d_b = ctx.device_empty(nb), ctx.device_empty(nb)
+    ctx.copy_kernel(d_b.ptr, d_a.ptr, nb, 2)
+    copy_ms = ctx.copy_kernel(d_b.ptr, d_a.ptr, nb, 10)
+    copy_gbs = 2 * nb / (copy_ms * 1e-3) / 1e9
+    d_a.free()
+    d_b.free()
+
+    # oracle: argmax match on the first pairs and the CPU baseline, pairs
+    # spread over host threads (the oracle is re-entrant C; ctypes releases
+    # the GIL), plus one pair on one thread for the latency
     from oracle import oracle as po
     S = min(args.cpu_sample, P)
     match = 0
-    tc0 = time.perf_counter()
-    for k in range(S):
-        s, t, _, _ = pairs[k]
-        o = po.register(t, s, bw=B, n_voxels=N, kdtree=True)
-        match += int(o["rot_index"] == res[k]["rot_index"] and o["trans_index"] == res[k]["trans_index"])
-    cpu_s = time.perf_counter() - tc0
+    cpu_lat_ms = None
+    cpu_value = None
+    threads = 0
+    if S > 0:
+        ta = time.perf_counter()
+        s0, t0_, _, _ = pairs[0]
+        o0 = po.register(t0_, s0, bw=B, n_voxels=N, kdtree=True)
+        cpu_lat_ms = (time.perf_counter() - ta) * 1e3
+        threads = max(1, min(args.cpu_threads, S))
+        with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+            tc0 = time.perf_counter()
+            outs = list(ex.map(lambda k: po.register(pairs[k][1], pairs[k][0], bw=B, n_voxels=N,
+                                                     kdtree=True), range(S)))
+            cpu_s = time.perf_counter() - tc0
+        cpu_value = S / cpu_s
+        for k, o in enumerate(outs):
+            match += int(o["rot_index"] == res[k]["rot_index"]
+                         and o["trans_index"] == res[k]["trans_index"])
+        assert o0["rot_index"] == outs[0]["rot_index"]
 
     out = {
         "metric": "registrations/sec at B=64 + 128^3 voxels",
@@ -149,14 +286,19 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "argmax_match": {"matched": match, "pairs": S, "rate": round(match / max(S, 1), 4)},
         "stage_ms_per_step": stage_ms,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "bytes_per_launch": int(alg[dom]),
-                     "avg_launch_ms": round(dom_ms, 4)},
-        "cpu_baseline": {"value": round(S / cpu_s, 4), "unit": "registrations/s", "cores": 1,
-                         "kind": "port",
+        "roofline": {"bound": "hbm", "kernel": kname, "stage": dom, "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": load_traffic(kname), "bytes_per_launch": int(kbytes),
+                     "avg_launch_ms": round(dom_ms, 4),
+                     "copy_kernel_gbs": round(copy_gbs, 1)},
+        "cpu_baseline": {"value": None if cpu_value is None else round(cpu_value, 4),
+                         "unit": "registrations/s", "cores": threads, "kind": "port",
+                         "latency_ms_1_thread": None if cpu_lat_ms is None else round(cpu_lat_ms, 1),
+                         "host_cpus": os.cpu_count(),
                          "sample": f"{S} pairs of the same workload through the FP64 oracle "
-                                   f"(kd-tree NN, own FFT), {cpu_s:.1f} s"},
+                                   f"(kd-tree NN, own FFT) on {threads} host threads, one pair "
+                                   f"per thread at a time"},
     }
     print(json.dumps(out), flush=True)
 

@@ -92,6 +92,16 @@ int phz_register_pairs_device(phz_ctx* ctx, int n, int shared_target,
                               phz_result* results, float* d_rot_corr, float* d_trans_corr,
                               void* stream);
 
+/* The same from host clouds (copied to the device on the context stream). */
+int phz_register_pairs(phz_ctx* ctx, int n, int shared_target, const float* const* targets,
+                       const int64_t* Pt, const float* const* sources, const int64_t* Ps,
+                       phz_result* results);
+/* Device bytes owned by the context (tables and workspaces). */
+int phz_memory_usage(const phz_ctx* ctx, size_t* bytes);
+/* The device bytes phz_create would allocate for cfg (validates cfg; no
+ * device call), for admission control by a memory budget. */
+int phz_estimate_memory(const phz_config* cfg, size_t* bytes);
+
 /* ---- stages (host buffers) ---------------------------------------------- */
 /* SphericalSampler::sampleUniformly + the range and intensity channels of
  * SphericalCombinedWorker::run (spherical-sampler.cc:20-27,
@@ -148,6 +158,11 @@ int phz_device_free(phz_ctx* ctx, void* ptr);
  * waits for the context's stream first. */
 int phz_memcpy(phz_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
 int phz_synchronize(phz_ctx* ctx);
+/* A plain streaming copy kernel (bytes a multiple of 16) run `repeats`
+ * times on the context's stream; *ms = mean duration per launch.  The HBM
+ * reference against which kernel bandwidth and traffic counters are read. */
+int phz_copy_kernel(phz_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, int repeats,
+                    float* ms);
 
 #ifdef __cplusplus
 }

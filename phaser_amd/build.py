@@ -3,6 +3,8 @@
 * lib/libphaser_amd.so   -- HIP kernels for gfx950 + the C ABI (include/phaser_amd.h)
 * lib/libphaser_host.so  -- the C++ mirror of the reference's phaser_core classes
                             (BaseRegistration, SphOptRegistration, ...) over the C ABI
+* lib/libphaser_serve.so -- continuous-batching request loop over the C ABI
+                            (include/phaser_serve.h)
 * lib/libphaser_synth.so -- host-side seeded scan-pair generator
 
 Run `python -m phaser_amd.build` (or __graft_entry__.build()).
@@ -79,6 +81,12 @@ def build(verbose=False, jobs=8):
                   "-o", dout + ".tmp", drv, "-L", LIB, "-lphaser_host", "-lphaser_amd",
                   "-Wl,-rpath,$ORIGIN"], verbose)
             os.replace(dout + ".tmp", dout)
+    vout = os.path.join(LIB, "libphaser_serve.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-pthread",
+          "-I", os.path.join(ROOT, "include"), "-o", vout + ".tmp",
+          os.path.join(CSRC, "host", "phaser_serve.cc"), "-L", LIB, "-lphaser_amd",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(vout + ".tmp", vout)
     sout = os.path.join(LIB, "libphaser_synth.so")
     _run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-Wall", "-o", sout + ".tmp",
           os.path.join(CSRC, "synth.c"), "-lm"], verbose)

@@ -53,6 +53,10 @@ EXPORTS = {
     "phz_register_pairs_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                             C.c_void_p, C.c_void_p, C.POINTER(Result),
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
+    "phz_register_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.POINTER(Result)]),
+    "phz_memory_usage": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "phz_estimate_memory": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_size_t)]),
     "phz_sample_s2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
     "phz_s2_transform": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     "phz_so3_correlate_coef": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -70,6 +74,8 @@ EXPORTS = {
     "phz_device_free": (C.c_int, [C.c_void_p, C.c_void_p]),
     "phz_memcpy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]),
     "phz_synchronize": (C.c_int, [C.c_void_p]),
+    "phz_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
+                                  C.POINTER(C.c_float)]),
 }
 
 
@@ -217,6 +223,24 @@ class Context:
                                                C.c_void_p(stream or None)))
         return [r.as_dict() for r in res]
 
+    def register_pairs(self, targets, sources, shared_target=False):
+        """Host clouds; targets holds one cloud when shared_target is set."""
+        tg = [np.ascontiguousarray(t, dtype=np.float32) for t in targets]
+        sr = [np.ascontiguousarray(s, dtype=np.float32) for s in sources]
+        n = len(sr)
+        T = (C.c_void_p * len(tg))(*[t.ctypes.data for t in tg])
+        S = (C.c_void_p * n)(*[s.ctypes.data for s in sr])
+        PT = (C.c_int64 * len(tg))(*[len(t) for t in tg])
+        PS = (C.c_int64 * n)(*[len(s) for s in sr])
+        res = (Result * n)()
+        _check(lib().phz_register_pairs(self.h, n, int(shared_target), T, PT, S, PS, res))
+        return [r.as_dict() for r in res]
+
+    def memory_usage(self):
+        b = C.c_size_t()
+        _check(lib().phz_memory_usage(self.h, C.byref(b)))
+        return b.value
+
     # ---- device memory and statistics
     def device_array(self, a):
         a = np.ascontiguousarray(a)
@@ -227,6 +251,13 @@ class Context:
 
     def synchronize(self):
         _check(lib().phz_synchronize(self.h))
+
+    def copy_kernel(self, d_dst, d_src, nbytes, repeats=1):
+        """Mean ms per launch of the streaming copy kernel."""
+        ms = C.c_float()
+        _check(lib().phz_copy_kernel(self.h, C.c_void_p(d_dst), C.c_void_p(d_src), nbytes,
+                                     repeats, C.byref(ms)))
+        return ms.value
 
     def set_profiling(self, on=True):
         _check(lib().phz_set_profiling(self.h, int(on)))

@@ -82,6 +82,7 @@ struct phz_ctx {
   float4* d_reg = nullptr;
   int* h_npts = nullptr;  // pinned staging for per-cloud point counts
   std::vector<void*> allocs;
+  size_t alloc_bytes = 0;
   // per-stage HIP event timing (phz_set_profiling)
   struct Timing {
     const char* name;
@@ -95,14 +96,17 @@ struct phz_ctx {
 };
 
 namespace {
+// one stage per kernel where a kernel dominates it (spatial passes), so
+// that stage time / launches is that kernel's mean launch duration
 enum Stage {
   kS2Cells, kS2NN, kS2Transform, kFuse, kSo3Synth, kSo3Alpha, kSo3Gamma, kRotPose,
-  kVoxelScatter, kSpatialForward, kVoxelClear, kSpatialInverse, kTransPose, kNumStages
+  kVoxelScatter, kSpatialR2c, kSpatialFftFwd, kVoxelClear, kSpatialCrossZ, kSpatialFftInv,
+  kSpatialC2r, kTransPose, kNumStages
 };
 const char* kStageNames[kNumStages] = {
     "s2_cells", "s2_nn", "s2_transform", "fuse", "so3_synth", "so3_fft_alpha", "so3_fft_gamma",
-    "rot_pose", "voxel_scatter", "spatial_forward", "voxel_clear", "spatial_inverse",
-    "trans_pose"};
+    "rot_pose", "voxel_scatter", "spatial_r2c", "spatial_fft_fwd", "voxel_clear",
+    "spatial_cross_z", "spatial_fft_inv", "spatial_c2r", "trans_pose"};
 
 struct StageScope {
   phz_ctx* c;
@@ -145,6 +149,7 @@ int dalloc(phz_ctx* c, T** p, size_t count) {
     return -2;
   }
   c->allocs.push_back(q);
+  c->alloc_bytes += bytes;
   *p = (T*)q;
   return 0;
 }
@@ -162,8 +167,63 @@ int upload(phz_ctx* c, T** p, const std::vector<T>& v) {
     if (_rc) return _rc;      \
   } while (0)
 
-int init_ctx(phz_ctx* c) {
-  const phz_config& cf = c->cfg;
+// Device bytes init_ctx allocates for a configuration, in closed form (the
+// same list of buffers; init_ctx checks that the two agree).
+size_t context_bytes(const phz_config& cf) {
+  const int64_t B = cf.bandwidth, n = 2 * B, N = cf.n_voxels;
+  const int64_t nq = n * n, T3 = N * N * N, H = N * N * (N / 2 + 1);
+  const int64_t P = cf.max_batch, M = 2 * P, cap = cf.max_points;
+  const int64_t G = std::max<int64_t>(2 * M, 16), MV = std::max<int64_t>(M, 16);
+  const int64_t ppp = std::max(so3_gamma_blocks((int)B), spatial_c2r_blocks((int)N));
+  size_t s = 0;
+  auto add = [&](int64_t count, size_t elem) { s += elem * (size_t)(count ? count : 1); };
+  // tables
+  add(nq, sizeof(float4));
+  add(n, sizeof(float));
+  add(n * B * (B + 1) / 2, sizeof(float));
+  add(B + 1, sizeof(int));
+  add(n * B * (B + 1) * (B + 2) / 6, sizeof(float));
+  add(B * (B + 1) / 2 + 1, sizeof(int64_t));
+  add(B, sizeof(float));
+  add(n, sizeof(float2));
+  add(N, sizeof(float2));
+  add(N, sizeof(float));
+  // clouds and the S^2 search grid
+  add(M * cap, sizeof(float4));
+  add(M, sizeof(int));
+  add(M * nq, sizeof(int));
+  add(M * (nq + 1), sizeof(int));
+  add(M * (n + 1), sizeof(int));
+  add(M * nq, sizeof(int));
+  add(M * nq, sizeof(int));
+  add(M * cap, sizeof(float4));
+  add(M * cap, sizeof(float2));
+  add(M * cap, sizeof(float4));
+  add(M * cap, sizeof(float2));
+  add(M * cap, sizeof(int));
+  // spherical spectra and SO(3) grids
+  add(G * nq, sizeof(float));
+  add(G * B * n, sizeof(float2));
+  add(G * B * B, sizeof(float2));
+  add(std::max<int64_t>(M, 2) * B * B, sizeof(float2));
+  add(P * B * (2 * B - 1) * n, sizeof(float2));
+  add(P * n * B * n, sizeof(float2));
+  add(nq * n, sizeof(float));
+  add(2 * P, sizeof(unsigned long long));
+  add(P * ppp, sizeof(unsigned long long));
+  add(P, sizeof(DevPose));
+  // voxel accumulators, spectra and the translation grid
+  add(MV * T3, sizeof(float));
+  add(M * T3, sizeof(float));
+  add(M * T3, sizeof(uint32_t));
+  add(G * H, sizeof(float2));
+  add(P * H, sizeof(float2));
+  add(T3, sizeof(float));
+  add(cap, sizeof(float4));
+  return s;
+}
+
+int check_config(const phz_config& cf) {
   PHZ_CHECK(cf.bandwidth >= 2 && cf.bandwidth <= 128, "bandwidth must be in [2, 128]");
   PHZ_CHECK(cf.spherical_zero_padding == 0, "spherical zero padding other than 0 is not supported");
   PHZ_CHECK(cf.spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
@@ -171,6 +231,14 @@ int init_ctx(phz_ctx* c) {
   PHZ_CHECK(cf.discretize_lower < cf.discretize_upper, "discretize bounds must be increasing");
   PHZ_CHECK(cf.max_points >= 1 && cf.max_points <= (1ll << 26), "max_points out of range");
   PHZ_CHECK(cf.max_batch >= 1 && cf.max_batch <= 1024, "max_batch out of range");
+  PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
+  PHZ_CHECK(make_plan(cf.n_voxels).nst > 0, "n_voxels must factor into 2, 3 and 5");
+  return 0;
+}
+
+int init_ctx(phz_ctx* c) {
+  const phz_config& cf = c->cfg;
+  TRY(check_config(cf));
   c->B = cf.bandwidth;
   c->n = 2 * c->B;
   c->N = cf.n_voxels;
@@ -181,8 +249,6 @@ int init_ctx(phz_ctx* c) {
   c->max_clouds = 2 * cf.max_batch;
   c->plan_n = make_plan(c->n);
   c->plan_N = make_plan(c->N);
-  PHZ_CHECK(c->plan_n.nst > 0, "2*bandwidth must factor into 2, 3 and 5");
-  PHZ_CHECK(c->plan_N.nst > 0, "n_voxels must factor into 2, 3 and 5");
   {
     const float v = (float)(std::abs(cf.low_pass_lower) + std::abs(cf.low_pass_upper)) /
                     (float)cf.n_voxels;  // point-cloud.cc:66-72
@@ -214,6 +280,11 @@ int init_ctx(phz_ctx* c) {
   TRY(upload(c, &c->d_edges, c->vox.edges));
   const int M = c->max_clouds, P = cf.max_batch;
   const int64_t cap = c->cap, nq = (int64_t)c->n * c->n;
+  // NN search grid at the query resolution: measured on gfx950 at B = 64,
+  // an (n/2)^2 grid (fewer cells, twice the points per cell) made the
+  // search 2.5x slower
+  c->cells.nb = c->n;
+  c->cells.ns = c->n;
   TRY(dalloc(c, &c->d_pts, (size_t)M * cap));
   TRY(dalloc(c, &c->d_npts, (size_t)M));
   TRY(dalloc(c, &c->cells.cell_count, (size_t)M * nq));
@@ -254,6 +325,7 @@ int init_ctx(phz_ctx* c) {
   launch_fill_u32((uint32_t*)c->d_sumR, (size_t)M * c->T3, 0u, c->stream);
   launch_fill_u32(c->d_cnt, (size_t)M * c->T3, 0u, c->stream);
   PHZ_HIP(hipStreamSynchronize(c->stream));
+  PHZ_CHECK(c->alloc_bytes == context_bytes(cf), "internal error: device memory estimate is stale");
   return 0;
 }
 
@@ -319,9 +391,12 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                          c->d_cnt + nt * c->T3, s);
   }
   {
-    StageScope t(c, kSpatialForward, s);
-    launch_spatial_forward(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
-                           c->d_spec, s);
+    StageScope t(c, kSpatialR2c, s);
+    launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N, c->d_spec, s);
+  }
+  {
+    StageScope t(c, kSpatialFftFwd, s);
+    launch_spatial_fft_fwd(N, 2 * ncl, c->d_spec, c->plan_N, c->d_tw_N, s);
   }
   {
     StageScope t(c, kVoxelClear, s);
@@ -331,16 +406,24 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                        c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
                        c->d_cnt + nt * c->T3, s);
   }
-  {
-    StageScope t(c, kSpatialInverse, s);
-    for (int p = 0; p < np; ++p) {  // spatial order {intensity, range, refl=0, amb=0}
-      const int tg = shared ? 0 : p;
-      const float2* F = c->d_spec + (int64_t)(2 * tg) * c->H;
-      const float2* G = c->d_spec + (int64_t)(2 * (nt + p)) * c->H;
-      launch_spatial_inverse(N, channels(F, F + c->H, 4), channels(G, G + c->H, 4), c->vox.runs,
-                             true, c->plan_N, c->d_tw_N, c->d_work + p * c->H,
-                             d_trans_corr ? d_trans_corr + p * c->T3 : nullptr,
-                             c->d_partials + p * c->partials_per_pair, c->d_keys + np + p, s);
+  {  // spatial order {intensity, range, refl=0, amb=0}; pair p: target p (or
+     // the shared target 0) against source nt+p
+    const float2* F = c->d_spec;
+    const float2* G = c->d_spec + (int64_t)(2 * nt) * c->H;
+    const int64_t fs = shared ? 0 : 2 * c->H, gs = 2 * c->H;
+    {
+      StageScope t(c, kSpatialCrossZ, s);
+      launch_spatial_cross_z(N, np, channels(F, F + c->H, 4), fs, channels(G, G + c->H, 4), gs,
+                             c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work, s);
+    }
+    {
+      StageScope t(c, kSpatialFftInv, s);
+      launch_spatial_fft_inv(N, np, c->d_work, c->plan_N, c->d_tw_N, s);
+    }
+    {
+      StageScope t(c, kSpatialC2r, s);
+      launch_spatial_c2r(N, np, c->d_work, c->plan_N, c->d_tw_N, d_trans_corr, c->d_partials,
+                         c->d_keys + np, s);
     }
   }
   {
@@ -485,6 +568,34 @@ int phz_register_pairs_device(phz_ctx* c, int n, int shared_target, const float*
   return fetch_results(c, n, results, s);
 }
 
+int phz_register_pairs(phz_ctx* c, int n, int shared_target, const float* const* targets,
+                       const int64_t* Pt, const float* const* sources, const int64_t* Ps,
+                       phz_result* results) {
+  PHZ_CHECK(c && targets && Pt && sources && Ps && results, "null argument");
+  PHZ_CHECK(n >= 1 && n <= c->cfg.max_batch, "pair count exceeds max_batch");
+  hipStream_t s = c->stream;
+  const int nt = shared_target ? 1 : n;
+  for (int t = 0; t < nt; ++t) TRY(load_cloud(c, t, targets[t], Pt[t], hipMemcpyHostToDevice, s));
+  for (int p = 0; p < n; ++p)
+    TRY(load_cloud(c, nt + p, sources[p], Ps[p], hipMemcpyHostToDevice, s));
+  TRY(commit_counts(c, nt + n, s));
+  TRY(run_pairs(c, n, shared_target != 0, s, nullptr, nullptr));
+  return fetch_results(c, n, results, s);
+}
+
+int phz_memory_usage(const phz_ctx* c, size_t* bytes) {
+  PHZ_CHECK(c && bytes, "null argument");
+  *bytes = c->alloc_bytes;
+  return 0;
+}
+
+int phz_estimate_memory(const phz_config* cfg, size_t* bytes) {
+  PHZ_CHECK(cfg && bytes, "null argument");
+  TRY(check_config(*cfg));
+  *bytes = context_bytes(*cfg);
+  return 0;
+}
+
 int phz_register_device(phz_ctx* c, const float* d_target, int64_t Pt, const float* d_source,
                         int64_t Ps, phz_result* result, float* d_rot_corr, float* d_trans_corr,
                         float* d_registered, void* stream) {
@@ -615,7 +726,7 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
     F.ch[k] = c->d_spec + k * H;
     G.ch[k] = c->d_spec + (C + k) * H;
   }
-  launch_spatial_inverse(c->N, F, G, c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work,
+  launch_spatial_inverse(c->N, 1, F, 0, G, 0, c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work,
                          corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys, s);
   // the dense channels were staged in the voxel accumulators: zero them again
   launch_fill_u32((uint32_t*)c->d_sumI, (size_t)2 * C * T, 0u, s);
@@ -669,7 +780,8 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
     F.ch[0] = d_spec;
     G.ch[0] = d_spec + H;
     FusionRuns none{};
-    launch_spatial_inverse(N, F, G, none, false, p, d_tw, d_work, d_corr, d_key + 1, d_key, s);
+    launch_spatial_inverse(N, 1, F, 0, G, 0, none, false, p, d_tw, d_work, d_corr, d_key + 1,
+                           d_key, s);
     if ((e = hipGetLastError()) != hipSuccess) fail("launch", e);
   }
   unsigned long long key = 0;
@@ -746,6 +858,26 @@ int phz_memcpy(phz_ctx* c, void* dst, const void* src, size_t bytes, int kind) {
                               hipMemcpyDeviceToDevice};
   PHZ_HIP(hipStreamSynchronize(c->stream));
   PHZ_HIP(hipMemcpy(dst, src, bytes, k[kind]));
+  return 0;
+}
+
+int phz_copy_kernel(phz_ctx* c, void* dst, const void* src, size_t bytes, int repeats,
+                    float* ms) {
+  PHZ_CHECK(c && dst && src, "null argument");
+  PHZ_CHECK(bytes % 16 == 0 && repeats >= 1, "bytes must be a multiple of 16, repeats >= 1");
+  hipEvent_t a, b;
+  PHZ_HIP(hipEventCreate(&a));
+  PHZ_HIP(hipEventCreate(&b));
+  PHZ_HIP(hipEventRecord(a, c->stream));
+  for (int r = 0; r < repeats; ++r)
+    launch_copy_f4((float4*)dst, (const float4*)src, bytes / 16, c->stream);
+  PHZ_HIP(hipEventRecord(b, c->stream));
+  PHZ_HIP(hipEventSynchronize(b));
+  float t = 0.f;
+  PHZ_HIP(hipEventElapsedTime(&t, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (ms) *ms = t / (float)repeats;
   return 0;
 }
 

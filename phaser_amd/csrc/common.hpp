@@ -104,18 +104,20 @@ struct FftPlan {
 FftPlan make_plan(int L);  // host: factor L into 4,2,3,5 (error if other)
 
 // One Stockham stage of radix R for nb lines of length L stored line-major
-// in `in`, written to `out`.  tw[e] = exp(-2 pi i e / L).  DIR = -1
-// forward (e^{-i}), +1 backward.
+// in `in` (line stride ls >= L; ls = L + 1 keeps transposed accesses free of
+// LDS bank conflicts), written to `out`.  tw[e] = exp(-2 pi i e / L).
+// DIR = -1 forward (e^{-i}), +1 backward.
 template <int R, int DIR>
 __device__ __forceinline__ void fft_stage(const float2* __restrict__ in, float2* __restrict__ out,
-                                          int L, int nb, int Ns, const float2* __restrict__ tw) {
+                                          int L, int ls, int nb, int Ns,
+                                          const float2* __restrict__ tw) {
   const int Lr = L / R;
   const int tstep = L / (Ns * R);
   for (int w = threadIdx.x; w < nb * Lr; w += blockDim.x) {
     const int line = w / Lr;
     const int j = w - line * Lr;
-    const float2* src = in + line * L;
-    float2* dst = out + line * L;
+    const float2* src = in + line * ls;
+    float2* dst = out + line * ls;
     float2 v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = src[j + r * Lr];
@@ -172,22 +174,24 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ in, float2*
   }
 }
 
-// nb FFTs of length plan.L on line-major LDS data in `a` (scratch `b` of the
-// same size).  Returns the buffer holding the result.  Ends with a barrier.
+// nb FFTs of length plan.L on line-major LDS data in `a` (line stride ls,
+// scratch `b` of the same size).  Returns the buffer holding the result.
+// Ends with a barrier.
 template <int DIR>
 __device__ float2* lds_fft(float2* a, float2* b, int nb, const FftPlan& plan,
-                           const float2* __restrict__ tw) {
+                           const float2* __restrict__ tw, int ls = 0) {
+  if (ls == 0) ls = plan.L;
   int Ns = 1;
   for (int s = 0; s < plan.nst; ++s) {
     const int R = plan.radix[s];
     if (R == 4)
-      fft_stage<4, DIR>(a, b, plan.L, nb, Ns, tw);
+      fft_stage<4, DIR>(a, b, plan.L, ls, nb, Ns, tw);
     else if (R == 2)
-      fft_stage<2, DIR>(a, b, plan.L, nb, Ns, tw);
+      fft_stage<2, DIR>(a, b, plan.L, ls, nb, Ns, tw);
     else if (R == 3)
-      fft_stage<3, DIR>(a, b, plan.L, nb, Ns, tw);
+      fft_stage<3, DIR>(a, b, plan.L, ls, nb, Ns, tw);
     else
-      fft_stage<5, DIR>(a, b, plan.L, nb, Ns, tw);
+      fft_stage<5, DIR>(a, b, plan.L, ls, nb, Ns, tw);
     __syncthreads();
     float2* t = a;
     a = b;

@@ -15,6 +15,7 @@ namespace phz {
 // points each, cloud c at pts + c*cap with npts[c] points.  Cells: n bands
 // (theta) x n sectors (phi centred on the query columns).
 struct CellBuffers {
+  int nb, ns;        // colatitude bands x azimuth sectors of the search grid
   int* cell_count;   // nclouds * n*n
   int* cell_start;   // nclouds * (n*n + 1)
   int* band_start;   // nclouds * (n + 1)
@@ -90,12 +91,20 @@ void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose
 // ------------------------------------------------------------- spatial
 // voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
+void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t s);
 void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
                           const DevPose* pose /*nullable: no transform*/, int pose_stride,
                           int N, const float* edges, float* sumI, float* sumR, uint32_t* cnt,
                           hipStream_t s);
 // forward 3-D r2c of (mean intensity, mean range) per cloud:
-// spec[cloud][ch in {I,R}][z][x][k2], k2 < N/2+1
+// spec[cloud][ch in {I,R}][z][x][k2], k2 < N/2+1.  launch_spatial_forward =
+// launch_spatial_r2c (voxel means -> r2c along y) + launch_spatial_fft_fwd
+// (FFT along x, then z, of nspec = 2*nclouds half spectra).
+void launch_spatial_r2c(int N, int nclouds, const float* sumI, const float* sumR,
+                        const uint32_t* cnt, const FftPlan& plan, const float2* tw, float2* spec,
+                        hipStream_t s);
+void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
+                            const float2* tw, hipStream_t s);
 void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
                             const uint32_t* cnt, const FftPlan& plan, const float2* tw,
                             float2* spec, hipStream_t s);
@@ -103,14 +112,27 @@ void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* 
 // nch channels (<= 8) -> spec[c][z][x][k2]
 void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
                                   const float2* tw, float2* spec, hipStream_t s);
-// fused cross spectrum + inverse 3-D c2r + argmax.  F channels and G
-// channels (null = zero channel) in the reference's spatial order.
-// partials: spatial_c2r_blocks(N) entries of scratch
+// fused cross spectrum + inverse 3-D c2r + argmax for npairs pairs.  F
+// channels and G channels (null = zero channel) in the reference's spatial
+// order; pair p reads its channels at ch[c] + p*stride (stride 0: shared).
+// work: npairs * N^2(N/2+1); corr (nullable): npairs * N^3; partials:
+// npairs * spatial_c2r_blocks(N) entries of scratch; keys: npairs.
 int spatial_c2r_blocks(int N);
-void launch_spatial_inverse(int N, ChannelSet F, ChannelSet G, const FusionRuns& runs,
-                            bool fuse, const FftPlan& plan, const float2* tw, float2* work,
+void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                            int64_t g_stride, const FusionRuns& runs, bool fuse,
+                            const FftPlan& plan, const float2* tw, float2* work,
                             float* corr /*nullable*/, unsigned long long* partials,
-                            unsigned long long* key, hipStream_t s);
+                            unsigned long long* keys, hipStream_t s);
+// its three passes: cross spectrum + inverse FFT along z; inverse FFT along
+// x; c2r along y + argmax
+void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                            int64_t g_stride, const FusionRuns& runs, bool fuse,
+                            const FftPlan& plan, const float2* tw, float2* work, hipStream_t s);
+void launch_spatial_fft_inv(int N, int npairs, float2* work, const FftPlan& plan,
+                            const float2* tw, hipStream_t s);
+void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& plan,
+                        const float2* tw, float* corr, unsigned long long* partials,
+                        unsigned long long* keys, hipStream_t s);
 // zero the accumulators of the voxels the points touched (same transform and
 // binning as launch_voxel_scatter), so they need no full clear before reuse
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,

@@ -38,39 +38,40 @@ __device__ __forceinline__ void project_point(float x, float y, float z, float* 
 }
 
 __global__ void k_project(const float4* __restrict__ pts, const int* __restrict__ npts,
-                          int64_t cap, int n, float4* __restrict__ proj, float2* __restrict__ val,
-                          int* __restrict__ cell, int* __restrict__ cell_count) {
+                          int64_t cap, int nb, int ns, float4* __restrict__ proj,
+                          float2* __restrict__ val, int* __restrict__ cell,
+                          int* __restrict__ cell_count) {
   const int c = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npts[c]) return;
   const float4 p = pts[c * cap + i];
   float px, py, pz, r;
   project_point(p.x, p.y, p.z, &px, &py, &pz, &r);
-  // cell of the stored float point: band from its colatitude, sector centred
-  // on the query columns phi_j = 2 pi j / n
+  // cell of the stored float point: nb colatitude bands, ns azimuth sectors
+  // centred on phi_s = 2 pi s / ns
   const float th = atan2f(sqrtf(px * px + py * py), pz);
-  int band = (int)(th * ((float)n / kPi));
-  band = band < 0 ? 0 : (band >= n ? n - 1 : band);
+  int band = (int)(th * ((float)nb / kPi));
+  band = band < 0 ? 0 : (band >= nb ? nb - 1 : band);
   float ph = atan2f(py, px);
   if (ph < 0.f) ph += 2.f * kPi;
-  int sec = (int)floorf(ph * ((float)n / (2.f * kPi)) + 0.5f);
-  sec = sec >= n ? sec - n : (sec < 0 ? 0 : sec);
-  const int cid = band * n + sec;
+  int sec = (int)floorf(ph * ((float)ns / (2.f * kPi)) + 0.5f);
+  sec = sec >= ns ? sec - ns : (sec < 0 ? 0 : sec);
+  const int cid = band * ns + sec;
   proj[c * cap + i] = make_float4(px, py, pz, __int_as_float(i));
   val[c * cap + i] = make_float2(r, p.w);
   cell[c * cap + i] = cid;
-  atomicAdd(&cell_count[(int64_t)c * n * n + cid], 1);
+  atomicAdd(&cell_count[(int64_t)c * nb * ns + cid], 1);
 }
 
 // exclusive scan of the n*n cell counts of a cloud (one 1024-thread
 // workgroup per cloud): per-thread serial sums, wave scans, wave totals.
-__global__ void k_cell_scan(int n, int* __restrict__ cell_count, int* __restrict__ cell_start,
-                            int* __restrict__ band_start) {
+__global__ void k_cell_scan(int nb, int ns, int* __restrict__ cell_count,
+                            int* __restrict__ cell_start, int* __restrict__ band_start) {
   const int c = blockIdx.x;
-  const int ncell = n * n;
+  const int ncell = nb * ns;
   int* cnt = cell_count + (int64_t)c * ncell;
   int* st = cell_start + (int64_t)c * (ncell + 1);
-  int* bs = band_start + (int64_t)c * (n + 1);
+  int* bs = band_start + (int64_t)c * (nb + 1);
   __shared__ int wsum[16];
   const int T = blockDim.x;  // 1024
   const int per = (ncell + T - 1) / T;
@@ -104,10 +105,10 @@ __global__ void k_cell_scan(int n, int* __restrict__ cell_count, int* __restrict
     cnt[k] = 0;  // reused as the scatter cursor
   }
   __syncthreads();
-  for (int bb = threadIdx.x; bb <= n; bb += T) bs[bb] = st[bb * n];
+  for (int bb = threadIdx.x; bb <= nb; bb += T) bs[bb] = st[bb * ns];
 }
 
-__global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int n,
+__global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int ncell,
                                const float4* __restrict__ proj, const float2* __restrict__ val,
                                const int* __restrict__ cell, int* __restrict__ cursor,
                                const int* __restrict__ cell_start, float4* __restrict__ sorted,
@@ -116,8 +117,8 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int n,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npts[c]) return;
   const int cid = cell[c * cap + i];
-  const int pos = atomicAdd(&cursor[(int64_t)c * n * n + cid], 1);
-  const int dst = cell_start[(int64_t)c * (n * n + 1) + cid] + pos;
+  const int pos = atomicAdd(&cursor[(int64_t)c * ncell + cid], 1);
+  const int dst = cell_start[(int64_t)c * (ncell + 1) + cid] + pos;
   sorted[c * cap + dst] = proj[c * cap + i];
   sorted_val[c * cap + dst] = val[c * cap + i];
 }
@@ -125,13 +126,14 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int n,
 // per cell: nearest non-empty sector of the same band to the right (>= s)
 // and to the left (<= s), cyclic; -1 in an empty band.  One wave per band:
 // the occupancy of the band's n <= 256 sectors is a 4 x 64-bit ballot mask.
-__global__ void k_jump_tables(int n, const int* __restrict__ cell_start, int* __restrict__ nr,
-                              int* __restrict__ nl) {
+__global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
+                              int* __restrict__ nr, int* __restrict__ nl) {
+  // n = sectors per band here
   const int b = blockIdx.x, c = blockIdx.y;
   const int lane = threadIdx.x;  // 64 lanes
-  const int* st = cell_start + (int64_t)c * (n * n + 1) + (int64_t)b * n;
-  int* r = nr + (int64_t)c * n * n + (int64_t)b * n;
-  int* l = nl + (int64_t)c * n * n + (int64_t)b * n;
+  const int* st = cell_start + (int64_t)c * (nb * n + 1) + (int64_t)b * n;
+  int* r = nr + (int64_t)c * nb * n + (int64_t)b * n;
+  int* l = nl + (int64_t)c * nb * n + (int64_t)b * n;
   unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
   const int nw = (n + 63) / 64;
   for (int w = 0; w < nw; ++w) {
@@ -185,7 +187,7 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
 // The eight lanes split each cell's points and agree on the best after it.
 constexpr int kNNLanes = 8;
 __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
-                     float max_sq, const int* __restrict__ cell_start,
+                     int nb, int ns, float max_sq, const int* __restrict__ cell_start,
                      const int* __restrict__ band_start, const int* __restrict__ next_right,
                      const int* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
@@ -193,18 +195,23 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   const int c = blockIdx.y;
   const int sub = threadIdx.x & (kNNLanes - 1);
   const int q = blockIdx.x * (blockDim.x / kNNLanes) + threadIdx.x / kNNLanes;
-  const int nq = n * n;
+  const int nq = n * n, ncell = nb * ns;
   if (q >= nq) return;  // whole groups leave together
-  const int qi = q / n, qj = q - qi * n;
+  const int qrow = q / n, qcol = q - qrow * n;
   const float4 qv = grid[q];
-  const float thq = theta[qi];
+  const float thq = theta[qrow];
+  const float phq = (2.f * kPi / n) * qcol;
   const float sq = sinf(thq);
-  const int* st = cell_start + (int64_t)c * (nq + 1);
-  const int* bs = band_start + (int64_t)c * (n + 1);
-  const int* nr = next_right + (int64_t)c * nq;
-  const int* nl = next_left + (int64_t)c * nq;
+  const int* st = cell_start + (int64_t)c * (ncell + 1);
+  const int* bs = band_start + (int64_t)c * (nb + 1);
+  const int* nr = next_right + (int64_t)c * ncell;
+  const int* nl = next_left + (int64_t)c * ncell;
   const float4* sp = sorted + c * cap;
-  const float dband = kPi / n, dsec = 2.f * kPi / n;
+  const float dband = kPi / nb, dsec = 2.f * kPi / ns;
+  int qi = (int)(thq / dband);
+  qi = qi < 0 ? 0 : (qi >= nb ? nb - 1 : qi);
+  int qj = (int)floorf(phq / dsec + 0.5f);
+  qj = qj >= ns ? qj - ns : qj;
   const float eps = 1e-5f, slack = 1e-5f;
   float best = INFINITY;
   int bidx = 0x7fffffff, bpos = -1;
@@ -238,14 +245,14 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const float t = sqrtf(best) + slack;
     return t * t;
   };
-  for (int t = 0; t < 2 * n && (up || down); ++t) {
+  for (int t = 0; t < 2 * nb && (up || down); ++t) {
     int b;
     if (t == 0) {
       b = qi;
     } else if (t & 1) {
       if (!up) continue;
       b = qi + (t + 1) / 2;
-      if (b >= n) {
+      if (b >= nb) {
         up = false;
         continue;
       }
@@ -273,27 +280,30 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     if (bs[b + 1] == bs[b]) continue;
     const float spm = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
     const float Bc = 2.f * sq * spm;
-    scan_cell(b * n + qj);
-    // right: sector distances 1..n/2; left: 1..(n-1)/2 (each sector once)
+    scan_cell(b * ns + qj);
+    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
     for (int side = 0; side < 2; ++side) {
       int u_prev = 0;
       while (true) {
         int sec, u;
         if (side == 0) {
-          sec = nr[b * n + (qj + u_prev + 1) % n];
+          sec = nr[b * ns + (qj + u_prev + 1) % ns];
           if (sec < 0) break;
-          u = (sec - qj + n) % n;
-          if (u <= u_prev || u > n / 2) break;
+          u = (sec - qj + ns) % ns;
+          if (u <= u_prev || u > ns / 2) break;
         } else {
-          sec = nl[b * n + ((qj - u_prev - 1) % n + n) % n];
+          sec = nl[b * ns + ((qj - u_prev - 1) % ns + ns) % ns];
           if (sec < 0) break;
-          u = (qj - sec + n) % n;
-          if (u <= u_prev || u > (n - 1) / 2) break;
+          u = (qj - sec + ns) % ns;
+          if (u <= u_prev || u > (ns - 1) / 2) break;
         }
-        const float dphi = fmaxf((u - 0.5f) * dsec - eps, 0.f);
+        // azimuth distance from the query to the sector's interval
+        float d = fabsf(phq - sec * dsec);
+        if (d > kPi) d = 2.f * kPi - d;
+        const float dphi = fminf(fmaxf(d - 0.5f * dsec - eps, 0.f), 0.5f * kPi);
         const float s2 = sinf(0.5f * dphi);
         if (A + Bc * 2.f * s2 * s2 > thr2()) break;
-        scan_cell(b * n + sec);
+        scan_cell(b * ns + sec);
         u_prev = u;
       }
     }
@@ -416,27 +426,28 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
 
 void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
                      CellBuffers cb, hipStream_t s) {
-  const int ncell = n * n;
+  (void)n;
+  const int ncell = cb.nb * cb.ns;
   launch_fill_u32((uint32_t*)cb.cell_count, (size_t)nclouds * ncell, 0u, s);
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
-  hipLaunchKernelGGL(k_project, g, dim3(256), 0, s, pts, npts, cap, n, cb.proj, cb.val, cb.cell,
-                     cb.cell_count);
-  hipLaunchKernelGGL(k_cell_scan, dim3(nclouds), dim3(1024), 0, s, n, cb.cell_count,
+  hipLaunchKernelGGL(k_project, g, dim3(256), 0, s, pts, npts, cap, cb.nb, cb.ns, cb.proj, cb.val,
+                     cb.cell, cb.cell_count);
+  hipLaunchKernelGGL(k_cell_scan, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns, cb.cell_count,
                      cb.cell_start, cb.band_start);
-  hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, n, cb.proj, cb.val, cb.cell,
-                     cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
-  hipLaunchKernelGGL(k_jump_tables, dim3(n, nclouds), dim3(64), 0, s, n, cb.cell_start,
-                     cb.next_right, cb.next_left);
+  hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj, cb.val,
+                     cb.cell, cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
+  hipLaunchKernelGGL(k_jump_tables, dim3(cb.nb, nclouds), dim3(64), 0, s, cb.nb, cb.ns,
+                     cb.cell_start, cb.next_right, cb.next_left);
 }
 
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, hipStream_t s) {
-  const int ncell = n * n;
+  const int nq = n * n;
   const int qpb = 256 / kNNLanes;
-  dim3 gq((unsigned)((ncell + qpb - 1) / qpb), nclouds);
-  hipLaunchKernelGGL(k_nn, gq, dim3(256), 0, s, grid, theta, n, max_sq, cb.cell_start,
-                     cb.band_start, cb.next_right, cb.next_left, cb.sorted, cb.sorted_val, cap,
-                     grids);
+  dim3 gq((unsigned)((nq + qpb - 1) / qpb), nclouds);
+  hipLaunchKernelGGL(k_nn, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+                     cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
+                     cb.sorted_val, cap, grids);
 }
 
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,

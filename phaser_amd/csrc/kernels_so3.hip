@@ -130,29 +130,30 @@ __global__ void k_so3_synth(int bw, const float2* __restrict__ fc, int64_t f_str
 }
 
 // FFT over m1 (M1 = m1 mod n, Nyquist row zero): U[k][m2][alpha]
+template <int KT>
 __global__ void k_so3_fft_alpha(int bw, const float2* __restrict__ S, FftPlan plan,
                                 const float2* __restrict__ tw, float2* __restrict__ U) {
-  constexpr int KT = 16;
   const int n = 2 * bw, w1 = 2 * bw - 1;
   const int m2 = blockIdx.x, k0 = blockIdx.y * KT, pair = blockIdx.z;
   extern __shared__ float2 sm[];
+  const int ls = n + 1;  // padded line stride for the transposed fill
   float2* A = sm;
-  float2* B = sm + KT * n;
+  float2* B = sm + KT * ls;
   const float2* Sp = S + ((int64_t)pair * bw + m2) * w1 * n;
-  for (int idx = threadIdx.x; idx < KT * n; idx += blockDim.x) A[idx] = make_float2(0.f, 0.f);
+  for (int idx = threadIdx.x; idx < KT * ls; idx += blockDim.x) A[idx] = make_float2(0.f, 0.f);
   __syncthreads();
   for (int idx = threadIdx.x; idx < w1 * KT; idx += blockDim.x) {
     const int mi = idx / KT, kk = idx - mi * KT;
     const int m1 = mi - (bw - 1);
     const int M1 = m1 < 0 ? m1 + n : m1;
-    if (k0 + kk < n) A[kk * n + M1] = Sp[(int64_t)mi * n + k0 + kk];
+    if (k0 + kk < n) A[kk * ls + M1] = Sp[(int64_t)mi * n + k0 + kk];
   }
   __syncthreads();
-  float2* R = lds_fft<-1>(A, B, KT, plan, tw);
+  float2* R = lds_fft<-1>(A, B, KT, plan, tw, ls);
   float2* Up = U + (int64_t)pair * n * bw * n;
   for (int idx = threadIdx.x; idx < KT * n; idx += blockDim.x) {
     const int kk = idx / n, al = idx - kk * n;
-    if (k0 + kk < n) Up[((int64_t)(k0 + kk) * bw + m2) * n + al] = R[idx];
+    if (k0 + kk < n) Up[((int64_t)(k0 + kk) * bw + m2) * n + al] = R[kk * ls + al];
   }
 }
 
@@ -165,19 +166,20 @@ __global__ void k_so3_fft_gamma(int bw, const float2* __restrict__ U, FftPlan pl
   const int n = 2 * bw;
   const int a0 = blockIdx.x * AT, k = blockIdx.y, pair = blockIdx.z;
   extern __shared__ float2 sm[];
-  float2* X = sm;             // AT x bw staging
-  float2* A = sm + AT * bw;   // NL x n
+  const int xs = bw + 1;      // padded staging stride
+  float2* X = sm;             // AT x (bw+1) staging
+  float2* A = sm + AT * xs;   // NL x n
   float2* B = A + NL * n;
   const float2* Up = U + ((int64_t)pair * n + k) * bw * n;
   for (int idx = threadIdx.x; idx < bw * AT; idx += blockDim.x) {
     const int m2 = idx / AT, aa = idx - m2 * AT;
-    X[aa * bw + m2] = (a0 + aa < n) ? Up[(int64_t)m2 * n + a0 + aa] : make_float2(0.f, 0.f);
+    X[aa * xs + m2] = (a0 + aa < n) ? Up[(int64_t)m2 * n + a0 + aa] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < NL * n; idx += blockDim.x) {
     const int p = idx / n, m = idx - p * n;
-    const float2* xa = X + (2 * p) * bw;
-    const float2* xb = X + (2 * p + 1) * bw;
+    const float2* xa = X + (2 * p) * xs;
+    const float2* xb = X + (2 * p + 1) * xs;
     float2 va, vb;
     if (m == 0) {
       va = make_float2(xa[0].x, 0.f);
@@ -300,9 +302,15 @@ void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const floa
 void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
                           const float2* tw, float2* U, hipStream_t s) {
   const int n = 2 * bw;
-  dim3 g(bw, (n + 15) / 16, npairs);
-  hipLaunchKernelGGL(k_so3_fft_alpha, g, dim3(256), 2 * 16 * n * sizeof(float2), s, bw, S, plan,
-                     tw, U);
+  if (n <= 128) {
+    dim3 g(bw, (n + 15) / 16, npairs);
+    hipLaunchKernelGGL(k_so3_fft_alpha<16>, g, dim3(256), 2 * 16 * (n + 1) * sizeof(float2), s,
+                       bw, S, plan, tw, U);
+  } else {
+    dim3 g(bw, (n + 7) / 8, npairs);
+    hipLaunchKernelGGL(k_so3_fft_alpha<8>, g, dim3(256), 2 * 8 * (n + 1) * sizeof(float2), s, bw,
+                       S, plan, tw, U);
+  }
 }
 
 int so3_gamma_blocks(int bw) {
@@ -316,7 +324,7 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
                           hipStream_t s) {
   const int n = 2 * bw;
   dim3 g((n + 15) / 16, n, npairs);
-  const size_t sh = (16 * bw + 2 * 8 * n) * sizeof(float2);
+  const size_t sh = (16 * (bw + 1) + 2 * 8 * n) * sizeof(float2);
   hipLaunchKernelGGL(k_so3_fft_gamma, g, dim3(256), sh, s, bw, U, plan, tw, corr, corr_stride,
                      partials);
   launch_reduce_keys(partials, so3_gamma_blocks(bw), npairs, keys, s);

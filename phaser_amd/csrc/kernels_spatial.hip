@@ -20,6 +20,13 @@ __global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
     p[i] = v;
 }
 
+// streaming 16-byte copy: the HBM reference (bandwidth, traffic counters)
+__global__ void k_copy_f4(float4* __restrict__ dst, const float4* __restrict__ src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // igl::histc bin of x against monotone float edges E[0..N)
 __device__ __forceinline__ int histc_bin(float x, const float* E, int N) {
   if (x < E[0] || x > E[N - 1]) return -1;
@@ -112,6 +119,7 @@ __global__ void k_spat_r2c(int N, const float* __restrict__ a0, const float* __r
         v.x = a0[off];
         v.y = a1 ? a1[off] : 0.f;
       } else {
+        // empty voxels (the vast majority) never touch the sum arrays
         const uint32_t k = cnt[off];
         if (k) {
           v.x = a0[off] / (float)k;
@@ -137,24 +145,24 @@ __global__ void k_spat_r2c(int N, const float* __restrict__ a0, const float* __r
 
 // in-place FFT along the middle axis of [outer][L][inner] arrays; one
 // workgroup per (inner tile, outer, batch array)
-template <int DIR>
+template <int DIR, int IT>
 __global__ void k_fft_cols(float2* __restrict__ data, int64_t batch_stride, int L, int inner,
                            FftPlan plan, const float2* __restrict__ tw) {
-  constexpr int IT = 16;
   const int i0 = blockIdx.x * IT, o = blockIdx.y;
   float2* base = data + blockIdx.z * batch_stride + (int64_t)o * L * inner;
   extern __shared__ float2 sm[];
+  const int ls = L + 1;  // padded line stride: the transposed accesses hit distinct banks
   float2* A = sm;
-  float2* B = sm + IT * L;
+  float2* B = sm + IT * ls;
   for (int idx = threadIdx.x; idx < IT * L; idx += blockDim.x) {
     const int e = idx / IT, ii = idx - e * IT;
-    A[ii * L + e] = (i0 + ii < inner) ? base[(int64_t)e * inner + i0 + ii] : make_float2(0.f, 0.f);
+    A[ii * ls + e] = (i0 + ii < inner) ? base[(int64_t)e * inner + i0 + ii] : make_float2(0.f, 0.f);
   }
   __syncthreads();
-  float2* R = lds_fft<DIR>(A, B, IT, plan, tw);
+  float2* R = lds_fft<DIR>(A, B, IT, plan, tw, ls);
   for (int idx = threadIdx.x; idx < IT * L; idx += blockDim.x) {
     const int e = idx / IT, ii = idx - e * IT;
-    if (i0 + ii < inner) base[(int64_t)e * inner + i0 + ii] = R[ii * L + e];
+    if (i0 + ii < inner) base[(int64_t)e * inner + i0 + ii] = R[ii * ls + e];
   }
 }
 
@@ -184,15 +192,19 @@ __device__ __forceinline__ float2 fuse_at(const ChannelSet& cs, int mode, int64_
 
 // inverse pass A: fused F, G -> C = F conj(G) (spatial-correlation.cc:158-159)
 // -> inverse FFT along z.  work[z][x][k2].
-__global__ void k_spat_cross_z(int N, ChannelSet F, ChannelSet G, FusionRuns runs, int fuse,
-                               FftPlan plan, const float2* __restrict__ tw,
-                               float2* __restrict__ work) {
-  constexpr int IT = 16;
+template <int IT>
+__global__ void k_spat_cross_z(int N, ChannelSet F, int64_t f_stride, ChannelSet G,
+                               int64_t g_stride, FusionRuns runs, int fuse, FftPlan plan,
+                               const float2* __restrict__ tw, float2* __restrict__ work) {
   const int h = N / 2 + 1, inner = N * h;
   const int i0 = blockIdx.x * IT;
+  // pair blockIdx.y: its F and G channels and its work array
+  const int64_t fo = blockIdx.y * f_stride, go = blockIdx.y * g_stride;
+  work += blockIdx.y * (int64_t)N * inner;
   extern __shared__ float2 sm[];
+  const int ls = N + 1;
   float2* A = sm;
-  float2* B = sm + IT * N;
+  float2* B = sm + IT * ls;
   for (int idx = threadIdx.x; idx < IT * N; idx += blockDim.x) {
     const int z = idx / IT, ii = idx - z * IT;
     float2 c = make_float2(0.f, 0.f);
@@ -203,22 +215,22 @@ __global__ void k_spat_cross_z(int N, ChannelSet F, ChannelSet G, FusionRuns run
         int mode = 0;
         for (int r = 0; r < runs.count; ++r)
           if ((uint64_t)q >= runs.start[r]) mode = (int)runs.mode[r];
-        f = fuse_at(F, mode, q);
-        g = fuse_at(G, mode, q);
+        f = fuse_at(F, mode, q + fo);
+        g = fuse_at(G, mode, q + go);
       } else {
-        f = F.ch[0][q];
-        g = G.ch[0][q];
+        f = F.ch[0][q + fo];
+        g = G.ch[0][q + go];
       }
       c.x = f.x * g.x - f.y * (-g.y);
       c.y = f.x * (-g.y) + f.y * g.x;
     }
-    A[ii * N + z] = c;
+    A[ii * ls + z] = c;
   }
   __syncthreads();
-  float2* R = lds_fft<+1>(A, B, IT, plan, tw);
+  float2* R = lds_fft<+1>(A, B, IT, plan, tw, ls);
   for (int idx = threadIdx.x; idx < IT * N; idx += blockDim.x) {
     const int z = idx / IT, ii = idx - z * IT;
-    if (i0 + ii < inner) work[(int64_t)z * inner + i0 + ii] = R[ii * N + z];
+    if (i0 + ii < inner) work[(int64_t)z * inner + i0 + ii] = R[ii * ls + z];
   }
 }
 
@@ -230,6 +242,10 @@ __global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
   constexpr int RT = 16, NL = RT / 2;
   const int h = N / 2 + 1, rows = N * N;
   const int row0 = blockIdx.x * RT;
+  // pair blockIdx.y: its half spectrum, output grid and argmax partials
+  work += blockIdx.y * (int64_t)rows * h;
+  if (corr) corr += blockIdx.y * (int64_t)rows * N;
+  partials += blockIdx.y * (int64_t)gridDim.x;
   extern __shared__ float2 sm[];
   float2* X = sm;  // RT x h
   float2* A = sm + RT * h;
@@ -327,6 +343,13 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
 }
 
+void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t s) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) return;
+  hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+}
+
 void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
                           const DevPose* pose, int pose_stride, int N, const float* edges,
                           float* sumI, float* sumR, uint32_t* cnt, hipStream_t s) {
@@ -350,27 +373,53 @@ void launch_reduce_keys(const unsigned long long* partials, int count, int npair
 
 int spatial_c2r_blocks(int N) { return (N * N + 15) / 16; }
 
-static void fft_cols(float2* data, int nbatch, int64_t batch_stride, int outer, int L, int inner,
-                     int dir, const FftPlan& plan, const float2* tw, hipStream_t s) {
-  dim3 g((inner + 15) / 16, outer, nbatch);
-  const size_t sh = 2 * 16 * L * sizeof(float2);
-  if (dir < 0)
-    hipLaunchKernelGGL(k_fft_cols<-1>, g, dim3(256), sh, s, data, batch_stride, L, inner, plan, tw);
-  else
-    hipLaunchKernelGGL(k_fft_cols<+1>, g, dim3(256), sh, s, data, batch_stride, L, inner, plan, tw);
+// tile width: 16 columns while two padded buffers fit 64 KB of LDS, else 8
+template <int DIR>
+static void fft_cols_t(float2* data, int nbatch, int64_t batch_stride, int outer, int L,
+                       int inner, const FftPlan& plan, const float2* tw, hipStream_t s) {
+  if (L <= 128) {
+    dim3 g((inner + 15) / 16, outer, nbatch);
+    hipLaunchKernelGGL((k_fft_cols<DIR, 16>), g, dim3(256), 2 * 16 * (L + 1) * sizeof(float2), s,
+                       data, batch_stride, L, inner, plan, tw);
+  } else {
+    dim3 g((inner + 7) / 8, outer, nbatch);
+    hipLaunchKernelGGL((k_fft_cols<DIR, 8>), g, dim3(256), 2 * 8 * (L + 1) * sizeof(float2), s,
+                       data, batch_stride, L, inner, plan, tw);
+  }
 }
 
-void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
-                            const uint32_t* cnt, const FftPlan& plan, const float2* tw,
-                            float2* spec, hipStream_t s) {
+static void fft_cols(float2* data, int nbatch, int64_t batch_stride, int outer, int L, int inner,
+                     int dir, const FftPlan& plan, const float2* tw, hipStream_t s) {
+  if (dir < 0)
+    fft_cols_t<-1>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s);
+  else
+    fft_cols_t<+1>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s);
+}
+
+void launch_spatial_r2c(int N, int nclouds, const float* sumI, const float* sumR,
+                        const uint32_t* cnt, const FftPlan& plan, const float2* tw, float2* spec,
+                        hipStream_t s) {
   const int h = N / 2 + 1;
   const int64_t H = (int64_t)N * N * h, T = (int64_t)N * N * N;
   dim3 g((N * N + 15) / 16, nclouds);
   hipLaunchKernelGGL(k_spat_r2c<false>, g, dim3(256), 2 * 16 * N * sizeof(float2), s, N, sumI,
                      sumR, cnt, T, plan, tw, spec, 2 * H, H);
-  // x then z, over all 2*nclouds spectra
-  fft_cols(spec, 2 * nclouds, H, N, N, h, -1, plan, tw, s);
-  fft_cols(spec, 2 * nclouds, H, 1, N, N * h, -1, plan, tw, s);
+}
+
+void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
+                            const float2* tw, hipStream_t s) {
+  const int h = N / 2 + 1;
+  const int64_t H = (int64_t)N * N * h;
+  // x then z, over all nspec half spectra
+  fft_cols(spec, nspec, H, N, N, h, -1, plan, tw, s);
+  fft_cols(spec, nspec, H, 1, N, N * h, -1, plan, tw, s);
+}
+
+void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
+                            const uint32_t* cnt, const FftPlan& plan, const float2* tw,
+                            float2* spec, hipStream_t s) {
+  launch_spatial_r2c(N, nclouds, sumI, sumR, cnt, plan, tw, spec, s);
+  launch_spatial_fft_fwd(N, 2 * nclouds, spec, plan, tw, s);
 }
 
 void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
@@ -389,19 +438,44 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
   fft_cols(spec, nch, H, 1, N, N * h, -1, plan, tw, s);
 }
 
-void launch_spatial_inverse(int N, ChannelSet F, ChannelSet G, const FusionRuns& runs, bool fuse,
-                            const FftPlan& plan, const float2* tw, float2* work, float* corr,
-                            unsigned long long* partials, unsigned long long* key,
-                            hipStream_t s) {
+void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                            int64_t g_stride, const FusionRuns& runs, bool fuse,
+                            const FftPlan& plan, const float2* tw, float2* work, hipStream_t s) {
+  const int inner = N * (N / 2 + 1);
+  if (N <= 128)
+    hipLaunchKernelGGL(k_spat_cross_z<16>, dim3((inner + 15) / 16, npairs), dim3(256),
+                       2 * 16 * (N + 1) * sizeof(float2), s, N, F, f_stride, G, g_stride, runs,
+                       (int)fuse, plan, tw, work);
+  else
+    hipLaunchKernelGGL(k_spat_cross_z<8>, dim3((inner + 7) / 8, npairs), dim3(256),
+                       2 * 8 * (N + 1) * sizeof(float2), s, N, F, f_stride, G, g_stride, runs,
+                       (int)fuse, plan, tw, work);
+}
+
+void launch_spatial_fft_inv(int N, int npairs, float2* work, const FftPlan& plan,
+                            const float2* tw, hipStream_t s) {
   const int h = N / 2 + 1;
-  const int inner = N * h;
-  hipLaunchKernelGGL(k_spat_cross_z, dim3((inner + 15) / 16), dim3(256),
-                     2 * 16 * N * sizeof(float2), s, N, F, G, runs, (int)fuse, plan, tw, work);
-  fft_cols(work, 1, 0, N, N, h, +1, plan, tw, s);
+  fft_cols(work, npairs, (int64_t)N * N * h, N, N, h, +1, plan, tw, s);
+}
+
+void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& plan,
+                        const float2* tw, float* corr, unsigned long long* partials,
+                        unsigned long long* keys, hipStream_t s) {
+  const int h = N / 2 + 1;
   const size_t sh = (16 * h + 2 * 8 * N) * sizeof(float2);
-  hipLaunchKernelGGL(k_spat_c2r, dim3(spatial_c2r_blocks(N)), dim3(256), sh, s, N, work, plan, tw,
-                     corr, partials);
-  launch_reduce_keys(partials, spatial_c2r_blocks(N), 1, key, s);
+  hipLaunchKernelGGL(k_spat_c2r, dim3(spatial_c2r_blocks(N), npairs), dim3(256), sh, s, N, work,
+                     plan, tw, corr, partials);
+  launch_reduce_keys(partials, spatial_c2r_blocks(N), npairs, keys, s);
+}
+
+void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                            int64_t g_stride, const FusionRuns& runs, bool fuse,
+                            const FftPlan& plan, const float2* tw, float2* work, float* corr,
+                            unsigned long long* partials, unsigned long long* keys,
+                            hipStream_t s) {
+  launch_spatial_cross_z(N, npairs, F, f_stride, G, g_stride, runs, fuse, plan, tw, work, s);
+  launch_spatial_fft_inv(N, npairs, work, plan, tw, s);
+  launch_spatial_c2r(N, npairs, work, plan, tw, corr, partials, keys, s);
 }
 
 void launch_trans_pose(int N, int lower, int upper, int npairs, const unsigned long long* keys,

@@ -8,8 +8,11 @@
 // phase-aligner.cc:27-30 (voxel edges, Eigen LinSpaced >= 3.3.4).
 #include "tables.hpp"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
+#include <thread>
 
 namespace phz {
 
@@ -241,16 +244,28 @@ void build_s2_tables(int bw, S2Tables* t) {
       t->wig_off[id + 1] = t->wig_off[id] + (int64_t)n * (bw - b);
     }
   t->wigner.assign((size_t)t->wig_off[nblk], 0.0f);
-  for (int b = 0; b < bw; ++b)
-    for (int a = 0; a <= b; ++a) {
-      const int id = b * (b + 1) / 2 + a;
-      float* dst = &t->wigner[(size_t)t->wig_off[id]];
-      for (int k = 0; k < n; ++k) {
-        const double beta = M_PI * (2.0 * k + 1.0) / (4.0 * bw);
-        wigner_column(bw, a, b, beta, col.data());
-        for (int l = b; l < bw; ++l) dst[(size_t)(l - b) * n + k] = (float)col[l - b];
-      }
-    }
+  {  // blocks are independent: spread the orders b over host threads
+    std::atomic<int> next_b{0};
+    auto work = [&]() {
+      std::vector<double> cb(bw);
+      for (int b; (b = next_b.fetch_add(1)) < bw;)
+        for (int a = 0; a <= b; ++a) {
+          const int id = b * (b + 1) / 2 + a;
+          float* dst = &t->wigner[(size_t)t->wig_off[id]];
+          for (int k = 0; k < n; ++k) {
+            const double beta = M_PI * (2.0 * k + 1.0) / (4.0 * bw);
+            wigner_column(bw, a, b, beta, cb.data());
+            for (int l = b; l < bw; ++l) dst[(size_t)(l - b) * n + k] = (float)cb[l - b];
+          }
+        }
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nth = (int)std::min<unsigned>(std::min(hw, 16u), (unsigned)std::max(1, bw / 8));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nth; ++i) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+  }
   t->wig_norm.resize(bw);
   for (int l = 0; l < bw; ++l) t->wig_norm[l] = (float)(2. * M_PI * std::sqrt(2. / (2. * l + 1.)));
   t->tw_n = twiddles(n);

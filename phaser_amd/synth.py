@@ -73,3 +73,19 @@ def pair(seed=0, **kw):
 CONFIG_B32 = dict(rings=64, azimuths=1024, noise_sigma=0.02)            # ~30k-40k points
 CONFIG_B64 = dict(rings=128, azimuths=1024, noise_sigma=0.02, overlap=0.7)  # ~80k points
 CONFIG_B128 = dict(rings=256, azimuths=1024, noise_sigma=0.02)          # ~200k points
+
+# rings per bandwidth for the mixed batching and serving workloads
+RINGS_FOR_BANDWIDTH = {32: 64, 48: 96, 64: 128, 96: 192, 128: 256}
+
+
+def mixed_config(seed, bandwidth):
+    """Batching/serving pair parameters: overlap U[40,90]%, noise sigma
+    U[1,5] cm and 5% uniform outliers, drawn from the seed."""
+    rng = np.random.default_rng([seed, bandwidth])
+    return dict(rings=RINGS_FOR_BANDWIDTH[bandwidth], azimuths=1024,
+                overlap=float(rng.uniform(0.4, 0.9)), noise_sigma=float(rng.uniform(0.01, 0.05)),
+                outlier_frac=0.05)
+
+
+def mixed_pair(seed, bandwidth):
+    return pair(seed=seed, **mixed_config(seed, bandwidth))

@@ -4,16 +4,17 @@ and the ctypes structures match the C layout."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared(header):
+def _declared(header, prefix="phz_"):
     text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(phz_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -54,3 +55,77 @@ def test_default_config_and_error_path_without_gpu():
     assert b"factor" in capi.lib().phz_last_error()
     bad2 = capi.default_config(spherical_zero_padding=3)
     assert capi.lib().phz_create(C.byref(bad2), C.byref(h)) == -1
+
+
+def test_serve_library_exports_every_declared_symbol():
+    from phaser_amd import serve
+    if not os.path.exists(serve.LIB_PATH):
+        pytest.fail("libphaser_serve.so is not built")
+    L = serve.lib()
+    names = _declared("phaser_serve.h", "phs_")
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in serve.EXPORTS, f"{n} has no ctypes binding"
+
+
+_LAYOUT_C = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "phaser_amd.h"
+#include "phaser_serve.h"
+#define S(t) printf(#t " %zu\n", sizeof(t))
+#define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f))
+int main(void) {
+  S(phz_config); S(phz_result); S(phz_stage_time);
+  S(phs_config); S(phs_request); S(phs_response); S(phs_stats);
+  O(phz_config, max_points); O(phz_config, device); O(phz_result, zyz);
+  O(phz_result, T_rot); O(phz_result, translation);
+  O(phs_config, max_points); O(phs_config, hbm_budget); O(phs_config, batch_window_ms);
+  O(phs_request, target); O(phs_request, Ps); O(phs_request, tag);
+  O(phs_response, result); O(phs_response, latency_ms); O(phs_response, error);
+  O(phs_stats, contexts_live); O(phs_stats, queued);
+  return 0;
+}
+"""
+
+
+def test_ctypes_layouts_match_the_c_headers(tmp_path):
+    """Sizes and field offsets of every ctypes structure against the C
+    compiler's view of include/*.h."""
+    from phaser_amd import capi, serve
+    src = tmp_path / "layout.c"
+    src.write_text(_LAYOUT_C)
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
+                                                       text=True, check=True).stdout.splitlines())
+    types = {"phz_config": capi.Config, "phz_result": capi.Result,
+             "phz_stage_time": capi.StageTime, "phs_config": serve.ServeConfig,
+             "phs_request": serve.Request, "phs_response": serve.Response,
+             "phs_stats": serve.Stats}
+    for key, val in got.items():
+        if "." in key:
+            t, f = key.split(".")
+            assert getattr(types[t], f).offset == int(val), key
+        else:
+            assert C.sizeof(types[key]) == int(val), key
+
+
+def test_memory_estimate_without_gpu():
+    """phz_estimate_memory validates like phz_create and makes no device call."""
+    from phaser_amd import capi
+    est = {}
+    for B, N, P in ((32, 64, 1), (64, 128, 1), (64, 128, 16), (128, 256, 1)):
+        cfg = capi.default_config(bandwidth=B, n_voxels=N, max_batch=P)
+        b = C.c_size_t()
+        assert capi.lib().phz_estimate_memory(C.byref(cfg), C.byref(b)) == 0
+        est[(B, N, P)] = b.value
+    # at least the voxel accumulators and spectra of the two clouds
+    assert est[(64, 128, 1)] > 16 * 128 ** 3 * 4
+    assert est[(32, 64, 1)] < est[(64, 128, 1)] < est[(64, 128, 16)]
+    assert est[(64, 128, 1)] < est[(128, 256, 1)]
+    bad = capi.default_config(bandwidth=7)
+    b = C.c_size_t()
+    assert capi.lib().phz_estimate_memory(C.byref(bad), C.byref(b)) == -1

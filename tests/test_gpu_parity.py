@@ -171,6 +171,26 @@ def test_register_b64_matches_oracle(ctx64, po, seed):
         assert _argmax_ok(r["trans_index"], o["trans_corr"])
 
 
+@pytest.mark.parametrize("B,N", [(48, 96), (96, 192)])
+def test_register_mixed_sizes_match_oracle(po, B, N):
+    """Radix-3 sizes of the batching/serving mix (2B = 96, 192; N = 96, 192)
+    on pairs with partial overlap, 1-5 cm noise and 5% outliers."""
+    from phaser_amd.capi import Context
+    from phaser_amd import synth
+    src, tgt, _, _ = synth.mixed_pair(0, B)
+    with Context(bandwidth=B, n_voxels=N) as ctx:
+        r = ctx.register(tgt, src, grids=True)
+    o = po.register(tgt, src, bw=B, n_voxels=N, want_grids=True)
+    assert _argmax_ok(r["rot_index"], o["rot_corr"])
+    assert _rel(r["rot_corr"], o["rot_corr"]) < 2e-3
+    assert _rel(r["trans_corr"], o["trans_corr"]) < 2e-3
+    if r["rot_index"] == o["rot_index"]:
+        _check_pose(r, o, N)
+        assert _argmax_ok(r["trans_index"], o["trans_corr"])
+        if r["trans_index"] == o["trans_index"]:
+            np.testing.assert_allclose(r["translation"], o["translation"], atol=1e-9)
+
+
 def test_registered_cloud_applies_pose(ctx32):
     src, tgt, _, _ = _pair(7, dict(rings=64, azimuths=1024))
     r = ctx32.register(tgt, src, registered=True)
