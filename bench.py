@@ -34,7 +34,7 @@ B, N = 64, 128
 WORKLOAD = dict(rings=128, azimuths=1024, noise_sigma=0.02, overlap=0.7)
 
 
-def kernel_bytes(B, N, P_mean, pairs, shared=False):
+def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False):
     """Algorithmic (compulsory) HBM bytes of ONE launch of the kernel that
     dominates each stage, for a batch of `pairs` registrations: inputs read
     once, outputs written once, FP32 / complex64 elements, C = 2 populated
@@ -55,8 +55,10 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False):
         "so3_fft_gamma": ("k_so3_fft_gamma", pairs * (8 * n * B * n + 4 * n ** 3)),
         # points in; three FP32 accumulators read-modify-written per point
         "voxel_scatter": ("k_voxel_scatter<false>", clouds * (16 + 24) * P_mean),
-        # three N^3 accumulators in, two half spectra out
-        "spatial_r2c": ("k_spat_r2c", clouds * (12 * N ** 3 + 2 * 8 * H)),
+        # three N^3 accumulators in, two half spectra out (the plane form also
+        # transforms along x on the way)
+        "spatial_r2c": ("k_spat_plane_fwd" if plane else "k_spat_r2c",
+                        clouds * (12 * N ** 3 + 2 * 8 * H)),
         # one FFT pass (x or z) over every half spectrum: read + write
         "spatial_fft_fwd": ("k_fft_cols<-1>", 2 * clouds * 16 * H),
         # C target + C source spectra in, cross spectrum out
@@ -65,6 +67,8 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False):
         "spatial_c2r": ("k_spat_c2r", pairs * (8 * H + 4 * N ** 3)),
     }
 
+
+FORM_NAMES = {0: "plain fp32", 1: "split fp16 mfma contractions"}
 
 # launches of the stage's dominant kernel per pipeline pass
 LAUNCHES = {"spatial_fft_fwd": 2, "voxel_scatter": 2}
@@ -100,7 +104,8 @@ def serve_bench(args):
     sweep = []
     with serve.Server(max_batch=args.serve_batch, contexts_per_shape=2,
                       hbm_budget=int(args.hbm_budget_gib * (1 << 30)),
-                      max_points=max_pts) as srv:
+                      max_points=max_pts, autotune=bool(args.autotune),
+                      precision=args.precision) as srv:
         for sh in SERVE_SHAPES:  # build the contexts before timing
             src, tgt, _, _ = pool[sh][0]
             srv.wait(srv.submit(tgt, src, bandwidth=sh[0], n_voxels=sh[1]))
@@ -144,7 +149,7 @@ def serve_bench(args):
         "config": {"workload": "serving mix: B 32/64/96/128 with 64^3/128^3/192^3/256^3 "
                                "voxels, overlap 40-90%, noise 1-5 cm, 5% outliers",
                    "max_batch": args.serve_batch, "contexts_per_shape": 2,
-                   "hbm_budget_gib": args.hbm_budget_gib, "form": "plain fp32"},
+                   "hbm_budget_gib": args.hbm_budget_gib, "form": FORM_NAMES[args.precision]},
         "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "sweep": sweep,
         "server": {k: int(v) for k, v in st.items()},
     }
@@ -161,6 +166,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--mode", choices=["batch", "serve"], default="batch")
+    ap.add_argument("--autotune", type=int, default=1, help="pick FFT column tiles first")
+    ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
+                    help="contractions: 0 fp32 (plain form), 1 split fp16 on MFMA")
+    ap.add_argument("--compare-forms", type=int, default=1,
+                    help="also time the other precision form on the same pairs")
     ap.add_argument("--rates", default="100,200,400,800", help="serve: offered requests/s")
     ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds per rate")
     ap.add_argument("--pool", type=int, default=6, help="serve: pairs per bandwidth")
@@ -183,7 +193,9 @@ def main():
     pairs = [synth.pair(seed=s, **WORKLOAD) for s in range(P)]
     max_pts = max(max(len(s), len(t)) for s, t, _, _ in pairs)
     P_mean = float(np.mean([len(s) + len(t) for s, t, _, _ in pairs]) / 2)
-    ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P)
+    ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P,
+                  precision=args.precision)
+    tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
     d_src = [ctx.device_array(s) for s, _, _, _ in pairs]
     d_tgt = [ctx.device_array(t) for _, t, _, _ in pairs]
     Ps = [len(s) for s, _, _, _ in pairs]
@@ -222,13 +234,49 @@ def main():
         step()
     st = ctx.stage_times()
     ctx.set_profiling(False)
-    kb = kernel_bytes(B, N, P_mean, P)
+    plane = tuning["plane_fwd"] == 1
+    kb = kernel_bytes(B, N, P_mean, P, plane=plane)
     dom = max((k for k in kb if st.get(k, (0, 0))[1] > 0), key=lambda k: st[k][0])
-    launches = st[dom][1] * LAUNCHES.get(dom, 1)
+    per_call = dict(LAUNCHES, spatial_fft_fwd=1 if plane else 2)
+    launches = st[dom][1] * per_call.get(dom, 1)
     dom_ms = st[dom][0] / launches
     kname, kbytes = kb[dom]
     achieved = kbytes / (dom_ms * 1e-3) / 1e9
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in st.items()}
+
+    # the other precision form on the same pairs (its own context)
+    other = None
+    if args.compare_forms:
+        ctx2 = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P,
+                       precision=1 - args.precision)
+        if args.autotune:
+            ctx2.autotune(repeats=5)
+        d_src2 = [ctx2.device_array(s) for s, _, _, _ in pairs]
+        d_tgt2 = [ctx2.device_array(t) for _, t, _, _ in pairs]
+
+        def step2():
+            return ctx2.register_pairs_device([a.ptr for a in d_tgt2], Pt, [a.ptr for a in d_src2],
+                                              Ps)
+
+        for _ in range(args.warmup):
+            step2()
+        ctx2.synchronize()
+        ta = time.perf_counter()
+        for _ in range(args.steps):
+            res2 = step2()
+        ctx2.synchronize()
+        el2 = time.perf_counter() - ta
+        ctx2.set_profiling(True)
+        step2()
+        st2 = ctx2.stage_times()
+        ctx2.set_profiling(False)
+        other = {"precision": 1 - args.precision, "value": P * args.steps / el2,
+                 "ms_per_step": el2 / args.steps * 1e3, "res": res2,
+                 "stage_ms": {k: round(st2[k][0] / max(st2[k][1], 1), 4)
+                              for k in ("s2_transform", "so3_synth")}}
+        for a in d_src2 + d_tgt2:
+            a.free()
+        ctx2.close()
 
     # HBM reference: the streaming copy kernel on 1 GiB
     nb = 1 << 30
@@ -263,6 +311,10 @@ def main():
         for k, o in enumerate(outs):
             match += int(o["rot_index"] == res[k]["rot_index"]
                          and o["trans_index"] == res[k]["trans_index"])
+        if other is not None:
+            other["match"] = sum(int(o["rot_index"] == other["res"][k]["rot_index"]
+                                     and o["trans_index"] == other["res"][k]["trans_index"])
+                                 for k, o in enumerate(outs))
         assert o0["rot_index"] == outs[0]["rot_index"]
 
     out = {
@@ -276,16 +328,26 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp32" if args.precision == 0 else "fp32 (contractions split fp16)",
         "data": "synthetic (seeded ray-cast scenes, seeds 0..batch-1)",
         "config": {"workload": "B=64 S2 grid 128x128, SO(3) 128^3, voxels 128^3 over +-50 m, "
                                "128x1024-ray scans, target 70% overlap, noise 2 cm",
                    "pairs_per_step": P, "mean_points_per_cloud": round(P_mean),
-                   "form": "plain fp32"},
+                   "form": FORM_NAMES[args.precision], "fft_column_tiles": tuning},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "argmax_match": {"matched": match, "pairs": S, "rate": round(match / max(S, 1), 4)},
         "stage_ms_per_step": stage_ms,
+        "forms": None if other is None else {
+            FORM_NAMES[args.precision]: {"value": round(value, 2),
+                                         "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+                                         "argmax_match": round(match / max(S, 1), 4),
+                                         "stage_ms": {k: stage_ms.get(k) for k in
+                                                      ("s2_transform", "so3_synth")}},
+            FORM_NAMES[other["precision"]]: {
+                "value": round(other["value"], 2), "ms_per_step": round(other["ms_per_step"], 4),
+                "argmax_match": round(other.get("match", 0) / max(S, 1), 4),
+                "stage_ms": other["stage_ms"]}},
         "roofline": {"bound": "hbm", "kernel": kname, "stage": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),

@@ -40,6 +40,9 @@ typedef struct {
   int64_t max_points;             /* capacity per cloud */
   int32_t max_batch;              /* pairs per batched call */
   int32_t device;                 /* HIP device ordinal */
+  int32_t precision;              /* Legendre and Wigner-d contractions: 0 = FP32,
+                                     1 = split FP16 (hi+lo) on MFMA, FP32 accumulation */
+  int32_t reserved;
 } phz_config;
 
 /* The pose parts of model::RegistrationResult (registration-result.cc:13-99):
@@ -135,6 +138,28 @@ int phz_spatial_correlate(phz_ctx* ctx, int C, const float* f, const float* g, f
  * (any size whose factors are 2, 3 and 5, <= 256). */
 int phz_spatial_correlate_single(phz_ctx* ctx, int n_voxels, const float* f, const float* g,
                                  float* corr, uint32_t* argmax);
+
+/* ---- tuning -------------------------------------------------------------
+ * No counterpart in the reference (FFTW plans are its only tuning,
+ * spatial-correlation.cc:52-60).  Kernel shape of the three column-FFT
+ * passes of the spatial correlation (forward along x, forward along z,
+ * inverse along x): 8, 16, 32 or 64 columns per workgroup of the LDS
+ * Stockham kernel (limited by LDS; the tile never changes a line's
+ * arithmetic, so these give bit-identical results), or 1 for the register
+ * four-step kernel (n = 64, 128, 256; same transform, other rounding). */
+typedef struct {
+  int32_t fwd_x, fwd_z, inv_x;   /* 0 = default */
+  int32_t plane_fwd;   /* first forward pass on whole z planes in LDS (N <= 128):
+                          1 = on, 0 or -1 = off (default) */
+  float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
+  float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
+} phz_tuning;
+/* Times every tile of each pass, and the plane pass against the r2c + x
+ * passes it replaces, on the context's buffers at max_batch (repeats
+ * launches each, after one warm-up round); keeps the fastest. */
+int phz_autotune(phz_ctx* ctx, int repeats, phz_tuning* out);
+int phz_get_tuning(const phz_ctx* ctx, phz_tuning* out);
+int phz_set_tuning(phz_ctx* ctx, const phz_tuning* tuning);
 
 /* ---- statistics --------------------------------------------------------- */
 /* Per-stage device time measured with HIP events on the context's stream,

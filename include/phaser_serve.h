@@ -44,11 +44,13 @@ typedef struct {
   int32_t contexts_per_shape;  /* >= 1 */
   int32_t discretize_lower;    /* voxel grid bounds of every shape */
   int32_t discretize_upper;
-  int32_t reserved;
+  int32_t autotune;            /* nonzero: phz_autotune each new context */
   int64_t max_points;          /* per cloud */
   uint64_t hbm_budget;         /* device bytes all contexts together may hold */
   double batch_window_ms;      /* a free worker with fewer than max_batch
                                   requests waits this long for more (0: none) */
+  int32_t precision;           /* phz_config.precision of every context */
+  int32_t reserved;
 } phs_config;
 
 typedef struct {

@@ -181,8 +181,10 @@ typedef struct {
   int n_nodes, cap;
 } kdtree;
 
-static int kd_cmp_dim;
-static const float* kd_cmp_pts;
+/* qsort comparator context: thread-local, so that registrations may run
+ * concurrently on several host threads */
+static _Thread_local int kd_cmp_dim;
+static _Thread_local const float* kd_cmp_pts;
 static int kd_cmp(const void* a, const void* b) {
   const float va = kd_cmp_pts[3 * *(const int32_t*)a + kd_cmp_dim];
   const float vb = kd_cmp_pts[3 * *(const int32_t*)b + kd_cmp_dim];

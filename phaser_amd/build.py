@@ -23,7 +23,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PHZ_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels_s2.hip", "kernels_so3.hip", "kernels_spatial.hip", "capi.hip"]
+HIP_SOURCES = ["kernels_s2.hip", "kernels_so3.hip", "kernels_spatial.hip", "kernels_mfma.hip",
+               "capi.hip"]
 CPP_SOURCES = ["tables.cpp"]
 HOST_SOURCES = ["host/phaser_host.cc"]
 

@@ -18,7 +18,8 @@ class Config(C.Structure):
                 ("n_voxels", C.c_int32), ("discretize_lower", C.c_int32),
                 ("discretize_upper", C.c_int32), ("spatial_zero_padding", C.c_int32),
                 ("low_pass_lower", C.c_int32), ("low_pass_upper", C.c_int32),
-                ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32)]
+                ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32),
+                ("precision", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -68,6 +69,9 @@ EXPORTS = {
                                         C.POINTER(C.c_uint32)]),
     "phz_spatial_correlate_single": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.POINTER(C.c_uint32)]),
+    "phz_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "phz_get_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phz_set_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "phz_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "phz_get_stage_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "phz_device_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -77,6 +81,16 @@ EXPORTS = {
     "phz_copy_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int,
                                   C.POINTER(C.c_float)]),
 }
+
+
+class Tuning(C.Structure):
+    _fields_ = [("fwd_x", C.c_int32), ("fwd_z", C.c_int32), ("inv_x", C.c_int32),
+                ("plane_fwd", C.c_int32), ("ms_fwd_x", C.c_float), ("ms_fwd_z", C.c_float),
+                ("ms_inv_x", C.c_float), ("ms_plane", C.c_float)]
+
+    def as_dict(self):
+        return {k: (round(getattr(self, k), 4) if k.startswith("ms") else getattr(self, k))
+                for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class StageTime(C.Structure):
@@ -258,6 +272,20 @@ class Context:
         _check(lib().phz_copy_kernel(self.h, C.c_void_p(d_dst), C.c_void_p(d_src), nbytes,
                                      repeats, C.byref(ms)))
         return ms.value
+
+    def autotune(self, repeats=5):
+        t = Tuning()
+        _check(lib().phz_autotune(self.h, repeats, C.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib().phz_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, fwd_x=0, fwd_z=0, inv_x=0, plane_fwd=0):
+        t = Tuning(fwd_x, fwd_z, inv_x, plane_fwd)
+        _check(lib().phz_set_tuning(self.h, C.byref(t)))
 
     def set_profiling(self, on=True):
         _check(lib().phz_set_profiling(self.h, int(on)))

@@ -46,6 +46,9 @@ struct phz_ctx {
   S2Tables s2;
   VoxelTables vox;
   FftPlan plan_n{}, plan_N{};
+  SpatialTiles tiles{};
+  bool plane_fwd = false;  // whole-plane first forward pass
+  phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
   float4* d_grid = nullptr;
@@ -231,6 +234,7 @@ int check_config(const phz_config& cf) {
   PHZ_CHECK(cf.discretize_lower < cf.discretize_upper, "discretize bounds must be increasing");
   PHZ_CHECK(cf.max_points >= 1 && cf.max_points <= (1ll << 26), "max_points out of range");
   PHZ_CHECK(cf.max_batch >= 1 && cf.max_batch <= 1024, "max_batch out of range");
+  PHZ_CHECK(cf.precision == 0 || cf.precision == 1, "precision must be 0 (fp32) or 1 (split fp16)");
   PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
   PHZ_CHECK(make_plan(cf.n_voxels).nst > 0, "n_voxels must factor into 2, 3 and 5");
   return 0;
@@ -326,6 +330,9 @@ int init_ctx(phz_ctx* c) {
   launch_fill_u32(c->d_cnt, (size_t)M * c->T3, 0u, c->stream);
   PHZ_HIP(hipStreamSynchronize(c->stream));
   PHZ_CHECK(c->alloc_bytes == context_bytes(cf), "internal error: device memory estimate is stale");
+  c->tuning.fwd_x = c->tuning.fwd_z = c->tuning.inv_x = fft_cols_default_tile(c->N);
+  c->plane_fwd = false;  // measured slower at N = 128 (one workgroup per CU)
+  c->tuning.plane_fwd = -1;
   return 0;
 }
 
@@ -356,7 +363,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kS2Transform, s);
-    launch_fst(c->d_grids, 2 * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+    launch_fst(c->d_grids, 2 * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef,
+               c->cfg.precision, s);
   }
   {  // spherical order {range, intensity, refl=0, amb=0}, one launch for all clouds
     StageScope t(c, kFuse, s);
@@ -366,7 +374,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     StageScope t(c, kSo3Synth, s);
     launch_so3_synth(B, c->d_fused, shared ? 0 : BB, c->d_fused + nt * BB, BB, np, c->d_wig,
-                     c->d_wig_off, c->d_wig_norm, c->d_S, s);
+                     c->d_wig_off, c->d_wig_norm, c->d_S, c->cfg.precision, s);
   }
   {
     StageScope t(c, kSo3Alpha, s);
@@ -390,13 +398,20 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
                          c->d_cnt + nt * c->T3, s);
   }
-  {
+  {  // y (and with the plane pass also x) first, then the remaining columns
     StageScope t(c, kSpatialR2c, s);
-    launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N, c->d_spec, s);
+    if (c->plane_fwd)
+      launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
+                               c->d_spec, s);
+    else
+      launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
+                         c->d_spec, s);
   }
   {
     StageScope t(c, kSpatialFftFwd, s);
-    launch_spatial_fft_fwd(N, 2 * ncl, c->d_spec, c->plan_N, c->d_tw_N, s);
+    if (!c->plane_fwd)
+      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
+    launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
   }
   {
     StageScope t(c, kVoxelClear, s);
@@ -418,7 +433,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     }
     {
       StageScope t(c, kSpatialFftInv, s);
-      launch_spatial_fft_inv(N, np, c->d_work, c->plan_N, c->d_tw_N, s);
+      launch_spatial_fft_inv(N, np, c->d_work, c->plan_N, c->d_tw_N, c->tiles, s);
     }
     {
       StageScope t(c, kSpatialC2r, s);
@@ -631,7 +646,8 @@ int phz_s2_transform(phz_ctx* c, int nsig, const float* grids, float* coef) {
   hipStream_t s = c->stream;
   const int64_t nq = (int64_t)c->n * c->n, BB = (int64_t)c->B * c->B;
   PHZ_HIP(hipMemcpyAsync(c->d_grids, grids, sizeof(float) * nsig * nq, hipMemcpyHostToDevice, s));
-  launch_fst(c->d_grids, nsig, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+  launch_fst(c->d_grids, nsig, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef,
+             c->cfg.precision, s);
   PHZ_HIP(hipGetLastError());
   PHZ_HIP(hipMemcpyAsync(coef, c->d_coef, sizeof(float2) * nsig * BB, hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipStreamSynchronize(s));
@@ -641,7 +657,8 @@ int phz_s2_transform(phz_ctx* c, int nsig, const float* grids, float* coef) {
 static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, float* corr,
                           float* synthesis, uint32_t* argmax, hipStream_t s) {
   const int B = c->B, n = c->n;
-  launch_so3_synth(B, d_f, 0, d_h, 0, 1, c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S, s);
+  launch_so3_synth(B, d_f, 0, d_h, 0, 1, c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S,
+                   c->cfg.precision, s);
   launch_so3_fft_alpha(B, 1, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
   launch_so3_fft_gamma(B, 1, c->d_U, c->plan_n, c->d_tw_n, corr ? c->d_corr_rot : nullptr,
                        (int64_t)n * n * n, c->d_partials, c->d_keys, s);
@@ -677,7 +694,8 @@ int phz_spherical_correlate(phz_ctx* c, int C, const float* f_grids, const float
   PHZ_HIP(hipMemcpyAsync(c->d_grids, f_grids, sizeof(float) * C * nq, hipMemcpyHostToDevice, s));
   PHZ_HIP(hipMemcpyAsync(c->d_grids + C * nq, h_grids, sizeof(float) * C * nq,
                          hipMemcpyHostToDevice, s));
-  launch_fst(c->d_grids, 2 * C, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef, s);
+  launch_fst(c->d_grids, 2 * C, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef,
+             c->cfg.precision, s);
   for (int w = 0; w < 2; ++w) {
     ChannelSet cs{};
     cs.C = C;
@@ -719,7 +737,8 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
   const int64_t T = c->T3, H = c->H;
   PHZ_HIP(hipMemcpyAsync(c->d_sumI, f, sizeof(float) * C * T, hipMemcpyHostToDevice, s));
   PHZ_HIP(hipMemcpyAsync(c->d_sumI + C * T, g, sizeof(float) * C * T, hipMemcpyHostToDevice, s));
-  launch_spatial_forward_dense(c->N, 2 * C, c->d_sumI, c->plan_N, c->d_tw_N, c->d_spec, s);
+  launch_spatial_forward_dense(c->N, 2 * C, c->d_sumI, c->plan_N, c->d_tw_N, c->d_spec, c->tiles,
+                               s);
   ChannelSet F{}, G{};
   F.C = G.C = C;
   for (int k = 0; k < C; ++k) {
@@ -727,7 +746,7 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
     G.ch[k] = c->d_spec + (C + k) * H;
   }
   launch_spatial_inverse(c->N, 1, F, 0, G, 0, c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work,
-                         corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys, s);
+                         corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys, c->tiles, s);
   // the dense channels were staged in the voxel accumulators: zero them again
   launch_fill_u32((uint32_t*)c->d_sumI, (size_t)2 * C * T, 0u, s);
   PHZ_HIP(hipGetLastError());
@@ -774,14 +793,14 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
       p.L = 1;
       p.nst = 0;
     }
-    launch_spatial_forward_dense(N, 2, d_in, p, d_tw, d_spec, s);
+    launch_spatial_forward_dense(N, 2, d_in, p, d_tw, d_spec, SpatialTiles{}, s);
     ChannelSet F{}, G{};
     F.C = G.C = 1;
     F.ch[0] = d_spec;
     G.ch[0] = d_spec + H;
     FusionRuns none{};
     launch_spatial_inverse(N, 1, F, 0, G, 0, none, false, p, d_tw, d_work, d_corr, d_key + 1,
-                           d_key, s);
+                           d_key, SpatialTiles{}, s);
     if ((e = hipGetLastError()) != hipSuccess) fail("launch", e);
   }
   unsigned long long key = 0;
@@ -858,6 +877,110 @@ int phz_memcpy(phz_ctx* c, void* dst, const void* src, size_t bytes, int kind) {
                               hipMemcpyDeviceToDevice};
   PHZ_HIP(hipStreamSynchronize(c->stream));
   PHZ_HIP(hipMemcpy(dst, src, bytes, k[kind]));
+  return 0;
+}
+
+static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
+  c->tuning = t;
+  c->tiles.fwd_x = fft_cols_tile_ok(c->N, t.fwd_x) ? t.fwd_x : 0;
+  c->tiles.fwd_z = fft_cols_tile_ok(c->N, t.fwd_z) ? t.fwd_z : 0;
+  c->tiles.inv_x = fft_cols_tile_ok(c->N, t.inv_x) ? t.inv_x : 0;
+  c->tuning.fwd_x = c->tiles.fwd_x ? c->tiles.fwd_x : fft_cols_default_tile(c->N);
+  c->tuning.fwd_z = c->tiles.fwd_z ? c->tiles.fwd_z : fft_cols_default_tile(c->N);
+  c->tuning.inv_x = c->tiles.inv_x ? c->tiles.inv_x : fft_cols_default_tile(c->N);
+  c->plane_fwd = t.plane_fwd > 0 && spatial_plane_ok(c->N);
+  c->tuning.plane_fwd = c->plane_fwd ? 1 : -1;
+}
+
+int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {
+  PHZ_CHECK(c && t, "null argument");
+  apply_tuning(c, *t);
+  return 0;
+}
+
+int phz_get_tuning(const phz_ctx* c, phz_tuning* t) {
+  PHZ_CHECK(c && t, "null argument");
+  *t = c->tuning;
+  return 0;
+}
+
+int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
+  PHZ_CHECK(c, "null argument");
+  PHZ_CHECK(repeats >= 1 && repeats <= 1000, "repeats out of range");
+  hipStream_t s = c->stream;
+  const int N = c->N, P = c->cfg.max_batch, nspec = 2 * c->max_clouds;
+  // timing runs on the context's own spectra buffers at full batch size
+  launch_fill_u32((uint32_t*)c->d_spec, (size_t)2 * nspec * c->H, 0u, s);
+  launch_fill_u32((uint32_t*)c->d_work, (size_t)2 * P * c->H, 0u, s);
+  hipEvent_t a, b;
+  PHZ_HIP(hipEventCreate(&a));
+  PHZ_HIP(hipEventCreate(&b));
+  const int cands[5] = {kRegisterEngine, 8, 16, 32, 64};
+  phz_tuning best{};
+  float best_ms[3] = {1e30f, 1e30f, 1e30f};
+  int rc = 0;
+  for (int site = 0; site < 3 && !rc; ++site)
+    for (int it : cands) {
+      if (!fft_cols_tile_ok(N, it)) continue;
+      float ms = 0.f;
+      for (int pass = 0; pass < 2; ++pass) {  // pass 0 warms up
+        if (hipEventRecord(a, s) != hipSuccess) rc = -2;
+        for (int r = 0; r < repeats; ++r)
+          launch_spatial_fft_pass(N, site == 2 ? P : nspec, site == 2 ? c->d_work : c->d_spec,
+                                  site, it, c->plan_N, c->d_tw_N, s);
+        if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess)
+          rc = -2;
+      }
+      ms /= (float)repeats;
+      if (ms < best_ms[site]) {
+        best_ms[site] = ms;
+        (site == 0 ? best.fwd_x : site == 1 ? best.fwd_z : best.inv_x) = it;
+      }
+    }
+  // the plane pass against the r2c + x passes it replaces (best x tile)
+  float plane_ms = 0.f, classic_ms = 0.f;
+  if (!rc && spatial_plane_ok(N)) {
+    const int ncl = c->max_clouds;
+    for (int variant = 0; variant < 2 && !rc; ++variant) {
+      float ms = 0.f;
+      for (int pass = 0; pass < 2; ++pass) {
+        if (hipEventRecord(a, s) != hipSuccess) rc = -2;
+        for (int r = 0; r < repeats; ++r) {
+          if (variant == 0) {
+            launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N,
+                                     c->d_tw_N, c->d_spec, s);
+          } else {
+            launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
+                               c->d_spec, s);
+            launch_spatial_fft_pass(N, nspec, c->d_spec, 0, best.fwd_x, c->plan_N, c->d_tw_N, s);
+          }
+        }
+        if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess)
+          rc = -2;
+      }
+      (variant == 0 ? plane_ms : classic_ms) = ms / (float)repeats;
+    }
+    best.plane_fwd = plane_ms <= classic_ms ? 1 : -1;
+  } else {
+    best.plane_fwd = -1;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc) {
+    set_error("autotune timing failed");
+    return rc;
+  }
+  best.ms_fwd_x = best_ms[0];
+  best.ms_fwd_z = best_ms[1];
+  best.ms_inv_x = best_ms[2];
+  apply_tuning(c, best);
+  c->tuning.ms_fwd_x = best_ms[0];
+  c->tuning.ms_fwd_z = best_ms[1];
+  c->tuning.ms_inv_x = best_ms[2];
+  c->tuning.ms_plane = best.plane_fwd > 0 ? plane_ms : classic_ms;
+  if (out) *out = c->tuning;
   return 0;
 }
 

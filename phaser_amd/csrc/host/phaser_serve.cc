@@ -99,6 +99,7 @@ phz_config shape_config(const phs_config& sc, int B, int N) {
   c.max_points = sc.max_points;
   c.max_batch = sc.max_batch;
   c.device = sc.device;
+  c.precision = sc.precision;
   return c;
 }
 
@@ -226,7 +227,14 @@ void worker(phs_server* s, Lane* L) {
   Shape* sh = L->shape;
   const phz_config cfg = shape_config(s->cfg, sh->B, sh->N);
   phz_ctx* ctx = nullptr;
-  const int crc = phz_create(&cfg, &ctx);
+  int crc = phz_create(&cfg, &ctx);
+  if (!crc && s->cfg.autotune) {
+    crc = phz_autotune(ctx, 3, nullptr);
+    if (crc) {
+      phz_destroy(ctx);
+      ctx = nullptr;
+    }
+  }
   const std::string cerr = crc ? std::string(phz_last_error()) : std::string();
   std::unique_lock<std::mutex> lk(s->mu);
   if (crc) {

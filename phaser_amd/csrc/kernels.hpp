@@ -40,8 +40,11 @@ void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const flo
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
 // complex (seanindex order).
+// precision 0: FP32 Legendre contraction; 1: split-FP16 MFMA (kernels_mfma.hip)
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,
-                const float2* tw_n, float2* F, float2* coef, hipStream_t s);
+                const float2* tw_n, float2* F, float2* coef, int precision, hipStream_t s);
+void launch_legendre_mfma(const float2* F, int S, int bw, const float* legendre,
+                          const int* leg_off, float2* coef, hipStream_t s);
 
 // Laplace fusion of C channels (null = all-zero channel) of `count` complex
 // coefficients, mode runs in unshifted order.  out: count complex.
@@ -59,10 +62,14 @@ void launch_reduce_keys(const unsigned long long* partials, int count, int npair
 
 // ------------------------------------------------------------- SO(3)
 // S buffer: [m2 in 0..B-1][m1+B-1 in 0..2B-2][k in 0..n-1] complex per pair.
+// precision 0: FP32 Wigner-d GEMV per pair; 1: split-FP16 MFMA over the batch
 void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
                       int64_t h_stride, int npairs, const float* wigner,
                       const int64_t* wig_off_dev, const float* wig_norm, float2* S,
-                      hipStream_t s);
+                      int precision, hipStream_t s);
+void launch_so3_synth_mfma(int bw, const float2* f, int64_t f_stride, const float2* h,
+                           int64_t h_stride, int npairs, const float* wig, const int64_t* wig_off,
+                           const float* wig_norm, float2* S, hipStream_t s);
 // FFT over m1 -> alpha: U[k][m2][alpha] per pair
 void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
                           const float2* tw, float2* U, hipStream_t s);
@@ -97,6 +104,22 @@ void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int n
                           int N, const float* edges, float* sumI, float* sumR, uint32_t* cnt,
                           hipStream_t s);
 // forward 3-D r2c of (mean intensity, mean range) per cloud:
+// columns per workgroup of the three column-FFT passes (0 = default);
+// chosen by phz_autotune.  The tile width never changes the arithmetic of a
+// line, so every choice gives bit-identical spectra.
+struct SpatialTiles {
+  int fwd_x = 0, fwd_z = 0, inv_x = 0;
+};
+// tile value kRegisterEngine selects the register four-step kernel
+// (fft_reg.hpp) for L = 64, 128, 256
+constexpr int kRegisterEngine = 1;
+int fft_cols_default_tile(int L);
+bool fft_cols_tile_ok(int L, int it);
+// one column pass over nbatch half spectra: which = 0 forward along x,
+// 1 forward along z, 2 inverse along x; it = columns per workgroup
+void launch_spatial_fft_pass(int N, int nbatch, float2* buf, int which, int it,
+                             const FftPlan& plan, const float2* tw, hipStream_t s);
+
 // spec[cloud][ch in {I,R}][z][x][k2], k2 < N/2+1.  launch_spatial_forward =
 // launch_spatial_r2c (voxel means -> r2c along y) + launch_spatial_fft_fwd
 // (FFT along x, then z, of nspec = 2*nclouds half spectra).
@@ -104,14 +127,21 @@ void launch_spatial_r2c(int N, int nclouds, const float* sumI, const float* sumR
                         const uint32_t* cnt, const FftPlan& plan, const float2* tw, float2* spec,
                         hipStream_t s);
 void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
-                            const float2* tw, hipStream_t s);
+                            const float2* tw, const SpatialTiles& tiles, hipStream_t s);
+// plane form of launch_spatial_r2c + the x pass (whole z planes in LDS), for
+// N where spatial_plane_ok(N); follow with launch_spatial_fft_pass(which=1)
+bool spatial_plane_ok(int N);
+void launch_spatial_plane_fwd(int N, int nclouds, const float* sumI, const float* sumR,
+                              const uint32_t* cnt, const FftPlan& plan, const float2* tw,
+                              float2* spec, hipStream_t s);
 void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
                             const uint32_t* cnt, const FftPlan& plan, const float2* tw,
                             float2* spec, hipStream_t s);
 // forward from dense real channels (stage API): chans[c][N^3] in order,
 // nch channels (<= 8) -> spec[c][z][x][k2]
 void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
-                                  const float2* tw, float2* spec, hipStream_t s);
+                                  const float2* tw, float2* spec, const SpatialTiles& tiles,
+                                  hipStream_t s);
 // fused cross spectrum + inverse 3-D c2r + argmax for npairs pairs.  F
 // channels and G channels (null = zero channel) in the reference's spatial
 // order; pair p reads its channels at ch[c] + p*stride (stride 0: shared).
@@ -122,14 +152,15 @@ void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, C
                             int64_t g_stride, const FusionRuns& runs, bool fuse,
                             const FftPlan& plan, const float2* tw, float2* work,
                             float* corr /*nullable*/, unsigned long long* partials,
-                            unsigned long long* keys, hipStream_t s);
+                            unsigned long long* keys, const SpatialTiles& tiles,
+                            hipStream_t s);
 // its three passes: cross spectrum + inverse FFT along z; inverse FFT along
 // x; c2r along y + argmax
 void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
                             int64_t g_stride, const FusionRuns& runs, bool fuse,
                             const FftPlan& plan, const float2* tw, float2* work, hipStream_t s);
 void launch_spatial_fft_inv(int N, int npairs, float2* work, const FftPlan& plan,
-                            const float2* tw, hipStream_t s);
+                            const float2* tw, const SpatialTiles& tiles, hipStream_t s);
 void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& plan,
                         const float2* tw, float* corr, unsigned long long* partials,
                         unsigned long long* keys, hipStream_t s);

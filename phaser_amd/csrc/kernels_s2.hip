@@ -297,10 +297,11 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
           u = (qj - sec + ns) % ns;
           if (u <= u_prev || u > (ns - 1) / 2) break;
         }
-        // azimuth distance from the query to the sector's interval
+        // azimuth distance from the query to the sector's interval, in
+        // [0, pi], where 1 - cos is increasing: the bound grows with u
         float d = fabsf(phq - sec * dsec);
         if (d > kPi) d = 2.f * kPi - d;
-        const float dphi = fminf(fmaxf(d - 0.5f * dsec - eps, 0.f), 0.5f * kPi);
+        const float dphi = fmaxf(d - 0.5f * dsec - eps, 0.f);
         const float s2 = sinf(0.5f * dphi);
         if (A + Bc * 2.f * s2 * s2 > thr2()) break;
         scan_cell(b * ns + sec);
@@ -451,11 +452,15 @@ void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const flo
 }
 
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,
-                const float2* tw_n, float2* F, float2* coef, hipStream_t s) {
+                const float2* tw_n, float2* F, float2* coef, int precision, hipStream_t s) {
   const int n = 2 * bw;
   const int th = ((bw + 63) / 64) * 64;
   hipLaunchKernelGGL(k_phi_dft, dim3(n, S), dim3(th), n * sizeof(float) + n * sizeof(float2), s,
                      sig, n, bw, tw_n, F);
+  if (precision == 1) {
+    launch_legendre_mfma(F, S, bw, legendre, leg_off_dev, coef, s);
+    return;
+  }
   hipLaunchKernelGGL(k_legendre, dim3(bw, S), dim3(th), n * sizeof(float2), s, F, n, bw, legendre,
                      leg_off_dev, coef);
 }

@@ -291,7 +291,12 @@ __global__ void k_rot_pose(int bw, int npairs, const unsigned long long* __restr
 void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
                       int64_t h_stride, int npairs, const float* wigner,
                       const int64_t* wig_off_dev, const float* wig_norm, float2* S,
-                      hipStream_t s) {
+                      int precision, hipStream_t s) {
+  if (precision == 1) {
+    launch_so3_synth_mfma(bw, f_coef, f_stride, h_coef, h_stride, npairs, wigner, wig_off_dev,
+                          wig_norm, S, s);
+    return;
+  }
   const int n = 2 * bw;
   const int nblk = bw * (bw + 1) / 2;
   const int th = ((n + 63) / 64) * 64;

@@ -8,6 +8,7 @@
 // spatial-correlation.cc:142-161): C = F conj(G) with no normalisation on the
 // r2c half spectrum, c2r of the first N^2(N/2+1) entries.  Grid layout is
 // the reference's: linear index x*N + y + N^2*z, i.e. [z][x][y].
+#include "fft_reg.hpp"
 #include "kernels.hpp"
 
 namespace phz {
@@ -140,6 +141,81 @@ __global__ void k_spat_r2c(int N, const float* __restrict__ a0, const float* __r
     const int64_t o = (int64_t)(row0 + r) * h + k;
     spec[c * spec_stride0 + o] = make_float2(0.5f * s.x, 0.5f * s.y);
     if (spec_stride1 >= 0) spec[c * spec_stride0 + spec_stride1 + o] = make_float2(0.5f * d.y, -0.5f * d.x);
+  }
+}
+
+// pass 1, plane form (N <= kPlaneMaxN): one workgroup per (z plane, cloud)
+// keeps the whole complex plane Z = mean I + i mean R in LDS, transforms it
+// along y and then x, and splits the 2-D spectrum into the two half spectra
+// with Z(-k) = conj of the real parts' transforms:
+//   I^(k) = (Z(k) + conj Z(-k)) / 2,  R^(k) = (Z(k) - conj Z(-k)) / 2i.
+// Replaces k_spat_r2c plus the x column pass (one HBM round trip less).
+constexpr int kPlaneMaxN = 128;
+constexpr int kPlaneChunk = 8;  // lines per LDS FFT batch
+
+size_t spat_plane_lds_bytes(int N) {
+  return ((size_t)N * (N + 1) + 2 * (size_t)kPlaneChunk * (N + 1)) * sizeof(float2);
+}
+
+__global__ void __launch_bounds__(256) k_spat_plane_fwd(
+    int N, const float* __restrict__ sumI, const float* __restrict__ sumR,
+    const uint32_t* __restrict__ cnt, int64_t cloud_stride, FftPlan plan,
+    const float2* __restrict__ tw, float2* __restrict__ spec, int64_t spec_stride0,
+    int64_t spec_stride1) {
+  const int z = blockIdx.x, c = blockIdx.y;
+  const int h = N / 2 + 1, ls = N + 1;
+  extern __shared__ float2 sm[];
+  float2* P = sm;                     // N x ls: the plane, row x, column y
+  float2* A = sm + (size_t)N * ls;    // kPlaneChunk x ls
+  float2* Bf = A + kPlaneChunk * ls;  // kPlaneChunk x ls
+  const int64_t base = c * cloud_stride + (int64_t)z * N * N;
+  // rows (x) in chunks: load means, FFT along y, park in P
+  for (int x0 = 0; x0 < N; x0 += kPlaneChunk) {
+    for (int idx = threadIdx.x; idx < kPlaneChunk * N; idx += blockDim.x) {
+      const int r = idx / N, y = idx - r * N;
+      float2 v = make_float2(0.f, 0.f);
+      if (x0 + r < N) {
+        const int64_t off = base + (int64_t)(x0 + r) * N + y;
+        const uint32_t k = cnt[off];  // empty voxels never touch the sums
+        if (k) {
+          v.x = sumI[off] / (float)k;
+          v.y = sumR[off] / (float)k;
+        }
+      }
+      A[r * ls + y] = v;
+    }
+    __syncthreads();
+    const float2* R = lds_fft<-1>(A, Bf, kPlaneChunk, plan, tw, ls);
+    for (int idx = threadIdx.x; idx < kPlaneChunk * N; idx += blockDim.x) {
+      const int r = idx / N, y = idx - r * N;
+      if (x0 + r < N) P[(x0 + r) * ls + y] = R[r * ls + y];
+    }
+    __syncthreads();
+  }
+  // columns (ky) in chunks: FFT along x in place through A/Bf
+  for (int k0 = 0; k0 < N; k0 += kPlaneChunk) {
+    for (int idx = threadIdx.x; idx < kPlaneChunk * N; idx += blockDim.x) {
+      const int x = idx / kPlaneChunk, j = idx - x * kPlaneChunk;
+      A[j * ls + x] = (k0 + j < N) ? P[x * ls + k0 + j] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const float2* R = lds_fft<-1>(A, Bf, kPlaneChunk, plan, tw, ls);
+    for (int idx = threadIdx.x; idx < kPlaneChunk * N; idx += blockDim.x) {
+      const int x = idx / kPlaneChunk, j = idx - x * kPlaneChunk;
+      if (k0 + j < N) P[x * ls + k0 + j] = R[j * ls + x];
+    }
+    __syncthreads();
+  }
+  // split into the two half spectra, spec[cloud][ch][z][kx][ky], ky < h
+  float2* o0 = spec + c * spec_stride0 + (int64_t)z * N * h;
+  float2* o1 = o0 + spec_stride1;
+  for (int idx = threadIdx.x; idx < N * h; idx += blockDim.x) {
+    const int kx = idx / h, ky = idx - kx * h;
+    const float2 zk = P[kx * ls + ky];
+    const float2 zc = cconj(P[((N - kx) % N) * ls + (N - ky) % N]);
+    const float2 sm2 = cadd(zk, zc), d = csub(zk, zc);
+    o0[idx] = make_float2(0.5f * sm2.x, 0.5f * sm2.y);
+    o1[idx] = make_float2(0.5f * d.y, -0.5f * d.x);
   }
 }
 
@@ -373,27 +449,68 @@ void launch_reduce_keys(const unsigned long long* partials, int count, int npair
 
 int spatial_c2r_blocks(int N) { return (N * N + 15) / 16; }
 
-// tile width: 16 columns while two padded buffers fit 64 KB of LDS, else 8
+// column tiles: IT columns per workgroup, two padded LDS line buffers
+int fft_cols_default_tile(int L) { return L <= 128 ? 16 : 8; }
+
+bool fft_cols_tile_ok(int L, int it) {
+  if (it == kRegisterEngine) return L == 64 || L == 128 || L == 256;
+  return (it == 8 || it == 16 || it == 32 || it == 64) &&
+         2 * (size_t)it * (L + 1) * sizeof(float2) <= 160 * 1024;
+}
+
+// register four-step engine: 256 threads, 256 / R2 lines per workgroup
+template <int DIR, int L>
+static void fft_lines_t(float2* data, int nbatch, int64_t batch_stride, int outer, int inner,
+                        const float2* tw, hipStream_t s) {
+  constexpr int lines = 256 / RegSplit<L>::R2;
+  const int nl = outer * inner;
+  dim3 g((nl + lines - 1) / lines, nbatch);
+  hipLaunchKernelGGL((k_fft_lines<DIR, L, lines>), g, dim3(256), 0, s, data, batch_stride, nl,
+                     inner, tw);
+}
+
+template <int DIR>
+static void fft_lines(float2* data, int nbatch, int64_t batch_stride, int outer, int L, int inner,
+                      const float2* tw, hipStream_t s) {
+  if (L == 64)
+    fft_lines_t<DIR, 64>(data, nbatch, batch_stride, outer, inner, tw, s);
+  else if (L == 128)
+    fft_lines_t<DIR, 128>(data, nbatch, batch_stride, outer, inner, tw, s);
+  else
+    fft_lines_t<DIR, 256>(data, nbatch, batch_stride, outer, inner, tw, s);
+}
+
+template <int DIR, int IT>
+static void fft_cols_it(float2* data, int nbatch, int64_t batch_stride, int outer, int L,
+                        int inner, const FftPlan& plan, const float2* tw, hipStream_t s) {
+  dim3 g((inner + IT - 1) / IT, outer, nbatch);
+  const int threads = IT <= 16 ? 256 : (IT == 32 ? 512 : 1024);
+  hipLaunchKernelGGL((k_fft_cols<DIR, IT>), g, dim3(threads), 2 * IT * (L + 1) * sizeof(float2),
+                     s, data, batch_stride, L, inner, plan, tw);
+}
+
 template <int DIR>
 static void fft_cols_t(float2* data, int nbatch, int64_t batch_stride, int outer, int L,
-                       int inner, const FftPlan& plan, const float2* tw, hipStream_t s) {
-  if (L <= 128) {
-    dim3 g((inner + 15) / 16, outer, nbatch);
-    hipLaunchKernelGGL((k_fft_cols<DIR, 16>), g, dim3(256), 2 * 16 * (L + 1) * sizeof(float2), s,
-                       data, batch_stride, L, inner, plan, tw);
-  } else {
-    dim3 g((inner + 7) / 8, outer, nbatch);
-    hipLaunchKernelGGL((k_fft_cols<DIR, 8>), g, dim3(256), 2 * 8 * (L + 1) * sizeof(float2), s,
-                       data, batch_stride, L, inner, plan, tw);
+                       int inner, int it, const FftPlan& plan, const float2* tw, hipStream_t s) {
+  if (!fft_cols_tile_ok(L, it)) it = fft_cols_default_tile(L);
+  if (it == kRegisterEngine) {
+    fft_lines<DIR>(data, nbatch, batch_stride, outer, L, inner, tw, s);
+    return;
+  }
+  switch (it) {
+    case 8: fft_cols_it<DIR, 8>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s); break;
+    case 16: fft_cols_it<DIR, 16>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s); break;
+    case 32: fft_cols_it<DIR, 32>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s); break;
+    default: fft_cols_it<DIR, 64>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s); break;
   }
 }
 
 static void fft_cols(float2* data, int nbatch, int64_t batch_stride, int outer, int L, int inner,
-                     int dir, const FftPlan& plan, const float2* tw, hipStream_t s) {
+                     int dir, int it, const FftPlan& plan, const float2* tw, hipStream_t s) {
   if (dir < 0)
-    fft_cols_t<-1>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s);
+    fft_cols_t<-1>(data, nbatch, batch_stride, outer, L, inner, it, plan, tw, s);
   else
-    fft_cols_t<+1>(data, nbatch, batch_stride, outer, L, inner, plan, tw, s);
+    fft_cols_t<+1>(data, nbatch, batch_stride, outer, L, inner, it, plan, tw, s);
 }
 
 void launch_spatial_r2c(int N, int nclouds, const float* sumI, const float* sumR,
@@ -406,24 +523,46 @@ void launch_spatial_r2c(int N, int nclouds, const float* sumI, const float* sumR
                      sumR, cnt, T, plan, tw, spec, 2 * H, H);
 }
 
-void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
-                            const float2* tw, hipStream_t s) {
+bool spatial_plane_ok(int N) { return N <= kPlaneMaxN && spat_plane_lds_bytes(N) <= 160 * 1024; }
+
+void launch_spatial_plane_fwd(int N, int nclouds, const float* sumI, const float* sumR,
+                              const uint32_t* cnt, const FftPlan& plan, const float2* tw,
+                              float2* spec, hipStream_t s) {
+  const int h = N / 2 + 1;
+  const int64_t H = (int64_t)N * N * h, T = (int64_t)N * N * N;
+  hipLaunchKernelGGL(k_spat_plane_fwd, dim3(N, nclouds), dim3(256), spat_plane_lds_bytes(N), s, N,
+                     sumI, sumR, cnt, T, plan, tw, spec, 2 * H, H);
+}
+
+void launch_spatial_fft_pass(int N, int nbatch, float2* buf, int which, int it,
+                             const FftPlan& plan, const float2* tw, hipStream_t s) {
   const int h = N / 2 + 1;
   const int64_t H = (int64_t)N * N * h;
+  if (which == 0)  // forward along x: [z][x][k2], columns over (z, k2)
+    fft_cols(buf, nbatch, H, N, N, h, -1, it, plan, tw, s);
+  else if (which == 1)  // forward along z: columns over (x, k2)
+    fft_cols(buf, nbatch, H, 1, N, N * h, -1, it, plan, tw, s);
+  else  // inverse along x
+    fft_cols(buf, nbatch, H, N, N, h, +1, it, plan, tw, s);
+}
+
+void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
+                            const float2* tw, const SpatialTiles& tiles, hipStream_t s) {
   // x then z, over all nspec half spectra
-  fft_cols(spec, nspec, H, N, N, h, -1, plan, tw, s);
-  fft_cols(spec, nspec, H, 1, N, N * h, -1, plan, tw, s);
+  launch_spatial_fft_pass(N, nspec, spec, 0, tiles.fwd_x, plan, tw, s);
+  launch_spatial_fft_pass(N, nspec, spec, 1, tiles.fwd_z, plan, tw, s);
 }
 
 void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* sumR,
                             const uint32_t* cnt, const FftPlan& plan, const float2* tw,
                             float2* spec, hipStream_t s) {
   launch_spatial_r2c(N, nclouds, sumI, sumR, cnt, plan, tw, spec, s);
-  launch_spatial_fft_fwd(N, 2 * nclouds, spec, plan, tw, s);
+  launch_spatial_fft_fwd(N, 2 * nclouds, spec, plan, tw, SpatialTiles{}, s);
 }
 
 void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
-                                  const float2* tw, float2* spec, hipStream_t s) {
+                                  const float2* tw, float2* spec, const SpatialTiles& tiles,
+                                  hipStream_t s) {
   const int h = N / 2 + 1;
   const int64_t H = (int64_t)N * N * h, T = (int64_t)N * N * N;
   for (int c = 0; c < nch; c += 2) {
@@ -434,8 +573,7 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
                        (const uint32_t*)nullptr, T, plan, tw, spec + c * H, 0,
                        pairc ? H : (int64_t)-1);
   }
-  fft_cols(spec, nch, H, N, N, h, -1, plan, tw, s);
-  fft_cols(spec, nch, H, 1, N, N * h, -1, plan, tw, s);
+  launch_spatial_fft_fwd(N, nch, spec, plan, tw, tiles, s);
 }
 
 void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
@@ -453,9 +591,8 @@ void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, C
 }
 
 void launch_spatial_fft_inv(int N, int npairs, float2* work, const FftPlan& plan,
-                            const float2* tw, hipStream_t s) {
-  const int h = N / 2 + 1;
-  fft_cols(work, npairs, (int64_t)N * N * h, N, N, h, +1, plan, tw, s);
+                            const float2* tw, const SpatialTiles& tiles, hipStream_t s) {
+  launch_spatial_fft_pass(N, npairs, work, 2, tiles.inv_x, plan, tw, s);
 }
 
 void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& plan,
@@ -472,9 +609,9 @@ void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, C
                             int64_t g_stride, const FusionRuns& runs, bool fuse,
                             const FftPlan& plan, const float2* tw, float2* work, float* corr,
                             unsigned long long* partials, unsigned long long* keys,
-                            hipStream_t s) {
+                            const SpatialTiles& tiles, hipStream_t s) {
   launch_spatial_cross_z(N, npairs, F, f_stride, G, g_stride, runs, fuse, plan, tw, work, s);
-  launch_spatial_fft_inv(N, npairs, work, plan, tw, s);
+  launch_spatial_fft_inv(N, npairs, work, plan, tw, tiles, s);
   launch_spatial_c2r(N, npairs, work, plan, tw, corr, partials, keys, s);
 }
 

@@ -23,7 +23,7 @@ COPY_KERNEL = "k_copy_f4"
 
 def short_name(name):
     """'void phz::k_fft_cols<-1, 16>(float2*, ...)' -> 'k_fft_cols<-1, 16>'."""
-    n = name.strip()
+    n = name.strip().replace("(anonymous namespace)::", "")
     n = re.sub(r"^void\s+", "", n)
     depth = 0
     for i, ch in enumerate(n):  # cut the argument list at the top-level '('
@@ -34,7 +34,7 @@ def short_name(name):
         elif ch == "(" and depth == 0:
             n = n[:i]
             break
-    return n.replace("phz::", "").replace("(anonymous namespace)::", "")
+    return n.replace("phz::", "")
 
 
 def matches(kernel, name):

@@ -21,9 +21,10 @@ _lib = None
 class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int32), ("max_batch", C.c_int32),
                 ("contexts_per_shape", C.c_int32), ("discretize_lower", C.c_int32),
-                ("discretize_upper", C.c_int32), ("reserved", C.c_int32),
+                ("discretize_upper", C.c_int32), ("autotune", C.c_int32),
                 ("max_points", C.c_int64), ("hbm_budget", C.c_uint64),
-                ("batch_window_ms", C.c_double)]
+                ("batch_window_ms", C.c_double), ("precision", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -97,7 +98,8 @@ def _check(rc):
 
 class Server:
     def __init__(self, max_batch=16, contexts_per_shape=2, hbm_budget=64 << 30,
-                 max_points=262144, batch_window_ms=0.0, device=0, bounds=(-50, 50)):
+                 max_points=262144, batch_window_ms=0.0, device=0, bounds=(-50, 50),
+                 autotune=False, precision=0):
         cfg = ServeConfig()
         lib().phs_default_config(C.byref(cfg))
         cfg.max_batch = max_batch
@@ -105,6 +107,8 @@ class Server:
         cfg.hbm_budget = int(hbm_budget)
         cfg.max_points = max_points
         cfg.batch_window_ms = batch_window_ms
+        cfg.autotune = int(autotune)
+        cfg.precision = int(precision)
         cfg.device = device
         cfg.discretize_lower, cfg.discretize_upper = bounds
         h = C.c_void_p()

@@ -97,6 +97,13 @@ int phz_create(const phz_config* c, phz_ctx** out) {
   return 0;
 }
 
+int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
+  (void)c;
+  (void)repeats;
+  if (out) std::memset(out, 0, sizeof(*out));
+  return 0;
+}
+
 void phz_destroy(phz_ctx* c) {
   if (!c) return;
   fake::live_bytes -= (long long)c->bytes;

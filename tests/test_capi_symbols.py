@@ -38,7 +38,7 @@ def test_synth_library_exports():
 
 def test_struct_layouts():
     from phaser_amd import capi
-    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 + 4
+    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 4
     # rot/trans index + values (16) + zyz,quat,euler (80) + T (64) + translation (24)
     assert C.sizeof(capi.Result) == 16 + 80 + 64 + 24
 
@@ -77,11 +77,13 @@ _LAYOUT_C = r"""
 #define S(t) printf(#t " %zu\n", sizeof(t))
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f))
 int main(void) {
-  S(phz_config); S(phz_result); S(phz_stage_time);
+  S(phz_config); S(phz_result); S(phz_stage_time); S(phz_tuning);
+  O(phz_tuning, plane_fwd); O(phz_tuning, ms_fwd_x); O(phz_tuning, ms_plane);
   S(phs_config); S(phs_request); S(phs_response); S(phs_stats);
   O(phz_config, max_points); O(phz_config, device); O(phz_result, zyz);
   O(phz_result, T_rot); O(phz_result, translation);
   O(phs_config, max_points); O(phs_config, hbm_budget); O(phs_config, batch_window_ms);
+  O(phs_config, precision); O(phz_config, precision);
   O(phs_request, target); O(phs_request, Ps); O(phs_request, tag);
   O(phs_response, result); O(phs_response, latency_ms); O(phs_response, error);
   O(phs_stats, contexts_live); O(phs_stats, queued);
@@ -102,7 +104,8 @@ def test_ctypes_layouts_match_the_c_headers(tmp_path):
     got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True,
                                                        text=True, check=True).stdout.splitlines())
     types = {"phz_config": capi.Config, "phz_result": capi.Result,
-             "phz_stage_time": capi.StageTime, "phs_config": serve.ServeConfig,
+             "phz_stage_time": capi.StageTime, "phz_tuning": capi.Tuning,
+             "phs_config": serve.ServeConfig,
              "phs_request": serve.Request, "phs_response": serve.Response,
              "phs_stats": serve.Stats}
     for key, val in got.items():

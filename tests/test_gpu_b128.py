@@ -29,8 +29,10 @@ def _bin_ok(idx, value, want_idx, top2):
     return near_tie and value >= top2[1] * (1 - 1e-4)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
 @pytest.mark.parametrize("seed", [0, 1])
-def test_register_b128_n256_matches_golden(seed):
+def test_register_b128_n256_matches_golden(seed, precision):
+    """precision 0: FP32 contractions; 1: split FP16 on MFMA."""
     from phaser_amd import synth
     from phaser_amd.capi import Context
     d = np.load(GOLDEN)
@@ -38,14 +40,14 @@ def test_register_b128_n256_matches_golden(seed):
     src, tgt, _, _ = synth.pair(seed=seed, **synth.CONFIG_B128)
     assert [len(src), len(tgt)] == list(d[k + "n_points"]), "generator drifted from the fixture"
     print(f"b128 seed {seed}: {len(src)} points", flush=True)
-    with Context(bandwidth=128, n_voxels=256) as ctx:
+    with Context(bandwidth=128, n_voxels=256, precision=precision) as ctx:
         r = ctx.register(tgt, src, grids=True)
     rc = r["rot_corr"].astype(np.float64)
     tc = r["trans_corr"].astype(np.float64)
     rot_err = np.max(np.abs(rc[d["rot_sample_index"]] - d[k + "rot_sample"])) / d[k + "rot_max_abs"]
     tr_err = np.max(np.abs(tc[d["trans_sample_index"]] - d[k + "trans_sample"])) / \
         d[k + "trans_max_abs"]
-    print(f"b128 seed {seed}: rot grid err {rot_err:.2e}, trans grid err {tr_err:.2e}, "
+    print(f"b128 seed {seed} precision {precision}: rot grid err {rot_err:.2e}, trans grid err {tr_err:.2e}, "
           f"rot {r['rot_index']} vs {int(d[k + 'rot_index'])}, "
           f"trans {r['trans_index']} vs {int(d[k + 'trans_index'])}", flush=True)
     assert rot_err < 2e-3

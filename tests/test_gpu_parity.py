@@ -232,4 +232,135 @@ def test_profiling_stage_times(ctx32):
     st = ctx32.stage_times()
     ctx32.set_profiling(False)
     assert st["s2_nn"][1] == 1 and st["s2_nn"][0] > 0
-    assert st["spatial_inverse"][0] > 0
+    for k in ("spatial_r2c", "spatial_fft_fwd", "spatial_cross_z", "spatial_fft_inv",
+              "spatial_c2r", "so3_synth", "voxel_scatter"):
+        assert st[k][1] == 1 and st[k][0] > 0, k
+
+
+def test_tile_choices_give_identical_spectra(ctx32, po):
+    """Column-FFT tile widths of the LDS kernel change the work split, not a
+    line's arithmetic: the spatial correlation is bit-identical for every
+    tile.  The register engine matches to FP32 rounding."""
+    rng = np.random.default_rng(11)
+    f = rng.uniform(0, 1, (2, 64 ** 3))
+    g = rng.uniform(0, 1, (2, 64 ** 3))
+    out = []
+    for it in (8, 16, 32, 64):
+        ctx32.set_tuning(it, it, it)
+        assert ctx32.tuning()["fwd_x"] == it
+        out.append(ctx32.spatial_correlate(f, g))
+    for corr, am in out[1:]:
+        np.testing.assert_array_equal(corr, out[0][0])
+        assert am == out[0][1]
+    want = po.spatial_correlate_laplace(f, g, 64)
+    assert _rel(out[0][0], want) < 2e-4
+    # the register four-step kernel: same transform, other rounding
+    ctx32.set_tuning(1, 1, 1)
+    assert ctx32.tuning()["fwd_z"] == 1
+    reg, am = ctx32.spatial_correlate(f, g)
+    assert _rel(reg, out[0][0].astype(np.float64)) < 1e-5
+    assert _rel(reg, want) < 2e-4
+    assert _argmax_ok(am, want)
+    t = ctx32.autotune(repeats=2)
+    assert t["fwd_x"] in (1, 8, 16, 32, 64) and t["ms_fwd_x"] > 0
+    ctx32.set_tuning(0, 0, 0)
+
+
+def test_plane_forward_matches_row_passes(ctx64):
+    """The whole-plane first forward pass (y and x in LDS) against the r2c +
+    x-column passes it replaces: same bins, grids within FP32 rounding."""
+    src, tgt, _, _ = _pair(2, dict(rings=128, azimuths=1024, overlap=0.7))
+    ctx64.set_tuning(plane_fwd=1)
+    assert ctx64.tuning()["plane_fwd"] == 1
+    a = ctx64.register(tgt, src, grids=True)
+    ctx64.set_tuning(plane_fwd=-1)
+    assert ctx64.tuning()["plane_fwd"] == -1
+    b = ctx64.register(tgt, src, grids=True)
+    ctx64.set_tuning()
+    assert a["rot_index"] == b["rot_index"]
+    assert a["trans_index"] == b["trans_index"]
+    assert _rel(a["trans_corr"], b["trans_corr"].astype(np.float64)) < 1e-5
+
+
+# ---------------------------------------------------------------- split FP16
+# The Legendre and Wigner-d contractions on MFMA in split FP16 (hi + lo
+# operands, FP32 accumulation): held to the FP32 bars above.
+
+@pytest.fixture(scope="module")
+def ctx32_mfma():
+    from phaser_amd.capi import Context
+    c = Context(bandwidth=32, n_voxels=64, max_points=200000, precision=1)
+    yield c
+    c.close()
+
+
+def test_mfma_s2_transform_matches_oracle(ctx32_mfma, po):
+    rng = np.random.default_rng(0)
+    grids = rng.normal(size=(3, 64 * 64))
+    g = ctx32_mfma.s2_transform(grids)
+    errs = [_rel(g[k], po.fst(32, grids[k])) for k in range(3)]
+    print(f"split-fp16 S2 transform rel err {max(errs):.2e}")
+    assert max(errs) < 2e-5
+
+
+def test_mfma_so3_matches_oracle(ctx32_mfma, po):
+    rng = np.random.default_rng(1)
+    f = po.fst(32, rng.normal(size=64 * 64))
+    h = po.fst(32, rng.normal(size=64 * 64))
+    corr, am, syn = ctx32_mfma.so3_correlate_coef(f.astype(np.complex64), h.astype(np.complex64),
+                                                  synthesis=True)
+    mag = po.so3_correlate(32, f, h)
+    S = po.so3_synthesis(32, f, h)
+    B, n = 32, 64
+    syn = syn.reshape(B, 2 * B - 1, n)
+    want = np.stack([np.stack([S[(m1 + n) % n, m2] for m1 in range(-(B - 1), B)])
+                     for m2 in range(B)])
+    e_syn, e_corr = _rel(syn, want), _rel(corr, mag)
+    print(f"split-fp16 SO(3) synthesis rel err {e_syn:.2e}, |C|^2 grid rel err {e_corr:.2e}")
+    assert e_syn < 1e-4
+    assert e_corr < 2e-4
+    assert _argmax_ok(am, mag)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_mfma_register_b32_matches_oracle(ctx32_mfma, po, seed):
+    src, tgt, _, _ = _pair(seed, dict(rings=64, azimuths=1024))
+    r = ctx32_mfma.register(tgt, src, grids=True)
+    o = po.register(tgt, src, bw=32, n_voxels=64, want_grids=True)
+    assert _argmax_ok(r["rot_index"], o["rot_corr"])
+    assert _rel(r["rot_corr"], o["rot_corr"]) < 2e-3
+    if r["rot_index"] == o["rot_index"]:
+        _check_pose(r, o, 64)
+        assert _argmax_ok(r["trans_index"], o["trans_corr"])
+
+
+@pytest.mark.parametrize("B,N", [(64, 128), (96, 192)])
+def test_mfma_register_matches_oracle(po, B, N):
+    from phaser_amd.capi import Context
+    from phaser_amd import synth
+    src, tgt, _, _ = synth.mixed_pair(3, B)
+    with Context(bandwidth=B, n_voxels=N, precision=1) as ctx:
+        r = ctx.register(tgt, src, grids=True)
+    o = po.register(tgt, src, bw=B, n_voxels=N, want_grids=True)
+    e = _rel(r["rot_corr"], o["rot_corr"])
+    print(f"split-fp16 B={B}: rotation grid rel err {e:.2e}")
+    assert _argmax_ok(r["rot_index"], o["rot_corr"])
+    assert e < 2e-3
+    if r["rot_index"] == o["rot_index"]:
+        _check_pose(r, o, N)
+        assert _argmax_ok(r["trans_index"], o["trans_corr"])
+
+
+def test_mfma_batch_matches_fp32_batch():
+    """Both forms over one batch with a shared target: same bins."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=64, azimuths=1024)) for s in range(6)]
+    out = []
+    for prec in (0, 1):
+        with Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=6,
+                     precision=prec) as ctx:
+            out.append(ctx.register_pairs([pairs[0][1]], [s for s, _, _, _ in pairs],
+                                          shared_target=True))
+    for a, b in zip(*out):
+        assert a["rot_index"] == b["rot_index"]
+        assert a["trans_index"] == b["trans_index"]

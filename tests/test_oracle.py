@@ -369,3 +369,17 @@ def test_projection_and_grid_point_selection(po):
     assert np.allclose(rng, 7.0, atol=1e-5)
     nn = po.nn(proj, q)
     np.testing.assert_array_equal(nn, np.arange(len(q)))
+
+
+def test_register_is_reentrant_across_threads(po):
+    """The CPU baseline runs oracle registrations on several host threads at
+    once (ctypes releases the GIL): results equal the sequential ones."""
+    import concurrent.futures as cf
+    from phaser_amd import synth
+    pairs = [synth.pair(seed=s, rings=16, azimuths=256, noise_sigma=0.02) for s in range(8)]
+    seq = [po.register(t, s, bw=8, n_voxels=16) for s, t, _, _ in pairs]
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        par = list(ex.map(lambda p: po.register(p[1], p[0], bw=8, n_voxels=16), pairs))
+    for a, b in zip(seq, par):
+        assert (a["rot_index"], a["trans_index"]) == (b["rot_index"], b["trans_index"])
+        assert a["rot_value"] == b["rot_value"] and a["trans_value"] == b["trans_value"]
