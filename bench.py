@@ -34,7 +34,8 @@ B, N = 64, 128
 WORKLOAD = dict(rings=128, azimuths=1024, noise_sigma=0.02, overlap=0.7)
 
 
-def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False):
+def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False,
+                 lines_fwd=False, lines_inv=False):
     """Algorithmic (compulsory) HBM bytes of ONE launch of the kernel that
     dominates each stage, for a batch of `pairs` registrations: inputs read
     once, outputs written once, FP32 / complex64 elements, C = 2 populated
@@ -57,14 +58,18 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False):
         "voxel_scatter": ("k_voxel_scatter<false>", clouds * (16 + 24) * P_mean),
         # three N^3 accumulators in, two half spectra out (the plane form also
         # transforms along x on the way)
-        "spatial_r2c": ("k_spat_plane_fwd" if plane else "k_spat_r2c",
+        "spatial_r2c": ("k_spat_plane_fwd" if plane else
+                        ("k_spat_r2c_reg" if fused_reg else "k_spat_r2c"),
                         clouds * (12 * N ** 3 + 2 * 8 * H)),
         # one FFT pass (x or z) over every half spectrum: read + write
-        "spatial_fft_fwd": ("k_fft_cols<-1>", 2 * clouds * 16 * H),
+        "spatial_fft_fwd": ("k_fft_lines<-1>" if lines_fwd else "k_fft_cols<-1>",
+                            2 * clouds * 16 * H),
         # C target + C source spectra in, cross spectrum out
-        "spatial_cross_z": ("k_spat_cross_z", nt * C * 8 * H + pairs * (C * 8 * H + 8 * H)),
-        "spatial_fft_inv": ("k_fft_cols<1>", pairs * 16 * H),
-        "spatial_c2r": ("k_spat_c2r", pairs * (8 * H + 4 * N ** 3)),
+        "spatial_cross_z": ("k_spat_cross_z_reg" if fused_reg else "k_spat_cross_z",
+                            nt * C * 8 * H + pairs * (C * 8 * H + 8 * H)),
+        "spatial_fft_inv": ("k_fft_lines<1>" if lines_inv else "k_fft_cols<1>", pairs * 16 * H),
+        "spatial_c2r": ("k_spat_c2r_reg" if fused_reg else "k_spat_c2r",
+                        pairs * (8 * H + 4 * N ** 3)),
     }
 
 
@@ -167,6 +172,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--mode", choices=["batch", "serve"], default="batch")
     ap.add_argument("--autotune", type=int, default=1, help="pick FFT column tiles first")
+    ap.add_argument("--nn-lanes", type=int, default=0, choices=[0, 2, 4, 8, 16],
+                    help="lanes per query of the S2 nearest-neighbour search (0: default)")
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
                     help="contractions: 0 fp32 (plain form), 1 split fp16 on MFMA")
     ap.add_argument("--compare-forms", type=int, default=1,
@@ -196,6 +203,11 @@ def main():
     ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P,
                   precision=args.precision)
     tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
+    if args.nn_lanes:
+        t = ctx.tuning()
+        ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], args.nn_lanes,
+                       t["fused_reg"], t["so3_reg"])
+        tuning = ctx.tuning()
     d_src = [ctx.device_array(s) for s, _, _, _ in pairs]
     d_tgt = [ctx.device_array(t) for _, t, _, _ in pairs]
     Ps = [len(s) for s, _, _, _ in pairs]
@@ -235,7 +247,8 @@ def main():
     st = ctx.stage_times()
     ctx.set_profiling(False)
     plane = tuning["plane_fwd"] == 1
-    kb = kernel_bytes(B, N, P_mean, P, plane=plane)
+    kb = kernel_bytes(B, N, P_mean, P, plane=plane, fused_reg=tuning["fused_reg"] == 1,
+                      lines_fwd=tuning["fwd_z"] == 1, lines_inv=tuning["inv_x"] == 1)
     dom = max((k for k in kb if st.get(k, (0, 0))[1] > 0), key=lambda k: st[k][0])
     per_call = dict(LAUNCHES, spatial_fft_fwd=1 if plane else 2)
     launches = st[dom][1] * per_call.get(dom, 1)

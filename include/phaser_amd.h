@@ -151,8 +151,17 @@ typedef struct {
   int32_t fwd_x, fwd_z, inv_x;   /* 0 = default */
   int32_t plane_fwd;   /* first forward pass on whole z planes in LDS (N <= 128):
                           1 = on, 0 or -1 = off (default) */
+  int32_t nn_lanes;    /* lanes per query of the S^2 nearest-neighbour search:
+                          2, 4, 8 or 16 (0 = 4); the search is exact for every value */
+  int32_t fused_reg;   /* the r2c, cross-spectrum and c2r passes on the register
+                          four-step kernels (N = 64, 128, 256): 1 = on, 0/-1 = off */
+  int32_t so3_reg;     /* the SO(3) alpha and gamma passes on the register kernels
+                          (B = 32, 64, 128): 1 = on, 0/-1 = off */
+  int32_t reserved;
   float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
   float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
+  float ms_fused;      /* the three fused passes at max_batch, chosen form */
+  float ms_so3;        /* SO(3) alpha + gamma at max_batch, chosen form */
 } phz_tuning;
 /* Times every tile of each pass, and the plane pass against the r2c + x
  * passes it replaces, on the context's buffers at max_batch (repeats

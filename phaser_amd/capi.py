@@ -85,8 +85,11 @@ EXPORTS = {
 
 class Tuning(C.Structure):
     _fields_ = [("fwd_x", C.c_int32), ("fwd_z", C.c_int32), ("inv_x", C.c_int32),
-                ("plane_fwd", C.c_int32), ("ms_fwd_x", C.c_float), ("ms_fwd_z", C.c_float),
-                ("ms_inv_x", C.c_float), ("ms_plane", C.c_float)]
+                ("plane_fwd", C.c_int32), ("nn_lanes", C.c_int32), ("fused_reg", C.c_int32),
+                ("so3_reg", C.c_int32), ("reserved", C.c_int32),
+                ("ms_fwd_x", C.c_float), ("ms_fwd_z", C.c_float),
+                ("ms_inv_x", C.c_float), ("ms_plane", C.c_float), ("ms_fused", C.c_float),
+                ("ms_so3", C.c_float)]
 
     def as_dict(self):
         return {k: (round(getattr(self, k), 4) if k.startswith("ms") else getattr(self, k))
@@ -283,8 +286,9 @@ class Context:
         _check(lib().phz_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, fwd_x=0, fwd_z=0, inv_x=0, plane_fwd=0):
-        t = Tuning(fwd_x, fwd_z, inv_x, plane_fwd)
+    def set_tuning(self, fwd_x=0, fwd_z=0, inv_x=0, plane_fwd=0, nn_lanes=0, fused_reg=0,
+                   so3_reg=0):
+        t = Tuning(fwd_x, fwd_z, inv_x, plane_fwd, nn_lanes, fused_reg, so3_reg)
         _check(lib().phz_set_tuning(self.h, C.byref(t)))
 
     def set_profiling(self, on=True):

@@ -48,6 +48,7 @@ struct phz_ctx {
   FftPlan plan_n{}, plan_N{};
   SpatialTiles tiles{};
   bool plane_fwd = false;  // whole-plane first forward pass
+  bool so3_reg = false;    // SO(3) alpha/gamma passes on the register engine
   phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
@@ -333,6 +334,11 @@ int init_ctx(phz_ctx* c) {
   c->tuning.fwd_x = c->tuning.fwd_z = c->tuning.inv_x = fft_cols_default_tile(c->N);
   c->plane_fwd = false;  // measured slower at N = 128 (one workgroup per CU)
   c->tuning.plane_fwd = -1;
+  c->tuning.nn_lanes = 4;  // measured fastest of 4, 8, 16 at B = 64
+  c->tiles.fused_reg = false;
+  c->tuning.fused_reg = -1;
+  c->so3_reg = false;
+  c->tuning.so3_reg = -1;
   return 0;
 }
 
@@ -359,7 +365,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kS2NN, s);
-    launch_s2_nn(c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq, c->cells, c->d_grids, s);
+    launch_s2_nn(c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq, c->cells, c->d_grids,
+                 c->tuning.nn_lanes, s);
   }
   {
     StageScope t(c, kS2Transform, s);
@@ -378,12 +385,12 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kSo3Alpha, s);
-    launch_so3_fft_alpha(B, np, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
+    launch_so3_fft_alpha(B, np, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s, c->so3_reg);
   }
   {
     StageScope t(c, kSo3Gamma, s);
     launch_so3_fft_gamma(B, np, c->d_U, c->plan_n, c->d_tw_n, d_rot_corr, nq * n, c->d_partials,
-                         c->d_keys, s);
+                         c->d_keys, s, c->so3_reg);
   }
   {
     StageScope t(c, kRotPose, s);
@@ -403,6 +410,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     if (c->plane_fwd)
       launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                                c->d_spec, s);
+    else if (c->tiles.fused_reg)
+      launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_tw_N, c->d_spec, s);
     else
       launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                          c->d_spec, s);
@@ -428,8 +437,12 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     const int64_t fs = shared ? 0 : 2 * c->H, gs = 2 * c->H;
     {
       StageScope t(c, kSpatialCrossZ, s);
-      launch_spatial_cross_z(N, np, channels(F, F + c->H, 4), fs, channels(G, G + c->H, 4), gs,
-                             c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work, s);
+      if (c->tiles.fused_reg)
+        launch_spatial_cross_z_reg(N, np, channels(F, F + c->H, 4), fs, channels(G, G + c->H, 4),
+                                   gs, c->vox.runs, true, c->d_tw_N, c->d_work, s);
+      else
+        launch_spatial_cross_z(N, np, channels(F, F + c->H, 4), fs, channels(G, G + c->H, 4), gs,
+                               c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work, s);
     }
     {
       StageScope t(c, kSpatialFftInv, s);
@@ -437,8 +450,12 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     }
     {
       StageScope t(c, kSpatialC2r, s);
-      launch_spatial_c2r(N, np, c->d_work, c->plan_N, c->d_tw_N, d_trans_corr, c->d_partials,
-                         c->d_keys + np, s);
+      if (c->tiles.fused_reg)
+        launch_spatial_c2r_reg(N, np, c->d_work, c->d_tw_N, d_trans_corr, c->d_partials,
+                               c->d_keys + np, s);
+      else
+        launch_spatial_c2r(N, np, c->d_work, c->plan_N, c->d_tw_N, d_trans_corr, c->d_partials,
+                           c->d_keys + np, s);
     }
   }
   {
@@ -659,9 +676,9 @@ static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, floa
   const int B = c->B, n = c->n;
   launch_so3_synth(B, d_f, 0, d_h, 0, 1, c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S,
                    c->cfg.precision, s);
-  launch_so3_fft_alpha(B, 1, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s);
+  launch_so3_fft_alpha(B, 1, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s, c->so3_reg);
   launch_so3_fft_gamma(B, 1, c->d_U, c->plan_n, c->d_tw_n, corr ? c->d_corr_rot : nullptr,
-                       (int64_t)n * n * n, c->d_partials, c->d_keys, s);
+                       (int64_t)n * n * n, c->d_partials, c->d_keys, s, c->so3_reg);
   PHZ_HIP(hipGetLastError());
   unsigned long long key = 0;
   PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
@@ -890,6 +907,12 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.inv_x = c->tiles.inv_x ? c->tiles.inv_x : fft_cols_default_tile(c->N);
   c->plane_fwd = t.plane_fwd > 0 && spatial_plane_ok(c->N);
   c->tuning.plane_fwd = c->plane_fwd ? 1 : -1;
+  c->tuning.nn_lanes =
+      (t.nn_lanes == 2 || t.nn_lanes == 4 || t.nn_lanes == 8 || t.nn_lanes == 16) ? t.nn_lanes : 4;
+  c->tiles.fused_reg = t.fused_reg > 0 && spatial_reg_ok(c->N);
+  c->tuning.fused_reg = c->tiles.fused_reg ? 1 : -1;
+  c->so3_reg = t.so3_reg > 0 && so3_reg_ok(c->B);
+  c->tuning.so3_reg = c->so3_reg ? 1 : -1;
 }
 
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {
@@ -966,12 +989,73 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   } else {
     best.plane_fwd = -1;
   }
+  // the fused passes (r2c, cross_z, c2r): register engine against LDS kernels
+  float fused_ms[2] = {0.f, 0.f};
+  if (!rc && spatial_reg_ok(N)) {
+    const int ncl = c->max_clouds;
+    const float2* F = c->d_spec;
+    const float2* G = c->d_spec + (int64_t)2 * c->H;
+    for (int variant = 0; variant < 2 && !rc; ++variant) {
+      float ms = 0.f;
+      for (int pass = 0; pass < 2; ++pass) {
+        if (hipEventRecord(a, s) != hipSuccess) rc = -2;
+        for (int r = 0; r < repeats; ++r) {
+          if (variant == 1) {
+            launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_tw_N, c->d_spec, s);
+            launch_spatial_cross_z_reg(N, P, channels(F, F + c->H, 4), 0, channels(G, G + c->H, 4),
+                                       0, c->vox.runs, true, c->d_tw_N, c->d_work, s);
+            launch_spatial_c2r_reg(N, P, c->d_work, c->d_tw_N, nullptr, c->d_partials,
+                                   c->d_keys, s);
+          } else {
+            launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
+                               c->d_spec, s);
+            launch_spatial_cross_z(N, P, channels(F, F + c->H, 4), 0, channels(G, G + c->H, 4), 0,
+                                   c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work, s);
+            launch_spatial_c2r(N, P, c->d_work, c->plan_N, c->d_tw_N, nullptr, c->d_partials,
+                               c->d_keys, s);
+          }
+        }
+        if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess)
+          rc = -2;
+      }
+      fused_ms[variant] = ms / (float)repeats;
+    }
+    best.fused_reg = fused_ms[1] <= fused_ms[0] ? 1 : -1;
+  } else {
+    best.fused_reg = -1;
+  }
+  // SO(3) alpha + gamma passes: register engine against LDS kernels
+  float so3_ms[2] = {0.f, 0.f};
+  if (!rc && so3_reg_ok(c->B)) {
+    const int B = c->B, n = c->n;
+    launch_fill_u32((uint32_t*)c->d_S, (size_t)2 * P * B * (2 * B - 1) * n, 0u, s);
+    for (int variant = 0; variant < 2 && !rc; ++variant) {
+      float ms = 0.f;
+      for (int pass = 0; pass < 2; ++pass) {
+        if (hipEventRecord(a, s) != hipSuccess) rc = -2;
+        for (int r = 0; r < repeats; ++r) {
+          launch_so3_fft_alpha(B, P, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s, variant == 1);
+          launch_so3_fft_gamma(B, P, c->d_U, c->plan_n, c->d_tw_n, nullptr, (int64_t)n * n * n,
+                               c->d_partials, c->d_keys, s, variant == 1);
+        }
+        if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+            hipEventElapsedTime(&ms, a, b) != hipSuccess)
+          rc = -2;
+      }
+      so3_ms[variant] = ms / (float)repeats;
+    }
+    best.so3_reg = so3_ms[1] <= so3_ms[0] ? 1 : -1;
+  } else {
+    best.so3_reg = -1;
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc) {
     set_error("autotune timing failed");
     return rc;
   }
+  best.nn_lanes = c->tuning.nn_lanes;  // data-dependent: set with phz_set_tuning
   best.ms_fwd_x = best_ms[0];
   best.ms_fwd_z = best_ms[1];
   best.ms_inv_x = best_ms[2];
@@ -980,6 +1064,8 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   c->tuning.ms_fwd_z = best_ms[1];
   c->tuning.ms_inv_x = best_ms[2];
   c->tuning.ms_plane = best.plane_fwd > 0 ? plane_ms : classic_ms;
+  c->tuning.ms_fused = best.fused_reg > 0 ? fused_ms[1] : fused_ms[0];
+  c->tuning.ms_so3 = best.so3_reg > 0 ? so3_ms[1] : so3_ms[0];
   if (out) *out = c->tuning;
   return 0;
 }

@@ -50,6 +50,27 @@ __host__ __device__ __forceinline__ float2 cscale(float2 a, float s) {
   return make_float2(a.x * s, a.y * s);
 }
 
+// ---------------------------------------------------------------- wave runs
+// Runs of equal keys in adjacent lanes of a 64-lane wave (every lane must
+// call; a negative key marks a lane that is its own run).  head_lane is the
+// run's first lane, len its length; lets one lane per run do the memory
+// atomic for the whole run.
+struct WaveRun {
+  int head_lane, len;
+  bool head;
+};
+__device__ __forceinline__ WaveRun wave_run(long long key) {
+  const int lane = threadIdx.x & 63;
+  const long long prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key || key < 0;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = (2ull << lane) - 1ull;  // lanes <= this one
+  const int hl = 63 - __clzll((long long)(heads & le));
+  const unsigned long long above = heads & ~le;
+  const int next = above ? __ffsll((long long)above) - 1 : 64;
+  return WaveRun{hl, next - hl, head};
+}
+
 // ---------------------------------------------------------------- argmax keys
 // key = (ordered value bits << 32) | (0xffffffff - index): atomicMax keeps the
 // largest value and, among equal values, the smallest index.

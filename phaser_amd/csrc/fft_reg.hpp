@@ -71,9 +71,70 @@ struct RegSplit {
   static constexpr int R2 = L / R1;
 };
 
+// As fft_four_step, with the results kept in registers:
+// out[j * R2 + k2] = X[k1 + R1 k2], k1 = n2 + R2 j.
+template <int DIR, int L>
+__device__ __forceinline__ void fft_four_step_reg(float2 (&v)[RegSplit<L>::R1], int n2,
+                                                  float2* lbuf, const float2* __restrict__ tw,
+                                                  float2 (&out)[RegSplit<L>::R1]) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
+  dft_reg<R1, DIR>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) {
+    float2 w = tw[n2 * k1];
+    if (DIR > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) lbuf[k1 * R2 + n2] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j) {
+    const int k1 = n2 + R2 * j;
+    float2 u[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) u[m] = lbuf[k1 * R2 + m];
+    dft_reg<R2, DIR>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) out[j * R2 + k2] = u[k2];
+  }
+}
+
+// The four-step transform of one line held by R2 threads of a workgroup.
+// On entry thread n2 holds v[n1] = x[n2 + R2 n1]; lbuf is the line's LDS
+// scratch (>= L entries).  store(k, X[k]) is called for k = k1 + R1 k2 with
+// k1 = n2 + R2 j, j < R1 / R2.  Contains one __syncthreads, so every thread
+// of the workgroup must call it.
+template <int DIR, int L, class Store>
+__device__ __forceinline__ void fft_four_step(float2 (&v)[RegSplit<L>::R1], int n2, float2* lbuf,
+                                              const float2* __restrict__ tw, Store store) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
+  dft_reg<R1, DIR>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) {
+    float2 w = tw[n2 * k1];  // n2 * k1 < L
+    if (DIR > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) lbuf[k1 * R2 + n2] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j) {
+    const int k1 = n2 + R2 * j;
+    float2 u[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) u[m] = lbuf[k1 * R2 + m];
+    dft_reg<R2, DIR>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) store(k1 + R1 * k2, u[k2]);
+  }
+}
+
 // Lines of length L (64, 128 or 256) of complex data: line q of batch z
 // starts at data + z*batch_stride + (q / inner) * L * inner + q % inner and
-// has stride `inner`.  kLines lines per workgroup, R2 threads per line.
+// has stride `inner`.  kLines lines per workgroup, R2 threads per line (the
+// line index runs fastest across threads, so strided lines load coalesced).
 // tw: exp(-2 pi i e / L), e < L (conjugated for DIR = +1).
 template <int DIR, int L, int kLines>
 __global__ void __launch_bounds__(kLines * RegSplit<L>::R2) k_fft_lines(
@@ -91,29 +152,9 @@ __global__ void __launch_bounds__(kLines * RegSplit<L>::R2) k_fft_lines(
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1)
     v[n1] = live ? base[(int64_t)(n2 + R2 * n1) * inner] : make_float2(0.f, 0.f);
-  dft_reg<R1, DIR>(v);
-#pragma unroll
-  for (int k1 = 1; k1 < R1; ++k1) {
-    float2 w = tw[n2 * k1];  // n2 * k1 < L
-    if (DIR > 0) w.y = -w.y;
-    v[k1] = cmul(v[k1], w);
-  }
-#pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) buf[c * S + k1 * R2 + n2] = v[k1];
-  __syncthreads();
-  // thread n2 now owns k1 = n2 + R2 * j, j < R1 / R2
-#pragma unroll
-  for (int j = 0; j < R1 / R2; ++j) {
-    const int k1 = n2 + R2 * j;
-    float2 u[R2];
-#pragma unroll
-    for (int m = 0; m < R2; ++m) u[m] = buf[c * S + k1 * R2 + m];
-    dft_reg<R2, DIR>(u);
-    if (live) {
-#pragma unroll
-      for (int k2 = 0; k2 < R2; ++k2) base[(int64_t)(k1 + R1 * k2) * inner] = u[k2];
-    }
-  }
+  fft_four_step<DIR, L>(v, n2, buf + c * S, tw, [&](int k, float2 x) {
+    if (live) base[(int64_t)k * inner] = x;
+  });
 }
 
 }  // namespace phz

@@ -34,8 +34,9 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
 // the two halves of launch_sample_s2: projection + cell sort, then the NN search
 void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
                      CellBuffers cb, hipStream_t s);
+// lanes per query: 2, 4, 8 or 16 (others: 4)
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
-                  float max_sq, CellBuffers cb, float* grids, hipStream_t s);
+                  float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
@@ -71,15 +72,17 @@ void launch_so3_synth_mfma(int bw, const float2* f, int64_t f_stride, const floa
                            int64_t h_stride, int npairs, const float* wig, const int64_t* wig_off,
                            const float* wig_norm, float2* S, hipStream_t s);
 // FFT over m1 -> alpha: U[k][m2][alpha] per pair
+// reg: register four-step kernels (bw = 32, 64, 128; see so3_reg_ok)
+bool so3_reg_ok(int bw);
 void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
-                          const float2* tw, float2* U, hipStream_t s);
+                          const float2* tw, float2* U, hipStream_t s, bool reg = false);
 // FFT over m2 -> gamma (Hermitian), |C|^2, optional grid write, argmax key
 // via per-workgroup partials (so3_gamma_blocks(bw) per pair)
 int so3_gamma_blocks(int bw);
 void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& plan,
                           const float2* tw, float* corr /*nullable*/, int64_t corr_stride,
                           unsigned long long* partials, unsigned long long* keys,
-                          hipStream_t s);
+                          hipStream_t s, bool reg = false);
 
 // Device-side pose from the rotation argmax: fills pose[pair] and the float
 // transform T[pair][16] (rotation-utils.cc:17-23,86-143,190-207).
@@ -109,6 +112,7 @@ void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int n
 // line, so every choice gives bit-identical spectra.
 struct SpatialTiles {
   int fwd_x = 0, fwd_z = 0, inv_x = 0;
+  bool fused_reg = false;  // r2c, cross_z and c2r passes on the register engine
 };
 // tile value kRegisterEngine selects the register four-step kernel
 // (fft_reg.hpp) for L = 64, 128, 256
@@ -128,6 +132,18 @@ void launch_spatial_r2c(int N, int nclouds, const float* sumI, const float* sumR
                         hipStream_t s);
 void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
                             const float2* tw, const SpatialTiles& tiles, hipStream_t s);
+// register-engine forms of the r2c, cross_z and c2r passes (fft_reg.hpp),
+// for N where spatial_reg_ok(N); same arguments as the LDS forms
+bool spatial_reg_ok(int N);
+void launch_spatial_r2c_reg(int N, int nclouds, const float* sumI, const float* sumR,
+                            const uint32_t* cnt, const float2* tw, float2* spec, hipStream_t s);
+void launch_spatial_cross_z_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                int64_t g_stride, const FusionRuns& runs, bool fuse,
+                                const float2* tw, float2* work, hipStream_t s);
+int spatial_c2r_reg_blocks(int N);
+void launch_spatial_c2r_reg(int N, int npairs, const float2* work, const float2* tw, float* corr,
+                            unsigned long long* partials, unsigned long long* keys,
+                            hipStream_t s);
 // plane form of launch_spatial_r2c + the x pass (whole z planes in LDS), for
 // N where spatial_plane_ok(N); follow with launch_spatial_fft_pass(which=1)
 bool spatial_plane_ok(int N);

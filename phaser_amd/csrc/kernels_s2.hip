@@ -43,8 +43,8 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
                           int* __restrict__ cell_count) {
   const int c = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npts[c]) return;
-  const float4 p = pts[c * cap + i];
+  const bool valid = i < npts[c];
+  const float4 p = valid ? pts[c * cap + i] : make_float4(0.f, 0.f, 1.f, 0.f);
   float px, py, pz, r;
   project_point(p.x, p.y, p.z, &px, &py, &pz, &r);
   // cell of the stored float point: nb colatitude bands, ns azimuth sectors
@@ -57,10 +57,13 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
   int sec = (int)floorf(ph * ((float)ns / (2.f * kPi)) + 0.5f);
   sec = sec >= ns ? sec - ns : (sec < 0 ? 0 : sec);
   const int cid = band * ns + sec;
+  // scans arrive in ring order: one count atomic per run of equal cells
+  const WaveRun run = wave_run(valid ? (long long)cid : -1ll - (threadIdx.x & 63));
+  if (!valid) return;
   proj[c * cap + i] = make_float4(px, py, pz, __int_as_float(i));
   val[c * cap + i] = make_float2(r, p.w);
   cell[c * cap + i] = cid;
-  atomicAdd(&cell_count[(int64_t)c * nb * ns + cid], 1);
+  if (run.head) atomicAdd(&cell_count[(int64_t)c * nb * ns + cid], run.len);
 }
 
 // exclusive scan of the n*n cell counts of a cloud (one 1024-thread
@@ -115,9 +118,16 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
                                float2* __restrict__ sorted_val) {
   const int c = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npts[c]) return;
-  const int cid = cell[c * cap + i];
-  const int pos = atomicAdd(&cursor[(int64_t)c * ncell + cid], 1);
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < npts[c];
+  const int cid = valid ? cell[c * cap + i] : 0;
+  // one cursor atomic per run of equal cells; members take consecutive slots
+  const WaveRun run = wave_run(valid ? (long long)cid : -1ll - lane);
+  int base = 0;
+  if (valid && run.head) base = atomicAdd(&cursor[(int64_t)c * ncell + cid], run.len);
+  base = __shfl(base, run.head_lane, 64);
+  if (!valid) return;
+  const int pos = base + (lane - run.head_lane);
   const int dst = cell_start[(int64_t)c * (ncell + 1) + cid] + pos;
   sorted[c * cap + dst] = proj[c * cap + i];
   sorted_val[c * cap + dst] = val[c * cap + i];
@@ -184,8 +194,9 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
 //   |q - p|^2 >= 2(1 - cos dth) + 2 sin(th_q) min sin(th_p) (1 - cos dphi),
 // exceeds the best distance found (with a float slack): points outside it are
 // provably farther.  Empty sectors are jumped over with the per-cell tables.
-// The eight lanes split each cell's points and agree on the best after it.
-constexpr int kNNLanes = 8;
+// The LANES lanes of a query split each cell's points and agree on the best
+// after it.
+template <int LANES>
 __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
                      int nb, int ns, float max_sq, const int* __restrict__ cell_start,
                      const int* __restrict__ band_start, const int* __restrict__ next_right,
@@ -193,8 +204,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids) {
   const int c = blockIdx.y;
-  const int sub = threadIdx.x & (kNNLanes - 1);
-  const int q = blockIdx.x * (blockDim.x / kNNLanes) + threadIdx.x / kNNLanes;
+  const int sub = threadIdx.x & (LANES - 1);
+  const int q = blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
   if (q >= nq) return;  // whole groups leave together
   const int qrow = q / n, qcol = q - qrow * n;
@@ -218,7 +229,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   bool up = true, down = true;
   auto scan_cell = [&](int cid) {
     const int e = st[cid + 1];
-    for (int p = st[cid] + sub; p < e; p += kNNLanes) {
+    for (int p = st[cid] + sub; p < e; p += LANES) {
       const float4 pv = sp[p];
       const float d = l2_simple(qv, pv);
       const int oi = __float_as_int(pv.w);
@@ -229,10 +240,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       }
     }
 #pragma unroll
-    for (int off = kNNLanes / 2; off > 0; off >>= 1) {
-      const float ob = __shfl_xor(best, off, kNNLanes);
-      const int oi = __shfl_xor(bidx, off, kNNLanes);
-      const int op = __shfl_xor(bpos, off, kNNLanes);
+    for (int off = LANES / 2; off > 0; off >>= 1) {
+      const float ob = __shfl_xor(best, off, LANES);
+      const int oi = __shfl_xor(bidx, off, LANES);
+      const int op = __shfl_xor(bpos, off, LANES);
       if (ob < best || (ob == best && oi < bidx)) {
         best = ob;
         bidx = oi;
@@ -422,7 +433,7 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
                       const float4* grid, const float* theta, float max_sq, CellBuffers cb,
                       float* grids, hipStream_t s) {
   launch_s2_cells(pts, npts, cap, nclouds, n, cb, s);
-  launch_s2_nn(cap, nclouds, n, grid, theta, max_sq, cb, grids, s);
+  launch_s2_nn(cap, nclouds, n, grid, theta, max_sq, cb, grids, 4, s);
 }
 
 void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
@@ -442,13 +453,27 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
 }
 
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
-                  float max_sq, CellBuffers cb, float* grids, hipStream_t s) {
+                  float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s) {
   const int nq = n * n;
-  const int qpb = 256 / kNNLanes;
+  const int L = (lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16) ? lanes : 4;
+  const int qpb = 256 / L;
   dim3 gq((unsigned)((nq + qpb - 1) / qpb), nclouds);
-  hipLaunchKernelGGL(k_nn, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
-                     cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.sorted_val, cap, grids);
+  if (L == 2)
+    hipLaunchKernelGGL(k_nn<2>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
+                       cb.sorted_val, cap, grids);
+  else if (L == 4)
+    hipLaunchKernelGGL(k_nn<4>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
+                       cb.sorted_val, cap, grids);
+  else if (L == 16)
+    hipLaunchKernelGGL(k_nn<16>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
+                       cb.sorted_val, cap, grids);
+  else if (L == 8)
+    hipLaunchKernelGGL(k_nn<8>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
+                       cb.sorted_val, cap, grids);
 }
 
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,

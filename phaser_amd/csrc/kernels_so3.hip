@@ -9,6 +9,7 @@
 // its result real; here the half m2 >= 0 is synthesised (with the reference's
 // own choice of which entries are direct and which are conjugate copies), the
 // transform over m1 is a complex FFT and the one over m2 a real-output FFT.
+#include "fft_reg.hpp"
 #include "kernels.hpp"
 
 namespace phz {
@@ -215,6 +216,110 @@ __global__ void k_so3_fft_gamma(int bw, const float2* __restrict__ U, FftPlan pl
                                  blockIdx.x] = best;
 }
 
+// Register four-step forms (n = 64, 128, 256) of the two passes above.
+// alpha: one line per beta index k (LINES consecutive k per workgroup), the
+// result transposed through LDS so that U rows are written contiguously.
+template <int L>
+__global__ void __launch_bounds__(256) k_so3_fft_alpha_reg(int bw, const float2* __restrict__ S,
+                                                           const float2* __restrict__ tw,
+                                                           float2* __restrict__ U) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, SS = L + 1;
+  __shared__ float2 buf[LINES * SS];
+  const int w1 = 2 * bw - 1;
+  const int m2 = blockIdx.x, k0 = blockIdx.y * LINES, pair = blockIdx.z;
+  const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
+  const float2* Sp = S + ((int64_t)pair * bw + m2) * w1 * L + k0 + c;
+  float2 v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    const int M1 = n2 + R2 * n1;  // m1 mod n; the Nyquist row m1 = -bw is zero
+    const int m1 = M1 < bw ? M1 : M1 - L;
+    v[n1] = (M1 == bw) ? make_float2(0.f, 0.f) : Sp[(int64_t)(m1 + bw - 1) * L];
+  }
+  float2 out[R1];
+  fft_four_step_reg<-1, L>(v, n2, buf + c * SS, tw, out);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) buf[c * SS + (n2 + R2 * j) + R1 * k2] = out[j * R2 + k2];
+  __syncthreads();
+  float2* Up = U + (int64_t)pair * L * bw * L;
+  for (int idx = threadIdx.x; idx < LINES * L; idx += blockDim.x) {
+    const int cc = idx / L, al = idx - cc * L;
+    Up[((int64_t)(k0 + cc) * bw + m2) * L + al] = buf[cc * SS + al];
+  }
+}
+
+// gamma: one line per pair of alpha values (Hermitian packing), |C|^2 and
+// the argmax from registers, the grid transposed through LDS for the write.
+template <int L>
+__global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2* __restrict__ U,
+                                                           const float2* __restrict__ tw,
+                                                           float* __restrict__ corr,
+                                                           int64_t corr_stride,
+                                                           unsigned long long* __restrict__ partials) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, SS = L + 1;
+  __shared__ float2 buf[LINES * SS];
+  const int a0 = blockIdx.x * 2 * LINES, k = blockIdx.y, pair = blockIdx.z;
+  const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
+  const int aa = a0 + 2 * c;  // alpha values aa and aa + 1
+  const float2* Up = U + ((int64_t)pair * L + k) * bw * L + aa;
+  float2 v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    const int m = n2 + R2 * n1;
+    float2 va, vb;
+    if (m == bw) {
+      va = vb = make_float2(0.f, 0.f);
+    } else {
+      const int mm = m < bw ? m : L - m;
+      const float4 ab = *(const float4*)(Up + (int64_t)mm * L);
+      va = make_float2(ab.x, ab.y);
+      vb = make_float2(ab.z, ab.w);
+      if (m == 0) {
+        va.y = 0.f;
+        vb.y = 0.f;
+      } else if (m > bw) {
+        va = cconj(va);
+        vb = cconj(vb);
+      }
+    }
+    v[n1] = make_float2(va.x - vb.y, va.y + vb.x);  // va + i vb
+  }
+  float2 out[R1];
+  fft_four_step_reg<-1, L>(v, n2, buf + c * SS, tw, out);
+  const float scale = (1.f / (float)L) * ((float)bw / 3.14159265358979323846f);
+  unsigned long long best = 0ull;
+  __syncthreads();
+  float* g2 = (float*)buf;  // 2 * LINES rows of L values, stride SS
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const int g = (n2 + R2 * j) + R1 * k2;
+      const float2 z = out[j * R2 + k2];
+      const float ca = z.x * scale, cb = z.y * scale;
+      const float va = ca * ca, vb = cb * cb;
+      const uint32_t ga = (uint32_t)(((int64_t)k * L + aa) * L + g);
+      const unsigned long long ka = make_key(va, ga), kb = make_key(vb, ga + (uint32_t)L);
+      best = ka > best ? ka : best;
+      best = kb > best ? kb : best;
+      g2[(2 * c) * SS + g] = va;
+      g2[(2 * c + 1) * SS + g] = vb;
+    }
+  best = block_max_key(best);  // contains a barrier
+  if (threadIdx.x == 0)
+    partials[(int64_t)pair * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = best;
+  if (corr) {
+    float* cp = corr + pair * corr_stride + ((int64_t)k * L + a0) * L;
+    for (int idx = threadIdx.x; idx < 2 * LINES * L; idx += blockDim.x) {
+      const int r = idx / L, g = idx - r * L;
+      cp[(int64_t)r * L + g] = g2[r * SS + g];
+    }
+  }
+}
+
 __device__ void mat4_mul_nc(const float* A, const float* B, float* R) {
 #pragma clang fp contract(off)
   for (int i = 0; i < 4; ++i)
@@ -304,9 +409,39 @@ void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const floa
                      f_coef, f_stride, h_coef, h_stride, wigner, wig_off_dev, wig_norm, S);
 }
 
+bool so3_reg_ok(int bw) { return bw == 32 || bw == 64 || bw == 128; }
+
+template <int L>
+static void so3_alpha_reg_t(int bw, int npairs, const float2* S, const float2* tw, float2* U,
+                            hipStream_t s) {
+  constexpr int LINES = 256 / RegSplit<L>::R2;
+  hipLaunchKernelGGL(k_so3_fft_alpha_reg<L>, dim3(bw, L / LINES, npairs), dim3(256), 0, s, bw, S,
+                     tw, U);
+}
+
+template <int L>
+static int so3_gamma_reg_blocks_t() {
+  constexpr int LINES = 256 / RegSplit<L>::R2;
+  return L / (2 * LINES) * L;
+}
+
+template <int L>
+static void so3_gamma_reg_t(int bw, int npairs, const float2* U, const float2* tw, float* corr,
+                            int64_t corr_stride, unsigned long long* partials, hipStream_t s) {
+  constexpr int LINES = 256 / RegSplit<L>::R2;
+  hipLaunchKernelGGL(k_so3_fft_gamma_reg<L>, dim3(L / (2 * LINES), L, npairs), dim3(256), 0, s,
+                     bw, U, tw, corr, corr_stride, partials);
+}
+
 void launch_so3_fft_alpha(int bw, int npairs, const float2* S, const FftPlan& plan,
-                          const float2* tw, float2* U, hipStream_t s) {
+                          const float2* tw, float2* U, hipStream_t s, bool reg) {
   const int n = 2 * bw;
+  if (reg && so3_reg_ok(bw)) {
+    if (n == 64) so3_alpha_reg_t<64>(bw, npairs, S, tw, U, s);
+    else if (n == 128) so3_alpha_reg_t<128>(bw, npairs, S, tw, U, s);
+    else so3_alpha_reg_t<256>(bw, npairs, S, tw, U, s);
+    return;
+  }
   if (n <= 128) {
     dim3 g(bw, (n + 15) / 16, npairs);
     hipLaunchKernelGGL(k_so3_fft_alpha<16>, g, dim3(256), 2 * 16 * (n + 1) * sizeof(float2), s,
@@ -326,8 +461,23 @@ int so3_gamma_blocks(int bw) {
 void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& plan,
                           const float2* tw, float* corr, int64_t corr_stride,
                           unsigned long long* partials, unsigned long long* keys,
-                          hipStream_t s) {
+                          hipStream_t s, bool reg) {
   const int n = 2 * bw;
+  if (reg && so3_reg_ok(bw)) {
+    int blocks;
+    if (n == 64) {
+      so3_gamma_reg_t<64>(bw, npairs, U, tw, corr, corr_stride, partials, s);
+      blocks = so3_gamma_reg_blocks_t<64>();
+    } else if (n == 128) {
+      so3_gamma_reg_t<128>(bw, npairs, U, tw, corr, corr_stride, partials, s);
+      blocks = so3_gamma_reg_blocks_t<128>();
+    } else {
+      so3_gamma_reg_t<256>(bw, npairs, U, tw, corr, corr_stride, partials, s);
+      blocks = so3_gamma_reg_blocks_t<256>();
+    }
+    launch_reduce_keys(partials, blocks, npairs, keys, s);
+    return;
+  }
   dim3 g((n + 15) / 16, n, npairs);
   const size_t sh = (16 * (bw + 1) + 2 * 8 * n) * sizeof(float2);
   hipLaunchKernelGGL(k_so3_fft_gamma, g, dim3(256), sh, s, bw, U, plan, tw, corr, corr_stride,

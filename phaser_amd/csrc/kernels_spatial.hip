@@ -43,7 +43,10 @@ __device__ __forceinline__ int histc_bin(float x, const float* E, int N) {
 }
 
 // kClear = false: accumulate (phase-aligner.cc:98-120); true: zero the same
-// voxels again after use.
+// voxels again after use.  Scans arrive in ring order, so neighbouring
+// points of a wave often share a voxel: runs of equal voxels among adjacent
+// lanes are summed with a segmented wave reduction and only the run's first
+// lane touches memory (one atomic per run instead of per point).
 template <bool kClear>
 __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __restrict__ npts,
                                 int64_t cap, const DevPose* __restrict__ pose, int pose_stride,
@@ -55,8 +58,13 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
   __syncthreads();
   const int c = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npts[c]) return;
-  float4 p = pts[c * cap + i];
+  const int lane = threadIdx.x & 63;
+  // every lane of the wave takes part in the reduction; lanes without a
+  // voxel carry a key no other lane has
+  int64_t key = -1 - (int64_t)lane;
+  float vi = 0.f, vr = 0.f;
+  if (i < npts[c]) {
+    float4 p = pts[c * cap + i];
   if (pose) {
     const float* T = pose[c * pose_stride].T;
     const float x = T[0] * p.x + T[1] * p.y + T[2] * p.z + T[3];
@@ -66,24 +74,47 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
     p.y = y;
     p.z = z;
   }
-  const uint32_t bx = (uint32_t)histc_bin(p.x, E, N);
-  const uint32_t by = (uint32_t)histc_bin(p.y, E, N);
-  const uint32_t bz = (uint32_t)histc_bin(p.z, E, N);
-  const uint32_t NN = (uint32_t)N;
-  const uint32_t lin = (bx * NN + by) + (NN * NN * bz);  // Sub2Ind in uint32
-  const uint32_t nf = NN * NN * NN;
-  if (lin >= nf) return;  // the reference skips lin > nf and overruns at nf
-  const size_t off = (size_t)c * nf + lin;
+    const uint32_t bx = (uint32_t)histc_bin(p.x, E, N);
+    const uint32_t by = (uint32_t)histc_bin(p.y, E, N);
+    const uint32_t bz = (uint32_t)histc_bin(p.z, E, N);
+    const uint32_t NN = (uint32_t)N;
+    const uint32_t lin = (bx * NN + by) + (NN * NN * bz);  // Sub2Ind in uint32
+    // the reference skips lin > nf and overruns at nf
+    if (lin < NN * NN * NN) {
+      key = (int64_t)c * NN * NN * NN + lin;
+      vi = p.w;
+      vr = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+    }
+  }
+  // runs of equal keys in adjacent lanes: head flags, segment ids, then a
+  // suffix sum within each segment that leaves the total in the head lane
+  const int64_t prev = __shfl_up(key, 1, 64);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const int seg = __popcll(heads & ((2ull << lane) - 1ull));
+  float n = (key >= 0) ? 1.f : 0.f;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float oi = __shfl_down(vi, d, 64), orr = __shfl_down(vr, d, 64),
+                on = __shfl_down(n, d, 64);
+    const int os = __shfl_down(seg, d, 64);
+    if (!kClear && lane + d < 64 && os == seg) {
+      vi += oi;
+      vr += orr;
+      n += on;
+    }
+  }
+  if (!head || key < 0) return;
+  const size_t off = (size_t)key;
   if (kClear) {
     sumI[off] = 0.f;
     sumR[off] = 0.f;
     cnt[off] = 0u;
     return;
   }
-  const float range = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
-  atomicAdd(&sumI[off], p.w);
-  atomicAdd(&sumR[off], range);
-  atomicAdd(&cnt[off], 1u);
+  atomicAdd(&sumI[off], vi);
+  atomicAdd(&sumR[off], vr);
+  atomicAdd(&cnt[off], (uint32_t)n);
 }
 
 __global__ void k_reduce_keys(const unsigned long long* __restrict__ partials, int count,
@@ -448,6 +479,192 @@ void launch_reduce_keys(const unsigned long long* partials, int count, int npair
 }
 
 int spatial_c2r_blocks(int N) { return (N * N + 15) / 16; }
+
+// ---------------------------------------------------------------- register
+// engine versions of the three fused passes (L = N in {64, 128, 256}); the
+// line transforms use fft_reg.hpp.
+
+// pass 1: rows (z, x) along y, contiguous; R2 threads per row, n2 fastest.
+template <int L>
+__global__ void __launch_bounds__(256) k_spat_r2c_reg(
+    const float* __restrict__ sumI, const float* __restrict__ sumR,
+    const uint32_t* __restrict__ cnt, int64_t cloud_stride, const float2* __restrict__ tw,
+    float2* __restrict__ spec, int64_t spec_stride0, int64_t spec_stride1) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
+  constexpr int h = L / 2 + 1;
+  __shared__ float2 buf[LINES * S];
+  const int ln = threadIdx.x / R2, n2 = threadIdx.x % R2;
+  const int row0 = blockIdx.x * LINES, c = blockIdx.y;  // L*L rows, a multiple of LINES
+  const int64_t off = c * cloud_stride + (int64_t)(row0 + ln) * L;
+  float2 v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    const int64_t o = off + n2 + R2 * n1;
+    const uint32_t k = cnt[o];  // empty voxels never touch the sums
+    v[n1] = k ? make_float2(sumI[o] / (float)k, sumR[o] / (float)k) : make_float2(0.f, 0.f);
+  }
+  float2 out[R1];
+  fft_four_step_reg<-1, L>(v, n2, buf + ln * S, tw, out);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) buf[ln * S + (n2 + R2 * j) + R1 * k2] = out[j * R2 + k2];
+  __syncthreads();
+  // split Z = I^ + i R^ into the two half spectra, coalesced over the rows
+  float2* o0 = spec + c * spec_stride0 + (int64_t)row0 * h;
+  float2* o1 = o0 + spec_stride1;
+  for (int idx = threadIdx.x; idx < LINES * h; idx += blockDim.x) {
+    const int r = idx / h, k = idx - r * h;
+    const float2 zk = buf[r * S + k];
+    const float2 zc = cconj(buf[r * S + (L - k) % L]);
+    const float2 sm = cadd(zk, zc), d = csub(zk, zc);
+    o0[idx] = make_float2(0.5f * sm.x, 0.5f * sm.y);
+    o1[idx] = make_float2(0.5f * d.y, -0.5f * d.x);
+  }
+}
+
+// inverse pass A: fused F, G -> C = F conj(G) -> inverse FFT along z, for
+// lines q = (x, k2) of stride inner = N h; pair blockIdx.y.
+template <int L>
+__global__ void __launch_bounds__(256) k_spat_cross_z_reg(
+    ChannelSet F, int64_t f_stride, ChannelSet G, int64_t g_stride, FusionRuns runs, int fuse,
+    const float2* __restrict__ tw, float2* __restrict__ work) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
+  constexpr int inner = L * (L / 2 + 1);
+  __shared__ float2 buf[LINES * S];
+  const int cl = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
+  const int q = blockIdx.x * LINES + cl;
+  const bool live = q < inner;
+  const int64_t fo = blockIdx.y * f_stride, go = blockIdx.y * g_stride;
+  float2* wp = work + blockIdx.y * (int64_t)L * inner + q;
+  float2 v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    float2 cz = make_float2(0.f, 0.f);
+    if (live) {
+      const int64_t idx = (int64_t)(n2 + R2 * n1) * inner + q;
+      float2 f, g;
+      if (fuse) {
+        int mode = 0;
+        for (int r = 0; r < runs.count; ++r)
+          if ((uint64_t)idx >= runs.start[r]) mode = (int)runs.mode[r];
+        f = fuse_at(F, mode, idx + fo);
+        g = fuse_at(G, mode, idx + go);
+      } else {
+        f = F.ch[0][idx + fo];
+        g = G.ch[0][idx + go];
+      }
+      cz.x = f.x * g.x - f.y * (-g.y);
+      cz.y = f.x * (-g.y) + f.y * g.x;
+    }
+    v[n1] = cz;
+  }
+  fft_four_step<+1, L>(v, n2, buf + cl * S, tw, [&](int k, float2 x) {
+    if (live) wp[(int64_t)k * inner] = x;
+  });
+}
+
+// last pass: c2r along y (real parts of k = 0, N/2, FFTW hc2r), two rows per
+// complex line, signed first-maximum argmax over x*N + y + N^2*z.
+template <int L>
+__global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__ work,
+                                                      const float2* __restrict__ tw,
+                                                      float* __restrict__ corr,
+                                                      unsigned long long* __restrict__ partials) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
+  constexpr int h = L / 2 + 1, rows = L * L;
+  __shared__ float2 buf[LINES * S];
+  const int ln = threadIdx.x / R2, n2 = threadIdx.x % R2;
+  const int ra = (blockIdx.x * LINES + ln) * 2;  // rows ra, ra + 1 (rows is a multiple)
+  work += blockIdx.y * (int64_t)rows * h;
+  if (corr) corr += blockIdx.y * (int64_t)rows * L;
+  partials += blockIdx.y * (int64_t)gridDim.x;
+  const float2* xa = work + (int64_t)ra * h;
+  const float2* xb = xa + h;
+  float2 v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    const int m = n2 + R2 * n1;
+    float2 va, vb;
+    if (m == 0 || 2 * m == L) {
+      va = make_float2(xa[m].x, 0.f);
+      vb = make_float2(xb[m].x, 0.f);
+    } else if (m < h) {
+      va = xa[m];
+      vb = xb[m];
+    } else {
+      va = cconj(xa[L - m]);
+      vb = cconj(xb[L - m]);
+    }
+    v[n1] = make_float2(va.x - vb.y, va.y + vb.x);
+  }
+  unsigned long long best = 0ull;
+  fft_four_step<+1, L>(v, n2, buf + ln * S, tw, [&](int y, float2 z) {
+    const uint32_t la = (uint32_t)((int64_t)ra * L + y), lb = la + (uint32_t)L;
+    if (corr) {
+      corr[la] = z.x;
+      corr[lb] = z.y;
+    }
+    const unsigned long long ka = make_key(z.x, la), kb = make_key(z.y, lb);
+    best = ka > best ? ka : best;
+    best = kb > best ? kb : best;
+  });
+  best = block_max_key(best);
+  if (threadIdx.x == 0) partials[blockIdx.x] = best;
+}
+
+bool spatial_reg_ok(int N) { return N == 64 || N == 128 || N == 256; }
+
+template <int L>
+static void launch_reg_passes_r2c(int nclouds, const float* sumI, const float* sumR,
+                                  const uint32_t* cnt, const float2* tw, float2* spec,
+                                  hipStream_t s) {
+  constexpr int LINES = 256 / RegSplit<L>::R2;
+  const int64_t H = (int64_t)L * L * (L / 2 + 1), T = (int64_t)L * L * L;
+  hipLaunchKernelGGL(k_spat_r2c_reg<L>, dim3(L * L / LINES, nclouds), dim3(256), 0, s, sumI, sumR,
+                     cnt, T, tw, spec, 2 * H, H);
+}
+
+void launch_spatial_r2c_reg(int N, int nclouds, const float* sumI, const float* sumR,
+                            const uint32_t* cnt, const float2* tw, float2* spec, hipStream_t s) {
+  if (N == 64) launch_reg_passes_r2c<64>(nclouds, sumI, sumR, cnt, tw, spec, s);
+  else if (N == 128) launch_reg_passes_r2c<128>(nclouds, sumI, sumR, cnt, tw, spec, s);
+  else launch_reg_passes_r2c<256>(nclouds, sumI, sumR, cnt, tw, spec, s);
+}
+
+template <int L>
+static void launch_cross_z_reg_t(int npairs, ChannelSet F, int64_t fs, ChannelSet G, int64_t gs,
+                                 const FusionRuns& runs, bool fuse, const float2* tw,
+                                 float2* work, hipStream_t s) {
+  constexpr int LINES = 256 / RegSplit<L>::R2, inner = L * (L / 2 + 1);
+  hipLaunchKernelGGL(k_spat_cross_z_reg<L>, dim3((inner + LINES - 1) / LINES, npairs), dim3(256),
+                     0, s, F, fs, G, gs, runs, (int)fuse, tw, work);
+}
+
+void launch_spatial_cross_z_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                int64_t g_stride, const FusionRuns& runs, bool fuse,
+                                const float2* tw, float2* work, hipStream_t s) {
+  if (N == 64) launch_cross_z_reg_t<64>(npairs, F, f_stride, G, g_stride, runs, fuse, tw, work, s);
+  else if (N == 128) launch_cross_z_reg_t<128>(npairs, F, f_stride, G, g_stride, runs, fuse, tw, work, s);
+  else launch_cross_z_reg_t<256>(npairs, F, f_stride, G, g_stride, runs, fuse, tw, work, s);
+}
+
+// rows / (2 rows per line * lines per workgroup), lines = 256 / R2
+int spatial_c2r_reg_blocks(int N) {
+  const int r2 = N / (N >= 128 ? 16 : 8);
+  return N * N / (2 * (256 / r2));
+}
+
+void launch_spatial_c2r_reg(int N, int npairs, const float2* work, const float2* tw, float* corr,
+                            unsigned long long* partials, unsigned long long* keys,
+                            hipStream_t s) {
+  const dim3 g(spatial_c2r_reg_blocks(N), npairs);
+  if (N == 64) hipLaunchKernelGGL(k_spat_c2r_reg<64>, g, dim3(256), 0, s, work, tw, corr, partials);
+  else if (N == 128) hipLaunchKernelGGL(k_spat_c2r_reg<128>, g, dim3(256), 0, s, work, tw, corr, partials);
+  else hipLaunchKernelGGL(k_spat_c2r_reg<256>, g, dim3(256), 0, s, work, tw, corr, partials);
+  launch_reduce_keys(partials, spatial_c2r_reg_blocks(N), npairs, keys, s);
+}
 
 // column tiles: IT columns per workgroup, two padded LDS line buffers
 int fft_cols_default_tile(int L) { return L <= 128 ? 16 : 8; }

@@ -3,9 +3,10 @@
 * kernel_stats_markdown(csv_path): the --kernel-trace --stats table
   (<dir>/<name>_kernel_stats.csv) as markdown, with readable kernel names.
 * traffic_summary(fetch_csv, write_csv, copy_bytes): per-kernel HBM bytes
-  per launch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE; both in KiB
-  per dispatch), each scaled by what the same counter reads on the streaming
-  copy kernel, whose true traffic is copy_bytes read and copy_bytes written.
+  per launch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE), each scaled
+  by what the same counter reads on the streaming copy kernel, whose true
+  traffic is copy_bytes read and copy_bytes written (on gfx950 one FETCH_SIZE
+  unit measured 2048 bytes and one WRITE_SIZE unit 1024 bytes this way).
 
 Usage:
   python -m phaser_amd.profiling stats <kernel_stats.csv>
@@ -57,10 +58,18 @@ def kernel_stats_markdown(path):
 
 
 def _per_kernel(path, counter):
-    vals = defaultdict(list)
+    """Counter values per kernel, keeping only the dispatches of the largest
+    grid size (the batched pipeline launches; a profiled bench run also holds
+    single-pair launches of the same kernels)."""
+    rows = defaultdict(list)
     for r in csv.DictReader(open(path)):
         if r.get("Counter_Name") == counter:
-            vals[short_name(r["Kernel_Name"])].append(float(r["Counter_Value"]))
+            rows[short_name(r["Kernel_Name"])].append((int(r["Grid_Size"]),
+                                                       float(r["Counter_Value"])))
+    vals = {}
+    for k, v in rows.items():
+        g = max(x[0] for x in v)
+        vals[k] = [x[1] for x in v if x[0] == g]
     return vals
 
 

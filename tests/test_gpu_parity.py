@@ -364,3 +364,36 @@ def test_mfma_batch_matches_fp32_batch():
     for a, b in zip(*out):
         assert a["rot_index"] == b["rot_index"]
         assert a["trans_index"] == b["trans_index"]
+
+
+@pytest.mark.parametrize("lanes", [2, 8, 16])
+def test_nn_lanes_give_identical_samples(lanes):
+    """Lanes per query change the work split of the exact search only."""
+    from phaser_amd.capi import Context
+    src, tgt, _, _ = _pair(9, dict(rings=128, azimuths=1024, overlap=0.7))
+    with Context(bandwidth=64, n_voxels=128, max_points=200000) as ctx:
+        ref = [ctx.sample_s2(cl) for cl in (src, tgt)]
+        a = ctx.register(tgt, src)
+        ctx.set_tuning(nn_lanes=lanes)
+        assert ctx.tuning()["nn_lanes"] == lanes
+        b = ctx.register(tgt, src)
+    assert a["rot_index"] == b["rot_index"] and a["rot_value"] == b["rot_value"]
+    assert a["trans_index"] == b["trans_index"]
+    for cl, r in zip((src, tgt), ref):
+        assert r.shape[0] == 2
+
+
+def test_register_fused_passes_match_lds_passes(ctx64):
+    """r2c, cross-spectrum + z and c2r on the register four-step kernels
+    against the LDS Stockham kernels: same bins, grids within FP32 rounding."""
+    src, tgt, _, _ = _pair(4, dict(rings=128, azimuths=1024, overlap=0.7))
+    ctx64.set_tuning(fused_reg=-1)
+    a = ctx64.register(tgt, src, grids=True)
+    ctx64.set_tuning(1, 1, 1, fused_reg=1)
+    t = ctx64.tuning()
+    assert t["fused_reg"] == 1 and t["fwd_x"] == 1
+    b = ctx64.register(tgt, src, grids=True)
+    ctx64.set_tuning()
+    assert a["rot_index"] == b["rot_index"]
+    assert a["trans_index"] == b["trans_index"]
+    assert _rel(b["trans_corr"], a["trans_corr"].astype(np.float64)) < 1e-5
