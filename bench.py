@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--mode", choices=["batch", "serve"], default="batch")
     ap.add_argument("--autotune", type=int, default=1, help="pick FFT column tiles first")
+    ap.add_argument("--graphs", type=int, default=1, help="replay the pipeline from HIP graphs")
     ap.add_argument("--nn-lanes", type=int, default=0, choices=[0, 2, 4, 8, 16],
                     help="lanes per query of the S2 nearest-neighbour search (0: default)")
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
@@ -203,6 +204,8 @@ def main():
     ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P,
                   precision=args.precision)
     tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
+    if args.graphs:
+        ctx.set_graphs(True)
     if args.nn_lanes:
         t = ctx.tuning()
         ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], args.nn_lanes,
@@ -264,6 +267,8 @@ def main():
                        precision=1 - args.precision)
         if args.autotune:
             ctx2.autotune(repeats=5)
+        if args.graphs:
+            ctx2.set_graphs(True)
         d_src2 = [ctx2.device_array(s) for s, _, _, _ in pairs]
         d_tgt2 = [ctx2.device_array(t) for _, t, _, _ in pairs]
 
@@ -346,7 +351,8 @@ def main():
         "config": {"workload": "B=64 S2 grid 128x128, SO(3) 128^3, voxels 128^3 over +-50 m, "
                                "128x1024-ray scans, target 70% overlap, noise 2 cm",
                    "pairs_per_step": P, "mean_points_per_cloud": round(P_mean),
-                   "form": FORM_NAMES[args.precision], "fft_column_tiles": tuning},
+                   "form": FORM_NAMES[args.precision], "fft_column_tiles": tuning,
+                   "hip_graphs": bool(args.graphs)},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "argmax_match": {"matched": match, "pairs": S, "rate": round(match / max(S, 1), 4)},

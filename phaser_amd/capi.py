@@ -72,6 +72,7 @@ EXPORTS = {
     "phz_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "phz_get_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "phz_set_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "phz_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "phz_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "phz_get_stage_times": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "phz_device_malloc": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
@@ -290,6 +291,9 @@ class Context:
                    so3_reg=0):
         t = Tuning(fwd_x, fwd_z, inv_x, plane_fwd, nn_lanes, fused_reg, so3_reg)
         _check(lib().phz_set_tuning(self.h, C.byref(t)))
+
+    def set_graphs(self, on=True):
+        _check(lib().phz_set_graphs(self.h, int(on)))
 
     def set_profiling(self, on=True):
         _check(lib().phz_set_profiling(self.h, int(on)))

@@ -97,6 +97,16 @@ struct phz_ctx {
   };
   bool profiling = false;
   std::vector<Timing> timings;
+  // captured pipelines (phz_set_graphs), keyed by batch shape and outputs
+  struct GraphEntry {
+    int np;
+    bool shared;
+    float* rot;
+    float* trans;
+    hipGraphExec_t exec;
+  };
+  bool use_graphs = false;
+  std::vector<GraphEntry> graphs;  // most recently built last
 };
 
 namespace {
@@ -198,8 +208,8 @@ size_t context_bytes(const phz_config& cf) {
   add(M * nq, sizeof(int));
   add(M * (nq + 1), sizeof(int));
   add(M * (n + 1), sizeof(int));
-  add(M * nq, sizeof(int));
-  add(M * nq, sizeof(int));
+  add(M * nq, sizeof(int4));
+  add(M * nq, sizeof(int4));
   add(M * cap, sizeof(float4));
   add(M * cap, sizeof(float2));
   add(M * cap, sizeof(float4));
@@ -467,6 +477,49 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   return 0;
 }
 
+void clear_graphs(phz_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
+// run_pairs through a captured graph when graphs are on (and stage timing is
+// off): one graph per (pairs, shared target, output grids); capture records
+// kernel launches only (fills are kernels, no memset nodes)
+int exec_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* rot, float* trans) {
+  if (!c->use_graphs || c->profiling) return run_pairs(c, np, shared, s, rot, trans);
+  for (const auto& g : c->graphs)
+    if (g.np == np && g.shared == shared && g.rot == rot && g.trans == trans) {
+      PHZ_HIP(hipGraphLaunch(g.exec, s));
+      return 0;
+    }
+  hipGraph_t graph = nullptr;
+  PHZ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = run_pairs(c, np, shared, s, rot, trans);
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) {
+    set_error(std::string("graph capture failed: ") + hipGetErrorString(e));
+    return -2;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) {
+    set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+    return -2;
+  }
+  if (c->graphs.size() >= 8) {
+    (void)hipGraphExecDestroy(c->graphs.front().exec);
+    c->graphs.erase(c->graphs.begin());
+  }
+  c->graphs.push_back({np, shared, rot, trans, exec});
+  PHZ_HIP(hipGraphLaunch(exec, s));
+  return 0;
+}
+
 void to_result(const DevPose& d, phz_result* r) {
   r->rot_index = d.rot_index;
   r->trans_index = d.trans_index;
@@ -547,6 +600,7 @@ int phz_create(const phz_config* cfg, phz_ctx** out) {
 void phz_destroy(phz_ctx* c) {
   if (!c) return;
   if (c->stream) hipStreamSynchronize(c->stream);
+  clear_graphs(c);
   for (auto& t : c->timings) {
     if (t.a) hipEventDestroy(t.a);
     if (t.b) hipEventDestroy(t.b);
@@ -570,8 +624,8 @@ int phz_register(phz_ctx* c, const float* target, int64_t Pt, const float* sourc
   TRY(load_cloud(c, 0, target, Pt, hipMemcpyHostToDevice, s));
   TRY(load_cloud(c, 1, source, Ps, hipMemcpyHostToDevice, s));
   TRY(commit_counts(c, 2, s));
-  TRY(run_pairs(c, 1, false, s, rot_corr ? c->d_corr_rot : nullptr,
-                trans_corr ? c->d_corr_trans : nullptr));
+  TRY(exec_pairs(c, 1, false, s, rot_corr ? c->d_corr_rot : nullptr,
+                 trans_corr ? c->d_corr_trans : nullptr));
   if (registered && Ps > 0) launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, c->d_reg, s);
   TRY(fetch_results(c, 1, result, s));
   const int64_t n3 = (int64_t)c->n * c->n * c->n;
@@ -596,7 +650,7 @@ int phz_register_pairs_device(phz_ctx* c, int n, int shared_target, const float*
   for (int p = 0; p < n; ++p)
     TRY(load_cloud(c, nt + p, d_sources[p], Ps[p], hipMemcpyDeviceToDevice, s));
   TRY(commit_counts(c, nt + n, s));
-  TRY(run_pairs(c, n, shared_target != 0, s, d_rot_corr, d_trans_corr));
+  TRY(exec_pairs(c, n, shared_target != 0, s, d_rot_corr, d_trans_corr));
   return fetch_results(c, n, results, s);
 }
 
@@ -611,7 +665,7 @@ int phz_register_pairs(phz_ctx* c, int n, int shared_target, const float* const*
   for (int p = 0; p < n; ++p)
     TRY(load_cloud(c, nt + p, sources[p], Ps[p], hipMemcpyHostToDevice, s));
   TRY(commit_counts(c, nt + n, s));
-  TRY(run_pairs(c, n, shared_target != 0, s, nullptr, nullptr));
+  TRY(exec_pairs(c, n, shared_target != 0, s, nullptr, nullptr));
   return fetch_results(c, n, results, s);
 }
 
@@ -636,7 +690,7 @@ int phz_register_device(phz_ctx* c, const float* d_target, int64_t Pt, const flo
   TRY(load_cloud(c, 0, d_target, Pt, hipMemcpyDeviceToDevice, s));
   TRY(load_cloud(c, 1, d_source, Ps, hipMemcpyDeviceToDevice, s));
   TRY(commit_counts(c, 2, s));
-  TRY(run_pairs(c, 1, false, s, d_rot_corr, d_trans_corr));
+  TRY(exec_pairs(c, 1, false, s, d_rot_corr, d_trans_corr));
   if (d_registered && Ps > 0)
     launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, (float4*)d_registered, s);
   return fetch_results(c, 1, result, s);
@@ -918,6 +972,14 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {
   PHZ_CHECK(c && t, "null argument");
   apply_tuning(c, *t);
+  clear_graphs(c);  // captured kernels follow the old choices
+  return 0;
+}
+
+int phz_set_graphs(phz_ctx* c, int enable) {
+  PHZ_CHECK(c, "null argument");
+  if (!enable) clear_graphs(c);
+  c->use_graphs = enable != 0;
   return 0;
 }
 
@@ -1060,6 +1122,7 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   best.ms_fwd_z = best_ms[1];
   best.ms_inv_x = best_ms[2];
   apply_tuning(c, best);
+  clear_graphs(c);
   c->tuning.ms_fwd_x = best_ms[0];
   c->tuning.ms_fwd_z = best_ms[1];
   c->tuning.ms_inv_x = best_ms[2];

@@ -19,8 +19,9 @@ struct CellBuffers {
   int* cell_count;   // nclouds * n*n
   int* cell_start;   // nclouds * (n*n + 1)
   int* band_start;   // nclouds * (n + 1)
-  int* next_right;   // nclouds * n*n: nearest non-empty sector to the right
-  int* next_left;    // nclouds * n*n: nearest non-empty sector to the left
+  int4* next_right;  // nclouds * n*n: nearest non-empty sector to the right and
+                     // its point range {sector, start, end, 0} (-1: empty band)
+  int4* next_left;   // nclouds * n*n: the same to the left
   float4* sorted;    // nclouds * cap: projected x,y,z + original index bits
   float2* sorted_val;  // nclouds * cap: (range, intensity)
   float4* proj;      // nclouds * cap

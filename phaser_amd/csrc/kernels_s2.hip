@@ -137,13 +137,13 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
 // and to the left (<= s), cyclic; -1 in an empty band.  One wave per band:
 // the occupancy of the band's n <= 256 sectors is a 4 x 64-bit ballot mask.
 __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
-                              int* __restrict__ nr, int* __restrict__ nl) {
+                              int4* __restrict__ nr, int4* __restrict__ nl) {
   // n = sectors per band here
   const int b = blockIdx.x, c = blockIdx.y;
   const int lane = threadIdx.x;  // 64 lanes
   const int* st = cell_start + (int64_t)c * (nb * n + 1) + (int64_t)b * n;
-  int* r = nr + (int64_t)c * nb * n + (int64_t)b * n;
-  int* l = nl + (int64_t)c * nb * n + (int64_t)b * n;
+  int4* r = nr + (int64_t)c * nb * n + (int64_t)b * n;
+  int4* l = nl + (int64_t)c * nb * n + (int64_t)b * n;
   unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
   const int nw = (n + 63) / 64;
   for (int w = 0; w < nw; ++w) {
@@ -172,8 +172,9 @@ __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
         if (mm) left = w * 64 + 63 - __builtin_clzll(mm);
       }
     }
-    r[s] = right;
-    l[s] = left;
+    // the target sector's point range rides along: one dependent load less
+    r[s] = right < 0 ? make_int4(-1, 0, 0, 0) : make_int4(right, st[right], st[right + 1], 0);
+    l[s] = left < 0 ? make_int4(-1, 0, 0, 0) : make_int4(left, st[left], st[left + 1], 0);
   }
 }
 
@@ -199,14 +200,21 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
 template <int LANES>
 __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
                      int nb, int ns, float max_sq, const int* __restrict__ cell_start,
-                     const int* __restrict__ band_start, const int* __restrict__ next_right,
-                     const int* __restrict__ next_left, const float4* __restrict__ sorted,
+                     const int* __restrict__ band_start, const int4* __restrict__ next_right,
+                     const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids) {
   const int c = blockIdx.y;
   const int sub = threadIdx.x & (LANES - 1);
   const int q = blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
+  // min sin(theta) over each band (nb <= 256), once per workgroup
+  __shared__ float band_sin_min[256];
+  for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
+    const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
+    band_sin_min[bb] = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
+  }
+  __syncthreads();
   if (q >= nq) return;  // whole groups leave together
   const int qrow = q / n, qcol = q - qrow * n;
   const float4 qv = grid[q];
@@ -215,8 +223,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   const float sq = sinf(thq);
   const int* st = cell_start + (int64_t)c * (ncell + 1);
   const int* bs = band_start + (int64_t)c * (nb + 1);
-  const int* nr = next_right + (int64_t)c * ncell;
-  const int* nl = next_left + (int64_t)c * ncell;
+  const int4* nr = next_right + (int64_t)c * ncell;
+  const int4* nl = next_left + (int64_t)c * ncell;
   const float4* sp = sorted + c * cap;
   const float dband = kPi / nb, dsec = 2.f * kPi / ns;
   int qi = (int)(thq / dband);
@@ -227,9 +235,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   float best = INFINITY;
   int bidx = 0x7fffffff, bpos = -1;
   bool up = true, down = true;
-  auto scan_cell = [&](int cid) {
-    const int e = st[cid + 1];
-    for (int p = st[cid] + sub; p < e; p += LANES) {
+  auto scan_range = [&](int s0, int e) {
+    for (int p = s0 + sub; p < e; p += LANES) {
       const float4 pv = sp[p];
       const float d = l2_simple(qv, pv);
       const int oi = __float_as_int(pv.w);
@@ -251,10 +258,14 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       }
     }
   };
+  // pruning threshold: squared (distance + slack) plus kBoundErr, the
+  // largest error of the bounds below as computed with the hardware sine
+  // (v_sin_f32, absolute error below 2^-20), so pruning stays exact
+  constexpr float kBoundErr = 3e-5f;
   auto thr2 = [&]() {
     if (bpos < 0) return INFINITY;
     const float t = sqrtf(best) + slack;
-    return t * t;
+    return t * t + kBoundErr;
   };
   for (int t = 0; t < 2 * nb && (up || down); ++t) {
     int b;
@@ -277,7 +288,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     }
     const float lo = b * dband - eps, hi = (b + 1) * dband + eps;
     const float dth = fmaxf(0.f, fmaxf(lo - thq, thq - hi));
-    const float s1 = sinf(0.5f * dth);
+    const float s1 = __sinf(0.5f * dth);
     const float A = 4.f * s1 * s1;
     if (A > thr2()) {  // every farther band in this direction is farther still
       if (t == 0)
@@ -289,34 +300,43 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       continue;
     }
     if (bs[b + 1] == bs[b]) continue;
-    const float spm = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
+    const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
-    scan_cell(b * ns + qj);
-    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
-    for (int side = 0; side < 2; ++side) {
-      int u_prev = 0;
-      while (true) {
-        int sec, u;
-        if (side == 0) {
-          sec = nr[b * ns + (qj + u_prev + 1) % ns];
-          if (sec < 0) break;
-          u = (sec - qj + ns) % ns;
-          if (u <= u_prev || u > ns / 2) break;
+    scan_range(st[b * ns + qj], st[b * ns + qj + 1]);
+    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once).
+    // The two walks alternate so that their table loads are in flight together.
+    int ur = 0, ul = 0;
+    bool rdone = false, ldone = false;
+    auto bound = [&](int sec) {
+      // azimuth distance from the query to the sector's interval, in
+      // [0, pi], where 1 - cos is increasing: the bound grows with u
+      float d = fabsf(phq - sec * dsec);
+      if (d > kPi) d = 2.f * kPi - d;
+      const float dphi = fmaxf(d - 0.5f * dsec - eps, 0.f);
+      const float s2 = __sinf(0.5f * dphi);
+      return A + Bc * 2.f * s2 * s2;
+    };
+    while (!rdone || !ldone) {
+      const int4 jr = rdone ? make_int4(-1, 0, 0, 0) : nr[b * ns + (qj + ur + 1) % ns];
+      const int4 jl = ldone ? make_int4(-1, 0, 0, 0)
+                            : nl[b * ns + ((qj - ul - 1) % ns + ns) % ns];
+      if (!rdone) {
+        const int u = (jr.x - qj + ns) % ns;
+        if (jr.x < 0 || u <= ur || u > ns / 2 || bound(jr.x) > thr2()) {
+          rdone = true;
         } else {
-          sec = nl[b * ns + ((qj - u_prev - 1) % ns + ns) % ns];
-          if (sec < 0) break;
-          u = (qj - sec + ns) % ns;
-          if (u <= u_prev || u > (ns - 1) / 2) break;
+          scan_range(jr.y, jr.z);
+          ur = u;
         }
-        // azimuth distance from the query to the sector's interval, in
-        // [0, pi], where 1 - cos is increasing: the bound grows with u
-        float d = fabsf(phq - sec * dsec);
-        if (d > kPi) d = 2.f * kPi - d;
-        const float dphi = fmaxf(d - 0.5f * dsec - eps, 0.f);
-        const float s2 = sinf(0.5f * dphi);
-        if (A + Bc * 2.f * s2 * s2 > thr2()) break;
-        scan_cell(b * ns + sec);
-        u_prev = u;
+      }
+      if (!ldone) {
+        const int u = (qj - jl.x + ns) % ns;
+        if (jl.x < 0 || u <= ul || u > (ns - 1) / 2 || bound(jl.x) > thr2()) {
+          ldone = true;
+        } else {
+          scan_range(jl.y, jl.z);
+          ul = u;
+        }
       }
     }
   }

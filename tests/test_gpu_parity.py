@@ -397,3 +397,57 @@ def test_register_fused_passes_match_lds_passes(ctx64):
     assert a["rot_index"] == b["rot_index"]
     assert a["trans_index"] == b["trans_index"]
     assert _rel(b["trans_corr"], a["trans_corr"].astype(np.float64)) < 1e-5
+
+
+def test_so3_register_engine_matches_oracle(po):
+    """SO(3) alpha/gamma passes on the register four-step kernels."""
+    from phaser_amd.capi import Context
+    rng = np.random.default_rng(21)
+    f = po.fst(32, rng.normal(size=64 * 64))
+    h = po.fst(32, rng.normal(size=64 * 64))
+    mag = po.so3_correlate(32, f, h)
+    with Context(bandwidth=32, n_voxels=64) as ctx:
+        ctx.set_tuning(so3_reg=1)
+        assert ctx.tuning()["so3_reg"] == 1
+        corr, am = ctx.so3_correlate_coef(f.astype(np.complex64), h.astype(np.complex64))
+    assert _rel(corr, mag) < 2e-4
+    assert _argmax_ok(am, mag)
+
+
+def test_so3_register_engine_matches_lds_engine(ctx64):
+    src, tgt, _, _ = _pair(5, dict(rings=128, azimuths=1024, overlap=0.7))
+    ctx64.set_tuning(so3_reg=-1)
+    a = ctx64.register(tgt, src, grids=True)
+    ctx64.set_tuning(so3_reg=1)
+    assert ctx64.tuning()["so3_reg"] == 1
+    b = ctx64.register(tgt, src, grids=True)
+    ctx64.set_tuning()
+    assert a["rot_index"] == b["rot_index"]
+    assert _rel(b["rot_corr"], a["rot_corr"].astype(np.float64)) < 1e-5
+
+
+def test_graph_replays_match_eager():
+    """The pipeline replayed from a captured graph, several times with eager
+    work of the same context in between: every replay gives the eager result."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=64, azimuths=1024)) for s in range(4)]
+    with Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=4) as ctx:
+        dt = [ctx.device_array(t) for _, t, _, _ in pairs]
+        ds = [ctx.device_array(s) for s, _, _, _ in pairs]
+        Pt = [len(t) for _, t, _, _ in pairs]
+        Ps = [len(s) for s, _, _, _ in pairs]
+        eager = ctx.register_pairs_device([a.ptr for a in dt], Pt, [a.ptr for a in ds], Ps)
+        other = _pair(9, dict(rings=64, azimuths=1024))
+        other_eager = ctx.register(other[1], other[0])
+        ctx.set_graphs(True)
+        for rep in range(3):
+            got = ctx.register_pairs_device([a.ptr for a in dt], Pt, [a.ptr for a in ds], Ps)
+            for a, b in zip(eager, got):
+                assert a["rot_index"] == b["rot_index"] and a["rot_value"] == b["rot_value"]
+                assert a["trans_index"] == b["trans_index"]
+                assert abs(a["trans_value"] - b["trans_value"]) <= 1e-5 * abs(a["trans_value"])
+            # eager-sized work between replays: a single registration (its own graph)
+            o = ctx.register(other[1], other[0])
+            assert o["rot_index"] == other_eager["rot_index"]
+            assert o["trans_index"] == other_eager["trans_index"]
+        ctx.set_graphs(False)
