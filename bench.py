@@ -175,6 +175,8 @@ def main():
     ap.add_argument("--graphs", type=int, default=1, help="replay the pipeline from HIP graphs")
     ap.add_argument("--nn-lanes", type=int, default=0, choices=[0, 2, 4, 8, 16],
                     help="lanes per query of the S2 nearest-neighbour search (0: default)")
+    ap.add_argument("--occ-fwd", type=int, default=0, choices=[-1, 0, 1],
+                    help="forward spatial passes skip empty voxel rows/planes (0: default, on)")
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
                     help="contractions: 0 fp32 (plain form), 1 split fp16 on MFMA")
     ap.add_argument("--compare-forms", type=int, default=1,
@@ -206,10 +208,11 @@ def main():
     tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
     if args.graphs:
         ctx.set_graphs(True)
-    if args.nn_lanes:
+    if args.nn_lanes or args.occ_fwd:
         t = ctx.tuning()
-        ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], args.nn_lanes,
-                       t["fused_reg"], t["so3_reg"])
+        ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"],
+                       args.nn_lanes or t["nn_lanes"], t["fused_reg"], t["so3_reg"],
+                       args.occ_fwd or t["occ_fwd"])
         tuning = ctx.tuning()
     d_src = [ctx.device_array(s) for s, _, _, _ in pairs]
     d_tgt = [ctx.device_array(t) for _, t, _, _ in pairs]

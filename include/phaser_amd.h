@@ -157,7 +157,13 @@ typedef struct {
                           four-step kernels (N = 64, 128, 256): 1 = on, 0/-1 = off */
   int32_t so3_reg;     /* the SO(3) alpha and gamma passes on the register kernels
                           (B = 32, 64, 128): 1 = on, 0/-1 = off */
-  int32_t reserved;
+  int32_t occ_fwd;     /* with fused_reg and the register engine for the forward
+                          z pass: that pass runs inside the cross-spectrum
+                          kernel (its output is never stored), and with the
+                          register engine for the x pass too, the r2c, x and z
+                          passes read and write only the voxel rows and z
+                          planes that hold points (the rest of the transform
+                          is zero): 0 or 1 = on (default), -1 = off */
   float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
   float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
   float ms_fused;      /* the three fused passes at max_batch, chosen form */

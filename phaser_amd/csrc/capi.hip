@@ -49,6 +49,7 @@ struct phz_ctx {
   SpatialTiles tiles{};
   bool plane_fwd = false;  // whole-plane first forward pass
   bool so3_reg = false;    // SO(3) alpha/gamma passes on the register engine
+  bool occ_fwd = true;     // forward passes skip rows / planes without points
   phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
@@ -80,6 +81,8 @@ struct phz_ctx {
   float* d_sumI = nullptr;
   float* d_sumR = nullptr;
   uint32_t* d_cnt = nullptr;
+  uint8_t* d_row_occ = nullptr;    // [cloud][z][x], 1 where the row holds points
+  uint8_t* d_plane_occ = nullptr;  // [cloud][z]
   float2* d_spec = nullptr;
   float2* d_work = nullptr;
   float* d_corr_trans = nullptr;
@@ -230,6 +233,8 @@ size_t context_bytes(const phz_config& cf) {
   add(MV * T3, sizeof(float));
   add(M * T3, sizeof(float));
   add(M * T3, sizeof(uint32_t));
+  add((M * N * N + 3) / 4 * 4, 1);
+  add((M * N + 3) / 4 * 4, 1);
   add(G * H, sizeof(float2));
   add(P * H, sizeof(float2));
   add(T3, sizeof(float));
@@ -329,6 +334,10 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_sumI, (size_t)MV * c->T3));
   TRY(dalloc(c, &c->d_sumR, (size_t)M * c->T3));
   TRY(dalloc(c, &c->d_cnt, (size_t)M * c->T3));
+  const size_t occ_row_bytes = ((size_t)M * c->N * c->N + 3) / 4 * 4;
+  const size_t occ_plane_bytes = ((size_t)M * c->N + 3) / 4 * 4;
+  TRY(dalloc(c, &c->d_row_occ, occ_row_bytes));
+  TRY(dalloc(c, &c->d_plane_occ, occ_plane_bytes));
   TRY(dalloc(c, &c->d_spec, (size_t)(2 * M > 16 ? 2 * M : 16) * c->H));
   TRY(dalloc(c, &c->d_work, (size_t)P * c->H));
   TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3));
@@ -339,6 +348,8 @@ int init_ctx(phz_ctx* c) {
   launch_fill_u32((uint32_t*)c->d_sumI, (size_t)MV * c->T3, 0u, c->stream);
   launch_fill_u32((uint32_t*)c->d_sumR, (size_t)M * c->T3, 0u, c->stream);
   launch_fill_u32(c->d_cnt, (size_t)M * c->T3, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->d_row_occ, occ_row_bytes / 4, 0u, c->stream);
+  launch_fill_u32((uint32_t*)c->d_plane_occ, occ_plane_bytes / 4, 0u, c->stream);
   PHZ_HIP(hipStreamSynchronize(c->stream));
   PHZ_CHECK(c->alloc_bytes == context_bytes(cf), "internal error: device memory estimate is stale");
   c->tuning.fwd_x = c->tuning.fwd_z = c->tuning.inv_x = fft_cols_default_tile(c->N);
@@ -349,7 +360,17 @@ int init_ctx(phz_ctx* c) {
   c->tuning.fused_reg = -1;
   c->so3_reg = false;
   c->tuning.so3_reg = -1;
+  c->occ_fwd = true;
+  c->tuning.occ_fwd = 1;
   return 0;
+}
+
+// occupancy bytes of cloud slots [first, ...)
+VoxelOcc occ_from(phz_ctx* c, int first) {
+  VoxelOcc o;
+  o.row = c->d_row_occ + (int64_t)first * c->N * c->N;
+  o.plane = c->d_plane_occ + (int64_t)first * c->N;
+  return o;
 }
 
 ChannelSet channels(const float2* a, const float2* b, int C) {
@@ -410,49 +431,66 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     StageScope t(c, kVoxelScatter, s);
     launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
-                         c->d_sumR, c->d_cnt, s);
+                         c->d_sumR, c->d_cnt, occ_from(c, 0), s);
     launch_voxel_scatter(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
-                         c->d_cnt + nt * c->T3, s);
+                         c->d_cnt + nt * c->T3, occ_from(c, nt), s);
   }
+  // spatial order {intensity, range, refl=0, amb=0}; pair p: target p (or
+  // the shared target 0) against source nt+p
+  const float2* F = c->d_spec;
+  const float2* G = c->d_spec + (int64_t)(2 * nt) * c->H;
+  const ChannelSet Fc = channels(F, F + c->H, 4), Gc = channels(G, G + c->H, 4);
+  const int64_t fs = shared ? 0 : 2 * c->H, gs = 2 * c->H;
+  // zz: the forward z pass runs inside the cross-spectrum kernel; occ: the
+  // r2c and x passes before it also skip rows / planes without points
+  const bool zz = c->occ_fwd && c->tiles.fused_reg && c->tuning.fwd_z == kRegisterEngine &&
+                  spatial_cross_zz_ok(N, Fc, Gc);
+  const bool occ = zz && !c->plane_fwd && c->tuning.fwd_x == kRegisterEngine;
   {  // y (and with the plane pass also x) first, then the remaining columns
     StageScope t(c, kSpatialR2c, s);
     if (c->plane_fwd)
       launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                                c->d_spec, s);
     else if (c->tiles.fused_reg)
-      launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_tw_N, c->d_spec, s);
+      launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_row_occ, occ,
+                             c->d_tw_N, c->d_spec, s);
     else
       launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                          c->d_spec, s);
   }
   {
     StageScope t(c, kSpatialFftFwd, s);
-    if (!c->plane_fwd)
+    if (occ)
+      launch_spatial_fft_x_occ(N, 2 * ncl, c->d_spec, occ_from(c, 0), c->d_tw_N, s);
+    else if (!c->plane_fwd)
       launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
-    launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
+    if (!zz)
+      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
   }
   {
-    StageScope t(c, kVoxelClear, s);
-    launch_voxel_clear(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
-                       c->d_sumR, c->d_cnt, s);
-    launch_voxel_clear(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
-                       c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
-                       c->d_cnt + nt * c->T3, s);
-  }
-  {  // spatial order {intensity, range, refl=0, amb=0}; pair p: target p (or
-     // the shared target 0) against source nt+p
-    const float2* F = c->d_spec;
-    const float2* G = c->d_spec + (int64_t)(2 * nt) * c->H;
-    const int64_t fs = shared ? 0 : 2 * c->H, gs = 2 * c->H;
     {
       StageScope t(c, kSpatialCrossZ, s);
-      if (c->tiles.fused_reg)
-        launch_spatial_cross_z_reg(N, np, channels(F, F + c->H, 4), fs, channels(G, G + c->H, 4),
-                                   gs, c->vox.runs, true, c->d_tw_N, c->d_work, s);
+      if (zz)
+        launch_spatial_cross_zz_reg(N, np, Fc, fs, Gc, gs, occ ? c->d_plane_occ : nullptr,
+                                    shared ? 0 : N, occ ? c->d_plane_occ + (int64_t)nt * N : nullptr,
+                                    N, c->vox.runs, c->d_tw_N, c->d_work, s);
+      else if (c->tiles.fused_reg)
+        launch_spatial_cross_z_reg(N, np, Fc, fs, Gc, gs, c->vox.runs, true, c->d_tw_N, c->d_work,
+                                   s);
       else
-        launch_spatial_cross_z(N, np, channels(F, F + c->H, 4), fs, channels(G, G + c->H, 4), gs,
-                               c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work, s);
+        launch_spatial_cross_z(N, np, Fc, fs, Gc, gs, c->vox.runs, true, c->plan_N, c->d_tw_N,
+                               c->d_work, s);
+    }
+    // after the last reader of the occupancy bytes (the cross spectrum
+    // reads the plane bytes)
+    {
+      StageScope t(c, kVoxelClear, s);
+      launch_voxel_clear(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
+                         c->d_sumR, c->d_cnt, occ_from(c, 0), s);
+      launch_voxel_clear(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
+                         c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
+                         c->d_cnt + nt * c->T3, occ_from(c, nt), s);
     }
     {
       StageScope t(c, kSpatialFftInv, s);
@@ -782,7 +820,7 @@ int phz_voxelize(phz_ctx* c, const float* pts, int64_t P, float* out, float* cou
   TRY(load_cloud(c, 0, pts, P, hipMemcpyHostToDevice, s));
   TRY(commit_counts(c, 1, s));
   launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, 1, nullptr, 0, c->N, c->d_edges, c->d_sumI,
-                       c->d_sumR, c->d_cnt, s);
+                       c->d_sumR, c->d_cnt, occ_from(c, 0), s);
   PHZ_HIP(hipGetLastError());
   std::vector<float> sI(c->T3), sR(c->T3);
   std::vector<uint32_t> k(c->T3);
@@ -790,7 +828,7 @@ int phz_voxelize(phz_ctx* c, const float* pts, int64_t P, float* out, float* cou
   PHZ_HIP(hipMemcpyAsync(sR.data(), c->d_sumR, sizeof(float) * c->T3, hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipMemcpyAsync(k.data(), c->d_cnt, sizeof(uint32_t) * c->T3, hipMemcpyDeviceToHost, s));
   launch_voxel_clear(c->d_pts, c->d_npts, c->cap, 1, nullptr, 0, c->N, c->d_edges, c->d_sumI,
-                     c->d_sumR, c->d_cnt, s);
+                     c->d_sumR, c->d_cnt, occ_from(c, 0), s);
   PHZ_HIP(hipStreamSynchronize(s));
   for (int64_t v = 0; v < c->T3; ++v) {
     out[v] = k[v] ? sI[v] / (float)k[v] : 0.f;
@@ -967,6 +1005,8 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.fused_reg = c->tiles.fused_reg ? 1 : -1;
   c->so3_reg = t.so3_reg > 0 && so3_reg_ok(c->B);
   c->tuning.so3_reg = c->so3_reg ? 1 : -1;
+  c->occ_fwd = t.occ_fwd >= 0;
+  c->tuning.occ_fwd = c->occ_fwd ? 1 : -1;
 }
 
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {
@@ -1063,7 +1103,8 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
         if (hipEventRecord(a, s) != hipSuccess) rc = -2;
         for (int r = 0; r < repeats; ++r) {
           if (variant == 1) {
-            launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_tw_N, c->d_spec, s);
+            launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, nullptr, false,
+                                   c->d_tw_N, c->d_spec, s);
             launch_spatial_cross_z_reg(N, P, channels(F, F + c->H, 4), 0, channels(G, G + c->H, 4),
                                        0, c->vox.runs, true, c->d_tw_N, c->d_work, s);
             launch_spatial_c2r_reg(N, P, c->d_work, c->d_tw_N, nullptr, c->d_partials,

@@ -103,10 +103,18 @@ void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose
 // voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t s);
+// occupancy of the voxel grids, kept beside the accumulators by
+// launch_voxel_scatter (sets) and launch_voxel_clear (resets):
+// row[cloud][z*N + x] and plane[cloud][z] are 1 where some voxel of that
+// row / z plane holds a point, else 0 (null: not kept)
+struct VoxelOcc {
+  uint8_t* row = nullptr;
+  uint8_t* plane = nullptr;
+};
 void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
                           const DevPose* pose /*nullable: no transform*/, int pose_stride,
                           int N, const float* edges, float* sumI, float* sumR, uint32_t* cnt,
-                          hipStream_t s);
+                          VoxelOcc occ, hipStream_t s);
 // forward 3-D r2c of (mean intensity, mean range) per cloud:
 // columns per workgroup of the three column-FFT passes (0 = default);
 // chosen by phz_autotune.  The tile width never changes the arithmetic of a
@@ -136,11 +144,37 @@ void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
 // register-engine forms of the r2c, cross_z and c2r passes (fft_reg.hpp),
 // for N where spatial_reg_ok(N); same arguments as the LDS forms
 bool spatial_reg_ok(int N);
+// row_occ (nullable: every row read) skips the empty rows' loads; with
+// sparse_out their (zero) spectrum rows are not written either, and only
+// launch_spatial_fft_x_occ may read the spectra afterwards
 void launch_spatial_r2c_reg(int N, int nclouds, const float* sumI, const float* sumR,
-                            const uint32_t* cnt, const float2* tw, float2* spec, hipStream_t s);
+                            const uint32_t* cnt, const uint8_t* row_occ, bool sparse_out,
+                            const float2* tw, float2* spec, hipStream_t s);
+// the forward x pass (launch_spatial_fft_pass which = 0) on the register
+// engine over the spectra of nbatch / 2 clouds left by a sparse_out r2c:
+// reads only the rows that hold points (the others are zero and were never
+// written) and skips the lines of planes without points, which stay
+// unwritten (read them as zero).  spatial_reg_ok(N) only.
+void launch_spatial_fft_x_occ(int N, int nbatch, float2* spec, VoxelOcc occ, const float2* tw,
+                              hipStream_t s);
 void launch_spatial_cross_z_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
                                 int64_t g_stride, const FusionRuns& runs, bool fuse,
                                 const float2* tw, float2* work, hipStream_t s);
+// the forward z pass fused into the cross spectrum: F and G hold the
+// x-transformed spectra (launch_spatial_fft_x_occ, or launch_spatial_fft_pass
+// which = 0), each pair transforms its channels
+// along z in registers, fuses, multiplies and transforms back along z into
+// work, as launch_spatial_cross_z_reg does after a separate z pass.  f_occ /
+// g_occ (nullable: every plane read): the z-plane occupancy of pair 0's
+// clouds, advanced per pair by the given strides.  Only for channel sets
+// whose channels 0 and 1 are present and the rest absent
+// (spatial_cross_zz_ok).
+bool spatial_cross_zz_ok(int N, const ChannelSet& F, const ChannelSet& G);
+void launch_spatial_cross_zz_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                 int64_t g_stride, const uint8_t* f_occ, int64_t f_occ_stride,
+                                 const uint8_t* g_occ, int64_t g_occ_stride,
+                                 const FusionRuns& runs, const float2* tw, float2* work,
+                                 hipStream_t s);
 int spatial_c2r_reg_blocks(int N);
 void launch_spatial_c2r_reg(int N, int npairs, const float2* work, const float2* tw, float* corr,
                             unsigned long long* partials, unsigned long long* keys,
@@ -185,7 +219,7 @@ void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& pl
 // binning as launch_voxel_scatter), so they need no full clear before reuse
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
                         const DevPose* pose, int pose_stride, int N, const float* edges,
-                        float* sumI, float* sumR, uint32_t* cnt, hipStream_t s);
+                        float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s);
 void launch_trans_pose(int N, int lower, int upper, int npairs,
                        const unsigned long long* keys, DevPose* pose, hipStream_t s);
 // out = T_rot * p + translation (PCL scalar order), intensity kept

@@ -46,12 +46,14 @@ __device__ __forceinline__ int histc_bin(float x, const float* E, int N) {
 // voxels again after use.  Scans arrive in ring order, so neighbouring
 // points of a wave often share a voxel: runs of equal voxels among adjacent
 // lanes are summed with a segmented wave reduction and only the run's first
-// lane touches memory (one atomic per run instead of per point).
+// lane touches memory (one atomic per run instead of per point).  The same
+// lanes set (or reset) the occupancy bytes of the voxel's row and z plane.
 template <bool kClear>
 __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __restrict__ npts,
                                 int64_t cap, const DevPose* __restrict__ pose, int pose_stride,
                                 int N, const float* __restrict__ edges, float* __restrict__ sumI,
-                                float* __restrict__ sumR, uint32_t* __restrict__ cnt) {
+                                float* __restrict__ sumR, uint32_t* __restrict__ cnt,
+                                VoxelOcc occ) {
 #pragma clang fp contract(off)
   extern __shared__ float E[];
   for (int i = threadIdx.x; i < N; i += blockDim.x) E[i] = edges[i];
@@ -106,6 +108,9 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
   }
   if (!head || key < 0) return;
   const size_t off = (size_t)key;
+  // key = cloud N^3 + z N^2 + x N + y: row (cloud, z, x) = key / N
+  if (occ.row) occ.row[off / (size_t)N] = kClear ? 0 : 1;
+  if (occ.plane) occ.plane[off / ((size_t)N * N)] = kClear ? 0 : 1;
   if (kClear) {
     sumI[off] = 0.f;
     sumR[off] = 0.f;
@@ -459,18 +464,18 @@ void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t s) {
 
 void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
                           const DevPose* pose, int pose_stride, int N, const float* edges,
-                          float* sumI, float* sumR, uint32_t* cnt, hipStream_t s) {
+                          float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s) {
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
   hipLaunchKernelGGL(k_voxel_scatter<false>, g, dim3(256), N * sizeof(float), s, pts, npts, cap,
-                     pose, pose_stride, N, edges, sumI, sumR, cnt);
+                     pose, pose_stride, N, edges, sumI, sumR, cnt, occ);
 }
 
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
                         const DevPose* pose, int pose_stride, int N, const float* edges,
-                        float* sumI, float* sumR, uint32_t* cnt, hipStream_t s) {
+                        float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s) {
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
   hipLaunchKernelGGL(k_voxel_scatter<true>, g, dim3(256), N * sizeof(float), s, pts, npts, cap,
-                     pose, pose_stride, N, edges, sumI, sumR, cnt);
+                     pose, pose_stride, N, edges, sumI, sumR, cnt, occ);
 }
 
 void launch_reduce_keys(const unsigned long long* partials, int count, int npairs,
@@ -485,22 +490,29 @@ int spatial_c2r_blocks(int N) { return (N * N + 15) / 16; }
 // line transforms use fft_reg.hpp.
 
 // pass 1: rows (z, x) along y, contiguous; R2 threads per row, n2 fastest.
+// A row without points (row_occ 0) is not read: its transform is zero.  With
+// sparse_out it is not written either, and a workgroup whose rows are all
+// empty ends at once.
 template <int L>
 __global__ void __launch_bounds__(256) k_spat_r2c_reg(
     const float* __restrict__ sumI, const float* __restrict__ sumR,
-    const uint32_t* __restrict__ cnt, int64_t cloud_stride, const float2* __restrict__ tw,
-    float2* __restrict__ spec, int64_t spec_stride0, int64_t spec_stride1) {
+    const uint32_t* __restrict__ cnt, int64_t cloud_stride, const uint8_t* __restrict__ row_occ,
+    bool sparse_out, const float2* __restrict__ tw, float2* __restrict__ spec,
+    int64_t spec_stride0, int64_t spec_stride1) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
   constexpr int h = L / 2 + 1;
   __shared__ float2 buf[LINES * S];
   const int ln = threadIdx.x / R2, n2 = threadIdx.x % R2;
   const int row0 = blockIdx.x * LINES, c = blockIdx.y;  // L*L rows, a multiple of LINES
+  const uint8_t* occ = row_occ ? row_occ + (int64_t)c * L * L + row0 : nullptr;
+  const bool live = !occ || occ[ln];
+  if (sparse_out && !__syncthreads_or(live)) return;  // uniform over the workgroup
   const int64_t off = c * cloud_stride + (int64_t)(row0 + ln) * L;
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
     const int64_t o = off + n2 + R2 * n1;
-    const uint32_t k = cnt[o];  // empty voxels never touch the sums
+    const uint32_t k = live ? cnt[o] : 0u;  // empty voxels never touch the sums
     v[n1] = k ? make_float2(sumI[o] / (float)k, sumR[o] / (float)k) : make_float2(0.f, 0.f);
   }
   float2 out[R1];
@@ -516,6 +528,7 @@ __global__ void __launch_bounds__(256) k_spat_r2c_reg(
   float2* o1 = o0 + spec_stride1;
   for (int idx = threadIdx.x; idx < LINES * h; idx += blockDim.x) {
     const int r = idx / h, k = idx - r * h;
+    if (sparse_out && occ && !occ[r]) continue;
     const float2 zk = buf[r * S + k];
     const float2 zc = cconj(buf[r * S + (L - k) % L]);
     const float2 sm = cadd(zk, zc), d = csub(zk, zc);
@@ -561,6 +574,123 @@ __global__ void __launch_bounds__(256) k_spat_cross_z_reg(
     v[n1] = cz;
   }
   fft_four_step<+1, L>(v, n2, buf + cl * S, tw, [&](int k, float2 x) {
+    if (live) wp[(int64_t)k * inner] = x;
+  });
+}
+
+// fuse_at over the channels {x0, x1} followed by C - 2 absent (zero) ones:
+// mode 0 keeps the first channel of largest energy (an absent channel never
+// wins, its energy 0 does not exceed a present one's), 1 averages over C,
+// 2 gives zero.
+__device__ __forceinline__ float2 fuse_pair(float2 x0, float2 x1, int C, int mode) {
+  if (mode == 0) {
+    const float e0 = x0.x * x0.x + x0.y * x0.y, e1 = x1.x * x1.x + x1.y * x1.y;
+    return e1 > e0 ? x1 : x0;
+  }
+  if (mode == 1) return make_float2((x0.x + x1.x) / (float)C, (x0.y + x1.y) / (float)C);
+  return make_float2(0.f, 0.f);
+}
+
+// Forward FFT along z of the two channels of every x-transformed half
+// spectrum of a pair, Laplace fusion, C = F conj(G) and the inverse FFT
+// along z, all in registers (one LDS exchange per transform): the z-pass
+// output is never stored.  Lines q = (x, k2) with stride inner = L h; pair
+// blockIdx.y reads F.ch[0..1] + p f_stride and G.ch[0..1] + p g_stride (the
+// other C - 2 channels are absent) and, where f_occ / g_occ are given (the
+// clouds' z-plane occupancy, advanced by p f_occ_stride / p g_occ_stride),
+// only the planes that hold points; the others are zero and were never
+// written by the x pass.  A thread's forward output (k = n2 + R2 j + R1 k2)
+// is the inverse transform's input of n1 = j + (R1 / R2) k2.
+template <int L>
+__global__ void __launch_bounds__(256) k_spat_cross_zz_reg(
+    ChannelSet F, int64_t f_stride, ChannelSet G, int64_t g_stride,
+    const uint8_t* __restrict__ f_occ, int64_t f_occ_stride, const uint8_t* __restrict__ g_occ,
+    int64_t g_occ_stride, FusionRuns runs, const float2* __restrict__ tw,
+    float2* __restrict__ work) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
+  constexpr int inner = L * (L / 2 + 1);
+  __shared__ float2 buf[LINES * S];
+  __shared__ uint32_t occ_w[2 * L / 4];
+  const int cl = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
+  const int q = blockIdx.x * LINES + cl;
+  const bool live = q < inner;
+  const int p = blockIdx.y;
+  for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
+    const uint8_t* o = i < L / 4 ? f_occ : g_occ;
+    const int64_t st = i < L / 4 ? f_occ_stride : g_occ_stride;
+    occ_w[i] = o ? ((const uint32_t*)(o + p * st))[i % (L / 4)] : 0x01010101u;
+  }
+  __syncthreads();
+  const uint8_t* fo = (const uint8_t*)occ_w;
+  const uint8_t* go = fo + L;
+  float2* lb = buf + cl * S;
+  // fused F over this thread's output frequencies
+  float2 ff[R1], gf[R1];
+  {
+    float2 a[R1], b[R1], xa[R1], xb[R1];
+    const float2* c0 = F.ch[0] + p * f_stride + q;
+    const float2* c1 = F.ch[1] + p * f_stride + q;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const int z = n2 + R2 * n1;
+      const bool ld = live && fo[z];
+      a[n1] = ld ? c0[(int64_t)z * inner] : make_float2(0.f, 0.f);
+      b[n1] = ld ? c1[(int64_t)z * inner] : make_float2(0.f, 0.f);
+    }
+    fft_four_step_reg<-1, L>(a, n2, lb, tw, xa);
+    __syncthreads();
+    fft_four_step_reg<-1, L>(b, n2, lb, tw, xb);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R1; ++i) {
+      ff[i] = xa[i];
+      gf[i] = xb[i];
+    }
+  }
+  int mode[R1];
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const uint64_t idx = (uint64_t)(n2 + R2 * j + R1 * k2) * inner + q;
+      int m = 0;
+      for (int r = 0; r < runs.count; ++r)
+        if (idx >= runs.start[r]) m = (int)runs.mode[r];
+      mode[j * R2 + k2] = m;
+    }
+#pragma unroll
+  for (int i = 0; i < R1; ++i) ff[i] = fuse_pair(ff[i], gf[i], F.C, mode[i]);
+  {
+    float2 a[R1], b[R1], xa[R1], xb[R1];
+    const float2* c0 = G.ch[0] + p * g_stride + q;
+    const float2* c1 = G.ch[1] + p * g_stride + q;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const int z = n2 + R2 * n1;
+      const bool ld = live && go[z];
+      a[n1] = ld ? c0[(int64_t)z * inner] : make_float2(0.f, 0.f);
+      b[n1] = ld ? c1[(int64_t)z * inner] : make_float2(0.f, 0.f);
+    }
+    fft_four_step_reg<-1, L>(a, n2, lb, tw, xa);
+    __syncthreads();
+    fft_four_step_reg<-1, L>(b, n2, lb, tw, xb);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R1; ++i) gf[i] = fuse_pair(xa[i], xb[i], G.C, mode[i]);
+  }
+  float2 v[R1];
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const float2 f = ff[j * R2 + k2], g = gf[j * R2 + k2];
+      float2 cz;
+      cz.x = f.x * g.x - f.y * (-g.y);
+      cz.y = f.x * (-g.y) + f.y * g.x;
+      v[j + (R1 / R2) * k2] = cz;
+    }
+  float2* wp = work + p * (int64_t)L * inner + q;
+  fft_four_step<+1, L>(v, n2, lb, tw, [&](int k, float2 x) {
     if (live) wp[(int64_t)k * inner] = x;
   });
 }
@@ -618,19 +748,93 @@ bool spatial_reg_ok(int N) { return N == 64 || N == 128 || N == 256; }
 
 template <int L>
 static void launch_reg_passes_r2c(int nclouds, const float* sumI, const float* sumR,
-                                  const uint32_t* cnt, const float2* tw, float2* spec,
-                                  hipStream_t s) {
+                                  const uint32_t* cnt, const uint8_t* row_occ, bool sparse_out,
+                                  const float2* tw, float2* spec, hipStream_t s) {
   constexpr int LINES = 256 / RegSplit<L>::R2;
+  static_assert((L * L) % LINES == 0 && L % LINES == 0, "workgroups must not straddle planes");
   const int64_t H = (int64_t)L * L * (L / 2 + 1), T = (int64_t)L * L * L;
   hipLaunchKernelGGL(k_spat_r2c_reg<L>, dim3(L * L / LINES, nclouds), dim3(256), 0, s, sumI, sumR,
-                     cnt, T, tw, spec, 2 * H, H);
+                     cnt, T, row_occ, sparse_out, tw, spec, 2 * H, H);
 }
 
 void launch_spatial_r2c_reg(int N, int nclouds, const float* sumI, const float* sumR,
-                            const uint32_t* cnt, const float2* tw, float2* spec, hipStream_t s) {
-  if (N == 64) launch_reg_passes_r2c<64>(nclouds, sumI, sumR, cnt, tw, spec, s);
-  else if (N == 128) launch_reg_passes_r2c<128>(nclouds, sumI, sumR, cnt, tw, spec, s);
-  else launch_reg_passes_r2c<256>(nclouds, sumI, sumR, cnt, tw, spec, s);
+                            const uint32_t* cnt, const uint8_t* row_occ, bool sparse_out,
+                            const float2* tw, float2* spec, hipStream_t s) {
+  if (sparse_out && !row_occ) sparse_out = false;
+  if (N == 64)
+    launch_reg_passes_r2c<64>(nclouds, sumI, sumR, cnt, row_occ, sparse_out, tw, spec, s);
+  else if (N == 128)
+    launch_reg_passes_r2c<128>(nclouds, sumI, sumR, cnt, row_occ, sparse_out, tw, spec, s);
+  else
+    launch_reg_passes_r2c<256>(nclouds, sumI, sumR, cnt, row_occ, sparse_out, tw, spec, s);
+}
+
+// Forward x pass over the spectra a sparse_out r2c left (spectrum b of
+// cloud b / 2, layout [z][x][k2], k2 < h = L/2+1): line q = (z, k2) along x
+// (stride h).  When plane z holds no points the line is zero and is neither
+// read nor written; otherwise row x is read only where row (z, x) holds
+// points.  The row bytes of the (at most two) planes a workgroup's lines
+// touch are staged in LDS first, so that the predicated data loads are all
+// in flight together.
+template <int L, int kLines>
+__global__ void __launch_bounds__(kLines * RegSplit<L>::R2) k_fft_x_occ(
+    float2* __restrict__ data, int64_t batch_stride, const uint8_t* __restrict__ row_occ,
+    const uint8_t* __restrict__ plane_occ, const float2* __restrict__ tw) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, h = L / 2 + 1, S = L + 1;
+  constexpr int nlines = L * h;
+  static_assert(kLines <= h, "a workgroup's lines span at most two planes");
+  __shared__ float2 buf[kLines * S];
+  __shared__ uint32_t occ_w[2 * L / 4 + 1];
+  const int c = threadIdx.x % kLines, n2 = threadIdx.x / kLines;
+  const int q = blockIdx.x * kLines + c;
+  const int cloud = blockIdx.y >> 1;
+  const int z0 = (blockIdx.x * kLines) / h;
+  for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
+    const int zz = z0 + (4 * i) / L;
+    occ_w[i] = zz < L ? ((const uint32_t*)(row_occ + ((int64_t)cloud * L + zz) * L))[i % (L / 4)]
+                      : 0u;
+  }
+  if (threadIdx.x == 0) {  // bytes 0, 1: planes z0, z0 + 1
+    const uint8_t* po = plane_occ + (int64_t)cloud * L;
+    occ_w[2 * L / 4] = po[z0] | (z0 + 1 < L ? (uint32_t)po[z0 + 1] << 8 : 0u);
+  }
+  __syncthreads();
+  const uint8_t* ob = (const uint8_t*)occ_w;
+  float2* base = data + blockIdx.y * batch_stride;
+  bool live = q < nlines;
+  const uint8_t* rb = ob;
+  if (live) {
+    const int z = q / h;
+    live = ob[2 * L + (z - z0)] != 0;
+    base += (int64_t)z * L * h + (q - z * h);
+    rb = ob + (z - z0) * L;
+  }
+  if (!__syncthreads_or(live)) return;  // uniform over the workgroup
+  float2 v[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    const int e = n2 + R2 * n1;
+    v[n1] = (live && rb[e]) ? base[(int64_t)e * h] : make_float2(0.f, 0.f);
+  }
+  fft_four_step<-1, L>(v, n2, buf + c * S, tw, [&](int k, float2 x) {
+    if (live) base[(int64_t)k * h] = x;
+  });
+}
+
+template <int L>
+static void fft_x_occ_t(int nbatch, float2* spec, VoxelOcc occ, const float2* tw, hipStream_t s) {
+  constexpr int lines = 256 / RegSplit<L>::R2, nl = L * (L / 2 + 1);
+  const int64_t H = (int64_t)nl * L;
+  dim3 g((nl + lines - 1) / lines, nbatch);
+  hipLaunchKernelGGL((k_fft_x_occ<L, lines>), g, dim3(256), 0, s, spec, H, occ.row, occ.plane,
+                     tw);
+}
+
+void launch_spatial_fft_x_occ(int N, int nbatch, float2* spec, VoxelOcc occ, const float2* tw,
+                              hipStream_t s) {
+  if (N == 64) fft_x_occ_t<64>(nbatch, spec, occ, tw, s);
+  else if (N == 128) fft_x_occ_t<128>(nbatch, spec, occ, tw, s);
+  else fft_x_occ_t<256>(nbatch, spec, occ, tw, s);
 }
 
 template <int L>
@@ -640,6 +844,42 @@ static void launch_cross_z_reg_t(int npairs, ChannelSet F, int64_t fs, ChannelSe
   constexpr int LINES = 256 / RegSplit<L>::R2, inner = L * (L / 2 + 1);
   hipLaunchKernelGGL(k_spat_cross_z_reg<L>, dim3((inner + LINES - 1) / LINES, npairs), dim3(256),
                      0, s, F, fs, G, gs, runs, (int)fuse, tw, work);
+}
+
+template <int L>
+static void launch_cross_zz_t(int npairs, ChannelSet F, int64_t fs, ChannelSet G, int64_t gs,
+                              const uint8_t* f_occ, int64_t f_occ_stride, const uint8_t* g_occ,
+                              int64_t g_occ_stride, const FusionRuns& runs, const float2* tw,
+                              float2* work, hipStream_t s) {
+  constexpr int LINES = 256 / RegSplit<L>::R2, inner = L * (L / 2 + 1);
+  hipLaunchKernelGGL(k_spat_cross_zz_reg<L>, dim3((inner + LINES - 1) / LINES, npairs), dim3(256),
+                     0, s, F, fs, G, gs, f_occ, f_occ_stride, g_occ, g_occ_stride, runs, tw, work);
+}
+
+bool spatial_cross_zz_ok(int N, const ChannelSet& F, const ChannelSet& G) {
+  if (!spatial_reg_ok(N)) return false;
+  for (const ChannelSet* cs : {&F, &G}) {
+    if (cs->C < 2 || cs->C > 8 || !cs->ch[0] || !cs->ch[1]) return false;
+    for (int c = 2; c < cs->C; ++c)
+      if (cs->ch[c]) return false;
+  }
+  return true;
+}
+
+void launch_spatial_cross_zz_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                 int64_t g_stride, const uint8_t* f_occ, int64_t f_occ_stride,
+                                 const uint8_t* g_occ, int64_t g_occ_stride,
+                                 const FusionRuns& runs, const float2* tw, float2* work,
+                                 hipStream_t s) {
+  if (N == 64)
+    launch_cross_zz_t<64>(npairs, F, f_stride, G, g_stride, f_occ, f_occ_stride, g_occ,
+                          g_occ_stride, runs, tw, work, s);
+  else if (N == 128)
+    launch_cross_zz_t<128>(npairs, F, f_stride, G, g_stride, f_occ, f_occ_stride, g_occ,
+                           g_occ_stride, runs, tw, work, s);
+  else
+    launch_cross_zz_t<256>(npairs, F, f_stride, G, g_stride, f_occ, f_occ_stride, g_occ,
+                           g_occ_stride, runs, tw, work, s);
 }
 
 void launch_spatial_cross_z_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,

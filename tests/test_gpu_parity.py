@@ -399,6 +399,53 @@ def test_register_fused_passes_match_lds_passes(ctx64):
     assert _rel(b["trans_corr"], a["trans_corr"].astype(np.float64)) < 1e-5
 
 
+@pytest.mark.parametrize("N", [64, 128])
+def test_occupancy_skipping_matches_dense_register_passes(N):
+    """The r2c / forward x / forward z register passes that skip voxel rows
+    and z planes without points against the same passes over every row:
+    identical translation grids (what is skipped is exactly zero).  Each
+    form registers a large pair and then a small one, so the small pair
+    runs over spectra rows the large pair left behind, which must not be
+    read.  The kernels differ in code shape, so the compiler's FMA
+    contraction (hence rounding) may differ: grids within 1e-6 of their
+    maximum (a dropped row or plane moves them by orders of magnitude
+    more), argmax and pose identical."""
+    from phaser_amd.capi import Context
+    big = _pair(6, dict(rings=N, azimuths=1024, overlap=0.7))
+    small = _pair(7, dict(rings=16, azimuths=256))
+    out = {}
+    with Context(bandwidth=N // 2, n_voxels=N, max_points=200000) as ctx:
+        for occ in (-1, 1):
+            ctx.set_tuning(1, 1, 1, fused_reg=1, occ_fwd=occ)
+            assert ctx.tuning()["occ_fwd"] == occ
+            out[occ] = [ctx.register(t, s, grids=True) for s, t, _, _ in (big, small)]
+    for a, b in zip(out[-1], out[1]):
+        err = _rel(b["trans_corr"], a["trans_corr"].astype(np.float64))
+        print(f"N {N}: trans {a['trans_index']} / {b['trans_index']}, grid err {err:.2e}",
+              flush=True)
+        assert err < 1e-6
+        assert a["trans_index"] == b["trans_index"] and a["rot_index"] == b["rot_index"]
+        np.testing.assert_array_equal(a["translation"], b["translation"])
+
+
+def test_occupancy_skipping_batch_matches_dense():
+    """As above for batches (separate and shared targets): same bins, peak
+    values within FP32 rounding."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=64, azimuths=1024)) for s in range(4)]
+    tg = [t for _, t, _, _ in pairs]
+    sr = [s for s, _, _, _ in pairs]
+    with Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=4) as ctx:
+        got = {}
+        for occ in (-1, 1):
+            ctx.set_tuning(1, 1, 1, fused_reg=1, occ_fwd=occ)
+            got[occ] = (ctx.register_pairs(tg, sr), ctx.register_pairs(tg[:1], sr, True))
+    for la, lb in zip(got[-1], got[1]):
+        for a, b in zip(la, lb):
+            assert a["trans_index"] == b["trans_index"] and a["rot_index"] == b["rot_index"]
+            assert abs(a["trans_value"] - b["trans_value"]) <= 1e-6 * abs(a["trans_value"])
+
+
 def test_so3_register_engine_matches_oracle(po):
     """SO(3) alpha/gamma passes on the register four-step kernels."""
     from phaser_amd.capi import Context
