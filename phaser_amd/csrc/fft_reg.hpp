@@ -71,17 +71,34 @@ struct RegSplit {
   static constexpr int R2 = L / R1;
 };
 
+// The twiddles W_L^(n2 k1), k1 < R1, of thread n2 (w[0] unused), read
+// from tw[e] = exp(-2 pi i e / L).  A kernel loads them once, ahead of its
+// data, so that both are in flight together and several transforms of the
+// same thread share them.
+template <int L>
+struct Twiddles {
+  float2 w[RegSplit<L>::R1];
+};
+template <int L>
+__device__ __forceinline__ Twiddles<L> load_twiddles(int n2, const float2* __restrict__ tw) {
+  Twiddles<L> t;
+  t.w[0] = make_float2(1.f, 0.f);
+#pragma unroll
+  for (int k1 = 1; k1 < RegSplit<L>::R1; ++k1) t.w[k1] = tw[n2 * k1];  // n2 * k1 < L
+  return t;
+}
+
 // As fft_four_step, with the results kept in registers:
 // out[j * R2 + k2] = X[k1 + R1 k2], k1 = n2 + R2 j.
 template <int DIR, int L>
 __device__ __forceinline__ void fft_four_step_reg(float2 (&v)[RegSplit<L>::R1], int n2,
-                                                  float2* lbuf, const float2* __restrict__ tw,
+                                                  float2* lbuf, const Twiddles<L>& tw,
                                                   float2 (&out)[RegSplit<L>::R1]) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
   dft_reg<R1, DIR>(v);
 #pragma unroll
   for (int k1 = 1; k1 < R1; ++k1) {
-    float2 w = tw[n2 * k1];
+    float2 w = tw.w[k1];
     if (DIR > 0) w.y = -w.y;
     v[k1] = cmul(v[k1], w);
   }
@@ -107,12 +124,12 @@ __device__ __forceinline__ void fft_four_step_reg(float2 (&v)[RegSplit<L>::R1], 
 // of the workgroup must call it.
 template <int DIR, int L, class Store>
 __device__ __forceinline__ void fft_four_step(float2 (&v)[RegSplit<L>::R1], int n2, float2* lbuf,
-                                              const float2* __restrict__ tw, Store store) {
+                                              const Twiddles<L>& tw, Store store) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
   dft_reg<R1, DIR>(v);
 #pragma unroll
   for (int k1 = 1; k1 < R1; ++k1) {
-    float2 w = tw[n2 * k1];  // n2 * k1 < L
+    float2 w = tw.w[k1];
     if (DIR > 0) w.y = -w.y;
     v[k1] = cmul(v[k1], w);
   }
@@ -146,13 +163,14 @@ __global__ void __launch_bounds__(kLines * RegSplit<L>::R2) k_fft_lines(
   const int c = threadIdx.x % kLines, n2 = threadIdx.x / kLines;
   const int q = blockIdx.x * kLines + c;
   const bool live = q < nlines;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   float2* base = data + blockIdx.y * batch_stride;
   if (live) base += (int64_t)(q / inner) * L * inner + (q % inner);
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1)
     v[n1] = live ? base[(int64_t)(n2 + R2 * n1) * inner] : make_float2(0.f, 0.f);
-  fft_four_step<DIR, L>(v, n2, buf + c * S, tw, [&](int k, float2 x) {
+  fft_four_step<DIR, L>(v, n2, buf + c * S, w, [&](int k, float2 x) {
     if (live) base[(int64_t)k * inner] = x;
   });
 }

@@ -229,6 +229,7 @@ __global__ void __launch_bounds__(256) k_so3_fft_alpha_reg(int bw, const float2*
   const int m2 = blockIdx.x, k0 = blockIdx.y * LINES, pair = blockIdx.z;
   const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
   const float2* Sp = S + ((int64_t)pair * bw + m2) * w1 * L + k0 + c;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
@@ -237,7 +238,7 @@ __global__ void __launch_bounds__(256) k_so3_fft_alpha_reg(int bw, const float2*
     v[n1] = (M1 == bw) ? make_float2(0.f, 0.f) : Sp[(int64_t)(m1 + bw - 1) * L];
   }
   float2 out[R1];
-  fft_four_step_reg<-1, L>(v, n2, buf + c * SS, tw, out);
+  fft_four_step_reg<-1, L>(v, n2, buf + c * SS, w, out);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < R1 / R2; ++j)
@@ -265,6 +266,7 @@ __global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2*
   const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
   const int aa = a0 + 2 * c;  // alpha values aa and aa + 1
   const float2* Up = U + ((int64_t)pair * L + k) * bw * L + aa;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
@@ -288,7 +290,7 @@ __global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2*
     v[n1] = make_float2(va.x - vb.y, va.y + vb.x);  // va + i vb
   }
   float2 out[R1];
-  fft_four_step_reg<-1, L>(v, n2, buf + c * SS, tw, out);
+  fft_four_step_reg<-1, L>(v, n2, buf + c * SS, w, out);
   const float scale = (1.f / (float)L) * ((float)bw / 3.14159265358979323846f);
   unsigned long long best = 0ull;
   __syncthreads();

@@ -502,21 +502,38 @@ __global__ void __launch_bounds__(256) k_spat_r2c_reg(
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
   constexpr int h = L / 2 + 1;
   __shared__ float2 buf[LINES * S];
+  __shared__ uint8_t row_live[LINES];
   const int ln = threadIdx.x / R2, n2 = threadIdx.x % R2;
   const int row0 = blockIdx.x * LINES, c = blockIdx.y;  // L*L rows, a multiple of LINES
   const uint8_t* occ = row_occ ? row_occ + (int64_t)c * L * L + row0 : nullptr;
   const bool live = !occ || occ[ln];
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  if (n2 == 0) row_live[ln] = live;  // read after the transform's barriers
   if (sparse_out && !__syncthreads_or(live)) return;  // uniform over the workgroup
   const int64_t off = c * cloud_stride + (int64_t)(row0 + ln) * L;
   float2 v[R1];
+  if (live) {
+    // count and sums of every voxel of the row in one batch of loads (the
+    // sums of empty voxels are zero), then the means
+    uint32_t k[R1];
+    float si[R1], sr[R1];
 #pragma unroll
-  for (int n1 = 0; n1 < R1; ++n1) {
-    const int64_t o = off + n2 + R2 * n1;
-    const uint32_t k = live ? cnt[o] : 0u;  // empty voxels never touch the sums
-    v[n1] = k ? make_float2(sumI[o] / (float)k, sumR[o] / (float)k) : make_float2(0.f, 0.f);
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const int64_t o = off + n2 + R2 * n1;
+      k[n1] = cnt[o];
+      si[n1] = sumI[o];
+      sr[n1] = sumR[o];
+    }
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1)
+      v[n1] = k[n1] ? make_float2(si[n1] / (float)k[n1], sr[n1] / (float)k[n1])
+                    : make_float2(0.f, 0.f);
+  } else {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) v[n1] = make_float2(0.f, 0.f);
   }
   float2 out[R1];
-  fft_four_step_reg<-1, L>(v, n2, buf + ln * S, tw, out);
+  fft_four_step_reg<-1, L>(v, n2, buf + ln * S, w, out);
   __syncthreads();
 #pragma unroll
   for (int j = 0; j < R1 / R2; ++j)
@@ -528,7 +545,7 @@ __global__ void __launch_bounds__(256) k_spat_r2c_reg(
   float2* o1 = o0 + spec_stride1;
   for (int idx = threadIdx.x; idx < LINES * h; idx += blockDim.x) {
     const int r = idx / h, k = idx - r * h;
-    if (sparse_out && occ && !occ[r]) continue;
+    if (sparse_out && !row_live[r]) continue;
     const float2 zk = buf[r * S + k];
     const float2 zc = cconj(buf[r * S + (L - k) % L]);
     const float2 sm = cadd(zk, zc), d = csub(zk, zc);
@@ -551,6 +568,7 @@ __global__ void __launch_bounds__(256) k_spat_cross_z_reg(
   const bool live = q < inner;
   const int64_t fo = blockIdx.y * f_stride, go = blockIdx.y * g_stride;
   float2* wp = work + blockIdx.y * (int64_t)L * inner + q;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
@@ -573,7 +591,7 @@ __global__ void __launch_bounds__(256) k_spat_cross_z_reg(
     }
     v[n1] = cz;
   }
-  fft_four_step<+1, L>(v, n2, buf + cl * S, tw, [&](int k, float2 x) {
+  fft_four_step<+1, L>(v, n2, buf + cl * S, w, [&](int k, float2 x) {
     if (live) wp[(int64_t)k * inner] = x;
   });
 }
@@ -615,6 +633,7 @@ __global__ void __launch_bounds__(256) k_spat_cross_zz_reg(
   const int q = blockIdx.x * LINES + cl;
   const bool live = q < inner;
   const int p = blockIdx.y;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
     const uint8_t* o = i < L / 4 ? f_occ : g_occ;
     const int64_t st = i < L / 4 ? f_occ_stride : g_occ_stride;
@@ -637,9 +656,9 @@ __global__ void __launch_bounds__(256) k_spat_cross_zz_reg(
       a[n1] = ld ? c0[(int64_t)z * inner] : make_float2(0.f, 0.f);
       b[n1] = ld ? c1[(int64_t)z * inner] : make_float2(0.f, 0.f);
     }
-    fft_four_step_reg<-1, L>(a, n2, lb, tw, xa);
+    fft_four_step_reg<-1, L>(a, n2, lb, w, xa);
     __syncthreads();
-    fft_four_step_reg<-1, L>(b, n2, lb, tw, xb);
+    fft_four_step_reg<-1, L>(b, n2, lb, w, xb);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < R1; ++i) {
@@ -671,9 +690,9 @@ __global__ void __launch_bounds__(256) k_spat_cross_zz_reg(
       a[n1] = ld ? c0[(int64_t)z * inner] : make_float2(0.f, 0.f);
       b[n1] = ld ? c1[(int64_t)z * inner] : make_float2(0.f, 0.f);
     }
-    fft_four_step_reg<-1, L>(a, n2, lb, tw, xa);
+    fft_four_step_reg<-1, L>(a, n2, lb, w, xa);
     __syncthreads();
-    fft_four_step_reg<-1, L>(b, n2, lb, tw, xb);
+    fft_four_step_reg<-1, L>(b, n2, lb, w, xb);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < R1; ++i) gf[i] = fuse_pair(xa[i], xb[i], G.C, mode[i]);
@@ -690,7 +709,7 @@ __global__ void __launch_bounds__(256) k_spat_cross_zz_reg(
       v[j + (R1 / R2) * k2] = cz;
     }
   float2* wp = work + p * (int64_t)L * inner + q;
-  fft_four_step<+1, L>(v, n2, lb, tw, [&](int k, float2 x) {
+  fft_four_step<+1, L>(v, n2, lb, w, [&](int k, float2 x) {
     if (live) wp[(int64_t)k * inner] = x;
   });
 }
@@ -712,6 +731,7 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
   partials += blockIdx.y * (int64_t)gridDim.x;
   const float2* xa = work + (int64_t)ra * h;
   const float2* xb = xa + h;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
@@ -730,7 +750,7 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
     v[n1] = make_float2(va.x - vb.y, va.y + vb.x);
   }
   unsigned long long best = 0ull;
-  fft_four_step<+1, L>(v, n2, buf + ln * S, tw, [&](int y, float2 z) {
+  fft_four_step<+1, L>(v, n2, buf + ln * S, w, [&](int y, float2 z) {
     const uint32_t la = (uint32_t)((int64_t)ra * L + y), lb = la + (uint32_t)L;
     if (corr) {
       corr[la] = z.x;
@@ -789,6 +809,7 @@ __global__ void __launch_bounds__(kLines * RegSplit<L>::R2) k_fft_x_occ(
   const int q = blockIdx.x * kLines + c;
   const int cloud = blockIdx.y >> 1;
   const int z0 = (blockIdx.x * kLines) / h;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
   for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
     const int zz = z0 + (4 * i) / L;
     occ_w[i] = zz < L ? ((const uint32_t*)(row_occ + ((int64_t)cloud * L + zz) * L))[i % (L / 4)]
@@ -816,7 +837,7 @@ __global__ void __launch_bounds__(kLines * RegSplit<L>::R2) k_fft_x_occ(
     const int e = n2 + R2 * n1;
     v[n1] = (live && rb[e]) ? base[(int64_t)e * h] : make_float2(0.f, 0.f);
   }
-  fft_four_step<-1, L>(v, n2, buf + c * S, tw, [&](int k, float2 x) {
+  fft_four_step<-1, L>(v, n2, buf + c * S, w, [&](int k, float2 x) {
     if (live) base[(int64_t)k * h] = x;
   });
 }
