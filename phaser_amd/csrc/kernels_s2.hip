@@ -188,15 +188,19 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
   return r;
 }
 
-// Exact nearest neighbour of every query, eight lanes per query.  Bands are
-// visited in order of colatitude distance (own, +1, -1, +2, ...), sectors of
-// a band outward from the query's own, and a cell is skipped once its lower
-// bound on the squared chord distance,
+// Exact nearest neighbour of every query, LANES lanes per query.  Bands
+// are visited in order of colatitude distance (own, +1, -1, +2, ...),
+// sectors of a band outward from the query's own, and a cell is skipped once
+// its lower bound on the squared chord distance,
 //   |q - p|^2 >= 2(1 - cos dth) + 2 sin(th_q) min sin(th_p) (1 - cos dphi),
 // exceeds the best distance found (with a float slack): points outside it are
-// provably farther.  Empty sectors are jumped over with the per-cell tables.
-// The LANES lanes of a query split each cell's points and agree on the best
-// after it.
+// provably farther.  Empty sectors are jumped over with the per-cell tables,
+// empty bands with the cloud's band table (in LDS).  The LANES lanes of a
+// query split each cell's points and agree on the best after it.  Loads are
+// batched so that a walk step costs about one memory round trip: the points
+// of a cell in groups of four per lane, the next jump-table entries while the
+// current cells are scanned (the walk then decides with a best that may lag
+// by one step, which only prunes less).
 template <int LANES>
 __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
                      int nb, int ns, float max_sq, const int* __restrict__ cell_start,
@@ -204,16 +208,21 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids) {
+  constexpr int kBatch = 4;  // point loads in flight per lane
   const int c = blockIdx.y;
   const int sub = threadIdx.x & (LANES - 1);
   const int q = blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
-  // min sin(theta) over each band (nb <= 256), once per workgroup
+  // min sin(theta) over each band and the cloud's band table (nb <= 256),
+  // once per workgroup
   __shared__ float band_sin_min[256];
+  __shared__ int bs[257];
   for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
     const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
     band_sin_min[bb] = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
   }
+  for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x)
+    bs[bb] = band_start[(int64_t)c * (nb + 1) + bb];
   __syncthreads();
   if (q >= nq) return;  // whole groups leave together
   const int qrow = q / n, qcol = q - qrow * n;
@@ -222,7 +231,6 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   const float phq = (2.f * kPi / n) * qcol;
   const float sq = sinf(thq);
   const int* st = cell_start + (int64_t)c * (ncell + 1);
-  const int* bs = band_start + (int64_t)c * (nb + 1);
   const int4* nr = next_right + (int64_t)c * ncell;
   const int4* nl = next_left + (int64_t)c * ncell;
   const float4* sp = sorted + c * cap;
@@ -236,14 +244,25 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   int bidx = 0x7fffffff, bpos = -1;
   bool up = true, down = true;
   auto scan_range = [&](int s0, int e) {
-    for (int p = s0 + sub; p < e; p += LANES) {
-      const float4 pv = sp[p];
-      const float d = l2_simple(qv, pv);
-      const int oi = __float_as_int(pv.w);
-      if (d < best || (d == best && oi < bidx)) {
-        best = d;
-        bidx = oi;
-        bpos = p;
+    for (int p0 = s0 + sub; p0 < e; p0 += kBatch * LANES) {
+      float4 pv[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int p = p0 + u * LANES;
+        pv[u] = p < e ? sp[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int p = p0 + u * LANES;
+        if (p < e) {
+          const float d = l2_simple(qv, pv[u]);
+          const int oi = __float_as_int(pv[u].w);
+          if (d < best || (d == best && oi < bidx)) {
+            best = d;
+            bidx = oi;
+            bpos = p;
+          }
+        }
       }
     }
 #pragma unroll
@@ -286,6 +305,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
         continue;
       }
     }
+    if (bs[b + 1] == bs[b]) continue;  // empty band (LDS)
     const float lo = b * dband - eps, hi = (b + 1) * dband + eps;
     const float dth = fmaxf(0.f, fmaxf(lo - thq, thq - hi));
     const float s1 = __sinf(0.5f * dth);
@@ -299,12 +319,15 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
         down = false;
       continue;
     }
-    if (bs[b + 1] == bs[b]) continue;
     const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
-    scan_range(st[b * ns + qj], st[b * ns + qj + 1]);
-    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once).
-    // The two walks alternate so that their table loads are in flight together.
+    const int cb = b * ns;
+    // the own cell's range and the first entries of both walks in one round
+    const int own0 = st[cb + qj], own1 = st[cb + qj + 1];
+    int4 jr = nr[cb + (qj + 1) % ns];
+    int4 jl = nl[cb + (qj - 1 + ns) % ns];
+    scan_range(own0, own1);
+    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
     int ur = 0, ul = 0;
     bool rdone = false, ldone = false;
     auto bound = [&](int sec) {
@@ -317,27 +340,32 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       return A + Bc * 2.f * s2 * s2;
     };
     while (!rdone || !ldone) {
-      const int4 jr = rdone ? make_int4(-1, 0, 0, 0) : nr[b * ns + (qj + ur + 1) % ns];
-      const int4 jl = ldone ? make_int4(-1, 0, 0, 0)
-                            : nl[b * ns + ((qj - ul - 1) % ns + ns) % ns];
+      int4 cr = jr, cl = jl;
+      bool scan_r = false, scan_l = false;
+      const float th = thr2();
       if (!rdone) {
         const int u = (jr.x - qj + ns) % ns;
-        if (jr.x < 0 || u <= ur || u > ns / 2 || bound(jr.x) > thr2()) {
+        if (jr.x < 0 || u <= ur || u > ns / 2 || bound(jr.x) > th) {
           rdone = true;
         } else {
-          scan_range(jr.y, jr.z);
+          scan_r = true;
           ur = u;
         }
       }
       if (!ldone) {
         const int u = (qj - jl.x + ns) % ns;
-        if (jl.x < 0 || u <= ul || u > (ns - 1) / 2 || bound(jl.x) > thr2()) {
+        if (jl.x < 0 || u <= ul || u > (ns - 1) / 2 || bound(jl.x) > th) {
           ldone = true;
         } else {
-          scan_range(jl.y, jl.z);
+          scan_l = true;
           ul = u;
         }
       }
+      // next entries first, so they are in flight during the scans
+      if (!rdone) jr = nr[cb + (qj + ur + 1) % ns];
+      if (!ldone) jl = nl[cb + ((qj - ul - 1) % ns + ns) % ns];
+      if (scan_r) scan_range(cr.y, cr.z);
+      if (scan_l) scan_range(cl.y, cl.z);
     }
   }
   if (sub == 0) {

@@ -60,6 +60,51 @@ def test_sample_s2_matches_oracle(ctx32, po):
         assert same.mean() >= 0.999, same.mean()
 
 
+@pytest.mark.parametrize("B", [64, 128])
+def test_s2_nearest_is_exact(po, B):
+    """Every query's sample comes from a point at the minimum chord distance
+    (the oracle's exact kd-tree over the oracle's projection), with zero
+    samples exactly where no point lies within max_sq.  The device projects
+    with its own acosf / atan2f, which may differ from the host's in the last
+    bit, moving a point by up to delta = 2.4e-7 per coordinate; that changes
+    a squared distance d by at most 2 sqrt(3 d) delta + 3 delta^2, so a pick
+    within twice that of the minimum is a near-tie the projections may order
+    either way; anything farther is a search error.  B = 64 keeps the
+    cell-start table in LDS, B = 128 reads it from memory."""
+    from phaser_amd.capi import Context
+    src, tgt, _, _ = _pair(11, dict(rings=128, azimuths=1024, overlap=0.7))
+    n = 2 * B
+    grid = np.asarray(po.sampler_grid(B), np.float32).reshape(n * n, -1)[:, :3]
+    max_sq = po.squared_voxel_size()
+    with Context(bandwidth=B, n_voxels=2 * B, max_points=200000) as ctx:
+        for cloud in (src, tgt):
+            g = ctx.sample_s2(cloud)
+            proj, rng = po.project(cloud)
+            inten = np.asarray(cloud, np.float32)[:, 3]
+            keys = {}
+            for i, k in enumerate(zip(rng.astype(np.float32).tolist(), inten.tolist())):
+                keys.setdefault(k, []).append(i)
+            near = po.nn(proj, grid, kdtree=True)
+            dmin = np.sum((grid.astype(np.float64) - proj[near].astype(np.float64)) ** 2, axis=1)
+            bad = 0
+            for q in range(n * n):
+                if dmin[q] > max_sq * (1 + 1e-6):
+                    bad += not (g[0][q] == 0 and g[1][q] == 0)
+                    continue
+                cand = keys.get((float(np.float32(g[0][q])), float(np.float32(g[1][q]))), [])
+                d = min((float(np.sum((grid[q].astype(np.float64) - proj[i]) ** 2)) for i in cand),
+                        default=np.inf)
+                delta = 2.4e-7
+                tol = 2 * (2 * np.sqrt(3 * dmin[q]) * delta + 3 * delta * delta)
+                if d > dmin[q]:
+                    print(f"  query {q}: pick d={d:.9e} min d={dmin[q]:.9e} gap={d - dmin[q]:.3e} "
+                          f"near-tie bound {tol:.3e}, {len(cand)} candidate(s)", flush=True)
+                bad += not (d <= dmin[q] + tol)
+            print(f"B {B}: {len(cloud)} points, {bad} of {n * n} queries off the minimum",
+                  flush=True)
+            assert bad == 0
+
+
 def test_s2_transform_matches_oracle(ctx32, po):
     rng = np.random.default_rng(0)
     grids = rng.normal(size=(3, 64 * 64))
