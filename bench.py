@@ -166,7 +166,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--latency-samples", type=int, default=40)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)

@@ -67,48 +67,80 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
 }
 
 // exclusive scan of the n*n cell counts of a cloud (one 1024-thread
-// workgroup per cloud): per-thread serial sums, wave scans, wave totals.
-__global__ void k_cell_scan(int nb, int ns, int* __restrict__ cell_count,
-                            int* __restrict__ cell_start, int* __restrict__ band_start) {
+// workgroup per cloud, ncell <= 1024 * PER): every count is loaded up front,
+// chunk k holding cells 1024 k + thread (coalesced); wave scans per chunk,
+// one wave scans the PER x 16 wave totals in (chunk, wave) order, and the
+// starts are written back, also coalesced.  The counts are zeroed for reuse
+// as the scatter cursors; band_start[b] = start of cell b ns.
+template <int PER>
+__global__ void __launch_bounds__(1024) k_cell_scan(int nb, int ns, int* __restrict__ cell_count,
+                                                    int* __restrict__ cell_start,
+                                                    int* __restrict__ band_start) {
   const int c = blockIdx.x;
   const int ncell = nb * ns;
   int* cnt = cell_count + (int64_t)c * ncell;
   int* st = cell_start + (int64_t)c * (ncell + 1);
   int* bs = band_start + (int64_t)c * (nb + 1);
-  __shared__ int wsum[16];
-  const int T = blockDim.x;  // 1024
-  const int per = (ncell + T - 1) / T;
-  const int a = threadIdx.x * per, b = min(ncell, a + per);
-  int s = 0;
-  for (int k = a; k < b; ++k) s += cnt[k];
-  // inclusive scan within the wave
+  constexpr int kW = 16;  // waves per workgroup
+  __shared__ int wsum[PER * kW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int x = s;
+  int v[PER];  // the counts, then their exclusive sums within the wave
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(x, off, 64);
-    if (lane >= off) x += y;
+  for (int k = 0; k < PER; ++k) {
+    const int i = k * 1024 + threadIdx.x;
+    v[k] = i < ncell ? cnt[i] : 0;
   }
-  if (lane == 63) wsum[wv] = x;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int acc = 0;
-    for (int w = 0; w < T / 64; ++w) {
-      const int v = wsum[w];
-      wsum[w] = acc;
-      acc += v;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int own = v[k];
+    int y = own;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(y, off, 64);
+      if (lane >= off) y += t;
     }
-    st[ncell] = acc;
+    v[k] = y - own;
+    if (lane == 63) wsum[k * kW + wv] = y;
   }
   __syncthreads();
-  int acc = wsum[wv] + x - s;
-  for (int k = a; k < b; ++k) {
-    st[k] = acc;
-    acc += cnt[k];
-    cnt[k] = 0;  // reused as the scatter cursor
+  if (wv == 0) {
+    constexpr int E = (PER * kW + 63) / 64;  // totals per lane
+    int t[E], sum = 0;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int idx = lane * E + j;
+      t[j] = idx < PER * kW ? wsum[idx] : 0;
+      sum += t[j];
+    }
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
+    }
+    int acc = incl - sum;
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      const int idx = lane * E + j;
+      if (idx < PER * kW) wsum[idx] = acc;
+      acc += t[j];
+    }
+    if (lane == 63) {
+      st[ncell] = incl;
+      bs[nb] = incl;
+    }
   }
   __syncthreads();
-  for (int bb = threadIdx.x; bb <= nb; bb += T) bs[bb] = st[bb * ns];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int i = k * 1024 + threadIdx.x;
+    if (i < ncell) {
+      const int start = wsum[k * kW + wv] + v[k];
+      st[i] = start;
+      cnt[i] = 0;  // reused as the scatter cursor
+      if (i % ns == 0) bs[i / ns] = start;
+    }
+  }
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int ncell,
@@ -492,8 +524,15 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
   hipLaunchKernelGGL(k_project, g, dim3(256), 0, s, pts, npts, cap, cb.nb, cb.ns, cb.proj, cb.val,
                      cb.cell, cb.cell_count);
-  hipLaunchKernelGGL(k_cell_scan, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns, cb.cell_count,
-                     cb.cell_start, cb.band_start);
+  if (ncell <= 4 * 1024)
+    hipLaunchKernelGGL(k_cell_scan<4>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
+                       cb.cell_count, cb.cell_start, cb.band_start);
+  else if (ncell <= 16 * 1024)
+    hipLaunchKernelGGL(k_cell_scan<16>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
+                       cb.cell_count, cb.cell_start, cb.band_start);
+  else
+    hipLaunchKernelGGL(k_cell_scan<64>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
+                       cb.cell_count, cb.cell_start, cb.band_start);
   hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj, cb.val,
                      cb.cell, cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
   hipLaunchKernelGGL(k_jump_tables, dim3(cb.nb, nclouds), dim3(64), 0, s, cb.nb, cb.ns,

@@ -723,15 +723,24 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
                                                       unsigned long long* __restrict__ partials) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
   constexpr int h = L / 2 + 1, rows = L * L;
-  __shared__ float2 buf[LINES * S];
+  // the workgroup's 2 LINES rows are contiguous: staged through LDS with
+  // 16-byte loads, then the transform's line buffers reuse the space
+  constexpr int kStage = 2 * LINES * h, kLds = kStage > LINES * S ? kStage : LINES * S;
+  __shared__ __align__(16) float2 buf[kLds];
   const int ln = threadIdx.x / R2, n2 = threadIdx.x % R2;
   const int ra = (blockIdx.x * LINES + ln) * 2;  // rows ra, ra + 1 (rows is a multiple)
   work += blockIdx.y * (int64_t)rows * h;
   if (corr) corr += blockIdx.y * (int64_t)rows * L;
   partials += blockIdx.y * (int64_t)gridDim.x;
-  const float2* xa = work + (int64_t)ra * h;
-  const float2* xb = xa + h;
   const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  {
+    static_assert(kStage % 2 == 0, "whole float4s");
+    const float4* src = (const float4*)(work + (int64_t)blockIdx.x * 2 * LINES * h);
+    for (int i = threadIdx.x; i < kStage / 2; i += blockDim.x) ((float4*)buf)[i] = src[i];
+  }
+  __syncthreads();
+  const float2* xa = buf + 2 * ln * h;
+  const float2* xb = xa + h;
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
@@ -749,6 +758,7 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
     }
     v[n1] = make_float2(va.x - vb.y, va.y + vb.x);
   }
+  __syncthreads();  // the staged rows are read; the line buffers overwrite them
   unsigned long long best = 0ull;
   fft_four_step<+1, L>(v, n2, buf + ln * S, w, [&](int y, float2 z) {
     const uint32_t la = (uint32_t)((int64_t)ra * L + y), lb = la + (uint32_t)L;
