@@ -275,6 +275,14 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   float best = INFINITY;
   int bidx = 0x7fffffff, bpos = -1;
   bool up = true, down = true;
+  // pruning threshold: squared (distance + slack) plus kBoundErr, the
+  // largest error of the bounds below as computed with the hardware sine
+  // (v_sin_f32, absolute error below 2^-20), so pruning stays exact;
+  // recomputed only when the best changes
+  constexpr float kBoundErr = 3e-5f;
+  float thr = INFINITY;
+  // x in [0, 2 ns) reduced mod ns without an integer division
+  auto wrap = [&](int x) { return x >= ns ? x - ns : x; };
   auto scan_range = [&](int s0, int e) {
     for (int p0 = s0 + sub; p0 < e; p0 += kBatch * LANES) {
       float4 pv[kBatch];
@@ -308,15 +316,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
         bpos = op;
       }
     }
-  };
-  // pruning threshold: squared (distance + slack) plus kBoundErr, the
-  // largest error of the bounds below as computed with the hardware sine
-  // (v_sin_f32, absolute error below 2^-20), so pruning stays exact
-  constexpr float kBoundErr = 3e-5f;
-  auto thr2 = [&]() {
-    if (bpos < 0) return INFINITY;
-    const float t = sqrtf(best) + slack;
-    return t * t + kBoundErr;
+    if (bpos >= 0) {
+      const float t = sqrtf(best) + slack;
+      thr = t * t + kBoundErr;
+    }
   };
   for (int t = 0; t < 2 * nb && (up || down); ++t) {
     int b;
@@ -342,7 +345,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const float dth = fmaxf(0.f, fmaxf(lo - thq, thq - hi));
     const float s1 = __sinf(0.5f * dth);
     const float A = 4.f * s1 * s1;
-    if (A > thr2()) {  // every farther band in this direction is farther still
+    if (A > thr) {  // every farther band in this direction is farther still
       if (t == 0)
         up = down = false;
       else if (t & 1)
@@ -356,8 +359,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const int cb = b * ns;
     // the own cell's range and the first entries of both walks in one round
     const int own0 = st[cb + qj], own1 = st[cb + qj + 1];
-    int4 jr = nr[cb + (qj + 1) % ns];
-    int4 jl = nl[cb + (qj - 1 + ns) % ns];
+    int4 jr = nr[cb + wrap(qj + 1)];
+    int4 jl = nl[cb + wrap(qj - 1 + ns)];
     scan_range(own0, own1);
     // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
     int ur = 0, ul = 0;
@@ -374,9 +377,9 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     while (!rdone || !ldone) {
       int4 cr = jr, cl = jl;
       bool scan_r = false, scan_l = false;
-      const float th = thr2();
+      const float th = thr;
       if (!rdone) {
-        const int u = (jr.x - qj + ns) % ns;
+        const int u = wrap(jr.x - qj + ns);
         if (jr.x < 0 || u <= ur || u > ns / 2 || bound(jr.x) > th) {
           rdone = true;
         } else {
@@ -385,7 +388,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
         }
       }
       if (!ldone) {
-        const int u = (qj - jl.x + ns) % ns;
+        const int u = wrap(qj - jl.x + ns);  // jl.x >= 0 when used
         if (jl.x < 0 || u <= ul || u > (ns - 1) / 2 || bound(jl.x) > th) {
           ldone = true;
         } else {
@@ -394,8 +397,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
         }
       }
       // next entries first, so they are in flight during the scans
-      if (!rdone) jr = nr[cb + (qj + ur + 1) % ns];
-      if (!ldone) jl = nl[cb + ((qj - ul - 1) % ns + ns) % ns];
+      if (!rdone) jr = nr[cb + wrap(qj + ur + 1)];
+      if (!ldone) jl = nl[cb + wrap(qj - ul - 1 + ns)];
       if (scan_r) scan_range(cr.y, cr.z);
       if (scan_l) scan_range(cl.y, cl.z);
     }
