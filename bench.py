@@ -79,12 +79,70 @@ FORM_NAMES = {0: "plain fp32", 1: "split fp16 mfma contractions"}
 LAUNCHES = {"spatial_fft_fwd": 2, "voxel_scatter": 2}
 
 
+def concurrent_throughput(args, pairs, max_pts, tuning):
+    """The batch split over args.streams contexts (each with its own HIP
+    stream and buffers), driven from one host thread each so that their
+    kernels overlap on the GPU; returns (registrations/s, seconds) over
+    args.steps steps of every context, timed from a common start."""
+    import threading
+    from phaser_amd.capi import Context
+    S, P = args.streams, len(pairs)
+    parts = [list(range(k, P, S)) for k in range(S)]
+    t = tuning
+    subs = []
+    for idx in parts:
+        c = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=len(idx),
+                    precision=args.precision)
+        c.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
+                     t["fused_reg"], t["so3_reg"], t["occ_fwd"])
+        if args.graphs:
+            c.set_graphs(True)
+        ds = [c.device_array(pairs[i][0]) for i in idx]
+        dt = [c.device_array(pairs[i][1]) for i in idx]
+        subs.append((c, ds, dt, [len(pairs[i][0]) for i in idx], [len(pairs[i][1]) for i in idx]))
+    start = threading.Barrier(S + 1)
+    done = threading.Barrier(S + 1)
+    errors = []
+
+    def run(c, ds, dt, ps, pt):
+        try:
+            for _ in range(args.warmup):
+                c.register_pairs_device([a.ptr for a in dt], pt, [a.ptr for a in ds], ps)
+            c.synchronize()
+            start.wait()
+            for _ in range(args.steps):
+                c.register_pairs_device([a.ptr for a in dt], pt, [a.ptr for a in ds], ps)
+            c.synchronize()
+        except Exception as e:  # reported after the join
+            errors.append(e)
+            start.abort()
+        finally:
+            done.wait()
+
+    threads = [threading.Thread(target=run, args=sub) for sub in subs]
+    for th in threads:
+        th.start()
+    start.wait()
+    t0 = time.perf_counter()
+    done.wait()
+    elapsed = time.perf_counter() - t0
+    for th in threads:
+        th.join()
+    for c, ds, dt, _, _ in subs:
+        for a in ds + dt:
+            a.free()
+        c.close()
+    if errors:
+        raise errors[0]
+    return P * args.steps / elapsed, elapsed
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on the
     streaming copy kernel), or None when no summary covers it."""
     from phaser_amd import profiling
-    path = os.path.join(ROOT, "profiles", "traffic_b64_n128_batch16.json")
+    path = os.path.join(ROOT, "profiles", "traffic_b64_n128_batch64.json")
     try:
         with open(path) as f:
             return profiling.lookup(json.load(f), kernel)
@@ -175,6 +233,8 @@ def main():
     ap.add_argument("--graphs", type=int, default=1, help="replay the pipeline from HIP graphs")
     ap.add_argument("--nn-lanes", type=int, default=0, choices=[0, 2, 4, 8, 16],
                     help="lanes per query of the S2 nearest-neighbour search (0: default)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="split each batch over this many contexts/streams driven concurrently")
     ap.add_argument("--occ-fwd", type=int, default=0, choices=[-1, 0, 1],
                     help="forward spatial passes skip empty voxel rows/planes (0: default, on)")
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
@@ -235,6 +295,8 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     value = P * args.steps / elapsed
+    if args.streams > 1:
+        value, elapsed = concurrent_throughput(args, pairs, max_pts, tuning)
 
     # single-pair latency (p50 / p99)
     lat = []
@@ -353,7 +415,7 @@ def main():
         "data": "synthetic (seeded ray-cast scenes, seeds 0..batch-1)",
         "config": {"workload": "B=64 S2 grid 128x128, SO(3) 128^3, voxels 128^3 over +-50 m, "
                                "128x1024-ray scans, target 70% overlap, noise 2 cm",
-                   "pairs_per_step": P, "mean_points_per_cloud": round(P_mean),
+                   "pairs_per_step": P, "streams": args.streams, "mean_points_per_cloud": round(P_mean),
                    "form": FORM_NAMES[args.precision], "fft_column_tiles": tuning,
                    "hip_graphs": bool(args.graphs)},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),

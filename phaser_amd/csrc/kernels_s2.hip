@@ -210,6 +210,35 @@ __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
   }
 }
 
+// lane exchange within aligned groups of four: DPP quad permutes
+// (xor 1: [1,0,3,2], xor 2: [2,3,0,1]) instead of an LDS-routed shuffle
+template <int CTRL>
+__device__ __forceinline__ int quad_xor(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
+}
+
+// the LANES lanes of a query agree on the smallest (distance, index)
+template <int LANES>
+__device__ __forceinline__ void agree_best(float& best, int& bidx, int& bpos) {
+  auto step = [&](float ob, int oi, int op) {
+    if (ob < best || (ob == best && oi < bidx)) {
+      best = ob;
+      bidx = oi;
+      bpos = op;
+    }
+  };
+  if constexpr (LANES >= 16)
+    step(__shfl_xor(best, 8, 64), __shfl_xor(bidx, 8, 64), __shfl_xor(bpos, 8, 64));
+  if constexpr (LANES >= 8)
+    step(__shfl_xor(best, 4, 64), __shfl_xor(bidx, 4, 64), __shfl_xor(bpos, 4, 64));
+  if constexpr (LANES >= 4)
+    step(__int_as_float(quad_xor<0x4E>(__float_as_int(best))), quad_xor<0x4E>(bidx),
+         quad_xor<0x4E>(bpos));
+  if constexpr (LANES >= 2)
+    step(__int_as_float(quad_xor<0xB1>(__float_as_int(best))), quad_xor<0xB1>(bidx),
+         quad_xor<0xB1>(bpos));
+}
+
 __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
 #pragma clang fp contract(off)
   const float d0 = q.x - p.x, d1 = q.y - p.y, d2 = q.z - p.z;
@@ -284,38 +313,25 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // x in [0, 2 ns) reduced mod ns without an integer division
   auto wrap = [&](int x) { return x >= ns ? x - ns : x; };
   auto scan_range = [&](int s0, int e) {
+    // branch-free batches: indices past the range load its last point
+    // (in bounds, the range being non-empty whenever the loop runs) and
+    // are masked out of the comparison
     for (int p0 = s0 + sub; p0 < e; p0 += kBatch * LANES) {
       float4 pv[kBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int p = p0 + u * LANES;
-        pv[u] = p < e ? sp[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-      }
+      for (int u = 0; u < kBatch; ++u) pv[u] = sp[min(p0 + u * LANES, e - 1)];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int p = p0 + u * LANES;
-        if (p < e) {
-          const float d = l2_simple(qv, pv[u]);
-          const int oi = __float_as_int(pv[u].w);
-          if (d < best || (d == best && oi < bidx)) {
-            best = d;
-            bidx = oi;
-            bpos = p;
-          }
-        }
+        const float d = l2_simple(qv, pv[u]);
+        const int oi = __float_as_int(pv[u].w);
+        const bool take = p < e && (d < best || (d == best && oi < bidx));
+        best = take ? d : best;
+        bidx = take ? oi : bidx;
+        bpos = take ? p : bpos;
       }
     }
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) {
-      const float ob = __shfl_xor(best, off, LANES);
-      const int oi = __shfl_xor(bidx, off, LANES);
-      const int op = __shfl_xor(bpos, off, LANES);
-      if (ob < best || (ob == best && oi < bidx)) {
-        best = ob;
-        bidx = oi;
-        bpos = op;
-      }
-    }
+    agree_best<LANES>(best, bidx, bpos);
     if (bpos >= 0) {
       const float t = sqrtf(best) + slack;
       thr = t * t + kBoundErr;

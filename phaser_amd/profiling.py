@@ -2,6 +2,10 @@
 
 * kernel_stats_markdown(csv_path): the --kernel-trace --stats table
   (<dir>/<name>_kernel_stats.csv) as markdown, with readable kernel names.
+* batched_stats_markdown(trace_csv): from <dir>/<name>_kernel_trace.csv,
+  per kernel the dispatches of its largest grid only (the batched pipeline
+  launches, apart from single-pair calls of the same kernels), with count,
+  mean, min and max duration.
 * traffic_summary(fetch_csv, write_csv, copy_bytes): per-kernel HBM bytes
   per launch from two --pmc runs (FETCH_SIZE, then WRITE_SIZE), each scaled
   by what the same counter reads on the streaming copy kernel, whose true
@@ -10,6 +14,7 @@
 
 Usage:
   python -m phaser_amd.profiling stats <kernel_stats.csv>
+  python -m phaser_amd.profiling batched <kernel_trace.csv>
   python -m phaser_amd.profiling traffic <fetch counter_collection.csv>
       <write counter_collection.csv> <copy bytes> <out.json>
 """
@@ -54,6 +59,31 @@ def kernel_stats_markdown(path):
         out.append(f"| {short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                    f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
                    f"{float(r['Percentage']):.2f} |")
+    return "\n".join(out)
+
+
+def batched_stats(trace_csv):
+    """{kernel: (grid size, [durations in us])} over each kernel's largest-grid
+    dispatches in a rocprofv3 --kernel-trace CSV."""
+    by = defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):
+        g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        us = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+        by[short_name(r["Kernel_Name"])].append((g, us))
+    out = {}
+    for k, v in by.items():
+        g = max(x[0] for x in v)
+        out[k] = (g, [x[1] for x in v if x[0] == g])
+    return out
+
+
+def batched_stats_markdown(trace_csv):
+    st = batched_stats(trace_csv)
+    rows = sorted(st.items(), key=lambda kv: -sum(kv[1][1]))
+    out = ["| kernel | grid (work-items) | launches | mean us | min us | max us |",
+           "|---|---|---|---|---|---|"]
+    for k, (g, d) in rows:
+        out.append(f"| {k} | {g} | {len(d)} | {sum(d) / len(d):.1f} | {min(d):.1f} | {max(d):.1f} |")
     return "\n".join(out)
 
 
@@ -108,6 +138,8 @@ def lookup(summary, kernel):
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         print(kernel_stats_markdown(sys.argv[2]))
+    elif sys.argv[1] == "batched":
+        print(batched_stats_markdown(sys.argv[2]))
     elif sys.argv[1] == "traffic":
         s = traffic_summary(sys.argv[2], sys.argv[3], int(sys.argv[4]))
         with open(sys.argv[5], "w") as f:
