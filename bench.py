@@ -231,7 +231,7 @@ def main():
     ap.add_argument("--mode", choices=["batch", "serve"], default="batch")
     ap.add_argument("--autotune", type=int, default=1, help="pick FFT column tiles first")
     ap.add_argument("--graphs", type=int, default=1, help="replay the pipeline from HIP graphs")
-    ap.add_argument("--nn-lanes", type=int, default=0, choices=[0, 2, 4, 8, 16],
+    ap.add_argument("--nn-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8, 16],
                     help="lanes per query of the S2 nearest-neighbour search (0: default)")
     ap.add_argument("--streams", type=int, default=2,
                     help="split each batch over this many contexts/streams driven concurrently")

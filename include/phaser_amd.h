@@ -152,7 +152,8 @@ typedef struct {
   int32_t plane_fwd;   /* first forward pass on whole z planes in LDS (N <= 128):
                           1 = on, 0 or -1 = off (default) */
   int32_t nn_lanes;    /* lanes per query of the S^2 nearest-neighbour search:
-                          2, 4, 8 or 16 (0 = 4); the search is exact for every value */
+                          1, 2, 4, 8 or 16, or 0 = automatic (2 for launches of 64
+                          clouds or more, else 4); exact for every value */
   int32_t fused_reg;   /* the r2c, cross-spectrum and c2r passes on the register
                           four-step kernels (N = 64, 128, 256): 1 = on, 0/-1 = off */
   int32_t so3_reg;     /* the SO(3) alpha and gamma passes on the register kernels

@@ -278,6 +278,8 @@ int init_ctx(phz_ctx* c) {
   PHZ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   build_s2_tables(c->B, &c->s2);
   build_voxel_tables(c->N, (float)cf.discretize_lower, (float)cf.discretize_upper, &c->vox);
+  for (int i = 1; i < c->N; ++i)  // the voxel binning relies on it
+    PHZ_CHECK(c->vox.edges[i] > c->vox.edges[i - 1], "voxel edges must be strictly increasing");
   PHZ_CHECK(c->s2.runs.count > 0 && c->vox.runs.count > 0, "fusion structure has too many runs");
   {
     std::vector<float4> g(c->s2.grid.size() / 4);
@@ -355,7 +357,7 @@ int init_ctx(phz_ctx* c) {
   c->tuning.fwd_x = c->tuning.fwd_z = c->tuning.inv_x = fft_cols_default_tile(c->N);
   c->plane_fwd = false;  // measured slower at N = 128 (one workgroup per CU)
   c->tuning.plane_fwd = -1;
-  c->tuning.nn_lanes = 4;  // measured fastest of 4, 8, 16 at B = 64
+  c->tuning.nn_lanes = 0;  // automatic, by clouds per launch
   c->tiles.fused_reg = false;
   c->tuning.fused_reg = -1;
   c->so3_reg = false;
@@ -999,8 +1001,10 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.inv_x = c->tiles.inv_x ? c->tiles.inv_x : fft_cols_default_tile(c->N);
   c->plane_fwd = t.plane_fwd > 0 && spatial_plane_ok(c->N);
   c->tuning.plane_fwd = c->plane_fwd ? 1 : -1;
-  c->tuning.nn_lanes =
-      (t.nn_lanes == 2 || t.nn_lanes == 4 || t.nn_lanes == 8 || t.nn_lanes == 16) ? t.nn_lanes : 4;
+  c->tuning.nn_lanes = (t.nn_lanes == 1 || t.nn_lanes == 2 || t.nn_lanes == 4 ||
+                        t.nn_lanes == 8 || t.nn_lanes == 16)
+                           ? t.nn_lanes
+                           : 0;
   c->tiles.fused_reg = t.fused_reg > 0 && spatial_reg_ok(c->N);
   c->tuning.fused_reg = c->tiles.fused_reg ? 1 : -1;
   c->so3_reg = t.so3_reg > 0 && so3_reg_ok(c->B);

@@ -35,7 +35,11 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
 // the two halves of launch_sample_s2: projection + cell sort, then the NN search
 void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
                      CellBuffers cb, hipStream_t s);
-// lanes per query: 2, 4, 8 or 16 (others: 4)
+// lanes per query: 1, 2, 4, 8 or 16 (others: 4); 0 = nn_auto_lanes(nclouds),
+// 2 for launches of 64 clouds or more (many queries: the search's control
+// flow is shared by more queries per wave) and 4 below (fewer queries: more
+// lanes split each one's scans)
+int nn_auto_lanes(int nclouds);
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s);
 

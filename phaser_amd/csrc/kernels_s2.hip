@@ -558,28 +558,28 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
                      cb.cell_start, cb.next_right, cb.next_left);
 }
 
+template <int L>
+static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
+                        float max_sq, const CellBuffers& cb, float* grids, hipStream_t s) {
+  const int nq = n * n, qpb = 256 / L;
+  dim3 gq((unsigned)((nq + qpb - 1) / qpb), nclouds);
+  hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+                     cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
+                     cb.sorted_val, cap, grids);
+}
+
+int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
+
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s) {
-  const int nq = n * n;
-  const int L = (lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16) ? lanes : 4;
-  const int qpb = 256 / L;
-  dim3 gq((unsigned)((nq + qpb - 1) / qpb), nclouds);
-  if (L == 2)
-    hipLaunchKernelGGL(k_nn<2>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
-                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                       cb.sorted_val, cap, grids);
-  else if (L == 4)
-    hipLaunchKernelGGL(k_nn<4>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
-                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                       cb.sorted_val, cap, grids);
-  else if (L == 16)
-    hipLaunchKernelGGL(k_nn<16>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
-                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                       cb.sorted_val, cap, grids);
-  else if (L == 8)
-    hipLaunchKernelGGL(k_nn<8>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
-                       cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                       cb.sorted_val, cap, grids);
+  if (lanes == 0) lanes = nn_auto_lanes(nclouds);
+  switch (lanes) {
+    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
+    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
+    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
+    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
+    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
+  }
 }
 
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,

@@ -28,18 +28,22 @@ __global__ void k_copy_f4(float4* __restrict__ dst, const float4* __restrict__ s
     dst[i] = src[i];
 }
 
-// igl::histc bin of x against monotone float edges E[0..N)
-__device__ __forceinline__ int histc_bin(float x, const float* E, int N) {
+// igl::histc bin of x against strictly increasing float edges E[0..N),
+// N >= 2.  The reference's binary search returns, for E[0] <= x <= E[N-1],
+// the largest k <= N-2 with E[k] <= x, or N-1 when x == E[N-1] (and N-2 for
+// a NaN, which passes both range tests); -1 outside.  The same k is found
+// from the uniform spacing (inv = (N-1) / (E[N-1] - E[0])) and corrected
+// against the actual edges, one or two lookups instead of log2 N dependent
+// ones.
+__device__ __forceinline__ int histc_bin(float x, const float* E, int N, float inv) {
   if (x < E[0] || x > E[N - 1]) return -1;
-  int l = 0, h = N - 1;
-  while (h - l > 1) {
-    const int k = (h + l) >> 1;
-    if (x < E[k])
-      h = k;
-    else
-      l = k;
-  }
-  return (x == E[h]) ? h : l;
+  if (x != x) return N - 2;
+  if (x == E[N - 1]) return N - 1;
+  int k = (int)((x - E[0]) * inv);
+  k = k < N - 2 ? k : N - 2;
+  while (k > 0 && E[k] > x) --k;
+  while (k < N - 2 && E[k + 1] <= x) ++k;
+  return k;
 }
 
 // kClear = false: accumulate (phase-aligner.cc:98-120); true: zero the same
@@ -58,6 +62,7 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
   extern __shared__ float E[];
   for (int i = threadIdx.x; i < N; i += blockDim.x) E[i] = edges[i];
   __syncthreads();
+  const float inv = (float)(N - 1) / (E[N - 1] - E[0]);
   const int c = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -76,9 +81,9 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
     p.y = y;
     p.z = z;
   }
-    const uint32_t bx = (uint32_t)histc_bin(p.x, E, N);
-    const uint32_t by = (uint32_t)histc_bin(p.y, E, N);
-    const uint32_t bz = (uint32_t)histc_bin(p.z, E, N);
+    const uint32_t bx = (uint32_t)histc_bin(p.x, E, N, inv);
+    const uint32_t by = (uint32_t)histc_bin(p.y, E, N, inv);
+    const uint32_t bz = (uint32_t)histc_bin(p.z, E, N, inv);
     const uint32_t NN = (uint32_t)N;
     const uint32_t lin = (bx * NN + by) + (NN * NN * bz);  // Sub2Ind in uint32
     // the reference skips lin > nf and overruns at nf

@@ -152,6 +152,28 @@ def test_voxelize_matches_oracle(ctx32, po):
     np.testing.assert_allclose(v[1], ch[1], rtol=2e-5, atol=1e-6)
 
 
+@pytest.mark.parametrize("N", [64, 128])
+def test_voxel_bins_at_edges_match_oracle(po, N):
+    """igl::histc at its boundaries: every coordinate exactly on a voxel
+    edge, one float ulp either side of it, and at and beyond the range
+    ends, against the oracle's binary search."""
+    from phaser_amd.capi import Context
+    E = po.linspaced(N, -50.0, 50.0)
+    vals = np.concatenate([E, np.nextafter(E, np.float32(np.inf)), np.nextafter(E, np.float32(-np.inf)),
+                           np.float32([-50.0, 50.0, -50.001, 50.001, 0.0])]).astype(np.float32)
+    rng = np.random.default_rng(N)
+    pts = np.zeros((len(vals) * 3, 4), np.float32)
+    for a in range(3):  # each value once on each axis, the other two random in range
+        blk = pts[a * len(vals):(a + 1) * len(vals)]
+        blk[:, :3] = rng.uniform(-49.0, 49.0, size=(len(vals), 3)).astype(np.float32)
+        blk[:, a] = vals
+        blk[:, 3] = rng.uniform(0, 1, size=len(vals)).astype(np.float32)
+    with Context(bandwidth=16, n_voxels=N, max_points=len(pts)) as ctx:
+        _, cnt = ctx.voxelize(pts)
+    _, hist = po.voxelize(pts, N, hist=True)
+    np.testing.assert_array_equal(cnt, hist.astype(np.float32))
+
+
 def test_spatial_laplace_matches_oracle(ctx32, po):
     src, tgt, _, _ = _pair(6, dict(rings=64, azimuths=1024))
     f = po.voxelize(tgt, 64)
@@ -411,7 +433,7 @@ def test_mfma_batch_matches_fp32_batch():
         assert a["trans_index"] == b["trans_index"]
 
 
-@pytest.mark.parametrize("lanes", [2, 8, 16])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16])
 def test_nn_lanes_give_identical_samples(lanes):
     """Lanes per query change the work split of the exact search only."""
     from phaser_amd.capi import Context
