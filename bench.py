@@ -94,7 +94,7 @@ def concurrent_throughput(args, pairs, max_pts, tuning):
         c = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=len(idx),
                     precision=args.precision)
         c.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
-                     t["fused_reg"], t["so3_reg"], t["occ_fwd"])
+                     t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"])
         if args.graphs:
             c.set_graphs(True)
         ds = [c.device_array(pairs[i][0]) for i in idx]
@@ -235,6 +235,8 @@ def main():
                     help="lanes per query of the S2 nearest-neighbour search (0: default)")
     ap.add_argument("--streams", type=int, default=2,
                     help="split each batch over this many contexts/streams driven concurrently")
+    ap.add_argument("--inv-plane", type=int, default=0, choices=[-1, 0, 1],
+                    help="inverse x + c2r as one pass over z planes in LDS (0: default, on)")
     ap.add_argument("--occ-fwd", type=int, default=0, choices=[-1, 0, 1],
                     help="forward spatial passes skip empty voxel rows/planes (0: default, on)")
     ap.add_argument("--precision", type=int, default=0, choices=[0, 1],
@@ -268,11 +270,11 @@ def main():
     tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
     if args.graphs:
         ctx.set_graphs(True)
-    if args.nn_lanes or args.occ_fwd:
+    if args.nn_lanes or args.occ_fwd or args.inv_plane:
         t = ctx.tuning()
         ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"],
                        args.nn_lanes or t["nn_lanes"], t["fused_reg"], t["so3_reg"],
-                       args.occ_fwd or t["occ_fwd"])
+                       args.occ_fwd or t["occ_fwd"], args.inv_plane or t["inv_plane"])
         tuning = ctx.tuning()
     d_src = [ctx.device_array(s) for s, _, _, _ in pairs]
     d_tgt = [ctx.device_array(t) for _, t, _, _ in pairs]

@@ -50,6 +50,7 @@ struct phz_ctx {
   bool plane_fwd = false;  // whole-plane first forward pass
   bool so3_reg = false;    // SO(3) alpha/gamma passes on the register engine
   bool occ_fwd = true;     // forward passes skip rows / planes without points
+  bool inv_plane = true;   // inverse x + c2r as one pass over z planes in LDS
   phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
@@ -364,6 +365,8 @@ int init_ctx(phz_ctx* c) {
   c->tuning.so3_reg = -1;
   c->occ_fwd = true;
   c->tuning.occ_fwd = 1;
+  c->inv_plane = true;
+  c->tuning.inv_plane = 1;
   return 0;
 }
 
@@ -494,11 +497,17 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
                          c->d_cnt + nt * c->T3, occ_from(c, nt), s);
     }
-    {
-      StageScope t(c, kSpatialFftInv, s);
-      launch_spatial_fft_inv(N, np, c->d_work, c->plan_N, c->d_tw_N, c->tiles, s);
-    }
-    {
+    const bool inv_plane = c->inv_plane && c->tiles.fused_reg &&
+                           c->tuning.inv_x == kRegisterEngine && spatial_inv_plane_ok(N);
+    if (inv_plane) {  // inverse x and c2r in one pass over whole z planes
+      StageScope t(c, kSpatialC2r, s);
+      launch_spatial_inv_plane(N, np, c->d_work, c->d_tw_N, d_trans_corr, c->d_partials,
+                               c->d_keys + np, s);
+    } else {
+      {
+        StageScope t(c, kSpatialFftInv, s);
+        launch_spatial_fft_inv(N, np, c->d_work, c->plan_N, c->d_tw_N, c->tiles, s);
+      }
       StageScope t(c, kSpatialC2r, s);
       if (c->tiles.fused_reg)
         launch_spatial_c2r_reg(N, np, c->d_work, c->d_tw_N, d_trans_corr, c->d_partials,
@@ -1011,6 +1020,8 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.so3_reg = c->so3_reg ? 1 : -1;
   c->occ_fwd = t.occ_fwd >= 0;
   c->tuning.occ_fwd = c->occ_fwd ? 1 : -1;
+  c->inv_plane = t.inv_plane >= 0;
+  c->tuning.inv_plane = c->inv_plane ? 1 : -1;
 }
 
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {

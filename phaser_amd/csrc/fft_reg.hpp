@@ -117,6 +117,37 @@ __device__ __forceinline__ void fft_four_step_reg(float2 (&v)[RegSplit<L>::R1], 
   }
 }
 
+// As fft_four_step_reg with the line's exchange buffer strided: entry i at
+// lbuf[i * stride] (lets a line exchange through its own storage in a
+// larger LDS array).
+template <int DIR, int L>
+__device__ __forceinline__ void fft_four_step_reg_strided(float2 (&v)[RegSplit<L>::R1], int n2,
+                                                          float2* lbuf, int stride,
+                                                          const Twiddles<L>& tw,
+                                                          float2 (&out)[RegSplit<L>::R1]) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
+  dft_reg<R1, DIR>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) {
+    float2 w = tw.w[k1];
+    if (DIR > 0) w.y = -w.y;
+    v[k1] = cmul(v[k1], w);
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) lbuf[(k1 * R2 + n2) * stride] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j) {
+    const int k1 = n2 + R2 * j;
+    float2 u[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) u[m] = lbuf[(k1 * R2 + m) * stride];
+    dft_reg<R2, DIR>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) out[j * R2 + k2] = u[k2];
+  }
+}
+
 // The four-step transform of one line held by R2 threads of a workgroup.
 // On entry thread n2 holds v[n1] = x[n2 + R2 n1]; lbuf is the line's LDS
 // scratch (>= L entries).  store(k, X[k]) is called for k = k1 + R1 k2 with

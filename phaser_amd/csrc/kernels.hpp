@@ -180,6 +180,13 @@ void launch_spatial_cross_zz_reg(int N, int npairs, ChannelSet F, int64_t f_stri
                                  const FusionRuns& runs, const float2* tw, float2* work,
                                  hipStream_t s);
 int spatial_c2r_reg_blocks(int N);
+// inverse FFT along x + c2r along y + argmax in one pass over whole z planes
+// held in LDS (N = 64, 128): replaces launch_spatial_fft_inv followed by
+// launch_spatial_c2r_reg; partials: npairs * N entries
+bool spatial_inv_plane_ok(int N);
+void launch_spatial_inv_plane(int N, int npairs, const float2* work, const float2* tw,
+                              float* corr, unsigned long long* partials,
+                              unsigned long long* keys, hipStream_t s);
 void launch_spatial_c2r_reg(int N, int npairs, const float2* work, const float2* tw, float* corr,
                             unsigned long long* partials, unsigned long long* keys,
                             hipStream_t s);

@@ -779,6 +779,108 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
   if (threadIdx.x == 0) partials[blockIdx.x] = best;
 }
 
+// Inverse FFT along x and c2r along y of one z plane of a pair, with the
+// plane in LDS (L h complex, 66.5 KB at L = 128): the x-transformed plane is
+// never stored.  Workgroup (z, pair); lines are spread as in the other
+// register kernels (line = thread % LINES fastest, n2 = thread / LINES).
+// Pass x: the h columns k2 in groups of LINES, each line exchanging through
+// its own column of the plane (stride h; lines past h use a dummy column and
+// their results are dropped), the results written back to the column.
+// Pass y: row pairs (2r, 2r + 1) packed as one complex line (real parts of
+// k = 0, L/2, FFTW hc2r), exchanging through the pair's own two rows; signed
+// first-maximum argmax over x N + y + N^2 z into partials[pair][z]; the
+// grid is written when corr is given.
+template <int L>
+__global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict__ work,
+                                                        const float2* __restrict__ tw,
+                                                        float* __restrict__ corr,
+                                                        unsigned long long* __restrict__ partials) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2;
+  constexpr int h = L / 2 + 1;
+  __shared__ __align__(16) float2 P[L * h];
+  __shared__ float2 dummy[L];
+  const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
+  const int z = blockIdx.x, pair = blockIdx.y;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  {
+    static_assert((L * h) % 2 == 0, "whole float4s");
+    const float4* src = (const float4*)(work + ((int64_t)pair * L + z) * L * h);
+    for (int i = threadIdx.x; i < L * h / 2; i += blockDim.x) ((float4*)P)[i] = src[i];
+  }
+  __syncthreads();
+  // inverse FFT along x of every column k2
+  for (int k0 = 0; k0 < h; k0 += LINES) {
+    const int k2 = k0 + c;
+    const bool live = k2 < h;
+    float2* col = live ? P + k2 : dummy;
+    const int stride = live ? h : 1;
+    float2 v[R1], out[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) v[n1] = live ? col[(n2 + R2 * n1) * stride] : make_float2(0.f, 0.f);
+    __syncthreads();  // every line has read its column before it is overwritten
+    fft_four_step_reg_strided<+1, L>(v, n2, col, stride, w, out);
+    __syncthreads();  // the exchange is read before the results go back
+    if (live) {
+#pragma unroll
+      for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+        for (int kk = 0; kk < R2; ++kk) col[((n2 + R2 * j) + R1 * kk) * h] = out[j * R2 + kk];
+    }
+    __syncthreads();
+  }
+  // c2r along y, two rows per complex line
+  unsigned long long best = 0ull;
+  for (int r0 = 0; r0 < L / 2; r0 += LINES) {
+    const int ra = 2 * (r0 + c);
+    float2* xa = P + ra * h;
+    const float2* xb = xa + h;
+    float2 v[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const int m = n2 + R2 * n1;
+      float2 va, vb;
+      if (m == 0 || 2 * m == L) {
+        va = make_float2(xa[m].x, 0.f);
+        vb = make_float2(xb[m].x, 0.f);
+      } else if (m < h) {
+        va = xa[m];
+        vb = xb[m];
+      } else {
+        va = cconj(xa[L - m]);
+        vb = cconj(xb[L - m]);
+      }
+      v[n1] = make_float2(va.x - vb.y, va.y + vb.x);
+    }
+    __syncthreads();  // the rows are read before the exchange overwrites them
+    const uint32_t rowa = (uint32_t)(z * L + ra);
+    fft_four_step<+1, L>(v, n2, xa, w, [&](int y, float2 zz) {
+      const uint32_t la = rowa * (uint32_t)L + (uint32_t)y, lb = la + (uint32_t)L;
+      if (corr) {
+        corr[(int64_t)pair * L * L * L + la] = zz.x;
+        corr[(int64_t)pair * L * L * L + lb] = zz.y;
+      }
+      const unsigned long long ka = make_key(zz.x, la), kb = make_key(zz.y, lb);
+      best = ka > best ? ka : best;
+      best = kb > best ? kb : best;
+    });
+  }
+  best = block_max_key(best);
+  if (threadIdx.x == 0) partials[(int64_t)pair * L + z] = best;
+}
+
+bool spatial_inv_plane_ok(int N) { return N == 64 || N == 128; }
+
+void launch_spatial_inv_plane(int N, int npairs, const float2* work, const float2* tw,
+                              float* corr, unsigned long long* partials,
+                              unsigned long long* keys, hipStream_t s) {
+  const dim3 g(N, npairs);
+  if (N == 64)
+    hipLaunchKernelGGL(k_spat_inv_plane<64>, g, dim3(256), 0, s, work, tw, corr, partials);
+  else
+    hipLaunchKernelGGL(k_spat_inv_plane<128>, g, dim3(256), 0, s, work, tw, corr, partials);
+  launch_reduce_keys(partials, N, npairs, keys, s);
+}
+
 bool spatial_reg_ok(int N) { return N == 64 || N == 128 || N == 256; }
 
 template <int L>

@@ -79,7 +79,8 @@ _LAYOUT_C = r"""
 int main(void) {
   S(phz_config); S(phz_result); S(phz_stage_time); S(phz_tuning);
   O(phz_tuning, plane_fwd); O(phz_tuning, nn_lanes); O(phz_tuning, fused_reg);
-  O(phz_tuning, so3_reg); O(phz_tuning, occ_fwd); O(phz_tuning, ms_fwd_x); O(phz_tuning, ms_plane);
+  O(phz_tuning, so3_reg); O(phz_tuning, occ_fwd); O(phz_tuning, inv_plane);
+  O(phz_tuning, ms_fwd_x); O(phz_tuning, ms_plane);
   O(phz_tuning, ms_fused); O(phz_tuning, ms_so3);
   S(phs_config); S(phs_request); S(phs_response); S(phs_stats);
   O(phz_config, max_points); O(phz_config, device); O(phz_result, zyz);

@@ -495,6 +495,33 @@ def test_occupancy_skipping_matches_dense_register_passes(N):
         np.testing.assert_array_equal(a["translation"], b["translation"])
 
 
+@pytest.mark.parametrize("N", [64, 128])
+def test_inverse_plane_pass_matches_two_passes(N):
+    """Inverse x + c2r + argmax in one pass over z planes in LDS against the
+    separate register passes: grids within 1e-6 of their maximum (the
+    arithmetic is the same, the compiler's FMA contraction may differ), bins
+    and pose identical, for a single pair and a batch."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=N, azimuths=1024, overlap=0.7)) for s in (12, 13)]
+    out = {}
+    with Context(bandwidth=N // 2, n_voxels=N, max_points=200000, max_batch=2) as ctx:
+        for ip in (-1, 1):
+            ctx.set_tuning(1, 1, 1, fused_reg=1, inv_plane=ip)
+            assert ctx.tuning()["inv_plane"] == ip
+            single = ctx.register(pairs[0][1], pairs[0][0], grids=True)
+            batch = ctx.register_pairs([t for _, t, _, _ in pairs], [s for s, _, _, _ in pairs])
+            out[ip] = (single, batch)
+    a, b = out[-1][0], out[1][0]
+    err = _rel(b["trans_corr"], a["trans_corr"].astype(np.float64))
+    print(f"N {N}: trans {a['trans_index']} / {b['trans_index']}, grid err {err:.2e}", flush=True)
+    assert err < 1e-6
+    assert a["trans_index"] == b["trans_index"]
+    np.testing.assert_array_equal(a["translation"], b["translation"])
+    for x, y in zip(out[-1][1], out[1][1]):
+        assert x["trans_index"] == y["trans_index"]
+        assert abs(x["trans_value"] - y["trans_value"]) <= 1e-6 * abs(x["trans_value"])
+
+
 def test_occupancy_skipping_batch_matches_dense():
     """As above for batches (separate and shared targets): same bins, peak
     values within FP32 rounding."""
