@@ -625,7 +625,7 @@ __device__ __forceinline__ float2 fuse_pair(float2 x0, float2 x1, int C, int mod
 // written by the x pass.  A thread's forward output (k = n2 + R2 j + R1 k2)
 // is the inverse transform's input of n1 = j + (R1 / R2) k2.
 template <int L>
-__global__ void __launch_bounds__(256) k_spat_cross_zz_reg(
+__global__ void __launch_bounds__(256, L <= 128 ? 3 : 2) k_spat_cross_zz_reg(
     ChannelSet F, int64_t f_stride, ChannelSet G, int64_t g_stride,
     const uint8_t* __restrict__ f_occ, int64_t f_occ_stride, const uint8_t* __restrict__ g_occ,
     int64_t g_occ_stride, FusionRuns runs, const float2* __restrict__ tw,
@@ -781,15 +781,17 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
 
 // Inverse FFT along x and c2r along y of one z plane of a pair, with the
 // plane in LDS (L h complex, 66.5 KB at L = 128): the x-transformed plane is
-// never stored.  Workgroup (z, pair); lines are spread as in the other
-// register kernels (line = thread % LINES fastest, n2 = thread / LINES).
-// Pass x: the h columns k2 in groups of LINES, each line exchanging through
-// its own column of the plane (stride h; lines past h use a dummy column and
-// their results are dropped), the results written back to the column.
-// Pass y: row pairs (2r, 2r + 1) packed as one complex line (real parts of
-// k = 0, L/2, FFTW hc2r), exchanging through the pair's own two rows; signed
-// first-maximum argmax over x N + y + N^2 z into partials[pair][z]; the
-// grid is written when corr is given.
+// never stored.  Workgroup (z, pair) of 256 threads: lines of R2 threads,
+// LINES = 256 / R2 at a time (line = thread % LINES fastest, n2 = thread /
+// LINES).  Pass x: L / 2 lines, one per column k2 = 1 .. L/2 - 1 and line 0
+// for columns 0 and L/2 together: c2r reads only the real parts of those two,
+// and Re IFFT(a) + i Re IFFT(b) = IFFT(a_h + i b_h) with the Hermitian parts
+// a_h[k] = (a[k] + conj a[-k]) / 2.  Each line exchanges through its own
+// column (stride h) and writes its results back there.  Pass y: row pairs
+// (2r, 2r + 1) packed as one complex line (real parts of k = 0, L/2, FFTW
+// hc2r), exchanging through the pair's own two rows; signed first-maximum
+// argmax over x N + y + N^2 z into partials[pair][z]; the grid is written
+// when corr is given.
 template <int L>
 __global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict__ work,
                                                         const float2* __restrict__ tw,
@@ -797,8 +799,8 @@ __global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict
                                                         unsigned long long* __restrict__ partials) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2;
   constexpr int h = L / 2 + 1;
+  static_assert((L / 2) % LINES == 0, "whole passes");
   __shared__ __align__(16) float2 P[L * h];
-  __shared__ float2 dummy[L];
   const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
   const int z = blockIdx.x, pair = blockIdx.y;
   const Twiddles<L> w = load_twiddles<L>(n2, tw);
@@ -808,36 +810,54 @@ __global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict
     for (int i = threadIdx.x; i < L * h / 2; i += blockDim.x) ((float4*)P)[i] = src[i];
   }
   __syncthreads();
-  // inverse FFT along x of every column k2
-  for (int k0 = 0; k0 < h; k0 += LINES) {
-    const int k2 = k0 + c;
-    const bool live = k2 < h;
-    float2* col = live ? P + k2 : dummy;
-    const int stride = live ? h : 1;
+  // inverse FFT along x
+  for (int g0 = 0; g0 < L / 2; g0 += LINES) {
+    const int k2 = g0 + c;  // line 0: columns 0 and L/2
     float2 v[R1], out[R1];
+    if (k2 == 0) {
 #pragma unroll
-    for (int n1 = 0; n1 < R1; ++n1) v[n1] = live ? col[(n2 + R2 * n1) * stride] : make_float2(0.f, 0.f);
-    __syncthreads();  // every line has read its column before it is overwritten
-    fft_four_step_reg_strided<+1, L>(v, n2, col, stride, w, out);
-    __syncthreads();  // the exchange is read before the results go back
-    if (live) {
+      for (int n1 = 0; n1 < R1; ++n1) {
+        const int x = n2 + R2 * n1, xm = (L - x) & (L - 1);
+        const float2 a = P[x * h], ar = P[xm * h], b = P[x * h + L / 2], br = P[xm * h + L / 2];
+        const float2 ah = make_float2(0.5f * (a.x + ar.x), 0.5f * (a.y - ar.y));
+        const float2 bh = make_float2(0.5f * (b.x + br.x), 0.5f * (b.y - br.y));
+        v[n1] = make_float2(ah.x - bh.y, ah.y + bh.x);
+      }
+    } else {
 #pragma unroll
-      for (int j = 0; j < R1 / R2; ++j)
-#pragma unroll
-        for (int kk = 0; kk < R2; ++kk) col[((n2 + R2 * j) + R1 * kk) * h] = out[j * R2 + kk];
+      for (int n1 = 0; n1 < R1; ++n1) v[n1] = P[(n2 + R2 * n1) * h + k2];
     }
+    __syncthreads();  // the columns are read before the exchanges overwrite them
+    fft_four_step_reg_strided<+1, L>(v, n2, P + k2, h, w, out);
+    __syncthreads();  // the exchanges are read before the results go back
+#pragma unroll
+    for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+      for (int kk = 0; kk < R2; ++kk) {
+        const int x = (n2 + R2 * j) + R1 * kk;
+        const float2 o = out[j * R2 + kk];
+        if (k2 == 0) {
+          P[x * h] = make_float2(o.x, 0.f);
+          P[x * h + L / 2] = make_float2(o.y, 0.f);
+        } else {
+          P[x * h + k2] = o;
+        }
+      }
     __syncthreads();
   }
-  // c2r along y, two rows per complex line
+  // c2r along y, two rows per complex line; here the R2 threads of a line
+  // are adjacent (n2 fastest) so that a line's grid stores are contiguous
+  const int ln = threadIdx.x / R2, m2 = threadIdx.x % R2;
+  const Twiddles<L> wy = load_twiddles<L>(m2, tw);
   unsigned long long best = 0ull;
   for (int r0 = 0; r0 < L / 2; r0 += LINES) {
-    const int ra = 2 * (r0 + c);
+    const int ra = 2 * (r0 + ln);
     float2* xa = P + ra * h;
     const float2* xb = xa + h;
     float2 v[R1];
 #pragma unroll
     for (int n1 = 0; n1 < R1; ++n1) {
-      const int m = n2 + R2 * n1;
+      const int m = m2 + R2 * n1;
       float2 va, vb;
       if (m == 0 || 2 * m == L) {
         va = make_float2(xa[m].x, 0.f);
@@ -853,7 +873,7 @@ __global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict
     }
     __syncthreads();  // the rows are read before the exchange overwrites them
     const uint32_t rowa = (uint32_t)(z * L + ra);
-    fft_four_step<+1, L>(v, n2, xa, w, [&](int y, float2 zz) {
+    fft_four_step<+1, L>(v, m2, xa, wy, [&](int y, float2 zz) {
       const uint32_t la = rowa * (uint32_t)L + (uint32_t)y, lb = la + (uint32_t)L;
       if (corr) {
         corr[(int64_t)pair * L * L * L + la] = zz.x;
