@@ -32,101 +32,130 @@ __device__ __forceinline__ float2 combine(const float2* f, const float2* h, floa
   return make_float2(wn * (s.x * pr - s.y * pi), wn * (s.x * pi + s.y * pr));
 }
 
-// One workgroup per Wigner block (a <= b) and pair; thread k = beta index.
-// Columns: 0 -> (a,b) direct, 1 -> (b,a) direct, 2/3 -> beta-flipped
-// (see the case table in the host launcher).
-__global__ void k_so3_synth(int bw, const float2* __restrict__ fc, int64_t f_stride,
+// One workgroup per Wigner block (a <= b) and group of NP pairs; thread
+// k = beta index.  The block's table column is read once and applied to the
+// NP pairs (each pair's sums in the same order as alone, so the result does
+// not depend on NP).  Columns: 0 -> (a,b) direct, 1 -> (b,a) direct, 2/3 ->
+// beta-flipped (see the case table in the host launcher).
+template <int NP>
+__global__ void k_so3_synth(int bw, int npairs, const float2* __restrict__ fc, int64_t f_stride,
                             const float2* __restrict__ hc, int64_t h_stride,
                             const float* __restrict__ wig,
                             const int64_t* __restrict__ wig_off, const float* __restrict__ wnorm,
                             float2* __restrict__ S) {
   const int n = 2 * bw;
-  const int blk = blockIdx.x, pair = blockIdx.y;
+  const int blk = blockIdx.x, p0 = blockIdx.y * NP;
   int b = (int)((sqrtf(8.f * blk + 1.f) - 1.f) * 0.5f);
   while ((b + 1) * (b + 2) / 2 <= blk) ++b;
   while (b * (b + 1) / 2 > blk) --b;
   const int a = blk - b * (b + 1) / 2;
   const int L = bw - b;
-  const float2* f = fc + pair * f_stride;
-  const float2* h = hc + pair * h_stride;
-  extern __shared__ float2 col[];  // 4 x bw
+  extern __shared__ float2 col[];  // NP x 4 x bw
   // case: 0 -> (0,0); 1 -> a=0<b; 2 -> 0<a=b; 3 -> 0<a<b
   const int cs = (b == 0) ? 0 : (a == 0 ? 1 : (a == b ? 2 : 3));
-  for (int t = threadIdx.x; t < L; t += blockDim.x) {
-    const int l = b + t;
-    const float wn = wnorm[l];
-    float2 c0 = combine(f, h, wn, l, a, b, bw);
-    float2 c1 = make_float2(0.f, 0.f), c2 = c1, c3 = c1;
-    if (cs == 1) {
-      c1 = combine(f, h, wn, l, b, 0, bw);
-      if (b & 1) c1 = make_float2(-c1.x, -c1.y);
-    } else if (cs == 2) {
-      c2 = combine(f, h, wn, l, -a, a, bw);
-      const float sg = ((l - a) & 1) ? -1.f : 1.f;
-      c2 = make_float2(sg * c2.x, sg * c2.y);
-    } else if (cs == 3) {
-      c1 = combine(f, h, wn, l, b, a, bw);
-      if ((b - a) & 1) c1 = make_float2(-c1.x, -c1.y);
-      const float sg = ((l + a) & 1) ? -1.f : 1.f;
-      c2 = combine(f, h, wn, l, a, -b, bw);
-      c2 = make_float2(sg * c2.x, sg * c2.y);
-      c3 = combine(f, h, wn, l, b, -a, bw);
-      c3 = make_float2(sg * c3.x, sg * c3.y);
+  for (int q = 0; q < NP; ++q) {
+    const int pair = p0 + q < npairs ? p0 + q : npairs - 1;  // spare slots repeat the last
+    const float2* f = fc + pair * f_stride;
+    const float2* h = hc + pair * h_stride;
+    float2* cq = col + q * 4 * bw;
+    for (int t = threadIdx.x; t < L; t += blockDim.x) {
+      const int l = b + t;
+      const float wn = wnorm[l];
+      float2 c0 = combine(f, h, wn, l, a, b, bw);
+      float2 c1 = make_float2(0.f, 0.f), c2 = c1, c3 = c1;
+      if (cs == 1) {
+        c1 = combine(f, h, wn, l, b, 0, bw);
+        if (b & 1) c1 = make_float2(-c1.x, -c1.y);
+      } else if (cs == 2) {
+        c2 = combine(f, h, wn, l, -a, a, bw);
+        const float sg = ((l - a) & 1) ? -1.f : 1.f;
+        c2 = make_float2(sg * c2.x, sg * c2.y);
+      } else if (cs == 3) {
+        c1 = combine(f, h, wn, l, b, a, bw);
+        if ((b - a) & 1) c1 = make_float2(-c1.x, -c1.y);
+        const float sg = ((l + a) & 1) ? -1.f : 1.f;
+        c2 = combine(f, h, wn, l, a, -b, bw);
+        c2 = make_float2(sg * c2.x, sg * c2.y);
+        c3 = combine(f, h, wn, l, b, -a, bw);
+        c3 = make_float2(sg * c3.x, sg * c3.y);
+      }
+      cq[t] = c0;
+      cq[bw + t] = c1;
+      cq[2 * bw + t] = c2;
+      cq[3 * bw + t] = c3;
     }
-    col[t] = c0;
-    col[bw + t] = c1;
-    col[2 * bw + t] = c2;
-    col[3 * bw + t] = c3;
   }
   __syncthreads();
   const int k = threadIdx.x;
   if (k >= n) return;
   const float* T = wig + wig_off[blk];
-  float2 a0 = make_float2(0.f, 0.f), a1 = a0, a2 = a0, a3 = a0;
+  float2 acc[NP][4];
+#pragma unroll
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[q][j] = make_float2(0.f, 0.f);
   if (cs <= 1) {
     for (int t = 0; t < L; ++t) {
       const float d = T[(int64_t)t * n + k];
-      a0.x += col[t].x * d;
-      a0.y += col[t].y * d;
-      a1.x += col[bw + t].x * d;
-      a1.y += col[bw + t].y * d;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const float2* cq = col + q * 4 * bw;
+        acc[q][0].x += cq[t].x * d;
+        acc[q][0].y += cq[t].y * d;
+        acc[q][1].x += cq[bw + t].x * d;
+        acc[q][1].y += cq[bw + t].y * d;
+      }
     }
   } else if (cs == 2) {
     for (int t = 0; t < L; ++t) {
       const float d = T[(int64_t)t * n + k];
       const float df = T[(int64_t)t * n + (n - 1 - k)];
-      a0.x += col[t].x * d;
-      a0.y += col[t].y * d;
-      a2.x += col[2 * bw + t].x * df;
-      a2.y += col[2 * bw + t].y * df;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const float2* cq = col + q * 4 * bw;
+        acc[q][0].x += cq[t].x * d;
+        acc[q][0].y += cq[t].y * d;
+        acc[q][2].x += cq[2 * bw + t].x * df;
+        acc[q][2].y += cq[2 * bw + t].y * df;
+      }
     }
   } else {
     for (int t = 0; t < L; ++t) {
       const float d = T[(int64_t)t * n + k];
       const float df = T[(int64_t)t * n + (n - 1 - k)];
-      a0.x += col[t].x * d;
-      a0.y += col[t].y * d;
-      a1.x += col[bw + t].x * d;
-      a1.y += col[bw + t].y * d;
-      a2.x += col[2 * bw + t].x * df;
-      a2.y += col[2 * bw + t].y * df;
-      a3.x += col[3 * bw + t].x * df;
-      a3.y += col[3 * bw + t].y * df;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const float2* cq = col + q * 4 * bw;
+        acc[q][0].x += cq[t].x * d;
+        acc[q][0].y += cq[t].y * d;
+        acc[q][1].x += cq[bw + t].x * d;
+        acc[q][1].y += cq[bw + t].y * d;
+        acc[q][2].x += cq[2 * bw + t].x * df;
+        acc[q][2].y += cq[2 * bw + t].y * df;
+        acc[q][3].x += cq[3 * bw + t].x * df;
+        acc[q][3].y += cq[3 * bw + t].y * df;
+      }
     }
   }
   const int w1 = 2 * bw - 1;
-  float2* Sp = S + (int64_t)pair * bw * w1 * n;
-  auto at = [&](int m1, int m2) -> float2* { return Sp + ((int64_t)m2 * w1 + (m1 + bw - 1)) * n + k; };
-  *at(a, b) = a0;  // (a,b); also (0,0) and (a,a)
-  if (cs == 1) {
-    *at(b, 0) = a1;
-    *at(-b, 0) = cconj(a1);
-  } else if (cs == 2) {
-    *at(-a, a) = a2;
-  } else if (cs == 3) {
-    *at(b, a) = a1;
-    *at(-a, b) = cconj(a2);
-    *at(-b, a) = cconj(a3);
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    if (p0 + q >= npairs) break;
+    float2* Sp = S + (int64_t)(p0 + q) * bw * w1 * n;
+    auto at = [&](int m1, int m2) -> float2* {
+      return Sp + ((int64_t)m2 * w1 + (m1 + bw - 1)) * n + k;
+    };
+    *at(a, b) = acc[q][0];  // (a,b); also (0,0) and (a,a)
+    if (cs == 1) {
+      *at(b, 0) = acc[q][1];
+      *at(-b, 0) = cconj(acc[q][1]);
+    } else if (cs == 2) {
+      *at(-a, a) = acc[q][2];
+    } else if (cs == 3) {
+      *at(b, a) = acc[q][1];
+      *at(-a, b) = cconj(acc[q][2]);
+      *at(-b, a) = cconj(acc[q][3]);
+    }
   }
 }
 
@@ -407,8 +436,18 @@ void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const floa
   const int n = 2 * bw;
   const int nblk = bw * (bw + 1) / 2;
   const int th = ((n + 63) / 64) * 64;
-  hipLaunchKernelGGL(k_so3_synth, dim3(nblk, npairs), dim3(th), 4 * bw * sizeof(float2), s, bw,
-                     f_coef, f_stride, h_coef, h_stride, wigner, wig_off_dev, wig_norm, S);
+  // table columns shared by four pairs per workgroup once there are enough
+  // pairs to keep the device busy
+  if (npairs >= 8) {
+    constexpr int NP = 4;
+    hipLaunchKernelGGL(k_so3_synth<NP>, dim3(nblk, (npairs + NP - 1) / NP), dim3(th),
+                       NP * 4 * bw * sizeof(float2), s, bw, npairs, f_coef, f_stride, h_coef,
+                       h_stride, wigner, wig_off_dev, wig_norm, S);
+  } else {
+    hipLaunchKernelGGL(k_so3_synth<1>, dim3(nblk, npairs), dim3(th), 4 * bw * sizeof(float2), s,
+                       bw, npairs, f_coef, f_stride, h_coef, h_stride, wigner, wig_off_dev,
+                       wig_norm, S);
+  }
 }
 
 bool so3_reg_ok(int bw) { return bw == 32 || bw == 64 || bw == 128; }

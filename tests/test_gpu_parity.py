@@ -292,6 +292,20 @@ def test_pairs_batch_matches_single(ctx32):
     ctx.close()
 
 
+def test_large_batch_matches_single(ctx32):
+    """Nine pairs in one launch (stages that group pairs per workgroup run
+    with a partly filled last group) give every pair its single-pair result;
+    the SO(3) peak value is bit-identical (same sums in the same order)."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=64, azimuths=1024)) for s in range(20, 29)]
+    singles = [ctx32.register(t, s) for s, t, _, _ in pairs]
+    with Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=9) as ctx:
+        res = ctx.register_pairs([t for _, t, _, _ in pairs], [s for s, _, _, _ in pairs])
+    for a, b in zip(res, singles):
+        assert a["rot_index"] == b["rot_index"] and a["rot_value"] == b["rot_value"]
+        assert a["trans_index"] == b["trans_index"]
+
+
 def test_profiling_stage_times(ctx32):
     src, tgt, _, _ = _pair(8, dict(rings=64, azimuths=1024))
     ctx32.set_profiling(True)
