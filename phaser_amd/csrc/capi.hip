@@ -457,8 +457,11 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     if (c->plane_fwd)
       launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                                c->d_spec, s);
+    else if (occ)
+      launch_spatial_r2c_rows(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, occ_from(c, 0), c->d_tw_N,
+                              c->d_spec, s);
     else if (c->tiles.fused_reg)
-      launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_row_occ, occ,
+      launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_row_occ, false,
                              c->d_tw_N, c->d_spec, s);
     else
       launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,

@@ -154,6 +154,11 @@ bool spatial_reg_ok(int N);
 void launch_spatial_r2c_reg(int N, int nclouds, const float* sumI, const float* sumR,
                             const uint32_t* cnt, const uint8_t* row_occ, bool sparse_out,
                             const float2* tw, float2* spec, hipStream_t s);
+// the sparse r2c over listed occupied rows, one workgroup per z plane (the
+// same output as launch_spatial_r2c_reg with sparse_out)
+void launch_spatial_r2c_rows(int N, int nclouds, const float* sumI, const float* sumR,
+                             const uint32_t* cnt, VoxelOcc occ, const float2* tw, float2* spec,
+                             hipStream_t s);
 // the forward x pass (launch_spatial_fft_pass which = 0) on the register
 // engine over the spectra of nbatch / 2 clouds left by a sparse_out r2c:
 // reads only the rows that hold points (the others are zero and were never

@@ -559,6 +559,93 @@ __global__ void __launch_bounds__(256) k_spat_r2c_reg(
   }
 }
 
+// pass 1 over the occupied rows only: one workgroup per (z plane, cloud).
+// A plane without points ends at once; otherwise its occupied rows are
+// listed (ascending) in LDS and transformed LINES at a time, so rows from
+// different parts of the plane share a batch.  Rows without points are
+// neither read nor written (their spectrum is zero; only
+// launch_spatial_fft_x_occ reads the result).
+template <int L>
+__global__ void __launch_bounds__(256) k_spat_r2c_rows(
+    const float* __restrict__ sumI, const float* __restrict__ sumR,
+    const uint32_t* __restrict__ cnt, int64_t cloud_stride, const uint8_t* __restrict__ row_occ,
+    const uint8_t* __restrict__ plane_occ, const float2* __restrict__ tw,
+    float2* __restrict__ spec, int64_t spec_stride0, int64_t spec_stride1) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
+  constexpr int h = L / 2 + 1;
+  static_assert(L <= 256, "one row per thread when listing");
+  __shared__ float2 buf[LINES * S];
+  __shared__ int rows[L];
+  __shared__ int nrows;
+  const int z = blockIdx.x, c = blockIdx.y;
+  if (plane_occ[(int64_t)c * L + z] == 0) return;  // uniform over the workgroup
+  const int ln = threadIdx.x / R2, n2 = threadIdx.x % R2;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  // list the occupied rows x of the plane in ascending order
+  {
+    const int x = threadIdx.x;
+    const bool occ = x < L && row_occ[((int64_t)c * L + z) * L + x] != 0;
+    const unsigned long long m = __ballot(occ);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ int wcount[4];
+    if (lane == 0) wcount[wv] = __popcll(m);
+    __syncthreads();
+    int base = 0;
+    for (int i = 0; i < wv; ++i) base += wcount[i];
+    if (occ) rows[base + __popcll(m & ((1ull << lane) - 1ull))] = x;
+    if (threadIdx.x == 0) nrows = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+    __syncthreads();
+  }
+  const int nr = nrows;
+  const int64_t plane_off = (int64_t)z * L;  // first row of the plane
+  for (int r0 = 0; r0 < nr; r0 += LINES) {
+    const bool live = r0 + ln < nr;
+    const int x = live ? rows[r0 + ln] : 0;
+    const int64_t off = c * cloud_stride + (plane_off + x) * L;
+    float2 v[R1];
+    if (live) {
+      uint32_t k[R1];
+      float si[R1], sr[R1];
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        const int64_t o = off + n2 + R2 * n1;
+        k[n1] = cnt[o];
+        si[n1] = sumI[o];
+        sr[n1] = sumR[o];
+      }
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1)
+        v[n1] = k[n1] ? make_float2(si[n1] / (float)k[n1], sr[n1] / (float)k[n1])
+                      : make_float2(0.f, 0.f);
+    } else {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) v[n1] = make_float2(0.f, 0.f);
+    }
+    float2 out[R1];
+    fft_four_step_reg<-1, L>(v, n2, buf + ln * S, w, out);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+      for (int k2 = 0; k2 < R2; ++k2) buf[ln * S + (n2 + R2 * j) + R1 * k2] = out[j * R2 + k2];
+    __syncthreads();
+    // split Z = I^ + i R^ into the two half spectra, one row of h at a time
+    float2* o0 = spec + c * spec_stride0 + plane_off * h;
+    float2* o1 = o0 + spec_stride1;
+    const int nlive = nr - r0 < LINES ? nr - r0 : LINES;
+    for (int idx = threadIdx.x; idx < nlive * h; idx += blockDim.x) {
+      const int r = idx / h, k = idx - r * h;
+      const int64_t o = (int64_t)rows[r0 + r] * h + k;
+      const float2 zk = buf[r * S + k];
+      const float2 zc = cconj(buf[r * S + (L - k) % L]);
+      const float2 sm = cadd(zk, zc), d = csub(zk, zc);
+      o0[o] = make_float2(0.5f * sm.x, 0.5f * sm.y);
+      o1[o] = make_float2(0.5f * d.y, -0.5f * d.x);
+    }
+    __syncthreads();  // buf is reused by the next batch
+  }
+}
+
 // inverse pass A: fused F, G -> C = F conj(G) -> inverse FFT along z, for
 // lines q = (x, k2) of stride inner = N h; pair blockIdx.y.
 template <int L>
@@ -912,6 +999,23 @@ static void launch_reg_passes_r2c(int nclouds, const float* sumI, const float* s
   const int64_t H = (int64_t)L * L * (L / 2 + 1), T = (int64_t)L * L * L;
   hipLaunchKernelGGL(k_spat_r2c_reg<L>, dim3(L * L / LINES, nclouds), dim3(256), 0, s, sumI, sumR,
                      cnt, T, row_occ, sparse_out, tw, spec, 2 * H, H);
+}
+
+template <int L>
+static void launch_r2c_rows_t(int nclouds, const float* sumI, const float* sumR,
+                              const uint32_t* cnt, VoxelOcc occ, const float2* tw, float2* spec,
+                              hipStream_t s) {
+  const int64_t H = (int64_t)L * L * (L / 2 + 1), T = (int64_t)L * L * L;
+  hipLaunchKernelGGL(k_spat_r2c_rows<L>, dim3(L, nclouds), dim3(256), 0, s, sumI, sumR, cnt, T,
+                     occ.row, occ.plane, tw, spec, 2 * H, H);
+}
+
+void launch_spatial_r2c_rows(int N, int nclouds, const float* sumI, const float* sumR,
+                             const uint32_t* cnt, VoxelOcc occ, const float2* tw, float2* spec,
+                             hipStream_t s) {
+  if (N == 64) launch_r2c_rows_t<64>(nclouds, sumI, sumR, cnt, occ, tw, spec, s);
+  else if (N == 128) launch_r2c_rows_t<128>(nclouds, sumI, sumR, cnt, occ, tw, spec, s);
+  else launch_r2c_rows_t<256>(nclouds, sumI, sumR, cnt, occ, tw, spec, s);
 }
 
 void launch_spatial_r2c_reg(int N, int nclouds, const float* sumI, const float* sumR,
