@@ -182,7 +182,7 @@ int phz_get_tuning(const phz_ctx* ctx, phz_tuning* out);
 int phz_set_tuning(phz_ctx* ctx, const phz_tuning* tuning);
 /* Nonzero: the register entry points replay the pipeline from captured HIP
  * graphs (one per batch size, shared-target flag and output grids; at most
- * 8 kept), which removes per-kernel launch overhead; inputs are still copied
+ * 32 kept), which removes per-kernel launch overhead; inputs are still copied
  * and results fetched outside the graph.  Off while stage timing is on. */
 int phz_set_graphs(phz_ctx* ctx, int enable);
 

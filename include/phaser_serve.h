@@ -50,7 +50,8 @@ typedef struct {
   double batch_window_ms;      /* a free worker with fewer than max_batch
                                   requests waits this long for more (0: none) */
   int32_t precision;           /* phz_config.precision of every context */
-  int32_t reserved;
+  int32_t graphs;              /* nonzero: contexts replay their batches from HIP graphs
+                                  (phz_set_graphs); default 1 */
 } phs_config;
 
 typedef struct {

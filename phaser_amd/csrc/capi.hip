@@ -563,7 +563,7 @@ int exec_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* rot, float
     set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ei));
     return -2;
   }
-  if (c->graphs.size() >= 8) {
+  if (c->graphs.size() >= 32) {
     (void)hipGraphExecDestroy(c->graphs.front().exec);
     c->graphs.erase(c->graphs.begin());
   }

@@ -228,12 +228,11 @@ void worker(phs_server* s, Lane* L) {
   const phz_config cfg = shape_config(s->cfg, sh->B, sh->N);
   phz_ctx* ctx = nullptr;
   int crc = phz_create(&cfg, &ctx);
-  if (!crc && s->cfg.autotune) {
-    crc = phz_autotune(ctx, 3, nullptr);
-    if (crc) {
-      phz_destroy(ctx);
-      ctx = nullptr;
-    }
+  if (!crc && s->cfg.autotune) crc = phz_autotune(ctx, 3, nullptr);
+  if (!crc && s->cfg.graphs) crc = phz_set_graphs(ctx, 1);
+  if (crc && ctx) {
+    phz_destroy(ctx);
+    ctx = nullptr;
   }
   const std::string cerr = crc ? std::string(phz_last_error()) : std::string();
   std::unique_lock<std::mutex> lk(s->mu);
@@ -344,6 +343,7 @@ void phs_default_config(phs_config* cfg) {
   cfg->max_points = 262144;
   cfg->hbm_budget = 64ull << 30;
   cfg->batch_window_ms = 0.0;
+  cfg->graphs = 1;
 }
 
 const char* phs_last_error(void) { return g_error.c_str(); }

@@ -104,6 +104,12 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   return 0;
 }
 
+int phz_set_graphs(phz_ctx* c, int enable) {
+  (void)c;
+  (void)enable;
+  return 0;
+}
+
 void phz_destroy(phz_ctx* c) {
   if (!c) return;
   fake::live_bytes -= (long long)c->bytes;

@@ -86,7 +86,7 @@ int main(void) {
   O(phz_config, max_points); O(phz_config, device); O(phz_result, zyz);
   O(phz_result, T_rot); O(phz_result, translation);
   O(phs_config, max_points); O(phs_config, hbm_budget); O(phs_config, batch_window_ms);
-  O(phs_config, precision); O(phz_config, precision);
+  O(phs_config, precision); O(phs_config, graphs); O(phz_config, precision);
   O(phs_request, target); O(phs_request, Ps); O(phs_request, tag);
   O(phs_response, result); O(phs_response, latency_ms); O(phs_response, error);
   O(phs_stats, contexts_live); O(phs_stats, queued);
