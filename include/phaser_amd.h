@@ -42,22 +42,45 @@ typedef struct {
   int32_t device;                 /* HIP device ordinal */
   int32_t precision;              /* Legendre and Wigner-d contractions: 0 = FP32,
                                      1 = split FP16 (hi+lo) on MFMA, FP32 accumulation */
-  int32_t reserved;
+  int32_t bingham_peak_neighbors;  /* bingham_peak_neighbors (bingham-peak-based-eval.cc:
+                                      8-10), 0 .. 2^20 */
+  int32_t gaussian_peak_neighbors; /* gaussian_peak_neighbors (gaussian-peak-based-eval.cc:
+                                      9-11), 0 .. 2^20 */
 } phz_config;
 
-/* The pose parts of model::RegistrationResult (registration-result.cc:13-99):
- * the rotation argmax and its Bingham mode, the translation argmax and its
- * Gaussian mean (single-sample fits at the default 0 peak neighbours). */
+/* The pose and uncertainty parts of model::RegistrationResult
+ * (registration-result.cc:13-99, sph-opt-registration.cc:49-95): the
+ * rotation argmax and the Bingham fit to its +-bingham_peak_neighbors
+ * linear-index window (BinghamPeakBasedEval, bingham-peak-based-eval.cc:
+ * 22-90; Bingham::fit, bingham.cc:239-296), the translation argmax and the
+ * Gaussian fit to its +-gaussian_peak_neighbors window
+ * (GaussianPeakBasedEval, gaussian-peak-based-eval.cc:26-102; gaussian.cc:
+ * 14-56).  The window weights are the grid values (the z-score normalisation
+ * is disabled in the reference, z-score-eval.cc:44-68).  With 0 neighbours
+ * (the default) both fits have one sample: quat is the argmax's quaternion
+ * and translation the argmax's translation.  A window the reference rejects
+ * with a failed CHECK (a Bingham window reaching index (2B)^3, or weights that
+ * do not sum to a positive value) makes the call return -1. */
 typedef struct {
   uint32_t rot_index;     /* argmax of the (2B)^3 SO(3) grid [beta][alpha][gamma] */
   uint32_t trans_index;   /* argmax of the N^3 grid, x*N + y + N^2*z */
   float rot_value;        /* |C|^2 at rot_index */
   float trans_value;      /* correlation at trans_index */
-  double zyz[3];          /* rotation-utils.cc:197-207 */
-  double quat[4];         /* w,x,y,z, rotation-utils.cc:101-107 */
-  double euler_xyz[3];    /* rotation-utils.cc:86-99 */
+  double zyz[3];          /* of the argmax, rotation-utils.cc:197-207 */
+  double quat[4];         /* w,x,y,z: the Bingham mode (bingham.cc:110-113), signed
+                             to agree with the argmax's quaternion
+                             (rotation-utils.cc:101-107) */
+  double euler_xyz[3];    /* of quat, rotation-utils.cc:86-99 */
   float T_rot[16];        /* row-major, applied to the source (RotateAroundXYZ) */
-  double translation[3];  /* translation-utils.cc:40-52 */
+  double translation[3];  /* the Gaussian mean (gaussian.cc:48-50), of the argmax's
+                             translation-utils.cc:40-52 translations */
+  double trans_cov[9];    /* row-major Gaussian covariance (gaussian.cc:51-55); NaN
+                             where the reference takes the root of a negative weight */
+  double rot_omega[4];    /* eigenvalues of the Bingham moment matrix over their sum,
+                             ascending (bingham.cc:241-246): the statistic the
+                             concentrations are fitted to */
+  double rot_axes[16];    /* row-major 4x4 whose columns are the matching unit
+                             eigenvectors (the Bingham M, last column = quat) */
 } phz_result;
 
 /* Defaults of core-gflags.cc (bandwidth 75 there; 64 here so that the

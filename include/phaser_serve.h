@@ -52,6 +52,8 @@ typedef struct {
   int32_t precision;           /* phz_config.precision of every context */
   int32_t graphs;              /* nonzero: contexts replay their batches from HIP graphs
                                   (phz_set_graphs); default 1 */
+  int32_t bingham_peak_neighbors;   /* phz_config fields of every context */
+  int32_t gaussian_peak_neighbors;
 } phs_config;
 
 typedef struct {

@@ -322,6 +322,116 @@ def translation_from_index(idx, N, lo=-50, hi=50):
     return lib().po_translation_from_index(float(idx), N, lo, hi)
 
 
+class PeakWindowError(ValueError):
+    """A case where the reference fails a CHECK (the process would abort)."""
+
+
+def peak_window(idx, n_corr, k, kind):
+    """Linear-index neighbours [start, end] of the peak, as
+    BinghamPeakBasedEval::calculateStartEndNeighbor
+    (bingham-peak-based-eval.cc:47-58) and its Gaussian counterpart
+    (gaussian-peak-based-eval.cc:26-37): no lower neighbours at all when
+    k > idx; for the Bingham fit end = idx when idx + k > n_corr, and
+    end == n_corr fails the reference's CHECK_LT(end, n) (:69); for the
+    Gaussian end = idx when idx + k >= n_corr."""
+    start = idx if k > idx else idx - k
+    if kind == "bingham":
+        end = idx if k + idx > n_corr else idx + k
+        if end >= n_corr:
+            raise PeakWindowError("peak window reaches past the grid")
+    else:
+        end = idx if k + idx >= n_corr else idx + k
+    return start, end
+
+
+def bingham_moment(samples, weights):
+    """The moment matrix C = S diag(w) S^T, symmetrised (bingham.cc:281-296)."""
+    S = np.asarray(samples, np.float64)
+    w = np.asarray(weights, np.float64)
+    Cm = S @ np.diag(w) @ S.T
+    return 0.5 * (Cm + Cm.T)
+
+
+def bingham_mode(samples, weights, ref=None):
+    """Bingham::fit -> fitToMoment -> mode (bingham.cc:281-296, 239-279,
+    110-113): the eigenvectors of the moment matrix are ordered by their
+    maximum-likelihood concentrations, which order like the eigenvalues, so
+    the mode is the eigenvector of the largest eigenvalue.  Its sign is free
+    (the pose depends on q q^T only): here the one with m . ref >= 0, or
+    m[0] >= 0 without ref."""
+    _, vecs = np.linalg.eigh(bingham_moment(samples, weights))
+    m = vecs[:, -1]
+    flip = (m @ np.asarray(ref, np.float64) < 0) if ref is not None else m[0] < 0
+    return -m if flip else m
+
+
+def rotation_window(bw, corr, idx, k):
+    """BinghamPeakBasedEval::retrievePeakNeighbors (bingham-peak-based-eval.cc:
+    60-90) on the window of peak_window: the quaternions (4 x m) of the
+    window's indices (rotation-utils.cc:101-107,190-207) and their weights,
+    the correlation values, normalised to sum 1 when m > 1 (CHECK_GT of the
+    sum, :88); also the peak's own quaternion."""
+    start, end = peak_window(idx, len(corr), k, "bingham")
+    peak = quat_from_zyz(zyz_from_index(idx, bw))
+    samples = np.stack([quat_from_zyz(zyz_from_index(i, bw)) for i in range(start, end + 1)],
+                       axis=1)
+    w = np.asarray(corr[start:end + 1], np.float64)
+    if end > start:
+        if not w.sum() > 0:
+            raise PeakWindowError("weights do not sum to a positive value")
+        w = w / w.sum()
+    return samples, w, peak
+
+
+def refine_rotation(bw, corr, idx, k):
+    """BinghamPeakBasedEval::fitRotationalBinghamDistribution with k
+    neighbours (bingham-peak-based-eval.cc:22-45): returns the mode quaternion
+    (w, x, y, z), signed to agree with the peak's quaternion, and omega, the
+    moment matrix's eigenvalues over their sum in ascending order
+    (fitToMoment, bingham.cc:239-246).  One sample: the mode of w q q^T is q
+    itself and omega = (0, 0, 0, 1)."""
+    samples, w, peak = rotation_window(bw, corr, idx, k)
+    if samples.shape[1] == 1:
+        return peak, np.array([0.0, 0.0, 0.0, 1.0])
+    ev = np.linalg.eigvalsh(bingham_moment(samples, w))
+    return bingham_mode(samples, w, ref=peak), ev / ev.sum()
+
+
+def gaussian_fit(samples, weights):
+    """Gaussian(samples, weights) -> setMeanAndCov (gaussian.cc:14-22, 48-56):
+    mean = S w; covariance = sum_i w_i (s_i - mean)(s_i - mean)^T, formed
+    from sqrt(w_i)-scaled columns, so a negative weight makes it NaN."""
+    S = np.asarray(samples, np.float64)
+    w = np.asarray(weights, np.float64)
+    mu = S @ w
+    with np.errstate(invalid="ignore"):
+        Z = (S - mu[:, None]) * np.sqrt(w)[None, :]
+    return mu, Z @ Z.T
+
+
+def translation_window(N, corr, idx, k, lo=-50, hi=50):
+    """GaussianPeakBasedEval::retrievePeakNeighbors (gaussian-peak-based-eval.cc:
+    75-102) on the window of peak_window: the translations (3 x m) of the
+    window's indices (signal-utils.cc:11-19, translation-utils.cc:40-52) and
+    the correlation values normalised to sum 1 (CHECK_GT of the sum, :100)."""
+    start, end = peak_window(idx, len(corr), k, "gaussian")
+    t = []
+    for i in range(start, end + 1):
+        x, y, z = ind2sub(i, N, N)
+        t.append([translation_from_index(x, N, lo, hi), translation_from_index(y, N, lo, hi),
+                  translation_from_index(z, N, lo, hi)])
+    w = np.asarray(corr[start:end + 1], np.float64)
+    if not w.sum() > 0:
+        raise PeakWindowError("weights do not sum to a positive value")
+    return np.asarray(t, np.float64).T, w / w.sum()
+
+
+def refine_translation(N, corr, idx, k, lo=-50, hi=50):
+    """GaussianPeakBasedEval::fitTranslationalNormalDist with k neighbours
+    (gaussian-peak-based-eval.cc:39-73): (mean, covariance) of the window."""
+    return gaussian_fit(*translation_window(N, corr, idx, k, lo, hi))
+
+
 def ind2sub(idx, rows, cols):
     out = np.zeros(3, np.uint32)
     lib().po_ind2sub(idx, rows, cols, out)

@@ -19,7 +19,8 @@ class Config(C.Structure):
                 ("discretize_upper", C.c_int32), ("spatial_zero_padding", C.c_int32),
                 ("low_pass_lower", C.c_int32), ("low_pass_upper", C.c_int32),
                 ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32),
-                ("precision", C.c_int32), ("reserved", C.c_int32)]
+                ("precision", C.c_int32), ("bingham_peak_neighbors", C.c_int32),
+                ("gaussian_peak_neighbors", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -27,7 +28,8 @@ class Result(C.Structure):
                 ("rot_value", C.c_float), ("trans_value", C.c_float),
                 ("zyz", C.c_double * 3), ("quat", C.c_double * 4),
                 ("euler_xyz", C.c_double * 3), ("T_rot", C.c_float * 16),
-                ("translation", C.c_double * 3)]
+                ("translation", C.c_double * 3), ("trans_cov", C.c_double * 9),
+                ("rot_omega", C.c_double * 4), ("rot_axes", C.c_double * 16)]
 
     def as_dict(self):
         return {
@@ -37,6 +39,9 @@ class Result(C.Structure):
             "euler_xyz": np.array(self.euler_xyz[:]),
             "T_rot": np.array(self.T_rot[:], np.float32).reshape(4, 4),
             "translation": np.array(self.translation[:]),
+            "trans_cov": np.array(self.trans_cov[:]).reshape(3, 3),
+            "rot_omega": np.array(self.rot_omega[:]),
+            "rot_axes": np.array(self.rot_axes[:]).reshape(4, 4),
         }
 
 

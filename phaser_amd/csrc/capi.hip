@@ -87,6 +87,8 @@ struct phz_ctx {
   float2* d_spec = nullptr;
   float2* d_work = nullptr;
   float* d_corr_trans = nullptr;
+  float* d_peak_rot = nullptr;    // per-pair grids for the peak-window fits when the
+  float* d_peak_trans = nullptr;  // caller asks for no grids (neighbours > 0 only)
   float4* d_reg = nullptr;
   int* h_npts = nullptr;  // pinned staging for per-cloud point counts
   std::vector<void*> allocs;
@@ -240,6 +242,8 @@ size_t context_bytes(const phz_config& cf) {
   add(P * H, sizeof(float2));
   add(T3, sizeof(float));
   add(cap, sizeof(float4));
+  if (cf.bingham_peak_neighbors > 0) add(P * n * nq, sizeof(float));
+  if (cf.gaussian_peak_neighbors > 0) add(P * T3, sizeof(float));
   return s;
 }
 
@@ -252,6 +256,10 @@ int check_config(const phz_config& cf) {
   PHZ_CHECK(cf.max_points >= 1 && cf.max_points <= (1ll << 26), "max_points out of range");
   PHZ_CHECK(cf.max_batch >= 1 && cf.max_batch <= 1024, "max_batch out of range");
   PHZ_CHECK(cf.precision == 0 || cf.precision == 1, "precision must be 0 (fp32) or 1 (split fp16)");
+  PHZ_CHECK(cf.bingham_peak_neighbors >= 0 && cf.bingham_peak_neighbors <= (1 << 20),
+            "bingham_peak_neighbors must be in [0, 2^20]");
+  PHZ_CHECK(cf.gaussian_peak_neighbors >= 0 && cf.gaussian_peak_neighbors <= (1 << 20),
+            "gaussian_peak_neighbors must be in [0, 2^20]");
   PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
   PHZ_CHECK(make_plan(cf.n_voxels).nst > 0, "n_voxels must factor into 2, 3 and 5");
   return 0;
@@ -345,6 +353,10 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_work, (size_t)P * c->H));
   TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3));
   TRY(dalloc(c, &c->d_reg, (size_t)cap));
+  if (cf.bingham_peak_neighbors > 0)
+    TRY(dalloc(c, &c->d_peak_rot, (size_t)P * nq * c->n));
+  if (cf.gaussian_peak_neighbors > 0)
+    TRY(dalloc(c, &c->d_peak_trans, (size_t)P * c->T3));
   PHZ_HIP(hipHostMalloc((void**)&c->h_npts, sizeof(int) * M, hipHostMallocDefault));
   // voxel accumulators stay zero between calls: every pipeline run clears
   // exactly the voxels it touched
@@ -390,6 +402,11 @@ ChannelSet channels(const float2* a, const float2* b, int C) {
 // [nt, nt+np) sources already resident in ctx->d_pts / d_npts.
 int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
               float* d_trans_corr) {
+  // the peak-window fits read the grids: written to the context's own
+  // buffers when the caller wants none
+  const int kb = c->cfg.bingham_peak_neighbors, kg = c->cfg.gaussian_peak_neighbors;
+  if (!d_rot_corr && kb > 0) d_rot_corr = c->d_peak_rot;
+  if (!d_trans_corr && kg > 0) d_trans_corr = c->d_peak_trans;
   const int nt = shared ? 1 : np;
   const int ncl = nt + np;
   const int B = c->B, n = c->n, N = c->N;
@@ -430,7 +447,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kRotPose, s);
-    launch_rot_pose(B, np, c->d_keys, c->d_pose, s);
+    launch_rot_pose(B, np, c->d_keys, d_rot_corr, nq * n, kb, c->d_pose, s);
   }
   // translation (accumulators are zero on entry and cleared again below)
   {
@@ -523,7 +540,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     StageScope t(c, kTransPose, s);
     launch_trans_pose(N, c->cfg.discretize_lower, c->cfg.discretize_upper, np, c->d_keys + np,
-                      c->d_pose, s);
+                      d_trans_corr, c->T3, kg, c->d_pose, s);
   }
   PHZ_HIP(hipGetLastError());
   return 0;
@@ -582,8 +599,15 @@ void to_result(const DevPose& d, phz_result* r) {
     r->euler_xyz[i] = d.euler[i];
     r->translation[i] = d.translation[i];
   }
-  for (int i = 0; i < 4; ++i) r->quat[i] = d.quat[i];
-  for (int i = 0; i < 16; ++i) r->T_rot[i] = d.T[i];
+  for (int i = 0; i < 4; ++i) {
+    r->quat[i] = d.quat[i];
+    r->rot_omega[i] = d.rot_omega[i];
+  }
+  for (int i = 0; i < 16; ++i) {
+    r->T_rot[i] = d.T[i];
+    r->rot_axes[i] = d.rot_axes[i];
+  }
+  for (int i = 0; i < 9; ++i) r->trans_cov[i] = d.trans_cov[i];
 }
 
 int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
@@ -592,6 +616,19 @@ int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
   PHZ_HIP(hipStreamSynchronize(s));
   collect_timings(c);
   for (int p = 0; p < np; ++p) to_result(hp[p], &out[p]);
+  // the reference's CHECKs on the peak windows (its process aborts there)
+  for (int p = 0; p < np; ++p) {
+    const int32_t st = hp[p].status;
+    PHZ_CHECK(!(st & kPoseRotWindow),
+              "pair " + std::to_string(p) + ": the Bingham peak window reaches past the "
+              "rotation grid (bingham-peak-based-eval.cc:69 CHECK_LT(end, n))");
+    PHZ_CHECK(!(st & kPoseRotWeights),
+              "pair " + std::to_string(p) + ": the Bingham peak window's weights do not sum "
+              "to a positive value (bingham-peak-based-eval.cc:88 CHECK_GT)");
+    PHZ_CHECK(!(st & kPoseTransWeights),
+              "pair " + std::to_string(p) + ": the Gaussian peak window's weights do not sum "
+              "to a positive value (gaussian-peak-based-eval.cc:100 CHECK_GT)");
+  }
   return 0;
 }
 

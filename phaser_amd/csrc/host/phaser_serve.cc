@@ -100,6 +100,8 @@ phz_config shape_config(const phs_config& sc, int B, int N) {
   c.max_batch = sc.max_batch;
   c.device = sc.device;
   c.precision = sc.precision;
+  c.bingham_peak_neighbors = sc.bingham_peak_neighbors;
+  c.gaussian_peak_neighbors = sc.gaussian_peak_neighbors;
   return c;
 }
 

@@ -89,8 +89,10 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
                           unsigned long long* partials, unsigned long long* keys,
                           hipStream_t s, bool reg = false);
 
-// Device-side pose from the rotation argmax: fills pose[pair] and the float
-// transform T[pair][16] (rotation-utils.cc:17-23,86-143,190-207).
+// Device-side pose of each pair: the argmaxes, the Bingham and Gaussian
+// fits to their peak windows and the float transform T (row-major 4x4)
+// applied to the source.  status: kPose* bits of the checks the reference
+// enforces with CHECK (a failed one aborts the reference's process).
 struct DevPose {
   uint32_t rot_index, trans_index;
   float rot_value, trans_value;
@@ -98,10 +100,21 @@ struct DevPose {
   double quat[4];
   double euler[3];
   double translation[3];
+  double trans_cov[9];
+  double rot_omega[4];
+  double rot_axes[16];
   float T[16];
+  int32_t status;
+  int32_t pad;
 };
-void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose* pose,
-                     hipStream_t s);
+constexpr int32_t kPoseRotWindow = 1;   // Bingham window end >= (2B)^3
+constexpr int32_t kPoseRotWeights = 2;  // Bingham window weights sum <= 0
+constexpr int32_t kPoseTransWeights = 4;  // Gaussian window weights sum <= 0
+
+// One wavefront per pair.  corr: the pairs' (2B)^3 grids at corr_stride
+// floats apart (read only when k > 0); k: bingham_peak_neighbors.
+void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
+                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s);
 
 // ------------------------------------------------------------- spatial
 // voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3
@@ -236,8 +249,11 @@ void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& pl
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
                         const DevPose* pose, int pose_stride, int N, const float* edges,
                         float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s);
+// One wavefront per pair.  corr: the pairs' N^3 grids at corr_stride floats
+// apart (read only when k > 0); k: gaussian_peak_neighbors.
 void launch_trans_pose(int N, int lower, int upper, int npairs,
-                       const unsigned long long* keys, DevPose* pose, hipStream_t s);
+                       const unsigned long long* keys, const float* corr, int64_t corr_stride,
+                       int k, DevPose* pose, hipStream_t s);
 // out = T_rot * p + translation (PCL scalar order), intensity kept
 void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,
                        hipStream_t s);

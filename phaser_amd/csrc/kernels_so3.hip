@@ -11,6 +11,7 @@
 // transform over m1 is a complex FFT and the one over m2 a real-output FFT.
 #include "fft_reg.hpp"
 #include "kernels.hpp"
+#include "pose.hpp"
 
 namespace phz {
 
@@ -351,75 +352,64 @@ __global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2*
   }
 }
 
-__device__ void mat4_mul_nc(const float* A, const float* B, float* R) {
-#pragma clang fp contract(off)
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j) {
-      float s = A[4 * i] * B[j];
-      s = s + A[4 * i + 1] * B[4 + j];
-      s = s + A[4 * i + 2] * B[8 + j];
-      s = s + A[4 * i + 3] * B[12 + j];
-      R[4 * i + j] = s;
-    }
-}
-
-// rotation-utils.cc:190-207 (index -> ZYZ), :101-107 (quaternion), :86-99,
-// :166-173 (XYZ Euler), :17-23,109-143 (T = Tx Ty Tz', float)
-__global__ void k_rot_pose(int bw, int npairs, const unsigned long long* __restrict__ keys,
-                           DevPose* __restrict__ pose) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npairs) return;
+// The rotation pose of pair blockIdx.x (one wavefront): the argmax's ZYZ
+// angles (rotation-utils.cc:190-207) and the Bingham fit to the
+// +-k linear-index window around it (BinghamPeakBasedEval,
+// bingham-peak-based-eval.cc:22-90; Bingham::fit / fitToMoment,
+// bingham.cc:239-296): weights = the grid values, normalised to sum 1 when
+// the window holds more than one index; moment C = sum w q q^T; mode = the
+// eigenvector of C's largest eigenvalue (the maximum-likelihood
+// concentrations order like the eigenvalues).  Then the XYZ Euler angles
+// and float transform of the mode (rotation-utils.cc:17-23,86-143).
+__global__ void __launch_bounds__(64) k_rot_pose(int bw, const unsigned long long* __restrict__ keys,
+                                                 const float* __restrict__ corr,
+                                                 int64_t corr_stride, int k,
+                                                 DevPose* __restrict__ pose) {
+  const int p = blockIdx.x, lane = threadIdx.x;
   const unsigned long long key = keys[p];
   const uint32_t idx = key_index(key);
-  const uint32_t n = 2 * bw;
-  const uint32_t ii = idx / (n * n), jj = (idx / n) % n, kk = idx % n;
+  const uint32_t n = 2 * bw, n_corr = n * n * n;
+  double zyz[3], qp[4];
+  zyz_of_index(idx, bw, zyz);
+  quat_of_zyz(zyz, qp);
+  // bingham-peak-based-eval.cc:47-58; an end of n_corr fails CHECK_LT (:69)
+  const uint32_t uk = (uint32_t)k;
+  const uint32_t start = uk > idx ? idx : idx - uk;
+  const uint32_t end = uk + idx > n_corr ? idx : idx + uk;
+  int32_t status = 0;
+  bool multi = false;
+  double C[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // upper triangle, row by row
+  if (end >= n_corr) {
+    status |= kPoseRotWindow;
+  } else if (end > start && idx < n_corr) {
+    const float* g = corr + p * corr_stride;
+    double sum = 0.0;
+    for (uint32_t i = start + lane; i <= end; i += 64) sum += (double)g[i];
+    sum = wave_sum(sum);
+    if (!(sum > 0.0)) {
+      status |= kPoseRotWeights;
+    } else {
+      multi = true;
+      for (uint32_t i = start + lane; i <= end; i += 64) {
+        const double w = (double)g[i] / sum;
+        double z[3], q[4];
+        zyz_of_index(i, bw, z);
+        quat_of_zyz(z, q);
+        int e = 0;
+        for (int r = 0; r < 4; ++r)
+          for (int c = r; c < 4; ++c) C[e++] += w * q[r] * q[c];
+      }
+      for (int e = 0; e < 10; ++e) C[e] = wave_sum(C[e]);
+    }
+  }
+  if (lane != 0) return;
   DevPose& P = pose[p];
   P.rot_index = idx;
   P.rot_value = unordered_value((uint32_t)(key >> 32));
-  const double bd = (double)bw;
-  P.zyz[0] = M_PI * jj / bd;
-  P.zyz[1] = M_PI * (2 * ii + 1) / (4. * bw);
-  P.zyz[2] = M_PI * kk / bd;
-  const double qa[4] = {cos(0.5 * P.zyz[0]), 0, 0, sin(0.5 * P.zyz[0])};
-  const double qb[4] = {cos(0.5 * P.zyz[1]), 0, sin(0.5 * P.zyz[1]), 0};
-  const double qc[4] = {cos(0.5 * P.zyz[2]), 0, 0, sin(0.5 * P.zyz[2])};
-  double t[4], q[4];
-  auto qmul = [](const double* x, const double* y, double* r) {
-    r[0] = x[0] * y[0] - x[1] * y[1] - x[2] * y[2] - x[3] * y[3];
-    r[1] = x[0] * y[1] + x[1] * y[0] + x[2] * y[3] - x[3] * y[2];
-    r[2] = x[0] * y[2] + x[2] * y[0] + x[3] * y[1] - x[1] * y[3];
-    r[3] = x[0] * y[3] + x[3] * y[0] + x[1] * y[2] - x[2] * y[1];
-  };
-  qmul(qa, qb, t);
-  qmul(t, qc, q);
-  for (int d = 0; d < 4; ++d) P.quat[d] = q[d];
-  const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
-  const double r11 = -2 * (qy * qz - qw * qx);
-  const double r12 = qw * qw - qx * qx - qy * qy + qz * qz;
-  const double r21 = 2 * (qx * qz + qw * qy);
-  const double r31 = -2 * (qx * qy - qw * qz);
-  const double r32 = qw * qw + qx * qx - qy * qy - qz * qz;
-  P.euler[2] = -atan2(r31, r32);
-  P.euler[1] = asin(r21);
-  P.euler[0] = atan2(r11, r12);
-  const float ea = (float)P.euler[0], eb = (float)P.euler[1], ec = (float)P.euler[2];
-  float X[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-  float Y[16], Z[16], XY[16];
-  for (int e = 0; e < 16; ++e) Y[e] = Z[e] = X[e];
-  X[5] = cosf(ea);
-  X[6] = -sinf(ea);
-  X[9] = sinf(ea);
-  X[10] = cosf(ea);
-  Y[0] = cosf(eb);
-  Y[2] = sinf(eb);
-  Y[8] = -sinf(eb);
-  Y[10] = cosf(eb);
-  Z[0] = cosf(ec);
-  Z[1] = sinf(ec);
-  Z[4] = -sinf(ec);
-  Z[5] = cosf(ec);
-  mat4_mul_nc(X, Y, XY);
-  mat4_mul_nc(XY, Z, P.T);
+  for (int d = 0; d < 3; ++d) P.zyz[d] = zyz[d];
+  bingham_from_moment(C, qp, multi, P.quat, P.rot_omega, P.rot_axes);
+  euler_and_transform(P.quat, P.euler, P.T);
+  P.status = status;
 }
 
 }  // namespace
@@ -526,9 +516,9 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
   launch_reduce_keys(partials, so3_gamma_blocks(bw), npairs, keys, s);
 }
 
-void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, DevPose* pose,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_rot_pose, dim3((npairs + 63) / 64), dim3(64), 0, s, bw, npairs, keys,
+void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
+                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s) {
+  hipLaunchKernelGGL(k_rot_pose, dim3(npairs), dim3(64), 0, s, bw, keys, corr, corr_stride, k,
                      pose);
 }
 

@@ -10,6 +10,7 @@
 // the reference's: linear index x*N + y + N^2*z, i.e. [z][x][y].
 #include "fft_reg.hpp"
 #include "kernels.hpp"
+#include "pose.hpp"
 
 namespace phz {
 
@@ -406,35 +407,73 @@ __global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
   if (threadIdx.x == 0) partials[blockIdx.x] = best;
 }
 
-// translation-utils.cc:40-52 + signal-utils.cc:11-19
-__global__ void k_trans_pose(int N, int lower, int upper, int npairs,
-                             const unsigned long long* __restrict__ keys,
-                             DevPose* __restrict__ pose) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npairs) return;
+// The translation pose of pair blockIdx.x (one wavefront): the Gaussian fit
+// to the +-k linear-index window around the argmax (GaussianPeakBasedEval,
+// gaussian-peak-based-eval.cc:26-102; gaussian.cc:14-56): weights = the grid
+// values normalised to sum 1 (CHECK_GT(sum, 0), :100), samples = the
+// translations of the window's indices (translation-utils.cc:40-52,
+// signal-utils.cc:11-19), mean = sum w t, covariance = sum w (t-mean)(t-mean)^T,
+// NaN where the reference takes the root of a negative weight.
+__global__ void __launch_bounds__(64) k_trans_pose(int N, int lower, int upper,
+                                                   const unsigned long long* __restrict__ keys,
+                                                   const float* __restrict__ corr,
+                                                   int64_t corr_stride, int k,
+                                                   DevPose* __restrict__ pose) {
+  const int p = blockIdx.x, lane = threadIdx.x;
   const unsigned long long key = keys[p];
   const uint32_t idx = key_index(key);
+  const float value = unordered_value((uint32_t)(key >> 32));
+  const uint32_t n_corr = (uint32_t)N * N * N;
+  const double width = (double)(abs(lower) + abs(upper) + 1);
+  // gaussian-peak-based-eval.cc:26-37
+  const uint32_t uk = (uint32_t)k;
+  const uint32_t start = uk > idx ? idx : idx - uk;
+  const uint32_t end = uk + idx >= n_corr ? idx : idx + uk;
+  int32_t status = 0;
+  double mean[3], cov[6] = {0, 0, 0, 0, 0, 0};  // upper triangle, row by row
+  bool negative = false;
+  if (end == start || idx >= n_corr) {  // one sample (the default 0 neighbours): value / value
+    if (!(value > 0.f)) status |= kPoseTransWeights;
+    translation_of_index(idx, N, width, mean);
+  } else {
+    const float* g = corr + p * corr_stride;
+    double sum = 0.0;
+    for (uint32_t i = start + lane; i <= end; i += 64) sum += (double)g[i];
+    sum = wave_sum(sum);
+    if (!(sum > 0.0)) status |= kPoseTransWeights;
+    double m[3] = {0, 0, 0};
+    for (uint32_t i = start + lane; i <= end; i += 64) {
+      const double w = (double)g[i] / sum;
+      negative |= w < 0.0;
+      double t[3];
+      translation_of_index(i, N, width, t);
+      for (int d = 0; d < 3; ++d) m[d] += w * t[d];
+    }
+    for (int d = 0; d < 3; ++d) mean[d] = wave_sum(m[d]);
+    for (uint32_t i = start + lane; i <= end; i += 64) {
+      const double w = (double)g[i] / sum;
+      double t[3];
+      translation_of_index(i, N, width, t);
+      for (int d = 0; d < 3; ++d) t[d] -= mean[d];
+      int e = 0;
+      for (int r = 0; r < 3; ++r)
+        for (int c = r; c < 3; ++c) cov[e++] += w * t[r] * t[c];
+    }
+    for (int e = 0; e < 6; ++e) cov[e] = wave_sum(cov[e]);
+    negative = __any(negative);
+  }
+  if (lane != 0) return;
   DevPose& P = pose[p];
   P.trans_index = idx;
-  P.trans_value = unordered_value((uint32_t)(key >> 32));
-  uint32_t xyz[3];
-  xyz[1] = idx % (uint32_t)N;
-  const uint32_t u = idx / (uint32_t)N;
-  xyz[0] = u % (uint32_t)N;
-  xyz[2] = u / (uint32_t)N;
-  const double width = (double)(abs(lower) + abs(upper) + 1);
-  const double half = N / 2.0;
-  for (int d = 0; d < 3; ++d) {
-    const double index = (double)xyz[d];
-    double t;
-    if (index == 0)
-      t = 0.0;
-    else if (index <= half)
-      t = index * width / N;
-    else
-      t = (index - N) * width / N;
-    P.translation[d] = t;
-  }
+  P.trans_value = value;
+  for (int d = 0; d < 3; ++d) P.translation[d] = mean[d];
+  int e = 0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = r; c < 3; ++c) {
+      const double v = cov[e++];
+      P.trans_cov[r * 3 + c] = P.trans_cov[c * 3 + r] = negative ? __builtin_nan("") : v;
+    }
+  P.status |= status;
 }
 
 __global__ void k_apply_pose(const float4* __restrict__ pts, int64_t P,
@@ -1335,9 +1374,10 @@ void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, C
 }
 
 void launch_trans_pose(int N, int lower, int upper, int npairs, const unsigned long long* keys,
-                       DevPose* pose, hipStream_t s) {
-  hipLaunchKernelGGL(k_trans_pose, dim3((npairs + 63) / 64), dim3(64), 0, s, N, lower, upper,
-                     npairs, keys, pose);
+                       const float* corr, int64_t corr_stride, int k, DevPose* pose,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_trans_pose, dim3(npairs), dim3(64), 0, s, N, lower, upper, keys, corr,
+                     corr_stride, k, pose);
 }
 
 void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,
