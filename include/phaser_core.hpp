@@ -43,11 +43,57 @@ extern int32_t FLAGS_phaser_core_spatial_low_pass_lower_bound;
 extern int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound;
 }  // namespace phaser_core
 
+// Defined at file scope in the reference: bingham-peak-based-eval.cc:8-10 and
+// gaussian-peak-based-eval.cc:9-11 (default 0: a single-sample fit at the
+// argmax).
+extern int32_t FLAGS_bingham_peak_neighbors;
+extern int32_t FLAGS_gaussian_peak_neighbors;
+
 namespace common {
 // pcl::PointXYZI fields used by the path (point-types.h:10-14)
 struct Point_t {
   float x = 0, y = 0, z = 0, intensity = 0;
 };
+}  // namespace common
+
+namespace common {
+// The fitted distributions of the registration's uncertainty estimates
+// (BaseDistribution::getEstimate, base-distribution.h).
+// Bingham (bingham.h): the mode, the moment matrix's eigenvalues over their
+// sum in ascending order (the statistic fitToMoment fits the concentrations
+// to, bingham.cc:239-246) and M, whose columns are the matching eigenvectors.
+class Bingham {
+ public:
+  Bingham() = default;
+  Bingham(const std::array<double, 4>& mode, const std::array<double, 4>& omega,
+          const std::array<double, 16>& M)
+      : mode_(mode), omega_(omega), M_(M) {}
+  std::vector<double> getEstimate() const { return {mode_.begin(), mode_.end()}; }
+  std::array<double, 4> mode() const { return mode_; }
+  std::array<double, 4> getOmega() const { return omega_; }
+  std::array<double, 16> getM() const { return M_; }  // row-major
+
+ private:
+  std::array<double, 4> mode_{1, 0, 0, 0}, omega_{0, 0, 0, 1};
+  std::array<double, 16> M_{};
+};
+using BinghamPtr = std::shared_ptr<Bingham>;
+
+// Gaussian (gaussian.h): weighted mean and covariance of the peak window.
+class Gaussian {
+ public:
+  Gaussian() = default;
+  Gaussian(const std::array<double, 3>& mu, const std::array<double, 9>& cov)
+      : mu_(mu), cov_(cov) {}
+  std::vector<double> getEstimate() const { return {mu_.begin(), mu_.end()}; }
+  std::array<double, 3> getMean() const { return mu_; }
+  std::array<double, 9> getCov() const { return cov_; }  // row-major
+
+ private:
+  std::array<double, 3> mu_{0, 0, 0};
+  std::array<double, 9> cov_{};
+};
+using GaussianPtr = std::shared_ptr<Gaussian>;
 }  // namespace common
 
 namespace model {
@@ -101,10 +147,18 @@ class RegistrationResult {
   void setTranslation(const std::array<double, 3>& t) { trans_ = t; }
   void setRotationCorrelation(std::vector<double> c) { rot_corr_ = std::move(c); }
   const std::vector<double>& getRotationCorrelation() const { return rot_corr_; }
+  // registration-result.cc: the Bingham fit of the rotation peak and the
+  // Gaussian fit of the translation peak (sph-opt-registration.cc:68,93)
+  void setRotUncertaintyEstimate(common::BinghamPtr b) { rot_dist_ = std::move(b); }
+  void setPosUncertaintyEstimate(common::GaussianPtr g) { pos_dist_ = std::move(g); }
+  common::BinghamPtr getRotUncertaintyEstimate() const { return rot_dist_; }
+  common::GaussianPtr getPosUncertaintyEstimate() const { return pos_dist_; }
   uint32_t rotation_index = 0, translation_index = 0;
 
  private:
   PointCloudPtr reg_cloud_;
+  common::BinghamPtr rot_dist_;
+  common::GaussianPtr pos_dist_;
   std::array<double, 4> quat_{1, 0, 0, 0};
   std::array<double, 3> trans_{0, 0, 0};
   std::vector<double> rot_corr_;

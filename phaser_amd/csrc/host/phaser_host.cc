@@ -24,6 +24,9 @@ int32_t FLAGS_phaser_core_spatial_low_pass_lower_bound = 0;
 int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound = 100000;
 }  // namespace phaser_core
 
+int32_t FLAGS_bingham_peak_neighbors = 0;
+int32_t FLAGS_gaussian_peak_neighbors = 0;
+
 namespace {
 
 // glog CHECK equivalent: the reference aborts on every failed precondition
@@ -50,13 +53,16 @@ namespace phaser_core {
 
 std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int64_t, int, int, int, int>, std::weak_ptr<phz_ctx>> cache;
+  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int>,
+                  std::weak_ptr<phz_ctx>>
+      cache;
   std::lock_guard<std::mutex> lock(mu);
   const int64_t cap = max_points > kDefaultCapacity ? max_points : kDefaultCapacity;
   const auto key = std::make_tuple(bw, n_voxels, cap, FLAGS_phaser_core_spatial_discretize_lower,
                                    FLAGS_phaser_core_spatial_discretize_upper,
                                    FLAGS_phaser_core_spatial_low_pass_lower_bound,
-                                   FLAGS_phaser_core_spatial_low_pass_upper_bound);
+                                   FLAGS_phaser_core_spatial_low_pass_upper_bound,
+                                   FLAGS_bingham_peak_neighbors, FLAGS_gaussian_peak_neighbors);
   if (auto p = cache[key].lock()) return p;
   phz_config cfg;
   phz_default_config(&cfg);
@@ -67,6 +73,8 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
   cfg.discretize_upper = FLAGS_phaser_core_spatial_discretize_upper;
   cfg.low_pass_lower = FLAGS_phaser_core_spatial_low_pass_lower_bound;
   cfg.low_pass_upper = FLAGS_phaser_core_spatial_low_pass_upper_bound;
+  cfg.bingham_peak_neighbors = FLAGS_bingham_peak_neighbors;
+  cfg.gaussian_peak_neighbors = FLAGS_gaussian_peak_neighbors;
   phz_ctx* raw = nullptr;
   check(phz_create(&cfg, &raw), "phz_create");
   std::shared_ptr<phz_ctx> p(raw, [](phz_ctx* c) { phz_destroy(c); });
@@ -246,6 +254,16 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   result.setRotationCorrelation(std::vector<double>(rot.begin(), rot.end()));
   result.rotation_index = r.rot_index;
   result.translation_index = r.trans_index;
+  std::array<double, 4> mode, omega;
+  std::array<double, 16> M;
+  std::array<double, 3> mu;
+  std::array<double, 9> cov;
+  for (int i = 0; i < 4; ++i) mode[i] = r.quat[i], omega[i] = r.rot_omega[i];
+  for (int i = 0; i < 16; ++i) M[i] = r.rot_axes[i];
+  for (int i = 0; i < 3; ++i) mu[i] = r.translation[i];
+  for (int i = 0; i < 9; ++i) cov[i] = r.trans_cov[i];
+  result.setRotUncertaintyEstimate(std::make_shared<common::Bingham>(mode, omega, M));
+  result.setPosUncertaintyEstimate(std::make_shared<common::Gaussian>(mu, cov));
   return result;
 }
 

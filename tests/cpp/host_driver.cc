@@ -3,6 +3,7 @@
 // phaser_core/test/driver.cc:30-57) and prints the result as JSON.
 // usage: host_driver target.bin source.bin bandwidth n_voxels
 // where a .bin file is float32 (x, y, z, intensity) quadruples.
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <vector>
@@ -23,12 +24,19 @@ static model::PointCloudPtr load(const char* path) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 5) {
-    std::fprintf(stderr, "usage: %s target.bin source.bin bandwidth n_voxels\n", argv[0]);
+  if (argc != 5 && argc != 7) {
+    std::fprintf(stderr,
+                 "usage: %s target.bin source.bin bandwidth n_voxels "
+                 "[bingham_peak_neighbors gaussian_peak_neighbors]\n",
+                 argv[0]);
     return 2;
   }
   phaser_core::FLAGS_phaser_core_spherical_bandwidth = std::atoi(argv[3]);
   phaser_core::FLAGS_phaser_core_spatial_n_voxels = std::atoi(argv[4]);
+  if (argc == 7) {
+    FLAGS_bingham_peak_neighbors = std::atoi(argv[5]);
+    FLAGS_gaussian_peak_neighbors = std::atoi(argv[6]);
+  }
   auto target = load(argv[1]);
   auto source = load(argv[2]);
   phaser_core::CloudController controller("sph-opt");
@@ -37,9 +45,21 @@ int main(int argc, char** argv) {
   const auto& corr = r.getRotationCorrelation();
   double cmax = 0;
   for (double v : corr) cmax = v > cmax ? v : cmax;
+  const auto omega = r.getRotUncertaintyEstimate()->getOmega();
+  const auto cov = r.getPosUncertaintyEstimate()->getCov();
+  // the covariance is NaN where a window weight is negative (as in the
+  // reference's Gaussian); JSON as Python reads it
+  auto num = [](double v) {
+    char b[32];
+    if (std::isnan(v)) return std::string("NaN");
+    std::snprintf(b, sizeof(b), "%.17g", v);
+    return std::string(b);
+  };
   std::printf("{\"rot_index\": %u, \"trans_index\": %u, \"state\": [%.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g], "
-              "\"registered_points\": %zu, \"rot_corr_size\": %zu, \"rot_corr_max\": %.9g}\n",
+              "\"registered_points\": %zu, \"rot_corr_size\": %zu, \"rot_corr_max\": %.9g, "
+              "\"rot_omega\": [%.17g, %.17g, %.17g, %.17g], \"trans_cov_diag\": [%s, %s, %s]}\n",
               r.rotation_index, r.translation_index, s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7],
-              r.getRegisteredCloud()->size(), corr.size(), cmax);
+              r.getRegisteredCloud()->size(), corr.size(), cmax, omega[0], omega[1], omega[2],
+              omega[3], num(cov[0]).c_str(), num(cov[4]).c_str(), num(cov[8]).c_str());
   return 0;
 }
