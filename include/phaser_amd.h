@@ -46,6 +46,9 @@ typedef struct {
                                       8-10), 0 .. 2^20 */
   int32_t gaussian_peak_neighbors; /* gaussian_peak_neighbors (gaussian-peak-based-eval.cc:
                                       9-11), 0 .. 2^20 */
+  int32_t channels;  /* per-point signal channels: 2 (range and intensity; reflectivity and
+                        ambient are zero, as for a PointXYZI cloud; default) or 4 (clouds
+                        may also carry reflectivity and ambient, phz_register_channels) */
 } phz_config;
 
 /* The pose and uncertainty parts of model::RegistrationResult
@@ -122,6 +125,17 @@ int phz_register_pairs_device(phz_ctx* ctx, int n, int shared_target,
 int phz_register_pairs(phz_ctx* ctx, int n, int shared_target, const float* const* targets,
                        const int64_t* Pt, const float* const* sources, const int64_t* Ps,
                        phz_result* results);
+/* phz_register for clouds that carry reflectivity and ambient as well
+ * (model::PointCloud's reflectivities_ and ambient_points_, filled from a
+ * PLY by point-cloud.cc:346-348): target_ra / source_ra hold (reflectivity,
+ * ambient) per point (Pt x 2 / Ps x 2 floats; NULL: none, a zero channel).
+ * They enter the S^2 samples (point-cloud.cc:152-155) and the voxel means
+ * (phase-aligner.cc:113-118), so all four channels of both Laplace fusions
+ * are live.  Needs a context created with channels = 4. */
+int phz_register_channels(phz_ctx* ctx, const float* target, const float* target_ra, int64_t Pt,
+                          const float* source, const float* source_ra, int64_t Ps,
+                          phz_result* result, float* rot_corr, float* trans_corr,
+                          float* registered);
 /* Device bytes owned by the context (tables and workspaces). */
 int phz_memory_usage(const phz_ctx* ctx, size_t* bytes);
 /* The device bytes phz_create would allocate for cfg (validates cfg; no

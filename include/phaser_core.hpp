@@ -48,6 +48,11 @@ extern int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound;
 // argmax).
 extern int32_t FLAGS_bingham_peak_neighbors;
 extern int32_t FLAGS_gaussian_peak_neighbors;
+// PLY field names (ply-helper.cc:8-22)
+extern std::string FLAGS_phaser_ply_intensity_str;     // "intensity"
+extern std::string FLAGS_phaser_ply_reflectivity_str;  // "reflectivity"
+extern std::string FLAGS_phaser_ply_ambient_str;       // "noise"
+extern std::string FLAGS_phaser_ply_range_str;         // "range"
 
 namespace common {
 // pcl::PointXYZI fields used by the path (point-types.h:10-14)
@@ -111,9 +116,22 @@ class PointCloud {
   // row-major 4x4, PCL scalar order (point-cloud.cc:183-185)
   void transformPointCloud(const std::array<float, 16>& T);
   PointCloud clone() const { return *this; }
+  // PointCloud::readFromFile (point-cloud.cc:313-349): x, y, z and intensity
+  // into the points, reflectivity and ambient (when the PLY has them) per
+  // point; field names from the FLAGS_phaser_ply_* flags
+  void readFromFile(const std::string& ply);
+  bool hasReflectivityPoints() const { return !reflectivities_.empty(); }
+  bool hasAmbientNoisePoints() const { return !ambient_points_.empty(); }
+  double getReflectivity(std::size_t i) const { return reflectivities_.at(i); }
+  double getAmbientNoise(std::size_t i) const { return ambient_points_.at(i); }
+  std::vector<double>& reflectivities() { return reflectivities_; }
+  std::vector<double>& ambientPoints() { return ambient_points_; }
+  std::string getPlyReadDirectory() const { return ply_read_directory_; }
 
  private:
   std::vector<common::Point_t> points_;
+  std::vector<double> reflectivities_, ambient_points_, ranges_;
+  std::string ply_read_directory_;
 };
 using PointCloudPtr = std::shared_ptr<PointCloud>;
 
@@ -173,7 +191,9 @@ using VectorXd = std::vector<double>;  // stands in for Eigen::VectorXd
 
 // One context per (bandwidth, voxel count, capacity); shared by the objects
 // below and released with the last of them.
-std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points);
+// channels: 2, or 4 for clouds that carry reflectivity and ambient.
+std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points,
+                                        int channels = 2);
 
 class SphericalCorrelation {
  public:

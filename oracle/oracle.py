@@ -49,6 +49,8 @@ def lib():
             "po_nn_kdtree": (None, [fp, C.c_int64, fp, C.c_int, C.c_float, i32p]),
             "po_squared_voxel_size": (C.c_float, [C.c_int, C.c_int, C.c_int]),
             "po_sample_cloud": (None, [C.c_int, fp, C.c_int64, C.c_float, C.c_int, dp]),
+            "po_sample_cloud_ex": (None, [C.c_int, fp, C.c_void_p, C.c_int64, C.c_float, C.c_int,
+                                          dp]),
             "po_seanindex": (C.c_int, [C.c_int, C.c_int, C.c_int]),
             "po_makeweights": (None, [C.c_int, dp]),
             "po_legendre": (None, [C.c_int, C.c_int, dp]),
@@ -68,6 +70,8 @@ def lib():
             "po_linspaced": (None, [C.c_int, C.c_float, C.c_float, fp]),
             "po_histc": (C.c_int, [C.c_double, fp, C.c_int]),
             "po_voxelize": (None, [fp, C.c_int64, C.c_int, C.c_float, C.c_float, dp, C.c_void_p]),
+            "po_voxelize_ex": (None, [fp, C.c_void_p, C.c_int64, C.c_int, C.c_float, C.c_float, dp,
+                                      C.c_void_p]),
             "po_r2c3": (None, [C.c_int, dp, dp]),
             "po_c2r3": (None, [C.c_int, dp, dp]),
             "po_spatial_correlate": (None, [C.c_int, C.c_int, dp, dp, dp]),
@@ -83,6 +87,9 @@ def lib():
             "po_sub2ind": (C.c_uint32, [C.c_uint32] * 5),
             "po_register": (None, [C.POINTER(Config), fp, C.c_int64, fp, C.c_int64,
                                    C.POINTER(Result), C.c_void_p, C.c_void_p, C.c_void_p]),
+            "po_register_ex": (None, [C.POINTER(Config), fp, C.c_void_p, C.c_int64, fp,
+                                      C.c_void_p, C.c_int64, C.POINTER(Result), C.c_void_p,
+                                      C.c_void_p, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -135,12 +142,25 @@ def squared_voxel_size(lp_lower=0, lp_upper=100000, n_voxels=100):
     return lib().po_squared_voxel_size(lp_lower, lp_upper, n_voxels)
 
 
-def sample_cloud(bw, pts, max_sq=None, kdtree=True):
+def _extra(extra, P):
+    """(reflectivity, ambient) per point as a contiguous (P, 2) float32 array,
+    or None."""
+    if extra is None:
+        return None
+    e = _c(extra, np.float32)
+    assert e.shape == (P, 2), e.shape
+    return e
+
+
+def sample_cloud(bw, pts, max_sq=None, kdtree=True, extra=None):
+    """extra: (P, 2) reflectivity and ambient per point, or None (zero
+    channels, as for a PointXYZI cloud)."""
     pts = _c(pts, np.float32)
     if max_sq is None:
         max_sq = squared_voxel_size()
     out = np.zeros((4, 4 * bw * bw), np.float64)
-    lib().po_sample_cloud(bw, pts, pts.shape[0], max_sq, int(kdtree), out)
+    e = _extra(extra, pts.shape[0])
+    lib().po_sample_cloud_ex(bw, pts, _ptr(e), pts.shape[0], max_sq, int(kdtree), out)
     return out
 
 
@@ -250,11 +270,12 @@ def linspaced(N, lo, hi):
     return E
 
 
-def voxelize(pts, N, lo=-50.0, hi=50.0, hist=False):
+def voxelize(pts, N, lo=-50.0, hi=50.0, hist=False, extra=None):
     pts = _c(pts, np.float32)
     ch = np.zeros((4, N ** 3))
     hs = np.zeros(N ** 3) if hist else None
-    lib().po_voxelize(pts, pts.shape[0], N, lo, hi, ch, _ptr(hs))
+    e = _extra(extra, pts.shape[0])
+    lib().po_voxelize_ex(pts, _ptr(e), pts.shape[0], N, lo, hi, ch, _ptr(hs))
     return (ch, hs) if hist else ch
 
 
@@ -443,16 +464,20 @@ def sub2ind(i, j, k, rows, cols):
 
 
 def register(target, source, bw=32, n_voxels=64, lower=-50, upper=50,
-             kdtree=True, want_grids=False, want_reg=False):
+             kdtree=True, want_grids=False, want_reg=False, target_extra=None,
+             source_extra=None):
+    """target_extra / source_extra: (P, 2) reflectivity and ambient per
+    point, or None (zero channels)."""
     cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree))
     t = _c(target, np.float32)
     s = _c(source, np.float32)
+    te, se = _extra(target_extra, t.shape[0]), _extra(source_extra, s.shape[0])
     res = Result()
     rot = np.zeros((2 * bw) ** 3) if want_grids else None
     tr = np.zeros(n_voxels ** 3) if want_grids else None
     reg = np.zeros_like(s) if want_reg else None
-    lib().po_register(C.byref(cfg), t, t.shape[0], s, s.shape[0], C.byref(res),
-                      _ptr(reg), _ptr(rot), _ptr(tr))
+    lib().po_register_ex(C.byref(cfg), t, _ptr(te), t.shape[0], s, _ptr(se), s.shape[0],
+                         C.byref(res), _ptr(reg), _ptr(rot), _ptr(tr))
     out = {
         "rot_index": res.rot_index, "trans_index": res.trans_index,
         "rot_value": res.rot_value, "trans_value": res.trans_value,

@@ -287,6 +287,11 @@ float po_squared_voxel_size(int low_pass_lower, int low_pass_upper,
  * function-value.cc:15-45: one neighbour, so the average is the value. */
 void po_sample_cloud(int bw, const float* pts, int64_t P, float max_sq_dist,
                      int use_kdtree, double* grids) {
+  po_sample_cloud_ex(bw, pts, NULL, P, max_sq_dist, use_kdtree, grids);
+}
+
+void po_sample_cloud_ex(int bw, const float* pts, const float* extra, int64_t P,
+                        float max_sq_dist, int use_kdtree, double* grids) {
   const int n = 2 * bw, nq = n * n;
   float* q = (float*)xcalloc(3 * (size_t)nq, sizeof(float));
   float* proj = (float*)xcalloc(3 * (size_t)P, sizeof(float));
@@ -303,6 +308,10 @@ void po_sample_cloud(int bw, const float* pts, int64_t P, float max_sq_dist,
     if (nn[i] < 0) continue;
     grids[i] = range[nn[i]];
     grids[(size_t)nq + i] = (double)pts[4 * (size_t)nn[i] + 3];
+    if (extra) { /* point-cloud.cc:152-155: the nearest point's own values */
+      grids[2 * (size_t)nq + i] = (double)extra[2 * (size_t)nn[i]];
+      grids[3 * (size_t)nq + i] = (double)extra[2 * (size_t)nn[i] + 1];
+    }
   }
   free(q);
   free(proj);
@@ -912,6 +921,11 @@ int po_histc(double x, const float* E, int N) {
 
 void po_voxelize(const float* pts, int64_t P, int N, float lo, float hi,
                  double* ch, double* hist) {
+  po_voxelize_ex(pts, NULL, P, N, lo, hi, ch, hist);
+}
+
+void po_voxelize_ex(const float* pts, const float* extra, int64_t P, int N, float lo,
+                    float hi, double* ch, double* hist) {
   const uint32_t nf = (uint32_t)N * N * N;
   float* E = (float*)xcalloc(N, sizeof(float));
   double* hs = hist ? hist : (double*)xcalloc(nf, sizeof(double));
@@ -929,6 +943,10 @@ void po_voxelize(const float* pts, int64_t P, int N, float lo, float hi,
     if (lin == nf) continue; /* the reference would write one past the end */
     ch[lin] = ch[lin] + (double)pts[4 * i + 3];
     ch[(size_t)nf + lin] = ch[(size_t)nf + lin] + (double)sqrtf(x * x + y * y + z * z);
+    if (extra) { /* phase-aligner.cc:113-118 */
+      ch[2 * (size_t)nf + lin] = ch[2 * (size_t)nf + lin] + (double)extra[2 * i];
+      ch[3 * (size_t)nf + lin] = ch[3 * (size_t)nf + lin] + (double)extra[2 * i + 1];
+    }
     hs[lin] = hs[lin] + 1;
   }
   for (int c = 0; c < 4; ++c)
@@ -1188,6 +1206,13 @@ double po_translation_from_index(double index, uint32_t N, int lo, int hi) {
 void po_register(const po_config* cfg, const float* target, int64_t Pt,
                  const float* source, int64_t Ps, po_result* res,
                  float* reg_out, double* rot_corr, double* trans_corr) {
+  po_register_ex(cfg, target, NULL, Pt, source, NULL, Ps, res, reg_out, rot_corr,
+                 trans_corr);
+}
+
+void po_register_ex(const po_config* cfg, const float* target, const float* target_extra,
+                    int64_t Pt, const float* source, const float* source_extra, int64_t Ps,
+                    po_result* res, float* reg_out, double* rot_corr, double* trans_corr) {
   const int bw = cfg->bw, n = 2 * bw, N = cfg->n_voxels;
   const size_t nn = (size_t)n * n, n3 = (size_t)n * n * n;
   const size_t T3 = (size_t)N * N * N;
@@ -1197,8 +1222,8 @@ void po_register(const po_config* cfg, const float* target, int64_t Pt,
   double t0 = now_ms();
   double* fg = (double*)xcalloc(4 * nn, sizeof(double));
   double* hg = (double*)xcalloc(4 * nn, sizeof(double));
-  po_sample_cloud(bw, target, Pt, max_sq, cfg->use_kdtree, fg);
-  po_sample_cloud(bw, source, Ps, max_sq, cfg->use_kdtree, hg);
+  po_sample_cloud_ex(bw, target, target_extra, Pt, max_sq, cfg->use_kdtree, fg);
+  po_sample_cloud_ex(bw, source, source_extra, Ps, max_sq, cfg->use_kdtree, hg);
   double t1 = now_ms();
   double* mag = rot_corr ? rot_corr : (double*)xcalloc(n3, sizeof(double));
   po_spherical_correlate_laplace(bw, 4, fg, hg, mag, NULL, NULL);
@@ -1214,8 +1239,9 @@ void po_register(const po_config* cfg, const float* target, int64_t Pt,
   double t2 = now_ms();
   double* fv = (double*)xcalloc(4 * T3, sizeof(double));
   double* gv = (double*)xcalloc(4 * T3, sizeof(double));
-  po_voxelize(target, Pt, N, (float)cfg->lower, (float)cfg->upper, fv, NULL);
-  po_voxelize(reg, Ps, N, (float)cfg->lower, (float)cfg->upper, gv, NULL);
+  /* the rotated source keeps each point's reflectivity and ambient */
+  po_voxelize_ex(target, target_extra, Pt, N, (float)cfg->lower, (float)cfg->upper, fv, NULL);
+  po_voxelize_ex(reg, source_extra, Ps, N, (float)cfg->lower, (float)cfg->upper, gv, NULL);
   double* c = trans_corr ? trans_corr : (double*)xcalloc(T3, sizeof(double));
   po_spatial_correlate_laplace(N, 4, fv, gv, c);
   res->trans_index = po_argmax(c, T3);

@@ -53,6 +53,11 @@ float po_squared_voxel_size(int low_pass_lower, int low_pass_upper,
  * (spherical-combined-worker.cc:23-57).  use_kdtree selects po_nn_kdtree. */
 void po_sample_cloud(int bw, const float* pts, int64_t P, float max_sq_dist,
                      int use_kdtree, double* grids);
+/* The same with per-point (reflectivity, ambient) pairs (extra: 2 x P floats,
+ * NULL = none, as for a PointXYZI cloud): grids 2 and 3 then hold the
+ * nearest point's values (point-cloud.cc:152-155). */
+void po_sample_cloud_ex(int bw, const float* pts, const float* extra, int64_t P,
+                        float max_sq_dist, int use_kdtree, double* grids);
 
 /* ---- S^2 transform ------------------------------------------------------ */
 int po_seanindex(int m, int l, int bw); /* s2_primitive.c:300-313 */
@@ -122,6 +127,10 @@ int po_histc(double x, const float* E, int N);
  * hist (optional): N^3 counts. */
 void po_voxelize(const float* pts, int64_t P, int N, float lo, float hi,
                  double* ch, double* hist);
+/* The same with per-point (reflectivity, ambient) pairs (NULL = none):
+ * channels 2 and 3 then hold their voxel means (phase-aligner.cc:113-131). */
+void po_voxelize_ex(const float* pts, const float* extra, int64_t P, int N, float lo,
+                    float hi, double* ch, double* hist);
 /* 3-D r2c into a flat N^3 complex buffer with a zero tail
  * (spatial-correlation.cc:30-57): out[d0][d1][k2], k2 <= N/2. */
 void po_r2c3(int N, const double* in, double* out_cplx);
@@ -183,6 +192,11 @@ typedef struct {
 void po_register(const po_config* cfg, const float* target, int64_t Pt,
                  const float* source, int64_t Ps, po_result* res,
                  float* reg_out, double* rot_corr, double* trans_corr);
+/* The same for clouds with (reflectivity, ambient) per point (2 x P floats
+ * each, NULL = none): all four channels of the fusions are then live. */
+void po_register_ex(const po_config* cfg, const float* target, const float* target_extra,
+                    int64_t Pt, const float* source, const float* source_extra, int64_t Ps,
+                    po_result* res, float* reg_out, double* rot_corr, double* trans_corr);
 
 #ifdef __cplusplus
 }

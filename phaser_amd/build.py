@@ -26,7 +26,7 @@ ARCH = os.environ.get("PHZ_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["kernels_s2.hip", "kernels_so3.hip", "kernels_spatial.hip", "kernels_mfma.hip",
                "capi.hip"]
 CPP_SOURCES = ["tables.cpp"]
-HOST_SOURCES = ["host/phaser_host.cc"]
+HOST_SOURCES = ["host/phaser_host.cc", "host/ply_reader.cc"]
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value",

@@ -20,7 +20,7 @@ class Config(C.Structure):
                 ("low_pass_lower", C.c_int32), ("low_pass_upper", C.c_int32),
                 ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32),
                 ("precision", C.c_int32), ("bingham_peak_neighbors", C.c_int32),
-                ("gaussian_peak_neighbors", C.c_int32)]
+                ("gaussian_peak_neighbors", C.c_int32), ("channels", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -61,6 +61,9 @@ EXPORTS = {
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "phz_register_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.POINTER(Result)]),
+    "phz_register_channels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                        C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Result),
+                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     "phz_memory_usage": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "phz_estimate_memory": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_size_t)]),
     "phz_sample_s2": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]),
@@ -209,14 +212,21 @@ class Context:
         self.close()
 
     # ---- end to end
-    def register(self, target, source, grids=False, registered=False):
+    def register(self, target, source, grids=False, registered=False, target_extra=None,
+                 source_extra=None):
+        """target_extra / source_extra: (P, 2) reflectivity and ambient per
+        point (a context with channels=4), or None (zero channels)."""
         t, s = _f32(target), _f32(source)
+        te = None if target_extra is None else _f32(target_extra)
+        se = None if source_extra is None else _f32(source_extra)
+        for e, cl in ((te, t), (se, s)):
+            assert e is None or e.shape == (cl.shape[0], 2)
         res = Result()
         rc = np.zeros(self.n ** 3, np.float32) if grids else None
         tc = np.zeros(self.N ** 3, np.float32) if grids else None
         reg = np.zeros_like(s) if registered else None
-        _check(lib().phz_register(self.h, _p(t), t.shape[0], _p(s), s.shape[0], C.byref(res),
-                                  _p(rc), _p(tc), _p(reg)))
+        _check(lib().phz_register_channels(self.h, _p(t), _p(te), t.shape[0], _p(s), _p(se),
+                                           s.shape[0], C.byref(res), _p(rc), _p(tc), _p(reg)))
         out = res.as_dict()
         if grids:
             out["rot_corr"] = rc

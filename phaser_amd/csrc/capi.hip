@@ -87,6 +87,11 @@ struct phz_ctx {
   float2* d_spec = nullptr;
   float2* d_work = nullptr;
   float* d_corr_trans = nullptr;
+  // clouds with reflectivity and ambient (channels = 4)
+  float2* d_extra = nullptr;   // [cloud slot][point] (reflectivity, ambient)
+  float* d_sumF = nullptr;     // voxel sums of reflectivity / ambient, as d_sumR
+  float* d_sumA = nullptr;
+  float2* d_spec2 = nullptr;   // their spectra, laid out as d_spec
   float* d_peak_rot = nullptr;    // per-pair grids for the peak-window fits when the
   float* d_peak_trans = nullptr;  // caller asks for no grids (neighbours > 0 only)
   float4* d_reg = nullptr;
@@ -174,11 +179,21 @@ int dalloc(phz_ctx* c, T** p, size_t count) {
   return 0;
 }
 
+// A synchronous copy on the context's own (non-blocking) stream.  Never the
+// legacy stream: while another thread captures a graph, HIP rejects legacy
+// stream work ("operation would make the legacy stream depend on a
+// capturing blocking stream"), and contexts are created and used by
+// concurrent threads (the request server, multi-stream callers).
+int copy_sync(phz_ctx* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  PHZ_HIP(hipMemcpyAsync(dst, src, bytes, kind, c->stream));
+  PHZ_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 template <class T>
 int upload(phz_ctx* c, T** p, const std::vector<T>& v) {
   if (int rc = dalloc(c, p, v.size())) return rc;
-  PHZ_HIP(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-  return 0;
+  return copy_sync(c, *p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice);
 }
 
 #define TRY(x)                \
@@ -193,7 +208,8 @@ size_t context_bytes(const phz_config& cf) {
   const int64_t B = cf.bandwidth, n = 2 * B, N = cf.n_voxels;
   const int64_t nq = n * n, T3 = N * N * N, H = N * N * (N / 2 + 1);
   const int64_t P = cf.max_batch, M = 2 * P, cap = cf.max_points;
-  const int64_t G = std::max<int64_t>(2 * M, 16), MV = std::max<int64_t>(M, 16);
+  const int64_t G = std::max<int64_t>((cf.channels == 4 ? 4 : 2) * M, 16);
+  const int64_t MV = std::max<int64_t>(M, 16);
   const int64_t ppp = std::max(so3_gamma_blocks((int)B), spatial_c2r_blocks((int)N));
   size_t s = 0;
   auto add = [&](int64_t count, size_t elem) { s += elem * (size_t)(count ? count : 1); };
@@ -244,6 +260,12 @@ size_t context_bytes(const phz_config& cf) {
   add(cap, sizeof(float4));
   if (cf.bingham_peak_neighbors > 0) add(P * n * nq, sizeof(float));
   if (cf.gaussian_peak_neighbors > 0) add(P * T3, sizeof(float));
+  if (cf.channels == 4) {
+    add(M * cap, sizeof(float2));
+    add(M * T3, sizeof(float));
+    add(M * T3, sizeof(float));
+    add(std::max<int64_t>(2 * M, 16) * H, sizeof(float2));
+  }
   return s;
 }
 
@@ -260,6 +282,8 @@ int check_config(const phz_config& cf) {
             "bingham_peak_neighbors must be in [0, 2^20]");
   PHZ_CHECK(cf.gaussian_peak_neighbors >= 0 && cf.gaussian_peak_neighbors <= (1 << 20),
             "gaussian_peak_neighbors must be in [0, 2^20]");
+  PHZ_CHECK(cf.channels == 0 || cf.channels == 2 || cf.channels == 4,
+            "channels must be 2 or 4 (0 means 2)");
   PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
   PHZ_CHECK(make_plan(cf.n_voxels).nst > 0, "n_voxels must factor into 2, 3 and 5");
   return 0;
@@ -328,8 +352,11 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));
-  // stage entry points use up to 16 grids / signals
-  const int G = M * 2 > 16 ? M * 2 : 16;
+  // stage entry points use up to 16 grids / signals; with four channels the
+  // reflectivity and ambient grids of all clouds follow the range and
+  // intensity grids
+  const int nch = cf.channels == 4 ? 4 : 2;
+  const int G = M * nch > 16 ? M * nch : 16;
   TRY(dalloc(c, &c->d_grids, (size_t)G * nq));
   TRY(dalloc(c, &c->d_F, (size_t)G * c->B * c->n));
   TRY(dalloc(c, &c->d_coef, (size_t)G * c->B * c->B));
@@ -357,6 +384,12 @@ int init_ctx(phz_ctx* c) {
     TRY(dalloc(c, &c->d_peak_rot, (size_t)P * nq * c->n));
   if (cf.gaussian_peak_neighbors > 0)
     TRY(dalloc(c, &c->d_peak_trans, (size_t)P * c->T3));
+  if (cf.channels == 4) {
+    TRY(dalloc(c, &c->d_extra, (size_t)M * cap));
+    TRY(dalloc(c, &c->d_sumF, (size_t)M * c->T3));
+    TRY(dalloc(c, &c->d_sumA, (size_t)M * c->T3));
+    TRY(dalloc(c, &c->d_spec2, (size_t)(2 * M > 16 ? 2 * M : 16) * c->H));
+  }
   PHZ_HIP(hipHostMalloc((void**)&c->h_npts, sizeof(int) * M, hipHostMallocDefault));
   // voxel accumulators stay zero between calls: every pipeline run clears
   // exactly the voxels it touched
@@ -365,6 +398,10 @@ int init_ctx(phz_ctx* c) {
   launch_fill_u32(c->d_cnt, (size_t)M * c->T3, 0u, c->stream);
   launch_fill_u32((uint32_t*)c->d_row_occ, occ_row_bytes / 4, 0u, c->stream);
   launch_fill_u32((uint32_t*)c->d_plane_occ, occ_plane_bytes / 4, 0u, c->stream);
+  if (cf.channels == 4) {
+    launch_fill_u32((uint32_t*)c->d_sumF, (size_t)M * c->T3, 0u, c->stream);
+    launch_fill_u32((uint32_t*)c->d_sumA, (size_t)M * c->T3, 0u, c->stream);
+  }
   PHZ_HIP(hipStreamSynchronize(c->stream));
   PHZ_CHECK(c->alloc_bytes == context_bytes(cf), "internal error: device memory estimate is stale");
   c->tuning.fwd_x = c->tuning.fwd_z = c->tuning.inv_x = fft_cols_default_tile(c->N);
@@ -390,6 +427,17 @@ VoxelOcc occ_from(phz_ctx* c, int first) {
   return o;
 }
 
+// reflectivity / ambient of cloud slots [first, ...) (none with two channels)
+VoxelExtra extra_from(phz_ctx* c, int first) {
+  VoxelExtra e;
+  if (c->cfg.channels == 4) {
+    e.extra = c->d_extra + (int64_t)first * c->cap;
+    e.sumF = c->d_sumF + (int64_t)first * c->T3;
+    e.sumA = c->d_sumA + (int64_t)first * c->T3;
+  }
+  return e;
+}
+
 ChannelSet channels(const float2* a, const float2* b, int C) {
   ChannelSet cs{};
   cs.C = C;
@@ -411,6 +459,9 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   const int ncl = nt + np;
   const int B = c->B, n = c->n, N = c->N;
   const int64_t nq = (int64_t)n * n, BB = (int64_t)B * B;
+  // four channels: each cloud's reflectivity and ambient grids follow all
+  // clouds' range and intensity grids (and so do their coefficients)
+  const bool four = c->cfg.channels == 4;
   // rotation
   {
     StageScope t(c, kS2Cells, s);
@@ -419,17 +470,23 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     StageScope t(c, kS2NN, s);
     launch_s2_nn(c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq, c->cells, c->d_grids,
-                 c->tuning.nn_lanes, s);
+                 c->tuning.nn_lanes, s, four ? c->d_extra : nullptr,
+                 four ? c->d_grids + 2 * ncl * nq : nullptr);
   }
   {
     StageScope t(c, kS2Transform, s);
-    launch_fst(c->d_grids, 2 * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef,
-               c->cfg.precision, s);
+    launch_fst(c->d_grids, (four ? 4 : 2) * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F,
+               c->d_coef, c->cfg.precision, s);
   }
-  {  // spherical order {range, intensity, refl=0, amb=0}, one launch for all clouds
+  {  // spherical order {range, intensity, refl, amb} (refl = amb = 0 with two
+     // channels), one launch for all clouds
     StageScope t(c, kFuse, s);
-    launch_fuse(channels(c->d_coef, c->d_coef + BB, 4), (uint32_t)BB, c->s2.runs, c->d_fused, s,
-                ncl, 2 * BB);
+    ChannelSet cs = channels(c->d_coef, c->d_coef + BB, 4);
+    if (four) {
+      cs.ch[2] = c->d_coef + 2 * ncl * BB;
+      cs.ch[3] = c->d_coef + 2 * ncl * BB + BB;
+    }
+    launch_fuse(cs, (uint32_t)BB, c->s2.runs, c->d_fused, s, ncl, 2 * BB);
   }
   {
     StageScope t(c, kSo3Synth, s);
@@ -453,10 +510,10 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     StageScope t(c, kVoxelScatter, s);
     launch_voxel_scatter(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
-                         c->d_sumR, c->d_cnt, occ_from(c, 0), s);
+                         c->d_sumR, c->d_cnt, occ_from(c, 0), s, extra_from(c, 0));
     launch_voxel_scatter(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
-                         c->d_cnt + nt * c->T3, occ_from(c, nt), s);
+                         c->d_cnt + nt * c->T3, occ_from(c, nt), s, extra_from(c, nt));
   }
   // spatial order {intensity, range, refl=0, amb=0}; pair p: target p (or
   // the shared target 0) against source nt+p
@@ -466,37 +523,62 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   const int64_t fs = shared ? 0 : 2 * c->H, gs = 2 * c->H;
   // zz: the forward z pass runs inside the cross-spectrum kernel; occ: the
   // r2c and x passes before it also skip rows / planes without points
-  const bool zz = c->occ_fwd && c->tiles.fused_reg && c->tuning.fwd_z == kRegisterEngine &&
-                  spatial_cross_zz_ok(N, Fc, Gc);
+  // four channels: the reflectivity / ambient spectra in d_spec2 (layout of
+  // d_spec) go through the dense passes and the channel-generic cross
+  // spectrum
+  const bool zz = !four && c->occ_fwd && c->tiles.fused_reg &&
+                  c->tuning.fwd_z == kRegisterEngine && spatial_cross_zz_ok(N, Fc, Gc);
   const bool occ = zz && !c->plane_fwd && c->tuning.fwd_x == kRegisterEngine;
+  const bool plane = c->plane_fwd && !four;
   {  // y (and with the plane pass also x) first, then the remaining columns
     StageScope t(c, kSpatialR2c, s);
-    if (c->plane_fwd)
+    if (plane)
       launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                                c->d_spec, s);
     else if (occ)
       launch_spatial_r2c_rows(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, occ_from(c, 0), c->d_tw_N,
                               c->d_spec, s);
-    else if (c->tiles.fused_reg)
+    else if (c->tiles.fused_reg) {
       launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_row_occ, false,
                              c->d_tw_N, c->d_spec, s);
-    else
+      if (four)
+        launch_spatial_r2c_reg(N, ncl, c->d_sumF, c->d_sumA, c->d_cnt, c->d_row_occ, false,
+                               c->d_tw_N, c->d_spec2, s);
+    } else {
       launch_spatial_r2c(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                          c->d_spec, s);
+      if (four)
+        launch_spatial_r2c(N, ncl, c->d_sumF, c->d_sumA, c->d_cnt, c->plan_N, c->d_tw_N,
+                           c->d_spec2, s);
+    }
   }
   {
     StageScope t(c, kSpatialFftFwd, s);
     if (occ)
       launch_spatial_fft_x_occ(N, 2 * ncl, c->d_spec, occ_from(c, 0), c->d_tw_N, s);
-    else if (!c->plane_fwd)
+    else if (!plane)
       launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
     if (!zz)
       launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
+    if (four) {
+      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec2, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
+      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec2, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
+    }
   }
   {
     {
       StageScope t(c, kSpatialCrossZ, s);
-      if (zz)
+      if (four) {
+        const float2* F2 = c->d_spec2;
+        const float2* G2 = c->d_spec2 + (int64_t)(2 * nt) * c->H;
+        ChannelSet F4 = Fc, G4 = Gc;
+        F4.ch[2] = F2;
+        F4.ch[3] = F2 + c->H;
+        G4.ch[2] = G2;
+        G4.ch[3] = G2 + c->H;
+        launch_spatial_cross_z(N, np, F4, fs, G4, gs, c->vox.runs, true, c->plan_N, c->d_tw_N,
+                               c->d_work, s);
+      } else if (zz)
         launch_spatial_cross_zz_reg(N, np, Fc, fs, Gc, gs, occ ? c->d_plane_occ : nullptr,
                                     shared ? 0 : N, occ ? c->d_plane_occ + (int64_t)nt * N : nullptr,
                                     N, c->vox.runs, c->d_tw_N, c->d_work, s);
@@ -512,10 +594,10 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     {
       StageScope t(c, kVoxelClear, s);
       launch_voxel_clear(c->d_pts, c->d_npts, c->cap, nt, nullptr, 0, N, c->d_edges, c->d_sumI,
-                         c->d_sumR, c->d_cnt, occ_from(c, 0), s);
+                         c->d_sumR, c->d_cnt, occ_from(c, 0), s, extra_from(c, 0));
       launch_voxel_clear(c->d_pts + nt * c->cap, c->d_npts + nt, c->cap, np, c->d_pose, 1, N,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
-                         c->d_cnt + nt * c->T3, occ_from(c, nt), s);
+                         c->d_cnt + nt * c->T3, occ_from(c, nt), s, extra_from(c, nt));
     }
     const bool inv_plane = c->inv_plane && c->tiles.fused_reg &&
                            c->tuning.inv_x == kRegisterEngine && spatial_inv_plane_ok(N);
@@ -632,12 +714,22 @@ int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
   return 0;
 }
 
+// extra: (reflectivity, ambient) per point, or null (zero channels); only a
+// context with four channels keeps them
 int load_cloud(phz_ctx* c, int slot, const float* pts, int64_t P, hipMemcpyKind kind,
-               hipStream_t s) {
+               hipStream_t s, const float* extra = nullptr) {
   PHZ_CHECK(P >= 0 && P <= c->cap, "point count exceeds max_points");
   PHZ_CHECK(P == 0 || pts != nullptr, "null point buffer");
+  PHZ_CHECK(!extra || c->cfg.channels == 4,
+            "reflectivity and ambient need a context created with channels = 4");
   if (P > 0)
     PHZ_HIP(hipMemcpyAsync(c->d_pts + slot * c->cap, pts, sizeof(float4) * P, kind, s));
+  if (c->cfg.channels == 4 && P > 0) {
+    if (extra)
+      PHZ_HIP(hipMemcpyAsync(c->d_extra + slot * c->cap, extra, sizeof(float2) * P, kind, s));
+    else
+      launch_fill_u32((uint32_t*)(c->d_extra + slot * c->cap), (size_t)2 * P, 0u, s);
+  }
   c->h_npts[slot] = (int)P;
   return 0;
 }
@@ -666,6 +758,7 @@ void phz_default_config(phz_config* cfg) {
   cfg->max_points = 262144;
   cfg->max_batch = 1;
   cfg->device = 0;
+  cfg->channels = 2;
 }
 
 const char* phz_last_error(void) { return g_last_error.c_str(); }
@@ -708,10 +801,18 @@ int phz_get_config(const phz_ctx* c, phz_config* out) {
 
 int phz_register(phz_ctx* c, const float* target, int64_t Pt, const float* source, int64_t Ps,
                  phz_result* result, float* rot_corr, float* trans_corr, float* registered) {
+  return phz_register_channels(c, target, nullptr, Pt, source, nullptr, Ps, result, rot_corr,
+                               trans_corr, registered);
+}
+
+int phz_register_channels(phz_ctx* c, const float* target, const float* target_ra, int64_t Pt,
+                          const float* source, const float* source_ra, int64_t Ps,
+                          phz_result* result, float* rot_corr, float* trans_corr,
+                          float* registered) {
   PHZ_CHECK(c && result, "null argument");
   hipStream_t s = c->stream;
-  TRY(load_cloud(c, 0, target, Pt, hipMemcpyHostToDevice, s));
-  TRY(load_cloud(c, 1, source, Ps, hipMemcpyHostToDevice, s));
+  TRY(load_cloud(c, 0, target, Pt, hipMemcpyHostToDevice, s, target_ra));
+  TRY(load_cloud(c, 1, source, Ps, hipMemcpyHostToDevice, s, source_ra));
   TRY(commit_counts(c, 2, s));
   TRY(exec_pairs(c, 1, false, s, rot_corr ? c->d_corr_rot : nullptr,
                  trans_corr ? c->d_corr_trans : nullptr));
@@ -719,11 +820,11 @@ int phz_register(phz_ctx* c, const float* target, int64_t Pt, const float* sourc
   TRY(fetch_results(c, 1, result, s));
   const int64_t n3 = (int64_t)c->n * c->n * c->n;
   if (rot_corr)
-    PHZ_HIP(hipMemcpy(rot_corr, c->d_corr_rot, sizeof(float) * n3, hipMemcpyDeviceToHost));
+    TRY(copy_sync(c, rot_corr, c->d_corr_rot, sizeof(float) * n3, hipMemcpyDeviceToHost));
   if (trans_corr)
-    PHZ_HIP(hipMemcpy(trans_corr, c->d_corr_trans, sizeof(float) * c->T3, hipMemcpyDeviceToHost));
+    TRY(copy_sync(c, trans_corr, c->d_corr_trans, sizeof(float) * c->T3, hipMemcpyDeviceToHost));
   if (registered && Ps > 0)
-    PHZ_HIP(hipMemcpy(registered, c->d_reg, sizeof(float4) * Ps, hipMemcpyDeviceToHost));
+    TRY(copy_sync(c, registered, c->d_reg, sizeof(float4) * Ps, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -828,9 +929,9 @@ static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, floa
   PHZ_HIP(hipStreamSynchronize(s));
   if (argmax) *argmax = key_index(key);
   if (corr)
-    PHZ_HIP(hipMemcpy(corr, c->d_corr_rot, sizeof(float) * n * n * n, hipMemcpyDeviceToHost));
+    TRY(copy_sync(c, corr, c->d_corr_rot, sizeof(float) * n * n * n, hipMemcpyDeviceToHost));
   if (synthesis)
-    PHZ_HIP(hipMemcpy(synthesis, c->d_S, sizeof(float2) * (size_t)B * (2 * B - 1) * n,
+    TRY(copy_sync(c, synthesis, c->d_S, sizeof(float2) * (size_t)B * (2 * B - 1) * n,
                       hipMemcpyDeviceToHost));
   return 0;
 }
@@ -914,7 +1015,7 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
   PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipStreamSynchronize(s));
   if (argmax) *argmax = key_index(key);
-  if (corr) PHZ_HIP(hipMemcpy(corr, c->d_corr_trans, sizeof(float) * T, hipMemcpyDeviceToHost));
+  if (corr) TRY(copy_sync(c, corr, c->d_corr_trans, sizeof(float) * T, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -967,7 +1068,10 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
   if (!rc && (e = hipMemcpyAsync(&key, d_key, sizeof(key), hipMemcpyDeviceToHost, s)) != hipSuccess)
     fail("copy", e);
   if (!rc && (e = hipStreamSynchronize(s)) != hipSuccess) fail("sync", e);
-  if (!rc && corr && (e = hipMemcpy(corr, d_corr, sizeof(float) * T, hipMemcpyDeviceToHost)) != hipSuccess)
+  if (!rc && corr &&
+      ((e = hipMemcpyAsync(corr, d_corr, sizeof(float) * T, hipMemcpyDeviceToHost, s)) !=
+           hipSuccess ||
+       (e = hipStreamSynchronize(s)) != hipSuccess))
     fail("copy", e);
   if (!rc && argmax) *argmax = key_index(key);
   hipFree(d_in);
@@ -1036,7 +1140,7 @@ int phz_memcpy(phz_ctx* c, void* dst, const void* src, size_t bytes, int kind) {
   const hipMemcpyKind k[4] = {hipMemcpyHostToHost, hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
                               hipMemcpyDeviceToDevice};
   PHZ_HIP(hipStreamSynchronize(c->stream));
-  PHZ_HIP(hipMemcpy(dst, src, bytes, k[kind]));
+  TRY(copy_sync(c, dst, src, bytes, k[kind]));
   return 0;
 }
 

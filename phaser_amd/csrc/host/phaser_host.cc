@@ -10,6 +10,7 @@
 #include <tuple>
 
 #include "phaser_amd.h"
+#include "phaser_ply.h"
 
 namespace phaser_core {
 int32_t FLAGS_phaser_core_spherical_bandwidth = 75;
@@ -26,6 +27,10 @@ int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound = 100000;
 
 int32_t FLAGS_bingham_peak_neighbors = 0;
 int32_t FLAGS_gaussian_peak_neighbors = 0;
+std::string FLAGS_phaser_ply_intensity_str = "intensity";
+std::string FLAGS_phaser_ply_reflectivity_str = "reflectivity";
+std::string FLAGS_phaser_ply_ambient_str = "noise";
+std::string FLAGS_phaser_ply_range_str = "range";
 
 namespace {
 
@@ -51,9 +56,9 @@ constexpr int64_t kDefaultCapacity = 262144;
 
 namespace phaser_core {
 
-std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points) {
+std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points, int channels) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int>,
+  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int>,
                   std::weak_ptr<phz_ctx>>
       cache;
   std::lock_guard<std::mutex> lock(mu);
@@ -62,7 +67,8 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
                                    FLAGS_phaser_core_spatial_discretize_upper,
                                    FLAGS_phaser_core_spatial_low_pass_lower_bound,
                                    FLAGS_phaser_core_spatial_low_pass_upper_bound,
-                                   FLAGS_bingham_peak_neighbors, FLAGS_gaussian_peak_neighbors);
+                                   FLAGS_bingham_peak_neighbors, FLAGS_gaussian_peak_neighbors,
+                                   channels);
   if (auto p = cache[key].lock()) return p;
   phz_config cfg;
   phz_default_config(&cfg);
@@ -75,6 +81,7 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
   cfg.low_pass_upper = FLAGS_phaser_core_spatial_low_pass_upper_bound;
   cfg.bingham_peak_neighbors = FLAGS_bingham_peak_neighbors;
   cfg.gaussian_peak_neighbors = FLAGS_gaussian_peak_neighbors;
+  cfg.channels = channels;
   phz_ctx* raw = nullptr;
   check(phz_create(&cfg, &raw), "phz_create");
   std::shared_ptr<phz_ctx> p(raw, [](phz_ctx* c) { phz_destroy(c); });
@@ -238,16 +245,37 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   CHECK_TRUE(cloud_prev != nullptr, "cloud_prev is null");
   CHECK_TRUE(cloud_cur != nullptr, "cloud_cur is null");
   const int64_t need = (int64_t)std::max(cloud_prev->size(), cloud_cur->size());
+  // clouds read from PLY files may carry reflectivity and ambient: then all
+  // four channels of the fusions are live
+  auto has_extra = [](const model::PointCloud& c) {
+    return c.hasReflectivityPoints() || c.hasAmbientNoisePoints();
+  };
+  const bool four = has_extra(*cloud_prev) || has_extra(*cloud_cur);
   std::shared_ptr<phz_ctx> ctx = ctx_;
-  if (need > kDefaultCapacity)
-    ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need);
+  if (need > kDefaultCapacity || four)
+    ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need, four ? 4 : 2);
+  auto extra_of = [](model::PointCloud& c) {
+    std::vector<float> e(2 * c.size(), 0.f);
+    for (size_t i = 0; i < c.size(); ++i) {
+      if (c.hasReflectivityPoints()) e[2 * i] = (float)c.getReflectivity(i);
+      if (c.hasAmbientNoisePoints()) e[2 * i + 1] = (float)c.getAmbientNoise(i);
+    }
+    return e;
+  };
+  std::vector<float> ep, ec;
+  if (four) {
+    ep = extra_of(*cloud_prev);
+    ec = extra_of(*cloud_cur);
+  }
   phz_result r;
   const size_t n3 = 8ull * bandwidth_ * bandwidth_ * bandwidth_;
   std::vector<float> rot(n3);
   model::PointCloud reg = cloud_cur->clone();
-  check(phz_register(ctx.get(), cloud_prev->data(), (int64_t)cloud_prev->size(), cloud_cur->data(),
-                     (int64_t)cloud_cur->size(), &r, rot.data(), nullptr, reg.data()),
-        "phz_register");
+  check(phz_register_channels(ctx.get(), cloud_prev->data(), four ? ep.data() : nullptr,
+                              (int64_t)cloud_prev->size(), cloud_cur->data(),
+                              four ? ec.data() : nullptr, (int64_t)cloud_cur->size(), &r,
+                              rot.data(), nullptr, reg.data()),
+        "phz_register_channels");
   model::RegistrationResult result(std::move(reg));
   result.setRotation({r.quat[0], r.quat[1], r.quat[2], r.quat[3]});
   result.setTranslation({r.translation[0], r.translation[1], r.translation[2]});
@@ -302,6 +330,37 @@ void PointCloud::transformPointCloud(const std::array<float, 16>& T) {
     p.y = y;
     p.z = z;
   }
+}
+
+void PointCloud::readFromFile(const std::string& ply) {
+  CHECK_TRUE(!ply.empty(), "empty PLY path");
+  phz_ply_fields f;
+  f.intensity = FLAGS_phaser_ply_intensity_str.c_str();
+  f.reflectivity = FLAGS_phaser_ply_reflectivity_str.c_str();
+  f.ambient = FLAGS_phaser_ply_ambient_str.c_str();
+  f.range = FLAGS_phaser_ply_range_str.c_str();
+  phz_ply_cloud c;
+  if (phz_ply_read(ply.c_str(), &f, &c) != 0)
+    fatal(std::string("reading ") + ply + ": " + phz_ply_last_error());
+  ply_read_directory_ = ply;
+  // parsePlyPointCloud appends to the cloud (point-cloud.cc:330-349)
+  for (int64_t i = 0; i < c.n; ++i) {
+    common::Point_t p;
+    p.x = c.xyzi[4 * i];
+    p.y = c.xyzi[4 * i + 1];
+    p.z = c.xyzi[4 * i + 2];
+    p.intensity = c.xyzi[4 * i + 3];
+    points_.push_back(p);
+  }
+  reflectivities_.clear();
+  ambient_points_.clear();
+  ranges_.clear();
+  if (c.has_reflectivity)
+    for (int64_t i = 0; i < c.n; ++i) reflectivities_.push_back(c.extra[2 * i]);
+  if (c.has_ambient)
+    for (int64_t i = 0; i < c.n; ++i) ambient_points_.push_back(c.extra[2 * i + 1]);
+  if (c.has_range) ranges_.assign(c.range, c.range + c.n);
+  phz_ply_free(&c);
 }
 
 std::array<double, 8> RegistrationResult::getStateAsVec() const {

@@ -40,8 +40,12 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
 // flow is shared by more queries per wave) and 4 below (fewer queries: more
 // lanes split each one's scans)
 int nn_auto_lanes(int nclouds);
+// extra (nullable): per-point (reflectivity, ambient) of each cloud slot
+// (cloud * cap + original index); xgrids then receives nclouds x 2 x n*n
+// grids of the nearest point's values (zero where none is accepted).
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
-                  float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s);
+                  float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
+                  const float2* extra = nullptr, float* xgrids = nullptr);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
@@ -128,10 +132,17 @@ struct VoxelOcc {
   uint8_t* row = nullptr;
   uint8_t* plane = nullptr;
 };
+// Clouds that also carry (reflectivity, ambient) per point: extra[cloud *
+// cap + i], summed into sumF / sumA (same layout as sumI); null: none.
+struct VoxelExtra {
+  const float2* extra = nullptr;
+  float* sumF = nullptr;
+  float* sumA = nullptr;
+};
 void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
                           const DevPose* pose /*nullable: no transform*/, int pose_stride,
                           int N, const float* edges, float* sumI, float* sumR, uint32_t* cnt,
-                          VoxelOcc occ, hipStream_t s);
+                          VoxelOcc occ, hipStream_t s, VoxelExtra ex = {});
 // forward 3-D r2c of (mean intensity, mean range) per cloud:
 // columns per workgroup of the three column-FFT passes (0 = default);
 // chosen by phz_autotune.  The tile width never changes the arithmetic of a
@@ -248,7 +259,8 @@ void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& pl
 // binning as launch_voxel_scatter), so they need no full clear before reuse
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
                         const DevPose* pose, int pose_stride, int N, const float* edges,
-                        float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s);
+                        float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s,
+                        VoxelExtra ex = {});
 // One wavefront per pair.  corr: the pairs' N^3 grids at corr_stride floats
 // apart (read only when k > 0); k: gaussian_peak_neighbors.
 void launch_trans_pose(int N, int lower, int upper, int npairs,

@@ -268,7 +268,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const int* __restrict__ band_start, const int4* __restrict__ next_right,
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
-                     float* __restrict__ grids) {
+                     float* __restrict__ grids, const float2* __restrict__ extra,
+                     float* __restrict__ xgrids) {
   constexpr int kBatch = 4;  // point loads in flight per lane
   const int c = blockIdx.y;
   const int sub = threadIdx.x & (LANES - 1);
@@ -428,6 +429,12 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     }
     grids[(int64_t)c * 2 * nq + q] = r;
     grids[(int64_t)c * 2 * nq + nq + q] = it;
+    if (extra) {  // the nearest point's (reflectivity, ambient), point-cloud.cc:152-155
+      float2 e = make_float2(0.f, 0.f);
+      if (bpos >= 0 && best <= max_sq) e = extra[c * cap + __float_as_int(sp[bpos].w)];
+      xgrids[(int64_t)c * 2 * nq + q] = e.x;
+      xgrids[(int64_t)c * 2 * nq + nq + q] = e.y;
+    }
   }
 }
 
@@ -560,25 +567,27 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
 
 template <int L>
 static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
-                        float max_sq, const CellBuffers& cb, float* grids, hipStream_t s) {
+                        float max_sq, const CellBuffers& cb, float* grids, const float2* extra,
+                        float* xgrids, hipStream_t s) {
   const int nq = n * n, qpb = 256 / L;
   dim3 gq((unsigned)((nq + qpb - 1) / qpb), nclouds);
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.sorted_val, cap, grids);
+                     cb.sorted_val, cap, grids, extra, xgrids);
 }
 
 int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
 
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
-                  float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s) {
+                  float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
+                  const float2* extra, float* xgrids) {
   if (lanes == 0) lanes = nn_auto_lanes(nclouds);
   switch (lanes) {
-    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
-    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
-    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
-    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
-    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, s); break;
+    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
+    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
+    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
+    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
+    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
   }
 }
 

@@ -53,12 +53,15 @@ __device__ __forceinline__ int histc_bin(float x, const float* E, int N, float i
 // lanes are summed with a segmented wave reduction and only the run's first
 // lane touches memory (one atomic per run instead of per point).  The same
 // lanes set (or reset) the occupancy bytes of the voxel's row and z plane.
-template <bool kClear>
+// kExtra: the clouds also carry (reflectivity, ambient) per point, summed
+// into sumF / sumA like the intensity (phase-aligner.cc:113-118).
+template <bool kClear, bool kExtra>
 __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __restrict__ npts,
                                 int64_t cap, const DevPose* __restrict__ pose, int pose_stride,
                                 int N, const float* __restrict__ edges, float* __restrict__ sumI,
                                 float* __restrict__ sumR, uint32_t* __restrict__ cnt,
-                                VoxelOcc occ) {
+                                VoxelOcc occ, const float2* __restrict__ extra,
+                                float* __restrict__ sumF, float* __restrict__ sumA) {
 #pragma clang fp contract(off)
   extern __shared__ float E[];
   for (int i = threadIdx.x; i < N; i += blockDim.x) E[i] = edges[i];
@@ -70,7 +73,7 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
   // every lane of the wave takes part in the reduction; lanes without a
   // voxel carry a key no other lane has
   int64_t key = -1 - (int64_t)lane;
-  float vi = 0.f, vr = 0.f;
+  float vi = 0.f, vr = 0.f, vf = 0.f, va = 0.f;
   if (i < npts[c]) {
     float4 p = pts[c * cap + i];
   if (pose) {
@@ -92,6 +95,11 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
       key = (int64_t)c * NN * NN * NN + lin;
       vi = p.w;
       vr = sqrtf(p.x * p.x + p.y * p.y + p.z * p.z);
+      if (kExtra) {
+        const float2 e = extra[c * cap + i];
+        vf = e.x;
+        va = e.y;
+      }
     }
   }
   // runs of equal keys in adjacent lanes: head flags, segment ids, then a
@@ -106,10 +114,19 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
     const float oi = __shfl_down(vi, d, 64), orr = __shfl_down(vr, d, 64),
                 on = __shfl_down(n, d, 64);
     const int os = __shfl_down(seg, d, 64);
+    float of = 0.f, oa = 0.f;
+    if (kExtra) {
+      of = __shfl_down(vf, d, 64);
+      oa = __shfl_down(va, d, 64);
+    }
     if (!kClear && lane + d < 64 && os == seg) {
       vi += oi;
       vr += orr;
       n += on;
+      if (kExtra) {
+        vf += of;
+        va += oa;
+      }
     }
   }
   if (!head || key < 0) return;
@@ -121,11 +138,19 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
     sumI[off] = 0.f;
     sumR[off] = 0.f;
     cnt[off] = 0u;
+    if (kExtra) {
+      sumF[off] = 0.f;
+      sumA[off] = 0.f;
+    }
     return;
   }
   atomicAdd(&sumI[off], vi);
   atomicAdd(&sumR[off], vr);
   atomicAdd(&cnt[off], (uint32_t)n);
+  if (kExtra) {
+    atomicAdd(&sumF[off], vf);
+    atomicAdd(&sumA[off], va);
+  }
 }
 
 __global__ void k_reduce_keys(const unsigned long long* __restrict__ partials, int count,
@@ -506,20 +531,36 @@ void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_f4, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
 }
 
+template <bool kClear>
+static void voxel_pass(const float4* pts, const int* npts, int64_t cap, int nclouds,
+                       const DevPose* pose, int pose_stride, int N, const float* edges,
+                       float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, VoxelExtra ex,
+                       hipStream_t s) {
+  dim3 g((unsigned)((cap + 255) / 256), nclouds);
+  if (ex.extra)
+    hipLaunchKernelGGL((k_voxel_scatter<kClear, true>), g, dim3(256), N * sizeof(float), s, pts,
+                       npts, cap, pose, pose_stride, N, edges, sumI, sumR, cnt, occ, ex.extra,
+                       ex.sumF, ex.sumA);
+  else
+    hipLaunchKernelGGL((k_voxel_scatter<kClear, false>), g, dim3(256), N * sizeof(float), s, pts,
+                       npts, cap, pose, pose_stride, N, edges, sumI, sumR, cnt, occ,
+                       (const float2*)nullptr, (float*)nullptr, (float*)nullptr);
+}
+
 void launch_voxel_scatter(const float4* pts, const int* npts, int64_t cap, int nclouds,
                           const DevPose* pose, int pose_stride, int N, const float* edges,
-                          float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s) {
-  dim3 g((unsigned)((cap + 255) / 256), nclouds);
-  hipLaunchKernelGGL(k_voxel_scatter<false>, g, dim3(256), N * sizeof(float), s, pts, npts, cap,
-                     pose, pose_stride, N, edges, sumI, sumR, cnt, occ);
+                          float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s,
+                          VoxelExtra ex) {
+  voxel_pass<false>(pts, npts, cap, nclouds, pose, pose_stride, N, edges, sumI, sumR, cnt, occ,
+                    ex, s);
 }
 
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
                         const DevPose* pose, int pose_stride, int N, const float* edges,
-                        float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s) {
-  dim3 g((unsigned)((cap + 255) / 256), nclouds);
-  hipLaunchKernelGGL(k_voxel_scatter<true>, g, dim3(256), N * sizeof(float), s, pts, npts, cap,
-                     pose, pose_stride, N, edges, sumI, sumR, cnt, occ);
+                        float* sumI, float* sumR, uint32_t* cnt, VoxelOcc occ, hipStream_t s,
+                        VoxelExtra ex) {
+  voxel_pass<true>(pts, npts, cap, nclouds, pose, pose_stride, N, edges, sumI, sumR, cnt, occ,
+                   ex, s);
 }
 
 void launch_reduce_keys(const unsigned long long* partials, int count, int npairs,

@@ -1,16 +1,25 @@
 // host_driver.cc -- registers two point clouds through the C++ mirror of the
 // reference API (phaser_core::CloudController("sph-opt"), the path of
 // phaser_core/test/driver.cc:30-57) and prints the result as JSON.
-// usage: host_driver target.bin source.bin bandwidth n_voxels
-// where a .bin file is float32 (x, y, z, intensity) quadruples.
+// usage: host_driver target source bandwidth n_voxels [bingham gaussian]
+// where a .bin file is float32 (x, y, z, intensity) quadruples and a .ply
+// file is read by model::PointCloud::readFromFile (with reflectivity and
+// ambient when it has them).
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <vector>
 
 #include "phaser_core.hpp"
 
 static model::PointCloudPtr load(const char* path) {
+  const size_t len = std::strlen(path);
+  if (len > 4 && std::strcmp(path + len - 4, ".ply") == 0) {
+    auto c = std::make_shared<model::PointCloud>();
+    c->readFromFile(path);
+    return c;
+  }
   std::ifstream f(path, std::ios::binary | std::ios::ate);
   if (!f) {
     std::fprintf(stderr, "cannot open %s\n", path);

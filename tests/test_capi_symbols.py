@@ -29,6 +29,16 @@ def test_library_exports_every_declared_symbol():
         assert n in capi.EXPORTS, f"{n} has no ctypes binding"
 
 
+def test_ply_reader_exports_every_declared_symbol():
+    from phaser_amd import ply
+    L = ply.lib()
+    names = _declared("phaser_ply.h")
+    assert len(names) == 4
+    for n in names:
+        assert hasattr(L, n), n
+        assert n in ply.EXPORTS, f"{n} has no ctypes binding"
+
+
 def test_synth_library_exports():
     from phaser_amd import synth
     L = synth._load()
@@ -38,7 +48,7 @@ def test_synth_library_exports():
 
 def test_struct_layouts():
     from phaser_amd import capi
-    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 5 + 4  # padded to 8 bytes
+    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 6
     # rot/trans index + values (16) + zyz,quat,euler (80) + T (64) + translation (24)
     # + trans_cov (72) + rot_omega (32) + rot_axes (128)
     assert C.sizeof(capi.Result) == 16 + 80 + 64 + 24 + 72 + 32 + 128
@@ -75,10 +85,12 @@ _LAYOUT_C = r"""
 #include <stddef.h>
 #include "phaser_amd.h"
 #include "phaser_serve.h"
+#include "phaser_ply.h"
 #define S(t) printf(#t " %zu\n", sizeof(t))
 #define O(t, f) printf(#t "." #f " %zu\n", offsetof(t, f))
 int main(void) {
   S(phz_config); S(phz_result); S(phz_stage_time); S(phz_tuning);
+  S(phz_ply_fields); S(phz_ply_cloud); O(phz_ply_cloud, range); O(phz_ply_cloud, has_range);
   O(phz_tuning, plane_fwd); O(phz_tuning, nn_lanes); O(phz_tuning, fused_reg);
   O(phz_tuning, so3_reg); O(phz_tuning, occ_fwd); O(phz_tuning, inv_plane);
   O(phz_tuning, ms_fwd_x); O(phz_tuning, ms_plane);
@@ -88,6 +100,7 @@ int main(void) {
   O(phz_result, T_rot); O(phz_result, translation); O(phz_result, trans_cov);
   O(phz_result, rot_omega); O(phz_result, rot_axes);
   O(phz_config, bingham_peak_neighbors); O(phz_config, gaussian_peak_neighbors);
+  O(phz_config, channels);
   O(phs_config, bingham_peak_neighbors); O(phs_config, gaussian_peak_neighbors);
   O(phs_config, max_points); O(phs_config, hbm_budget); O(phs_config, batch_window_ms);
   O(phs_config, precision); O(phs_config, graphs); O(phz_config, precision);
@@ -102,7 +115,7 @@ int main(void) {
 def test_ctypes_layouts_match_the_c_headers(tmp_path):
     """Sizes and field offsets of every ctypes structure against the C
     compiler's view of include/*.h."""
-    from phaser_amd import capi, serve
+    from phaser_amd import capi, ply, serve
     src = tmp_path / "layout.c"
     src.write_text(_LAYOUT_C)
     exe = tmp_path / "layout"
@@ -114,7 +127,7 @@ def test_ctypes_layouts_match_the_c_headers(tmp_path):
              "phz_stage_time": capi.StageTime, "phz_tuning": capi.Tuning,
              "phs_config": serve.ServeConfig,
              "phs_request": serve.Request, "phs_response": serve.Response,
-             "phs_stats": serve.Stats}
+             "phs_stats": serve.Stats, "phz_ply_fields": ply.Fields, "phz_ply_cloud": ply.Cloud}
     for key, val in got.items():
         if "." in key:
             t, f = key.split(".")
