@@ -35,7 +35,7 @@ WORKLOAD = dict(rings=128, azimuths=1024, noise_sigma=0.02, overlap=0.7)
 
 
 def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False,
-                 lines_fwd=False, lines_inv=False):
+                 lines_fwd=False, lines_inv=False, nn_lanes=4):
     """Algorithmic (compulsory) HBM bytes of ONE launch of the kernel that
     dominates each stage, for a batch of `pairs` registrations: inputs read
     once, outputs written once, FP32 / complex64 elements, C = 2 populated
@@ -48,7 +48,7 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False
     clouds = nt + pairs
     return {
         # sorted points + values read once, grid queries once, C grids written
-        "s2_nn": ("k_nn", clouds * (24 * P_mean + 4 * C * n * n) + 16 * n * n),
+        "s2_nn": (f"k_nn<{nn_lanes}>", clouds * (24 * P_mean + 4 * C * n * n) + 16 * n * n),
         # Wigner table once per batch + fused coefficients in, S half-plane out
         "so3_synth": ("k_so3_synth", 4 * n * B * (B + 1) * (B + 2) // 6
                       + clouds * 8 * B * B + pairs * 8 * n * B * (2 * B - 1)),
@@ -94,7 +94,7 @@ def concurrent_throughput(args, pairs, max_pts, tuning):
         c = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=len(idx),
                     precision=args.precision)
         c.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
-                     t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"])
+                     t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"])
         if args.graphs:
             c.set_graphs(True)
         ds = [c.device_array(pairs[i][0]) for i in idx]
@@ -235,6 +235,8 @@ def main():
                     help="lanes per query of the S2 nearest-neighbour search (0: default)")
     ap.add_argument("--streams", type=int, default=2,
                     help="split each batch over this many contexts/streams driven concurrently")
+    ap.add_argument("--nn-xcd", type=int, default=0, choices=[-1, 0, 1],
+                    help="XCD-ordered NN workgroups: 1 on, -1 off, 0 library default")
     ap.add_argument("--inv-plane", type=int, default=0, choices=[-1, 0, 1],
                     help="inverse x + c2r as one pass over z planes in LDS (0: default, on)")
     ap.add_argument("--occ-fwd", type=int, default=0, choices=[-1, 0, 1],
@@ -270,11 +272,12 @@ def main():
     tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
     if args.graphs:
         ctx.set_graphs(True)
-    if args.nn_lanes or args.occ_fwd or args.inv_plane:
+    if args.nn_lanes or args.occ_fwd or args.inv_plane or args.nn_xcd:
         t = ctx.tuning()
         ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"],
                        args.nn_lanes or t["nn_lanes"], t["fused_reg"], t["so3_reg"],
-                       args.occ_fwd or t["occ_fwd"], args.inv_plane or t["inv_plane"])
+                       args.occ_fwd or t["occ_fwd"], args.inv_plane or t["inv_plane"],
+                       args.nn_xcd or t["nn_xcd"])
         tuning = ctx.tuning()
     d_src = [ctx.device_array(s) for s, _, _, _ in pairs]
     d_tgt = [ctx.device_array(t) for _, t, _, _ in pairs]
@@ -317,8 +320,12 @@ def main():
     st = ctx.stage_times()
     ctx.set_profiling(False)
     plane = tuning["plane_fwd"] == 1
+    # lanes per NN query: the tuned value, or the library's choice by clouds
+    # per launch (2 for 64 clouds or more, else 4)
+    lanes = tuning["nn_lanes"] or (2 if 2 * P >= 64 else 4)
     kb = kernel_bytes(B, N, P_mean, P, plane=plane, fused_reg=tuning["fused_reg"] == 1,
-                      lines_fwd=tuning["fwd_z"] == 1, lines_inv=tuning["inv_x"] == 1)
+                      lines_fwd=tuning["fwd_z"] == 1, lines_inv=tuning["inv_x"] == 1,
+                      nn_lanes=lanes)
     dom = max((k for k in kb if st.get(k, (0, 0))[1] > 0), key=lambda k: st[k][0])
     per_call = dict(LAUNCHES, spatial_fft_fwd=1 if plane else 2)
     launches = st[dom][1] * per_call.get(dom, 1)

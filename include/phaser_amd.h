@@ -205,7 +205,9 @@ typedef struct {
   int32_t inv_plane;   /* with fused_reg and the register engine for the inverse x
                           pass (N = 64, 128): inverse x, c2r and argmax in one pass
                           over whole z planes in LDS: 0 or 1 = on (default), -1 = off */
-  int32_t reserved;
+  int32_t nn_xcd;      /* S^2 nearest-neighbour workgroups ordered so that each of the
+                          8 XCDs works through its own clouds (launches of a multiple
+                          of 8 clouds): 0 = default, 1 = on, -1 = off */
   float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
   float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
   float ms_fused;      /* the three fused passes at max_batch, chosen form */

@@ -43,9 +43,11 @@ int nn_auto_lanes(int nclouds);
 // extra (nullable): per-point (reflectivity, ambient) of each cloud slot
 // (cloud * cap + original index); xgrids then receives nclouds x 2 x n*n
 // grids of the nearest point's values (zero where none is accepted).
+// xcd: order the workgroups so that each XCD works through its own clouds
+// (needs a multiple of 8 clouds; otherwise the plain 2-D grid).
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
-                  const float2* extra = nullptr, float* xgrids = nullptr);
+                  const float2* extra = nullptr, float* xgrids = nullptr, bool xcd = false);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B

@@ -269,11 +269,24 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
-                     float* __restrict__ xgrids) {
+                     float* __restrict__ xgrids, int xcd_blocks) {
   constexpr int kBatch = 4;  // point loads in flight per lane
-  const int c = blockIdx.y;
+  // xcd_blocks > 0: a 1-D grid of xcd_blocks query blocks per cloud, where
+  // consecutive workgroups go to the 8 XCDs in turn: XCD x takes clouds
+  // x, x + 8, ... one after the other, so the clouds whose points an XCD's
+  // L2 holds are the few it is working on (instead of every cloud on every
+  // XCD).  0: blockIdx.y is the cloud.
+  int c, qb;
+  if (xcd_blocks > 0) {
+    const int b = blockIdx.x, k = b >> 3, j = k / xcd_blocks;
+    c = (b & 7) + 8 * j;
+    qb = k - j * xcd_blocks;
+  } else {
+    c = blockIdx.y;
+    qb = blockIdx.x;
+  }
   const int sub = threadIdx.x & (LANES - 1);
-  const int q = blockIdx.x * (blockDim.x / LANES) + threadIdx.x / LANES;
+  const int q = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
   // min sin(theta) over each band and the cloud's band table (nb <= 256),
   // once per workgroup
@@ -568,26 +581,29 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
 template <int L>
 static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                         float max_sq, const CellBuffers& cb, float* grids, const float2* extra,
-                        float* xgrids, hipStream_t s) {
+                        float* xgrids, bool xcd, hipStream_t s) {
   const int nq = n * n, qpb = 256 / L;
-  dim3 gq((unsigned)((nq + qpb - 1) / qpb), nclouds);
+  const int nqb = (nq + qpb - 1) / qpb;
+  // the XCD-ordered grid needs whole groups of 8 clouds
+  const bool by_xcd = xcd && nclouds % 8 == 0;
+  const dim3 gq = by_xcd ? dim3((unsigned)(nqb * nclouds)) : dim3((unsigned)nqb, nclouds);
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.sorted_val, cap, grids, extra, xgrids);
+                     cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0);
 }
 
 int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
 
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
-                  const float2* extra, float* xgrids) {
+                  const float2* extra, float* xgrids, bool xcd) {
   if (lanes == 0) lanes = nn_auto_lanes(nclouds);
   switch (lanes) {
-    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
-    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
-    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
-    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
-    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, s); break;
+    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
+    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
+    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
+    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
+    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
   }
 }
 

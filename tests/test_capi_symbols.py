@@ -92,7 +92,7 @@ int main(void) {
   S(phz_config); S(phz_result); S(phz_stage_time); S(phz_tuning);
   S(phz_ply_fields); S(phz_ply_cloud); O(phz_ply_cloud, range); O(phz_ply_cloud, has_range);
   O(phz_tuning, plane_fwd); O(phz_tuning, nn_lanes); O(phz_tuning, fused_reg);
-  O(phz_tuning, so3_reg); O(phz_tuning, occ_fwd); O(phz_tuning, inv_plane);
+  O(phz_tuning, so3_reg); O(phz_tuning, occ_fwd); O(phz_tuning, inv_plane); O(phz_tuning, nn_xcd);
   O(phz_tuning, ms_fwd_x); O(phz_tuning, ms_plane);
   O(phz_tuning, ms_fused); O(phz_tuning, ms_so3);
   S(phs_config); S(phs_request); S(phs_response); S(phs_stats);

@@ -464,6 +464,23 @@ def test_nn_lanes_give_identical_samples(lanes):
         assert r.shape[0] == 2
 
 
+def test_nn_xcd_order_gives_identical_results():
+    """The XCD-ordered NN grid (16 clouds per launch, a multiple of 8)
+    reorders workgroups only: every pair's rotation peak is bit-identical."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=64, azimuths=1024)) for s in range(40, 48)]
+    with Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=8) as ctx:
+        got = {}
+        for mode in (-1, 1):
+            ctx.set_tuning(nn_xcd=mode)
+            assert ctx.tuning()["nn_xcd"] == mode
+            print(f"nn_xcd {mode}: 8 pairs", flush=True)
+            got[mode] = ctx.register_pairs([t for _, t, _, _ in pairs], [s for s, _, _, _ in pairs])
+    for a, b in zip(got[-1], got[1]):
+        assert a["rot_index"] == b["rot_index"] and a["rot_value"] == b["rot_value"]
+        assert a["trans_index"] == b["trans_index"]
+
+
 def test_register_fused_passes_match_lds_passes(ctx64):
     """r2c, cross-spectrum + z and c2r on the register four-step kernels
     against the LDS Stockham kernels: same bins, grids within FP32 rounding."""
