@@ -191,9 +191,10 @@ using VectorXd = std::vector<double>;  // stands in for Eigen::VectorXd
 
 // One context per (bandwidth, voxel count, capacity); shared by the objects
 // below and released with the last of them.
-// channels: 2, or 4 for clouds that carry reflectivity and ambient.
+// channels: 2, or 4 for clouds that carry reflectivity and ambient;
+// sph_pad: spherical zero padding of the SO(3) correlation.
 std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points,
-                                        int channels = 2);
+                                        int channels = 2, int sph_pad = 0);
 
 class SphericalCorrelation {
  public:

@@ -22,7 +22,7 @@ u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 class Config(C.Structure):
     _fields_ = [("bw", C.c_int), ("n_voxels", C.c_int), ("lower", C.c_int),
                 ("upper", C.c_int), ("lp_lower", C.c_int), ("lp_upper", C.c_int),
-                ("use_kdtree", C.c_int)]
+                ("use_kdtree", C.c_int), ("sph_pad", C.c_int)]
 
 
 class Result(C.Structure):
@@ -465,15 +465,16 @@ def sub2ind(i, j, k, rows, cols):
 
 def register(target, source, bw=32, n_voxels=64, lower=-50, upper=50,
              kdtree=True, want_grids=False, want_reg=False, target_extra=None,
-             source_extra=None):
+             source_extra=None, sph_pad=0):
     """target_extra / source_extra: (P, 2) reflectivity and ambient per
-    point, or None (zero channels)."""
-    cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree))
+    point, or None (zero channels).  sph_pad: spherical zero padding (the
+    SO(3) grid at bw + sph_pad)."""
+    cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree), int(sph_pad))
     t = _c(target, np.float32)
     s = _c(source, np.float32)
     te, se = _extra(target_extra, t.shape[0]), _extra(source_extra, s.shape[0])
     res = Result()
-    rot = np.zeros((2 * bw) ** 3) if want_grids else None
+    rot = np.zeros((2 * (bw + sph_pad)) ** 3) if want_grids else None
     tr = np.zeros(n_voxels ** 3) if want_grids else None
     reg = np.zeros_like(s) if want_reg else None
     lib().po_register_ex(C.byref(cfg), t, _ptr(te), t.shape[0], s, _ptr(se), s.shape[0],

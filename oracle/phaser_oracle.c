@@ -802,6 +802,12 @@ void po_so3_correlate(int bw, const double* f_re, const double* f_im,
 void po_spherical_correlate_laplace(int bw, int C, const double* f,
                                     const double* h, double* mag,
                                     double* f_fused, double* h_fused) {
+  po_spherical_correlate_laplace_pad(bw, bw, C, f, h, mag, f_fused, h_fused);
+}
+
+void po_spherical_correlate_laplace_pad(int bw, int bw_out, int C, const double* f,
+                                        const double* h, double* mag,
+                                        double* f_fused, double* h_fused) {
   const int n = 2 * bw;
   const size_t nc = (size_t)bw * bw;
   double* fch = (double*)xcalloc(2 * nc * C, sizeof(double));
@@ -849,7 +855,24 @@ void po_spherical_correlate_laplace(int bw, int C, const double* f,
       h_fused[2 * i] = hr[i];
       h_fused[2 * i + 1] = hi[i];
     }
-  po_so3_correlate(bw, fr, fi, hr, hi, NULL, mag);
+  if (bw_out == bw) {
+    po_so3_correlate(bw, fr, fi, hr, hi, NULL, mag);
+  } else {
+    /* so3CombineCoef_fftw(bw, bw_out, bw - 1, ...) then the inverse SO(3)
+     * transform at bw_out (spherical-correlation.cc:248-278): the same as
+     * the correlation at bw_out of coefficients that are zero for l >= bw */
+    const size_t no = (size_t)bw_out * bw_out;
+    double* p[4];
+    const double* src[4] = {fr, fi, hr, hi};
+    for (int k = 0; k < 4; ++k) {
+      p[k] = (double*)xcalloc(no, sizeof(double));
+      for (int l = 0; l < bw; ++l)
+        for (int m = -l; m <= l; ++m)
+          p[k][po_seanindex(m, l, bw_out)] = src[k][po_seanindex(m, l, bw)];
+    }
+    po_so3_correlate(bw_out, p[0], p[1], p[2], p[3], NULL, mag);
+    for (int k = 0; k < 4; ++k) free(p[k]);
+  }
   free(fch);
   free(hch);
   free(re);
@@ -1214,7 +1237,7 @@ void po_register_ex(const po_config* cfg, const float* target, const float* targ
                     int64_t Pt, const float* source, const float* source_extra, int64_t Ps,
                     po_result* res, float* reg_out, double* rot_corr, double* trans_corr) {
   const int bw = cfg->bw, n = 2 * bw, N = cfg->n_voxels;
-  const size_t nn = (size_t)n * n, n3 = (size_t)n * n * n;
+  const size_t nn = (size_t)n * n;
   const size_t T3 = (size_t)N * N * N;
   memset(res, 0, sizeof(*res));
   const float max_sq =
@@ -1225,11 +1248,14 @@ void po_register_ex(const po_config* cfg, const float* target, const float* targ
   po_sample_cloud_ex(bw, target, target_extra, Pt, max_sq, cfg->use_kdtree, fg);
   po_sample_cloud_ex(bw, source, source_extra, Ps, max_sq, cfg->use_kdtree, hg);
   double t1 = now_ms();
-  double* mag = rot_corr ? rot_corr : (double*)xcalloc(n3, sizeof(double));
-  po_spherical_correlate_laplace(bw, 4, fg, hg, mag, NULL, NULL);
-  res->rot_index = po_argmax(mag, n3);
+  /* the SO(3) grid at bw_out = bw + spherical zero padding */
+  const int bw_out = bw + cfg->sph_pad;
+  const size_t no3 = (size_t)8 * bw_out * bw_out * bw_out;
+  double* mag = rot_corr ? rot_corr : (double*)xcalloc(no3, sizeof(double));
+  po_spherical_correlate_laplace_pad(bw, bw_out, 4, fg, hg, mag, NULL, NULL);
+  res->rot_index = po_argmax(mag, no3);
   res->rot_value = mag[res->rot_index];
-  po_zyz_from_index(res->rot_index, bw, res->zyz);
+  po_zyz_from_index(res->rot_index, bw_out, res->zyz);
   po_quat_from_zyz(res->zyz, res->quat);
   po_xyz_from_quat(res->quat, res->euler_xyz);
   po_transform_xyz((float)res->euler_xyz[0], (float)res->euler_xyz[1],

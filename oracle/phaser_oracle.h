@@ -171,6 +171,7 @@ typedef struct {
   int lp_lower;      /* phaser_core_spatial_low_pass_lower_bound (NN radius) */
   int lp_upper;      /* phaser_core_spatial_low_pass_upper_bound */
   int use_kdtree;    /* NN route (results identical) */
+  int sph_pad;       /* phaser_core_spherical_zero_padding: SO(3) grid at bw + sph_pad */
 } po_config;
 
 typedef struct {
@@ -185,6 +186,13 @@ typedef struct {
   double translation[3];
   double t_sample_ms, t_rot_ms, t_trans_ms;
 } po_result;
+
+/* SphericalCorrelationLaplace with zero padding (spherical-correlation.cc:
+ * 21-27, 248-278): fused coefficients at bw, correlation on the SO(3) grid
+ * of bw_out >= bw (mag: (2 bw_out)^3). */
+void po_spherical_correlate_laplace_pad(int bw, int bw_out, int C, const double* f,
+                                        const double* h, double* mag,
+                                        double* f_fused, double* h_fused);
 
 /* target = cloud_prev (signal f), source = cloud_cur (pattern h).
  * reg_out (optional): registered source, P_s x 4.  rot_corr: (2bw)^3,

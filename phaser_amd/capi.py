@@ -191,7 +191,8 @@ class Context:
         _check(lib().phz_create(C.byref(self.cfg), C.byref(h)))
         self.h = h
         self.B = self.cfg.bandwidth
-        self.n = 2 * self.B
+        self.n = 2 * self.B  # S^2 grid
+        self.no = 2 * (self.B + self.cfg.spherical_zero_padding)  # SO(3) grid
         self.N = self.cfg.n_voxels
 
     def close(self):
@@ -222,7 +223,7 @@ class Context:
         for e, cl in ((te, t), (se, s)):
             assert e is None or e.shape == (cl.shape[0], 2)
         res = Result()
-        rc = np.zeros(self.n ** 3, np.float32) if grids else None
+        rc = np.zeros(self.no ** 3, np.float32) if grids else None
         tc = np.zeros(self.N ** 3, np.float32) if grids else None
         reg = np.zeros_like(s) if registered else None
         _check(lib().phz_register_channels(self.h, _p(t), _p(te), t.shape[0], _p(s), _p(se),

@@ -44,12 +44,15 @@ constexpr bool kNnXcdDefault = true;
 struct phz_ctx {
   phz_config cfg{};
   int B = 0, n = 0, N = 0, h = 0;
+  // SO(3) correlation bandwidth and grid: Bo = B + spherical_zero_padding
+  // (spherical-correlation.cc:21-27), no = 2 Bo
+  int Bo = 0, no = 0;
   int64_t H = 0, T3 = 0, cap = 0;
   int max_clouds = 0;
   hipStream_t stream = nullptr;
   S2Tables s2;
   VoxelTables vox;
-  FftPlan plan_n{}, plan_N{};
+  FftPlan plan_n{}, plan_N{}, plan_no{};
   SpatialTiles tiles{};
   bool plane_fwd = false;  // whole-plane first forward pass
   bool so3_reg = false;    // SO(3) alpha/gamma passes on the register engine
@@ -67,6 +70,9 @@ struct phz_ctx {
   int64_t* d_wig_off = nullptr;
   float* d_wig_norm = nullptr;
   float2* d_tw_n = nullptr;
+  float2* d_tw_no = nullptr;     // exp(-2 pi i e / no) (d_tw_n when unpadded)
+  int* d_pad_map = nullptr;      // Bo-layout coefficient -> B-layout index or -1
+  float2* d_fused_o = nullptr;   // fused coefficients in the Bo layout (d_fused when unpadded)
   float* d_edges = nullptr;
   float2* d_tw_N = nullptr;
   // workspaces
@@ -211,11 +217,12 @@ int upload(phz_ctx* c, T** p, const std::vector<T>& v) {
 // same list of buffers; init_ctx checks that the two agree).
 size_t context_bytes(const phz_config& cf) {
   const int64_t B = cf.bandwidth, n = 2 * B, N = cf.n_voxels;
+  const int64_t pad = cf.spherical_zero_padding, Bo = B + pad, no = 2 * Bo;
   const int64_t nq = n * n, T3 = N * N * N, H = N * N * (N / 2 + 1);
   const int64_t P = cf.max_batch, M = 2 * P, cap = cf.max_points;
   const int64_t G = std::max<int64_t>((cf.channels == 4 ? 4 : 2) * M, 16);
   const int64_t MV = std::max<int64_t>(M, 16);
-  const int64_t ppp = std::max(so3_gamma_blocks((int)B), spatial_c2r_blocks((int)N));
+  const int64_t ppp = std::max(so3_gamma_blocks((int)Bo), spatial_c2r_blocks((int)N));
   size_t s = 0;
   auto add = [&](int64_t count, size_t elem) { s += elem * (size_t)(count ? count : 1); };
   // tables
@@ -223,12 +230,17 @@ size_t context_bytes(const phz_config& cf) {
   add(n, sizeof(float));
   add(n * B * (B + 1) / 2, sizeof(float));
   add(B + 1, sizeof(int));
-  add(n * B * (B + 1) * (B + 2) / 6, sizeof(float));
-  add(B * (B + 1) / 2 + 1, sizeof(int64_t));
-  add(B, sizeof(float));
+  add(no * Bo * (Bo + 1) * (Bo + 2) / 6, sizeof(float));
+  add(Bo * (Bo + 1) / 2 + 1, sizeof(int64_t));
+  add(Bo, sizeof(float));
   add(n, sizeof(float2));
   add(N, sizeof(float2));
   add(N, sizeof(float));
+  if (pad > 0) {
+    add(no, sizeof(float2));
+    add(Bo * Bo, sizeof(int));
+    add(std::max<int64_t>(M, 2) * Bo * Bo, sizeof(float2));
+  }
   // clouds and the S^2 search grid
   add(M * cap, sizeof(float4));
   add(M, sizeof(int));
@@ -247,9 +259,9 @@ size_t context_bytes(const phz_config& cf) {
   add(G * B * n, sizeof(float2));
   add(G * B * B, sizeof(float2));
   add(std::max<int64_t>(M, 2) * B * B, sizeof(float2));
-  add(P * B * (2 * B - 1) * n, sizeof(float2));
-  add(P * n * B * n, sizeof(float2));
-  add(nq * n, sizeof(float));
+  add(P * Bo * (2 * Bo - 1) * no, sizeof(float2));
+  add(P * no * Bo * no, sizeof(float2));
+  add(no * no * no, sizeof(float));
   add(2 * P, sizeof(unsigned long long));
   add(P * ppp, sizeof(unsigned long long));
   add(P, sizeof(DevPose));
@@ -263,7 +275,7 @@ size_t context_bytes(const phz_config& cf) {
   add(P * H, sizeof(float2));
   add(T3, sizeof(float));
   add(cap, sizeof(float4));
-  if (cf.bingham_peak_neighbors > 0) add(P * n * nq, sizeof(float));
+  if (cf.bingham_peak_neighbors > 0) add(P * no * no * no, sizeof(float));
   if (cf.gaussian_peak_neighbors > 0) add(P * T3, sizeof(float));
   if (cf.channels == 4) {
     add(M * cap, sizeof(float2));
@@ -276,7 +288,8 @@ size_t context_bytes(const phz_config& cf) {
 
 int check_config(const phz_config& cf) {
   PHZ_CHECK(cf.bandwidth >= 2 && cf.bandwidth <= 128, "bandwidth must be in [2, 128]");
-  PHZ_CHECK(cf.spherical_zero_padding == 0, "spherical zero padding other than 0 is not supported");
+  PHZ_CHECK(cf.spherical_zero_padding >= 0 && cf.bandwidth + cf.spherical_zero_padding <= 128,
+            "spherical zero padding must be >= 0 with bandwidth + padding <= 128");
   PHZ_CHECK(cf.spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
   PHZ_CHECK(cf.n_voxels >= 2 && cf.n_voxels <= 256, "n_voxels must be in [2, 256]");
   PHZ_CHECK(cf.discretize_lower < cf.discretize_upper, "discretize bounds must be increasing");
@@ -290,6 +303,8 @@ int check_config(const phz_config& cf) {
   PHZ_CHECK(cf.channels == 0 || cf.channels == 2 || cf.channels == 4,
             "channels must be 2 or 4 (0 means 2)");
   PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
+  PHZ_CHECK(make_plan(2 * (cf.bandwidth + cf.spherical_zero_padding)).nst > 0,
+            "2*(bandwidth + spherical zero padding) must factor into 2, 3 and 5");
   PHZ_CHECK(make_plan(cf.n_voxels).nst > 0, "n_voxels must factor into 2, 3 and 5");
   return 0;
 }
@@ -299,6 +314,8 @@ int init_ctx(phz_ctx* c) {
   TRY(check_config(cf));
   c->B = cf.bandwidth;
   c->n = 2 * c->B;
+  c->Bo = c->B + cf.spherical_zero_padding;
+  c->no = 2 * c->Bo;
   c->N = cf.n_voxels;
   c->h = c->N / 2 + 1;
   c->H = (int64_t)c->N * c->N * c->h;
@@ -306,6 +323,7 @@ int init_ctx(phz_ctx* c) {
   c->cap = cf.max_points;
   c->max_clouds = 2 * cf.max_batch;
   c->plan_n = make_plan(c->n);
+  c->plan_no = make_plan(c->no);
   c->plan_N = make_plan(c->N);
   {
     const float v = (float)(std::abs(cf.low_pass_lower) + std::abs(cf.low_pass_upper)) /
@@ -327,14 +345,33 @@ int init_ctx(phz_ctx* c) {
   TRY(upload(c, &c->d_theta, c->s2.theta));
   TRY(upload(c, &c->d_leg, c->s2.legendre));
   TRY(upload(c, &c->d_leg_off, c->s2.leg_off));
-  TRY(upload(c, &c->d_wig, c->s2.wigner));
-  TRY(upload(c, &c->d_wig_off, c->s2.wig_off));
-  TRY(upload(c, &c->d_wig_norm, c->s2.wig_norm));
+  // the SO(3) tables at Bo (with padding: the Wigner-d blocks and beta grid
+  // of the larger bandwidth; the S^2 tables stay at B)
+  S2Tables so3_pad;
+  if (c->Bo != c->B) build_s2_tables(c->Bo, &so3_pad);
+  const S2Tables& so3t = c->Bo != c->B ? so3_pad : c->s2;
+  TRY(upload(c, &c->d_wig, so3t.wigner));
+  TRY(upload(c, &c->d_wig_off, so3t.wig_off));
+  TRY(upload(c, &c->d_wig_norm, so3t.wig_norm));
   {
     std::vector<float2> t(c->n), u(c->N);
     std::memcpy(t.data(), c->s2.tw_n.data(), sizeof(float2) * c->n);
     std::memcpy(u.data(), c->vox.tw_N.data(), sizeof(float2) * c->N);
     TRY(upload(c, &c->d_tw_n, t));
+    if (c->Bo != c->B) {
+      std::vector<float2> to(c->no);
+      std::memcpy(to.data(), so3t.tw_n.data(), sizeof(float2) * c->no);
+      TRY(upload(c, &c->d_tw_no, to));
+      // Bo-layout index j of (m, l) -> the B-layout index, or -1 for l >= B
+      // (so3CombineCoef_fftw with degLim = B - 1, so3_correlate_fftw.c:102-118)
+      std::vector<int> map((size_t)c->Bo * c->Bo, -1);
+      for (int l = 0; l < c->Bo; ++l)
+        for (int m = -l; m <= l; ++m)
+          map[seanindex(m, l, c->Bo)] = l < c->B ? seanindex(m, l, c->B) : -1;
+      TRY(upload(c, &c->d_pad_map, map));
+    } else {
+      c->d_tw_no = c->d_tw_n;
+    }
     TRY(upload(c, &c->d_tw_N, u));
   }
   TRY(upload(c, &c->d_edges, c->vox.edges));
@@ -366,11 +403,16 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_F, (size_t)G * c->B * c->n));
   TRY(dalloc(c, &c->d_coef, (size_t)G * c->B * c->B));
   TRY(dalloc(c, &c->d_fused, (size_t)(M > 2 ? M : 2) * c->B * c->B));
-  TRY(dalloc(c, &c->d_S, (size_t)P * c->B * (2 * c->B - 1) * c->n));
-  TRY(dalloc(c, &c->d_U, (size_t)P * c->n * c->B * c->n));
-  TRY(dalloc(c, &c->d_corr_rot, (size_t)nq * c->n));
+  const int64_t Bo = c->Bo, no = c->no;
+  if (c->Bo != c->B)
+    TRY(dalloc(c, &c->d_fused_o, (size_t)(M > 2 ? M : 2) * Bo * Bo));
+  else
+    c->d_fused_o = c->d_fused;
+  TRY(dalloc(c, &c->d_S, (size_t)P * Bo * (2 * Bo - 1) * no));
+  TRY(dalloc(c, &c->d_U, (size_t)P * no * Bo * no));
+  TRY(dalloc(c, &c->d_corr_rot, (size_t)no * no * no));
   TRY(dalloc(c, &c->d_keys, (size_t)2 * P));
-  c->partials_per_pair = std::max(so3_gamma_blocks(c->B), spatial_c2r_blocks(c->N));
+  c->partials_per_pair = std::max(so3_gamma_blocks(c->Bo), spatial_c2r_blocks(c->N));
   TRY(dalloc(c, &c->d_partials, (size_t)P * c->partials_per_pair));
   TRY(dalloc(c, &c->d_pose, (size_t)P));
   const int MV = M > 16 ? M : 16;
@@ -386,7 +428,7 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3));
   TRY(dalloc(c, &c->d_reg, (size_t)cap));
   if (cf.bingham_peak_neighbors > 0)
-    TRY(dalloc(c, &c->d_peak_rot, (size_t)P * nq * c->n));
+    TRY(dalloc(c, &c->d_peak_rot, (size_t)P * no * no * no));
   if (cf.gaussian_peak_neighbors > 0)
     TRY(dalloc(c, &c->d_peak_trans, (size_t)P * c->T3));
   if (cf.channels == 4) {
@@ -494,24 +536,31 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
       cs.ch[3] = c->d_coef + 2 * ncl * BB + BB;
     }
     launch_fuse(cs, (uint32_t)BB, c->s2.runs, c->d_fused, s, ncl, 2 * BB);
+    // spherical zero padding: the coefficients of degree < B in the layout of
+    // the correlation bandwidth Bo, zero above
+    if (c->Bo != B)
+      launch_pad_coef(c->d_fused, BB, c->d_pad_map, (int64_t)c->Bo * c->Bo, c->d_fused_o, ncl, s);
   }
+  // the SO(3) correlation at Bo = B + spherical_zero_padding
+  const int Bo = c->Bo;
+  const int64_t BBo = (int64_t)Bo * Bo, no3 = (int64_t)c->no * c->no * c->no;
   {
     StageScope t(c, kSo3Synth, s);
-    launch_so3_synth(B, c->d_fused, shared ? 0 : BB, c->d_fused + nt * BB, BB, np, c->d_wig,
-                     c->d_wig_off, c->d_wig_norm, c->d_S, c->cfg.precision, s);
+    launch_so3_synth(Bo, c->d_fused_o, shared ? 0 : BBo, c->d_fused_o + nt * BBo, BBo, np,
+                     c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S, c->cfg.precision, s);
   }
   {
     StageScope t(c, kSo3Alpha, s);
-    launch_so3_fft_alpha(B, np, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s, c->so3_reg);
+    launch_so3_fft_alpha(Bo, np, c->d_S, c->plan_no, c->d_tw_no, c->d_U, s, c->so3_reg);
   }
   {
     StageScope t(c, kSo3Gamma, s);
-    launch_so3_fft_gamma(B, np, c->d_U, c->plan_n, c->d_tw_n, d_rot_corr, nq * n, c->d_partials,
+    launch_so3_fft_gamma(Bo, np, c->d_U, c->plan_no, c->d_tw_no, d_rot_corr, no3, c->d_partials,
                          c->d_keys, s, c->so3_reg);
   }
   {
     StageScope t(c, kRotPose, s);
-    launch_rot_pose(B, np, c->d_keys, d_rot_corr, nq * n, kb, c->d_pose, s);
+    launch_rot_pose(Bo, np, c->d_keys, d_rot_corr, no3, kb, c->d_pose, s);
   }
   // translation (accumulators are zero on entry and cleared again below)
   {
@@ -825,7 +874,7 @@ int phz_register_channels(phz_ctx* c, const float* target, const float* target_r
                  trans_corr ? c->d_corr_trans : nullptr));
   if (registered && Ps > 0) launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, c->d_reg, s);
   TRY(fetch_results(c, 1, result, s));
-  const int64_t n3 = (int64_t)c->n * c->n * c->n;
+  const int64_t n3 = (int64_t)c->no * c->no * c->no;
   if (rot_corr)
     TRY(copy_sync(c, rot_corr, c->d_corr_rot, sizeof(float) * n3, hipMemcpyDeviceToHost));
   if (trans_corr)
@@ -946,6 +995,7 @@ static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, floa
 int phz_so3_correlate_coef(phz_ctx* c, const float* f_coef, const float* h_coef, float* corr,
                            float* synthesis, uint32_t* argmax) {
   PHZ_CHECK(c && f_coef && h_coef, "null argument");
+  PHZ_CHECK(c->Bo == c->B, "the SO(3) stage entry points run without spherical zero padding");
   hipStream_t s = c->stream;
   const int64_t BB = (int64_t)c->B * c->B;
   PHZ_HIP(hipMemcpyAsync(c->d_fused, f_coef, sizeof(float2) * BB, hipMemcpyHostToDevice, s));
@@ -957,6 +1007,7 @@ int phz_spherical_correlate(phz_ctx* c, int C, const float* f_grids, const float
                             float* corr, uint32_t* argmax) {
   PHZ_CHECK(c && f_grids && h_grids, "null argument");
   PHZ_CHECK(C >= 2 && C <= 8, "channel count must be in [2, 8]");
+  PHZ_CHECK(c->Bo == c->B, "the SO(3) stage entry points run without spherical zero padding");
   hipStream_t s = c->stream;
   const int64_t nq = (int64_t)c->n * c->n, BB = (int64_t)c->B * c->B;
   PHZ_HIP(hipMemcpyAsync(c->d_grids, f_grids, sizeof(float) * C * nq, hipMemcpyHostToDevice, s));
@@ -1167,7 +1218,7 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
                            : 0;
   c->tiles.fused_reg = t.fused_reg > 0 && spatial_reg_ok(c->N);
   c->tuning.fused_reg = c->tiles.fused_reg ? 1 : -1;
-  c->so3_reg = t.so3_reg > 0 && so3_reg_ok(c->B);
+  c->so3_reg = t.so3_reg > 0 && so3_reg_ok(c->Bo);
   c->tuning.so3_reg = c->so3_reg ? 1 : -1;
   c->occ_fwd = t.occ_fwd >= 0;
   c->tuning.occ_fwd = c->occ_fwd ? 1 : -1;
@@ -1298,16 +1349,16 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   }
   // SO(3) alpha + gamma passes: register engine against LDS kernels
   float so3_ms[2] = {0.f, 0.f};
-  if (!rc && so3_reg_ok(c->B)) {
-    const int B = c->B, n = c->n;
+  if (!rc && so3_reg_ok(c->Bo)) {
+    const int B = c->Bo, n = c->no;
     launch_fill_u32((uint32_t*)c->d_S, (size_t)2 * P * B * (2 * B - 1) * n, 0u, s);
     for (int variant = 0; variant < 2 && !rc; ++variant) {
       float ms = 0.f;
       for (int pass = 0; pass < 2; ++pass) {
         if (hipEventRecord(a, s) != hipSuccess) rc = -2;
         for (int r = 0; r < repeats; ++r) {
-          launch_so3_fft_alpha(B, P, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s, variant == 1);
-          launch_so3_fft_gamma(B, P, c->d_U, c->plan_n, c->d_tw_n, nullptr, (int64_t)n * n * n,
+          launch_so3_fft_alpha(B, P, c->d_S, c->plan_no, c->d_tw_no, c->d_U, s, variant == 1);
+          launch_so3_fft_gamma(B, P, c->d_U, c->plan_no, c->d_tw_no, nullptr, (int64_t)n * n * n,
                                c->d_partials, c->d_keys, s, variant == 1);
         }
         if (hipEventRecord(b, s) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||

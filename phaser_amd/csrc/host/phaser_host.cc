@@ -56,9 +56,10 @@ constexpr int64_t kDefaultCapacity = 262144;
 
 namespace phaser_core {
 
-std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points, int channels) {
+std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points, int channels,
+                                        int sph_pad) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int>,
+  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int, int>,
                   std::weak_ptr<phz_ctx>>
       cache;
   std::lock_guard<std::mutex> lock(mu);
@@ -68,7 +69,7 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
                                    FLAGS_phaser_core_spatial_low_pass_lower_bound,
                                    FLAGS_phaser_core_spatial_low_pass_upper_bound,
                                    FLAGS_bingham_peak_neighbors, FLAGS_gaussian_peak_neighbors,
-                                   channels);
+                                   channels, sph_pad);
   if (auto p = cache[key].lock()) return p;
   phz_config cfg;
   phz_default_config(&cfg);
@@ -82,6 +83,7 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
   cfg.bingham_peak_neighbors = FLAGS_bingham_peak_neighbors;
   cfg.gaussian_peak_neighbors = FLAGS_gaussian_peak_neighbors;
   cfg.channels = channels;
+  cfg.spherical_zero_padding = sph_pad;
   phz_ctx* raw = nullptr;
   check(phz_create(&cfg, &raw), "phz_create");
   std::shared_ptr<phz_ctx> p(raw, [](phz_ctx* c) { phz_destroy(c); });
@@ -229,15 +231,17 @@ void PhaseAligner::alignRegistered(const model::PointCloud& cloud_prev,
 SphOptRegistration::SphOptRegistration()
     : BaseRegistration("SphOptRegistration"),
       bandwidth_(FLAGS_phaser_core_spherical_bandwidth) {
-  CHECK_TRUE(FLAGS_phaser_core_spherical_zero_padding == 0, "spherical zero padding other than 0 is not supported");
+  CHECK_TRUE(FLAGS_phaser_core_spherical_zero_padding >= 0, "negative spherical zero padding");
   CHECK_TRUE(FLAGS_phaser_core_spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
-  ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity);
+  ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
+                        FLAGS_phaser_core_spherical_zero_padding);
   check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
 }
 
 void SphOptRegistration::setBandwith(const int bandwith) {
   bandwidth_ = bandwith;
-  ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity);
+  ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
+                        FLAGS_phaser_core_spherical_zero_padding);
 }
 
 model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
@@ -253,7 +257,8 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   const bool four = has_extra(*cloud_prev) || has_extra(*cloud_cur);
   std::shared_ptr<phz_ctx> ctx = ctx_;
   if (need > kDefaultCapacity || four)
-    ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need, four ? 4 : 2);
+    ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need, four ? 4 : 2,
+                         FLAGS_phaser_core_spherical_zero_padding);
   auto extra_of = [](model::PointCloud& c) {
     std::vector<float> e(2 * c.size(), 0.f);
     for (size_t i = 0; i < c.size(); ++i) {
@@ -268,7 +273,10 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
     ec = extra_of(*cloud_cur);
   }
   phz_result r;
-  const size_t n3 = 8ull * bandwidth_ * bandwidth_ * bandwidth_;
+  // the SO(3) grid of bandwidth + spherical zero padding (getBandwidth()
+  // returns bw_out, spherical-correlation.cc:118-120)
+  const size_t bo = (size_t)bandwidth_ + (size_t)FLAGS_phaser_core_spherical_zero_padding;
+  const size_t n3 = 8ull * bo * bo * bo;
   std::vector<float> rot(n3);
   model::PointCloud reg = cloud_cur->clone();
   check(phz_register_channels(ctx.get(), cloud_prev->data(), four ? ep.data() : nullptr,

@@ -119,6 +119,11 @@ constexpr int32_t kPoseTransWeights = 4;  // Gaussian window weights sum <= 0
 
 // One wavefront per pair.  corr: the pairs' (2B)^3 grids at corr_stride
 // floats apart (read only when k > 0); k: bingham_peak_neighbors.
+// nsets coefficient sets from the layout of stride in_stride into the
+// layout of `count` entries: out[set][j] = map[j] >= 0 ? in[set][map[j]] : 0
+// (spherical zero padding: degree-limited coefficients at a larger bandwidth)
+void launch_pad_coef(const float2* in, int64_t in_stride, const int* map, int64_t count,
+                     float2* out, int nsets, hipStream_t s);
 void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
                      int64_t corr_stride, int k, DevPose* pose, hipStream_t s);
 

@@ -412,7 +412,25 @@ __global__ void __launch_bounds__(64) k_rot_pose(int bw, const unsigned long lon
   P.status = status;
 }
 
+// Coefficients of each set in the Bo layout: out[j] = in[map[j]] (map: the
+// B-layout index of the same (m, l), or -1 for degrees the input lacks).
+__global__ void k_pad_coef(const float2* __restrict__ in, int64_t in_stride,
+                           const int* __restrict__ map, int64_t count,
+                           float2* __restrict__ out) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int src = map[j];
+  out[blockIdx.y * count + j] =
+      src >= 0 ? in[blockIdx.y * in_stride + src] : make_float2(0.f, 0.f);
+}
+
 }  // namespace
+
+void launch_pad_coef(const float2* in, int64_t in_stride, const int* map, int64_t count,
+                     float2* out, int nsets, hipStream_t s) {
+  hipLaunchKernelGGL(k_pad_coef, dim3((unsigned)((count + 255) / 256), nsets), dim3(256), 0, s,
+                     in, in_stride, map, count, out);
+}
 
 void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
                       int64_t h_stride, int npairs, const float* wigner,
