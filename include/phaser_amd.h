@@ -30,7 +30,11 @@ typedef struct phz_ctx phz_ctx;
 /* Flags of core-gflags.cc:6-39 plus capacities. */
 typedef struct {
   int32_t bandwidth;              /* phaser_core_spherical_bandwidth */
-  int32_t spherical_zero_padding; /* phaser_core_spherical_zero_padding (0 only) */
+  int32_t spherical_zero_padding; /* phaser_core_spherical_zero_padding: the SO(3)
+                                     correlation on the grid of Bo = bandwidth + padding
+                                     (<= 128) from coefficients of degree < bandwidth
+                                     (spherical-correlation.cc:21-27, 248-278); rotation
+                                     grids and indices are then (2 Bo)^3 */
   int32_t n_voxels;               /* phaser_core_spatial_n_voxels */
   int32_t discretize_lower;       /* phaser_core_spatial_discretize_lower */
   int32_t discretize_upper;       /* phaser_core_spatial_discretize_upper */
@@ -99,7 +103,7 @@ int phz_get_config(const phz_ctx* ctx, phz_config* out);
 
 /* ---- end to end -------------------------------------------------------- */
 /* SphOptRegistration::registerPointCloud(cloud_prev = target,
- * cloud_cur = source) (sph-opt-registration.cc:34-47).  rot_corr ((2B)^3
+ * cloud_cur = source) (sph-opt-registration.cc:34-47).  rot_corr ((2Bo)^3
  * floats, |C|^2), trans_corr (N^3 floats) and registered (Ps x 4 floats,
  * the registered source) are optional outputs (NULL to skip). */
 int phz_register(phz_ctx* ctx, const float* target, int64_t Pt, const float* source,
