@@ -972,9 +972,23 @@ __global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict
   const int z = blockIdx.x, pair = blockIdx.y;
   const Twiddles<L> w = load_twiddles<L>(n2, tw);
   {
+    // the plane (L h complex) in one batch of loads per thread, then into
+    // LDS: a strided copy loop waits for each load before its LDS store,
+    // one HBM round trip per iteration
     static_assert((L * h) % 2 == 0, "whole float4s");
+    constexpr int NV = L * h / 2, PER = (NV + 255) / 256;
     const float4* src = (const float4*)(work + ((int64_t)pair * L + z) * L * h);
-    for (int i = threadIdx.x; i < L * h / 2; i += blockDim.x) ((float4*)P)[i] = src[i];
+    float4 t[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      t[i] = src[idx < NV ? idx : NV - 1];  // in bounds; the store below is predicated
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      if (idx < NV) ((float4*)P)[idx] = t[i];
+    }
   }
   __syncthreads();
   // inverse FFT along x
