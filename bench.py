@@ -137,6 +137,65 @@ def concurrent_throughput(args, pairs, max_pts, tuning):
     return P * args.steps / elapsed, elapsed
 
 
+def variant_throughput(args, pairs, max_pts, tuning, steps=5, warmup=2):
+    """Registrations/s of the rows beyond the plain path on the same pairs,
+    one context and stream each (HIP graphs, the main run's tuning), against
+    the plain path measured the same way: clouds with reflectivity and ambient
+    (four live channels), peak-window fits with 3 neighbours each, and
+    spherical zero padding 16 (SO(3) grid of 160^3)."""
+    from phaser_amd.capi import Context
+    P = len(pairs)
+    B, N = 64, 128
+    Ps = [len(s) for s, _, _, _ in pairs]
+    Pt = [len(t) for _, t, _, _ in pairs]
+
+    def extras(cloud):
+        i = cloud[:, 3].astype(np.float32)
+        return np.stack([i * i / np.float32(100), np.float32(50) + np.float32(0.5) * i], 1)
+
+    def run(name, rot_side, **kw):
+        ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P, **kw)
+        t = tuning
+        ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
+                       t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"])
+        if args.graphs:
+            ctx.set_graphs(True)
+        ds = [ctx.device_array(s) for s, _, _, _ in pairs]
+        dt = [ctx.device_array(t) for _, t, _, _ in pairs]
+        four = kw.get("channels") == 4
+        dse = [ctx.device_array(extras(s)) for s, _, _, _ in pairs] if four else []
+        dte = [ctx.device_array(extras(t)) for _, t, _, _ in pairs] if four else []
+        d_rot = ctx.device_empty(P * rot_side ** 3 * 4)
+        d_trans = ctx.device_empty(P * N ** 3 * 4)
+
+        def step():
+            if four:
+                return ctx.register_pairs_channels_device(
+                    [a.ptr for a in dt], [a.ptr for a in dte], Pt, [a.ptr for a in ds],
+                    [a.ptr for a in dse], Ps, d_rot=d_rot.ptr, d_trans=d_trans.ptr)
+            return ctx.register_pairs_device([a.ptr for a in dt], Pt, [a.ptr for a in ds], Ps,
+                                             d_rot=d_rot.ptr, d_trans=d_trans.ptr)
+        for _ in range(warmup):
+            step()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ctx.synchronize()
+        dt_s = time.perf_counter() - t0
+        for a in ds + dt + dse + dte + [d_rot, d_trans]:
+            a.free()
+        ctx.close()
+        return {"value": round(P * steps / dt_s, 2), "ms_per_step": round(dt_s / steps * 1e3, 4)}
+
+    out = {"plain_one_stream": run("plain", 2 * B)}
+    out["four_channels"] = run("four", 2 * B, channels=4)
+    out["peak_neighbors_3"] = run("peak", 2 * B, bingham_peak_neighbors=3,
+                                  gaussian_peak_neighbors=3)
+    out["spherical_zero_padding_16"] = run("pad", 2 * (B + 16), spherical_zero_padding=16)
+    return out
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the same counters on the
@@ -245,6 +304,8 @@ def main():
                     help="contractions: 0 fp32 (plain form), 1 split fp16 on MFMA")
     ap.add_argument("--compare-forms", type=int, default=1,
                     help="also time the other precision form on the same pairs")
+    ap.add_argument("--variants", type=int, default=1,
+                    help="also time four channels, peak windows and spherical padding")
     ap.add_argument("--rates", default="100,200,400,800", help="serve: offered requests/s")
     ap.add_argument("--duration", type=float, default=3.0, help="serve: seconds per rate")
     ap.add_argument("--pool", type=int, default=6, help="serve: pairs per bandwidth")
@@ -409,6 +470,8 @@ def main():
                                  for k, o in enumerate(outs))
         assert o0["rot_index"] == outs[0]["rot_index"]
 
+    variants = variant_throughput(args, pairs, max_pts, tuning) if args.variants else None
+
     out = {
         "metric": "registrations/sec at B=64 + 128^3 voxels",
         "value": round(value, 2),
@@ -441,6 +504,7 @@ def main():
                 "value": round(other["value"], 2), "ms_per_step": round(other["ms_per_step"], 4),
                 "argmax_match": round(other.get("match", 0) / max(S, 1), 4),
                 "stage_ms": other["stage_ms"]}},
+        "variants": variants,
         "roofline": {"bound": "hbm", "kernel": kname, "stage": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),

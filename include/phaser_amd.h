@@ -140,6 +140,17 @@ int phz_register_channels(phz_ctx* ctx, const float* target, const float* target
                           const float* source, const float* source_ra, int64_t Ps,
                           phz_result* result, float* rot_corr, float* trans_corr,
                           float* registered);
+/* phz_register_pairs_device for clouds with reflectivity and ambient:
+ * d_targets_ra / d_sources_ra hold one device pointer per cloud to its
+ * (reflectivity, ambient) pairs (either array, or any entry, may be NULL: a
+ * zero channel).  Needs a context created with channels = 4. */
+int phz_register_pairs_channels_device(phz_ctx* ctx, int n, int shared_target,
+                                       const float* const* d_targets,
+                                       const float* const* d_targets_ra, const int64_t* Pt,
+                                       const float* const* d_sources,
+                                       const float* const* d_sources_ra, const int64_t* Ps,
+                                       phz_result* results, float* d_rot_corr,
+                                       float* d_trans_corr, void* stream);
 /* Device bytes owned by the context (tables and workspaces). */
 int phz_memory_usage(const phz_ctx* ctx, size_t* bytes);
 /* The device bytes phz_create would allocate for cfg (validates cfg; no

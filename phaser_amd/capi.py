@@ -61,6 +61,10 @@ EXPORTS = {
                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "phz_register_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_void_p, C.POINTER(Result)]),
+    "phz_register_pairs_channels_device": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                                     C.c_void_p, C.c_void_p, C.c_void_p,
+                                                     C.c_void_p, C.c_void_p, C.POINTER(Result),
+                                                     C.c_void_p, C.c_void_p, C.c_void_p]),
     "phz_register_channels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                         C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(Result),
                                         C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -256,6 +260,24 @@ class Context:
                                                C.c_void_p(d_rot or None),
                                                C.c_void_p(d_trans or None),
                                                C.c_void_p(stream or None)))
+        return [r.as_dict() for r in res]
+
+    def register_pairs_channels_device(self, d_targets, d_targets_ra, Pt, d_sources,
+                                       d_sources_ra, Ps, shared_target=False, d_rot=0,
+                                       d_trans=0, stream=0):
+        """As register_pairs_device with one (reflectivity, ambient) device
+        array per cloud (0 for none), on a context with channels=4."""
+        n = len(d_sources)
+        T = (C.c_void_p * len(d_targets))(*d_targets)
+        TR = (C.c_void_p * len(d_targets))(*[p or None for p in d_targets_ra])
+        S = (C.c_void_p * n)(*d_sources)
+        SR = (C.c_void_p * n)(*[p or None for p in d_sources_ra])
+        PT = (C.c_int64 * len(Pt))(*Pt)
+        PS = (C.c_int64 * n)(*Ps)
+        res = (Result * n)()
+        _check(lib().phz_register_pairs_channels_device(
+            self.h, n, int(shared_target), T, TR, PT, S, SR, PS, res, C.c_void_p(d_rot or None),
+            C.c_void_p(d_trans or None), C.c_void_p(stream or None)))
         return [r.as_dict() for r in res]
 
     def register_pairs(self, targets, sources, shared_target=False):

@@ -271,7 +271,7 @@ size_t context_bytes(const phz_config& cf) {
   add(M * T3, sizeof(uint32_t));
   add((M * N * N + 3) / 4 * 4, 1);
   add((M * N + 3) / 4 * 4, 1);
-  add(G * H, sizeof(float2));
+  add(std::max<int64_t>(2 * M, 16) * H, sizeof(float2));
   add(P * H, sizeof(float2));
   add(T3, sizeof(float));
   add(cap, sizeof(float4));
@@ -888,13 +888,28 @@ int phz_register_pairs_device(phz_ctx* c, int n, int shared_target, const float*
                               const int64_t* Pt, const float* const* d_sources, const int64_t* Ps,
                               phz_result* results, float* d_rot_corr, float* d_trans_corr,
                               void* stream) {
+  return phz_register_pairs_channels_device(c, n, shared_target, d_targets, nullptr, Pt,
+                                            d_sources, nullptr, Ps, results, d_rot_corr,
+                                            d_trans_corr, stream);
+}
+
+int phz_register_pairs_channels_device(phz_ctx* c, int n, int shared_target,
+                                       const float* const* d_targets,
+                                       const float* const* d_targets_ra, const int64_t* Pt,
+                                       const float* const* d_sources,
+                                       const float* const* d_sources_ra, const int64_t* Ps,
+                                       phz_result* results, float* d_rot_corr,
+                                       float* d_trans_corr, void* stream) {
   PHZ_CHECK(c && d_targets && Pt && d_sources && Ps && results, "null argument");
   PHZ_CHECK(n >= 1 && n <= c->cfg.max_batch, "pair count exceeds max_batch");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const int nt = shared_target ? 1 : n;
-  for (int t = 0; t < nt; ++t) TRY(load_cloud(c, t, d_targets[t], Pt[t], hipMemcpyDeviceToDevice, s));
+  for (int t = 0; t < nt; ++t)
+    TRY(load_cloud(c, t, d_targets[t], Pt[t], hipMemcpyDeviceToDevice, s,
+                   d_targets_ra ? d_targets_ra[t] : nullptr));
   for (int p = 0; p < n; ++p)
-    TRY(load_cloud(c, nt + p, d_sources[p], Ps[p], hipMemcpyDeviceToDevice, s));
+    TRY(load_cloud(c, nt + p, d_sources[p], Ps[p], hipMemcpyDeviceToDevice, s,
+                   d_sources_ra ? d_sources_ra[p] : nullptr));
   TRY(commit_counts(c, nt + n, s));
   TRY(exec_pairs(c, n, shared_target != 0, s, d_rot_corr, d_trans_corr));
   return fetch_results(c, n, results, s);

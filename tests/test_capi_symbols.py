@@ -136,6 +136,27 @@ def test_ctypes_layouts_match_the_c_headers(tmp_path):
             assert C.sizeof(types[key]) == int(val), key
 
 
+def test_memory_estimate_of_the_optional_buffers():
+    """The extra device bytes of four channels (per-point pairs, two voxel
+    sums, a second spectra set, twice the S^2 grids) and of the peak windows
+    (one rotation and one translation grid per pair), from the estimate."""
+    from phaser_amd import capi
+
+    def est(**kw):
+        cfg = capi.default_config(bandwidth=32, n_voxels=64, max_points=200000, max_batch=4,
+                                  **kw)
+        b = C.c_size_t()
+        assert capi.lib().phz_estimate_memory(C.byref(cfg), C.byref(b)) == 0
+        return b.value
+    B, n, N, P, M, cap = 32, 64, 64, 4, 8, 200000
+    nq, T3, H = n * n, N ** 3, N * N * (N // 2 + 1)
+    base = est()
+    grids = (max(4 * M, 16) - max(2 * M, 16)) * (nq * 4 + B * n * 8 + B * B * 8)
+    assert est(channels=4) - base == M * cap * 8 + 2 * M * T3 * 4 + max(2 * M, 16) * H * 8 + grids
+    assert est(bingham_peak_neighbors=2, gaussian_peak_neighbors=1) - base == \
+        P * n ** 3 * 4 + P * T3 * 4
+
+
 def test_memory_estimate_without_gpu():
     """phz_estimate_memory validates like phz_create and makes no device call."""
     from phaser_amd import capi

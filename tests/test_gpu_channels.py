@@ -80,6 +80,32 @@ def test_four_channel_context_without_channels_equals_two(ctx4):
     assert _rel(a["trans_corr"], b["trans_corr"]) < 1e-5
 
 
+def test_batched_four_channels_match_single_pairs():
+    """phz_register_pairs_channels_device: each pair of a batch (one source
+    without channels) equals its single-pair registration."""
+    from phaser_amd import synth
+    from phaser_amd.capi import Context
+    pairs = [synth.pair(seed=s, rings=64, azimuths=1024) for s in (10, 11, 12)]
+    with Context(bandwidth=32, n_voxels=64, max_points=200000, max_batch=4, channels=4) as ctx:
+        dt = [ctx.device_array(t) for _, t, _, _ in pairs]
+        ds = [ctx.device_array(s) for s, _, _, _ in pairs]
+        dte = [ctx.device_array(extras(t)) for _, t, _, _ in pairs]
+        dse = [ctx.device_array(extras(s)) for s, _, _, _ in pairs]
+        src_ra = [dse[0].ptr, 0, dse[2].ptr]  # the second source carries no channels
+        print("batched four channels, 3 pairs", flush=True)
+        res = ctx.register_pairs_channels_device(
+            [a.ptr for a in dt], [a.ptr for a in dte], [len(t) for _, t, _, _ in pairs],
+            [a.ptr for a in ds], src_ra, [len(s) for s, _, _, _ in pairs])
+        for k, (s, t, _, _) in enumerate(pairs):
+            alone = ctx.register(t, s, target_extra=extras(t),
+                                 source_extra=None if k == 1 else extras(s))
+            assert res[k]["rot_index"] == alone["rot_index"]
+            assert res[k]["trans_index"] == alone["trans_index"]
+            np.testing.assert_allclose(res[k]["quat"], alone["quat"], atol=1e-12)
+        for a in dt + ds + dte + dse:
+            a.free()
+
+
 def test_ply_clouds_through_the_cpp_mirror(tmp_path, ctx4):
     """The C++ driver reads both clouds from PLY files (x, y, z, intensity,
     reflectivity, noise) and registers them with all four channels."""
