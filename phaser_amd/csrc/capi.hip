@@ -580,10 +580,16 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   // zz: the forward z pass runs inside the cross-spectrum kernel; occ: the
   // r2c and x passes before it also skip rows / planes without points
   // four channels: the reflectivity / ambient spectra in d_spec2 (layout of
-  // d_spec) go through the dense passes and the channel-generic cross
-  // spectrum
-  const bool zz = !four && c->occ_fwd && c->tiles.fused_reg &&
-                  c->tuning.fwd_z == kRegisterEngine && spatial_cross_zz_ok(N, Fc, Gc);
+  // d_spec) take the same passes; channels 2 and 3 of both sides come from it
+  ChannelSet F4 = Fc, G4 = Gc;
+  if (four) {
+    F4.ch[2] = c->d_spec2;
+    F4.ch[3] = c->d_spec2 + c->H;
+    G4.ch[2] = c->d_spec2 + (int64_t)(2 * nt) * c->H;
+    G4.ch[3] = G4.ch[2] + c->H;
+  }
+  const bool zz = c->occ_fwd && c->tiles.fused_reg && c->tuning.fwd_z == kRegisterEngine &&
+                  (four ? spatial_cross_zz4_ok(N, F4, G4) : spatial_cross_zz_ok(N, Fc, Gc));
   const bool occ = zz && !c->plane_fwd && c->tuning.fwd_x == kRegisterEngine;
   const bool plane = c->plane_fwd && !four;
   {  // y (and with the plane pass also x) first, then the remaining columns
@@ -591,10 +597,13 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     if (plane)
       launch_spatial_plane_fwd(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->plan_N, c->d_tw_N,
                                c->d_spec, s);
-    else if (occ)
+    else if (occ) {
       launch_spatial_r2c_rows(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, occ_from(c, 0), c->d_tw_N,
                               c->d_spec, s);
-    else if (c->tiles.fused_reg) {
+      if (four)
+        launch_spatial_r2c_rows(N, ncl, c->d_sumF, c->d_sumA, c->d_cnt, occ_from(c, 0),
+                                c->d_tw_N, c->d_spec2, s);
+    } else if (c->tiles.fused_reg) {
       launch_spatial_r2c_reg(N, ncl, c->d_sumI, c->d_sumR, c->d_cnt, c->d_row_occ, false,
                              c->d_tw_N, c->d_spec, s);
       if (four)
@@ -610,31 +619,31 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kSpatialFftFwd, s);
-    if (occ)
-      launch_spatial_fft_x_occ(N, 2 * ncl, c->d_spec, occ_from(c, 0), c->d_tw_N, s);
-    else if (!plane)
-      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
-    if (!zz)
-      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
-    if (four) {
-      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec2, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
-      launch_spatial_fft_pass(N, 2 * ncl, c->d_spec2, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
+    for (int set = 0; set < (four ? 2 : 1); ++set) {
+      float2* spec = set ? c->d_spec2 : c->d_spec;
+      if (occ)
+        launch_spatial_fft_x_occ(N, 2 * ncl, spec, occ_from(c, 0), c->d_tw_N, s);
+      else if (!plane)
+        launch_spatial_fft_pass(N, 2 * ncl, spec, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
+      if (!zz)
+        launch_spatial_fft_pass(N, 2 * ncl, spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
     }
   }
   {
     {
       StageScope t(c, kSpatialCrossZ, s);
-      if (four) {
-        const float2* F2 = c->d_spec2;
-        const float2* G2 = c->d_spec2 + (int64_t)(2 * nt) * c->H;
-        ChannelSet F4 = Fc, G4 = Gc;
-        F4.ch[2] = F2;
-        F4.ch[3] = F2 + c->H;
-        G4.ch[2] = G2;
-        G4.ch[3] = G2 + c->H;
+      if (four && zz)
+        launch_spatial_cross_zz4_reg(N, np, F4, fs, G4, gs, occ ? c->d_plane_occ : nullptr,
+                                     shared ? 0 : N,
+                                     occ ? c->d_plane_occ + (int64_t)nt * N : nullptr, N,
+                                     c->vox.runs, c->d_tw_N, c->d_work, s);
+      else if (four && c->tiles.fused_reg)
+        launch_spatial_cross_z_reg(N, np, F4, fs, G4, gs, c->vox.runs, true, c->d_tw_N,
+                                   c->d_work, s);
+      else if (four)
         launch_spatial_cross_z(N, np, F4, fs, G4, gs, c->vox.runs, true, c->plan_N, c->d_tw_N,
                                c->d_work, s);
-      } else if (zz)
+      else if (zz)
         launch_spatial_cross_zz_reg(N, np, Fc, fs, Gc, gs, occ ? c->d_plane_occ : nullptr,
                                     shared ? 0 : N, occ ? c->d_plane_occ + (int64_t)nt * N : nullptr,
                                     N, c->vox.runs, c->d_tw_N, c->d_work, s);

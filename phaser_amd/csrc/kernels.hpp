@@ -209,6 +209,13 @@ void launch_spatial_cross_z_reg(int N, int npairs, ChannelSet F, int64_t f_strid
 // clouds, advanced per pair by the given strides.  Only for channel sets
 // whose channels 0 and 1 are present and the rest absent
 // (spatial_cross_zz_ok).
+// The same for four live channels (F.ch[0..3], G.ch[0..3], C = 4).
+bool spatial_cross_zz4_ok(int N, const ChannelSet& F, const ChannelSet& G);
+void launch_spatial_cross_zz4_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                  int64_t g_stride, const uint8_t* f_occ, int64_t f_occ_stride,
+                                  const uint8_t* g_occ, int64_t g_occ_stride,
+                                  const FusionRuns& runs, const float2* tw, float2* work,
+                                  hipStream_t s);
 bool spatial_cross_zz_ok(int N, const ChannelSet& F, const ChannelSet& G);
 void launch_spatial_cross_zz_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
                                  int64_t g_stride, const uint8_t* f_occ, int64_t f_occ_stride,

@@ -886,6 +886,115 @@ __global__ void __launch_bounds__(256, L <= 128 ? 3 : 2) k_spat_cross_zz_reg(
   });
 }
 
+// One side (F or G) of the four-channel cross spectrum: each channel's
+// forward FFT along z (occupied planes only), folded into the fusion state
+// in channel order as fuse_at does: mode 0 keeps the first channel of
+// largest energy, 1 sums ((c0 + c1) + c2) + c3 and divides by C, 2 is zero.
+template <int L>
+__device__ __forceinline__ void fused_side4(const ChannelSet& X, int64_t off, const uint8_t* occ,
+                                            int q, bool live, int n2, float2* lb,
+                                            const Twiddles<L>& w,
+                                            const int (&mode)[RegSplit<L>::R1],
+                                            float2 (&acc)[RegSplit<L>::R1]) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
+  constexpr int inner = L * (L / 2 + 1);
+  float best[R1];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float2 a[R1], xa[R1];
+    const float2* src = X.ch[c] + off + q;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const int z = n2 + R2 * n1;
+      const bool ld = live && occ[z];
+      a[n1] = ld ? src[(int64_t)z * inner] : make_float2(0.f, 0.f);
+    }
+    fft_four_step_reg<-1, L>(a, n2, lb, w, xa);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < R1; ++i) {
+      const float e = xa[i].x * xa[i].x + xa[i].y * xa[i].y;
+      if (c == 0) {
+        acc[i] = xa[i];
+        best[i] = e;
+      } else if (mode[i] == 0) {
+        if (e > best[i]) {
+          acc[i] = xa[i];
+          best[i] = e;
+        }
+      } else {
+        acc[i].x += xa[i].x;
+        acc[i].y += xa[i].y;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R1; ++i) {
+    if (mode[i] == 1)
+      acc[i] = make_float2(acc[i].x / (float)X.C, acc[i].y / (float)X.C);
+    else if (mode[i] != 0)
+      acc[i] = make_float2(0.f, 0.f);
+  }
+}
+
+// k_spat_cross_zz_reg for four live channels (clouds with reflectivity and
+// ambient): F.ch[0..3] and G.ch[0..3] present; the channels of a side are
+// transformed one at a time to bound the registers held.
+template <int L>
+__global__ void __launch_bounds__(256, 2) k_spat_cross_zz4_reg(
+    ChannelSet F, int64_t f_stride, ChannelSet G, int64_t g_stride,
+    const uint8_t* __restrict__ f_occ, int64_t f_occ_stride, const uint8_t* __restrict__ g_occ,
+    int64_t g_occ_stride, FusionRuns runs, const float2* __restrict__ tw,
+    float2* __restrict__ work) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2, S = L + 1;
+  constexpr int inner = L * (L / 2 + 1);
+  __shared__ float2 buf[LINES * S];
+  __shared__ uint32_t occ_w[2 * L / 4];
+  const int cl = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
+  const int q = blockIdx.x * LINES + cl;
+  const bool live = q < inner;
+  const int p = blockIdx.y;
+  const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
+    const uint8_t* o = i < L / 4 ? f_occ : g_occ;
+    const int64_t st = i < L / 4 ? f_occ_stride : g_occ_stride;
+    occ_w[i] = o ? ((const uint32_t*)(o + p * st))[i % (L / 4)] : 0x01010101u;
+  }
+  __syncthreads();
+  const uint8_t* fo = (const uint8_t*)occ_w;
+  const uint8_t* go = fo + L;
+  float2* lb = buf + cl * S;
+  int mode[R1];
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const uint64_t idx = (uint64_t)(n2 + R2 * j + R1 * k2) * inner + q;
+      int m = 0;
+      for (int r = 0; r < runs.count; ++r)
+        if (idx >= runs.start[r]) m = (int)runs.mode[r];
+      mode[j * R2 + k2] = m;
+    }
+  float2 ff[R1], gf[R1];
+  fused_side4<L>(F, p * f_stride, fo, q, live, n2, lb, w, mode, ff);
+  fused_side4<L>(G, p * g_stride, go, q, live, n2, lb, w, mode, gf);
+  float2 v[R1];
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) {
+      const float2 f = ff[j * R2 + k2], g = gf[j * R2 + k2];
+      float2 cz;
+      cz.x = f.x * g.x - f.y * (-g.y);
+      cz.y = f.x * (-g.y) + f.y * g.x;
+      v[j + (R1 / R2) * k2] = cz;
+    }
+  float2* wp = work + p * (int64_t)L * inner + q;
+  fft_four_step<+1, L>(v, n2, lb, w, [&](int k, float2 x) {
+    if (live) wp[(int64_t)k * inner] = x;
+  });
+}
+
 // last pass: c2r along y (real parts of k = 0, N/2, FFTW hc2r), two rows per
 // complex line, signed first-maximum argmax over x*N + y + N^2*z.
 template <int L>
@@ -1210,6 +1319,43 @@ static void launch_cross_zz_t(int npairs, ChannelSet F, int64_t fs, ChannelSet G
   constexpr int LINES = 256 / RegSplit<L>::R2, inner = L * (L / 2 + 1);
   hipLaunchKernelGGL(k_spat_cross_zz_reg<L>, dim3((inner + LINES - 1) / LINES, npairs), dim3(256),
                      0, s, F, fs, G, gs, f_occ, f_occ_stride, g_occ, g_occ_stride, runs, tw, work);
+}
+
+template <int L>
+static void launch_cross_zz4_t(int npairs, ChannelSet F, int64_t fs, ChannelSet G, int64_t gs,
+                               const uint8_t* f_occ, int64_t f_occ_stride, const uint8_t* g_occ,
+                               int64_t g_occ_stride, const FusionRuns& runs, const float2* tw,
+                               float2* work, hipStream_t s) {
+  constexpr int LINES = 256 / RegSplit<L>::R2, inner = L * (L / 2 + 1);
+  hipLaunchKernelGGL(k_spat_cross_zz4_reg<L>, dim3((inner + LINES - 1) / LINES, npairs),
+                     dim3(256), 0, s, F, fs, G, gs, f_occ, f_occ_stride, g_occ, g_occ_stride,
+                     runs, tw, work);
+}
+
+bool spatial_cross_zz4_ok(int N, const ChannelSet& F, const ChannelSet& G) {
+  if (!spatial_reg_ok(N)) return false;
+  for (const ChannelSet* cs : {&F, &G}) {
+    if (cs->C != 4) return false;
+    for (int c = 0; c < 4; ++c)
+      if (!cs->ch[c]) return false;
+  }
+  return true;
+}
+
+void launch_spatial_cross_zz4_reg(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                  int64_t g_stride, const uint8_t* f_occ, int64_t f_occ_stride,
+                                  const uint8_t* g_occ, int64_t g_occ_stride,
+                                  const FusionRuns& runs, const float2* tw, float2* work,
+                                  hipStream_t s) {
+  if (N == 64)
+    launch_cross_zz4_t<64>(npairs, F, f_stride, G, g_stride, f_occ, f_occ_stride, g_occ,
+                           g_occ_stride, runs, tw, work, s);
+  else if (N == 128)
+    launch_cross_zz4_t<128>(npairs, F, f_stride, G, g_stride, f_occ, f_occ_stride, g_occ,
+                            g_occ_stride, runs, tw, work, s);
+  else
+    launch_cross_zz4_t<256>(npairs, F, f_stride, G, g_stride, f_occ, f_occ_stride, g_occ,
+                            g_occ_stride, runs, tw, work, s);
 }
 
 bool spatial_cross_zz_ok(int N, const ChannelSet& F, const ChannelSet& G) {
