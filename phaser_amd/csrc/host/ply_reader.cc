@@ -8,6 +8,7 @@
 #include "phaser_ply.h"
 
 #include <cstdlib>
+#include <exception>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -103,7 +104,20 @@ void phz_ply_free(phz_ply_cloud* c) {
   std::memset(c, 0, sizeof(*c));
 }
 
+static int ply_read(const char* path, const phz_ply_fields* fields, phz_ply_cloud* out);
+
 int phz_ply_read(const char* path, const phz_ply_fields* fields, phz_ply_cloud* out) {
+  // no C++ exception may cross the C ABI (a header's vertex count can ask
+  // for more memory than the host has)
+  try {
+    return ply_read(path, fields, out);
+  } catch (const std::exception& e) {
+    if (out) phz_ply_free(out);
+    return fail(std::string("reading the PLY file failed: ") + e.what());
+  }
+}
+
+static int ply_read(const char* path, const phz_ply_fields* fields, phz_ply_cloud* out) {
   if (!path || !out) return fail("null argument");
   std::memset(out, 0, sizeof(*out));
   phz_ply_fields names;
@@ -201,6 +215,9 @@ int phz_ply_read(const char* path, const phz_ply_fields* fields, phz_ply_cloud* 
   if (pi < 0) return fail(std::string("no intensity field '") + names.intensity + "'");
 
   const int64_t n = V.count;
+  // a binary body holds at least one byte per vertex property
+  if (format != Format::kAscii && n > (int64_t)(data.size() - pos))
+    return fail("vertex count exceeds the file size");
   std::vector<double> row(V.props.size());
   std::vector<float> xyzi((size_t)n * 4), extra((size_t)n * 2, 0.f);
   std::vector<double> range(pg >= 0 ? (size_t)n : 0);

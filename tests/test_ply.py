@@ -89,3 +89,9 @@ def test_required_fields_and_malformed_files(tmp_path):
         ply.read_ply(tmp_path / "bad.ply")
     with pytest.raises(ply.PlyError, match="unable to open"):
         ply.read_ply(tmp_path / "missing.ply")
+    huge = (b"ply\nformat binary_little_endian 1.0\nelement vertex 99999999999\n"
+            b"property float x\nproperty float y\nproperty float z\nproperty float intensity\n"
+            b"end_header\n" + bytes(64))
+    (tmp_path / "huge.ply").write_bytes(huge)
+    with pytest.raises(ply.PlyError, match="exceeds the file size"):
+        ply.read_ply(tmp_path / "huge.ply")
