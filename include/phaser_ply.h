@@ -51,6 +51,11 @@ void phz_ply_default_fields(phz_ply_fields* fields);
 int phz_ply_read(const char* path, const phz_ply_fields* fields, phz_ply_cloud* out);
 void phz_ply_free(phz_ply_cloud* cloud);
 const char* phz_ply_last_error(void);
+/* Writes n points (x, y, z, intensity) as an ASCII PLY, the form of
+ * pcl::io::savePLYFileASCII of a PointXYZI cloud that the reference's
+ * driver writes the registered cloud with (driver.cc:17-23); floats with 9
+ * significant digits, which read back exactly.  Returns 0 or -1. */
+int phz_ply_write_xyzi(const char* path, int64_t n, const float* xyzi);
 
 #ifdef __cplusplus
 }

@@ -82,6 +82,12 @@ def build(verbose=False, jobs=8):
                   "-o", dout + ".tmp", drv, "-L", LIB, "-lphaser_host", "-lphaser_amd",
                   "-Wl,-rpath,$ORIGIN"], verbose)
             os.replace(dout + ".tmp", dout)
+        # the reference's command-line driver (driver.cc) over the mirror
+        cout = os.path.join(LIB, "phaser_core_driver")
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+              "-o", cout + ".tmp", os.path.join(CSRC, "host", "phaser_core_driver.cc"), "-L", LIB,
+              "-lphaser_host", "-lphaser_amd", "-Wl,-rpath,$ORIGIN"], verbose)
+        os.replace(cout + ".tmp", cout)
     vout = os.path.join(LIB, "libphaser_serve.so")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-pthread",
           "-I", os.path.join(ROOT, "include"), "-o", vout + ".tmp",

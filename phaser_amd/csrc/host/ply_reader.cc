@@ -7,6 +7,7 @@
 // point-cloud.cc:330-349).
 #include "phaser_ply.h"
 
+#include <cstdio>
 #include <cstdlib>
 #include <exception>
 #include <cstring>
@@ -314,6 +315,21 @@ static int ply_read(const char* path, const phz_ply_fields* fields, phz_ply_clou
   out->has_ambient = pa >= 0;
   out->has_range = pg >= 0;
   return 0;
+}
+
+int phz_ply_write_xyzi(const char* path, int64_t n, const float* xyzi) {
+  if (!path || n < 0 || (n > 0 && !xyzi)) return fail("null argument");
+  FILE* f = std::fopen(path, "w");
+  if (!f) return fail(std::string("unable to write ply file: ") + path);
+  std::fprintf(f,
+               "ply\nformat ascii 1.0\nelement vertex %lld\nproperty float x\nproperty float y\n"
+               "property float z\nproperty float intensity\nend_header\n",
+               (long long)n);
+  for (int64_t i = 0; i < n; ++i)
+    std::fprintf(f, "%.9g %.9g %.9g %.9g\n", xyzi[4 * i], xyzi[4 * i + 1], xyzi[4 * i + 2],
+                 xyzi[4 * i + 3]);
+  const bool ok = std::fclose(f) == 0;
+  return ok ? 0 : fail(std::string("writing ply file failed: ") + path);
 }
 
 }  // extern "C"

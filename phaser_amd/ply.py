@@ -31,6 +31,7 @@ EXPORTS = {
     "phz_ply_read": (C.c_int, [C.c_char_p, C.POINTER(Fields), C.POINTER(Cloud)]),
     "phz_ply_free": (None, [C.POINTER(Cloud)]),
     "phz_ply_last_error": (C.c_char_p, []),
+    "phz_ply_write_xyzi": (C.c_int, [C.c_char_p, C.c_int64, C.c_void_p]),
 }
 
 
@@ -73,6 +74,14 @@ def read_ply(path, intensity=None, reflectivity=None, ambient=None, range_field=
     finally:
         lib().phz_ply_free(C.byref(c))
     return out
+
+
+def write_ply_xyzi(path, xyzi):
+    """The ASCII PLY the command-line driver writes (phz_ply_write_xyzi)."""
+    a = np.ascontiguousarray(xyzi, np.float32)
+    assert a.ndim == 2 and a.shape[1] == 4
+    if lib().phz_ply_write_xyzi(str(path).encode(), a.shape[0], a.ctypes.data) != 0:
+        raise PlyError(lib().phz_ply_last_error().decode())
 
 
 _TYPES = {"char": "i1", "uchar": "u1", "short": "i2", "ushort": "u2", "int": "i4",

@@ -33,7 +33,7 @@ def test_ply_reader_exports_every_declared_symbol():
     from phaser_amd import ply
     L = ply.lib()
     names = _declared("phaser_ply.h")
-    assert len(names) == 4
+    assert len(names) == 5
     for n in names:
         assert hasattr(L, n), n
         assert n in ply.EXPORTS, f"{n} has no ctypes binding"

@@ -128,3 +128,36 @@ def test_ply_clouds_through_the_cpp_mirror(tmp_path, ctx4):
     assert out["rot_index"] == want["rot_index"] and out["trans_index"] == want["trans_index"]
     np.testing.assert_allclose(out["state"][:4], want["quat"], atol=1e-12)
     np.testing.assert_allclose(out["state"][5:], want["translation"], atol=1e-9)
+
+
+def test_command_line_driver(tmp_path):
+    """phaser_core_driver (the reference's driver.cc flow): PLY target and
+    source with reflectivity and ambient in, registered source PLY out,
+    equal to the C ABI's registered cloud; the state line matches the pose."""
+    from phaser_amd import ply, synth
+    from phaser_amd.capi import Context
+    src, tgt, _, _ = synth.pair(seed=5, rings=64, azimuths=1024)
+    paths = {}
+    for name, cl in (("target", tgt), ("source", src)):
+        e = extras(cl)
+        paths[name] = tmp_path / f"{name}.ply"
+        ply.write_ply(paths[name], {"x": ("float", cl[:, 0]), "y": ("float", cl[:, 1]),
+                                    "z": ("float", cl[:, 2]), "intensity": ("float", cl[:, 3]),
+                                    "reflectivity": ("float", e[:, 0]),
+                                    "noise": ("float", e[:, 1])})
+    out = tmp_path / "reg.ply"
+    drv = os.path.join(ROOT, "phaser_amd", "lib", "phaser_core_driver")
+    print("command-line driver", flush=True)
+    r = subprocess.run([drv, f"--target_cloud={paths['target']}",
+                        f"--source_cloud={paths['source']}", f"--reg_cloud={out}",
+                        "--phaser_core_spherical_bandwidth=32", "--phaser_core_spatial_n_voxels=64"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    state = [float(v) for v in r.stdout.split("Registration result:")[1].split()[:8]]
+    with Context(bandwidth=32, n_voxels=64, max_points=262144, channels=4) as ctx:
+        want = ctx.register(tgt, src, registered=True, target_extra=extras(tgt),
+                            source_extra=extras(src))
+    np.testing.assert_allclose(state[:4], want["quat"], atol=1e-12)
+    np.testing.assert_allclose(state[5:], want["translation"], atol=1e-9)
+    got = ply.read_ply(out)["xyzi"]
+    np.testing.assert_allclose(got, want["registered"], atol=1e-5)

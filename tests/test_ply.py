@@ -95,3 +95,12 @@ def test_required_fields_and_malformed_files(tmp_path):
     (tmp_path / "huge.ply").write_bytes(huge)
     with pytest.raises(ply.PlyError, match="exceeds the file size"):
         ply.read_ply(tmp_path / "huge.ply")
+
+
+def test_ascii_writer_round_trips(tmp_path):
+    rng = np.random.default_rng(5)
+    pts = (rng.normal(size=(300, 4)) * [30, 30, 3, 50]).astype(np.float32)
+    p = tmp_path / "reg.ply"
+    ply.write_ply_xyzi(p, pts)
+    assert p.read_text().startswith("ply\nformat ascii 1.0\nelement vertex 300\n")
+    np.testing.assert_array_equal(ply.read_ply(p)["xyzi"], pts)
