@@ -4,7 +4,8 @@ Compiles oracle/phaser_oracle.c into oracle/libphaser_oracle.so with gcc and
 -ffp-contract=off, so float expressions round as the reference's scalar C++
 does.  The reference's own SOFT/FFTW path is not built here: it needs FFTW3,
 Eigen, PCL, glog and nlopt headers, none of which exist in this image, so
-oracle/_ref stays empty (see the oracle section of DESIGN.md).
+oracle/_ref stays empty; the restatement is pinned by the reference's own
+known-answer vectors, closed forms and brute force instead (tests/).
 """
 import os
 import subprocess

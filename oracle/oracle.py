@@ -1,7 +1,11 @@
 """ctypes wrapper around the FP64 oracle (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module; it is the checker, never the product path.
+this module; it is the checker, never the product path.  The C functions are
+described in phaser_oracle.h; the peak-window fits below (peak_window ...
+refine_translation, numpy FP64) are pinned by tests/test_peak_fit.py: the
+known answers of the reference's bingham-test.cc and gaussian-test.cc, the
+window rules case by case, and closed forms.
 """
 import ctypes as C
 import os

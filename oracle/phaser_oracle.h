@@ -11,8 +11,14 @@
  * FP64 except where the reference itself computes in float (projection, query
  * grid, voxel edges, point transforms); those places use float on purpose.
  *
- * Pinning status of each function is listed in DESIGN.md's oracle table and is
- * checked by tests/test_oracle_*.py.
+ * Each function is pinned by tests/test_oracle.py, tests/test_golden.py,
+ * tests/test_channels_oracle.py and tests/test_sph_padding_oracle.py (the
+ * reference tests' known answers, closed forms, brute force, library routines).
+ * Parity unpinned (no reference output fixes them): the last ulp of SOFT's
+ * semi-naive DCT route and of FFTW (here direct quadrature and an own FFT, equal
+ * in exact arithmetic); FLANN's choice among exactly equidistant neighbours
+ * (here the lowest index); PCL's transform summation order (PCL 1.8 scalar
+ * order assumed); Eigen LinSpaced (>= 3.3.4 semantics assumed).
  */
 #ifndef PHASER_ORACLE_H_
 #define PHASER_ORACLE_H_
