@@ -2,7 +2,9 @@
  * phaser_amd.h -- C ABI of the MI355X correlation core of PHASER.
  *
  * Plain C types only.  Every call returns 0 on success, a negative status
- * on error (-1 invalid argument, -2 HIP runtime error, -3 out of capacity);
+ * on error (-1 invalid argument, -2 HIP runtime error, -3 out of capacity,
+ * -4 a registered pair failed one of the reference's CHECKs on its peak
+ * windows: every pair's result is filled and phz_result.status names it);
  * phz_last_error() returns the message of the calling thread's last error.
  * A context owns one HIP device, one stream, the resident tables for its
  * bandwidth and voxel count, and all device workspaces.  A context is not
@@ -67,7 +69,7 @@ typedef struct {
  * (the default) both fits have one sample: quat is the argmax's quaternion
  * and translation the argmax's translation.  A window the reference rejects
  * with a failed CHECK (a Bingham window reaching index (2B)^3, or weights that
- * do not sum to a positive value) makes the call return -1. */
+ * do not sum to a positive value) makes the call return -4 (phz_result.status). */
 typedef struct {
   uint32_t rot_index;     /* argmax of the (2B)^3 SO(3) grid [beta][alpha][gamma] */
   uint32_t trans_index;   /* argmax of the N^3 grid, x*N + y + N^2*z */
@@ -88,7 +90,15 @@ typedef struct {
                              concentrations are fitted to */
   double rot_axes[16];    /* row-major 4x4 whose columns are the matching unit
                              eigenvectors (the Bingham M, last column = quat) */
+  int32_t status;         /* 0, or the failed checks (the call then returns -4):
+                             PHZ_PAIR_ROT_WINDOW, PHZ_PAIR_ROT_WEIGHTS,
+                             PHZ_PAIR_TRANS_WEIGHTS */
+  int32_t reserved;
 } phz_result;
+
+#define PHZ_PAIR_ROT_WINDOW 1    /* Bingham window end >= (2Bo)^3: CHECK_LT, bingham-peak-based-eval.cc:69 */
+#define PHZ_PAIR_ROT_WEIGHTS 2   /* Bingham window weights sum <= 0: CHECK_GT, :88 */
+#define PHZ_PAIR_TRANS_WEIGHTS 4 /* Gaussian window weights sum <= 0: CHECK_GT, gaussian-peak-based-eval.cc:100 */
 
 /* Defaults of core-gflags.cc (bandwidth 75 there; 64 here so that the
  * default is the benchmarked configuration), max_points 262144. */

@@ -29,7 +29,8 @@ class Result(C.Structure):
                 ("zyz", C.c_double * 3), ("quat", C.c_double * 4),
                 ("euler_xyz", C.c_double * 3), ("T_rot", C.c_float * 16),
                 ("translation", C.c_double * 3), ("trans_cov", C.c_double * 9),
-                ("rot_omega", C.c_double * 4), ("rot_axes", C.c_double * 16)]
+                ("rot_omega", C.c_double * 4), ("rot_axes", C.c_double * 16),
+                ("status", C.c_int32), ("reserved", C.c_int32)]
 
     def as_dict(self):
         return {
@@ -42,6 +43,7 @@ class Result(C.Structure):
             "trans_cov": np.array(self.trans_cov[:]).reshape(3, 3),
             "rot_omega": np.array(self.rot_omega[:]),
             "rot_axes": np.array(self.rot_axes[:]).reshape(4, 4),
+            "status": int(self.status),
         }
 
 

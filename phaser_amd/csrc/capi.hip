@@ -736,6 +736,17 @@ int exec_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* rot, float
   return 0;
 }
 
+std::string pair_status_text(int32_t st) {
+  if (st & kPoseRotWindow)
+    return "the Bingham peak window reaches past the rotation grid "
+           "(bingham-peak-based-eval.cc:69 CHECK_LT(end, n))";
+  if (st & kPoseRotWeights)
+    return "the Bingham peak window's weights do not sum to a positive value "
+           "(bingham-peak-based-eval.cc:88 CHECK_GT)";
+  return "the Gaussian peak window's weights do not sum to a positive value "
+         "(gaussian-peak-based-eval.cc:100 CHECK_GT)";
+}
+
 void to_result(const DevPose& d, phz_result* r) {
   r->rot_index = d.rot_index;
   r->trans_index = d.trans_index;
@@ -755,6 +766,11 @@ void to_result(const DevPose& d, phz_result* r) {
     r->rot_axes[i] = d.rot_axes[i];
   }
   for (int i = 0; i < 9; ++i) r->trans_cov[i] = d.trans_cov[i];
+  static_assert(kPoseRotWindow == PHZ_PAIR_ROT_WINDOW && kPoseRotWeights == PHZ_PAIR_ROT_WEIGHTS &&
+                    kPoseTransWeights == PHZ_PAIR_TRANS_WEIGHTS,
+                "pose status bits are the API's");
+  r->status = d.status;
+  r->reserved = 0;
 }
 
 int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
@@ -763,18 +779,13 @@ int fetch_results(phz_ctx* c, int np, phz_result* out, hipStream_t s) {
   PHZ_HIP(hipStreamSynchronize(s));
   collect_timings(c);
   for (int p = 0; p < np; ++p) to_result(hp[p], &out[p]);
-  // the reference's CHECKs on the peak windows (its process aborts there)
+  // the reference's CHECKs on the peak windows (its process aborts there):
+  // every result is filled, the first failing pair is named
   for (int p = 0; p < np; ++p) {
     const int32_t st = hp[p].status;
-    PHZ_CHECK(!(st & kPoseRotWindow),
-              "pair " + std::to_string(p) + ": the Bingham peak window reaches past the "
-              "rotation grid (bingham-peak-based-eval.cc:69 CHECK_LT(end, n))");
-    PHZ_CHECK(!(st & kPoseRotWeights),
-              "pair " + std::to_string(p) + ": the Bingham peak window's weights do not sum "
-              "to a positive value (bingham-peak-based-eval.cc:88 CHECK_GT)");
-    PHZ_CHECK(!(st & kPoseTransWeights),
-              "pair " + std::to_string(p) + ": the Gaussian peak window's weights do not sum "
-              "to a positive value (gaussian-peak-based-eval.cc:100 CHECK_GT)");
+    if (!st) continue;
+    set_error("pair " + std::to_string(p) + ": " + pair_status_text(st));
+    return -4;
   }
   return 0;
 }

@@ -297,8 +297,20 @@ void worker(phs_server* s, Lane* L) {
     const std::string err = rc ? std::string(phz_last_error()) : std::string();
 
     lk.lock();
-    for (int i = 0; i < n; ++i)
-      finish_locked(s, batch[i], rc, err, rc ? nullptr : &res[i], n, shared, t0, t1);
+    for (int i = 0; i < n; ++i) {
+      if (rc == -4) {
+        // the registration ran; only the pairs that failed one of the
+        // reference's peak-window checks fail (their results say which)
+        const bool bad = res[i].status != 0;
+        finish_locked(s, batch[i], bad ? -4 : 0,
+                      bad ? "a peak-window check failed (phz_result.status " +
+                                std::to_string(res[i].status) + ")"
+                          : std::string(),
+                      &res[i], n, shared, t0, t1);
+      } else {
+        finish_locked(s, batch[i], rc, err, rc ? nullptr : &res[i], n, shared, t0, t1);
+      }
+    }
     ++s->st.batches;
     s->st.pairs_in_batches += (uint64_t)n;
     L->busy = false;

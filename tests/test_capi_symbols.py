@@ -50,8 +50,8 @@ def test_struct_layouts():
     from phaser_amd import capi
     assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 6
     # rot/trans index + values (16) + zyz,quat,euler (80) + T (64) + translation (24)
-    # + trans_cov (72) + rot_omega (32) + rot_axes (128)
-    assert C.sizeof(capi.Result) == 16 + 80 + 64 + 24 + 72 + 32 + 128
+    # + trans_cov (72) + rot_omega (32) + rot_axes (128) + status, reserved (8)
+    assert C.sizeof(capi.Result) == 16 + 80 + 64 + 24 + 72 + 32 + 128 + 8
 
 
 def test_default_config_and_error_path_without_gpu():
@@ -98,7 +98,7 @@ int main(void) {
   S(phs_config); S(phs_request); S(phs_response); S(phs_stats);
   O(phz_config, max_points); O(phz_config, device); O(phz_result, zyz);
   O(phz_result, T_rot); O(phz_result, translation); O(phz_result, trans_cov);
-  O(phz_result, rot_omega); O(phz_result, rot_axes);
+  O(phz_result, rot_omega); O(phz_result, rot_axes); O(phz_result, status);
   O(phz_config, bingham_peak_neighbors); O(phz_config, gaussian_peak_neighbors);
   O(phz_config, channels);
   O(phs_config, bingham_peak_neighbors); O(phs_config, gaussian_peak_neighbors);

@@ -10,7 +10,7 @@ Also: a call without output grids (the context's own buffers) gives the
 same pose as one with them (to 1e-12 for the rotation; to 1e-5 m for the
 translation, whose voxel sums are float atomics in arbitrary order); batched pairs each get the fit of their own
 grids; 0 neighbours keep the argmax pose with omega (0, 0, 0, 1) and a zero
-covariance; a Bingham window that ends at (2B)^3 fails with -1, as the
+covariance; a Bingham window that ends at (2B)^3 fails with -4, as the
 reference's CHECK_LT(end, n) (bingham-peak-based-eval.cc:69) aborts.
 """
 import numpy as np
@@ -119,7 +119,7 @@ def test_zero_neighbours_keep_the_argmax_pose():
 
 def test_bingham_window_past_the_grid_fails():
     """k = (2B)^3 - argmax makes the window end at (2B)^3: the call returns
-    -1 (the reference's CHECK_LT aborts); one neighbour fewer is a valid
+    -4 (the reference's CHECK_LT aborts); one neighbour fewer is a valid
     window, which ends at the grid's last index."""
     from phaser_amd.capi import Context, PhaserError
     B, N = 16, 32
@@ -129,7 +129,7 @@ def test_bingham_window_past_the_grid_fails():
     k = (2 * B) ** 3 - idx
     print(f"argmax {idx}, bingham_peak_neighbors {k}", flush=True)
     with Context(bandwidth=B, n_voxels=N, max_points=200000, bingham_peak_neighbors=k) as ctx:
-        with pytest.raises(PhaserError, match="error -1: pair 0: the Bingham peak window"):
+        with pytest.raises(PhaserError, match="error -4: pair 0: the Bingham peak window"):
             ctx.register(tgt, src)
         with pytest.raises(oracle.PeakWindowError):
             oracle.peak_window(idx, (2 * B) ** 3, k, "bingham")

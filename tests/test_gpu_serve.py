@@ -107,3 +107,29 @@ def test_served_under_tight_budget():
     for a, g in zip(alone, got):
         _same(a, g)
     assert st["bytes_peak"] <= budget
+
+
+def test_failed_peak_window_fails_only_its_request():
+    """A pair whose Bingham window would end at (2B)^3 (the reference's
+    CHECK_LT(end, n), bingham-peak-based-eval.cc:69) fails with -4 and
+    status PHZ_PAIR_ROT_WINDOW; the other pairs served with it succeed
+    and equal their registration alone."""
+    B, N = 32, 64
+    pairs = [synth.mixed_pair(seed, B) for seed in (20, 21, 22)]
+    with capi.Context(bandwidth=B, n_voxels=N) as c:
+        idx = [c.register(t, s)["rot_index"] for (s, t, _, _) in pairs]
+    assert idx[0] not in idx[1:], "pick seeds with distinct rotation peaks"
+    k = (2 * B) ** 3 - idx[0]
+    print(f"serve peak window: argmax {idx}, bingham_peak_neighbors {k}", flush=True)
+    with capi.Context(bandwidth=B, n_voxels=N, bingham_peak_neighbors=k) as c:
+        alone = [c.register(t, s) for (s, t, _, _) in pairs[1:]]
+    with serve.Server(max_batch=4, batch_window_ms=5.0, peak_neighbors=(k, 0)) as srv:
+        tickets = [srv.submit(t, s, bandwidth=B, n_voxels=N, tag=i)
+                   for i, (s, t, _, _) in enumerate(pairs)]
+        got = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert got[0]["status"] == -4 and "status 1" in got[0]["error"], got[0]
+    for a, g in zip(alone, got[1:]):
+        _same(a, g)
+        np.testing.assert_allclose(g["rot_omega"], a["rot_omega"], atol=1e-12)
+    assert st["failed"] == 1 and st["completed"] == 2
