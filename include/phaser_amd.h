@@ -55,7 +55,20 @@ typedef struct {
   int32_t channels;  /* per-point signal channels: 2 (range and intensity; reflectivity and
                         ambient are zero, as for a PointXYZI cloud; default) or 4 (clouds
                         may also carry reflectivity and ambient, phz_register_channels) */
+  int32_t method;    /* PHZ_METHOD_SPH_OPT (0, SphOptRegistration: Laplace fusion of the
+                        channels) or PHZ_METHOD_SPH (1, SphRegistration, sph-registration.cc:
+                        82-158: one channel 0.6 intensity + 0.4 range, spherical-correlation.cc:
+                        54-112; no spherical zero padding, which its SphericalCorrelation(bw)
+                        does not take; reflectivity and ambient, with channels = 4, reach the
+                        translation's fusion only).  Applies to the registration entry points;
+                        the SO(3) stage entry points are SphericalCorrelationLaplace's */
+  int32_t skip_rot_axes; /* bits 0/1/2: refine_rot_x/y/z false (sph-registration.cc:114-119),
+                            that Euler angle left out of T_rot and the registered cloud;
+                            PHZ_METHOD_SPH only */
 } phz_config;
+
+#define PHZ_METHOD_SPH_OPT 0
+#define PHZ_METHOD_SPH 1
 
 /* The pose and uncertainty parts of model::RegistrationResult
  * (registration-result.cc:13-99, sph-opt-registration.cc:49-95): the

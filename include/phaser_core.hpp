@@ -46,6 +46,13 @@ extern int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound;
 // Defined at file scope in the reference: bingham-peak-based-eval.cc:8-10 and
 // gaussian-peak-based-eval.cc:9-11 (default 0: a single-sample fit at the
 // argmax).
+// sph-registration.cc:16-32 (SphRegistration, CloudController("sph"))
+extern int32_t FLAGS_spherical_bandwith;               // 75
+extern std::string FLAGS_alignment_algorithm;          // "phase"
+extern std::string FLAGS_rot_evaluation_algorithm;     // "bingham"
+extern std::string FLAGS_pos_evaluation_algorithm;     // "gaussian"
+extern bool FLAGS_refine_rot_x, FLAGS_refine_rot_y, FLAGS_refine_rot_z;  // true
+extern bool FLAGS_estimate_rotation, FLAGS_estimate_translation;         // true
 extern int32_t FLAGS_bingham_peak_neighbors;
 extern int32_t FLAGS_gaussian_peak_neighbors;
 // PLY field names (ply-helper.cc:8-22)
@@ -192,9 +199,12 @@ using VectorXd = std::vector<double>;  // stands in for Eigen::VectorXd
 // One context per (bandwidth, voxel count, capacity); shared by the objects
 // below and released with the last of them.
 // channels: 2, or 4 for clouds that carry reflectivity and ambient;
-// sph_pad: spherical zero padding of the SO(3) correlation.
+// sph_pad: spherical zero padding of the SO(3) correlation; method and
+// skip_rot_axes: phz_config's.
 std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points,
-                                        int channels = 2, int sph_pad = 0);
+                                        int channels = 2, int sph_pad = 0,
+                                        int method = 0 /* PHZ_METHOD_SPH_OPT */,
+                                        int skip_rot_axes = 0);
 
 class SphericalCorrelation {
  public:
@@ -325,15 +335,37 @@ class SphOptRegistration : public BaseRegistration {
   // per-stage device time (ms, calls) since the last call
   std::vector<std::pair<std::string, double>> getStatistics() const;
 
- private:
+ protected:
+  // method: PHZ_METHOD_SPH_OPT or PHZ_METHOD_SPH; skip_rot_axes as phz_config
+  SphOptRegistration(const std::string& name, int bandwidth, int method, int skip_rot_axes,
+                     int sph_pad);
   int bandwidth_;
+  int method_, skip_rot_axes_, sph_pad_;
   std::shared_ptr<phz_ctx> ctx_;
 };
 
-// cloud-controller.cc:10-34 (preprocessing is not part of this build)
+// sph-registration.cc:35-158 on the device: the same pipeline with one
+// sampled channel 0.6 intensity + 0.4 range instead of the Laplace fusion
+// (spherical-correlation.cc:54-112), bandwidth FLAGS_spherical_bandwith, no
+// spherical zero padding, and the refine_rot_x/y/z masks on the applied
+// rotation.  Evaluations "bingham" and "gaussian" (the defaults); "bmm"
+// fits its mixture to Glover samples of a random sampler
+// (bingham-mixture.cc:54-80) and "gmm" is its positional counterpart: not
+// part of this build.  Alignment "phase" only (the reference has no other).
+class SphRegistration : public SphOptRegistration {
+ public:
+  SphRegistration();
+  SphRegistration(std::string&& alignment_algorithm, std::string&& rot_evaluation_algorithm,
+                  std::string&& pos_evaluation_algorithm);
+  model::RegistrationResult registerPointCloud(model::PointCloudPtr cloud_prev,
+                                               model::PointCloudPtr cloud_cur) override;
+};
+
+// cloud-controller.cc:10-34 (preprocessing is not part of this build);
+// "sph" (the reference's default) or "sph-opt"
 class CloudController {
  public:
-  explicit CloudController(const std::string& method = "sph-opt");
+  explicit CloudController(const std::string& method = "sph");
   model::RegistrationResult registerPointCloud(model::PointCloudPtr cloud_prev,
                                                model::PointCloudPtr cloud_cur);
 

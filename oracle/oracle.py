@@ -26,7 +26,8 @@ u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 class Config(C.Structure):
     _fields_ = [("bw", C.c_int), ("n_voxels", C.c_int), ("lower", C.c_int),
                 ("upper", C.c_int), ("lp_lower", C.c_int), ("lp_upper", C.c_int),
-                ("use_kdtree", C.c_int), ("sph_pad", C.c_int)]
+                ("use_kdtree", C.c_int), ("sph_pad", C.c_int), ("method", C.c_int),
+                ("skip_rot_axes", C.c_int)]
 
 
 class Result(C.Structure):
@@ -256,6 +257,20 @@ def so3_synthesis(bw, f, h):
     return S
 
 
+def spherical_correlate_blend(bw, f, h):
+    """SphericalCorrelation::correlateSignals (spherical-correlation.cc:54-80,
+    103-112) on sampled grids f, h (4 x (2bw)^2, channel order range,
+    intensity, reflectivity, ambient): |C|^2 on the (2bw)^3 grid of the
+    channel 0.6 intensity + 0.4 range."""
+    n = 2 * bw
+    f = np.ascontiguousarray(f, np.float64).reshape(4, n * n)
+    h = np.ascontiguousarray(h, np.float64).reshape(4, n * n)
+    mag = np.zeros(n ** 3)
+    lib().po_spherical_correlate_blend(bw, f[0].copy(), f[1].copy(), h[0].copy(), h[1].copy(),
+                                       mag)
+    return mag
+
+
 def spherical_correlate_laplace(bw, f, h, fused=False):
     f = _c(f, np.float64)
     h = _c(h, np.float64)
@@ -469,11 +484,15 @@ def sub2ind(i, j, k, rows, cols):
 
 def register(target, source, bw=32, n_voxels=64, lower=-50, upper=50,
              kdtree=True, want_grids=False, want_reg=False, target_extra=None,
-             source_extra=None, sph_pad=0):
+             source_extra=None, sph_pad=0, method=0, skip_rot_axes=0):
     """target_extra / source_extra: (P, 2) reflectivity and ambient per
     point, or None (zero channels).  sph_pad: spherical zero padding (the
-    SO(3) grid at bw + sph_pad)."""
-    cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree), int(sph_pad))
+    SO(3) grid at bw + sph_pad).  method 1: SphRegistration's single blended
+    channel (no padding); skip_rot_axes: its refine_rot_x/y/z masks."""
+    cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree), int(sph_pad), int(method),
+                 int(skip_rot_axes))
+    if method == 1:
+        sph_pad = 0
     t = _c(target, np.float32)
     s = _c(source, np.float32)
     te, se = _extra(target_extra, t.shape[0]), _extra(source_extra, s.shape[0])

@@ -1249,17 +1249,22 @@ void po_register_ex(const po_config* cfg, const float* target, const float* targ
   po_sample_cloud_ex(bw, source, source_extra, Ps, max_sq, cfg->use_kdtree, hg);
   double t1 = now_ms();
   /* the SO(3) grid at bw_out = bw + spherical zero padding */
-  const int bw_out = bw + cfg->sph_pad;
+  const int bw_out = bw + (cfg->method == 1 ? 0 : cfg->sph_pad);
   const size_t no3 = (size_t)8 * bw_out * bw_out * bw_out;
   double* mag = rot_corr ? rot_corr : (double*)xcalloc(no3, sizeof(double));
-  po_spherical_correlate_laplace_pad(bw, bw_out, 4, fg, hg, mag, NULL, NULL);
+  if (cfg->method == 1)
+    po_spherical_correlate_blend(bw, fg, fg + nn, hg, hg + nn, mag);
+  else
+    po_spherical_correlate_laplace_pad(bw, bw_out, 4, fg, hg, mag, NULL, NULL);
   res->rot_index = po_argmax(mag, no3);
   res->rot_value = mag[res->rot_index];
   po_zyz_from_index(res->rot_index, bw_out, res->zyz);
   po_quat_from_zyz(res->zyz, res->quat);
   po_xyz_from_quat(res->quat, res->euler_xyz);
-  po_transform_xyz((float)res->euler_xyz[0], (float)res->euler_xyz[1],
-                   (float)res->euler_xyz[2], res->T_rot);
+  const int skip = cfg->method == 1 ? cfg->skip_rot_axes : 0;
+  po_transform_xyz((skip & 1) ? 0.f : (float)res->euler_xyz[0],
+                   (skip & 2) ? 0.f : (float)res->euler_xyz[1],
+                   (skip & 4) ? 0.f : (float)res->euler_xyz[2], res->T_rot);
   float* reg = (float*)xcalloc(4 * (size_t)Ps, sizeof(float));
   po_apply_transform(res->T_rot, source, Ps, reg);
   double t2 = now_ms();

@@ -178,6 +178,12 @@ typedef struct {
   int lp_upper;      /* phaser_core_spatial_low_pass_upper_bound */
   int use_kdtree;    /* NN route (results identical) */
   int sph_pad;       /* phaser_core_spherical_zero_padding: SO(3) grid at bw + sph_pad */
+  int method;        /* 0: SphOptRegistration (Laplace fusion of the four channels);
+                        1: SphRegistration (sph-registration.cc:82-158): one sampled
+                        channel 0.6 intensity + 0.4 range (spherical-correlation.cc:54-80,
+                        103-112), transformed and correlated unshifted, no padding */
+  int skip_rot_axes; /* method 1: bits 0/1/2 = refine_rot_x/y/z false, that Euler angle
+                        zeroed before RotateAroundXYZ (sph-registration.cc:114-119) */
 } po_config;
 
 typedef struct {

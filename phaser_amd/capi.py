@@ -20,7 +20,8 @@ class Config(C.Structure):
                 ("low_pass_lower", C.c_int32), ("low_pass_upper", C.c_int32),
                 ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32),
                 ("precision", C.c_int32), ("bingham_peak_neighbors", C.c_int32),
-                ("gaussian_peak_neighbors", C.c_int32), ("channels", C.c_int32)]
+                ("gaussian_peak_neighbors", C.c_int32), ("channels", C.c_int32),
+                ("method", C.c_int32), ("skip_rot_axes", C.c_int32)]
 
 
 class Result(C.Structure):

@@ -302,6 +302,13 @@ int check_config(const phz_config& cf) {
             "gaussian_peak_neighbors must be in [0, 2^20]");
   PHZ_CHECK(cf.channels == 0 || cf.channels == 2 || cf.channels == 4,
             "channels must be 2 or 4 (0 means 2)");
+  PHZ_CHECK(cf.method == PHZ_METHOD_SPH_OPT || cf.method == PHZ_METHOD_SPH,
+            "method must be PHZ_METHOD_SPH_OPT (0) or PHZ_METHOD_SPH (1)");
+  PHZ_CHECK(cf.skip_rot_axes >= 0 && cf.skip_rot_axes <= 7, "skip_rot_axes must be in [0, 7]");
+  PHZ_CHECK(cf.method == PHZ_METHOD_SPH || cf.skip_rot_axes == 0,
+            "skip_rot_axes (refine_rot_x/y/z) belongs to PHZ_METHOD_SPH");
+  PHZ_CHECK(cf.method != PHZ_METHOD_SPH || cf.spherical_zero_padding == 0,
+            "PHZ_METHOD_SPH correlates without spherical zero padding");
   PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
   PHZ_CHECK(make_plan(2 * (cf.bandwidth + cf.spherical_zero_padding)).nst > 0,
             "2*(bandwidth + spherical zero padding) must factor into 2, 3 and 5");
@@ -527,8 +534,12 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     launch_fst(c->d_grids, (four ? 4 : 2) * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F,
                c->d_coef, c->cfg.precision, s);
   }
-  {  // spherical order {range, intensity, refl, amb} (refl = amb = 0 with two
-     // channels), one launch for all clouds
+  if (c->cfg.method == PHZ_METHOD_SPH) {
+    // SphRegistration: one channel 0.6 intensity + 0.4 range, no fusion
+    StageScope t(c, kFuse, s);
+    launch_blend(c->d_coef, 2 * BB, (uint32_t)BB, ncl, c->d_fused, s);
+  } else {  // spherical order {range, intensity, refl, amb} (refl = amb = 0 with
+            // two channels), one launch for all clouds
     StageScope t(c, kFuse, s);
     ChannelSet cs = channels(c->d_coef, c->d_coef + BB, 4);
     if (four) {
@@ -560,7 +571,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kRotPose, s);
-    launch_rot_pose(Bo, np, c->d_keys, d_rot_corr, no3, kb, c->d_pose, s);
+    launch_rot_pose(Bo, np, c->d_keys, d_rot_corr, no3, kb, c->d_pose, s, c->cfg.skip_rot_axes);
   }
   // translation (accumulators are zero on entry and cleared again below)
   {

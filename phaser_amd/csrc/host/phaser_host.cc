@@ -25,6 +25,12 @@ int32_t FLAGS_phaser_core_spatial_low_pass_lower_bound = 0;
 int32_t FLAGS_phaser_core_spatial_low_pass_upper_bound = 100000;
 }  // namespace phaser_core
 
+int32_t FLAGS_spherical_bandwith = 75;
+std::string FLAGS_alignment_algorithm = "phase";
+std::string FLAGS_rot_evaluation_algorithm = "bingham";
+std::string FLAGS_pos_evaluation_algorithm = "gaussian";
+bool FLAGS_refine_rot_x = true, FLAGS_refine_rot_y = true, FLAGS_refine_rot_z = true;
+bool FLAGS_estimate_rotation = true, FLAGS_estimate_translation = true;
 int32_t FLAGS_bingham_peak_neighbors = 0;
 int32_t FLAGS_gaussian_peak_neighbors = 0;
 std::string FLAGS_phaser_ply_intensity_str = "intensity";
@@ -57,9 +63,9 @@ constexpr int64_t kDefaultCapacity = 262144;
 namespace phaser_core {
 
 std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points, int channels,
-                                        int sph_pad) {
+                                        int sph_pad, int method, int skip_rot_axes) {
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int, int>,
+  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int, int, int, int>,
                   std::weak_ptr<phz_ctx>>
       cache;
   std::lock_guard<std::mutex> lock(mu);
@@ -69,7 +75,7 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
                                    FLAGS_phaser_core_spatial_low_pass_lower_bound,
                                    FLAGS_phaser_core_spatial_low_pass_upper_bound,
                                    FLAGS_bingham_peak_neighbors, FLAGS_gaussian_peak_neighbors,
-                                   channels, sph_pad);
+                                   channels, sph_pad, method, skip_rot_axes);
   if (auto p = cache[key].lock()) return p;
   phz_config cfg;
   phz_default_config(&cfg);
@@ -84,6 +90,8 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
   cfg.gaussian_peak_neighbors = FLAGS_gaussian_peak_neighbors;
   cfg.channels = channels;
   cfg.spherical_zero_padding = sph_pad;
+  cfg.method = method;
+  cfg.skip_rot_axes = skip_rot_axes;
   phz_ctx* raw = nullptr;
   check(phz_create(&cfg, &raw), "phz_create");
   std::shared_ptr<phz_ctx> p(raw, [](phz_ctx* c) { phz_destroy(c); });
@@ -229,19 +237,66 @@ void PhaseAligner::alignRegistered(const model::PointCloud& cloud_prev,
 
 // ---------------------------------------------------------------- registration
 SphOptRegistration::SphOptRegistration()
-    : BaseRegistration("SphOptRegistration"),
-      bandwidth_(FLAGS_phaser_core_spherical_bandwidth) {
-  CHECK_TRUE(FLAGS_phaser_core_spherical_zero_padding >= 0, "negative spherical zero padding");
+    : SphOptRegistration("SphOptRegistration", FLAGS_phaser_core_spherical_bandwidth,
+                         PHZ_METHOD_SPH_OPT, 0, FLAGS_phaser_core_spherical_zero_padding) {}
+
+SphOptRegistration::SphOptRegistration(const std::string& name, int bandwidth, int method,
+                                       int skip_rot_axes, int sph_pad)
+    : BaseRegistration(name),
+      bandwidth_(bandwidth),
+      method_(method),
+      skip_rot_axes_(skip_rot_axes),
+      sph_pad_(sph_pad) {
+  CHECK_TRUE(sph_pad_ >= 0, "negative spherical zero padding");
   CHECK_TRUE(FLAGS_phaser_core_spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
   ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
-                        FLAGS_phaser_core_spherical_zero_padding);
+                        sph_pad_, method_, skip_rot_axes_);
   check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
 }
 
 void SphOptRegistration::setBandwith(const int bandwith) {
   bandwidth_ = bandwith;
   ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
-                        FLAGS_phaser_core_spherical_zero_padding);
+                        sph_pad_, method_, skip_rot_axes_);
+}
+
+static int refine_mask() {
+  return (FLAGS_refine_rot_x ? 0 : 1) | (FLAGS_refine_rot_y ? 0 : 2) |
+         (FLAGS_refine_rot_z ? 0 : 4);
+}
+
+SphRegistration::SphRegistration()
+    : SphRegistration(std::string(FLAGS_alignment_algorithm),
+                      std::string(FLAGS_rot_evaluation_algorithm),
+                      std::string(FLAGS_pos_evaluation_algorithm)) {}
+
+SphRegistration::SphRegistration(std::string&& alignment_algorithm,
+                                 std::string&& rot_evaluation_algorithm,
+                                 std::string&& pos_evaluation_algorithm)
+    : SphOptRegistration("SphRegistration", FLAGS_spherical_bandwith, PHZ_METHOD_SPH,
+                         refine_mask(), 0) {
+  // sph-registration.cc:56-79
+  CHECK_TRUE(alignment_algorithm == "phase",
+             "unknown alignment algorithm '" + alignment_algorithm + "' (phase only)");
+  if (rot_evaluation_algorithm == "bmm")
+    fatal("rot evaluation 'bmm' (a Bingham mixture fitted to random Glover samples, "
+          "bingham-mixture.cc:54-80) is not part of this build");
+  CHECK_TRUE(rot_evaluation_algorithm == "bingham",
+             "Unknown rot evaluation algorithm specificed: " + rot_evaluation_algorithm);
+  if (pos_evaluation_algorithm == "gmm")
+    fatal("pos evaluation 'gmm' is not part of this build");
+  CHECK_TRUE(pos_evaluation_algorithm == "gaussian",
+             "Unknown pos evaluation algorithm specificed: " + pos_evaluation_algorithm);
+}
+
+model::RegistrationResult SphRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
+                                                              model::PointCloudPtr cloud_cur) {
+  // sph-registration.cc:89-96: no rotation estimate is "not yet implemented"
+  // there; translation without rotation is not on this path either
+  CHECK_TRUE(FLAGS_estimate_rotation, "not yet implemented.");
+  CHECK_TRUE(FLAGS_estimate_translation,
+             "estimate_translation = false is not part of this build");
+  return SphOptRegistration::registerPointCloud(cloud_prev, cloud_cur);
 }
 
 model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
@@ -258,7 +313,7 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   std::shared_ptr<phz_ctx> ctx = ctx_;
   if (need > kDefaultCapacity || four)
     ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need, four ? 4 : 2,
-                         FLAGS_phaser_core_spherical_zero_padding);
+                         sph_pad_, method_, skip_rot_axes_);
   auto extra_of = [](model::PointCloud& c) {
     std::vector<float> e(2 * c.size(), 0.f);
     for (size_t i = 0; i < c.size(); ++i) {
@@ -275,7 +330,7 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   phz_result r;
   // the SO(3) grid of bandwidth + spherical zero padding (getBandwidth()
   // returns bw_out, spherical-correlation.cc:118-120)
-  const size_t bo = (size_t)bandwidth_ + (size_t)FLAGS_phaser_core_spherical_zero_padding;
+  const size_t bo = (size_t)bandwidth_ + (size_t)sph_pad_;
   const size_t n3 = 8ull * bo * bo * bo;
   std::vector<float> rot(n3);
   model::PointCloud reg = cloud_cur->clone();
@@ -313,10 +368,12 @@ std::vector<std::pair<std::string, double>> SphOptRegistration::getStatistics() 
 }
 
 CloudController::CloudController(const std::string& method) {
-  if (method == "sph-opt")
+  if (method == "sph")
+    registrator_.reset(new SphRegistration());
+  else if (method == "sph-opt")
     registrator_.reset(new SphOptRegistration());
   else
-    fatal("registration method '" + method + "' is not part of this build (use sph-opt)");
+    fatal("Unknown registration method specified: " + method);
 }
 
 model::RegistrationResult CloudController::registerPointCloud(model::PointCloudPtr cloud_prev,

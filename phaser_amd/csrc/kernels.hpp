@@ -68,6 +68,12 @@ struct ChannelSet {
 // are fused in one launch into out[set][count].
 void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
                  hipStream_t s, int nsets = 1, int64_t set_stride = 0);
+// SphRegistration's single channel (spherical-correlation.cc:103-112):
+// out[set] = 0.6 coef(intensity) + 0.4 coef(range) of each set's [range,
+// intensity] coefficients (set_stride apart), by linearity of the S^2
+// transform the coefficients of the blended samples.
+void launch_blend(const float2* coef, int64_t set_stride, uint32_t count, int nsets, float2* out,
+                  hipStream_t s);
 // keys[pair] = max over partials[pair*count .. +count)
 void launch_reduce_keys(const unsigned long long* partials, int count, int npairs,
                         unsigned long long* keys, hipStream_t s);
@@ -124,8 +130,10 @@ constexpr int32_t kPoseTransWeights = 4;  // Gaussian window weights sum <= 0
 // (spherical zero padding: degree-limited coefficients at a larger bandwidth)
 void launch_pad_coef(const float2* in, int64_t in_stride, const int* map, int64_t count,
                      float2* out, int nsets, hipStream_t s);
+// skip_axes: bits 0/1/2 leave the x/y/z Euler angle out of the transform
 void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
-                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s);
+                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s,
+                     int skip_axes = 0);
 
 // ------------------------------------------------------------- spatial
 // voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3

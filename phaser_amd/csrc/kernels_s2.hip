@@ -621,6 +621,22 @@ void launch_fst(const float* sig, int S, int bw, const float* legendre, const in
                      leg_off_dev, coef);
 }
 
+__global__ void __launch_bounds__(256) k_blend(const float2* __restrict__ coef, int64_t set_stride,
+                                               uint32_t count, float2* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= count) return;
+  const float2* c = coef + (int64_t)blockIdx.y * set_stride;
+  const float2 r = c[i], t = c[count + i];
+  out[(int64_t)blockIdx.y * count + i] =
+      make_float2(0.6f * t.x + 0.4f * r.x, 0.6f * t.y + 0.4f * r.y);
+}
+
+void launch_blend(const float2* coef, int64_t set_stride, uint32_t count, int nsets, float2* out,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_blend, dim3((count + 255) / 256, nsets), dim3(256), 0, s, coef, set_stride,
+                     count, out);
+}
+
 void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
                  hipStream_t s, int nsets, int64_t set_stride) {
   hipLaunchKernelGGL(k_fuse, dim3((count + 255) / 256, nsets), dim3(256), 0, s, cs, set_stride,

@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2*
 __global__ void __launch_bounds__(64) k_rot_pose(int bw, const unsigned long long* __restrict__ keys,
                                                  const float* __restrict__ corr,
                                                  int64_t corr_stride, int k,
-                                                 DevPose* __restrict__ pose) {
+                                                 DevPose* __restrict__ pose, int skip_axes) {
   const int p = blockIdx.x, lane = threadIdx.x;
   const unsigned long long key = keys[p];
   const uint32_t idx = key_index(key);
@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(64) k_rot_pose(int bw, const unsigned long lon
   P.rot_value = unordered_value((uint32_t)(key >> 32));
   for (int d = 0; d < 3; ++d) P.zyz[d] = zyz[d];
   bingham_from_moment(C, qp, multi, P.quat, P.rot_omega, P.rot_axes);
-  euler_and_transform(P.quat, P.euler, P.T);
+  euler_and_transform(P.quat, P.euler, P.T, skip_axes);
   P.status = status;
 }
 
@@ -535,9 +535,9 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
 }
 
 void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
-                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s) {
+                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s, int skip_axes) {
   hipLaunchKernelGGL(k_rot_pose, dim3(npairs), dim3(64), 0, s, bw, keys, corr, corr_stride, k,
-                     pose);
+                     pose, skip_axes);
 }
 
 }  // namespace phz

@@ -69,8 +69,11 @@ __host__ __device__ inline void mat4_mul_nc(const float* A, const float* B, floa
 }
 
 // rotation-utils.cc:86-99 (XYZ Euler angles of q) and :17-23,109-143
-// (T = Tx Ty Tz in float, as RotateAroundXYZ builds it)
-__host__ __device__ inline void euler_and_transform(const double q[4], double euler[3], float T[16]) {
+// (T = Tx Ty Tz in float, as RotateAroundXYZ builds it).  skip bits 0/1/2:
+// the x/y/z angle is left out of T (refine_rot_x/y/z false,
+// sph-registration.cc:114-119); euler keeps all three.
+__host__ __device__ inline void euler_and_transform(const double q[4], double euler[3], float T[16],
+                                                    int skip = 0) {
   const double qw = q[0], qx = q[1], qy = q[2], qz = q[3];
   const double r11 = -2 * (qy * qz - qw * qx);
   const double r12 = qw * qw - qx * qx - qy * qy + qz * qz;
@@ -80,7 +83,9 @@ __host__ __device__ inline void euler_and_transform(const double q[4], double eu
   euler[2] = -atan2(r31, r32);
   euler[1] = asin(r21);
   euler[0] = atan2(r11, r12);
-  const float ea = (float)euler[0], eb = (float)euler[1], ec = (float)euler[2];
+  const float ea = (skip & 1) ? 0.f : (float)euler[0];
+  const float eb = (skip & 2) ? 0.f : (float)euler[1];
+  const float ec = (skip & 4) ? 0.f : (float)euler[2];
   float X[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
   float Y[16], Z[16], XY[16];
   for (int e = 0; e < 16; ++e) Y[e] = Z[e] = X[e];

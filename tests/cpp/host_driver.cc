@@ -33,22 +33,30 @@ static model::PointCloudPtr load(const char* path) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 5 && argc != 7) {
+  if (argc != 5 && argc != 7 && argc != 9) {
     std::fprintf(stderr,
                  "usage: %s target.bin source.bin bandwidth n_voxels "
-                 "[bingham_peak_neighbors gaussian_peak_neighbors]\n",
+                 "[bingham_peak_neighbors gaussian_peak_neighbors [method refine_rot_xyz]]\n",
                  argv[0]);
     return 2;
   }
   phaser_core::FLAGS_phaser_core_spherical_bandwidth = std::atoi(argv[3]);
+  FLAGS_spherical_bandwith = std::atoi(argv[3]);
   phaser_core::FLAGS_phaser_core_spatial_n_voxels = std::atoi(argv[4]);
-  if (argc == 7) {
+  if (argc >= 7) {
     FLAGS_bingham_peak_neighbors = std::atoi(argv[5]);
     FLAGS_gaussian_peak_neighbors = std::atoi(argv[6]);
   }
+  std::string method = "sph-opt";
+  if (argc == 9) {  // method ("sph" or "sph-opt") and refine_rot_x/y/z as "101"
+    method = argv[7];
+    FLAGS_refine_rot_x = argv[8][0] == '1';
+    FLAGS_refine_rot_y = argv[8][1] == '1';
+    FLAGS_refine_rot_z = argv[8][2] == '1';
+  }
   auto target = load(argv[1]);
   auto source = load(argv[2]);
-  phaser_core::CloudController controller("sph-opt");
+  phaser_core::CloudController controller(std::move(method));
   model::RegistrationResult r = controller.registerPointCloud(target, source);
   const auto s = r.getStateAsVec();
   const auto& corr = r.getRotationCorrelation();

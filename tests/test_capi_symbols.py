@@ -48,7 +48,7 @@ def test_synth_library_exports():
 
 def test_struct_layouts():
     from phaser_amd import capi
-    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 6
+    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 8
     # rot/trans index + values (16) + zyz,quat,euler (80) + T (64) + translation (24)
     # + trans_cov (72) + rot_omega (32) + rot_axes (128) + status, reserved (8)
     assert C.sizeof(capi.Result) == 16 + 80 + 64 + 24 + 72 + 32 + 128 + 8
@@ -100,7 +100,7 @@ int main(void) {
   O(phz_result, T_rot); O(phz_result, translation); O(phz_result, trans_cov);
   O(phz_result, rot_omega); O(phz_result, rot_axes); O(phz_result, status);
   O(phz_config, bingham_peak_neighbors); O(phz_config, gaussian_peak_neighbors);
-  O(phz_config, channels);
+  O(phz_config, channels); O(phz_config, method); O(phz_config, skip_rot_axes);
   O(phs_config, bingham_peak_neighbors); O(phs_config, gaussian_peak_neighbors);
   O(phs_config, max_points); O(phs_config, hbm_budget); O(phs_config, batch_window_ms);
   O(phs_config, precision); O(phs_config, graphs); O(phz_config, precision);
