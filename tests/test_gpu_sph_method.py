@@ -6,7 +6,8 @@ the oracle's closed form of the same bins.  The refine_rot_x/y/z masks leave
 their Euler angles out of T_rot (and so out of the registered cloud and the
 translation) as the oracle does; reflectivity and ambient (channels = 4) do
 not change the rotation grid of this method; the C++ mirror's
-CloudController("sph") equals the C ABI."""
+CloudController("sph") equals the C ABI; both methods at the reference's
+default sizes (bandwidth 75, 100 voxels)."""
 import json
 import os
 import subprocess
@@ -105,3 +106,18 @@ def test_cloud_controller_sph_through_the_cpp_mirror(tmp_path):
     assert out["rot_index"] == want["rot_index"] and out["trans_index"] == want["trans_index"]
     np.testing.assert_allclose(out["state"][:4], want["quat"], atol=1e-12)
     np.testing.assert_allclose(out["state"][5:], want["translation"], atol=1e-9)
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_reference_default_sizes_match_oracle(po, method):
+    """The reference's default flags: bandwidth 75 (2B = 150, radix-5 SO(3)
+    FFTs) and 100 voxels (radix 5 in every spatial pass), for both
+    registration methods."""
+    from phaser_amd import synth
+    from phaser_amd.capi import Context
+    src, tgt, _, _ = synth.pair(seed=7, rings=64, azimuths=1024)
+    print(f"B=75 N=100 method {method}", flush=True)
+    with Context(bandwidth=75, n_voxels=100, max_points=200000, method=method) as ctx:
+        r = ctx.register(tgt, src, grids=True)
+    o = po.register(tgt, src, bw=75, n_voxels=100, want_grids=True, method=method)
+    _check(r, o)
