@@ -141,8 +141,9 @@ def variant_throughput(args, pairs, max_pts, tuning, steps=5, warmup=2):
     """Registrations/s of the rows beyond the plain path on the same pairs,
     one context and stream each (HIP graphs, the main run's tuning), against
     the plain path measured the same way: clouds with reflectivity and ambient
-    (four live channels), peak-window fits with 3 neighbours each, and
-    spherical zero padding 16 (SO(3) grid of 160^3)."""
+    (four live channels), peak-window fits with 3 neighbours each,
+    spherical zero padding 16 (SO(3) grid of 160^3) and SphRegistration's
+    blended single channel (method 1)."""
     from phaser_amd.capi import Context
     P = len(pairs)
     B, N = 64, 128
@@ -193,6 +194,7 @@ def variant_throughput(args, pairs, max_pts, tuning, steps=5, warmup=2):
     out["peak_neighbors_3"] = run("peak", 2 * B, bingham_peak_neighbors=3,
                                   gaussian_peak_neighbors=3)
     out["spherical_zero_padding_16"] = run("pad", 2 * (B + 16), spherical_zero_padding=16)
+    out["sph_method"] = run("sph", 2 * B, method=1)
     return out
 
 
