@@ -54,6 +54,8 @@ typedef struct {
                                   (phz_set_graphs); default 1 */
   int32_t bingham_peak_neighbors;   /* phz_config fields of every context */
   int32_t gaussian_peak_neighbors;
+  int32_t method;                   /* phz_config.method: PHZ_METHOD_SPH_OPT or PHZ_METHOD_SPH */
+  int32_t skip_rot_axes;            /* phz_config.skip_rot_axes (PHZ_METHOD_SPH) */
 } phs_config;
 
 typedef struct {

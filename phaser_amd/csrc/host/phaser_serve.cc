@@ -102,6 +102,8 @@ phz_config shape_config(const phs_config& sc, int B, int N) {
   c.precision = sc.precision;
   c.bingham_peak_neighbors = sc.bingham_peak_neighbors;
   c.gaussian_peak_neighbors = sc.gaussian_peak_neighbors;
+  c.method = sc.method;
+  c.skip_rot_axes = sc.skip_rot_axes;
   return c;
 }
 

@@ -25,7 +25,8 @@ class ServeConfig(C.Structure):
                 ("max_points", C.c_int64), ("hbm_budget", C.c_uint64),
                 ("batch_window_ms", C.c_double), ("precision", C.c_int32),
                 ("graphs", C.c_int32), ("bingham_peak_neighbors", C.c_int32),
-                ("gaussian_peak_neighbors", C.c_int32)]
+                ("gaussian_peak_neighbors", C.c_int32), ("method", C.c_int32),
+                ("skip_rot_axes", C.c_int32)]
 
 
 class Request(C.Structure):
@@ -100,11 +101,13 @@ def _check(rc):
 class Server:
     def __init__(self, max_batch=16, contexts_per_shape=2, hbm_budget=64 << 30,
                  max_points=262144, batch_window_ms=0.0, device=0, bounds=(-50, 50),
-                 autotune=False, precision=0, graphs=True, peak_neighbors=(0, 0)):
+                 autotune=False, precision=0, graphs=True, peak_neighbors=(0, 0), method=0,
+                 skip_rot_axes=0):
         cfg = ServeConfig()
         lib().phs_default_config(C.byref(cfg))
         cfg.graphs = int(graphs)
         cfg.bingham_peak_neighbors, cfg.gaussian_peak_neighbors = peak_neighbors
+        cfg.method, cfg.skip_rot_axes = method, skip_rot_axes
         cfg.max_batch = max_batch
         cfg.contexts_per_shape = contexts_per_shape
         cfg.hbm_budget = int(hbm_budget)

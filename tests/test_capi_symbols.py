@@ -102,6 +102,7 @@ int main(void) {
   O(phz_config, bingham_peak_neighbors); O(phz_config, gaussian_peak_neighbors);
   O(phz_config, channels); O(phz_config, method); O(phz_config, skip_rot_axes);
   O(phs_config, bingham_peak_neighbors); O(phs_config, gaussian_peak_neighbors);
+  O(phs_config, method); O(phs_config, skip_rot_axes);
   O(phs_config, max_points); O(phs_config, hbm_budget); O(phs_config, batch_window_ms);
   O(phs_config, precision); O(phs_config, graphs); O(phz_config, precision);
   O(phs_request, target); O(phs_request, Ps); O(phs_request, tag);

@@ -133,3 +133,19 @@ def test_failed_peak_window_fails_only_its_request():
         _same(a, g)
         np.testing.assert_allclose(g["rot_omega"], a["rot_omega"], atol=1e-12)
     assert st["failed"] == 1 and st["completed"] == 2
+
+
+def test_served_sph_method_equals_alone():
+    """A server whose contexts run SphRegistration (method 1, refine_rot_z
+    false) serves each pair as a context of that method registers it alone."""
+    B, N = 32, 64
+    pairs = [synth.mixed_pair(seed, B) for seed in (30, 31, 32, 33)]
+    with capi.Context(bandwidth=B, n_voxels=N, method=1, skip_rot_axes=4) as c:
+        alone = [c.register(t, s) for (s, t, _, _) in pairs]
+    print("served sph method", flush=True)
+    with serve.Server(max_batch=4, batch_window_ms=5.0, method=1, skip_rot_axes=4) as srv:
+        tickets = [srv.submit(t, s, bandwidth=B, n_voxels=N) for (s, t, _, _) in pairs]
+        got = [srv.wait(t) for t in tickets]
+    for a, g in zip(alone, got):
+        _same(a, g)
+        np.testing.assert_array_equal(g["quat"], a["quat"])
