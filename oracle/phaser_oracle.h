@@ -12,8 +12,10 @@
  * grid, voxel edges, point transforms); those places use float on purpose.
  *
  * Each function is pinned by tests/test_oracle.py, tests/test_golden.py,
- * tests/test_channels_oracle.py and tests/test_sph_padding_oracle.py (the
- * reference tests' known answers, closed forms, brute force, library routines).
+ * tests/test_channels_oracle.py, tests/test_sph_padding_oracle.py and
+ * tests/test_sph_method_oracle.py (po_spherical_correlate_blend and
+ * po_config.method) (the reference tests' known answers, closed forms, brute
+ * force, library routines).
  * Parity unpinned (no reference output fixes them): the last ulp of SOFT's
  * semi-naive DCT route and of FFTW (here direct quadrature and an own FFT, equal
  * in exact arithmetic); FLANN's choice among exactly equidistant neighbours
