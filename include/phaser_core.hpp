@@ -179,6 +179,9 @@ class RegistrationResult {
   common::BinghamPtr getRotUncertaintyEstimate() const { return rot_dist_; }
   common::GaussianPtr getPosUncertaintyEstimate() const { return pos_dist_; }
   uint32_t rotation_index = 0, translation_index = 0;
+  // the row-major float transform the rotation estimate applied to the source
+  // (RotateAroundXYZ, rotation-utils.cc:17-23)
+  std::array<float, 16> rotation_transform{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 
  private:
   PointCloudPtr reg_cloud_;

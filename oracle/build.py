@@ -24,16 +24,18 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libphaser_oracle.so")
 
 
-def build(verbose=False):
-    src = os.path.join(HERE, "phaser_oracle.c")
-    tmp = LIB + ".tmp"
+def build(verbose=False, src=None, out=None):
+    """Compiles phaser_oracle.c (or the given source) into LIB (or out)."""
+    src = src or os.path.join(HERE, "phaser_oracle.c")
+    out = out or LIB
+    tmp = out + f".tmp{os.getpid()}"  # concurrent builds (test workers) do not collide
     cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
-           "-fno-fast-math", "-Wall", "-Wno-unused-function", "-o", tmp, src, "-lm"]
+           "-fno-fast-math", "-Wall", "-Wno-unused-function", "-I", HERE, "-o", tmp, src, "-lm"]
     if verbose:
         print(" ".join(cmd))
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 REF_ROOT = "/root/reference"

@@ -13,7 +13,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(HERE, "libphaser_oracle.so")
+# PHZ_ORACLE_LIB names another build of phaser_oracle.c (the mutation tests
+# load deliberately broken builds through it, to show the pins fail them)
+_LIB_PATH = os.environ.get("PHZ_ORACLE_LIB") or os.path.join(HERE, "libphaser_oracle.so")
 _lib = None
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

@@ -1054,6 +1054,12 @@ uint32_t po_sub2ind(uint32_t i, uint32_t j, uint32_t k, uint32_t rows,
 }
 #define sub2ind po_sub2ind
 
+static int all_zero(const double* v, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (v[i] != 0.0) return 0;
+  return 1;
+}
+
 /* spatial-correlation.cc:154-160 */
 static void cmul_conj(const double* F, const double* G, double* C) {
   C[0] = F[0] * G[0] - F[1] * (-G[1]);
@@ -1096,10 +1102,16 @@ void po_spatial_correlate_laplace(int N, int C, const double* f,
   double* Fc = (double*)xcalloc(2 * (size_t)T * C, sizeof(double));
   double* Gc = (double*)xcalloc(2 * (size_t)T * C, sizeof(double));
   for (int ch = 0; ch < C; ++ch) {
-    po_r2c3(N, f + (size_t)ch * T, Fc + 2 * (size_t)ch * T);
-    po_r2c3(N, g + (size_t)ch * T, Gc + 2 * (size_t)ch * T);
-    po_fftshift(Fc + 2 * (size_t)ch * T, T);
-    po_fftshift(Gc + 2 * (size_t)ch * T, T);
+    /* an all-zero channel (reflectivity / ambient of a PointXYZI cloud)
+     * transforms and shifts to exact zeros: its calloc'ed block stays */
+    if (!all_zero(f + (size_t)ch * T, T)) {
+      po_r2c3(N, f + (size_t)ch * T, Fc + 2 * (size_t)ch * T);
+      po_fftshift(Fc + 2 * (size_t)ch * T, T);
+    }
+    if (!all_zero(g + (size_t)ch * T, T)) {
+      po_r2c3(N, g + (size_t)ch * T, Gc + 2 * (size_t)ch * T);
+      po_fftshift(Gc + 2 * (size_t)ch * T, T);
+    }
   }
   double* F = (double*)xcalloc(2 * (size_t)T, sizeof(double));
   double* G = (double*)xcalloc(2 * (size_t)T, sizeof(double));

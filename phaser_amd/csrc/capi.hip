@@ -523,22 +523,27 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     StageScope t(c, kS2Cells, s);
     launch_s2_cells(c->d_pts, c->d_npts, c->cap, ncl, n, c->cells, s);
   }
+  // SphRegistration: one sampled channel 0.6 intensity + 0.4 range per cloud
+  // (spherical-correlation.cc:103-112), transformed straight into the
+  // correlation's coefficients; no fusion, and reflectivity / ambient take
+  // no part in the rotation
+  const bool sph = c->cfg.method == PHZ_METHOD_SPH;
   {
     StageScope t(c, kS2NN, s);
     launch_s2_nn(c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq, c->cells, c->d_grids,
-                 c->tuning.nn_lanes, s, four ? c->d_extra : nullptr,
-                 four ? c->d_grids + 2 * ncl * nq : nullptr, c->nn_xcd);
+                 c->tuning.nn_lanes, s, four && !sph ? c->d_extra : nullptr,
+                 four && !sph ? c->d_grids + 2 * ncl * nq : nullptr, c->nn_xcd, sph);
   }
   {
     StageScope t(c, kS2Transform, s);
-    launch_fst(c->d_grids, (four ? 4 : 2) * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F,
-               c->d_coef, c->cfg.precision, s);
+    if (sph)
+      launch_fst(c->d_grids, ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_fused,
+                 c->cfg.precision, s);
+    else
+      launch_fst(c->d_grids, (four ? 4 : 2) * ncl, B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F,
+                 c->d_coef, c->cfg.precision, s);
   }
-  if (c->cfg.method == PHZ_METHOD_SPH) {
-    // SphRegistration: one channel 0.6 intensity + 0.4 range, no fusion
-    StageScope t(c, kFuse, s);
-    launch_blend(c->d_coef, 2 * BB, (uint32_t)BB, ncl, c->d_fused, s);
-  } else {  // spherical order {range, intensity, refl, amb} (refl = amb = 0 with
+  if (!sph) {  // spherical order {range, intensity, refl, amb} (refl = amb = 0 with
             // two channels), one launch for all clouds
     StageScope t(c, kFuse, s);
     ChannelSet cs = channels(c->d_coef, c->d_coef + BB, 4);
@@ -904,7 +909,11 @@ int phz_register_channels(phz_ctx* c, const float* target, const float* target_r
   TRY(exec_pairs(c, 1, false, s, rot_corr ? c->d_corr_rot : nullptr,
                  trans_corr ? c->d_corr_trans : nullptr));
   if (registered && Ps > 0) launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, c->d_reg, s);
-  TRY(fetch_results(c, 1, result, s));
+  // -4 (a failed peak-window check) still fills every output: the grids and
+  // the registered cloud are copied before the status is returned
+  const int rc = fetch_results(c, 1, result, s);
+  if (rc && rc != -4) return rc;
+  const std::string pair_msg = g_last_error;
   const int64_t n3 = (int64_t)c->no * c->no * c->no;
   if (rot_corr)
     TRY(copy_sync(c, rot_corr, c->d_corr_rot, sizeof(float) * n3, hipMemcpyDeviceToHost));
@@ -912,7 +921,8 @@ int phz_register_channels(phz_ctx* c, const float* target, const float* target_r
     TRY(copy_sync(c, trans_corr, c->d_corr_trans, sizeof(float) * c->T3, hipMemcpyDeviceToHost));
   if (registered && Ps > 0)
     TRY(copy_sync(c, registered, c->d_reg, sizeof(float4) * Ps, hipMemcpyDeviceToHost));
-  return 0;
+  if (rc) set_error(pair_msg);
+  return rc;
 }
 
 int phz_register_pairs_device(phz_ctx* c, int n, int shared_target, const float* const* d_targets,

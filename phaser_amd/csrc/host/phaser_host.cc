@@ -292,11 +292,31 @@ SphRegistration::SphRegistration(std::string&& alignment_algorithm,
 model::RegistrationResult SphRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
                                                               model::PointCloudPtr cloud_cur) {
   // sph-registration.cc:89-96: no rotation estimate is "not yet implemented"
-  // there; translation without rotation is not on this path either
   CHECK_TRUE(FLAGS_estimate_rotation, "not yet implemented.");
-  CHECK_TRUE(FLAGS_estimate_translation,
-             "estimate_translation = false is not part of this build");
-  return SphOptRegistration::registerPointCloud(cloud_prev, cloud_cur);
+  // the refine_rot flags are read on every registration (sph-registration.cc:
+  // 108-113): a change since the last call selects the context of the new mask
+  const int mask = refine_mask();
+  if (mask != skip_rot_axes_) {
+    skip_rot_axes_ = mask;
+    ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
+                          sph_pad_, method_, skip_rot_axes_);
+    check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
+  }
+  if (FLAGS_estimate_translation)
+    return SphOptRegistration::registerPointCloud(cloud_prev, cloud_cur);
+  // rotation only (sph-registration.cc:91-93): the registered cloud is the
+  // source rotated by the estimate, with no positional estimate.  The device
+  // pipeline always runs both stages; its translation is not used.
+  model::PointCloud rotated = cloud_cur->clone();
+  model::RegistrationResult full = SphOptRegistration::registerPointCloud(cloud_prev, cloud_cur);
+  rotated.transformPointCloud(full.rotation_transform);
+  model::RegistrationResult result(std::move(rotated));
+  result.setRotation(full.getRotation());
+  result.setRotationCorrelation(full.getRotationCorrelation());
+  result.rotation_index = full.rotation_index;
+  result.rotation_transform = full.rotation_transform;
+  result.setRotUncertaintyEstimate(full.getRotUncertaintyEstimate());
+  return result;
 }
 
 model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
@@ -345,6 +365,7 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   result.setRotationCorrelation(std::vector<double>(rot.begin(), rot.end()));
   result.rotation_index = r.rot_index;
   result.translation_index = r.trans_index;
+  for (int i = 0; i < 16; ++i) result.rotation_transform[i] = r.T_rot[i];
   std::array<double, 4> mode, omega;
   std::array<double, 16> M;
   std::array<double, 3> mu;

@@ -45,9 +45,12 @@ int nn_auto_lanes(int nclouds);
 // grids of the nearest point's values (zero where none is accepted).
 // xcd: order the workgroups so that each XCD works through its own clouds
 // (needs a multiple of 8 clouds; otherwise the plain 2-D grid).
+// blend: one grid per cloud, 0.6 intensity + 0.4 range (SphRegistration's
+// channel, spherical-correlation.cc:103-112), at grids + cloud * n*n.
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
-                  const float2* extra = nullptr, float* xgrids = nullptr, bool xcd = false);
+                  const float2* extra = nullptr, float* xgrids = nullptr, bool xcd = false,
+                  bool blend = false);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
@@ -68,12 +71,6 @@ struct ChannelSet {
 // are fused in one launch into out[set][count].
 void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,
                  hipStream_t s, int nsets = 1, int64_t set_stride = 0);
-// SphRegistration's single channel (spherical-correlation.cc:103-112):
-// out[set] = 0.6 coef(intensity) + 0.4 coef(range) of each set's [range,
-// intensity] coefficients (set_stride apart), by linearity of the S^2
-// transform the coefficients of the blended samples.
-void launch_blend(const float2* coef, int64_t set_stride, uint32_t count, int nsets, float2* out,
-                  hipStream_t s);
 // keys[pair] = max over partials[pair*count .. +count)
 void launch_reduce_keys(const unsigned long long* partials, int count, int npairs,
                         unsigned long long* keys, hipStream_t s);
@@ -255,6 +252,18 @@ void launch_spatial_forward(int N, int nclouds, const float* sumI, const float* 
 void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftPlan& plan,
                                   const float2* tw, float2* spec, const SpatialTiles& tiles,
                                   hipStream_t s);
+// the same for two channels at separate addresses a and b (spec[0], spec[1]),
+// with the default column tiles
+void launch_spatial_forward_two(int N, const float* a, const float* b, const FftPlan& plan,
+                                const float2* tw, float2* spec, hipStream_t s);
+// SpatialCorrelation::correlateSignals on one channel pair (no padding, no
+// fusion): twiddles of N computed into tw, spectra of f and g into spec
+// (2 N^2(N/2+1)), cross spectrum and inverse into work (N^2(N/2+1)), the grid
+// into corr (nullable, N^3), argmax key into keys[0] with
+// spatial_c2r_blocks(N) partials from keys + 1 (noalloc.hip)
+void launch_spatial_single(int N, const FftPlan& plan, const float* f, const float* g,
+                           float2* tw, float2* spec, float2* work, float* corr,
+                           unsigned long long* keys, hipStream_t s);
 // fused cross spectrum + inverse 3-D c2r + argmax for npairs pairs.  F
 // channels and G channels (null = zero channel) in the reference's spatial
 // order; pair p reads its channels at ch[c] + p*stride (stride 0: shared).

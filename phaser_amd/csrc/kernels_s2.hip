@@ -269,7 +269,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
-                     float* __restrict__ xgrids, int xcd_blocks) {
+                     float* __restrict__ xgrids, int xcd_blocks, int blend) {
   constexpr int kBatch = 4;  // point loads in flight per lane
   // xcd_blocks > 0: a 1-D grid of xcd_blocks query blocks per cloud, where
   // consecutive workgroups go to the 8 XCDs in turn: XCD x takes clouds
@@ -440,8 +440,12 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       r = v.x;
       it = v.y;
     }
-    grids[(int64_t)c * 2 * nq + q] = r;
-    grids[(int64_t)c * 2 * nq + nq + q] = it;
+    if (blend) {  // SphRegistration's one channel, spherical-correlation.cc:103-112
+      grids[(int64_t)c * nq + q] = 0.6f * it + 0.4f * r;
+    } else {
+      grids[(int64_t)c * 2 * nq + q] = r;
+      grids[(int64_t)c * 2 * nq + nq + q] = it;
+    }
     if (extra) {  // the nearest point's (reflectivity, ambient), point-cloud.cc:152-155
       float2 e = make_float2(0.f, 0.f);
       if (bpos >= 0 && best <= max_sq) e = extra[c * cap + __float_as_int(sp[bpos].w)];
@@ -581,7 +585,7 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
 template <int L>
 static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                         float max_sq, const CellBuffers& cb, float* grids, const float2* extra,
-                        float* xgrids, bool xcd, hipStream_t s) {
+                        float* xgrids, bool xcd, bool blend, hipStream_t s) {
   const int nq = n * n, qpb = 256 / L;
   const int nqb = (nq + qpb - 1) / qpb;
   // the XCD-ordered grid needs whole groups of 8 clouds
@@ -589,21 +593,21 @@ static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, con
   const dim3 gq = by_xcd ? dim3((unsigned)(nqb * nclouds)) : dim3((unsigned)nqb, nclouds);
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0);
+                     cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend);
 }
 
 int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
 
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
-                  const float2* extra, float* xgrids, bool xcd) {
+                  const float2* extra, float* xgrids, bool xcd, bool blend) {
   if (lanes == 0) lanes = nn_auto_lanes(nclouds);
   switch (lanes) {
-    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
-    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
-    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
-    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
-    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, s); break;
+    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
+    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
+    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
+    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
+    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
   }
 }
 
@@ -619,22 +623,6 @@ void launch_fst(const float* sig, int S, int bw, const float* legendre, const in
   }
   hipLaunchKernelGGL(k_legendre, dim3(bw, S), dim3(th), n * sizeof(float2), s, F, n, bw, legendre,
                      leg_off_dev, coef);
-}
-
-__global__ void __launch_bounds__(256) k_blend(const float2* __restrict__ coef, int64_t set_stride,
-                                               uint32_t count, float2* __restrict__ out) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= count) return;
-  const float2* c = coef + (int64_t)blockIdx.y * set_stride;
-  const float2 r = c[i], t = c[count + i];
-  out[(int64_t)blockIdx.y * count + i] =
-      make_float2(0.6f * t.x + 0.4f * r.x, 0.6f * t.y + 0.4f * r.y);
-}
-
-void launch_blend(const float2* coef, int64_t set_stride, uint32_t count, int nsets, float2* out,
-                  hipStream_t s) {
-  hipLaunchKernelGGL(k_blend, dim3((count + 255) / 256, nsets), dim3(256), 0, s, coef, set_stride,
-                     count, out);
 }
 
 void launch_fuse(ChannelSet cs, uint32_t count, const FusionRuns& runs, float2* out,

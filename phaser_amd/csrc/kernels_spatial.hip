@@ -1535,6 +1535,15 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
   launch_spatial_fft_fwd(N, nch, spec, plan, tw, tiles, s);
 }
 
+void launch_spatial_forward_two(int N, const float* a, const float* b, const FftPlan& plan,
+                                const float2* tw, float2* spec, hipStream_t s) {
+  const int64_t H = (int64_t)N * N * (N / 2 + 1), T = (int64_t)N * N * N;
+  hipLaunchKernelGGL(k_spat_r2c<true>, dim3((N * N + 15) / 16, 1), dim3(256),
+                     2 * 16 * N * sizeof(float2), s, N, a, b, (const uint32_t*)nullptr, T, plan,
+                     tw, spec, 0, H);
+  launch_spatial_fft_fwd(N, 2, spec, plan, tw, SpatialTiles{}, s);
+}
+
 void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
                             int64_t g_stride, const FusionRuns& runs, bool fuse,
                             const FftPlan& plan, const float2* tw, float2* work, hipStream_t s) {

@@ -1,10 +1,14 @@
 // host_driver.cc -- registers two point clouds through the C++ mirror of the
 // reference API (phaser_core::CloudController("sph-opt"), the path of
 // phaser_core/test/driver.cc:30-57) and prints the result as JSON.
-// usage: host_driver target source bandwidth n_voxels [bingham gaussian]
-// where a .bin file is float32 (x, y, z, intensity) quadruples and a .ply
+// usage: host_driver target source bandwidth n_voxels [bingham gaussian
+// [method refine_rot_xyz [refine_rot_xyz2 estimate_translation2]]]: with the
+// last two, the same controller registers the pair a second time after the
+// refine_rot flags and FLAGS_estimate_translation were changed (one JSON
+// line per registration); a .bin file is float32 (x, y, z, intensity) quadruples and a .ply
 // file is read by model::PointCloud::readFromFile (with reflectivity and
 // ambient when it has them).
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +16,8 @@
 #include <vector>
 
 #include "phaser_core.hpp"
+
+static void print(const model::RegistrationResult& r);
 
 static model::PointCloudPtr load(const char* path) {
   const size_t len = std::strlen(path);
@@ -33,7 +39,7 @@ static model::PointCloudPtr load(const char* path) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 5 && argc != 7 && argc != 9) {
+  if (argc != 5 && argc != 7 && argc != 9 && argc != 11) {
     std::fprintf(stderr,
                  "usage: %s target.bin source.bin bandwidth n_voxels "
                  "[bingham_peak_neighbors gaussian_peak_neighbors [method refine_rot_xyz]]\n",
@@ -48,7 +54,7 @@ int main(int argc, char** argv) {
     FLAGS_gaussian_peak_neighbors = std::atoi(argv[6]);
   }
   std::string method = "sph-opt";
-  if (argc == 9) {  // method ("sph" or "sph-opt") and refine_rot_x/y/z as "101"
+  if (argc >= 9) {  // method ("sph" or "sph-opt") and refine_rot_x/y/z as "101"
     method = argv[7];
     FLAGS_refine_rot_x = argv[8][0] == '1';
     FLAGS_refine_rot_y = argv[8][1] == '1';
@@ -57,13 +63,32 @@ int main(int argc, char** argv) {
   auto target = load(argv[1]);
   auto source = load(argv[2]);
   phaser_core::CloudController controller(std::move(method));
-  model::RegistrationResult r = controller.registerPointCloud(target, source);
+  print(controller.registerPointCloud(target, source));
+  if (argc == 11) {
+    FLAGS_refine_rot_x = argv[9][0] == '1';
+    FLAGS_refine_rot_y = argv[9][1] == '1';
+    FLAGS_refine_rot_z = argv[9][2] == '1';
+    FLAGS_estimate_translation = argv[10][0] == '1';
+    print(controller.registerPointCloud(target, source));
+  }
+  return 0;
+}
+
+static void print(const model::RegistrationResult& r) {
   const auto s = r.getStateAsVec();
   const auto& corr = r.getRotationCorrelation();
   double cmax = 0;
   for (double v : corr) cmax = v > cmax ? v : cmax;
   const auto omega = r.getRotUncertaintyEstimate()->getOmega();
-  const auto cov = r.getPosUncertaintyEstimate()->getCov();
+  // no positional estimate when the translation was not estimated
+  std::array<double, 9> cov{};
+  if (r.getPosUncertaintyEstimate()) cov = r.getPosUncertaintyEstimate()->getCov();
+  double reg_sum = 0.0;
+  const auto& reg = *r.getRegisteredCloud();
+  for (size_t i = 0; i < reg.size(); ++i) {
+    const auto& p = reg.pointAt(i);
+    reg_sum += (double)p.x + (double)p.y + (double)p.z;
+  }
   // the covariance is NaN where a window weight is negative (as in the
   // reference's Gaussian); JSON as Python reads it
   auto num = [](double v) {
@@ -74,9 +99,11 @@ int main(int argc, char** argv) {
   };
   std::printf("{\"rot_index\": %u, \"trans_index\": %u, \"state\": [%.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g, %.17g], "
               "\"registered_points\": %zu, \"rot_corr_size\": %zu, \"rot_corr_max\": %.9g, "
-              "\"rot_omega\": [%.17g, %.17g, %.17g, %.17g], \"trans_cov_diag\": [%s, %s, %s]}\n",
+              "\"rot_omega\": [%.17g, %.17g, %.17g, %.17g], \"trans_cov_diag\": [%s, %s, %s], "
+              "\"has_pos_estimate\": %d, \"registered_sum\": %.17g}\n",
               r.rotation_index, r.translation_index, s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7],
               r.getRegisteredCloud()->size(), corr.size(), cmax, omega[0], omega[1], omega[2],
-              omega[3], num(cov[0]).c_str(), num(cov[4]).c_str(), num(cov[8]).c_str());
-  return 0;
+              omega[3], num(cov[0]).c_str(), num(cov[4]).c_str(), num(cov[8]).c_str(),
+              r.getPosUncertaintyEstimate() ? 1 : 0, reg_sum);
+  std::fflush(stdout);
 }

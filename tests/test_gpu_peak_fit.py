@@ -133,8 +133,38 @@ def test_bingham_window_past_the_grid_fails():
             ctx.register(tgt, src)
         with pytest.raises(oracle.PeakWindowError):
             oracle.peak_window(idx, (2 * B) ** 3, k, "bingham")
+        # -4 still fills every output: the result, both grids, the registered cloud
+        failed = _raw_register(ctx, tgt, src, B, N)
+    assert failed["rc"] == -4 and failed["status"] == 1
     with Context(bandwidth=B, n_voxels=N, max_points=200000,
                  bingham_peak_neighbors=k - 1) as ctx:
-        r = ctx.register(tgt, src, grids=True)
+        r = ctx.register(tgt, src, grids=True, registered=True)
         assert r["rot_index"] == idx
         _check_fit(r, r["rot_corr"], r["trans_corr"], B, N, k - 1, 0)
+    assert failed["rot_index"] == idx and failed["trans_index"] == r["trans_index"]
+    np.testing.assert_array_equal(failed["rot_corr"], r["rot_corr"])
+    np.testing.assert_array_equal(failed["trans_corr"], r["trans_corr"])
+    # the failed fit leaves the argmax pose
+    np.testing.assert_allclose(failed["quat"], oracle.quat_from_zyz(oracle.zyz_from_index(idx, B)),
+                               atol=1e-15)
+    assert np.all(np.isfinite(failed["registered"]))
+    np.testing.assert_array_equal(failed["registered"][:, 3], src[:, 3])
+
+
+def _raw_register(ctx, tgt, src, B, N):
+    """phz_register_channels through ctypes, keeping the outputs of a
+    nonzero return."""
+    import ctypes as C
+    from phaser_amd.capi import Result, lib
+    t = np.ascontiguousarray(tgt, np.float32)
+    s = np.ascontiguousarray(src, np.float32)
+    res = Result()
+    rot = np.full((2 * B) ** 3, np.nan, np.float32)
+    tr = np.full(N ** 3, np.nan, np.float32)
+    reg = np.full_like(s, np.nan)
+    ptr = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    rc = lib().phz_register_channels(ctx.h, ptr(t), None, t.shape[0], ptr(s), None, s.shape[0],
+                                     C.byref(res), ptr(rot), ptr(tr), ptr(reg))
+    out = res.as_dict()
+    out.update(rc=rc, rot_corr=rot, trans_corr=tr, registered=reg)
+    return out

@@ -121,3 +121,39 @@ def test_reference_default_sizes_match_oracle(po, method):
         r = ctx.register(tgt, src, grids=True)
     o = po.register(tgt, src, bw=75, n_voxels=100, want_grids=True, method=method)
     _check(r, o)
+
+
+def test_refine_flags_and_rotation_only_are_read_per_registration(tmp_path):
+    """One CloudController("sph") registers twice: first with refine_rot_y
+    false, then with every axis refined and FLAGS_estimate_translation false
+    (sph-registration.cc:91-93, 108-113).  The second result is the
+    rotation-only one: the C ABI's rotation with skip_rot_axes 0, no
+    positional estimate, and the source rotated by T_rot but not moved."""
+    from phaser_amd import synth
+    from phaser_amd.capi import Context
+    src, tgt, _, _ = synth.pair(seed=4, rings=64, azimuths=1024)
+    paths = []
+    for name, cl in (("target", tgt), ("source", src)):
+        p = tmp_path / f"{name}.bin"
+        np.ascontiguousarray(cl, np.float32).tofile(p)
+        paths.append(str(p))
+    print("C++ driver, CloudController(\"sph\"), two registrations", flush=True)
+    r = subprocess.run([DRIVER, paths[0], paths[1], "32", "64", "0", "0", "sph", "101", "111",
+                        "0"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    first, second = [json.loads(l) for l in r.stdout.strip().splitlines()[-2:]]
+    with Context(bandwidth=32, n_voxels=64, max_points=262144, method=1, skip_rot_axes=2) as ctx:
+        masked = ctx.register(tgt, src)
+    with Context(bandwidth=32, n_voxels=64, max_points=262144, method=1) as ctx:
+        full = ctx.register(tgt, src)
+    assert first["rot_index"] == masked["rot_index"] and first["has_pos_estimate"] == 1
+    np.testing.assert_allclose(first["state"][5:], masked["translation"], atol=1e-9)
+    assert second["rot_index"] == full["rot_index"] and second["has_pos_estimate"] == 0
+    np.testing.assert_allclose(second["state"][:4], full["quat"], atol=1e-12)
+    assert second["state"][5:] == [0.0, 0.0, 0.0]
+    T = np.asarray(full["T_rot"], np.float32).reshape(4, 4)
+    p = np.ascontiguousarray(src[:, :3], np.float32)
+    rot = np.stack([((T[i, 0] * p[:, 0] + T[i, 1] * p[:, 1]) + T[i, 2] * p[:, 2]) + T[i, 3]
+                    for i in range(3)], 1).astype(np.float32)
+    np.testing.assert_allclose(second["registered_sum"], float(rot.astype(np.float64).sum()),
+                               rtol=1e-12)
