@@ -5,15 +5,19 @@ Workload: seeded synthetic scan pairs with 128 rings x 1024 azimuths (about
 a translation in +-5 m, cropped to 70% of its azimuth ring, noise 2 cm;
 B = 64 (128 x 128 S^2 grid, 128^3 SO(3) bins), 128^3 voxels over +-50 m.
 One step registers `--batch` independent pairs whose clouds are already in
-HBM (one phz_register_pairs_device call, correlation grids written out), so
-value = batch * steps / time.  Latency p50/p99 comes from single-pair calls.
-The argmax-match rate compares the rotation bin and translation voxel of the
-first --cpu-sample pairs with the FP64 oracle, which is also timed on up to
---cpu-threads host threads (one pair per thread) as the CPU baseline.  The
-roofline names the kernel of the stage with the largest device time: its
-algorithmic bytes per launch over its mean launch duration (HIP events on
-the context stream), against the 8 TB/s HBM peak; the streaming copy
-kernel's rate is reported beside it.
+HBM, split over `--streams` contexts (one phz_register_pairs_device call per
+context, both correlation grids of every pair written to HBM), so value =
+batch * steps / time.  Latency p50/p99 comes from single-pair calls.  The
+argmax-match rate registers the 1000 seeded pairs of the committed FP64
+oracle fixture (tests/golden/parity_b64_n128_mode0.npz) and compares the
+rotation bin and translation voxel of each; the FP64 oracle itself is timed
+on --cpu-sample pairs over up to --cpu-threads host threads (one pair per
+thread) as the CPU baseline.  The roofline names the kernel of the stage
+with the largest device time: its algorithmic bytes per launch over its mean
+launch duration (HIP events on the context stream), against the 8 TB/s HBM
+peak; the streaming copy kernel's rate is reported beside it.  `served`:
+a short Poisson sweep of the serving mix through the request loop, per
+precision form.
 
 Prints one JSON line.
 """
@@ -47,8 +51,10 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False
     nt = 1 if shared else pairs
     clouds = nt + pairs
     return {
-        # sorted points + values read once, grid queries once, C grids written
-        "s2_nn": (f"k_nn<{nn_lanes}>", clouds * (24 * P_mean + 4 * C * n * n) + 16 * n * n),
+        # S^2 sampling per cloud: its points read once (16 B each) and its C
+        # grids written once (the query grid and the kernel's own sorted
+        # layout are not counted)
+        "s2_nn": (f"k_nn<{nn_lanes}>", clouds * (16 * P_mean + 4 * C * n * n)),
         # Wigner table once per batch + fused coefficients in, S half-plane out
         "so3_synth": ("k_so3_synth", 4 * n * B * (B + 1) * (B + 2) // 6
                       + clouds * 8 * B * B + pairs * 8 * n * B * (2 * B - 1)),
@@ -79,11 +85,14 @@ FORM_NAMES = {0: "plain fp32", 1: "split fp16 mfma contractions"}
 LAUNCHES = {"spatial_fft_fwd": 2, "voxel_scatter": 2}
 
 
-def concurrent_throughput(args, pairs, max_pts, tuning):
+def concurrent_throughput(args, pairs, max_pts, tuning, grids=True):
     """The batch split over args.streams contexts (each with its own HIP
     stream and buffers), driven from one host thread each so that their
     kernels overlap on the GPU; returns (registrations/s, seconds) over
-    args.steps steps of every context, timed from a common start."""
+    args.steps steps of every context, timed from a common start.  grids:
+    every pair's rotation and translation grids are written to HBM (the
+    reference returns the rotation correlation with its result,
+    sph-opt-registration.cc:71)."""
     import threading
     from phaser_amd.capi import Context
     S, P = args.streams, len(pairs)
@@ -99,19 +108,23 @@ def concurrent_throughput(args, pairs, max_pts, tuning):
             c.set_graphs(True)
         ds = [c.device_array(pairs[i][0]) for i in idx]
         dt = [c.device_array(pairs[i][1]) for i in idx]
-        subs.append((c, ds, dt, [len(pairs[i][0]) for i in idx], [len(pairs[i][1]) for i in idx]))
+        outs = ([c.device_empty(len(idx) * (2 * B) ** 3 * 4), c.device_empty(len(idx) * N ** 3 * 4)]
+                if grids else [])
+        subs.append((c, ds, dt, [len(pairs[i][0]) for i in idx], [len(pairs[i][1]) for i in idx],
+                     outs))
     start = threading.Barrier(S + 1)
     done = threading.Barrier(S + 1)
     errors = []
 
-    def run(c, ds, dt, ps, pt):
+    def run(c, ds, dt, ps, pt, outs):
+        kw = dict(d_rot=outs[0].ptr, d_trans=outs[1].ptr) if outs else {}
         try:
             for _ in range(args.warmup):
-                c.register_pairs_device([a.ptr for a in dt], pt, [a.ptr for a in ds], ps)
+                c.register_pairs_device([a.ptr for a in dt], pt, [a.ptr for a in ds], ps, **kw)
             c.synchronize()
             start.wait()
             for _ in range(args.steps):
-                c.register_pairs_device([a.ptr for a in dt], pt, [a.ptr for a in ds], ps)
+                c.register_pairs_device([a.ptr for a in dt], pt, [a.ptr for a in ds], ps, **kw)
             c.synchronize()
         except Exception as e:  # reported after the join
             errors.append(e)
@@ -128,8 +141,8 @@ def concurrent_throughput(args, pairs, max_pts, tuning):
     elapsed = time.perf_counter() - t0
     for th in threads:
         th.join()
-    for c, ds, dt, _, _ in subs:
-        for a in ds + dt:
+    for c, ds, dt, _, _, outs in subs:
+        for a in ds + dt + outs:
             a.free()
         c.close()
     if errors:
@@ -214,27 +227,32 @@ def load_traffic(kernel):
 SERVE_SHAPES = [(32, 64), (64, 128), (96, 192), (128, 256)]
 
 
-def serve_bench(args):
+def serve_pool(args):
+    from phaser_amd import synth
+    with cf.ThreadPoolExecutor(max_workers=16) as ex:
+        return {sh: list(ex.map(lambda seed: synth.mixed_pair(seed, sh[0]), range(args.pool)))
+                for sh in SERVE_SHAPES}
+
+
+def serve_sweep(args, pool, precision, rates, duration):
     """Poisson arrivals of scan pairs with bandwidth B in {32, 64, 96, 128}
     (voxel grids 64^3 .. 256^3, overlap 40-90%, noise 1-5 cm, 5% outliers)
-    through the continuous-batching request loop, at each offered rate of
-    --rates: throughput and p50/p99 latency (submit -> result)."""
-    from phaser_amd import serve, synth
+    through the continuous-batching request loop, at each offered rate:
+    throughput and p50/p99 latency (submit -> result).  Returns (sweep, server
+    statistics)."""
+    from phaser_amd import serve
     rng = np.random.default_rng(0)
-    pool = {sh: [synth.mixed_pair(seed, sh[0]) for seed in range(args.pool)]
-            for sh in SERVE_SHAPES}
     max_pts = max(max(len(s), len(t)) for v in pool.values() for s, t, _, _ in v)
-    rates = [float(x) for x in args.rates.split(",")]
     sweep = []
     with serve.Server(max_batch=args.serve_batch, contexts_per_shape=2,
                       hbm_budget=int(args.hbm_budget_gib * (1 << 30)),
                       max_points=max_pts, autotune=bool(args.autotune),
-                      precision=args.precision) as srv:
+                      precision=precision) as srv:
         for sh in SERVE_SHAPES:  # build the contexts before timing
             src, tgt, _, _ = pool[sh][0]
             srv.wait(srv.submit(tgt, src, bandwidth=sh[0], n_voxels=sh[1]))
         for lam in rates:
-            n_req = max(int(lam * args.duration), 8)
+            n_req = max(int(lam * duration), 8)
             tickets, shapes = [], []
             t_start = time.perf_counter()
             next_t = t_start
@@ -262,8 +280,22 @@ def serve_bench(args):
                 "failed": int(sum(g["status"] != 0 for g in got)),
                 "p50_ms_by_bandwidth": {str(b): round(float(np.percentile(v, 50)), 3)
                                         for b, v in sorted(per_b.items())}})
-            print(f"serve rate {lam}: {sweep[-1]}", file=sys.stderr, flush=True)
+            print(f"serve {FORM_NAMES[precision]} rate {lam}: {sweep[-1]}", file=sys.stderr,
+                  flush=True)
         st = srv.stats()
+    return sweep, {k: int(v) for k, v in st.items()}
+
+
+def serve_summary(sweep):
+    best = max(sweep, key=lambda r: r["throughput"])
+    return {"value": best["throughput"], "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"],
+            "at_rate": best["rate_offered"], "sweep": sweep}
+
+
+def serve_bench(args):
+    pool = serve_pool(args)
+    rates = [float(x) for x in args.rates.split(",")]
+    sweep, st = serve_sweep(args, pool, args.precision, rates, args.duration)
     best = max(sweep, key=lambda r: r["throughput"])
     out = {
         "metric": "served registrations/sec, mixed B in {32,64,96,128}, Poisson arrivals",
@@ -275,7 +307,7 @@ def serve_bench(args):
                    "max_batch": args.serve_batch, "contexts_per_shape": 2,
                    "hbm_budget_gib": args.hbm_budget_gib, "form": FORM_NAMES[args.precision]},
         "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "sweep": sweep,
-        "server": {k: int(v) for k, v in st.items()},
+        "server": st,
     }
     print(json.dumps(out), flush=True)
 
@@ -313,6 +345,11 @@ def main():
     ap.add_argument("--pool", type=int, default=6, help="serve: pairs per bandwidth")
     ap.add_argument("--serve-batch", type=int, default=8)
     ap.add_argument("--hbm-budget-gib", type=float, default=64.0)
+    ap.add_argument("--parity-pairs", type=int, default=1000,
+                    help="argmax match over this many fixture pairs (0: the timed CPU sample)")
+    ap.add_argument("--serve-rates", default="500,2000",
+                    help="batch mode: offered rates of the short served sweep per form ('' = off)")
+    ap.add_argument("--serve-duration", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); registrations are "
@@ -363,8 +400,12 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     value = P * args.steps / elapsed
+    one_stream = {"value": round(value, 2), "ms_per_step": round(elapsed / args.steps * 1e3, 4)}
+    no_grids = None
     if args.streams > 1:
-        value, elapsed = concurrent_throughput(args, pairs, max_pts, tuning)
+        value, elapsed = concurrent_throughput(args, pairs, max_pts, tuning, grids=True)
+        v2, e2 = concurrent_throughput(args, pairs, max_pts, tuning, grids=False)
+        no_grids = {"value": round(v2, 2), "ms_per_step": round(e2 / args.steps * 1e3, 4)}
 
     # single-pair latency (p50 / p99)
     lat = []
@@ -408,10 +449,12 @@ def main():
             ctx2.set_graphs(True)
         d_src2 = [ctx2.device_array(s) for s, _, _, _ in pairs]
         d_tgt2 = [ctx2.device_array(t) for _, t, _, _ in pairs]
+        d_rot2 = ctx2.device_empty(P * (2 * B) ** 3 * 4)
+        d_trans2 = ctx2.device_empty(P * N ** 3 * 4)
 
         def step2():
             return ctx2.register_pairs_device([a.ptr for a in d_tgt2], Pt, [a.ptr for a in d_src2],
-                                              Ps)
+                                              Ps, d_rot=d_rot2.ptr, d_trans=d_trans2.ptr)
 
         for _ in range(args.warmup):
             step2()
@@ -429,7 +472,7 @@ def main():
                  "ms_per_step": el2 / args.steps * 1e3, "res": res2,
                  "stage_ms": {k: round(st2[k][0] / max(st2[k][1], 1), 4)
                               for k in ("s2_transform", "so3_synth")}}
-        for a in d_src2 + d_tgt2:
+        for a in d_src2 + d_tgt2 + [d_rot2, d_trans2]:
             a.free()
         ctx2.close()
 
@@ -472,7 +515,43 @@ def main():
                                  for k, o in enumerate(outs))
         assert o0["rot_index"] == outs[0]["rot_index"]
 
-    variants = variant_throughput(args, pairs, max_pts, tuning) if args.variants else None
+    variants = variant_throughput(args, pairs, max_pts, tuning) if args.variants else {}
+    variants["one_stream_with_grids"] = one_stream
+    if no_grids is not None:
+        variants[f"{args.streams}_streams_no_grids"] = no_grids
+
+    # argmax match over the committed FP64 oracle fixture (seeds 0..999 of the
+    # same workload), both forms; falls back to the timed sample above
+    from phaser_amd import parity
+    sweep_res = {}
+    fx_path = parity.fixture_path(ROOT, 0)
+    if args.parity_pairs and os.path.exists(fx_path):
+        fx = parity.load(fx_path)
+        for prec in ([args.precision, 1 - args.precision] if args.compare_forms
+                     else [args.precision]):
+            r = parity.sweep(fx, precision=prec, limit=args.parity_pairs)
+            sweep_res[prec] = {"matched": r["matched"], "pairs": r["pairs"],
+                               "rate": round(r["rate"], 5), "rot_matched": r["rot_matched"],
+                               "trans_matched": r["trans_matched"], "near_ties": r["near_ties"],
+                               "mismatches": r["mismatches"][:20],
+                               "fixture": os.path.relpath(fx_path, ROOT)}
+            print(f"argmax sweep {FORM_NAMES[prec]}: {sweep_res[prec]['matched']}/"
+                  f"{sweep_res[prec]['pairs']}", file=sys.stderr, flush=True)
+    if args.precision in sweep_res:
+        argmax_match = sweep_res[args.precision]
+    else:
+        argmax_match = {"matched": match, "pairs": S, "rate": round(match / max(S, 1), 4),
+                        "fixture": None}
+
+    # served forms: a short Poisson sweep of the serving mix per form
+    served = None
+    if args.serve_rates:
+        pool = serve_pool(args)
+        rates = [float(x) for x in args.serve_rates.split(",")]
+        served = {}
+        for prec in ([0, 1] if args.compare_forms else [args.precision]):
+            sw, stats = serve_sweep(args, pool, prec, rates, args.serve_duration)
+            served[FORM_NAMES[prec]] = dict(serve_summary(sw), server=stats)
 
     out = {
         "metric": "registrations/sec at B=64 + 128^3 voxels",
@@ -494,19 +573,23 @@ def main():
                    "hip_graphs": bool(args.graphs)},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
-        "argmax_match": {"matched": match, "pairs": S, "rate": round(match / max(S, 1), 4)},
+        "argmax_match": argmax_match,
         "stage_ms_per_step": stage_ms,
         "forms": None if other is None else {
             FORM_NAMES[args.precision]: {"value": round(value, 2),
                                          "ms_per_step": round(elapsed / args.steps * 1e3, 4),
-                                         "argmax_match": round(match / max(S, 1), 4),
+                                         "argmax_match": sweep_res.get(args.precision, {}).get(
+                                             "rate", round(match / max(S, 1), 4)),
                                          "stage_ms": {k: stage_ms.get(k) for k in
                                                       ("s2_transform", "so3_synth")}},
             FORM_NAMES[other["precision"]]: {
                 "value": round(other["value"], 2), "ms_per_step": round(other["ms_per_step"], 4),
-                "argmax_match": round(other.get("match", 0) / max(S, 1), 4),
+                "argmax_match": sweep_res.get(other["precision"], {}).get(
+                    "rate", round(other.get("match", 0) / max(S, 1), 4)),
+                "one_stream_vs_this_form_one_stream": round(other["value"] / one_stream["value"], 4),
                 "stage_ms": other["stage_ms"]}},
         "variants": variants,
+        "served": served,
         "roofline": {"bound": "hbm", "kernel": kname, "stage": dom, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),

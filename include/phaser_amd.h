@@ -34,13 +34,19 @@ typedef struct {
   int32_t bandwidth;              /* phaser_core_spherical_bandwidth */
   int32_t spherical_zero_padding; /* phaser_core_spherical_zero_padding: the SO(3)
                                      correlation on the grid of Bo = bandwidth + padding
-                                     (<= 128) from coefficients of degree < bandwidth
+                                     (<= 160) from coefficients of degree < bandwidth
                                      (spherical-correlation.cc:21-27, 248-278); rotation
                                      grids and indices are then (2 Bo)^3 */
   int32_t n_voxels;               /* phaser_core_spatial_n_voxels */
   int32_t discretize_lower;       /* phaser_core_spatial_discretize_lower */
   int32_t discretize_upper;       /* phaser_core_spatial_discretize_upper */
-  int32_t spatial_zero_padding;   /* phaser_core_spatial_zero_padding (0 only) */
+  int32_t spatial_zero_padding;   /* phaser_core_spatial_zero_padding: the translation grid
+                                     of Np = n_voxels + 2 padding per axis (Np <= 256;
+                                     spatial-correlation.cc:11-61,221-230), trans_corr
+                                     outputs then hold Np^3 floats, trans_index decodes
+                                     with Np (phase-aligner.cc:137-139) and the argmax
+                                     skips entries past the reference's truncated padded
+                                     size (spatial-correlation.cc:17-24) */
   int32_t low_pass_lower;         /* phaser_core_spatial_low_pass_lower_bound */
   int32_t low_pass_upper;         /* phaser_core_spatial_low_pass_upper_bound */
   int64_t max_points;             /* capacity per cloud */
@@ -85,7 +91,8 @@ typedef struct {
  * do not sum to a positive value) makes the call return -4 (phz_result.status). */
 typedef struct {
   uint32_t rot_index;     /* argmax of the (2B)^3 SO(3) grid [beta][alpha][gamma] */
-  uint32_t trans_index;   /* argmax of the N^3 grid, x*N + y + N^2*z */
+  uint32_t trans_index;   /* argmax of the Np^3 grid (Np = N + 2 spatial padding),
+                             x*Np + y + Np^2*z */
   float rot_value;        /* |C|^2 at rot_index */
   float trans_value;      /* correlation at trans_index */
   double zyz[3];          /* of the argmax, rotation-utils.cc:197-207 */
@@ -127,7 +134,8 @@ int phz_get_config(const phz_ctx* ctx, phz_config* out);
 /* ---- end to end -------------------------------------------------------- */
 /* SphOptRegistration::registerPointCloud(cloud_prev = target,
  * cloud_cur = source) (sph-opt-registration.cc:34-47).  rot_corr ((2Bo)^3
- * floats, |C|^2), trans_corr (N^3 floats) and registered (Ps x 4 floats,
+ * floats, |C|^2), trans_corr ((N + 2 spatial padding)^3 floats) and
+ * registered (Ps x 4 floats,
  * the registered source) are optional outputs (NULL to skip). */
 int phz_register(phz_ctx* ctx, const float* target, int64_t Pt, const float* source,
                  int64_t Ps, phz_result* result, float* rot_corr, float* trans_corr,
@@ -210,9 +218,21 @@ int phz_spatial_correlate(phz_ctx* ctx, int C, const float* f, const float* g, f
                           uint32_t* argmax);
 /* SpatialCorrelation::correlateSignals (spatial-correlation.cc:163-206),
  * single channel, no padding.  n_voxels may differ from the context's
- * (any size whose factors are 2, 3 and 5, <= 256). */
+ * (any size <= 256). */
 int phz_spatial_correlate_single(phz_ctx* ctx, int n_voxels, const float* f, const float* g,
                                  float* corr, uint32_t* argmax);
+/* SpatialCorrelation(n_voxels, zero_padding)::correlateSignals
+ * (spatial-correlation.cc:11-61,142-161,163-206,221-230), single channel:
+ * each product of the two r2c spectra goes to computeZeroPaddedIndex in a
+ * zeroed buffer of (N + 2 pad)^3 entries, whose c2r of size N + 2 pad is the
+ * correlation.  corr: (N + 2 pad)^3 floats (optional); argmax over the
+ * reference's correlation vector, whose length is the truncated double
+ * product N^3 ((N + 2 pad) / N)^3 (getCorrelationSize, :17-24,219-221:
+ * (N + 2 pad)^3 - 1 for some sizes, leaving the last entry out).
+ * n_voxels + 2 zero_padding <= 256. */
+int phz_spatial_correlate_single_padded(phz_ctx* ctx, int n_voxels, int zero_padding,
+                                        const float* f, const float* g, float* corr,
+                                        uint32_t* argmax);
 
 /* ---- tuning -------------------------------------------------------------
  * No counterpart in the reference (FFTW plans are its only tuning,

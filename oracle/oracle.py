@@ -29,7 +29,7 @@ class Config(C.Structure):
     _fields_ = [("bw", C.c_int), ("n_voxels", C.c_int), ("lower", C.c_int),
                 ("upper", C.c_int), ("lp_lower", C.c_int), ("lp_upper", C.c_int),
                 ("use_kdtree", C.c_int), ("sph_pad", C.c_int), ("method", C.c_int),
-                ("skip_rot_axes", C.c_int)]
+                ("skip_rot_axes", C.c_int), ("spat_pad", C.c_int)]
 
 
 class Result(C.Structure):
@@ -83,6 +83,9 @@ def lib():
             "po_c2r3": (None, [C.c_int, dp, dp]),
             "po_spatial_correlate": (None, [C.c_int, C.c_int, dp, dp, dp]),
             "po_spatial_correlate_laplace": (None, [C.c_int, C.c_int, dp, dp, dp]),
+            "po_padded_size": (C.c_uint32, [C.c_int, C.c_int]),
+            "po_spatial_pad_inverse": (None, [C.c_int, C.c_int, C.c_int, dp, dp]),
+            "po_spatial_correlate_laplace_pad": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
             "po_argmax": (C.c_uint32, [dp, C.c_size_t]),
             "po_zyz_from_index": (None, [C.c_uint32, C.c_int, dp]),
             "po_quat_from_zyz": (None, [dp, dp]),
@@ -321,11 +324,23 @@ def spatial_correlate(f, g, N, pad=0):
     return out
 
 
-def spatial_correlate_laplace(f, g, N):
+def spatial_correlate_laplace(f, g, N, pad=0):
     f = _c(f, np.float64)
     g = _c(g, np.float64)
-    out = np.zeros(N ** 3)
-    lib().po_spatial_correlate_laplace(N, f.shape[0], f, g, out)
+    out = np.zeros((N + 2 * pad) ** 3)
+    lib().po_spatial_correlate_laplace_pad(N, pad, f.shape[0], f, g, out)
+    return out
+
+
+def padded_size(N, pad):
+    return int(lib().po_padded_size(N, pad))
+
+
+def spatial_pad_inverse(prod, N, pad, shifted):
+    """prod: N^3 complex products in their own order -> (N + 2 pad)^3 grid."""
+    p = _c(np.asarray(prod, np.complex128).view(np.float64), np.float64)
+    out = np.zeros((N + 2 * pad) ** 3)
+    lib().po_spatial_pad_inverse(N, pad, int(shifted), p, out)
     return out
 
 
@@ -486,13 +501,17 @@ def sub2ind(i, j, k, rows, cols):
 
 def register(target, source, bw=32, n_voxels=64, lower=-50, upper=50,
              kdtree=True, want_grids=False, want_reg=False, target_extra=None,
-             source_extra=None, sph_pad=0, method=0, skip_rot_axes=0):
+             source_extra=None, sph_pad=0, method=0, skip_rot_axes=0, lp_lower=0,
+             lp_upper=100000, spat_pad=0):
     """target_extra / source_extra: (P, 2) reflectivity and ambient per
     point, or None (zero channels).  sph_pad: spherical zero padding (the
     SO(3) grid at bw + sph_pad).  method 1: SphRegistration's single blended
-    channel (no padding); skip_rot_axes: its refine_rot_x/y/z masks."""
-    cfg = Config(bw, n_voxels, lower, upper, 0, 100000, int(kdtree), int(sph_pad), int(method),
-                 int(skip_rot_axes))
+    channel (no padding); skip_rot_axes: its refine_rot_x/y/z masks.
+    lp_lower / lp_upper: the spatial low-pass bounds (they set the S^2
+    nearest-neighbour acceptance radius).  spat_pad: spatial zero padding
+    (translation grid (n_voxels + 2 spat_pad)^3)."""
+    cfg = Config(bw, n_voxels, lower, upper, int(lp_lower), int(lp_upper), int(kdtree),
+                 int(sph_pad), int(method), int(skip_rot_axes), int(spat_pad))
     if method == 1:
         sph_pad = 0
     t = _c(target, np.float32)
@@ -500,7 +519,7 @@ def register(target, source, bw=32, n_voxels=64, lower=-50, upper=50,
     te, se = _extra(target_extra, t.shape[0]), _extra(source_extra, s.shape[0])
     res = Result()
     rot = np.zeros((2 * (bw + sph_pad)) ** 3) if want_grids else None
-    tr = np.zeros(n_voxels ** 3) if want_grids else None
+    tr = np.zeros((n_voxels + 2 * spat_pad) ** 3) if want_grids else None
     reg = np.zeros_like(s) if want_reg else None
     lib().po_register_ex(C.byref(cfg), t, _ptr(te), t.shape[0], s, _ptr(se), s.shape[0],
                          C.byref(res), _ptr(reg), _ptr(rot), _ptr(tr))

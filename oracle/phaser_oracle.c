@@ -1066,16 +1066,33 @@ static void cmul_conj(const double* F, const double* G, double* C) {
   C[1] = F[0] * (-G[1]) + F[1] * G[0];
 }
 
-void po_spatial_correlate(int N, int pad, const double* f, const double* g,
-                          double* c) {
+/* SpatialCorrelation's padded correlation size, spatial-correlation.cc:
+ * 17-24: N^3 times ((N + 2 pad) / N)^3 computed in double and truncated to
+ * uint32, which is (N + 2 pad)^3 - 1 for many (N, pad) (e.g. N = 3, pad = 4:
+ * 1330, not 1331).  The correlation vector has this many entries
+ * (getCorrelationSize, :219-221), so the argmax never sees the last c2r
+ * output then. */
+uint32_t po_padded_size(int N, int pad) {
+  const uint32_t T = (uint32_t)N * N * N;
+  const uint32_t padding_per_dim = 2u * (uint32_t)pad;
+  const double ratio = (N + padding_per_dim) / (double)N;
+  const double factor = ratio * ratio * ratio;
+  return (uint32_t)(T * factor);
+}
+
+/* The padded inverse of spatial-correlation.cc:142-161,221-230 and
+ * spatial-correlation-laplace.cc:29-37: product entry i (i < N^3, in the
+ * product's own order) goes to Sub2Ind(x + pad, y + pad, z + pad, Np, Np) of
+ * a zeroed buffer of (N + 2 pad)^3 entries, with (x, y, z) = Ind2Sub(i, N, N)
+ * (computeZeroPaddedIndex; the identity when pad = 0); `shifted` (the Laplace
+ * class): IFFTShift of the buffer's first N^3 entries; then the c2r of size
+ * Np = N + 2 pad over the buffer's first Np^2 (Np/2 + 1) entries.
+ * c: Np^3. */
+void po_spatial_pad_inverse(int N, int pad, int shifted, const double* prod, double* c) {
   const uint32_t T = (uint32_t)N * N * N;
   const int Np = N + 2 * pad;
   const size_t Tp = (size_t)Np * Np * Np;
-  double* F = (double*)xcalloc(2 * (size_t)T, sizeof(double));
-  double* G = (double*)xcalloc(2 * (size_t)T, sizeof(double));
   double* C = (double*)xcalloc(2 * Tp, sizeof(double));
-  po_r2c3(N, f, F);
-  po_r2c3(N, g, G);
   for (uint32_t idx = 0; idx < T; ++idx) {
     uint32_t p = idx;
     if (pad != 0) {
@@ -1083,12 +1100,28 @@ void po_spatial_correlate(int N, int pad, const double* f, const double* g,
       po_ind2sub(idx, N, N, ijk);
       p = sub2ind(ijk[0] + pad, ijk[1] + pad, ijk[2] + pad, Np, Np);
     }
-    cmul_conj(F + 2 * (size_t)idx, G + 2 * (size_t)idx, C + 2 * (size_t)p);
+    C[2 * (size_t)p] = prod[2 * (size_t)idx];
+    C[2 * (size_t)p + 1] = prod[2 * (size_t)idx + 1];
   }
+  if (shifted) po_ifftshift(C, T);
   po_c2r3(Np, C, c);
+  free(C);
+}
+
+void po_spatial_correlate(int N, int pad, const double* f, const double* g,
+                          double* c) {
+  const uint32_t T = (uint32_t)N * N * N;
+  double* F = (double*)xcalloc(2 * (size_t)T, sizeof(double));
+  double* G = (double*)xcalloc(2 * (size_t)T, sizeof(double));
+  double* P = (double*)xcalloc(2 * (size_t)T, sizeof(double));
+  po_r2c3(N, f, F);
+  po_r2c3(N, g, G);
+  for (uint32_t idx = 0; idx < T; ++idx)
+    cmul_conj(F + 2 * (size_t)idx, G + 2 * (size_t)idx, P + 2 * (size_t)idx);
+  po_spatial_pad_inverse(N, pad, 0, P, c);
   free(F);
   free(G);
-  free(C);
+  free(P);
 }
 
 /* spatial-correlation-laplace.cc:17-89.  The reference reuses one r2c
@@ -1098,6 +1131,13 @@ void po_spatial_correlate(int N, int pad, const double* f, const double* g,
  * tail never reaches the output and a zero tail is used here. */
 void po_spatial_correlate_laplace(int N, int C, const double* f,
                                   const double* g, double* c) {
+  po_spatial_correlate_laplace_pad(N, 0, C, f, g, c);
+}
+
+/* The same with phaser_core_spatial_zero_padding = pad: the fused product
+ * (shifted order) through po_spatial_pad_inverse; c: (N + 2 pad)^3. */
+void po_spatial_correlate_laplace_pad(int N, int pad, int C, const double* f,
+                                      const double* g, double* c) {
   const uint32_t T = (uint32_t)N * N * N;
   double* Fc = (double*)xcalloc(2 * (size_t)T * C, sizeof(double));
   double* Gc = (double*)xcalloc(2 * (size_t)T * C, sizeof(double));
@@ -1122,8 +1162,7 @@ void po_spatial_correlate_laplace(int N, int C, const double* f,
   double* Cc = (double*)xcalloc(2 * (size_t)T, sizeof(double));
   for (uint32_t i = 0; i < T; ++i)
     cmul_conj(F + 2 * (size_t)i, G + 2 * (size_t)i, Cc + 2 * (size_t)i);
-  po_ifftshift(Cc, T);
-  po_c2r3(N, Cc, c);
+  po_spatial_pad_inverse(N, pad, 1, Cc, c);
   free(F);
   free(G);
   free(Cc);
@@ -1285,14 +1324,21 @@ void po_register_ex(const po_config* cfg, const float* target, const float* targ
   /* the rotated source keeps each point's reflectivity and ambient */
   po_voxelize_ex(target, target_extra, Pt, N, (float)cfg->lower, (float)cfg->upper, fv, NULL);
   po_voxelize_ex(reg, source_extra, Ps, N, (float)cfg->lower, (float)cfg->upper, gv, NULL);
-  double* c = trans_corr ? trans_corr : (double*)xcalloc(T3, sizeof(double));
-  po_spatial_correlate_laplace(N, 4, fv, gv, c);
-  res->trans_index = po_argmax(c, T3);
+  /* spatial zero padding: the grid of Np = N + 2 pad per axis, of which the
+   * first po_padded_size entries form the correlation vector (phase-aligner.
+   * cc:66-68), indexed and converted to metres with Np voxels
+   * (getNumberOfVoxels, :137-139; phase-correlation-eval.cc:22-27) */
+  const int Np = N + 2 * cfg->spat_pad;
+  const size_t Tg = (size_t)Np * Np * Np;
+  const size_t Tc = cfg->spat_pad ? po_padded_size(N, cfg->spat_pad) : T3;
+  double* c = trans_corr ? trans_corr : (double*)xcalloc(Tg, sizeof(double));
+  po_spatial_correlate_laplace_pad(N, cfg->spat_pad, 4, fv, gv, c);
+  res->trans_index = po_argmax(c, Tc);
   res->trans_value = c[res->trans_index];
   uint32_t xyz[3];
-  po_ind2sub(res->trans_index, N, N, xyz);
+  po_ind2sub(res->trans_index, Np, Np, xyz);
   for (int d = 0; d < 3; ++d)
-    res->translation[d] = po_translation_from_index((double)xyz[d], N,
+    res->translation[d] = po_translation_from_index((double)xyz[d], Np,
                                                     cfg->lower, cfg->upper);
   if (reg_out)
     for (int64_t i = 0; i < Ps; ++i) {

@@ -149,6 +149,17 @@ void po_c2r3(int N, const double* in_cplx, double* out);
  * (single channel, optional zero padding).  c: (N+2pad)^3. */
 void po_spatial_correlate(int N, int pad, const double* f, const double* g,
                           double* c);
+/* spatial-correlation.cc:17-24: entries of the padded correlation vector,
+ * (uint32)(N^3 ((N + 2 pad) / N)^3) in double (may be (N + 2 pad)^3 - 1). */
+uint32_t po_padded_size(int N, int pad);
+/* spatial-correlation.cc:142-161,221-230 (+ spatial-correlation-laplace.cc:
+ * 36-37 when shifted): N^3 complex products (interleaved) -> the padded
+ * buffer, IFFTShift of its first N^3 entries if shifted, c2r of size
+ * N + 2 pad.  c: (N + 2 pad)^3. */
+void po_spatial_pad_inverse(int N, int pad, int shifted, const double* prod, double* c);
+/* SpatialCorrelationLaplace with zero padding pad.  c: (N + 2 pad)^3. */
+void po_spatial_correlate_laplace_pad(int N, int pad, int C, const double* f,
+                                      const double* g, double* c);
 /* a16-a19: SpatialCorrelationLaplace::correlateSignals,
  * spatial-correlation-laplace.cc:17-51.  f,g: C x N^3.  c: N^3. */
 void po_spatial_correlate_laplace(int N, int C, const double* f,
@@ -186,6 +197,9 @@ typedef struct {
                         103-112), transformed and correlated unshifted, no padding */
   int skip_rot_axes; /* method 1: bits 0/1/2 = refine_rot_x/y/z false, that Euler angle
                         zeroed before RotateAroundXYZ (sph-registration.cc:114-119) */
+  int spat_pad;      /* phaser_core_spatial_zero_padding: translation grid of
+                        (n_voxels + 2 spat_pad)^3, argmax over its first
+                        po_padded_size entries */
 } po_config;
 
 typedef struct {
@@ -209,8 +223,8 @@ void po_spherical_correlate_laplace_pad(int bw, int bw_out, int C, const double*
                                         double* f_fused, double* h_fused);
 
 /* target = cloud_prev (signal f), source = cloud_cur (pattern h).
- * reg_out (optional): registered source, P_s x 4.  rot_corr: (2bw)^3,
- * trans_corr: N^3 (both optional). */
+ * reg_out (optional): registered source, P_s x 4.  rot_corr: (2 (bw +
+ * sph_pad))^3, trans_corr: (N + 2 spat_pad)^3 (both optional). */
 void po_register(const po_config* cfg, const float* target, int64_t Pt,
                  const float* source, int64_t Ps, po_result* res,
                  float* reg_out, double* rot_corr, double* trans_corr);

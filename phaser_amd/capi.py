@@ -84,6 +84,9 @@ EXPORTS = {
                                         C.POINTER(C.c_uint32)]),
     "phz_spatial_correlate_single": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.POINTER(C.c_uint32)]),
+    "phz_spatial_correlate_single_padded": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                                      C.c_void_p, C.c_void_p,
+                                                      C.POINTER(C.c_uint32)]),
     "phz_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "phz_get_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "phz_set_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -201,6 +204,7 @@ class Context:
         self.n = 2 * self.B  # S^2 grid
         self.no = 2 * (self.B + self.cfg.spherical_zero_padding)  # SO(3) grid
         self.N = self.cfg.n_voxels
+        self.Np = self.N + 2 * self.cfg.spatial_zero_padding  # translation grid
 
     def close(self):
         if self.h:
@@ -231,7 +235,7 @@ class Context:
             assert e is None or e.shape == (cl.shape[0], 2)
         res = Result()
         rc = np.zeros(self.no ** 3, np.float32) if grids else None
-        tc = np.zeros(self.N ** 3, np.float32) if grids else None
+        tc = np.zeros(self.Np ** 3, np.float32) if grids else None
         reg = np.zeros_like(s) if registered else None
         _check(lib().phz_register_channels(self.h, _p(t), _p(te), t.shape[0], _p(s), _p(se),
                                            s.shape[0], C.byref(res), _p(rc), _p(tc), _p(reg)))
@@ -399,4 +403,13 @@ class Context:
         corr = np.zeros(N ** 3, np.float32)
         am = C.c_uint32()
         _check(lib().phz_spatial_correlate_single(self.h, N, _p(f), _p(g), _p(corr), C.byref(am)))
+        return corr, int(am.value)
+
+    def spatial_correlate_single_padded(self, f, g, N, pad):
+        f = _f32(f).reshape(-1)
+        g = _f32(g).reshape(-1)
+        corr = np.zeros((N + 2 * pad) ** 3, np.float32)
+        am = C.c_uint32()
+        _check(lib().phz_spatial_correlate_single_padded(self.h, N, pad, _p(f), _p(g), _p(corr),
+                                                         C.byref(am)))
         return corr, int(am.value)

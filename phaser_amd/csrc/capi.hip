@@ -18,14 +18,24 @@ FftPlan make_plan(int L) {
   FftPlan p;
   p.L = L;
   p.nst = 0;
+  if (L < 1) {
+    p.nst = -1;
+    return p;
+  }
   int r = L;
-  while (r % 4 == 0 && p.nst < 10) {
+  while (r % 4 == 0 && p.nst < 12) {
     p.radix[p.nst++] = 4;
     r /= 4;
   }
-  const int others[3] = {2, 3, 5};
-  for (int f : others)
-    while (r % f == 0 && p.nst < 10) {
+  const int small[3] = {2, 3, 5};
+  for (int f : small)
+    while (r % f == 0 && p.nst < 12) {
+      p.radix[p.nst++] = f;
+      r /= f;
+    }
+  // remaining primes, smallest first, each a direct-DFT stage
+  for (int f = 7; r > 1 && p.nst < 12; f += 2)
+    while (r % f == 0 && p.nst < 12) {
       p.radix[p.nst++] = f;
       r /= f;
     }
@@ -33,9 +43,32 @@ FftPlan make_plan(int L) {
   return p;
 }
 
+// SpatialCorrelation's total_n_voxels_padded_ (spatial-correlation.cc:17-24):
+// N^3 ((N + 2 pad) / N)^3 in double, truncated
+uint32_t padded_size(int N, int pad) {
+  const uint32_t T = (uint32_t)N * N * N;
+  const uint32_t per_dim = 2u * (uint32_t)pad;
+  const double ratio = (N + per_dim) / (double)N;
+  const double factor = ratio * ratio * ratio;
+  return (uint32_t)(T * factor);
+}
+
+bool smooth235(int L) {
+  if (L < 1) return false;
+  for (int f : {2, 3, 5})
+    while (L % f == 0) L /= f;
+  return L == 1;
+}
+
 }  // namespace phz
 
 using namespace phz;
+
+// largest bandwidth (and bandwidth + spherical zero padding): the S^2
+// search grid of the nearest-neighbour kernels holds up to 2 * 160 bands
+// and sectors (the reference's tuned driver run uses 150 padded to 160,
+// run_phaser_core_driver:11-15)
+constexpr int kMaxBandwidth = 160;
 
 // XCD-ordered S^2 nearest-neighbour workgroups by default (measured on
 // gfx950 at B = 64, 64 clouds per launch: k_nn 1.34 -> 1.18 ms)
@@ -48,11 +81,19 @@ struct phz_ctx {
   // (spherical-correlation.cc:21-27), no = 2 Bo
   int Bo = 0, no = 0;
   int64_t H = 0, T3 = 0, cap = 0;
+  // spatial zero padding sp (spatial-correlation.cc:11-61): the translation
+  // grid of Np = N + 2 sp per axis (T3p = Np^3 floats, Hp = Np^2 (Np/2 + 1)
+  // half-spectrum entries), of which the first Tc entries are the
+  // correlation vector (the reference's truncated padded size); without
+  // padding Np = N, T3p = Tc = N^3
+  int sp = 0, Np = 0;
+  int64_t Hp = 0, T3p = 0;
+  uint32_t Tc = 0;
   int max_clouds = 0;
   hipStream_t stream = nullptr;
   S2Tables s2;
   VoxelTables vox;
-  FftPlan plan_n{}, plan_N{}, plan_no{};
+  FftPlan plan_n{}, plan_N{}, plan_no{}, plan_Np{};
   SpatialTiles tiles{};
   bool plane_fwd = false;  // whole-plane first forward pass
   bool so3_reg = false;    // SO(3) alpha/gamma passes on the register engine
@@ -75,6 +116,8 @@ struct phz_ctx {
   float2* d_fused_o = nullptr;   // fused coefficients in the Bo layout (d_fused when unpadded)
   float* d_edges = nullptr;
   float2* d_tw_N = nullptr;
+  float2* d_tw_Np = nullptr;     // exp(-2 pi i e / Np) (d_tw_N when unpadded)
+  float2* d_work_pad = nullptr;  // padded c2r inputs, Hp per pair (spatial padding only)
   // workspaces
   float4* d_pts = nullptr;
   int* d_npts = nullptr;
@@ -219,10 +262,12 @@ size_t context_bytes(const phz_config& cf) {
   const int64_t B = cf.bandwidth, n = 2 * B, N = cf.n_voxels;
   const int64_t pad = cf.spherical_zero_padding, Bo = B + pad, no = 2 * Bo;
   const int64_t nq = n * n, T3 = N * N * N, H = N * N * (N / 2 + 1);
+  const int64_t sp = cf.spatial_zero_padding, Np = N + 2 * sp, T3p = Np * Np * Np;
+  const int64_t Hp = Np * Np * (Np / 2 + 1);
   const int64_t P = cf.max_batch, M = 2 * P, cap = cf.max_points;
   const int64_t G = std::max<int64_t>((cf.channels == 4 ? 4 : 2) * M, 16);
   const int64_t MV = std::max<int64_t>(M, 16);
-  const int64_t ppp = std::max(so3_gamma_blocks((int)Bo), spatial_c2r_blocks((int)N));
+  const int64_t ppp = std::max(so3_gamma_blocks((int)Bo), spatial_c2r_blocks((int)Np));
   size_t s = 0;
   auto add = [&](int64_t count, size_t elem) { s += elem * (size_t)(count ? count : 1); };
   // tables
@@ -273,10 +318,14 @@ size_t context_bytes(const phz_config& cf) {
   add((M * N + 3) / 4 * 4, 1);
   add(std::max<int64_t>(2 * M, 16) * H, sizeof(float2));
   add(P * H, sizeof(float2));
-  add(T3, sizeof(float));
+  add(T3p, sizeof(float));
   add(cap, sizeof(float4));
+  if (sp > 0) {
+    add(Np, sizeof(float2));
+    add(P * Hp, sizeof(float2));
+  }
   if (cf.bingham_peak_neighbors > 0) add(P * no * no * no, sizeof(float));
-  if (cf.gaussian_peak_neighbors > 0) add(P * T3, sizeof(float));
+  if (cf.gaussian_peak_neighbors > 0) add(P * T3p, sizeof(float));
   if (cf.channels == 4) {
     add(M * cap, sizeof(float2));
     add(M * T3, sizeof(float));
@@ -287,11 +336,14 @@ size_t context_bytes(const phz_config& cf) {
 }
 
 int check_config(const phz_config& cf) {
-  PHZ_CHECK(cf.bandwidth >= 2 && cf.bandwidth <= 128, "bandwidth must be in [2, 128]");
-  PHZ_CHECK(cf.spherical_zero_padding >= 0 && cf.bandwidth + cf.spherical_zero_padding <= 128,
-            "spherical zero padding must be >= 0 with bandwidth + padding <= 128");
-  PHZ_CHECK(cf.spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
+  PHZ_CHECK(cf.bandwidth >= 2 && cf.bandwidth <= kMaxBandwidth, "bandwidth must be in [2, 160]");
+  PHZ_CHECK(cf.spherical_zero_padding >= 0 &&
+                cf.bandwidth + cf.spherical_zero_padding <= kMaxBandwidth,
+            "spherical zero padding must be >= 0 with bandwidth + padding <= 160");
   PHZ_CHECK(cf.n_voxels >= 2 && cf.n_voxels <= 256, "n_voxels must be in [2, 256]");
+  PHZ_CHECK(cf.spatial_zero_padding >= 0 &&
+                cf.n_voxels + 2 * (int64_t)cf.spatial_zero_padding <= 256,
+            "spatial zero padding must be >= 0 with n_voxels + 2 * padding <= 256");
   PHZ_CHECK(cf.discretize_lower < cf.discretize_upper, "discretize bounds must be increasing");
   PHZ_CHECK(cf.max_points >= 1 && cf.max_points <= (1ll << 26), "max_points out of range");
   PHZ_CHECK(cf.max_batch >= 1 && cf.max_batch <= 1024, "max_batch out of range");
@@ -309,10 +361,6 @@ int check_config(const phz_config& cf) {
             "skip_rot_axes (refine_rot_x/y/z) belongs to PHZ_METHOD_SPH");
   PHZ_CHECK(cf.method != PHZ_METHOD_SPH || cf.spherical_zero_padding == 0,
             "PHZ_METHOD_SPH correlates without spherical zero padding");
-  PHZ_CHECK(make_plan(2 * cf.bandwidth).nst > 0, "2*bandwidth must factor into 2, 3 and 5");
-  PHZ_CHECK(make_plan(2 * (cf.bandwidth + cf.spherical_zero_padding)).nst > 0,
-            "2*(bandwidth + spherical zero padding) must factor into 2, 3 and 5");
-  PHZ_CHECK(make_plan(cf.n_voxels).nst > 0, "n_voxels must factor into 2, 3 and 5");
   return 0;
 }
 
@@ -327,11 +375,17 @@ int init_ctx(phz_ctx* c) {
   c->h = c->N / 2 + 1;
   c->H = (int64_t)c->N * c->N * c->h;
   c->T3 = (int64_t)c->N * c->N * c->N;
+  c->sp = cf.spatial_zero_padding;
+  c->Np = c->N + 2 * c->sp;
+  c->T3p = (int64_t)c->Np * c->Np * c->Np;
+  c->Hp = (int64_t)c->Np * c->Np * (c->Np / 2 + 1);
+  c->Tc = c->sp ? padded_size(c->N, c->sp) : (uint32_t)c->T3;
   c->cap = cf.max_points;
   c->max_clouds = 2 * cf.max_batch;
   c->plan_n = make_plan(c->n);
   c->plan_no = make_plan(c->no);
   c->plan_N = make_plan(c->N);
+  c->plan_Np = make_plan(c->Np);
   {
     const float v = (float)(std::abs(cf.low_pass_lower) + std::abs(cf.low_pass_upper)) /
                     (float)cf.n_voxels;  // point-cloud.cc:66-72
@@ -380,6 +434,14 @@ int init_ctx(phz_ctx* c) {
       c->d_tw_no = c->d_tw_n;
     }
     TRY(upload(c, &c->d_tw_N, u));
+    if (c->sp) {
+      const std::vector<float> tp = twiddles(c->Np);
+      std::vector<float2> v(c->Np);
+      std::memcpy(v.data(), tp.data(), sizeof(float2) * c->Np);
+      TRY(upload(c, &c->d_tw_Np, v));
+    } else {
+      c->d_tw_Np = c->d_tw_N;
+    }
   }
   TRY(upload(c, &c->d_edges, c->vox.edges));
   const int M = c->max_clouds, P = cf.max_batch;
@@ -419,7 +481,7 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_U, (size_t)P * no * Bo * no));
   TRY(dalloc(c, &c->d_corr_rot, (size_t)no * no * no));
   TRY(dalloc(c, &c->d_keys, (size_t)2 * P));
-  c->partials_per_pair = std::max(so3_gamma_blocks(c->Bo), spatial_c2r_blocks(c->N));
+  c->partials_per_pair = std::max(so3_gamma_blocks(c->Bo), spatial_c2r_blocks(c->Np));
   TRY(dalloc(c, &c->d_partials, (size_t)P * c->partials_per_pair));
   TRY(dalloc(c, &c->d_pose, (size_t)P));
   const int MV = M > 16 ? M : 16;
@@ -432,12 +494,13 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_plane_occ, occ_plane_bytes));
   TRY(dalloc(c, &c->d_spec, (size_t)(2 * M > 16 ? 2 * M : 16) * c->H));
   TRY(dalloc(c, &c->d_work, (size_t)P * c->H));
-  TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3));
+  TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3p));
   TRY(dalloc(c, &c->d_reg, (size_t)cap));
+  if (c->sp) TRY(dalloc(c, &c->d_work_pad, (size_t)P * c->Hp));
   if (cf.bingham_peak_neighbors > 0)
     TRY(dalloc(c, &c->d_peak_rot, (size_t)P * no * no * no));
   if (cf.gaussian_peak_neighbors > 0)
-    TRY(dalloc(c, &c->d_peak_trans, (size_t)P * c->T3));
+    TRY(dalloc(c, &c->d_peak_trans, (size_t)P * c->T3p));
   if (cf.channels == 4) {
     TRY(dalloc(c, &c->d_extra, (size_t)M * cap));
     TRY(dalloc(c, &c->d_sumF, (size_t)M * c->T3));
@@ -604,7 +667,11 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     G4.ch[2] = c->d_spec2 + (int64_t)(2 * nt) * c->H;
     G4.ch[3] = G4.ch[2] + c->H;
   }
-  const bool zz = c->occ_fwd && c->tiles.fused_reg && c->tuning.fwd_z == kRegisterEngine &&
+  // spatial zero padding: dense forward passes, the cross spectrum alone,
+  // then the padded inverse at Np (below)
+  const bool padded = c->sp > 0;
+  const bool zz = !padded && c->occ_fwd && c->tiles.fused_reg &&
+                  c->tuning.fwd_z == kRegisterEngine &&
                   (four ? spatial_cross_zz4_ok(N, F4, G4) : spatial_cross_zz_ok(N, Fc, Gc));
   const bool occ = zz && !c->plane_fwd && c->tuning.fwd_x == kRegisterEngine;
   const bool plane = c->plane_fwd && !four;
@@ -648,7 +715,10 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     {
       StageScope t(c, kSpatialCrossZ, s);
-      if (four && zz)
+      if (padded)
+        launch_spatial_cross_product(N, np, four ? F4 : Fc, fs, four ? G4 : Gc, gs, c->vox.runs,
+                                     true, c->d_work, s);
+      else if (four && zz)
         launch_spatial_cross_zz4_reg(N, np, F4, fs, G4, gs, occ ? c->d_plane_occ : nullptr,
                                      shared ? 0 : N,
                                      occ ? c->d_plane_occ + (int64_t)nt * N : nullptr, N,
@@ -680,9 +750,20 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
                          c->d_cnt + nt * c->T3, occ_from(c, nt), s, extra_from(c, nt));
     }
-    const bool inv_plane = c->inv_plane && c->tiles.fused_reg &&
+    const bool inv_plane = !padded && c->inv_plane && c->tiles.fused_reg &&
                            c->tuning.inv_x == kRegisterEngine && spatial_inv_plane_ok(N);
-    if (inv_plane) {  // inverse x and c2r in one pass over whole z planes
+    if (padded) {  // spatial-correlation.cc:142-161,221-230; spatial-correlation-laplace.cc:36-37
+      const int Np = c->Np, tile = fft_cols_default_tile(Np);
+      {
+        StageScope t(c, kSpatialFftInv, s);
+        launch_spatial_pad_gather(N, c->sp, true, np, c->d_work, c->d_work_pad, s);
+        launch_spatial_fft_pass(Np, np, c->d_work_pad, 3, tile, c->plan_Np, c->d_tw_Np, s);
+        launch_spatial_fft_pass(Np, np, c->d_work_pad, 2, tile, c->plan_Np, c->d_tw_Np, s);
+      }
+      StageScope t(c, kSpatialC2r, s);
+      launch_spatial_c2r(Np, np, c->d_work_pad, c->plan_Np, c->d_tw_Np, d_trans_corr,
+                         c->d_partials, c->d_keys + np, s, c->Tc);
+    } else if (inv_plane) {  // inverse x and c2r in one pass over whole z planes
       StageScope t(c, kSpatialC2r, s);
       launch_spatial_inv_plane(N, np, c->d_work, c->d_tw_N, d_trans_corr, c->d_partials,
                                c->d_keys + np, s);
@@ -702,8 +783,10 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kTransPose, s);
-    launch_trans_pose(N, c->cfg.discretize_lower, c->cfg.discretize_upper, np, c->d_keys + np,
-                      d_trans_corr, c->T3, kg, c->d_pose, s);
+    // with padding the window and the metres use Np voxels and the
+    // correlation vector's Tc entries (phase-correlation-eval.cc:19-27)
+    launch_trans_pose(c->Np, c->cfg.discretize_lower, c->cfg.discretize_upper, np,
+                      c->d_keys + np, d_trans_corr, c->T3p, kg, c->d_pose, s, c->Tc);
   }
   PHZ_HIP(hipGetLastError());
   return 0;
@@ -918,7 +1001,7 @@ int phz_register_channels(phz_ctx* c, const float* target, const float* target_r
   if (rot_corr)
     TRY(copy_sync(c, rot_corr, c->d_corr_rot, sizeof(float) * n3, hipMemcpyDeviceToHost));
   if (trans_corr)
-    TRY(copy_sync(c, trans_corr, c->d_corr_trans, sizeof(float) * c->T3, hipMemcpyDeviceToHost));
+    TRY(copy_sync(c, trans_corr, c->d_corr_trans, sizeof(float) * c->T3p, hipMemcpyDeviceToHost));
   if (registered && Ps > 0)
     TRY(copy_sync(c, registered, c->d_reg, sizeof(float4) * Ps, hipMemcpyDeviceToHost));
   if (rc) set_error(pair_msg);
@@ -1108,6 +1191,7 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
                           uint32_t* argmax) {
   PHZ_CHECK(c && f && g, "null argument");
   PHZ_CHECK(C >= 2 && C <= 8, "channel count must be in [2, 8]");
+  PHZ_CHECK(c->sp == 0, "the spatial stage entry point runs without spatial zero padding");
   hipStream_t s = c->stream;
   const int64_t T = c->T3, H = c->H;
   PHZ_HIP(hipMemcpyAsync(c->d_sumI, f, sizeof(float) * C * T, hipMemcpyHostToDevice, s));
@@ -1133,17 +1217,25 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
   return 0;
 }
 
-int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float* g, float* corr,
-                                 uint32_t* argmax) {
+// SpatialCorrelation(N, pad)::correlateSignals on one channel pair, with
+// buffers allocated for the call (spatial-correlation.cc:11-61,163-206)
+static int spatial_single(phz_ctx* c, int N, int pad, const float* f, const float* g,
+                          float* corr, uint32_t* argmax) {
   PHZ_CHECK(c && f && g, "null argument");
   PHZ_CHECK(N >= 1 && N <= 256, "n_voxels must be in [1, 256]");
-  const FftPlan plan = make_plan(N);
-  PHZ_CHECK(plan.nst > 0 || N == 1, "n_voxels must factor into 2, 3 and 5");
+  PHZ_CHECK(pad >= 0 && N + 2 * pad <= 256, "zero padding must be >= 0 with n_voxels + 2 * padding <= 256");
+  PHZ_CHECK(N > 1 || pad == 0, "zero padding needs n_voxels >= 2");
+  FftPlan plan = make_plan(N);
+  const int Np = N + 2 * pad;
+  const FftPlan plan_p = make_plan(Np);
   hipStream_t s = c->stream;
   const int64_t T = (int64_t)N * N * N, H = (int64_t)N * N * (N / 2 + 1);
-  std::vector<float> tw = twiddles(N);
+  const int64_t Tp = (int64_t)Np * Np * Np, Hp = (int64_t)Np * Np * (Np / 2 + 1);
+  const uint32_t limit = pad ? padded_size(N, pad) : (uint32_t)T;
+  std::vector<float> tw = twiddles(N), twp = twiddles(Np);
   float *d_in = nullptr, *d_corr = nullptr;
-  float2 *d_spec = nullptr, *d_work = nullptr, *d_tw = nullptr;
+  float2 *d_spec = nullptr, *d_work = nullptr, *d_work_pad = nullptr, *d_tw = nullptr,
+         *d_twp = nullptr;
   unsigned long long* d_key = nullptr;
   int rc = 0;
   auto fail = [&](const char* what, hipError_t e) {
@@ -1155,27 +1247,39 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
   if (!rc && (e = hipMalloc(&d_spec, sizeof(float2) * 2 * H)) != hipSuccess) fail("hipMalloc", e);
   if (!rc && (e = hipMalloc(&d_work, sizeof(float2) * H)) != hipSuccess) fail("hipMalloc", e);
   if (!rc && (e = hipMalloc(&d_tw, sizeof(float2) * N)) != hipSuccess) fail("hipMalloc", e);
-  if (!rc && (e = hipMalloc(&d_corr, sizeof(float) * T)) != hipSuccess) fail("hipMalloc", e);
-  const int nblk = spatial_c2r_blocks(N);
+  if (!rc && (e = hipMalloc(&d_corr, sizeof(float) * Tp)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && pad && (e = hipMalloc(&d_work_pad, sizeof(float2) * Hp)) != hipSuccess)
+    fail("hipMalloc", e);
+  if (!rc && pad && (e = hipMalloc(&d_twp, sizeof(float2) * Np)) != hipSuccess) fail("hipMalloc", e);
+  const int nblk = spatial_c2r_blocks(Np);
   if (!rc && (e = hipMalloc(&d_key, sizeof(unsigned long long) * (1 + nblk))) != hipSuccess)
     fail("hipMalloc", e);
   if (!rc) {
     hipMemcpyAsync(d_in, f, sizeof(float) * T, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_in + T, g, sizeof(float) * T, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s);
-    FftPlan p = plan;
+    if (pad) hipMemcpyAsync(d_twp, twp.data(), sizeof(float2) * Np, hipMemcpyHostToDevice, s);
     if (N == 1) {
-      p.L = 1;
-      p.nst = 0;
+      plan.L = 1;
+      plan.nst = 0;
     }
-    launch_spatial_forward_dense(N, 2, d_in, p, d_tw, d_spec, SpatialTiles{}, s);
+    launch_spatial_forward_dense(N, 2, d_in, plan, d_tw, d_spec, SpatialTiles{}, s);
     ChannelSet F{}, G{};
     F.C = G.C = 1;
     F.ch[0] = d_spec;
     G.ch[0] = d_spec + H;
     FusionRuns none{};
-    launch_spatial_inverse(N, 1, F, 0, G, 0, none, false, p, d_tw, d_work, d_corr, d_key + 1,
-                           d_key, SpatialTiles{}, s);
+    if (pad) {  // spatial-correlation.cc:142-161,221-230, no shift in this class
+      launch_spatial_cross_product(N, 1, F, 0, G, 0, none, false, d_work, s);
+      launch_spatial_pad_gather(N, pad, false, 1, d_work, d_work_pad, s);
+      const int tile = fft_cols_default_tile(Np);
+      launch_spatial_fft_pass(Np, 1, d_work_pad, 3, tile, plan_p, d_twp, s);
+      launch_spatial_fft_pass(Np, 1, d_work_pad, 2, tile, plan_p, d_twp, s);
+      launch_spatial_c2r(Np, 1, d_work_pad, plan_p, d_twp, d_corr, d_key + 1, d_key, s, limit);
+    } else {
+      launch_spatial_inverse(N, 1, F, 0, G, 0, none, false, plan, d_tw, d_work, d_corr, d_key + 1,
+                             d_key, SpatialTiles{}, s);
+    }
     if ((e = hipGetLastError()) != hipSuccess) fail("launch", e);
   }
   unsigned long long key = 0;
@@ -1183,7 +1287,7 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
     fail("copy", e);
   if (!rc && (e = hipStreamSynchronize(s)) != hipSuccess) fail("sync", e);
   if (!rc && corr &&
-      ((e = hipMemcpyAsync(corr, d_corr, sizeof(float) * T, hipMemcpyDeviceToHost, s)) !=
+      ((e = hipMemcpyAsync(corr, d_corr, sizeof(float) * Tp, hipMemcpyDeviceToHost, s)) !=
            hipSuccess ||
        (e = hipStreamSynchronize(s)) != hipSuccess))
     fail("copy", e);
@@ -1191,10 +1295,22 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
   hipFree(d_in);
   hipFree(d_spec);
   hipFree(d_work);
+  hipFree(d_work_pad);
   hipFree(d_tw);
+  hipFree(d_twp);
   hipFree(d_corr);
   hipFree(d_key);
   return rc;
+}
+
+int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float* g, float* corr,
+                                 uint32_t* argmax) {
+  return spatial_single(c, N, 0, f, g, corr, argmax);
+}
+
+int phz_spatial_correlate_single_padded(phz_ctx* c, int N, int zero_padding, const float* f,
+                                        const float* g, float* corr, uint32_t* argmax) {
+  return spatial_single(c, N, zero_padding, f, g, corr, argmax);
 }
 
 // ---------------------------------------------------------------- statistics

@@ -123,12 +123,18 @@ __device__ __forceinline__ unsigned long long block_max_key(unsigned long long k
 
 // ---------------------------------------------------------------- FFT plan
 struct FftPlan {
-  int L;          // transform length (<= 256)
+  int L;          // transform length
   int nst;        // stages
-  int radix[10];  // radices in stage order
+  int radix[12];  // radices in stage order: 4, 2, 3, 5, then other primes
 };
 
-FftPlan make_plan(int L);  // host: factor L into 4,2,3,5 (error if other)
+// host: factor L into radix-4, 2, 3 and 5 stages followed by one stage per
+// other prime factor (fft_stage_prime); nst = -1 if L < 1 or L needs more
+// than 12 stages (never for L <= 4096)
+FftPlan make_plan(int L);
+// host: true when L factors into 2, 3 and 5 only (the register engines and
+// the radix-specialised stages)
+bool smooth235(int L);
 
 // One Stockham stage of radix R for nb lines of length L stored line-major
 // in `in` (line stride ls >= L; ls = L + 1 keeps transposed accesses free of
@@ -201,6 +207,39 @@ __device__ __forceinline__ void fft_stage(const float2* __restrict__ in, float2*
   }
 }
 
+// One Stockham stage of a prime radix R > 5 (lengths with other factors
+// than 2, 3 and 5, e.g. 201 = 3 * 67): one thread per output of a butterfly,
+// the length-R DFT summed directly.  With W = exp(-2 pi i / L) (conjugated
+// for DIR = +1), butterfly j (k = j mod Ns) has inputs x_r = in[j + r L/R],
+// and output q is sum_r x_r W^(r (k tstep + q L/R)): the stage twiddle
+// W^(r k tstep) and the DFT factor W_R^(q r) = W^(q r L/R) in one lookup.
+// Same data layout and output order as fft_stage.
+template <int DIR>
+__device__ __forceinline__ void fft_stage_prime(const float2* __restrict__ in,
+                                                float2* __restrict__ out, int L, int ls, int nb,
+                                                int Ns, int R, const float2* __restrict__ tw) {
+  const int Lr = L / R;
+  const int tstep = L / (Ns * R);
+  for (int w = threadIdx.x; w < nb * L; w += blockDim.x) {
+    const int line = w / L;
+    const int rem = w - line * L;  // q * Lr + j
+    const int q = rem / Lr, j = rem - q * Lr;
+    const float2* src = in + line * ls;
+    const int k = j % Ns;
+    const int e = k * tstep + q * Lr;  // < L/R + (R - 1) L/R = L
+    float2 acc = src[j];
+    int ex = 0;
+    for (int r = 1; r < R; ++r) {
+      ex += e;
+      if (ex >= L) ex -= L;
+      float2 t = tw[ex];
+      if (DIR > 0) t.y = -t.y;
+      acc = cadd(acc, cmul(src[j + r * Lr], t));
+    }
+    out[line * ls + (j / Ns) * Ns * R + k + q * Ns] = acc;
+  }
+}
+
 // nb FFTs of length plan.L on line-major LDS data in `a` (line stride ls,
 // scratch `b` of the same size).  Returns the buffer holding the result.
 // Ends with a barrier.
@@ -217,8 +256,10 @@ __device__ float2* lds_fft(float2* a, float2* b, int nb, const FftPlan& plan,
       fft_stage<2, DIR>(a, b, plan.L, ls, nb, Ns, tw);
     else if (R == 3)
       fft_stage<3, DIR>(a, b, plan.L, ls, nb, Ns, tw);
-    else
+    else if (R == 5)
       fft_stage<5, DIR>(a, b, plan.L, ls, nb, Ns, tw);
+    else
+      fft_stage_prime<DIR>(a, b, plan.L, ls, nb, Ns, R, tw);
     __syncthreads();
     float2* t = a;
     a = b;

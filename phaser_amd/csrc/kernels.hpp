@@ -169,7 +169,8 @@ constexpr int kRegisterEngine = 1;
 int fft_cols_default_tile(int L);
 bool fft_cols_tile_ok(int L, int it);
 // one column pass over nbatch half spectra: which = 0 forward along x,
-// 1 forward along z, 2 inverse along x; it = columns per workgroup
+// 1 forward along z, 2 inverse along x, 3 inverse along z; it = columns per
+// workgroup
 void launch_spatial_fft_pass(int N, int nbatch, float2* buf, int which, int it,
                              const FftPlan& plan, const float2* tw, hipStream_t s);
 
@@ -256,14 +257,6 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
 // with the default column tiles
 void launch_spatial_forward_two(int N, const float* a, const float* b, const FftPlan& plan,
                                 const float2* tw, float2* spec, hipStream_t s);
-// SpatialCorrelation::correlateSignals on one channel pair (no padding, no
-// fusion): twiddles of N computed into tw, spectra of f and g into spec
-// (2 N^2(N/2+1)), cross spectrum and inverse into work (N^2(N/2+1)), the grid
-// into corr (nullable, N^3), argmax key into keys[0] with
-// spatial_c2r_blocks(N) partials from keys + 1 (noalloc.hip)
-void launch_spatial_single(int N, const FftPlan& plan, const float* f, const float* g,
-                           float2* tw, float2* spec, float2* work, float* corr,
-                           unsigned long long* keys, hipStream_t s);
 // fused cross spectrum + inverse 3-D c2r + argmax for npairs pairs.  F
 // channels and G channels (null = zero channel) in the reference's spatial
 // order; pair p reads its channels at ch[c] + p*stride (stride 0: shared).
@@ -283,9 +276,23 @@ void launch_spatial_cross_z(int N, int npairs, ChannelSet F, int64_t f_stride, C
                             const FftPlan& plan, const float2* tw, float2* work, hipStream_t s);
 void launch_spatial_fft_inv(int N, int npairs, float2* work, const FftPlan& plan,
                             const float2* tw, const SpatialTiles& tiles, hipStream_t s);
+// limit: only linear indices below it enter the argmax (the reference's
+// padded correlation vector is shorter than the c2r output, po_padded_size)
 void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& plan,
                         const float2* tw, float* corr, unsigned long long* partials,
-                        unsigned long long* keys, hipStream_t s);
+                        unsigned long long* keys, hipStream_t s, uint32_t limit = 0xffffffffu);
+// spatial zero padding (spatial-correlation.cc:142-161,221-230): the fused
+// (or single-channel) cross spectrum over the H unshifted half-spectrum
+// entries, without the inverse transform -> work[pair][H] ...
+void launch_spatial_cross_product(int N, int npairs, ChannelSet F, int64_t f_stride,
+                                  ChannelSet G, int64_t g_stride, const FusionRuns& runs,
+                                  bool fuse, float2* work, hipStream_t s);
+// ... then the c2r input of size Np = N + 2 pad: out[pair][Np^2 (Np/2+1)],
+// with the Laplace class's IFFTShift when shifted; follow with
+// launch_spatial_fft_pass which = 3 (inverse z) and 2 (inverse x) and
+// launch_spatial_c2r at Np
+void launch_spatial_pad_gather(int N, int pad, bool shifted, int npairs, const float2* work,
+                               float2* out, hipStream_t s);
 // zero the accumulators of the voxels the points touched (same transform and
 // binning as launch_voxel_scatter), so they need no full clear before reuse
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,
@@ -294,9 +301,11 @@ void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int ncl
                         VoxelExtra ex = {});
 // One wavefront per pair.  corr: the pairs' N^3 grids at corr_stride floats
 // apart (read only when k > 0); k: gaussian_peak_neighbors.
+// n_corr: entries of the correlation vector (0 = N^3; with spatial zero
+// padding the reference's truncated padded size), the Gaussian window's end
 void launch_trans_pose(int N, int lower, int upper, int npairs,
                        const unsigned long long* keys, const float* corr, int64_t corr_stride,
-                       int k, DevPose* pose, hipStream_t s);
+                       int k, DevPose* pose, hipStream_t s, uint32_t n_corr = 0);
 // out = T_rot * p + translation (PCL scalar order), intensity kept
 void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,
                        hipStream_t s);

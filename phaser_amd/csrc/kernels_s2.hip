@@ -165,9 +165,15 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
   sorted_val[c * cap + dst] = val[c * cap + i];
 }
 
+// The S^2 search grid has at most kMaxSectors bands and sectors (n = 2B,
+// B <= 160).
+constexpr int kMaxSectors = 320;
+constexpr int kMaskWords = (kMaxSectors + 63) / 64;
+
 // per cell: nearest non-empty sector of the same band to the right (>= s)
 // and to the left (<= s), cyclic; -1 in an empty band.  One wave per band:
-// the occupancy of the band's n <= 256 sectors is a 4 x 64-bit ballot mask.
+// the occupancy of the band's n <= kMaxSectors sectors is a ballot mask of
+// kMaskWords 64-bit words.
 __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
                               int4* __restrict__ nr, int4* __restrict__ nl) {
   // n = sectors per band here
@@ -176,14 +182,17 @@ __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
   const int* st = cell_start + (int64_t)c * (nb * n + 1) + (int64_t)b * n;
   int4* r = nr + (int64_t)c * nb * n + (int64_t)b * n;
   int4* l = nl + (int64_t)c * nb * n + (int64_t)b * n;
-  unsigned long long m[4] = {0ull, 0ull, 0ull, 0ull};
+  unsigned long long m[kMaskWords];
   const int nw = (n + 63) / 64;
-  for (int w = 0; w < nw; ++w) {
+  unsigned long long all = 0ull;
+#pragma unroll
+  for (int w = 0; w < kMaskWords; ++w) {
     const int s = w * 64 + lane;
-    const bool ne = s < n && st[s + 1] > st[s];
+    const bool ne = w < nw && s < n && st[s + 1] > st[s];
     m[w] = __ballot(ne);
+    all |= m[w];
   }
-  const bool any = (m[0] | m[1] | m[2] | m[3]) != 0ull;
+  const bool any = all != 0ull;
   for (int s = lane; s < n; s += 64) {
     int right = -1, left = -1;
     if (any) {
@@ -288,10 +297,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   const int sub = threadIdx.x & (LANES - 1);
   const int q = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
-  // min sin(theta) over each band and the cloud's band table (nb <= 256),
-  // once per workgroup
-  __shared__ float band_sin_min[256];
-  __shared__ int bs[257];
+  // min sin(theta) over each band and the cloud's band table (nb <=
+  // kMaxSectors), once per workgroup
+  __shared__ float band_sin_min[kMaxSectors];
+  __shared__ int bs[kMaxSectors + 1];
   for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
     const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
     band_sin_min[bb] = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
@@ -573,8 +582,11 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   else if (ncell <= 16 * 1024)
     hipLaunchKernelGGL(k_cell_scan<16>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
                        cb.cell_count, cb.cell_start, cb.band_start);
-  else
+  else if (ncell <= 64 * 1024)
     hipLaunchKernelGGL(k_cell_scan<64>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
+                       cb.cell_count, cb.cell_start, cb.band_start);
+  else  // up to kMaxSectors^2 = 100 * 1024 cells
+    hipLaunchKernelGGL(k_cell_scan<100>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
                        cb.cell_count, cb.cell_start, cb.band_start);
   hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj, cb.val,
                      cb.cell, cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);

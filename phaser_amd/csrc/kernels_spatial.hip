@@ -381,7 +381,7 @@ __global__ void k_spat_cross_z(int N, ChannelSet F, int64_t f_stride, ChannelSet
 // per complex line, signed argmax over x*N + y + N^2*z.
 __global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
                            const float2* __restrict__ tw, float* __restrict__ corr,
-                           unsigned long long* __restrict__ partials) {
+                           unsigned long long* __restrict__ partials, uint32_t limit) {
   constexpr int RT = 16, NL = RT / 2;
   const int h = N / 2 + 1, rows = N * N;
   const int row0 = blockIdx.x * RT;
@@ -425,11 +425,76 @@ __global__ void k_spat_c2r(int N, const float2* __restrict__ work, FftPlan plan,
     const float v = (r & 1) ? z.y : z.x;
     const uint32_t lin = (uint32_t)((int64_t)(row0 + r) * N + y);
     if (corr) corr[lin] = v;
+    if (lin >= limit) continue;  // past the correlation vector's length
     const unsigned long long kk = make_key(v, lin);
     best = kk > best ? kk : best;
   }
   best = block_max_key(best);
   if (threadIdx.x == 0) partials[blockIdx.x] = best;
+}
+
+// SpatialCorrelation::complexMulSeq (spatial-correlation.cc:142-161) on the
+// fused (or, fuse = 0, single-channel) spectra without the inverse
+// transform: work[pair][q] = F[q] conj(G[q]) over the H unshifted
+// half-spectrum entries (the rest of the N^3 buffer is zero)
+__global__ void k_spat_cross_product(int64_t H, ChannelSet F, int64_t f_stride, ChannelSet G,
+                                     int64_t g_stride, FusionRuns runs, int fuse,
+                                     float2* __restrict__ work) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= H) return;
+  const int64_t fo = blockIdx.y * f_stride, go = blockIdx.y * g_stride;
+  float2 f, g;
+  if (fuse) {
+    int mode = 0;
+    for (int r = 0; r < runs.count; ++r)
+      if ((uint64_t)q >= runs.start[r]) mode = (int)runs.mode[r];
+    f = fuse_at(F, mode, q + fo);
+    g = fuse_at(G, mode, q + go);
+  } else {
+    f = F.ch[0][q + fo];
+    g = G.ch[0][q + go];
+  }
+  float2 c;
+  c.x = f.x * g.x - f.y * (-g.y);
+  c.y = f.x * (-g.y) + f.y * g.x;
+  work[blockIdx.y * H + q] = c;
+}
+
+// The c2r input of the zero-padded correlation (spatial-correlation.cc:
+// 142-161,221-230): the reference writes product entry i (i < T = N^3, in
+// the product's own order) to Sub2Ind(x + pad, y + pad, z + pad, Np, Np) of a
+// zeroed buffer, (x, y, z) = Ind2Sub(i, N, N); the Laplace class (shifted)
+// then rotates the buffer's first T entries left by T/2 (IFFTShift,
+// spatial-correlation-laplace.cc:36) and the c2r of size Np reads its first
+// Hp = Np^2 (Np/2 + 1) entries.  Gathered per output q < Hp: the buffer
+// entry r it holds, r's padded coordinates, the product index i, and, for
+// the shifted order of the Laplace class, the unshifted half-spectrum entry
+// u = (i + s) mod T (FFTShift rotates left by s = ceil(T/2), signal-utils.h:
+// 29-37) that work holds when u < H.
+__global__ void k_spat_pad_gather(int N, int pad, int shifted, const float2* __restrict__ work,
+                                  int64_t H, float2* __restrict__ out, int64_t Hp) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= Hp) return;
+  const int64_t T = (int64_t)N * N * N;
+  const int64_t Np = N + 2 * pad;
+  int64_t r = q;
+  if (shifted && q < T) {
+    r = q + T / 2;
+    if (r >= T) r -= T;
+  }
+  const int64_t zp = r / (Np * Np), rem = r - zp * Np * Np;
+  const int64_t xp = rem / Np, yp = rem - xp * Np;
+  float2 v = make_float2(0.f, 0.f);
+  if (zp >= pad && zp < pad + N && xp >= pad && xp < pad + N && yp >= pad && yp < pad + N) {
+    const int64_t i = (xp - pad) * N + (yp - pad) + (int64_t)N * N * (zp - pad);
+    int64_t u = i;
+    if (shifted) {
+      u = i + (T % 2 ? T / 2 + 1 : T / 2);
+      if (u >= T) u -= T;
+    }
+    if (u < H) v = work[blockIdx.y * H + u];
+  }
+  out[blockIdx.y * Hp + q] = v;
 }
 
 // The translation pose of pair blockIdx.x (one wavefront): the Gaussian fit
@@ -443,12 +508,11 @@ __global__ void __launch_bounds__(64) k_trans_pose(int N, int lower, int upper,
                                                    const unsigned long long* __restrict__ keys,
                                                    const float* __restrict__ corr,
                                                    int64_t corr_stride, int k,
-                                                   DevPose* __restrict__ pose) {
+                                                   DevPose* __restrict__ pose, uint32_t n_corr) {
   const int p = blockIdx.x, lane = threadIdx.x;
   const unsigned long long key = keys[p];
   const uint32_t idx = key_index(key);
   const float value = unordered_value((uint32_t)(key >> 32));
-  const uint32_t n_corr = (uint32_t)N * N * N;
   const double width = (double)(abs(lower) + abs(upper) + 1);
   // gaussian-peak-based-eval.cc:26-37
   const uint32_t uk = (uint32_t)k;
@@ -1501,8 +1565,10 @@ void launch_spatial_fft_pass(int N, int nbatch, float2* buf, int which, int it,
     fft_cols(buf, nbatch, H, N, N, h, -1, it, plan, tw, s);
   else if (which == 1)  // forward along z: columns over (x, k2)
     fft_cols(buf, nbatch, H, 1, N, N * h, -1, it, plan, tw, s);
-  else  // inverse along x
+  else if (which == 2)  // inverse along x
     fft_cols(buf, nbatch, H, N, N, h, +1, it, plan, tw, s);
+  else  // inverse along z
+    fft_cols(buf, nbatch, H, 1, N, N * h, +1, it, plan, tw, s);
 }
 
 void launch_spatial_fft_fwd(int N, int nspec, float2* spec, const FftPlan& plan,
@@ -1565,11 +1631,11 @@ void launch_spatial_fft_inv(int N, int npairs, float2* work, const FftPlan& plan
 
 void launch_spatial_c2r(int N, int npairs, const float2* work, const FftPlan& plan,
                         const float2* tw, float* corr, unsigned long long* partials,
-                        unsigned long long* keys, hipStream_t s) {
+                        unsigned long long* keys, hipStream_t s, uint32_t limit) {
   const int h = N / 2 + 1;
   const size_t sh = (16 * h + 2 * 8 * N) * sizeof(float2);
   hipLaunchKernelGGL(k_spat_c2r, dim3(spatial_c2r_blocks(N), npairs), dim3(256), sh, s, N, work,
-                     plan, tw, corr, partials);
+                     plan, tw, corr, partials, limit);
   launch_reduce_keys(partials, spatial_c2r_blocks(N), npairs, keys, s);
 }
 
@@ -1585,9 +1651,26 @@ void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, C
 
 void launch_trans_pose(int N, int lower, int upper, int npairs, const unsigned long long* keys,
                        const float* corr, int64_t corr_stride, int k, DevPose* pose,
-                       hipStream_t s) {
+                       hipStream_t s, uint32_t n_corr) {
+  if (n_corr == 0) n_corr = (uint32_t)N * N * N;
   hipLaunchKernelGGL(k_trans_pose, dim3(npairs), dim3(64), 0, s, N, lower, upper, keys, corr,
-                     corr_stride, k, pose);
+                     corr_stride, k, pose, n_corr);
+}
+
+void launch_spatial_cross_product(int N, int npairs, ChannelSet F, int64_t f_stride,
+                                  ChannelSet G, int64_t g_stride, const FusionRuns& runs,
+                                  bool fuse, float2* work, hipStream_t s) {
+  const int64_t H = (int64_t)N * N * (N / 2 + 1);
+  hipLaunchKernelGGL(k_spat_cross_product, dim3((unsigned)((H + 255) / 256), npairs), dim3(256),
+                     0, s, H, F, f_stride, G, g_stride, runs, (int)fuse, work);
+}
+
+void launch_spatial_pad_gather(int N, int pad, bool shifted, int npairs, const float2* work,
+                               float2* out, hipStream_t s) {
+  const int64_t H = (int64_t)N * N * (N / 2 + 1);
+  const int64_t Np = N + 2 * pad, Hp = Np * Np * (Np / 2 + 1);
+  hipLaunchKernelGGL(k_spat_pad_gather, dim3((unsigned)((Hp + 255) / 256), npairs), dim3(256),
+                     0, s, N, pad, (int)shifted, work, H, out, Hp);
 }
 
 void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,

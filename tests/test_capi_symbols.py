@@ -59,13 +59,22 @@ def test_default_config_and_error_path_without_gpu():
     cfg = capi.default_config()
     assert cfg.bandwidth == 64 and cfg.n_voxels == 128
     assert cfg.discretize_lower == -50 and cfg.discretize_upper == 50
-    bad = capi.default_config(bandwidth=7)  # 14 = 2*7 is not a 2/3/5 size
+    bad = capi.default_config(bandwidth=161)  # the search grid holds 2 * 160 bands
     h = C.c_void_p()
     rc = capi.lib().phz_create(C.byref(bad), C.byref(h))
     assert rc == -1
-    assert b"factor" in capi.lib().phz_last_error()
-    bad2 = capi.default_config(spherical_zero_padding=3)
+    assert b"bandwidth" in capi.lib().phz_last_error()
+    bad2 = capi.default_config(bandwidth=150, spherical_zero_padding=11)
     assert capi.lib().phz_create(C.byref(bad2), C.byref(h)) == -1
+    bad3 = capi.default_config(n_voxels=257)
+    assert capi.lib().phz_create(C.byref(bad3), C.byref(h)) == -1
+    # any bandwidth and voxel count in range is accepted by the checks (the
+    # estimate runs them without a device)
+    for kw in (dict(bandwidth=150, spherical_zero_padding=10, n_voxels=201),
+               dict(bandwidth=7, n_voxels=67)):
+        b = C.c_size_t()
+        assert capi.lib().phz_estimate_memory(C.byref(capi.default_config(**kw)), C.byref(b)) == 0
+        assert b.value > 0
 
 
 def test_serve_library_exports_every_declared_symbol():
@@ -171,6 +180,6 @@ def test_memory_estimate_without_gpu():
     assert est[(64, 128, 1)] > 16 * 128 ** 3 * 4
     assert est[(32, 64, 1)] < est[(64, 128, 1)] < est[(64, 128, 16)]
     assert est[(64, 128, 1)] < est[(128, 256, 1)]
-    bad = capi.default_config(bandwidth=7)
+    bad = capi.default_config(bandwidth=161)
     b = C.c_size_t()
     assert capi.lib().phz_estimate_memory(C.byref(bad), C.byref(b)) == -1

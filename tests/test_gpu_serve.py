@@ -149,3 +149,44 @@ def test_served_sph_method_equals_alone():
     for a, g in zip(alone, got):
         _same(a, g)
         np.testing.assert_array_equal(g["quat"], a["quat"])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_served_full_mix_both_forms_equals_alone(precision):
+    """The serving configuration's whole bandwidth mix, B in {32, 64, 96, 128}
+    with 64^3 .. 256^3 voxels, in the plain FP32 form and the split-FP16
+    MFMA form: every served pair equals its registration alone in the same
+    form."""
+    shapes = [(32, 64), (64, 128), (96, 192), (128, 256)]
+    rng = random.Random(11 + precision)
+    work = []
+    for (B, N) in shapes:
+        for seed in range(3):
+            src, tgt, _, _ = synth.mixed_pair(200 + seed, B)
+            work.append(((B, N), tgt, src))
+    rng.shuffle(work)
+    print(f"serve full mix precision={precision}: {len(work)} pairs", flush=True)
+    alone = []
+    ctxs = {}
+    try:
+        for (shape, tgt, src) in work:
+            if shape not in ctxs:
+                ctxs[shape] = capi.Context(bandwidth=shape[0], n_voxels=shape[1],
+                                           precision=precision)
+            alone.append(ctxs[shape].register(tgt, src))
+    finally:
+        for c in ctxs.values():
+            c.close()
+    with serve.Server(max_batch=4, contexts_per_shape=1, hbm_budget=96 << 30,
+                      precision=precision) as srv:
+        tickets = []
+        for i, (shape, tgt, src) in enumerate(work):
+            tickets.append(srv.submit(tgt, src, bandwidth=shape[0], n_voxels=shape[1], tag=i))
+            if i % 3 == 2:
+                time.sleep(rng.uniform(0.0, 0.003))
+        got = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    print(f"serve full mix stats: {st}", flush=True)
+    for a, g in zip(alone, got):
+        _same(a, g)
+    assert st["completed"] == len(work) and st["failed"] == 0

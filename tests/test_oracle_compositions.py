@@ -175,3 +175,91 @@ def test_spatial_fusion_is_shift_fuse_unshift(po, N):
     C0 = F0 * np.conj(G0)
     other = np.fft.irfftn(C0[: N * N * h].reshape(N, N, h), s=(N, N, N), axes=(0, 1, 2)) * T
     assert np.max(np.abs(other - expect)) > 1e-3 * scale
+
+
+# ---------------------------------------------------------------- spatial padding
+def test_padded_size_truncates_like_the_reference(po):
+    """spatial-correlation.cc:17-24 multiplies N^3 by ((N + 2 pad) / N)^3 in
+    double and truncates to uint32: one less than (N + 2 pad)^3 for some
+    sizes (values from the same double arithmetic, by hand)."""
+    assert po.padded_size(3, 2) == 343
+    assert po.padded_size(3, 4) == 1330   # 11^3 = 1331
+    assert po.padded_size(5, 1) == 342    # 7^3 = 343
+    assert po.padded_size(128, 16) == 160 ** 3
+    assert po.padded_size(201, 10) == int(201 ** 3 * ((221 / 201.0) ** 3))
+
+
+def _pad_index(i, N, pad):
+    """computeZeroPaddedIndex (spatial-correlation.cc:223-230): Ind2Sub(i, N,
+    N) = (x, y, z) with y = i mod N, x = (i / N) mod N, z = i / N^2
+    (signal-utils.cc:11-19), then Sub2Ind(x + pad, y + pad, z + pad, Np, Np)
+    = (x + pad) Np + (y + pad) + Np^2 (z + pad) (signal-utils.cc:5-9)."""
+    Np = N + 2 * pad
+    x, y, z = (i // N) % N, i % N, i // (N * N)
+    return (x + pad) * Np + (y + pad) + Np * Np * (z + pad)
+
+
+@pytest.mark.parametrize("N,pad,shifted,want_inside", [(12, 1, 1, True), (6, 2, 1, False),
+                                                        (12, 1, 0, True), (9, 2, 1, True)])
+def test_padded_inverse_of_one_product(po, N, pad, shifted, want_inside):
+    """One nonzero product v at index i0: the padded buffer holds v at
+    p0 = computeZeroPaddedIndex(i0); the Laplace class's IFFTShift of the
+    buffer's first N^3 entries (rotate left by N^3 / 2, signal-utils.h:39-47)
+    moves it to q0 = (p0 - N^3 / 2) mod N^3 when p0 < N^3; the c2r of size Np
+    reads q0 as (k0, k1, k2) of an Np x Np x (Np/2 + 1) half spectrum and,
+    for 0 < k2 < Np/2, returns the closed form 2 Re(v e^{2 pi i k.n / Np})."""
+    T, Np = N ** 3, N + 2 * pad
+    hp = Np // 2 + 1
+    v = 0.7 - 0.3j
+    found = None
+    for i0 in range(T):
+        p0 = _pad_index(i0, N, pad)
+        inside = p0 < T
+        if inside != want_inside:
+            continue
+        q0 = (p0 - T // 2) % T if (shifted and inside) else p0
+        k0, k1, k2 = q0 // (Np * hp), (q0 // hp) % Np, q0 % hp
+        if q0 < Np * Np * hp and 0 < k2 and 2 * k2 < Np:
+            found = (i0, k0, k1, k2)
+            break
+    assert found is not None
+    i0, k0, k1, k2 = found
+    prod = np.zeros(T, np.complex128)
+    prod[i0] = v
+    got = po.spatial_pad_inverse(prod, N, pad, shifted)
+    n0, n1, n2 = np.meshgrid(np.arange(Np), np.arange(Np), np.arange(Np), indexing="ij")
+    expect = 2.0 * np.real(v * np.exp(2j * np.pi * (k0 * n0 + k1 * n1 + k2 * n2) / Np))
+    np.testing.assert_allclose(got, expect.reshape(-1), atol=1e-12)
+
+
+@pytest.mark.parametrize("N,pad", [(8, 1), (7, 2)])
+def test_padded_spatial_fusion_composition(po, N, pad):
+    """SpatialCorrelationLaplace with zero padding (spatial-correlation-
+    laplace.cc:17-51, spatial-correlation.cc:142-161,221-230): the fused
+    product of the shifted spectra, placed by computeZeroPaddedIndex,
+    IFFTShift of the first N^3 entries, c2r of size N + 2 pad -- composed
+    from numpy's rfftn / fftshift / ifftshift / irfftn and the pinned fusion."""
+    rng = np.random.default_rng(70 + N)
+    C = 4
+    f = rng.normal(size=(C, N, N, N))
+    g = rng.normal(size=(C, N, N, N))
+    f[1] *= 3.0
+    g[2] *= 3.0
+    T, h, Np = N ** 3, N // 2 + 1, N + 2 * pad
+    hp = Np // 2 + 1
+
+    def spectra(x):
+        out = np.zeros((C, T), np.complex128)
+        for c in range(C):
+            out[c, : N * N * h] = np.fft.rfftn(x[c]).reshape(-1)
+        return out
+
+    F = po.fuse_channels(np.stack([np.fft.fftshift(s) for s in spectra(f)]), levels=5, div=4.0)
+    G = po.fuse_channels(np.stack([np.fft.fftshift(s) for s in spectra(g)]), levels=5, div=4.0)
+    Cpad = np.zeros(Np ** 3, np.complex128)
+    Cpad[_pad_index(np.arange(T), N, pad)] = F * np.conj(G)
+    Cpad[:T] = np.fft.ifftshift(Cpad[:T])
+    expect = np.fft.irfftn(Cpad[: Np * Np * hp].reshape(Np, Np, hp), s=(Np, Np, Np),
+                           axes=(0, 1, 2)) * Np ** 3
+    got = po.spatial_correlate_laplace(f.reshape(C, -1), g.reshape(C, -1), N, pad=pad)
+    np.testing.assert_allclose(got, expect.reshape(-1), atol=1e-10 * np.max(np.abs(expect)))

@@ -10,7 +10,11 @@ Mistakes:
 * the B^2 coefficient FFTShift / IFFTShift around the spherical fusion
   dropped (spherical-correlation-laplace.cc:23-31, 101-113);
 * the N^3 FFTShift / IFFTShift around the spatial fusion dropped
-  (spatial-correlation-laplace.cc:36, 87-88).
+  (spatial-correlation-laplace.cc:36, 87-88);
+* with spatial zero padding: the IFFTShift of the padded buffer dropped, the
+  padded index map with x and y swapped (spatial-correlation.cc:223-230),
+  and the padded size taken as (N + 2 pad)^3 instead of the reference's
+  truncated double product (spatial-correlation.cc:17-24).
 """
 import os
 import subprocess
@@ -38,13 +42,31 @@ def _spherical_shift(s):
 def _spatial_shift(s):
     return [("      po_fftshift(Fc + 2 * (size_t)ch * T, T);\n", ""),
             ("      po_fftshift(Gc + 2 * (size_t)ch * T, T);\n", ""),
-            ("  po_ifftshift(Cc, T);\n", "")]
+            ("  if (shifted) po_ifftshift(C, T);\n", "")]
+
+
+def _padded_shift(s):
+    return [("  if (shifted) po_ifftshift(C, T);\n", "")]
+
+
+def _pad_map(s):
+    return [("      p = sub2ind(ijk[0] + pad, ijk[1] + pad, ijk[2] + pad, Np, Np);",
+             "      p = sub2ind(ijk[1] + pad, ijk[0] + pad, ijk[2] + pad, Np, Np);")]
+
+
+def _padded_size(s):
+    return [("  return (uint32_t)(T * factor);",
+             "  (void)factor;\n  return (uint32_t)(N + padding_per_dim) * (N + padding_per_dim) *"
+             " (N + padding_per_dim);")]
 
 
 @pytest.mark.parametrize("mutation,caught_by", [
     (_weight, "test_so3_correlation"),
     (_spherical_shift, "test_spherical_fusion_is_shift_fuse_unshift"),
     (_spatial_shift, "test_spatial_fusion_is_shift_fuse_unshift"),
+    (_padded_shift, "test_padded_inverse_of_one_product"),
+    (_pad_map, "test_padded_inverse_of_one_product"),
+    (_padded_size, "test_padded_size_truncates_like_the_reference"),
 ])
 def test_pins_fail_a_planted_mistake(tmp_path, mutation, caught_by):
     from oracle import build
