@@ -103,7 +103,8 @@ def concurrent_throughput(args, pairs, max_pts, tuning, grids=True):
         c = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=len(idx),
                     precision=args.precision)
         c.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
-                     t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"])
+                     t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"],
+                     t["nn_seed"])
         if args.graphs:
             c.set_graphs(True)
         ds = [c.device_array(pairs[i][0]) for i in idx]
@@ -171,7 +172,8 @@ def variant_throughput(args, pairs, max_pts, tuning, steps=5, warmup=2):
         ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P, **kw)
         t = tuning
         ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
-                       t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"])
+                       t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"],
+                     t["nn_seed"])
         if args.graphs:
             ctx.set_graphs(True)
         ds = [ctx.device_array(s) for s, _, _, _ in pairs]
@@ -330,6 +332,8 @@ def main():
                     help="split each batch over this many contexts/streams driven concurrently")
     ap.add_argument("--nn-xcd", type=int, default=0, choices=[-1, 0, 1],
                     help="XCD-ordered NN workgroups: 1 on, -1 off, 0 library default")
+    ap.add_argument("--nn-seed", type=int, default=0, choices=[-1, 0, 1],
+                    help="two-pass seeded NN search: 1 on, -1 off, 0 library default")
     ap.add_argument("--inv-plane", type=int, default=0, choices=[-1, 0, 1],
                     help="inverse x + c2r as one pass over z planes in LDS (0: default, on)")
     ap.add_argument("--occ-fwd", type=int, default=0, choices=[-1, 0, 1],
@@ -372,12 +376,12 @@ def main():
     tuning = ctx.autotune(repeats=5) if args.autotune else ctx.tuning()
     if args.graphs:
         ctx.set_graphs(True)
-    if args.nn_lanes or args.occ_fwd or args.inv_plane or args.nn_xcd:
+    if args.nn_lanes or args.occ_fwd or args.inv_plane or args.nn_xcd or args.nn_seed:
         t = ctx.tuning()
         ctx.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"],
                        args.nn_lanes or t["nn_lanes"], t["fused_reg"], t["so3_reg"],
                        args.occ_fwd or t["occ_fwd"], args.inv_plane or t["inv_plane"],
-                       args.nn_xcd or t["nn_xcd"])
+                       args.nn_xcd or t["nn_xcd"], args.nn_seed or t["nn_seed"])
         tuning = ctx.tuning()
     d_src = [ctx.device_array(s) for s, _, _, _ in pairs]
     d_tgt = [ctx.device_array(t) for _, t, _, _ in pairs]

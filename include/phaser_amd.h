@@ -71,7 +71,28 @@ typedef struct {
   int32_t skip_rot_axes; /* bits 0/1/2: refine_rot_x/y/z false (sph-registration.cc:114-119),
                             that Euler angle left out of T_rot and the registered cloud;
                             PHZ_METHOD_SPH only */
+  int32_t rot_eval;  /* rot_evaluation_algorithm (sph-registration.cc:21-23,58-65):
+                        PHZ_EVAL_BINGHAM (0, BinghamPeakBasedEval, the window of
+                        bingham_peak_neighbors) or PHZ_EVAL_MIXTURE (1, BmmPeakBasedEval:
+                        bmm_peak_neighbors, weights normalised also for one sample, their
+                        sum checked > 0; with the single argmax signal of ZScoreEval,
+                        z-score-eval.cc:56-70, the mixture's one component is this
+                        Bingham fit, whose mode is returned: the reference refits
+                        Metropolis-Hastings samples of it drawn with an unseeded random
+                        generator, bingham-mixture.cc:55-78, bingham.cc:126-190) */
+  int32_t pos_eval;  /* pos_evaluation_algorithm: PHZ_EVAL_GAUSSIAN (0,
+                        GaussianPeakBasedEval, gaussian_peak_neighbors) or PHZ_EVAL_MIXTURE
+                        (1, GmmPeakBasedEval, gmm-peak-based-eval.cc:27-105: the window of
+                        gmm_peak_neighbors ends at the peak when peak + k > N^3, and an end
+                        of N^3 fails its CHECK_LT (status PHZ_PAIR_TRANS_WINDOW); one
+                        signal's mixture is that Gaussian, gaussian-mixture.cc:57-78) */
+  int32_t bmm_peak_neighbors;  /* bmm-peak-based-eval.cc:7-9 (default 2), 0 .. 2^20 */
+  int32_t gmm_peak_neighbors;  /* gmm-peak-based-eval.cc:12-14 (default 2), 0 .. 2^20 */
 } phz_config;
+
+#define PHZ_EVAL_BINGHAM 0
+#define PHZ_EVAL_GAUSSIAN 0
+#define PHZ_EVAL_MIXTURE 1
 
 #define PHZ_METHOD_SPH_OPT 0
 #define PHZ_METHOD_SPH 1
@@ -119,6 +140,7 @@ typedef struct {
 #define PHZ_PAIR_ROT_WINDOW 1    /* Bingham window end >= (2Bo)^3: CHECK_LT, bingham-peak-based-eval.cc:69 */
 #define PHZ_PAIR_ROT_WEIGHTS 2   /* Bingham window weights sum <= 0: CHECK_GT, :88 */
 #define PHZ_PAIR_TRANS_WEIGHTS 4 /* Gaussian window weights sum <= 0: CHECK_GT, gaussian-peak-based-eval.cc:100 */
+#define PHZ_PAIR_TRANS_WINDOW 8  /* GMM window end >= N^3: CHECK_LT, gmm-peak-based-eval.cc:81 */
 
 /* Defaults of core-gflags.cc (bandwidth 75 there; 64 here so that the
  * default is the benchmarked configuration), max_points 262144. */
@@ -213,7 +235,10 @@ int phz_spherical_correlate(phz_ctx* ctx, int C, const float* f_grids, const flo
  * (N^3 floats) optional. */
 int phz_voxelize(phz_ctx* ctx, const float* pts, int64_t P, float* out, float* counts);
 /* SpatialCorrelationLaplace::correlateSignals (spatial-correlation-laplace.
- * cc:17-51): C voxel channels per signal (2 <= C <= 8), corr N^3 optional. */
+ * cc:17-51): C voxel channels per signal (2 <= C <= 8), corr optional:
+ * (N + 2 spatial_zero_padding)^3 floats, the padded correlation of the
+ * context's spatial zero padding (argmax over the reference's truncated
+ * padded size). */
 int phz_spatial_correlate(phz_ctx* ctx, int C, const float* f, const float* g, float* corr,
                           uint32_t* argmax);
 /* SpatialCorrelation::correlateSignals (spatial-correlation.cc:163-206),
@@ -233,6 +258,25 @@ int phz_spatial_correlate_single(phz_ctx* ctx, int n_voxels, const float* f, con
 int phz_spatial_correlate_single_padded(phz_ctx* ctx, int n_voxels, int zero_padding,
                                         const float* f, const float* g, float* corr,
                                         uint32_t* argmax);
+/* ZScorePeakExtraction (z-score-peak-extraction.cc:12-112): the smoothed
+ * z-score detector over a correlation vector (the peak extraction that
+ * ZScoreEval can run, z-score-eval.cc:70, before the Bingham / Gaussian /
+ * mixture fits).  Defaults of the flags in phz_zscore_default_params. */
+typedef struct {
+  double lag_percentile;   /* z_score_lag_percentile (0.05), in (0, 1) */
+  uint32_t lag_max;        /* z_score_lag_max (20); the window lag =
+                              min(lag_max, (uint32)(n lag_percentile)) <= 64 */
+  uint32_t reserved;
+  double threshold;        /* z_score_threshold (4.33) */
+  double influence;        /* z_score_influence (0.05), in [0, 1] */
+  double filter_threshold; /* z_score_filter_threshold (0.405) */
+} phz_zscore_params;
+void phz_zscore_default_params(phz_zscore_params* p);
+/* corr: n host floats; the signal indices in increasing order into peaks (at
+ * most max_peaks stored), *count = all signals.  Runs on the device: one
+ * wavefront, n / 64 speculative steps plus one per replaced input. */
+int phz_zscore_peaks(phz_ctx* ctx, const float* corr, int64_t n, const phz_zscore_params* params,
+                     uint32_t* peaks, int64_t max_peaks, int64_t* count);
 
 /* ---- tuning -------------------------------------------------------------
  * No counterpart in the reference (FFTW plans are its only tuning,
@@ -266,6 +310,12 @@ typedef struct {
   int32_t nn_xcd;      /* S^2 nearest-neighbour workgroups ordered so that each of the
                           8 XCDs works through its own clouds (launches of a multiple
                           of 8 clouds): 0 = default, 1 = on, -1 = off */
+  int32_t nn_seed;     /* S^2 nearest-neighbour search in two passes: every 4th query
+                          row and column first, then every query starting from the
+                          nearest of its four surrounding first-pass points (an upper
+                          bound that prunes the walk from its first band; the result
+                          is the same exact nearest point): 0 or 1 = on (default),
+                          -1 = off */
   float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
   float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
   float ms_fused;      /* the three fused passes at max_batch, chosen form */

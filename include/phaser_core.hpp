@@ -55,6 +55,8 @@ extern bool FLAGS_refine_rot_x, FLAGS_refine_rot_y, FLAGS_refine_rot_z;  // true
 extern bool FLAGS_estimate_rotation, FLAGS_estimate_translation;         // true
 extern int32_t FLAGS_bingham_peak_neighbors;
 extern int32_t FLAGS_gaussian_peak_neighbors;
+extern int32_t FLAGS_bmm_peak_neighbors;               // 2
+extern int32_t FLAGS_gmm_peak_neighbors;               // 2
 // PLY field names (ply-helper.cc:8-22)
 extern std::string FLAGS_phaser_ply_intensity_str;     // "intensity"
 extern std::string FLAGS_phaser_ply_reflectivity_str;  // "reflectivity"
@@ -203,11 +205,13 @@ using VectorXd = std::vector<double>;  // stands in for Eigen::VectorXd
 // below and released with the last of them.
 // channels: 2, or 4 for clouds that carry reflectivity and ambient;
 // sph_pad: spherical zero padding of the SO(3) correlation; method and
-// skip_rot_axes: phz_config's.
+// skip_rot_axes: phz_config's; eval_mask bit 0 / 1: rot / pos evaluation
+// PHZ_EVAL_MIXTURE; spat_pad: spatial zero padding (-1: the flag's).
 std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points,
                                         int channels = 2, int sph_pad = 0,
                                         int method = 0 /* PHZ_METHOD_SPH_OPT */,
-                                        int skip_rot_axes = 0);
+                                        int skip_rot_axes = 0, int eval_mask = 0,
+                                        int spat_pad = -1);
 
 class SphericalCorrelation {
  public:
@@ -256,10 +260,11 @@ class SpatialCorrelation : public BaseSpatialCorrelation {
   double* correlateSignals(const std::vector<VectorXd*>& f,
                            const std::vector<VectorXd*>& g) override;
   uint32_t getZeroPadding() const override { return zero_padding_; }
-  uint32_t getCorrelationSize() const { return total_; }
+  // spatial-correlation.cc:219-221: (N + 2 pad)^3 in double, truncated
+  uint32_t getCorrelationSize() const { return total_padded_; }
 
  protected:
-  uint32_t n_voxels_, zero_padding_, total_;
+  uint32_t n_voxels_, zero_padding_, total_, total_padded_;
   std::shared_ptr<phz_ctx> ctx_;
   std::vector<double> c_;
 };
@@ -281,7 +286,10 @@ class PhaseAligner {
                        const model::PointCloud& cloud_reg,
                        const std::vector<model::FunctionValue>& f_reg);
   std::vector<double> getCorrelation() const { return previous_correlation_; }
-  uint32_t getNumberOfVoxels() const noexcept { return n_voxels_; }
+  // phase-aligner.cc:137-139: the padded voxel count
+  uint32_t getNumberOfVoxels() const noexcept {
+    return n_voxels_ + 2 * spatial_correlation_->getZeroPadding();
+  }
   int32_t getLowerBound() const noexcept { return lower_bound_; }
   int32_t getUpperBound() const noexcept { return upper_bound_; }
   // {intensity, range, reflectivity, ambient} means per voxel
@@ -344,6 +352,7 @@ class SphOptRegistration : public BaseRegistration {
                      int sph_pad);
   int bandwidth_;
   int method_, skip_rot_axes_, sph_pad_;
+  int eval_mask_ = 0;  // bit 0: rot evaluation "bmm", bit 1: pos evaluation "gmm"
   std::shared_ptr<phz_ctx> ctx_;
 };
 
@@ -351,10 +360,14 @@ class SphOptRegistration : public BaseRegistration {
 // sampled channel 0.6 intensity + 0.4 range instead of the Laplace fusion
 // (spherical-correlation.cc:54-112), bandwidth FLAGS_spherical_bandwith, no
 // spherical zero padding, and the refine_rot_x/y/z masks on the applied
-// rotation.  Evaluations "bingham" and "gaussian" (the defaults); "bmm"
-// fits its mixture to Glover samples of a random sampler
-// (bingham-mixture.cc:54-80) and "gmm" is its positional counterpart: not
-// part of this build.  Alignment "phase" only (the reference has no other).
+// rotation.  Evaluations "bingham" and "gaussian" (the defaults), or "bmm"
+// and "gmm" (phz_config.rot_eval / pos_eval = PHZ_EVAL_MIXTURE, with
+// FLAGS_bmm_peak_neighbors / FLAGS_gmm_peak_neighbors): with ZScoreEval's
+// single argmax signal the Gaussian mixture is the GMM window's Gaussian and
+// the Bingham mixture's one component is the BMM window's Bingham, whose mode
+// is returned (the reference refits random Metropolis-Hastings samples of
+// it, bingham-mixture.cc:55-78).  Alignment "phase" only (the reference has
+// no other).
 class SphRegistration : public SphOptRegistration {
  public:
   SphRegistration();

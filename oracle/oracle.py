@@ -87,6 +87,8 @@ def lib():
             "po_spatial_pad_inverse": (None, [C.c_int, C.c_int, C.c_int, dp, dp]),
             "po_spatial_correlate_laplace_pad": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
             "po_argmax": (C.c_uint32, [dp, C.c_size_t]),
+            "po_zscore_peaks": (C.c_size_t, [dp, C.c_size_t, C.c_double, C.c_uint32, C.c_double,
+                                             C.c_double, C.c_double, u32p, C.c_size_t]),
             "po_zyz_from_index": (None, [C.c_uint32, C.c_int, dp]),
             "po_quat_from_zyz": (None, [dp, dp]),
             "po_xyz_from_quat": (None, [dp, dp]),
@@ -390,9 +392,12 @@ def peak_window(idx, n_corr, k, kind):
     (gaussian-peak-based-eval.cc:26-37): no lower neighbours at all when
     k > idx; for the Bingham fit end = idx when idx + k > n_corr, and
     end == n_corr fails the reference's CHECK_LT(end, n) (:69); for the
-    Gaussian end = idx when idx + k >= n_corr."""
+    Gaussian end = idx when idx + k >= n_corr.  The mixture evaluations:
+    "bmm" as "bingham" (bmm-peak-based-eval.cc:46-56, CHECK_LT :68), "gmm"
+    with end = idx when idx + k > n_corr and CHECK_LT(end, n) (gmm-peak-
+    based-eval.cc:58-68, :81)."""
     start = idx if k > idx else idx - k
-    if kind == "bingham":
+    if kind in ("bingham", "bmm", "gmm"):
         end = idx if k + idx > n_corr else idx + k
         if end >= n_corr:
             raise PeakWindowError("peak window reaches past the grid")
@@ -422,32 +427,34 @@ def bingham_mode(samples, weights, ref=None):
     return -m if flip else m
 
 
-def rotation_window(bw, corr, idx, k):
+def rotation_window(bw, corr, idx, k, kind="bingham"):
     """BinghamPeakBasedEval::retrievePeakNeighbors (bingham-peak-based-eval.cc:
     60-90) on the window of peak_window: the quaternions (4 x m) of the
     window's indices (rotation-utils.cc:101-107,190-207) and their weights,
     the correlation values, normalised to sum 1 when m > 1 (CHECK_GT of the
-    sum, :88); also the peak's own quaternion."""
-    start, end = peak_window(idx, len(corr), k, "bingham")
+    sum, :88); also the peak's own quaternion.  kind "bmm"
+    (BmmPeakBasedEval::retrievePeakNeighbors, bmm-peak-based-eval.cc:58-86)
+    normalises and checks one sample too."""
+    start, end = peak_window(idx, len(corr), k, kind)
     peak = quat_from_zyz(zyz_from_index(idx, bw))
     samples = np.stack([quat_from_zyz(zyz_from_index(i, bw)) for i in range(start, end + 1)],
                        axis=1)
     w = np.asarray(corr[start:end + 1], np.float64)
-    if end > start:
+    if end > start or kind == "bmm":
         if not w.sum() > 0:
             raise PeakWindowError("weights do not sum to a positive value")
         w = w / w.sum()
     return samples, w, peak
 
 
-def refine_rotation(bw, corr, idx, k):
+def refine_rotation(bw, corr, idx, k, kind="bingham"):
     """BinghamPeakBasedEval::fitRotationalBinghamDistribution with k
     neighbours (bingham-peak-based-eval.cc:22-45): returns the mode quaternion
     (w, x, y, z), signed to agree with the peak's quaternion, and omega, the
     moment matrix's eigenvalues over their sum in ascending order
     (fitToMoment, bingham.cc:239-246).  One sample: the mode of w q q^T is q
     itself and omega = (0, 0, 0, 1)."""
-    samples, w, peak = rotation_window(bw, corr, idx, k)
+    samples, w, peak = rotation_window(bw, corr, idx, k, kind)
     if samples.shape[1] == 1:
         return peak, np.array([0.0, 0.0, 0.0, 1.0])
     ev = np.linalg.eigvalsh(bingham_moment(samples, w))
@@ -466,12 +473,12 @@ def gaussian_fit(samples, weights):
     return mu, Z @ Z.T
 
 
-def translation_window(N, corr, idx, k, lo=-50, hi=50):
+def translation_window(N, corr, idx, k, lo=-50, hi=50, kind="gaussian"):
     """GaussianPeakBasedEval::retrievePeakNeighbors (gaussian-peak-based-eval.cc:
     75-102) on the window of peak_window: the translations (3 x m) of the
     window's indices (signal-utils.cc:11-19, translation-utils.cc:40-52) and
     the correlation values normalised to sum 1 (CHECK_GT of the sum, :100)."""
-    start, end = peak_window(idx, len(corr), k, "gaussian")
+    start, end = peak_window(idx, len(corr), k, kind)
     t = []
     for i in range(start, end + 1):
         x, y, z = ind2sub(i, N, N)
@@ -483,10 +490,47 @@ def translation_window(N, corr, idx, k, lo=-50, hi=50):
     return np.asarray(t, np.float64).T, w / w.sum()
 
 
-def refine_translation(N, corr, idx, k, lo=-50, hi=50):
+def refine_translation(N, corr, idx, k, lo=-50, hi=50, kind="gaussian"):
     """GaussianPeakBasedEval::fitTranslationalNormalDist with k neighbours
-    (gaussian-peak-based-eval.cc:39-73): (mean, covariance) of the window."""
-    return gaussian_fit(*translation_window(N, corr, idx, k, lo, hi))
+    (gaussian-peak-based-eval.cc:39-73): (mean, covariance) of the window.
+    kind "gmm": GmmPeakBasedEval's window (gmm-peak-based-eval.cc:58-105);
+    its one-signal mixture is that Gaussian (gaussian_mixture)."""
+    return gaussian_fit(*translation_window(N, corr, idx, k, lo, hi, kind))
+
+
+def gaussian_mixture(means, covs, weights):
+    """GaussianMixture::calcMixutreParameters (gaussian-mixture.cc:57-78): the
+    mean and covariance of the component means under the weights (a
+    Gaussian(means, weights), gaussian.cc:14-22,48-56) plus the weighted sum
+    of the component covariances."""
+    mu, cov_mean = gaussian_fit(means, weights)
+    w = np.asarray(weights, np.float64)
+    return mu, sum(np.asarray(c, np.float64) * w[i] for i, c in enumerate(covs)) + cov_mean
+
+
+def refine_translation_gmm(N, corr, signals, k, lo=-50, hi=50):
+    """GmmPeakBasedEval::fitTranslationalGmmDistribution (gmm-peak-based-eval.
+    cc:27-56): one Gaussian per signal over its window, mixture weights the
+    signals' correlation values normalised to sum 1."""
+    comps = [refine_translation(N, corr, i, k, lo, hi, "gmm") for i in signals]
+    w = np.asarray([corr[i] for i in signals], np.float64)
+    w = w / w.sum()
+    return gaussian_mixture(np.stack([m for m, _ in comps], axis=1), [c for _, c in comps], w)
+
+
+ZSCORE_DEFAULTS = dict(lag_percentile=0.05, lag_max=20, threshold=4.33, influence=0.05,
+                      filter_threshold=0.405)  # z-score-peak-extraction.cc:12-25
+
+
+def zscore_peaks(corr, lag_percentile=0.05, lag_max=20, threshold=4.33, influence=0.05,
+                 filter_threshold=0.405, max_peaks=1 << 20):
+    """ZScorePeakExtraction::extractPeaks (po_zscore_peaks): the signal
+    indices in increasing order."""
+    c = _c(corr, np.float64)
+    out = np.zeros(max_peaks, np.uint32)
+    k = lib().po_zscore_peaks(c, c.size, lag_percentile, lag_max, threshold, influence,
+                              filter_threshold, out, max_peaks)
+    return out[: min(k, max_peaks)].copy()
 
 
 def ind2sub(idx, rows, cols):

@@ -1359,3 +1359,59 @@ void po_register_ex(const po_config* cfg, const float* target, const float* targ
   free(gv);
   if (!trans_corr) free(c);
 }
+
+/* ZScorePeakExtraction::extractPeaks + calculateSmoothedZScore
+ * (z-score-peak-extraction.cc:42-112), the smoothed z-score detector over a
+ * correlation vector.  lag = min(lag_max, (uint32)(n lag_percentile)) (:44-45);
+ * nothing when n <= lag + 2 (:57-58); the running mean and standard
+ * deviation (divisor lag - 1) of the window of the lag values before the
+ * last processed index, starting with [0, lag) (:68-72); from i = lag + 1,
+ * inputs <= filter_threshold are skipped without updating the window
+ * (:77-79); an input farther than threshold deviations from the mean is a
+ * signal when it is above the mean (:81-83) and is replaced by
+ * influence x + (1 - influence) * (the previous, possibly replaced, input)
+ * (:86); the window statistics then move to [i - lag, i) (:107-110).  The
+ * signals come out in increasing order (a std::set); at most max_peaks are
+ * stored, the return value counts all.  Window sums run in index order (the
+ * reference's Eigen reductions may add in another order: last-bit
+ * differences decide only exact threshold ties). */
+size_t po_zscore_peaks(const double* corr, size_t n, double lag_percentile, uint32_t lag_max,
+                       double threshold, double influence, double filter_threshold,
+                       uint32_t* peaks, size_t max_peaks) {
+  const uint32_t by_pct = (uint32_t)((double)n * lag_percentile);
+  const double lag = (double)(lag_max < by_pct ? lag_max : by_pct);
+  const uint32_t n_input = (uint32_t)n;
+  if (n_input <= lag + 2) return 0;
+  const uint32_t L = (uint32_t)lag;
+  double* in = (double*)xcalloc(n, sizeof(double));
+  memcpy(in, corr, n * sizeof(double));
+  double sum = 0.0;
+  for (uint32_t j = 0; j < L; ++j) sum += in[j];
+  double avg = sum / (double)L;
+  const double lagm1 = lag - 1;
+  double sq = 0.0;
+  for (uint32_t j = 0; j < L; ++j) sq += (in[j] - avg) * (in[j] - avg);
+  double dev = sqrt(sq / lagm1);
+  const double inv_influence = 1.0 - influence;
+  size_t count = 0;
+  for (uint32_t i = (uint32_t)(lag + 1u); i < n_input; ++i) {
+    const double x = in[i];
+    if (x <= filter_threshold) continue;
+    if (fabs(x - avg) > threshold * dev) {
+      if (x >= avg) {
+        if (count < max_peaks) peaks[count] = i;
+        ++count;
+      }
+      in[i] = influence * x + inv_influence * in[i - 1];
+    }
+    const uint32_t start = (uint32_t)(i - lag);
+    sum = 0.0;
+    for (uint32_t j = start; j < start + L; ++j) sum += in[j];
+    avg = sum / (double)L;
+    sq = 0.0;
+    for (uint32_t j = start; j < start + L; ++j) sq += (in[j] - avg) * (in[j] - avg);
+    dev = sqrt(sq / lagm1);
+  }
+  free(in);
+  return count;
+}

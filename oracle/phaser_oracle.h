@@ -181,6 +181,15 @@ void po_ind2sub(uint32_t idx, uint32_t rows, uint32_t cols, uint32_t* xyz);
 uint32_t po_sub2ind(uint32_t i, uint32_t j, uint32_t k, uint32_t rows,
                     uint32_t cols);
 
+/* ZScorePeakExtraction (z-score-peak-extraction.cc:42-112): the signals of
+ * a correlation vector in increasing order; stores at most max_peaks,
+ * returns the count.  Flags z_score_lag_percentile (0.05), z_score_lag_max
+ * (20), z_score_threshold (4.33), z_score_influence (0.05),
+ * z_score_filter_threshold (0.405) (:12-25). */
+size_t po_zscore_peaks(const double* corr, size_t n, double lag_percentile, uint32_t lag_max,
+                       double threshold, double influence, double filter_threshold,
+                       uint32_t* peaks, size_t max_peaks);
+
 /* ---- a21: SphOptRegistration::registerPointCloud ------------------------ */
 typedef struct {
   int bw;            /* phaser_core_spherical_bandwidth */

@@ -21,7 +21,9 @@ class Config(C.Structure):
                 ("max_points", C.c_int64), ("max_batch", C.c_int32), ("device", C.c_int32),
                 ("precision", C.c_int32), ("bingham_peak_neighbors", C.c_int32),
                 ("gaussian_peak_neighbors", C.c_int32), ("channels", C.c_int32),
-                ("method", C.c_int32), ("skip_rot_axes", C.c_int32)]
+                ("method", C.c_int32), ("skip_rot_axes", C.c_int32), ("rot_eval", C.c_int32),
+                ("pos_eval", C.c_int32), ("bmm_peak_neighbors", C.c_int32),
+                ("gmm_peak_neighbors", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -87,6 +89,9 @@ EXPORTS = {
     "phz_spatial_correlate_single_padded": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                                       C.c_void_p, C.c_void_p,
                                                       C.POINTER(C.c_uint32)]),
+    "phz_zscore_default_params": (None, [C.c_void_p]),
+    "phz_zscore_peaks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
+                                   C.c_int64, C.POINTER(C.c_int64)]),
     "phz_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
     "phz_get_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
     "phz_set_tuning": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -102,11 +107,17 @@ EXPORTS = {
 }
 
 
+class ZScoreParams(C.Structure):
+    _fields_ = [("lag_percentile", C.c_double), ("lag_max", C.c_uint32),
+                ("reserved", C.c_uint32), ("threshold", C.c_double), ("influence", C.c_double),
+                ("filter_threshold", C.c_double)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("fwd_x", C.c_int32), ("fwd_z", C.c_int32), ("inv_x", C.c_int32),
                 ("plane_fwd", C.c_int32), ("nn_lanes", C.c_int32), ("fused_reg", C.c_int32),
                 ("so3_reg", C.c_int32), ("occ_fwd", C.c_int32), ("inv_plane", C.c_int32),
-                ("nn_xcd", C.c_int32),
+                ("nn_xcd", C.c_int32), ("nn_seed", C.c_int32),
                 ("ms_fwd_x", C.c_float), ("ms_fwd_z", C.c_float),
                 ("ms_inv_x", C.c_float), ("ms_plane", C.c_float), ("ms_fused", C.c_float),
                 ("ms_so3", C.c_float)]
@@ -334,9 +345,9 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, fwd_x=0, fwd_z=0, inv_x=0, plane_fwd=0, nn_lanes=0, fused_reg=0,
-                   so3_reg=0, occ_fwd=0, inv_plane=0, nn_xcd=0):
+                   so3_reg=0, occ_fwd=0, inv_plane=0, nn_xcd=0, nn_seed=0):
         t = Tuning(fwd_x, fwd_z, inv_x, plane_fwd, nn_lanes, fused_reg, so3_reg, occ_fwd,
-                   inv_plane, nn_xcd)
+                   inv_plane, nn_xcd, nn_seed)
         _check(lib().phz_set_tuning(self.h, C.byref(t)))
 
     def set_graphs(self, on=True):
@@ -404,6 +415,21 @@ class Context:
         am = C.c_uint32()
         _check(lib().phz_spatial_correlate_single(self.h, N, _p(f), _p(g), _p(corr), C.byref(am)))
         return corr, int(am.value)
+
+    def zscore_peaks(self, corr, max_peaks=1 << 20, **kw):
+        """ZScorePeakExtraction on the device; kw: lag_percentile, lag_max,
+        threshold, influence, filter_threshold (the flags' defaults
+        otherwise).  Returns the signal indices."""
+        p = ZScoreParams()
+        lib().phz_zscore_default_params(C.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        c = _f32(corr).reshape(-1)
+        out = np.zeros(max(max_peaks, 1), np.uint32)
+        cnt = C.c_int64()
+        _check(lib().phz_zscore_peaks(self.h, _p(c), c.size, C.byref(p), _p(out), max_peaks,
+                                      C.byref(cnt)))
+        return out[: min(cnt.value, max_peaks)].copy()
 
     def spatial_correlate_single_padded(self, f, g, N, pad):
         f = _f32(f).reshape(-1)

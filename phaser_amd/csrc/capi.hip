@@ -73,6 +73,9 @@ constexpr int kMaxBandwidth = 160;
 // XCD-ordered S^2 nearest-neighbour workgroups by default (measured on
 // gfx950 at B = 64, 64 clouds per launch: k_nn 1.34 -> 1.18 ms)
 constexpr bool kNnXcdDefault = true;
+// the exact NN search seeded by a first pass over every 4th query row and
+// column (the full pass then prunes from its first band)
+constexpr int kNnSeedStride = 4;
 
 struct phz_ctx {
   phz_config cfg{};
@@ -100,6 +103,7 @@ struct phz_ctx {
   bool occ_fwd = true;     // forward passes skip rows / planes without points
   bool inv_plane = true;   // inverse x + c2r as one pass over z planes in LDS
   bool nn_xcd = kNnXcdDefault;  // XCD-ordered NN workgroups
+  int nn_seed = kNnSeedStride;  // stride of the NN seeding pass (0: off)
   phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
@@ -256,6 +260,15 @@ int upload(phz_ctx* c, T** p, const std::vector<T>& v) {
     if (_rc) return _rc;      \
   } while (0)
 
+// peak-window neighbours of the chosen evaluations: BinghamPeakBasedEval /
+// BmmPeakBasedEval and GaussianPeakBasedEval / GmmPeakBasedEval
+int rot_neighbors(const phz_config& cf) {
+  return cf.rot_eval == PHZ_EVAL_MIXTURE ? cf.bmm_peak_neighbors : cf.bingham_peak_neighbors;
+}
+int pos_neighbors(const phz_config& cf) {
+  return cf.pos_eval == PHZ_EVAL_MIXTURE ? cf.gmm_peak_neighbors : cf.gaussian_peak_neighbors;
+}
+
 // Device bytes init_ctx allocates for a configuration, in closed form (the
 // same list of buffers; init_ctx checks that the two agree).
 size_t context_bytes(const phz_config& cf) {
@@ -299,6 +312,10 @@ size_t context_bytes(const phz_config& cf) {
   add(M * cap, sizeof(float4));
   add(M * cap, sizeof(float2));
   add(M * cap, sizeof(int));
+  {
+    const int64_t nc = (n + kNnSeedStride - 1) / kNnSeedStride;
+    add(M * nc * nc, sizeof(int));
+  }
   // spherical spectra and SO(3) grids
   add(G * nq, sizeof(float));
   add(G * B * n, sizeof(float2));
@@ -324,8 +341,8 @@ size_t context_bytes(const phz_config& cf) {
     add(Np, sizeof(float2));
     add(P * Hp, sizeof(float2));
   }
-  if (cf.bingham_peak_neighbors > 0) add(P * no * no * no, sizeof(float));
-  if (cf.gaussian_peak_neighbors > 0) add(P * T3p, sizeof(float));
+  if (rot_neighbors(cf) > 0) add(P * no * no * no, sizeof(float));
+  if (pos_neighbors(cf) > 0) add(P * T3p, sizeof(float));
   if (cf.channels == 4) {
     add(M * cap, sizeof(float2));
     add(M * T3, sizeof(float));
@@ -352,6 +369,14 @@ int check_config(const phz_config& cf) {
             "bingham_peak_neighbors must be in [0, 2^20]");
   PHZ_CHECK(cf.gaussian_peak_neighbors >= 0 && cf.gaussian_peak_neighbors <= (1 << 20),
             "gaussian_peak_neighbors must be in [0, 2^20]");
+  PHZ_CHECK(cf.bmm_peak_neighbors >= 0 && cf.bmm_peak_neighbors <= (1 << 20),
+            "bmm_peak_neighbors must be in [0, 2^20]");
+  PHZ_CHECK(cf.gmm_peak_neighbors >= 0 && cf.gmm_peak_neighbors <= (1 << 20),
+            "gmm_peak_neighbors must be in [0, 2^20]");
+  PHZ_CHECK(cf.rot_eval == PHZ_EVAL_BINGHAM || cf.rot_eval == PHZ_EVAL_MIXTURE,
+            "rot_eval must be PHZ_EVAL_BINGHAM (0) or PHZ_EVAL_MIXTURE (1)");
+  PHZ_CHECK(cf.pos_eval == PHZ_EVAL_GAUSSIAN || cf.pos_eval == PHZ_EVAL_MIXTURE,
+            "pos_eval must be PHZ_EVAL_GAUSSIAN (0) or PHZ_EVAL_MIXTURE (1)");
   PHZ_CHECK(cf.channels == 0 || cf.channels == 2 || cf.channels == 4,
             "channels must be 2 or 4 (0 means 2)");
   PHZ_CHECK(cf.method == PHZ_METHOD_SPH_OPT || cf.method == PHZ_METHOD_SPH,
@@ -463,6 +488,10 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));
+  {
+    const size_t nc = (size_t)(c->n + kNnSeedStride - 1) / kNnSeedStride;
+    TRY(dalloc(c, &c->cells.seed_pos, (size_t)M * nc * nc));
+  }
   // stage entry points use up to 16 grids / signals; with four channels the
   // reflectivity and ambient grids of all clouds follow the range and
   // intensity grids
@@ -497,9 +526,9 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_corr_trans, (size_t)c->T3p));
   TRY(dalloc(c, &c->d_reg, (size_t)cap));
   if (c->sp) TRY(dalloc(c, &c->d_work_pad, (size_t)P * c->Hp));
-  if (cf.bingham_peak_neighbors > 0)
+  if (rot_neighbors(cf) > 0)
     TRY(dalloc(c, &c->d_peak_rot, (size_t)P * no * no * no));
-  if (cf.gaussian_peak_neighbors > 0)
+  if (pos_neighbors(cf) > 0)
     TRY(dalloc(c, &c->d_peak_trans, (size_t)P * c->T3p));
   if (cf.channels == 4) {
     TRY(dalloc(c, &c->d_extra, (size_t)M * cap));
@@ -535,6 +564,8 @@ int init_ctx(phz_ctx* c) {
   c->tuning.inv_plane = 1;
   c->nn_xcd = kNnXcdDefault;
   c->tuning.nn_xcd = kNnXcdDefault ? 1 : -1;
+  c->nn_seed = kNnSeedStride;
+  c->tuning.nn_seed = 1;
   return 0;
 }
 
@@ -571,7 +602,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
               float* d_trans_corr) {
   // the peak-window fits read the grids: written to the context's own
   // buffers when the caller wants none
-  const int kb = c->cfg.bingham_peak_neighbors, kg = c->cfg.gaussian_peak_neighbors;
+  const int kb = rot_neighbors(c->cfg), kg = pos_neighbors(c->cfg);
+  const bool bmm = c->cfg.rot_eval == PHZ_EVAL_MIXTURE, gmm = c->cfg.pos_eval == PHZ_EVAL_MIXTURE;
   if (!d_rot_corr && kb > 0) d_rot_corr = c->d_peak_rot;
   if (!d_trans_corr && kg > 0) d_trans_corr = c->d_peak_trans;
   const int nt = shared ? 1 : np;
@@ -595,7 +627,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     StageScope t(c, kS2NN, s);
     launch_s2_nn(c->cap, ncl, n, c->d_grid, c->d_theta, c->max_sq, c->cells, c->d_grids,
                  c->tuning.nn_lanes, s, four && !sph ? c->d_extra : nullptr,
-                 four && !sph ? c->d_grids + 2 * ncl * nq : nullptr, c->nn_xcd, sph);
+                 four && !sph ? c->d_grids + 2 * ncl * nq : nullptr, c->nn_xcd, sph,
+                 c->nn_seed);
   }
   {
     StageScope t(c, kS2Transform, s);
@@ -639,7 +672,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   }
   {
     StageScope t(c, kRotPose, s);
-    launch_rot_pose(Bo, np, c->d_keys, d_rot_corr, no3, kb, c->d_pose, s, c->cfg.skip_rot_axes);
+    launch_rot_pose(Bo, np, c->d_keys, d_rot_corr, no3, kb, c->d_pose, s, c->cfg.skip_rot_axes,
+                    bmm);
   }
   // translation (accumulators are zero on entry and cleared again below)
   {
@@ -786,7 +820,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     // with padding the window and the metres use Np voxels and the
     // correlation vector's Tc entries (phase-correlation-eval.cc:19-27)
     launch_trans_pose(c->Np, c->cfg.discretize_lower, c->cfg.discretize_upper, np,
-                      c->d_keys + np, d_trans_corr, c->T3p, kg, c->d_pose, s, c->Tc);
+                      c->d_keys + np, d_trans_corr, c->T3p, kg, c->d_pose, s, c->Tc, gmm);
   }
   PHZ_HIP(hipGetLastError());
   return 0;
@@ -836,6 +870,9 @@ int exec_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* rot, float
 }
 
 std::string pair_status_text(int32_t st) {
+  if (st & kPoseTransWindow)
+    return "the Gaussian-mixture peak window reaches past the translation grid "
+           "(gmm-peak-based-eval.cc:81 CHECK_LT(end, n))";
   if (st & kPoseRotWindow)
     return "the Bingham peak window reaches past the rotation grid "
            "(bingham-peak-based-eval.cc:69 CHECK_LT(end, n))";
@@ -866,7 +903,8 @@ void to_result(const DevPose& d, phz_result* r) {
   }
   for (int i = 0; i < 9; ++i) r->trans_cov[i] = d.trans_cov[i];
   static_assert(kPoseRotWindow == PHZ_PAIR_ROT_WINDOW && kPoseRotWeights == PHZ_PAIR_ROT_WEIGHTS &&
-                    kPoseTransWeights == PHZ_PAIR_TRANS_WEIGHTS,
+                    kPoseTransWeights == PHZ_PAIR_TRANS_WEIGHTS &&
+                    kPoseTransWindow == PHZ_PAIR_TRANS_WINDOW,
                 "pose status bits are the API's");
   r->status = d.status;
   r->reserved = 0;
@@ -934,9 +972,21 @@ void phz_default_config(phz_config* cfg) {
   cfg->max_batch = 1;
   cfg->device = 0;
   cfg->channels = 2;
+  cfg->bmm_peak_neighbors = 2;  // bmm-peak-based-eval.cc:7-9
+  cfg->gmm_peak_neighbors = 2;  // gmm-peak-based-eval.cc:12-14
 }
 
 const char* phz_last_error(void) { return g_last_error.c_str(); }
+
+void phz_zscore_default_params(phz_zscore_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->lag_percentile = 0.05;  // z-score-peak-extraction.cc:12-25
+  p->lag_max = 20;
+  p->threshold = 4.33;
+  p->influence = 0.05;
+  p->filter_threshold = 0.405;
+}
 
 int phz_create(const phz_config* cfg, phz_ctx** out) {
   PHZ_CHECK(cfg != nullptr && out != nullptr, "null argument");
@@ -1191,7 +1241,6 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
                           uint32_t* argmax) {
   PHZ_CHECK(c && f && g, "null argument");
   PHZ_CHECK(C >= 2 && C <= 8, "channel count must be in [2, 8]");
-  PHZ_CHECK(c->sp == 0, "the spatial stage entry point runs without spatial zero padding");
   hipStream_t s = c->stream;
   const int64_t T = c->T3, H = c->H;
   PHZ_HIP(hipMemcpyAsync(c->d_sumI, f, sizeof(float) * C * T, hipMemcpyHostToDevice, s));
@@ -1204,8 +1253,19 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
     F.ch[k] = c->d_spec + k * H;
     G.ch[k] = c->d_spec + (C + k) * H;
   }
-  launch_spatial_inverse(c->N, 1, F, 0, G, 0, c->vox.runs, true, c->plan_N, c->d_tw_N, c->d_work,
-                         corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys, c->tiles, s);
+  if (c->sp) {  // the context's spatial zero padding: grid (N + 2 pad)^3
+    launch_spatial_cross_product(c->N, 1, F, 0, G, 0, c->vox.runs, true, c->d_work, s);
+    launch_spatial_pad_gather(c->N, c->sp, true, 1, c->d_work, c->d_work_pad, s);
+    const int tile = fft_cols_default_tile(c->Np);
+    launch_spatial_fft_pass(c->Np, 1, c->d_work_pad, 3, tile, c->plan_Np, c->d_tw_Np, s);
+    launch_spatial_fft_pass(c->Np, 1, c->d_work_pad, 2, tile, c->plan_Np, c->d_tw_Np, s);
+    launch_spatial_c2r(c->Np, 1, c->d_work_pad, c->plan_Np, c->d_tw_Np,
+                       corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys, s, c->Tc);
+  } else {
+    launch_spatial_inverse(c->N, 1, F, 0, G, 0, c->vox.runs, true, c->plan_N, c->d_tw_N,
+                           c->d_work, corr ? c->d_corr_trans : nullptr, c->d_partials, c->d_keys,
+                           c->tiles, s);
+  }
   // the dense channels were staged in the voxel accumulators: zero them again
   launch_fill_u32((uint32_t*)c->d_sumI, (size_t)2 * C * T, 0u, s);
   PHZ_HIP(hipGetLastError());
@@ -1213,7 +1273,7 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
   PHZ_HIP(hipMemcpyAsync(&key, c->d_keys, sizeof(key), hipMemcpyDeviceToHost, s));
   PHZ_HIP(hipStreamSynchronize(s));
   if (argmax) *argmax = key_index(key);
-  if (corr) TRY(copy_sync(c, corr, c->d_corr_trans, sizeof(float) * T, hipMemcpyDeviceToHost));
+  if (corr) TRY(copy_sync(c, corr, c->d_corr_trans, sizeof(float) * c->T3p, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -1300,6 +1360,56 @@ static int spatial_single(phz_ctx* c, int N, int pad, const float* f, const floa
   hipFree(d_twp);
   hipFree(d_corr);
   hipFree(d_key);
+  return rc;
+}
+
+int phz_zscore_peaks(phz_ctx* c, const float* corr, int64_t n, const phz_zscore_params* zp,
+                     uint32_t* peaks, int64_t max_peaks, int64_t* count) {
+  PHZ_CHECK(c && corr && zp && count && (peaks || max_peaks == 0), "null argument");
+  PHZ_CHECK(n >= 0 && n <= 0xffffffffll, "vector length must fit uint32 (the reference's n_input)");
+  PHZ_CHECK(max_peaks >= 0, "max_peaks must be >= 0");
+  PHZ_CHECK(zp->lag_percentile > 0 && zp->lag_percentile < 1,
+            "z_score_lag_percentile must be in (0, 1)");  // z-score-peak-extraction.cc:35-36
+  PHZ_CHECK(zp->threshold > 0, "z_score_threshold must be > 0");
+  PHZ_CHECK(zp->influence >= 0 && zp->influence <= 1, "z_score_influence must be in [0, 1]");
+  // :44-45: lag = min(lag_max, (uint32)(n * lag_percentile))
+  const uint32_t by_pct = (uint32_t)((double)n * zp->lag_percentile);
+  const uint32_t lag = zp->lag_max < by_pct ? zp->lag_max : by_pct;
+  PHZ_CHECK(lag <= 64, "the device detector keeps windows of at most 64 values (lag)");
+  hipStream_t s = c->stream;
+  float* d_corr = nullptr;
+  uint32_t* d_peaks = nullptr;
+  unsigned long long* d_count = nullptr;
+  int rc = 0;
+  hipError_t e;
+  auto fail = [&](const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    rc = -2;
+  };
+  if ((e = hipMalloc(&d_corr, sizeof(float) * (n ? n : 1))) != hipSuccess) fail("hipMalloc");
+  if (!rc && (e = hipMalloc(&d_peaks, sizeof(uint32_t) * (max_peaks ? max_peaks : 1))) != hipSuccess)
+    fail("hipMalloc");
+  if (!rc && (e = hipMalloc(&d_count, sizeof(unsigned long long))) != hipSuccess) fail("hipMalloc");
+  unsigned long long k = 0;
+  if (!rc) {
+    if (n) hipMemcpyAsync(d_corr, corr, sizeof(float) * n, hipMemcpyHostToDevice, s);
+    launch_zscore_peaks(d_corr, n, 1, (uint32_t)n, (int)lag, zp->threshold, zp->influence,
+                        zp->filter_threshold, d_peaks, max_peaks, d_count, s);
+    if ((e = hipGetLastError()) != hipSuccess) fail("launch");
+  }
+  if (!rc && (e = hipMemcpyAsync(&k, d_count, sizeof(k), hipMemcpyDeviceToHost, s)) != hipSuccess)
+    fail("copy");
+  if (!rc && (e = hipStreamSynchronize(s)) != hipSuccess) fail("sync");
+  const int64_t stored = (int64_t)k < max_peaks ? (int64_t)k : max_peaks;
+  if (!rc && stored > 0 &&
+      ((e = hipMemcpyAsync(peaks, d_peaks, sizeof(uint32_t) * stored, hipMemcpyDeviceToHost, s)) !=
+           hipSuccess ||
+       (e = hipStreamSynchronize(s)) != hipSuccess))
+    fail("copy");
+  if (!rc) *count = (int64_t)k;
+  hipFree(d_corr);
+  hipFree(d_peaks);
+  hipFree(d_count);
   return rc;
 }
 
@@ -1398,6 +1508,8 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.inv_plane = c->inv_plane ? 1 : -1;
   c->nn_xcd = t.nn_xcd == 0 ? kNnXcdDefault : t.nn_xcd > 0;
   c->tuning.nn_xcd = c->nn_xcd ? 1 : -1;
+  c->nn_seed = t.nn_seed >= 0 ? kNnSeedStride : 0;
+  c->tuning.nn_seed = c->nn_seed ? 1 : -1;
 }
 
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {
@@ -1551,6 +1663,7 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   }
   best.nn_lanes = c->tuning.nn_lanes;  // data-dependent: set with phz_set_tuning
   best.nn_xcd = c->tuning.nn_xcd;
+  best.nn_seed = c->tuning.nn_seed;
   best.ms_fwd_x = best_ms[0];
   best.ms_fwd_z = best_ms[1];
   best.ms_inv_x = best_ms[2];

@@ -33,6 +33,8 @@ bool FLAGS_refine_rot_x = true, FLAGS_refine_rot_y = true, FLAGS_refine_rot_z = 
 bool FLAGS_estimate_rotation = true, FLAGS_estimate_translation = true;
 int32_t FLAGS_bingham_peak_neighbors = 0;
 int32_t FLAGS_gaussian_peak_neighbors = 0;
+int32_t FLAGS_bmm_peak_neighbors = 2;  // bmm-peak-based-eval.cc:7-9
+int32_t FLAGS_gmm_peak_neighbors = 2;  // gmm-peak-based-eval.cc:12-14
 std::string FLAGS_phaser_ply_intensity_str = "intensity";
 std::string FLAGS_phaser_ply_reflectivity_str = "reflectivity";
 std::string FLAGS_phaser_ply_ambient_str = "noise";
@@ -63,9 +65,12 @@ constexpr int64_t kDefaultCapacity = 262144;
 namespace phaser_core {
 
 std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points, int channels,
-                                        int sph_pad, int method, int skip_rot_axes) {
+                                        int sph_pad, int method, int skip_rot_axes,
+                                        int eval_mask, int spat_pad) {
+  if (spat_pad < 0) spat_pad = FLAGS_phaser_core_spatial_zero_padding;
   static std::mutex mu;
-  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int, int, int, int>,
+  static std::map<std::tuple<int, int, int64_t, int, int, int, int, int, int, int, int, int, int,
+                             int, int, int, int>,
                   std::weak_ptr<phz_ctx>>
       cache;
   std::lock_guard<std::mutex> lock(mu);
@@ -75,7 +80,9 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
                                    FLAGS_phaser_core_spatial_low_pass_lower_bound,
                                    FLAGS_phaser_core_spatial_low_pass_upper_bound,
                                    FLAGS_bingham_peak_neighbors, FLAGS_gaussian_peak_neighbors,
-                                   channels, sph_pad, method, skip_rot_axes);
+                                   channels, sph_pad, method, skip_rot_axes,
+                                   spat_pad, eval_mask,
+                                   FLAGS_bmm_peak_neighbors, FLAGS_gmm_peak_neighbors);
   if (auto p = cache[key].lock()) return p;
   phz_config cfg;
   phz_default_config(&cfg);
@@ -92,6 +99,11 @@ std::shared_ptr<phz_ctx> acquireContext(int bw, int n_voxels, int64_t max_points
   cfg.spherical_zero_padding = sph_pad;
   cfg.method = method;
   cfg.skip_rot_axes = skip_rot_axes;
+  cfg.spatial_zero_padding = spat_pad;
+  cfg.rot_eval = (eval_mask & 1) ? PHZ_EVAL_MIXTURE : PHZ_EVAL_BINGHAM;
+  cfg.pos_eval = (eval_mask & 2) ? PHZ_EVAL_MIXTURE : PHZ_EVAL_GAUSSIAN;
+  cfg.bmm_peak_neighbors = FLAGS_bmm_peak_neighbors;
+  cfg.gmm_peak_neighbors = FLAGS_gmm_peak_neighbors;
   phz_ctx* raw = nullptr;
   check(phz_create(&cfg, &raw), "phz_create");
   std::shared_ptr<phz_ctx> p(raw, [](phz_ctx* c) { phz_destroy(c); });
@@ -156,10 +168,15 @@ void SphericalCorrelationLaplace::correlateSampledSignals(const std::vector<Samp
 }
 
 // ---------------------------------------------------------------- spatial
+// spatial-correlation.cc:11-61: the correlation of N + 2 pad voxels per
+// axis, of which the first getCorrelationSize() entries are the vector
 SpatialCorrelation::SpatialCorrelation(const uint32_t n_voxels, const uint32_t zero_padding)
     : n_voxels_(n_voxels), zero_padding_(zero_padding), total_(n_voxels * n_voxels * n_voxels) {
-  CHECK_TRUE(zero_padding == 0, "spatial zero padding other than 0 is not supported");
-  ctx_ = acquireContext(FLAGS_phaser_core_spherical_bandwidth, (int)n_voxels, kDefaultCapacity);
+  const uint32_t per_dim = 2u * zero_padding;
+  const double ratio = (n_voxels + per_dim) / static_cast<double>(n_voxels);
+  total_padded_ = zero_padding ? static_cast<uint32_t>(total_ * (ratio * ratio * ratio)) : total_;
+  ctx_ = acquireContext(FLAGS_phaser_core_spherical_bandwidth, (int)n_voxels, kDefaultCapacity, 2,
+                        0, 0, 0, 0, (int)zero_padding);
 }
 
 double* SpatialCorrelation::correlateSignals(const std::vector<VectorXd*>& f,
@@ -167,10 +184,11 @@ double* SpatialCorrelation::correlateSignals(const std::vector<VectorXd*>& f,
   CHECK_TRUE(!f.empty() && !g.empty(), "no channels");
   CHECK_TRUE(f[0]->size() == total_ && g[0]->size() == total_, "signal size is not N^3");
   std::vector<float> a(f[0]->begin(), f[0]->end()), b(g[0]->begin(), g[0]->end());
-  std::vector<float> corr(total_);
-  check(phz_spatial_correlate_single(ctx_.get(), (int)n_voxels_, a.data(), b.data(), corr.data(),
-                                     nullptr),
-        "phz_spatial_correlate_single");
+  const size_t np = (size_t)n_voxels_ + 2u * zero_padding_;
+  std::vector<float> corr(np * np * np);
+  check(phz_spatial_correlate_single_padded(ctx_.get(), (int)n_voxels_, (int)zero_padding_,
+                                            a.data(), b.data(), corr.data(), nullptr),
+        "phz_spatial_correlate_single_padded");
   c_.assign(corr.begin(), corr.end());
   return c_.data();
 }
@@ -187,7 +205,8 @@ double* SpatialCorrelationLaplace::correlateSignals(const std::vector<VectorXd*>
       b[(size_t)c * total_ + i] = (float)(*g[c])[i];
     }
   }
-  std::vector<float> corr(total_);
+  const size_t np = (size_t)n_voxels_ + 2u * zero_padding_;
+  std::vector<float> corr(np * np * np);
   check(phz_spatial_correlate(ctx_.get(), C, a.data(), b.data(), corr.data(), nullptr),
         "phz_spatial_correlate");
   c_.assign(corr.begin(), corr.end());
@@ -248,9 +267,8 @@ SphOptRegistration::SphOptRegistration(const std::string& name, int bandwidth, i
       skip_rot_axes_(skip_rot_axes),
       sph_pad_(sph_pad) {
   CHECK_TRUE(sph_pad_ >= 0, "negative spherical zero padding");
-  CHECK_TRUE(FLAGS_phaser_core_spatial_zero_padding == 0, "spatial zero padding other than 0 is not supported");
   ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
-                        sph_pad_, method_, skip_rot_axes_);
+                        sph_pad_, method_, skip_rot_axes_, eval_mask_);
   check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
 }
 
@@ -278,15 +296,18 @@ SphRegistration::SphRegistration(std::string&& alignment_algorithm,
   // sph-registration.cc:56-79
   CHECK_TRUE(alignment_algorithm == "phase",
              "unknown alignment algorithm '" + alignment_algorithm + "' (phase only)");
-  if (rot_evaluation_algorithm == "bmm")
-    fatal("rot evaluation 'bmm' (a Bingham mixture fitted to random Glover samples, "
-          "bingham-mixture.cc:54-80) is not part of this build");
-  CHECK_TRUE(rot_evaluation_algorithm == "bingham",
+  CHECK_TRUE(rot_evaluation_algorithm == "bingham" || rot_evaluation_algorithm == "bmm",
              "Unknown rot evaluation algorithm specificed: " + rot_evaluation_algorithm);
-  if (pos_evaluation_algorithm == "gmm")
-    fatal("pos evaluation 'gmm' is not part of this build");
-  CHECK_TRUE(pos_evaluation_algorithm == "gaussian",
+  CHECK_TRUE(pos_evaluation_algorithm == "gaussian" || pos_evaluation_algorithm == "gmm",
              "Unknown pos evaluation algorithm specificed: " + pos_evaluation_algorithm);
+  const int mask = (rot_evaluation_algorithm == "bmm" ? 1 : 0) |
+                   (pos_evaluation_algorithm == "gmm" ? 2 : 0);
+  if (mask != eval_mask_) {
+    eval_mask_ = mask;
+    ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
+                          sph_pad_, method_, skip_rot_axes_, eval_mask_);
+    check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
+  }
 }
 
 model::RegistrationResult SphRegistration::registerPointCloud(model::PointCloudPtr cloud_prev,
@@ -299,7 +320,7 @@ model::RegistrationResult SphRegistration::registerPointCloud(model::PointCloudP
   if (mask != skip_rot_axes_) {
     skip_rot_axes_ = mask;
     ctx_ = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, kDefaultCapacity, 2,
-                          sph_pad_, method_, skip_rot_axes_);
+                          sph_pad_, method_, skip_rot_axes_, eval_mask_);
     check(phz_set_profiling(ctx_.get(), 1), "phz_set_profiling");
   }
   if (FLAGS_estimate_translation)
@@ -333,7 +354,7 @@ model::RegistrationResult SphOptRegistration::registerPointCloud(model::PointClo
   std::shared_ptr<phz_ctx> ctx = ctx_;
   if (need > kDefaultCapacity || four)
     ctx = acquireContext(bandwidth_, FLAGS_phaser_core_spatial_n_voxels, need, four ? 4 : 2,
-                         sph_pad_, method_, skip_rot_axes_);
+                         sph_pad_, method_, skip_rot_axes_, eval_mask_);
   auto extra_of = [](model::PointCloud& c) {
     std::vector<float> e(2 * c.size(), 0.f);
     for (size_t i = 0; i < c.size(); ++i) {

@@ -27,6 +27,19 @@ struct CellBuffers {
   float4* proj;      // nclouds * cap
   float2* val;       // nclouds * cap
   int* cell;         // nclouds * cap
+  int* seed_pos;     // nclouds * ceil(n / S)^2: sorted position of each coarse
+                     // query's nearest point (-1: none), null = no seeding
+};
+
+// Seeding of the exact nearest-neighbour search: pos holds, per cloud, the
+// nearest point (sorted position) of the coarse queries at every stride-th
+// row and column; record = 1: this launch runs only those queries and
+// writes pos; record = 0 and stride > 0: every query starts from the
+// nearest of its four surrounding coarse queries' points.
+struct NnSeeds {
+  int* pos = nullptr;
+  int stride = 0;
+  int record = 0;
 };
 
 void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
@@ -47,10 +60,12 @@ int nn_auto_lanes(int nclouds);
 // (needs a multiple of 8 clouds; otherwise the plain 2-D grid).
 // blend: one grid per cloud, 0.6 intensity + 0.4 range (SphRegistration's
 // channel, spherical-correlation.cc:103-112), at grids + cloud * n*n.
+// seed_stride > 1 (with cb.seed_pos): two passes, the coarse queries at that
+// stride first, then every query seeded by them (the same exact result)
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
                   const float2* extra = nullptr, float* xgrids = nullptr, bool xcd = false,
-                  bool blend = false);
+                  bool blend = false, int seed_stride = 0);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B
@@ -98,6 +113,13 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
                           unsigned long long* partials, unsigned long long* keys,
                           hipStream_t s, bool reg = false);
 
+// ZScorePeakExtraction (kernels_peaks.hip): nvec vectors of n floats,
+// stride apart; lag <= 64; signals of vector v in increasing order at
+// peaks + v * max_peaks (at most max_peaks stored), their count in counts[v]
+void launch_zscore_peaks(const float* corr, int64_t stride, int nvec, uint32_t n, int lag,
+                         double threshold, double influence, double filter, uint32_t* peaks,
+                         int64_t max_peaks, unsigned long long* counts, hipStream_t s);
+
 // Device-side pose of each pair: the argmaxes, the Bingham and Gaussian
 // fits to their peak windows and the float transform T (row-major 4x4)
 // applied to the source.  status: kPose* bits of the checks the reference
@@ -119,6 +141,7 @@ struct DevPose {
 constexpr int32_t kPoseRotWindow = 1;   // Bingham window end >= (2B)^3
 constexpr int32_t kPoseRotWeights = 2;  // Bingham window weights sum <= 0
 constexpr int32_t kPoseTransWeights = 4;  // Gaussian window weights sum <= 0
+constexpr int32_t kPoseTransWindow = 8;   // GMM window end >= N^3
 
 // One wavefront per pair.  corr: the pairs' (2B)^3 grids at corr_stride
 // floats apart (read only when k > 0); k: bingham_peak_neighbors.
@@ -127,10 +150,12 @@ constexpr int32_t kPoseTransWeights = 4;  // Gaussian window weights sum <= 0
 // (spherical zero padding: degree-limited coefficients at a larger bandwidth)
 void launch_pad_coef(const float2* in, int64_t in_stride, const int* map, int64_t count,
                      float2* out, int nsets, hipStream_t s);
-// skip_axes: bits 0/1/2 leave the x/y/z Euler angle out of the transform
+// skip_axes: bits 0/1/2 leave the x/y/z Euler angle out of the transform;
+// mixture: BmmPeakBasedEval's rules (weights normalised and checked also
+// for a single sample)
 void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
                      int64_t corr_stride, int k, DevPose* pose, hipStream_t s,
-                     int skip_axes = 0);
+                     int skip_axes = 0, bool mixture = false);
 
 // ------------------------------------------------------------- spatial
 // voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3
@@ -303,9 +328,12 @@ void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int ncl
 // apart (read only when k > 0); k: gaussian_peak_neighbors.
 // n_corr: entries of the correlation vector (0 = N^3; with spatial zero
 // padding the reference's truncated padded size), the Gaussian window's end
+// mixture: GmmPeakBasedEval's window rule (end = peak when peak + k > n_corr,
+// an end of n_corr fails kPoseTransWindow)
 void launch_trans_pose(int N, int lower, int upper, int npairs,
                        const unsigned long long* keys, const float* corr, int64_t corr_stride,
-                       int k, DevPose* pose, hipStream_t s, uint32_t n_corr = 0);
+                       int k, DevPose* pose, hipStream_t s, uint32_t n_corr = 0,
+                       bool mixture = false);
 // out = T_rot * p + translation (PCL scalar order), intensity kept
 void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,
                        hipStream_t s);

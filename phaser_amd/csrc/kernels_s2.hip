@@ -278,7 +278,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ sorted_val, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
-                     float* __restrict__ xgrids, int xcd_blocks, int blend) {
+                     float* __restrict__ xgrids, int xcd_blocks, int blend, NnSeeds seeds) {
   constexpr int kBatch = 4;  // point loads in flight per lane
   // xcd_blocks > 0: a 1-D grid of xcd_blocks query blocks per cloud, where
   // consecutive workgroups go to the 8 XCDs in turn: XCD x takes clouds
@@ -295,8 +295,12 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     qb = blockIdx.x;
   }
   const int sub = threadIdx.x & (LANES - 1);
-  const int q = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
+  const int ql = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
+  // seeds.record: this pass runs the coarse queries (every S-th row and
+  // column, nc = ceil(n / S) of each) and records their nearest points
+  const int S = seeds.stride, nc = S > 0 ? (n + S - 1) / S : 0;
+  const int nql = seeds.record ? nc * nc : nq;
   // min sin(theta) over each band and the cloud's band table (nb <=
   // kMaxSectors), once per workgroup
   __shared__ float band_sin_min[kMaxSectors];
@@ -308,7 +312,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x)
     bs[bb] = band_start[(int64_t)c * (nb + 1) + bb];
   __syncthreads();
-  if (q >= nq) return;  // whole groups leave together
+  if (ql >= nql) return;  // whole groups leave together
+  const int q = seeds.record ? (ql / nc) * S * n + (ql % nc) * S : ql;
   const int qrow = q / n, qcol = q - qrow * n;
   const float4 qv = grid[q];
   const float thq = theta[qrow];
@@ -333,6 +338,32 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // recomputed only when the best changes
   constexpr float kBoundErr = 3e-5f;
   float thr = INFINITY;
+  // a seeded pass starts from the nearest of the points the four coarse
+  // queries around this one found: real points of the cloud, so their
+  // distance bounds the exact nearest one from above and the walk below
+  // (unchanged, with the same tie rule) prunes from its first band
+  if (S > 0 && !seeds.record) {
+    const int r0 = qrow / S, c0 = qcol / S;
+    const int r1 = r0 + 1 < nc ? r0 + 1 : r0, c1 = c0 + 1 < nc ? c0 + 1 : 0;
+    const int* sd = seeds.pos + (int64_t)c * nc * nc;
+    const int cand[4] = {sd[r0 * nc + c0], sd[r0 * nc + c1], sd[r1 * nc + c0], sd[r1 * nc + c1]};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (cand[k] < 0) continue;
+      const float4 pv = sp[cand[k]];
+      const float d = l2_simple(qv, pv);
+      const int oi = __float_as_int(pv.w);
+      if (d < best || (d == best && oi < bidx)) {
+        best = d;
+        bidx = oi;
+        bpos = cand[k];
+      }
+    }
+    if (bpos >= 0) {
+      const float t = sqrtf(best) + slack;
+      thr = t * t + kBoundErr;
+    }
+  }
   // x in [0, 2 ns) reduced mod ns without an integer division
   auto wrap = [&](int x) { return x >= ns ? x - ns : x; };
   auto scan_range = [&](int s0, int e) {
@@ -441,6 +472,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       if (scan_r) scan_range(cr.y, cr.z);
       if (scan_l) scan_range(cl.y, cl.z);
     }
+  }
+  if (seeds.record) {
+    if (sub == 0) seeds.pos[(int64_t)c * nc * nc + ql] = bpos;
+    return;
   }
   if (sub == 0) {
     float r = 0.f, it = 0.f;
@@ -597,30 +632,52 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
 template <int L>
 static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                         float max_sq, const CellBuffers& cb, float* grids, const float2* extra,
-                        float* xgrids, bool xcd, bool blend, hipStream_t s) {
-  const int nq = n * n, qpb = 256 / L;
+                        float* xgrids, bool xcd, bool blend, NnSeeds seeds, hipStream_t s) {
+  const int S = seeds.stride, nc = S > 0 ? (n + S - 1) / S : 0;
+  const int nq = seeds.record ? nc * nc : n * n, qpb = 256 / L;
   const int nqb = (nq + qpb - 1) / qpb;
   // the XCD-ordered grid needs whole groups of 8 clouds
   const bool by_xcd = xcd && nclouds % 8 == 0;
   const dim3 gq = by_xcd ? dim3((unsigned)(nqb * nclouds)) : dim3((unsigned)nqb, nclouds);
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend);
+                     cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
+                     seeds);
 }
 
 int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
 
+static void launch_nn_lanes(int lanes, int64_t cap, int nclouds, int n, const float4* grid,
+                            const float* theta, float max_sq, const CellBuffers& cb, float* grids,
+                            const float2* extra, float* xgrids, bool xcd, bool blend,
+                            NnSeeds seeds, hipStream_t s) {
+  switch (lanes) {
+    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, seeds, s); break;
+    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, seeds, s); break;
+    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, seeds, s); break;
+    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, seeds, s); break;
+    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, seeds, s); break;
+  }
+}
+
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
-                  const float2* extra, float* xgrids, bool xcd, bool blend) {
+                  const float2* extra, float* xgrids, bool xcd, bool blend, int seed_stride) {
   if (lanes == 0) lanes = nn_auto_lanes(nclouds);
-  switch (lanes) {
-    case 1: launch_nn_t<1>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
-    case 2: launch_nn_t<2>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
-    case 8: launch_nn_t<8>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
-    case 16: launch_nn_t<16>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
-    default: launch_nn_t<4>(cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd, blend, s); break;
+  NnSeeds none{};
+  if (seed_stride > 1 && cb.seed_pos) {
+    // coarse queries first (a sixteenth of them at stride 4, more lanes
+    // each: they run unseeded), then every query seeded by them
+    NnSeeds rec{cb.seed_pos, seed_stride, 1};
+    launch_nn_lanes(lanes * 2 <= 16 ? lanes * 2 : 16, cap, nclouds, n, grid, theta, max_sq, cb,
+                    grids, extra, xgrids, xcd, blend, rec, s);
+    NnSeeds use{cb.seed_pos, seed_stride, 0};
+    launch_nn_lanes(lanes, cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd,
+                    blend, use, s);
+    return;
   }
+  launch_nn_lanes(lanes, cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd,
+                  blend, none, s);
 }
 
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,

@@ -364,7 +364,8 @@ __global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2*
 __global__ void __launch_bounds__(64) k_rot_pose(int bw, const unsigned long long* __restrict__ keys,
                                                  const float* __restrict__ corr,
                                                  int64_t corr_stride, int k,
-                                                 DevPose* __restrict__ pose, int skip_axes) {
+                                                 DevPose* __restrict__ pose, int skip_axes,
+                                                 int mixture) {
   const int p = blockIdx.x, lane = threadIdx.x;
   const unsigned long long key = keys[p];
   const uint32_t idx = key_index(key);
@@ -381,6 +382,11 @@ __global__ void __launch_bounds__(64) k_rot_pose(int bw, const unsigned long lon
   double C[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // upper triangle, row by row
   if (end >= n_corr) {
     status |= kPoseRotWindow;
+  } else if (mixture && end == start && idx < n_corr) {
+    // BmmPeakBasedEval normalises one sample too (bmm-peak-based-eval.cc:
+    // 83-85): its weight must be positive; the fit of one sample is its
+    // quaternion either way
+    if (!(unordered_value((uint32_t)(key >> 32)) > 0.f)) status |= kPoseRotWeights;
   } else if (end > start && idx < n_corr) {
     const float* g = corr + p * corr_stride;
     double sum = 0.0;
@@ -535,9 +541,10 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
 }
 
 void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const float* corr,
-                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s, int skip_axes) {
+                     int64_t corr_stride, int k, DevPose* pose, hipStream_t s, int skip_axes,
+                     bool mixture) {
   hipLaunchKernelGGL(k_rot_pose, dim3(npairs), dim3(64), 0, s, bw, keys, corr, corr_stride, k,
-                     pose, skip_axes);
+                     pose, skip_axes, (int)mixture);
 }
 
 }  // namespace phz

@@ -508,7 +508,8 @@ __global__ void __launch_bounds__(64) k_trans_pose(int N, int lower, int upper,
                                                    const unsigned long long* __restrict__ keys,
                                                    const float* __restrict__ corr,
                                                    int64_t corr_stride, int k,
-                                                   DevPose* __restrict__ pose, uint32_t n_corr) {
+                                                   DevPose* __restrict__ pose, uint32_t n_corr,
+                                                   int mixture) {
   const int p = blockIdx.x, lane = threadIdx.x;
   const unsigned long long key = keys[p];
   const uint32_t idx = key_index(key);
@@ -517,11 +518,17 @@ __global__ void __launch_bounds__(64) k_trans_pose(int N, int lower, int upper,
   // gaussian-peak-based-eval.cc:26-37
   const uint32_t uk = (uint32_t)k;
   const uint32_t start = uk > idx ? idx : idx - uk;
-  const uint32_t end = uk + idx >= n_corr ? idx : idx + uk;
+  // GmmPeakBasedEval (gmm-peak-based-eval.cc:58-68): end = idx only when
+  // idx + k > n_corr, so an end of n_corr fails its CHECK_LT (:81)
+  const uint32_t end = mixture ? (uk + idx > n_corr ? idx : idx + uk)
+                               : (uk + idx >= n_corr ? idx : idx + uk);
   int32_t status = 0;
   double mean[3], cov[6] = {0, 0, 0, 0, 0, 0};  // upper triangle, row by row
   bool negative = false;
-  if (end == start || idx >= n_corr) {  // one sample (the default 0 neighbours): value / value
+  if (end >= n_corr) {
+    status |= kPoseTransWindow;
+    translation_of_index(idx, N, width, mean);
+  } else if (end == start || idx >= n_corr) {  // one sample (the default 0 neighbours): value / value
     if (!(value > 0.f)) status |= kPoseTransWeights;
     translation_of_index(idx, N, width, mean);
   } else {
@@ -1651,10 +1658,10 @@ void launch_spatial_inverse(int N, int npairs, ChannelSet F, int64_t f_stride, C
 
 void launch_trans_pose(int N, int lower, int upper, int npairs, const unsigned long long* keys,
                        const float* corr, int64_t corr_stride, int k, DevPose* pose,
-                       hipStream_t s, uint32_t n_corr) {
+                       hipStream_t s, uint32_t n_corr, bool mixture) {
   if (n_corr == 0) n_corr = (uint32_t)N * N * N;
   hipLaunchKernelGGL(k_trans_pose, dim3(npairs), dim3(64), 0, s, N, lower, upper, keys, corr,
-                     corr_stride, k, pose, n_corr);
+                     corr_stride, k, pose, n_corr, (int)mixture);
 }
 
 void launch_spatial_cross_product(int N, int npairs, ChannelSet F, int64_t f_stride,

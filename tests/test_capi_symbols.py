@@ -48,7 +48,7 @@ def test_synth_library_exports():
 
 def test_struct_layouts():
     from phaser_amd import capi
-    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 8
+    assert C.sizeof(capi.Config) == 4 * 8 + 8 + 4 * 12
     # rot/trans index + values (16) + zyz,quat,euler (80) + T (64) + translation (24)
     # + trans_cov (72) + rot_omega (32) + rot_axes (128) + status, reserved (8)
     assert C.sizeof(capi.Result) == 16 + 80 + 64 + 24 + 72 + 32 + 128 + 8
@@ -102,6 +102,7 @@ int main(void) {
   S(phz_ply_fields); S(phz_ply_cloud); O(phz_ply_cloud, range); O(phz_ply_cloud, has_range);
   O(phz_tuning, plane_fwd); O(phz_tuning, nn_lanes); O(phz_tuning, fused_reg);
   O(phz_tuning, so3_reg); O(phz_tuning, occ_fwd); O(phz_tuning, inv_plane); O(phz_tuning, nn_xcd);
+  O(phz_tuning, nn_seed);
   O(phz_tuning, ms_fwd_x); O(phz_tuning, ms_plane);
   O(phz_tuning, ms_fused); O(phz_tuning, ms_so3);
   S(phs_config); S(phs_request); S(phs_response); S(phs_stats);

@@ -57,7 +57,8 @@ def test_shared_target_batch_of_256(po, B, N):
     t = dict(tuning)
     with Context(bandwidth=B, n_voxels=N, max_points=cap, max_batch=1) as one:
         one.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
-                       t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"])
+                       t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"],
+                       t["nn_seed"])
         for i, s in enumerate(sources):
             a = one.register(target, s)
             r = res[i]

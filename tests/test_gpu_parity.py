@@ -509,6 +509,29 @@ def test_nn_lanes_give_identical_samples(lanes):
         assert r.shape[0] == 2
 
 
+@pytest.mark.parametrize("B,N,rings,nb", [(32, 64, 64, 8), (64, 128, 128, 8), (75, 100, 128, 2),
+                                          (150, 64, 128, 2)])
+def test_nn_seeding_gives_identical_results(B, N, rings, nb):
+    """The seeded two-pass NN search (every 4th query row and column first)
+    changes where the exact walk starts only: both correlation grids of
+    every pair are bit-identical with seeding on and off, also for 2B not a
+    multiple of 4 (150) and the largest search grid (300 bands)."""
+    from phaser_amd.capi import Context
+    pairs = [_pair(s, dict(rings=rings, azimuths=1024, overlap=0.7)) for s in range(60, 60 + nb)]
+    cap = max(max(len(s), len(t)) for s, t, _, _ in pairs)
+    got = {}
+    with Context(bandwidth=B, n_voxels=N, max_points=cap) as ctx:
+        for mode in (-1, 1):
+            ctx.set_tuning(nn_seed=mode)
+            assert ctx.tuning()["nn_seed"] == mode
+            print(f"nn_seed {mode}: B={B}, {nb} pairs", flush=True)
+            got[mode] = [ctx.register(t, s, grids=True) for s, t, _, _ in pairs]
+    for a, b in zip(got[-1], got[1]):
+        assert a["rot_index"] == b["rot_index"] and a["trans_index"] == b["trans_index"]
+        assert np.array_equal(a["rot_corr"], b["rot_corr"])
+        assert np.array_equal(a["trans_corr"], b["trans_corr"])
+
+
 def test_nn_xcd_order_gives_identical_results():
     """The XCD-ordered NN grid (16 clouds per launch, a multiple of 8)
     reorders workgroups only: every pair's rotation peak is bit-identical."""

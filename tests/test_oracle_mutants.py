@@ -14,7 +14,10 @@ Mistakes:
 * with spatial zero padding: the IFFTShift of the padded buffer dropped, the
   padded index map with x and y swapped (spatial-correlation.cc:223-230),
   and the padded size taken as (N + 2 pad)^3 instead of the reference's
-  truncated double product (spatial-correlation.cc:17-24).
+  truncated double product (spatial-correlation.cc:17-24);
+* the z-score detector's window moved to include the current index, and
+  its filtered inputs processed instead of skipped (z-score-peak-
+  extraction.cc:77-79,107-110).
 """
 import os
 import subprocess
@@ -25,6 +28,7 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "oracle", "phaser_oracle.c")
 PINS = os.path.join(ROOT, "tests", "test_oracle_compositions.py")
+PEAK_PINS = os.path.join(ROOT, "tests", "test_peak_eval_oracle.py")
 
 
 def _weight(s):
@@ -54,21 +58,33 @@ def _pad_map(s):
              "      p = sub2ind(ijk[1] + pad, ijk[0] + pad, ijk[2] + pad, Np, Np);")]
 
 
+def _zscore_window(s):
+    return [("    const uint32_t start = (uint32_t)(i - lag);",
+             "    const uint32_t start = (uint32_t)(i - lag) + 1u;")]
+
+
+def _zscore_filter(s):
+    return [("    if (x <= filter_threshold) continue;",
+             "    if (x <= filter_threshold) { in[i] = x; }")]
+
+
 def _padded_size(s):
     return [("  return (uint32_t)(T * factor);",
              "  (void)factor;\n  return (uint32_t)(N + padding_per_dim) * (N + padding_per_dim) *"
              " (N + padding_per_dim);")]
 
 
-@pytest.mark.parametrize("mutation,caught_by", [
-    (_weight, "test_so3_correlation"),
-    (_spherical_shift, "test_spherical_fusion_is_shift_fuse_unshift"),
-    (_spatial_shift, "test_spatial_fusion_is_shift_fuse_unshift"),
-    (_padded_shift, "test_padded_inverse_of_one_product"),
-    (_pad_map, "test_padded_inverse_of_one_product"),
-    (_padded_size, "test_padded_size_truncates_like_the_reference"),
+@pytest.mark.parametrize("mutation,caught_by,pins", [
+    (_weight, "test_so3_correlation", PINS),
+    (_spherical_shift, "test_spherical_fusion_is_shift_fuse_unshift", PINS),
+    (_spatial_shift, "test_spatial_fusion_is_shift_fuse_unshift", PINS),
+    (_padded_shift, "test_padded_inverse_of_one_product", PINS),
+    (_pad_map, "test_padded_inverse_of_one_product", PINS),
+    (_padded_size, "test_padded_size_truncates_like_the_reference", PINS),
+    (_zscore_window, "test_zscore_window_excludes_the_current_index", PEAK_PINS),
+    (_zscore_filter, "test_zscore_filter_skips_without_updating", PEAK_PINS),
 ])
-def test_pins_fail_a_planted_mistake(tmp_path, mutation, caught_by):
+def test_pins_fail_a_planted_mistake(tmp_path, mutation, caught_by, pins):
     from oracle import build
     src = open(SRC).read()
     for old, new in mutation(src):
@@ -79,7 +95,7 @@ def test_pins_fail_a_planted_mistake(tmp_path, mutation, caught_by):
     lib = build.build(src=str(mutant_c), out=str(tmp_path / "libphaser_oracle_mutant.so"))
     env = dict(os.environ, PHZ_ORACLE_LIB=lib)
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-p", "no:cacheprovider",
-                        "-p", "no:xdist", PINS], cwd=ROOT, env=env, capture_output=True,
+                        "-p", "no:xdist", pins], cwd=ROOT, env=env, capture_output=True,
                        text=True, timeout=600)
     failed = [l for l in r.stdout.splitlines() if l.startswith("FAILED")]
     assert r.returncode == 1, r.stdout[-2000:] + r.stderr[-2000:]
