@@ -73,14 +73,19 @@ def sweep(fx, bandwidth=64, n_voxels=128, batch=64, threads=16, precision=0, lim
                 for which, gap, idx in (("rot", gr, r["rot_index"]), ("trans", gt, r["trans_index"])):
                     if gap is None:
                         continue
-                    tie = gap <= NEAR_TIE
+                    # a translation after another rotation bin is the
+                    # correlation of a differently rotated source: it
+                    # follows the rotation's mismatch and is not judged
+                    follows = which == "trans" and gr is not None
+                    tie = (not follows) and gap <= NEAR_TIE
                     out["near_ties"] += tie
                     out["mismatches"].append({"seed": int(seeds[i]), "grid": which,
                                               "gpu_index": int(idx),
                                               "oracle_index": int(fx[f"{which}_index"][i]),
                                               # None: not among the oracle's top entries
                                               "oracle_rel_gap": gap if np.isfinite(gap) else None,
-                                              "near_tie": bool(tie)})
+                                              "near_tie": bool(tie),
+                                              "follows_rotation": bool(follows)})
     out["rate"] = out["matched"] / max(out["pairs"], 1)
     return out
 

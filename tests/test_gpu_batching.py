@@ -5,10 +5,14 @@ shared), per bandwidth / voxel-count shape B in {32, 48, 64, 96} with
 (overlap 40-90%, noise 1-5 cm, 5% outliers), on an autotuned context
 replaying the pipeline from a HIP graph.
 
-Bars: every pair's argmaxes, values and pose equal the same pair registered
-alone on a one-pair context (bit for bit: the batched kernels keep each
-pair's arithmetic), and one pair per shape matches the FP64 oracle (grids
-within 2e-3 of its largest magnitude, argmax bit-exact or a near-tie).
+Bars: every pair's rotation bin, value and pose equal the same pair
+registered alone on a one-pair context (bit for bit: the batched kernels
+keep each pair's arithmetic); its translation voxel equal too and the
+translation value within 1e-5 relative (the voxel sums are FP32 atomics
+whose order varies from run to run, so the translation grid can change in
+its last bits), unless the alone run's top two translation values are a
+near-tie; and one pair per shape matches the FP64 oracle (grids within 2e-3
+of its largest magnitude, argmax bit-exact or a near-tie).
 """
 import numpy as np
 import pytest
@@ -59,14 +63,23 @@ def test_shared_target_batch_of_256(po, B, N):
         one.set_tuning(t["fwd_x"], t["fwd_z"], t["inv_x"], t["plane_fwd"], t["nn_lanes"],
                        t["fused_reg"], t["so3_reg"], t["occ_fwd"], t["inv_plane"], t["nn_xcd"],
                        t["nn_seed"])
+        ties = 0
         for i, s in enumerate(sources):
             a = one.register(target, s)
             r = res[i]
-            for key in ("rot_index", "trans_index", "rot_value", "trans_value"):
+            for key in ("rot_index", "rot_value"):
                 assert r[key] == a[key], (i, key, r[key], a[key])
-            for key in ("zyz", "quat", "translation"):
+            for key in ("zyz", "quat"):
                 np.testing.assert_array_equal(r[key], a[key])
             np.testing.assert_array_equal(r["T_rot"], a["T_rot"])
+            assert abs(r["trans_value"] - a["trans_value"]) <= 1e-5 * abs(a["trans_value"])
+            if r["trans_index"] != a["trans_index"]:
+                g = one.register(target, s, grids=True)["trans_corr"].astype(np.float64)
+                assert g[r["trans_index"]] >= g.max() * (1 - 1e-4), (i, "translation bin")
+                ties += 1
+                continue
+            np.testing.assert_array_equal(r["translation"], a["translation"])
+        print(f"  translation near-ties: {ties}", flush=True)
         # one pair of the shape against the oracle
         g = one.register(target, sources[0], grids=True)
     o = po.register(target, sources[0], bw=B, n_voxels=N, want_grids=True)

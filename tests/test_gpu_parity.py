@@ -513,9 +513,11 @@ def test_nn_lanes_give_identical_samples(lanes):
                                           (150, 64, 128, 2)])
 def test_nn_seeding_gives_identical_results(B, N, rings, nb):
     """The seeded two-pass NN search (every 4th query row and column first)
-    changes where the exact walk starts only: both correlation grids of
-    every pair are bit-identical with seeding on and off, also for 2B not a
-    multiple of 4 (150) and the largest search grid (300 bands)."""
+    changes where the exact walk starts only: the rotation grid of every
+    pair is bit-identical with seeding on and off, also for 2B not a
+    multiple of 4 (150) and the largest search grid (300 bands); the
+    translation grid within 1e-5 of its maximum (its voxel sums are FP32
+    atomics whose order varies from run to run)."""
     from phaser_amd.capi import Context
     pairs = [_pair(s, dict(rings=rings, azimuths=1024, overlap=0.7)) for s in range(60, 60 + nb)]
     cap = max(max(len(s), len(t)) for s, t, _, _ in pairs)
@@ -527,9 +529,10 @@ def test_nn_seeding_gives_identical_results(B, N, rings, nb):
             print(f"nn_seed {mode}: B={B}, {nb} pairs", flush=True)
             got[mode] = [ctx.register(t, s, grids=True) for s, t, _, _ in pairs]
     for a, b in zip(got[-1], got[1]):
-        assert a["rot_index"] == b["rot_index"] and a["trans_index"] == b["trans_index"]
+        assert a["rot_index"] == b["rot_index"]
         assert np.array_equal(a["rot_corr"], b["rot_corr"])
-        assert np.array_equal(a["trans_corr"], b["trans_corr"])
+        assert _rel(a["trans_corr"], b["trans_corr"]) < 1e-5
+        assert _argmax_ok(b["trans_index"], a["trans_corr"])
 
 
 def test_nn_xcd_order_gives_identical_results():

@@ -141,13 +141,23 @@ def test_bingham_window_past_the_grid_fails():
         r = ctx.register(tgt, src, grids=True, registered=True)
         assert r["rot_index"] == idx
         _check_fit(r, r["rot_corr"], r["trans_corr"], B, N, k - 1, 0)
-    assert failed["rot_index"] == idx and failed["trans_index"] == r["trans_index"]
-    np.testing.assert_array_equal(failed["rot_corr"], r["rot_corr"])
-    np.testing.assert_array_equal(failed["trans_corr"], r["trans_corr"])
-    # the failed fit leaves the argmax pose
+    # the failed fit leaves the argmax pose: the outputs are those of a
+    # registration without peak neighbours (its translation grid up to the
+    # last bits of the FP32 atomic voxel sums, whose order varies)
+    with Context(bandwidth=B, n_voxels=N, max_points=200000) as ctx:
+        a = ctx.register(tgt, src, grids=True, registered=True)
+    assert failed["rot_index"] == idx == a["rot_index"]
+    np.testing.assert_array_equal(failed["rot_corr"], a["rot_corr"])
+    np.testing.assert_array_equal(failed["quat"], a["quat"])
     np.testing.assert_allclose(failed["quat"], oracle.quat_from_zyz(oracle.zyz_from_index(idx, B)),
                                atol=1e-15)
+    assert np.all(np.isfinite(failed["trans_corr"]))
+    scale = np.max(np.abs(a["trans_corr"]))
+    assert np.max(np.abs(failed["trans_corr"] - a["trans_corr"])) <= 1e-5 * scale
+    assert failed["trans_index"] == a["trans_index"]
+    np.testing.assert_array_equal(failed["translation"], a["translation"])
     assert np.all(np.isfinite(failed["registered"]))
+    np.testing.assert_allclose(failed["registered"], a["registered"], atol=1e-5)
     np.testing.assert_array_equal(failed["registered"][:, 3], src[:, 3])
 
 
