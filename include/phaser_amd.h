@@ -258,6 +258,28 @@ int phz_spatial_correlate_single(phz_ctx* ctx, int n_voxels, const float* f, con
 int phz_spatial_correlate_single_padded(phz_ctx* ctx, int n_voxels, int zero_padding,
                                         const float* f, const float* g, float* corr,
                                         uint32_t* argmax);
+/* SpatialCorrelationLowPass(n_voxels, zero_padding, lower, upper)::
+ * correlateSignals (spatial-correlation-low-pass.cc:21-94): the r2c spectra
+ * of one channel (the reference takes f[1], the range channel of
+ * PhaseAligner's order), FFTShifted over N^3, multiplied only at the shifted
+ * indices Sub2Ind(i, j, k, N, N), i, j, k in [lower, min(upper, N)]
+ * (padded as phz_spatial_correlate_single_padded), IFFTShift, c2r.  The
+ * reference leaves the rest of its product buffer and the spectra's tails as
+ * the previous call left them; this is its first call on a fresh object
+ * (zeros).  A cube reaching coordinate N, or an empty one, is rejected (-1)
+ * where the reference's CHECKs abort.  corr: (N + 2 pad)^3 floats or NULL. */
+int phz_spatial_correlate_low_pass(phz_ctx* ctx, int n_voxels, int zero_padding, int lower,
+                                   int upper, const float* f, const float* g, float* corr,
+                                   uint32_t* argmax);
+/* SphericalCorrelationLowPass::correlateSampledSignals (spherical-
+ * correlation-low-pass.cc:14-59): the S^2 transforms of one sampled grid
+ * per signal (f1[0], f2[0]: (2B)^2 floats each), every coefficient zeroed
+ * outside [min(B^2, lower), min(B^2, upper)) of the FFTShifted seanindex
+ * order (phaser_core_spherical_low_pass_lower/upper_bound), then the SO(3)
+ * correlation |C|^2 (corr: (2B)^3 floats or NULL).  A context without
+ * spherical zero padding. */
+int phz_spherical_correlate_low_pass(phz_ctx* ctx, const float* f_grid, const float* h_grid,
+                                     int lower, int upper, float* corr, uint32_t* argmax);
 /* ZScorePeakExtraction (z-score-peak-extraction.cc:12-112): the smoothed
  * z-score detector over a correlation vector (the peak extraction that
  * ZScoreEval can run, z-score-eval.cc:70, before the Bingham / Gaussian /

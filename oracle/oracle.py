@@ -334,6 +334,50 @@ def spatial_correlate_laplace(f, g, N, pad=0):
     return out
 
 
+def spherical_correlate_low_pass(bw, f_grid, h_grid, lower, upper):
+    """SphericalCorrelationLowPass::correlateSampledSignals (spherical-
+    correlation-low-pass.cc:14-59): the S^2 transforms of one grid per
+    signal, FFTShifted over B^2 (both parts alike), zero outside
+    [min(B^2, lower), min(B^2, upper)), IFFTShifted back, then the SO(3)
+    correlation |C|^2 (the combine and inverse of po_so3_correlate)."""
+    if lower < 0 or upper <= 0:
+        raise PeakWindowError("low-pass bounds fail the reference's CHECKs (:26-27)")
+    n = 2 * bw
+    full = bw * bw
+    lo, hi = min(full, lower), min(full, upper)
+    coefs = []
+    for grid in (f_grid, h_grid):
+        c = fftshift(fst(bw, np.asarray(grid, np.float64).reshape(n, n)))
+        c[:lo] = 0
+        c[hi:] = 0
+        coefs.append(ifftshift(c))
+    return so3_correlate(bw, coefs[0], coefs[1])
+
+
+def spatial_correlate_low_pass(f, g, N, lower, upper, pad=0):
+    """SpatialCorrelationLowPass::correlateSignals (spatial-correlation-low-
+    pass.cc:21-94) on its first call (a fresh object: zero product buffer and
+    zero spectrum tails): the r2c spectra of the one channel, FFTShifted over
+    N^3; products only at the shifted indices Sub2Ind(i, j, k, N, N) = i N +
+    j + N^2 k, i, j, k in [lower, min(upper, N)]; the padded inverse with
+    IFFTShift (po_spatial_pad_inverse).  An index >= N^3 fails the CHECK_LT of
+    complexMulSeqUsingIndices (spatial-correlation.cc:153), an empty cube its
+    CHECK (:145)."""
+    hi = min(upper, N)
+    if lower > hi:
+        raise PeakWindowError("empty low-pass cube (spatial-correlation.cc:145)")
+    if hi >= N:
+        raise PeakWindowError("low-pass cube reaches N^3 (spatial-correlation.cc:153)")
+    F = fftshift(r2c3(np.asarray(f, np.float64).reshape(-1)))
+    G = fftshift(r2c3(np.asarray(g, np.float64).reshape(-1)))
+    r = np.arange(lower, hi + 1)
+    i, j, k = np.meshgrid(r, r, r, indexing="ij")
+    idx = (i * N + j + N * N * k).reshape(-1)
+    P = np.zeros(N ** 3, np.complex128)
+    P[idx] = F[idx] * np.conj(G[idx])
+    return spatial_pad_inverse(P, N, pad, True)
+
+
 def padded_size(N, pad):
     return int(lib().po_padded_size(N, pad))
 

@@ -89,6 +89,11 @@ EXPORTS = {
     "phz_spatial_correlate_single_padded": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                                       C.c_void_p, C.c_void_p,
                                                       C.POINTER(C.c_uint32)]),
+    "phz_spatial_correlate_low_pass": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.POINTER(C.c_uint32)]),
+    "phz_spherical_correlate_low_pass": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                                   C.c_int, C.c_void_p, C.POINTER(C.c_uint32)]),
     "phz_zscore_default_params": (None, [C.c_void_p]),
     "phz_zscore_peaks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.POINTER(C.c_int64)]),
@@ -414,6 +419,24 @@ class Context:
         corr = np.zeros(N ** 3, np.float32)
         am = C.c_uint32()
         _check(lib().phz_spatial_correlate_single(self.h, N, _p(f), _p(g), _p(corr), C.byref(am)))
+        return corr, int(am.value)
+
+    def spatial_correlate_low_pass(self, f, g, N, lower, upper, pad=0):
+        f = _f32(f).reshape(-1)
+        g = _f32(g).reshape(-1)
+        corr = np.zeros((N + 2 * pad) ** 3, np.float32)
+        am = C.c_uint32()
+        _check(lib().phz_spatial_correlate_low_pass(self.h, N, pad, lower, upper, _p(f), _p(g),
+                                                    _p(corr), C.byref(am)))
+        return corr, int(am.value)
+
+    def spherical_correlate_low_pass(self, f_grid, h_grid, lower, upper):
+        f = _f32(f_grid).reshape(-1)
+        h = _f32(h_grid).reshape(-1)
+        corr = np.zeros(self.n ** 3, np.float32)
+        am = C.c_uint32()
+        _check(lib().phz_spherical_correlate_low_pass(self.h, _p(f), _p(h), lower, upper,
+                                                      _p(corr), C.byref(am)))
         return corr, int(am.value)
 
     def zscore_peaks(self, corr, max_peaks=1 << 20, **kw):

@@ -1278,10 +1278,15 @@ int phz_spatial_correlate(phz_ctx* c, int C, const float* f, const float* g, flo
 }
 
 // SpatialCorrelation(N, pad)::correlateSignals on one channel pair, with
-// buffers allocated for the call (spatial-correlation.cc:11-61,163-206)
+// buffers allocated for the call (spatial-correlation.cc:11-61,163-206);
+// cube_lo <= cube_hi: SpatialCorrelationLowPass (spatial-correlation-low-
+// pass.cc:39-94), the product of the FFTShifted spectra at the shifted
+// indices of the [cube_lo, cube_hi]^3 cube only, IFFTShift, c2r
 static int spatial_single(phz_ctx* c, int N, int pad, const float* f, const float* g,
-                          float* corr, uint32_t* argmax) {
+                          float* corr, uint32_t* argmax, int cube_lo = 1, int cube_hi = 0) {
   PHZ_CHECK(c && f && g, "null argument");
+  const bool low_pass = cube_lo <= cube_hi;
+  const bool gather = pad > 0 || low_pass;
   PHZ_CHECK(N >= 1 && N <= 256, "n_voxels must be in [1, 256]");
   PHZ_CHECK(pad >= 0 && N + 2 * pad <= 256, "zero padding must be >= 0 with n_voxels + 2 * padding <= 256");
   PHZ_CHECK(N > 1 || pad == 0, "zero padding needs n_voxels >= 2");
@@ -1308,9 +1313,10 @@ static int spatial_single(phz_ctx* c, int N, int pad, const float* f, const floa
   if (!rc && (e = hipMalloc(&d_work, sizeof(float2) * H)) != hipSuccess) fail("hipMalloc", e);
   if (!rc && (e = hipMalloc(&d_tw, sizeof(float2) * N)) != hipSuccess) fail("hipMalloc", e);
   if (!rc && (e = hipMalloc(&d_corr, sizeof(float) * Tp)) != hipSuccess) fail("hipMalloc", e);
-  if (!rc && pad && (e = hipMalloc(&d_work_pad, sizeof(float2) * Hp)) != hipSuccess)
+  if (!rc && gather && (e = hipMalloc(&d_work_pad, sizeof(float2) * Hp)) != hipSuccess)
     fail("hipMalloc", e);
-  if (!rc && pad && (e = hipMalloc(&d_twp, sizeof(float2) * Np)) != hipSuccess) fail("hipMalloc", e);
+  if (!rc && gather && (e = hipMalloc(&d_twp, sizeof(float2) * Np)) != hipSuccess)
+    fail("hipMalloc", e);
   const int nblk = spatial_c2r_blocks(Np);
   if (!rc && (e = hipMalloc(&d_key, sizeof(unsigned long long) * (1 + nblk))) != hipSuccess)
     fail("hipMalloc", e);
@@ -1318,7 +1324,7 @@ static int spatial_single(phz_ctx* c, int N, int pad, const float* f, const floa
     hipMemcpyAsync(d_in, f, sizeof(float) * T, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_in + T, g, sizeof(float) * T, hipMemcpyHostToDevice, s);
     hipMemcpyAsync(d_tw, tw.data(), sizeof(float2) * N, hipMemcpyHostToDevice, s);
-    if (pad) hipMemcpyAsync(d_twp, twp.data(), sizeof(float2) * Np, hipMemcpyHostToDevice, s);
+    if (gather) hipMemcpyAsync(d_twp, twp.data(), sizeof(float2) * Np, hipMemcpyHostToDevice, s);
     if (N == 1) {
       plan.L = 1;
       plan.nst = 0;
@@ -1329,9 +1335,9 @@ static int spatial_single(phz_ctx* c, int N, int pad, const float* f, const floa
     F.ch[0] = d_spec;
     G.ch[0] = d_spec + H;
     FusionRuns none{};
-    if (pad) {  // spatial-correlation.cc:142-161,221-230, no shift in this class
+    if (gather) {  // spatial-correlation.cc:142-161,221-230; shifted in the low-pass class
       launch_spatial_cross_product(N, 1, F, 0, G, 0, none, false, d_work, s);
-      launch_spatial_pad_gather(N, pad, false, 1, d_work, d_work_pad, s);
+      launch_spatial_pad_gather(N, pad, low_pass, 1, d_work, d_work_pad, s, cube_lo, cube_hi);
       const int tile = fft_cols_default_tile(Np);
       launch_spatial_fft_pass(Np, 1, d_work_pad, 3, tile, plan_p, d_twp, s);
       launch_spatial_fft_pass(Np, 1, d_work_pad, 2, tile, plan_p, d_twp, s);
@@ -1421,6 +1427,39 @@ int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float*
 int phz_spatial_correlate_single_padded(phz_ctx* c, int N, int zero_padding, const float* f,
                                         const float* g, float* corr, uint32_t* argmax) {
   return spatial_single(c, N, zero_padding, f, g, corr, argmax);
+}
+
+int phz_spatial_correlate_low_pass(phz_ctx* c, int N, int zero_padding, int lower, int upper,
+                                   const float* f, const float* g, float* corr, uint32_t* argmax) {
+  PHZ_CHECK(N >= 2, "n_voxels must be >= 2");
+  // spatial-correlation-low-pass.cc:11-19: the upper bound is capped at
+  // n_voxels; an index of the cube with a coordinate N reaches past N^3
+  // (Sub2Ind, :86-87) and fails the CHECK_LT of complexMulSeqUsingIndices
+  // (spatial-correlation.cc:153), an empty cube its CHECK(!indices.empty())
+  const int hi = upper < N ? upper : N;
+  PHZ_CHECK(lower >= 0, "the low-pass lower bound must be >= 0");
+  PHZ_CHECK(lower <= hi, "empty low-pass cube (spatial-correlation.cc:145 CHECK)");
+  PHZ_CHECK(hi < N, "low-pass cube reaches index N^3 (spatial-correlation.cc:153 CHECK_LT)");
+  return spatial_single(c, N, zero_padding, f, g, corr, argmax, lower, hi);
+}
+
+int phz_spherical_correlate_low_pass(phz_ctx* c, const float* f_grid, const float* h_grid,
+                                     int lower, int upper, float* corr, uint32_t* argmax) {
+  PHZ_CHECK(c && f_grid && h_grid, "null argument");
+  PHZ_CHECK(c->Bo == c->B, "the SO(3) stage entry points run without spherical zero padding");
+  // spherical-correlation-low-pass.cc:26-27
+  PHZ_CHECK(lower >= 0, "the spherical low-pass lower bound must be >= 0");
+  PHZ_CHECK(upper > 0, "the spherical low-pass upper bound must be > 0");
+  hipStream_t s = c->stream;
+  const int64_t nq = (int64_t)c->n * c->n, BB = (int64_t)c->B * c->B;
+  const uint32_t lo = (uint32_t)std::min<int64_t>(BB, lower);
+  const uint32_t hi = (uint32_t)std::min<int64_t>(BB, upper);
+  PHZ_HIP(hipMemcpyAsync(c->d_grids, f_grid, sizeof(float) * nq, hipMemcpyHostToDevice, s));
+  PHZ_HIP(hipMemcpyAsync(c->d_grids + nq, h_grid, sizeof(float) * nq, hipMemcpyHostToDevice, s));
+  launch_fst(c->d_grids, 2, c->B, c->d_leg, c->d_leg_off, c->d_tw_n, c->d_F, c->d_coef,
+             c->cfg.precision, s);
+  launch_coef_band(c->d_coef, BB, 2, (uint32_t)BB, lo, hi, s);
+  return so3_from_fused(c, c->d_coef, c->d_coef + BB, corr, nullptr, argmax, s);
 }
 
 // ---------------------------------------------------------------- statistics

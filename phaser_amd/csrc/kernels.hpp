@@ -315,9 +315,16 @@ void launch_spatial_cross_product(int N, int npairs, ChannelSet F, int64_t f_str
 // ... then the c2r input of size Np = N + 2 pad: out[pair][Np^2 (Np/2+1)],
 // with the Laplace class's IFFTShift when shifted; follow with
 // launch_spatial_fft_pass which = 3 (inverse z) and 2 (inverse x) and
-// launch_spatial_c2r at Np
+// launch_spatial_c2r at Np.  cube_lo <= cube_hi keeps only the products at
+// shifted indices Sub2Ind(i, j, k) with i, j, k in [cube_lo, cube_hi]
+// (SpatialCorrelationLowPass); cube_lo > cube_hi keeps all.
 void launch_spatial_pad_gather(int N, int pad, bool shifted, int npairs, const float2* work,
-                               float2* out, hipStream_t s);
+                               float2* out, hipStream_t s, int cube_lo = 1, int cube_hi = 0);
+// SphericalCorrelationLowPass (spherical-correlation-low-pass.cc:25-59):
+// nsets coefficient sets of count = B^2 entries (stride apart), each zeroed
+// outside [lower, upper) of its FFTShift order
+void launch_coef_band(float2* coef, int64_t stride, int nsets, uint32_t count, uint32_t lower,
+                      uint32_t upper, hipStream_t s);
 // zero the accumulators of the voxels the points touched (same transform and
 // binning as launch_voxel_scatter), so they need no full clear before reuse
 void launch_voxel_clear(const float4* pts, const int* npts, int64_t cap, int nclouds,

@@ -432,6 +432,23 @@ __global__ void k_pad_coef(const float2* __restrict__ in, int64_t in_stride,
 
 }  // namespace
 
+// coefficient q kept when its FFTShift position (q - s) mod count, s =
+// ceil(count / 2) (signal-utils.h:29-37), lies in [lower, upper)
+__global__ void k_coef_band(float2* __restrict__ coef, int64_t stride, uint32_t count,
+                            uint32_t lower, uint32_t upper) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= count) return;
+  const uint32_t s = count % 2 ? count / 2 + 1 : count / 2;
+  const uint32_t pos = q >= s ? q - s : q + count - s;
+  if (pos < lower || pos >= upper) coef[blockIdx.y * stride + q] = make_float2(0.f, 0.f);
+}
+
+void launch_coef_band(float2* coef, int64_t stride, int nsets, uint32_t count, uint32_t lower,
+                      uint32_t upper, hipStream_t s) {
+  hipLaunchKernelGGL(k_coef_band, dim3((count + 255) / 256, nsets), dim3(256), 0, s, coef, stride,
+                     count, lower, upper);
+}
+
 void launch_pad_coef(const float2* in, int64_t in_stride, const int* map, int64_t count,
                      float2* out, int nsets, hipStream_t s) {
   hipLaunchKernelGGL(k_pad_coef, dim3((unsigned)((count + 255) / 256), nsets), dim3(256), 0, s,

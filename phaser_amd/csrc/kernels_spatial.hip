@@ -471,8 +471,12 @@ __global__ void k_spat_cross_product(int64_t H, ChannelSet F, int64_t f_stride, 
 // the shifted order of the Laplace class, the unshifted half-spectrum entry
 // u = (i + s) mod T (FFTShift rotates left by s = ceil(T/2), signal-utils.h:
 // 29-37) that work holds when u < H.
+// cube_lo <= cube_hi: SpatialCorrelationLowPass (spatial-correlation-low-pass.
+// cc:39-94) multiplies only the product entries whose shifted index is
+// Sub2Ind(i, j, k, N, N) with i, j, k in [cube_lo, cube_hi]; the rest stay zero
 __global__ void k_spat_pad_gather(int N, int pad, int shifted, const float2* __restrict__ work,
-                                  int64_t H, float2* __restrict__ out, int64_t Hp) {
+                                  int64_t H, float2* __restrict__ out, int64_t Hp, int cube_lo,
+                                  int cube_hi) {
   const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (q >= Hp) return;
   const int64_t T = (int64_t)N * N * N;
@@ -485,7 +489,11 @@ __global__ void k_spat_pad_gather(int N, int pad, int shifted, const float2* __r
   const int64_t zp = r / (Np * Np), rem = r - zp * Np * Np;
   const int64_t xp = rem / Np, yp = rem - xp * Np;
   float2 v = make_float2(0.f, 0.f);
-  if (zp >= pad && zp < pad + N && xp >= pad && xp < pad + N && yp >= pad && yp < pad + N) {
+  bool in = zp >= pad && zp < pad + N && xp >= pad && xp < pad + N && yp >= pad && yp < pad + N;
+  if (in && cube_lo <= cube_hi)
+    in = xp - pad >= cube_lo && xp - pad <= cube_hi && yp - pad >= cube_lo &&
+         yp - pad <= cube_hi && zp - pad >= cube_lo && zp - pad <= cube_hi;
+  if (in) {
     const int64_t i = (xp - pad) * N + (yp - pad) + (int64_t)N * N * (zp - pad);
     int64_t u = i;
     if (shifted) {
@@ -1673,11 +1681,11 @@ void launch_spatial_cross_product(int N, int npairs, ChannelSet F, int64_t f_str
 }
 
 void launch_spatial_pad_gather(int N, int pad, bool shifted, int npairs, const float2* work,
-                               float2* out, hipStream_t s) {
+                               float2* out, hipStream_t s, int cube_lo, int cube_hi) {
   const int64_t H = (int64_t)N * N * (N / 2 + 1);
   const int64_t Np = N + 2 * pad, Hp = Np * Np * (Np / 2 + 1);
   hipLaunchKernelGGL(k_spat_pad_gather, dim3((unsigned)((Hp + 255) / 256), npairs), dim3(256),
-                     0, s, N, pad, (int)shifted, work, H, out, Hp);
+                     0, s, N, pad, (int)shifted, work, H, out, Hp, cube_lo, cube_hi);
 }
 
 void launch_apply_pose(const float4* pts, int64_t P, const DevPose* pose, float4* out,

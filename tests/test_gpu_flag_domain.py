@@ -171,3 +171,41 @@ def test_reference_driver_run_flags_match_fixture():
             np.testing.assert_allclose(r["T_rot"].reshape(4, 4), d[k + "T_rot"], atol=2e-6)
             assert r["trans_index"] == int(d[k + "trans_index"])
             np.testing.assert_allclose(r["translation"], d[k + "translation"], atol=1e-9)
+
+
+@pytest.mark.parametrize("band", [(0, 10000), (128, 128 + 16), (40, 200), (0, 90)])
+def test_spherical_low_pass_matches_oracle(po, band):
+    """SphericalCorrelationLowPass (spherical-correlation-low-pass.cc:14-59):
+    coefficient bands of the FFTShifted order, the defaults 0 / 10000 (all)
+    included."""
+    from phaser_amd.capi import Context
+    rng = np.random.default_rng(band[0] + band[1])
+    B = 16
+    n = 2 * B
+    f, h = rng.uniform(0, 1, (n, n)), rng.uniform(0, 1, (n, n))
+    with Context(bandwidth=B, n_voxels=16, max_points=16) as c:
+        corr, am = c.spherical_correlate_low_pass(f, h, *band)
+    o = po.spherical_correlate_low_pass(B, f, h, *band)
+    assert _rel(corr, o) < 2e-4
+    assert _argmax_ok(am, o)
+
+
+@pytest.mark.parametrize("N,lo,hi,pad", [(12, 0, 11, 0), (12, 3, 9, 0), (12, 2, 7, 2),
+                                         (64, 20, 45, 0), (67, 30, 40, 3)])
+def test_spatial_low_pass_matches_oracle(po, N, lo, hi, pad):
+    """SpatialCorrelationLowPass (spatial-correlation-low-pass.cc:21-94): the
+    product only on the shifted indices of the [lo, hi]^3 cube, with and
+    without zero padding; a cube reaching coordinate N is refused as the
+    reference's CHECK_LT aborts."""
+    from phaser_amd.capi import Context, PhaserError
+    rng = np.random.default_rng(N + lo)
+    f = np.where(rng.uniform(size=N ** 3) < 0.3, rng.uniform(0, 1, N ** 3), 0.0)
+    g = np.roll(f.reshape(N, N, N), (1, 2, 3), axis=(0, 1, 2)).reshape(-1)
+    with Context(bandwidth=16, n_voxels=16, max_points=16) as c:
+        corr, am = c.spatial_correlate_low_pass(f, g, N, lo, hi, pad=pad)
+        with pytest.raises(PhaserError):
+            c.spatial_correlate_low_pass(f, g, N, 0, N, pad=pad)
+    o = po.spatial_correlate_low_pass(f, g, N, lo, hi, pad=pad)
+    assert _rel(corr, o) < 1e-5
+    limit = po.padded_size(N, pad) if pad else N ** 3
+    assert _argmax_ok(am, o, limit)

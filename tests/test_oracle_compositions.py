@@ -263,3 +263,70 @@ def test_padded_spatial_fusion_composition(po, N, pad):
                            axes=(0, 1, 2)) * Np ** 3
     got = po.spatial_correlate_laplace(f.reshape(C, -1), g.reshape(C, -1), N, pad=pad)
     np.testing.assert_allclose(got, expect.reshape(-1), atol=1e-10 * np.max(np.abs(expect)))
+
+
+# ---------------------------------------------------------------- low-pass classes
+def test_spherical_low_pass_full_band_is_the_plain_correlation(po):
+    """Bounds covering all B^2 coefficients (the flags' defaults 0 / 10000)
+    leave the correlation of the first grid of each signal unchanged."""
+    rng = np.random.default_rng(5)
+    bw = 6
+    n = 2 * bw
+    f, h = rng.normal(size=(n, n)), rng.normal(size=(n, n))
+    got = po.spherical_correlate_low_pass(bw, f, h, 0, 10000)
+    np.testing.assert_allclose(got, po.so3_correlate(bw, po.fst(bw, f), po.fst(bw, h)),
+                               rtol=1e-12, atol=1e-14)
+
+
+def test_spherical_low_pass_keeps_the_zonal_coefficients(po):
+    """Seanindex j sits at FFTShift position (j - B^2 / 2) mod B^2 (B^2
+    even): the band [B^2 / 2, B^2 / 2 + B) keeps j < B, the order m = 0 of
+    every degree.  The combine then pairs f_{l,0} with h_{l,0} only
+    (so3_correlate_fftw.c:120-157), so |C|^2 is constant along alpha and
+    gamma and, along beta, proportional to (sum_l 2 pi sqrt(2/(2l+1))
+    f_{l,0} h_{l,0} d~^l_{0,0}(beta))^2 with the pinned Wigner-d; an empty
+    band gives zero."""
+    rng = np.random.default_rng(6)
+    bw = 6
+    n, full = 2 * bw, bw * bw
+    f, h = rng.normal(size=(n, n)), rng.normal(size=(n, n))
+    got = po.spherical_correlate_low_pass(bw, f, h, full // 2, full // 2 + bw)
+    got = got.reshape(n, n, n)  # [beta][alpha][gamma]
+    np.testing.assert_allclose(got, got[:, :1, :1] * np.ones((1, n, n)), rtol=1e-9, atol=1e-12)
+    fc, hc = po.fst(bw, f), po.fst(bw, h)
+    d = po.wigner_d(bw, 0, 0)  # [l][beta_k]
+    w = np.array([2 * np.pi * np.sqrt(2.0 / (2 * l + 1)) for l in range(bw)])
+    amp = (w * fc[:bw].real * hc[:bw].real) @ d
+    np.testing.assert_allclose(got[:, 0, 0] / got[:, 0, 0].max(), amp ** 2 / np.max(amp ** 2),
+                               atol=1e-9)
+    assert np.max(po.spherical_correlate_low_pass(bw, f, h, 5, 5)) == 0.0
+
+
+def test_spatial_low_pass_full_cube_is_the_plain_correlation(po):
+    """The whole cube [0, N-1]^3 multiplies every shifted index: shift,
+    product, unshift is the plain product, so the result is
+    SpatialCorrelation's (pinned by the reference's known answers)."""
+    rng = np.random.default_rng(8)
+    N = 6
+    f, g = rng.normal(size=N ** 3), rng.normal(size=N ** 3)
+    np.testing.assert_allclose(po.spatial_correlate_low_pass(f, g, N, 0, N - 1),
+                               po.spatial_correlate(f, g, N), atol=1e-10)
+    with pytest.raises(po.PeakWindowError):
+        po.spatial_correlate_low_pass(f, g, N, 0, N)  # the capped bound N reaches N^3
+
+
+def test_spatial_low_pass_one_index(po):
+    """A cube of one index keeps the product at shifted index i0 = Sub2Ind(
+    b, b, b, N, N) = b N + b + N^2 b, i.e. the spectra's entry u0 =
+    (i0 + ceil(N^3 / 2)) mod N^3 of the r2c buffer: the padded inverse of
+    that one product (the pinned closed form)."""
+    rng = np.random.default_rng(9)
+    N, b, pad = 8, 5, 1
+    f, g = rng.normal(size=N ** 3), rng.normal(size=N ** 3)
+    i0 = b * N + b + N * N * b
+    u0 = (i0 + N ** 3 // 2) % N ** 3
+    F, G = po.r2c3(f), po.r2c3(g)
+    P = np.zeros(N ** 3, np.complex128)
+    P[i0] = F[u0] * np.conj(G[u0])
+    np.testing.assert_allclose(po.spatial_correlate_low_pass(f, g, N, b, b, pad=pad),
+                               po.spatial_pad_inverse(P, N, pad, True), atol=1e-10)
