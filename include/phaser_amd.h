@@ -280,6 +280,25 @@ int phz_spatial_correlate_low_pass(phz_ctx* ctx, int n_voxels, int zero_padding,
  * spherical zero padding. */
 int phz_spherical_correlate_low_pass(phz_ctx* ctx, const float* f_grid, const float* h_grid,
                                      int lower, int upper, float* corr, uint32_t* argmax);
+/* ---- preprocessing (CloudPreProcessor, cloud-pre-processor.cc:30-44) --- */
+/* VoxelGridCmd (voxel-grid-cmd.cc:11-24): PCL VoxelGrid (1.8) with leaf
+ * size `leaf` on all three axes (phaser_voxel_grid_leaf_size, 0.25) and
+ * every field averaged.  pts: P x 4 floats (x, y, z, intensity), replaced in
+ * place by the voxel centroids ordered by voxel index; *P_out = their count.
+ * Non-finite points are dropped; a grid of more than INT32_MAX voxels leaves
+ * the cloud unchanged (as PCL).  Each voxel's points are summed in their
+ * input order (PCL's unstable sort may add them in another order: last-bit
+ * differences of voxels with three or more points). */
+int phz_voxel_grid_filter(phz_ctx* ctx, float* pts, int64_t P, float leaf, int64_t* P_out);
+/* PassThroughGndFilterCmd (pass-through-gnd-filter-cmd.cc:14-26): PCL
+ * PassThrough on field 0 (x), 1 (y), 2 (z) or 3 (intensity): with negative = 0 the points
+ * whose field lies in [lower, upper] are kept, with negative != 0 those
+ * outside it (the command: z, -5, 3, negative, phaser_pass_through_gnd_
+ * filter_limit_min/max); non-finite points are dropped.  pts is compacted
+ * in place in input order; *P_out = the kept count. */
+int phz_pass_through_filter(phz_ctx* ctx, float* pts, int64_t P, int field, float lower,
+                            float upper, int negative, int64_t* P_out);
+
 /* ZScorePeakExtraction (z-score-peak-extraction.cc:12-112): the smoothed
  * z-score detector over a correlation vector (the peak extraction that
  * ZScoreEval can run, z-score-eval.cc:70, before the Bingham / Gaussian /

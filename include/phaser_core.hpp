@@ -12,6 +12,10 @@
 //   PhaseAligner                           phase-aligner.h / base-aligner.h:12-21
 //   common::SphericalSampler               spherical-sampler.h
 //   CloudController                        cloud-controller.h:14-34
+//   preproc::CloudPreProcessor / VoxelGridCmd / PassThroughGndFilterCmd
+//                                          cloud-pre-processor.h,
+//                                          voxel-grid-cmd.h,
+//                                          pass-through-gnd-filter-cmd.h
 //   model::PointCloud / FunctionValue / RegistrationResult (the fields the
 //   correlation path reads and writes)
 // Flags are the gflags of core-gflags.cc:6-39, as plain globals read by the
@@ -377,8 +381,69 @@ class SphRegistration : public SphOptRegistration {
                                                model::PointCloudPtr cloud_cur) override;
 };
 
-// cloud-controller.cc:10-34 (preprocessing is not part of this build);
-// "sph" (the reference's default) or "sph-opt"
+}  // namespace phaser_core
+
+// pre-processor-gflags.cc, voxel-grid-cmd.cc:5-7, pass-through-gnd-filter-cmd.cc:5-10
+extern bool FLAGS_phaser_pre_enable_voxel_grid_downsampling;       // false
+extern bool FLAGS_phaser_pre_enable_pass_through_gnd_filtering;    // false
+extern bool FLAGS_phaser_pre_enable_geometric_cloud_segmentation;  // false
+extern double FLAGS_phaser_voxel_grid_leaf_size;                   // 0.25
+extern double FLAGS_phaser_pass_through_gnd_filter_limit_min;      // -5
+extern double FLAGS_phaser_pass_through_gnd_filter_limit_max;      // 3
+
+namespace preproc {
+
+// cloud-pre-processor.h: which commands process() runs, in this order
+struct CloudPreProcessorSettings {
+  bool enable_voxel_grid_downsampling = false;
+  bool enable_pass_through_gnd_filtering = false;
+  bool enable_geometric_cloud_segmentation = false;
+  static CloudPreProcessorSettings fromGflags();
+};
+
+// base-command.h: one filter applied to the cloud in place
+class BaseCommand {
+ public:
+  virtual ~BaseCommand() = default;
+  virtual void execute(model::PointCloudPtr cloud) = 0;
+};
+
+// voxel-grid-cmd.cc:11-24: phz_voxel_grid_filter with leaf
+// FLAGS_phaser_voxel_grid_leaf_size on all three axes
+class VoxelGridCmd : public BaseCommand {
+ public:
+  void execute(model::PointCloudPtr cloud) override;
+};
+
+// pass-through-gnd-filter-cmd.cc:14-26: phz_pass_through_filter on z with
+// the flags' limits, negative (the points inside [min, max] are removed)
+class PassThroughGndFilterCmd : public BaseCommand {
+ public:
+  void execute(model::PointCloudPtr cloud) override;
+};
+
+// cloud-pre-processor.cc:20-44.  The geometric segmentation command
+// (SegmentCloudCmd, a PCL region-growing segmentation) is not part of this
+// build: enabling it aborts at construction.  A filter that changes the
+// point count drops the cloud's per-point reflectivity and ambient values.
+class CloudPreProcessor {
+ public:
+  CloudPreProcessor();
+  explicit CloudPreProcessor(const CloudPreProcessorSettings& settings);
+  void process(model::PointCloudPtr cloud);
+
+ private:
+  void initializeCommandFromSettings(const CloudPreProcessorSettings& settings);
+  std::vector<std::unique_ptr<BaseCommand>> processors_;
+};
+
+}  // namespace preproc
+
+namespace phaser_core {
+
+// cloud-controller.cc:10-34: both clouds go through the CloudPreProcessor
+// (the pre-processor flags; every command off by default), then the
+// registration.  "sph" (the reference's default) or "sph-opt"
 class CloudController {
  public:
   explicit CloudController(const std::string& method = "sph");
@@ -387,6 +452,7 @@ class CloudController {
 
  private:
   std::unique_ptr<BaseRegistration> registrator_;
+  preproc::CloudPreProcessor preprocessor_;
 };
 
 }  // namespace phaser_core

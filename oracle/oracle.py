@@ -577,6 +577,57 @@ def zscore_peaks(corr, lag_percentile=0.05, lag_max=20, threshold=4.33, influenc
     return out[: min(k, max_peaks)].copy()
 
 
+def voxel_grid_filter(pts, leaf=0.25):
+    """VoxelGridCmd (voxel-grid-cmd.cc:11-24): PCL 1.8 VoxelGrid::applyFilter
+    (voxel_grid.hpp) with every field averaged, in float32 as PCL: the
+    finite points' bounding box, the overflow check (more than INT32_MAX
+    voxels: the cloud unchanged), min_b = floor(min * inv), the voxel index
+    (int)(floor(p * inv) - (float)min_b) . (1, div_x, div_x div_y), the
+    points sorted by index (stably: each voxel's points in input order; PCL's
+    std::sort may take them in another order), one centroid per voxel (float
+    sums in order, divided by the count), ordered by voxel index.
+    pts: (P, 4) x, y, z, intensity."""
+    a = np.asarray(pts, np.float32).reshape(-1, 4)
+    fin = np.all(np.isfinite(a[:, :3]), axis=1)
+    if not fin.any():
+        return np.zeros((0, 4), np.float32)
+    inv = np.float32(1.0) / np.float32(leaf)
+    mn = a[fin, :3].min(axis=0)
+    mx = a[fin, :3].max(axis=0)
+    d = [int(np.float32((mx[k] - mn[k]) * inv)) + 1 for k in range(3)]
+    if d[0] * d[1] * d[2] > 2 ** 31 - 1:
+        return a.copy()
+    min_b = np.floor(mn * inv).astype(np.int64)
+    max_b = np.floor(mx * inv).astype(np.int64)
+    div = max_b - min_b + 1
+    q = a[fin]
+    ijk = (np.floor(q[:, :3] * inv) - min_b.astype(np.float32)).astype(np.int64)
+    idx = ijk[:, 0] + ijk[:, 1] * div[0] + ijk[:, 2] * div[0] * div[1]
+    order = np.argsort(idx, kind="stable")
+    q = q[order]
+    keys, start, count = np.unique(idx[order], return_index=True, return_counts=True)
+    sums = np.zeros((len(keys), 4), np.float32)
+    for k in range(int(count.max())):  # sequential float32 sums per voxel
+        live = count > k
+        sums[live] += q[start[live] + k]
+    return (sums / count.astype(np.float32)[:, None]).astype(np.float32)
+
+
+def pass_through_filter(pts, field=2, lower=-5.0, upper=3.0, negative=True):
+    """PassThroughGndFilterCmd (pass-through-gnd-filter-cmd.cc:14-26): PCL 1.8
+    PassThrough::applyFilterIndices (passthrough.hpp): non-finite points and
+    non-finite field values dropped; kept, in input order, the points whose
+    field (0 x, 1 y, 2 z, 3 intensity) lies in [lower, upper] (float), or
+    outside it with negative (the command: z, -5, 3, negative)."""
+    a = np.asarray(pts, np.float32).reshape(-1, 4)
+    v = a[:, field]
+    lo, hi = np.float32(lower), np.float32(upper)
+    ok = np.all(np.isfinite(a[:, :3]), axis=1) & np.isfinite(v)
+    inside = (v >= lo) & (v <= hi)
+    keep = ok & (~inside if negative else inside)
+    return a[keep].copy()
+
+
 def ind2sub(idx, rows, cols):
     out = np.zeros(3, np.uint32)
     lib().po_ind2sub(idx, rows, cols, out)

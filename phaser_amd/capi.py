@@ -94,6 +94,10 @@ EXPORTS = {
                                                  C.POINTER(C.c_uint32)]),
     "phz_spherical_correlate_low_pass": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                                    C.c_int, C.c_void_p, C.POINTER(C.c_uint32)]),
+    "phz_voxel_grid_filter": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_float,
+                                        C.POINTER(C.c_int64)]),
+    "phz_pass_through_filter": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_float,
+                                          C.c_float, C.c_int, C.POINTER(C.c_int64)]),
     "phz_zscore_default_params": (None, [C.c_void_p]),
     "phz_zscore_peaks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.POINTER(C.c_int64)]),
@@ -438,6 +442,21 @@ class Context:
         _check(lib().phz_spherical_correlate_low_pass(self.h, _p(f), _p(h), lower, upper,
                                                       _p(corr), C.byref(am)))
         return corr, int(am.value)
+
+    def voxel_grid_filter(self, pts, leaf=0.25):
+        """VoxelGridCmd: the voxel centroids, ordered by voxel index."""
+        a = np.array(_f32(pts).reshape(-1, 4))
+        n = C.c_int64()
+        _check(lib().phz_voxel_grid_filter(self.h, _p(a), a.shape[0], leaf, C.byref(n)))
+        return a[: n.value].copy()
+
+    def pass_through_filter(self, pts, field=2, lower=-5.0, upper=3.0, negative=True):
+        """PassThroughGndFilterCmd's defaults: drop z in [-5, 3]."""
+        a = np.array(_f32(pts).reshape(-1, 4))
+        n = C.c_int64()
+        _check(lib().phz_pass_through_filter(self.h, _p(a), a.shape[0], field, lower, upper,
+                                             int(negative), C.byref(n)))
+        return a[: n.value].copy()
 
     def zscore_peaks(self, corr, max_peaks=1 << 20, **kw):
         """ZScorePeakExtraction on the device; kw: lag_percentile, lag_max,

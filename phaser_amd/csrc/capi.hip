@@ -1,8 +1,10 @@
 // capi.hip -- context, resident tables, workspaces and the C ABI entry
 // points of phaser_amd (see include/phaser_amd.h for the reference
 // interfaces each one replaces).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -955,6 +957,24 @@ int commit_counts(phz_ctx* c, int ncl, hipStream_t s) {
 
 }  // namespace
 
+namespace {
+// device buffers of one stage call, freed on every return
+struct CallBuffers {
+  std::vector<void*> ptrs;
+  ~CallBuffers() {
+    for (void* p : ptrs) hipFree(p);
+  }
+  template <class T>
+  hipError_t get(T** p, size_t count) {
+    void* q = nullptr;
+    const hipError_t e = hipMalloc(&q, sizeof(T) * (count ? count : 1));
+    if (e == hipSuccess) ptrs.push_back(q);
+    *p = (T*)q;
+    return e;
+  }
+};
+}  // namespace
+
 extern "C" {
 
 void phz_default_config(phz_config* cfg) {
@@ -1417,6 +1437,98 @@ int phz_zscore_peaks(phz_ctx* c, const float* corr, int64_t n, const phz_zscore_
   hipFree(d_peaks);
   hipFree(d_count);
   return rc;
+}
+
+int phz_voxel_grid_filter(phz_ctx* c, float* pts, int64_t P, float leaf, int64_t* P_out) {
+  PHZ_CHECK(c && P_out && (pts || P == 0), "null argument");
+  PHZ_CHECK(P >= 0 && P < (1ll << 31), "point count must be in [0, 2^31)");
+  PHZ_CHECK(leaf > 0.f && std::isfinite(leaf), "leaf size must be positive");
+  *P_out = P;
+  if (P == 0) return 0;
+  hipStream_t s = c->stream;
+  CallBuffers buf;
+  float4 *d_pts, *d_out;
+  float *d_part, *d_box;
+  uint32_t *kin, *kout, *vin, *vout, *uniq, *cnt, *off;
+  int* d_nruns;
+  PHZ_HIP(buf.get(&d_pts, P));
+  PHZ_HIP(buf.get(&d_out, P));
+  PHZ_HIP(buf.get(&d_part, 256 * 6));
+  PHZ_HIP(buf.get(&d_box, 6));
+  for (uint32_t** q : {&kin, &kout, &vin, &vout, &uniq, &cnt, &off}) PHZ_HIP(buf.get(q, P));
+  PHZ_HIP(buf.get(&d_nruns, 1));
+  PHZ_HIP(hipMemcpyAsync(d_pts, pts, sizeof(float4) * P, hipMemcpyHostToDevice, s));
+  launch_bbox(d_pts, P, d_part, d_box, s);
+  float box[6];
+  PHZ_HIP(hipMemcpyAsync(box, d_box, sizeof(box), hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  if (!(box[0] <= box[3])) {  // no finite point: nothing survives the filter
+    *P_out = 0;
+    return 0;
+  }
+  // voxel_grid.hpp (PCL 1.8): inverse leaf in float, the overflow check
+  // (the input is returned unchanged when the grid exceeds INT32_MAX voxels)
+  const float inv = 1.0f / leaf;
+  int64_t d[3];
+  for (int k = 0; k < 3; ++k) d[k] = static_cast<int64_t>((box[3 + k] - box[k]) * inv) + 1;
+  if (d[0] * d[1] * d[2] > static_cast<int64_t>(std::numeric_limits<int32_t>::max())) return 0;
+  int min_b[3], max_b[3];
+  for (int k = 0; k < 3; ++k) {
+    min_b[k] = static_cast<int>(std::floor(box[k] * inv));
+    max_b[k] = static_cast<int>(std::floor(box[3 + k] * inv));
+  }
+  const int div0 = max_b[0] - min_b[0] + 1, div1 = max_b[1] - min_b[1] + 1;
+  launch_voxel_grid_keys(d_pts, P, make_float3(inv, inv, inv), make_int3(min_b[0], min_b[1], min_b[2]),
+                         make_int3(1, div0, div0 * div1), kin, vin, s);
+  size_t tb = 0;
+  PHZ_HIP(voxel_grid_sort_runs(nullptr, &tb, kin, kout, vin, vout, (int)P, uniq, cnt, off, d_nruns, s));
+  void* temp = nullptr;
+  PHZ_HIP(buf.get((unsigned char**)&temp, tb));
+  PHZ_HIP(voxel_grid_sort_runs(temp, &tb, kin, kout, vin, vout, (int)P, uniq, cnt, off, d_nruns, s));
+  launch_voxel_grid_centroids(d_pts, vout, cnt, off, d_nruns, (int)P, d_out, s);
+  PHZ_HIP(hipGetLastError());
+  int nruns = 0;
+  PHZ_HIP(hipMemcpyAsync(&nruns, d_nruns, sizeof(int), hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  uint32_t last = 0;
+  if (nruns > 0) TRY(copy_sync(c, &last, uniq + nruns - 1, sizeof(last), hipMemcpyDeviceToHost));
+  if (nruns > 0 && last == 0xffffffffu) --nruns;  // the non-finite points' run
+  if (nruns > 0)
+    TRY(copy_sync(c, pts, d_out, sizeof(float4) * nruns, hipMemcpyDeviceToHost));
+  *P_out = nruns;
+  return 0;
+}
+
+int phz_pass_through_filter(phz_ctx* c, float* pts, int64_t P, int field, float lower, float upper,
+                            int negative, int64_t* P_out) {
+  PHZ_CHECK(c && P_out && (pts || P == 0), "null argument");
+  PHZ_CHECK(P >= 0 && P < (1ll << 31), "point count must be in [0, 2^31)");
+  PHZ_CHECK(field >= 0 && field <= 3, "field must be 0 (x), 1 (y), 2 (z) or 3 (intensity)");
+  *P_out = 0;
+  if (P == 0) return 0;
+  hipStream_t s = c->stream;
+  CallBuffers buf;
+  float4 *d_pts, *d_out;
+  unsigned char* keep;
+  int* d_n;
+  PHZ_HIP(buf.get(&d_pts, P));
+  PHZ_HIP(buf.get(&d_out, P));
+  PHZ_HIP(buf.get(&keep, P));
+  PHZ_HIP(buf.get(&d_n, 1));
+  PHZ_HIP(hipMemcpyAsync(d_pts, pts, sizeof(float4) * P, hipMemcpyHostToDevice, s));
+  size_t tb = 0;
+  PHZ_HIP(pass_through_select(nullptr, &tb, d_pts, P, field, lower, upper, negative, keep, d_out,
+                              d_n, s));
+  void* temp = nullptr;
+  PHZ_HIP(buf.get((unsigned char**)&temp, tb));
+  PHZ_HIP(pass_through_select(temp, &tb, d_pts, P, field, lower, upper, negative, keep, d_out, d_n,
+                              s));
+  int n = 0;
+  PHZ_HIP(hipMemcpyAsync(&n, d_n, sizeof(int), hipMemcpyDeviceToHost, s));
+  PHZ_HIP(hipStreamSynchronize(s));
+  if (n > 0) TRY(copy_sync(c, pts, d_out, sizeof(float4) * n, hipMemcpyDeviceToHost));
+  *P_out = n;
+  return 0;
 }
 
 int phz_spatial_correlate_single(phz_ctx* c, int N, const float* f, const float* g, float* corr,

@@ -420,10 +420,99 @@ CloudController::CloudController(const std::string& method) {
 
 model::RegistrationResult CloudController::registerPointCloud(model::PointCloudPtr cloud_prev,
                                                               model::PointCloudPtr cloud_cur) {
+  CHECK_TRUE(cloud_prev && cloud_cur, "CloudController: null cloud");
+  preprocessor_.process(cloud_prev);
+  preprocessor_.process(cloud_cur);
   return registrator_->registerPointCloud(cloud_prev, cloud_cur);
 }
 
 }  // namespace phaser_core
+
+bool FLAGS_phaser_pre_enable_voxel_grid_downsampling = false;
+bool FLAGS_phaser_pre_enable_pass_through_gnd_filtering = false;
+bool FLAGS_phaser_pre_enable_geometric_cloud_segmentation = false;
+double FLAGS_phaser_voxel_grid_leaf_size = 0.25;
+double FLAGS_phaser_pass_through_gnd_filter_limit_min = -5.0;
+double FLAGS_phaser_pass_through_gnd_filter_limit_max = 3.0;
+
+namespace preproc {
+namespace {
+
+// the filters need only a stream on the current device: one small context
+// shared by every command of the process
+phz_ctx* filterContext() {
+  static std::mutex mu;
+  static std::shared_ptr<phz_ctx> ctx;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ctx) {
+    phz_config cfg;
+    phz_default_config(&cfg);
+    cfg.bandwidth = 8;
+    cfg.n_voxels = 8;
+    cfg.max_points = 16;
+    phz_ctx* c = nullptr;
+    check(phz_create(&cfg, &c), "phz_create");
+    ctx.reset(c, phz_destroy);
+  }
+  return ctx.get();
+}
+
+void resizeAfterFilter(model::PointCloud& cloud, int64_t kept) {
+  if ((size_t)kept == cloud.size()) return;
+  cloud.points().resize((size_t)kept);
+  cloud.reflectivities().clear();
+  cloud.ambientPoints().clear();
+}
+
+}  // namespace
+
+CloudPreProcessorSettings CloudPreProcessorSettings::fromGflags() {
+  CloudPreProcessorSettings s;
+  s.enable_voxel_grid_downsampling = FLAGS_phaser_pre_enable_voxel_grid_downsampling;
+  s.enable_pass_through_gnd_filtering = FLAGS_phaser_pre_enable_pass_through_gnd_filtering;
+  s.enable_geometric_cloud_segmentation = FLAGS_phaser_pre_enable_geometric_cloud_segmentation;
+  return s;
+}
+
+void VoxelGridCmd::execute(model::PointCloudPtr cloud) {
+  CHECK_TRUE(cloud != nullptr, "VoxelGridCmd: null cloud");
+  int64_t kept = 0;
+  check(phz_voxel_grid_filter(filterContext(), cloud->data(), (int64_t)cloud->size(),
+                              (float)FLAGS_phaser_voxel_grid_leaf_size, &kept),
+        "phz_voxel_grid_filter");
+  resizeAfterFilter(*cloud, kept);
+}
+
+void PassThroughGndFilterCmd::execute(model::PointCloudPtr cloud) {
+  CHECK_TRUE(cloud != nullptr, "PassThroughGndFilterCmd: null cloud");
+  int64_t kept = 0;
+  check(phz_pass_through_filter(filterContext(), cloud->data(), (int64_t)cloud->size(), 2,
+                                (float)FLAGS_phaser_pass_through_gnd_filter_limit_min,
+                                (float)FLAGS_phaser_pass_through_gnd_filter_limit_max, 1, &kept),
+        "phz_pass_through_filter");
+  resizeAfterFilter(*cloud, kept);
+}
+
+CloudPreProcessor::CloudPreProcessor()
+    : CloudPreProcessor(CloudPreProcessorSettings::fromGflags()) {}
+
+CloudPreProcessor::CloudPreProcessor(const CloudPreProcessorSettings& settings) {
+  initializeCommandFromSettings(settings);
+}
+
+void CloudPreProcessor::initializeCommandFromSettings(const CloudPreProcessorSettings& settings) {
+  if (settings.enable_voxel_grid_downsampling) processors_.emplace_back(new VoxelGridCmd());
+  if (settings.enable_pass_through_gnd_filtering)
+    processors_.emplace_back(new PassThroughGndFilterCmd());
+  if (settings.enable_geometric_cloud_segmentation)
+    fatal("SegmentCloudCmd (geometric cloud segmentation) is not part of this build");
+}
+
+void CloudPreProcessor::process(model::PointCloudPtr cloud) {
+  for (auto& p : processors_) p->execute(cloud);
+}
+
+}  // namespace preproc
 
 // ---------------------------------------------------------------- model
 namespace model {

@@ -113,6 +113,26 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
                           unsigned long long* partials, unsigned long long* keys,
                           hipStream_t s, bool reg = false);
 
+// CloudPreProcessor's filters (kernels_pre.hip).  launch_bbox: min / max of
+// the finite points' x, y, z into d_out[6] (d_part: 6 x the returned block
+// count floats, at most 256 blocks).
+int launch_bbox(const float4* pts, int64_t P, float* d_part, float* d_out, hipStream_t s);
+void launch_voxel_grid_keys(const float4* pts, int64_t P, float3 inv, int3 min_b, int3 mul,
+                            uint32_t* keys, uint32_t* vals, hipStream_t s);
+// stable sort of (key, point) pairs, runs of equal keys (unique, counts,
+// nruns) and their offsets; temp == null: *temp_bytes = the size needed
+hipError_t voxel_grid_sort_runs(void* temp, size_t* temp_bytes, const uint32_t* keys_in,
+                                uint32_t* keys_out, const uint32_t* vals_in, uint32_t* vals_out,
+                                int n, uint32_t* unique, uint32_t* counts, uint32_t* offsets,
+                                int* nruns, hipStream_t s);
+void launch_voxel_grid_centroids(const float4* pts, const uint32_t* order, const uint32_t* counts,
+                                 const uint32_t* offsets, const int* nruns, int max_runs,
+                                 float4* out, hipStream_t s);
+// kept points in order into out, their count into nsel; temp == null: size query
+hipError_t pass_through_select(void* temp, size_t* temp_bytes, const float4* pts, int64_t P,
+                               int field, float lo, float hi, int negative, unsigned char* keep,
+                               float4* out, int* nsel, hipStream_t s);
+
 // ZScorePeakExtraction (kernels_peaks.hip): nvec vectors of n floats,
 // stride apart; lag <= 64; signals of vector v in increasing order at
 // peaks + v * max_peaks (at most max_peaks stored), their count in counts[v]

@@ -7,10 +7,15 @@
 // refine_rot flags and FLAGS_estimate_translation were changed (one JSON
 // line per registration); a .bin file is float32 (x, y, z, intensity) quadruples and a .ply
 // file is read by model::PointCloud::readFromFile (with reflectivity and
-// ambient when it has them).
+// ambient when it has them).  PHZ_TEST_PRE="vg", "pt" or "vg,pt" switches the
+// controller's CloudPreProcessor commands on (the pre-processor flags).
+// usage: host_driver preprocess in.bin out.bin voxel_grid pass_through [leaf]:
+// a preproc::CloudPreProcessor with those settings over one cloud, the
+// filtered cloud written as .bin.
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <vector>
@@ -38,7 +43,32 @@ static model::PointCloudPtr load(const char* path) {
   return std::make_shared<model::PointCloud>(std::move(pts));
 }
 
+static int preprocess(int argc, char** argv) {
+  if (argc != 6 && argc != 7) {
+    std::fprintf(stderr, "usage: %s preprocess in.bin out.bin voxel_grid pass_through [leaf]\n",
+                 argv[0]);
+    return 2;
+  }
+  preproc::CloudPreProcessorSettings settings;
+  settings.enable_voxel_grid_downsampling = argv[4][0] == '1';
+  settings.enable_pass_through_gnd_filtering = argv[5][0] == '1';
+  if (argc == 7) FLAGS_phaser_voxel_grid_leaf_size = std::atof(argv[6]);
+  auto cloud = load(argv[2]);
+  preproc::CloudPreProcessor pre(settings);
+  pre.process(cloud);
+  std::ofstream f(argv[3], std::ios::binary);
+  f.write(reinterpret_cast<const char*>(cloud->data()),
+          (std::streamsize)(cloud->size() * sizeof(common::Point_t)));
+  std::printf("{\"points\": %zu}\n", cloud->size());
+  return f ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
+  if (argc >= 2 && std::strcmp(argv[1], "preprocess") == 0) return preprocess(argc, argv);
+  if (const char* pre = std::getenv("PHZ_TEST_PRE")) {
+    FLAGS_phaser_pre_enable_voxel_grid_downsampling = std::strstr(pre, "vg") != nullptr;
+    FLAGS_phaser_pre_enable_pass_through_gnd_filtering = std::strstr(pre, "pt") != nullptr;
+  }
   if (argc != 5 && argc != 7 && argc != 9 && argc != 11) {
     std::fprintf(stderr,
                  "usage: %s target.bin source.bin bandwidth n_voxels "
