@@ -280,6 +280,23 @@ int phz_spatial_correlate_low_pass(phz_ctx* ctx, int n_voxels, int zero_padding,
  * spherical zero padding. */
 int phz_spherical_correlate_low_pass(phz_ctx* ctx, const float* f_grid, const float* h_grid,
                                      int lower, int upper, float* corr, uint32_t* argmax);
+/* No counterpart in the reference.  phz_spatial_correlate_single for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): it calls no hipMalloc,
+ * hipFree or allocating routine and runs on a file-scope device array of
+ * 25.6 MB per device (twiddle tables, spectra, workspace and argmax keys),
+ * set up once per device by phz_create and held until the process ends.
+ * n_voxels in [2, 128] (other sizes return -1).
+ * d_f, d_g: N^3 device floats; d_corr: N^3 device floats or NULL; argmax:
+ * host pointer or NULL (then the call does not wait for the stream).
+ * stream: a hipStream_t of the context's device (NULL = the context's).
+ * Every call on one device shares that device's array: calls that overlap
+ * on one device race and must be serialised by the caller.  Results equal
+ * phz_spatial_correlate_single's on the same inputs. */
+int phz_spatial_correlate_single_noalloc(phz_ctx* ctx, int n_voxels, const float* d_f,
+                                         const float* d_g, float* d_corr, uint32_t* argmax,
+                                         void* stream);
+
 /* ---- preprocessing (CloudPreProcessor, cloud-pre-processor.cc:30-44) --- */
 /* VoxelGridCmd (voxel-grid-cmd.cc:11-24): PCL VoxelGrid (1.8) with leaf
  * size `leaf` on all three axes (phaser_voxel_grid_leaf_size, 0.25) and

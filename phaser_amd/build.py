@@ -24,7 +24,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("PHZ_OFFLOAD_ARCH", "gfx950")
 
 HIP_SOURCES = ["kernels_s2.hip", "kernels_so3.hip", "kernels_spatial.hip", "kernels_mfma.hip",
-               "kernels_peaks.hip", "kernels_pre.hip", "capi.hip"]
+               "kernels_peaks.hip", "kernels_pre.hip", "noalloc.hip", "capi.hip"]
 CPP_SOURCES = ["tables.cpp"]
 HOST_SOURCES = ["host/phaser_host.cc", "host/ply_reader.cc"]
 

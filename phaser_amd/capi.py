@@ -89,6 +89,9 @@ EXPORTS = {
     "phz_spatial_correlate_single_padded": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p,
                                                       C.c_void_p, C.c_void_p,
                                                       C.POINTER(C.c_uint32)]),
+    "phz_spatial_correlate_single_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p,
+                                                       C.c_void_p, C.c_void_p,
+                                                       C.POINTER(C.c_uint32), C.c_void_p]),
     "phz_spatial_correlate_low_pass": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                                  C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.POINTER(C.c_uint32)]),
@@ -481,3 +484,12 @@ class Context:
         _check(lib().phz_spatial_correlate_single_padded(self.h, N, pad, _p(f), _p(g), _p(corr),
                                                          C.byref(am)))
         return corr, int(am.value)
+
+    def spatial_correlate_single_noalloc(self, d_f, d_g, d_corr, N):
+        """phz_spatial_correlate_single_noalloc on device buffers (DeviceArray
+        or raw addresses; d_corr may be None); returns the argmax."""
+        addr = lambda a: None if a is None else C.c_void_p(getattr(a, "ptr", a))
+        am = C.c_uint32()
+        _check(lib().phz_spatial_correlate_single_noalloc(self.h, N, addr(d_f), addr(d_g),
+                                                          addr(d_corr), C.byref(am), None))
+        return int(am.value)

@@ -420,6 +420,9 @@ int init_ctx(phz_ctx* c) {
   }
   PHZ_HIP(hipSetDevice(cf.device));
   PHZ_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  // the no-allocation entry point's device array: its address and twiddle
+  // tables, once per device (this device is current)
+  TRY(noalloc_setup(cf.device, c->stream));
   build_s2_tables(c->B, &c->s2);
   build_voxel_tables(c->N, (float)cf.discretize_lower, (float)cf.discretize_upper, &c->vox);
   for (int i = 1; i < c->N; ++i)  // the voxel binning relies on it
@@ -1572,6 +1575,14 @@ int phz_spherical_correlate_low_pass(phz_ctx* c, const float* f_grid, const floa
              c->cfg.precision, s);
   launch_coef_band(c->d_coef, BB, 2, (uint32_t)BB, lo, hi, s);
   return so3_from_fused(c, c->d_coef, c->d_coef + BB, corr, nullptr, argmax, s);
+}
+
+int phz_spatial_correlate_single_noalloc(phz_ctx* c, int n_voxels, const float* d_f,
+                                         const float* d_g, float* d_corr, uint32_t* argmax,
+                                         void* stream) {
+  PHZ_CHECK(c, "null argument");
+  return spatial_single_noalloc(c->cfg.device, n_voxels, d_f, d_g, d_corr, argmax,
+                                stream ? (hipStream_t)stream : c->stream);
 }
 
 // ---------------------------------------------------------------- statistics

@@ -302,6 +302,16 @@ void launch_spatial_forward_dense(int N, int nch, const float* chans, const FftP
 // with the default column tiles
 void launch_spatial_forward_two(int N, const float* a, const float* b, const FftPlan& plan,
                                 const float2* tw, float2* spec, hipStream_t s);
+// phz_spatial_correlate_single_noalloc (noalloc.hip): the single-channel
+// spatial correlation on a file-scope device array.  noalloc_setup runs once
+// per device with that device current (phz_create): it takes the array's
+// device address and writes the twiddle tables; spatial_single_noalloc then
+// runs on that address only.
+size_t noalloc_scratch_bytes();
+int noalloc_max_n();
+int noalloc_setup(int device, hipStream_t s);
+int spatial_single_noalloc(int device, int N, const float* d_f, const float* d_g, float* d_corr,
+                           uint32_t* argmax, hipStream_t s);
 // fused cross spectrum + inverse 3-D c2r + argmax for npairs pairs.  F
 // channels and G channels (null = zero channel) in the reference's spatial
 // order; pair p reads its channels at ch[c] + p*stride (stride 0: shared).
