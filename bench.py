@@ -332,8 +332,9 @@ def main():
                     help="split each batch over this many contexts/streams driven concurrently")
     ap.add_argument("--nn-xcd", type=int, default=0, choices=[-1, 0, 1],
                     help="XCD-ordered NN workgroups: 1 on, -1 off, 0 library default")
-    ap.add_argument("--nn-seed", type=int, default=0, choices=[-1, 0, 1],
-                    help="two-pass seeded NN search: 1 on, -1 off, 0 library default")
+    ap.add_argument("--nn-seed", type=int, default=0, choices=[-1, 0, 1, 2, 3],
+                    help="NN search: bit 0 seeds, bit 1 LDS band skipping; -1 neither, "
+                         "0 library default")
     ap.add_argument("--inv-plane", type=int, default=0, choices=[-1, 0, 1],
                     help="inverse x + c2r as one pass over z planes in LDS (0: default, on)")
     ap.add_argument("--occ-fwd", type=int, default=0, choices=[-1, 0, 1],

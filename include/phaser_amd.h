@@ -368,12 +368,14 @@ typedef struct {
   int32_t nn_xcd;      /* S^2 nearest-neighbour workgroups ordered so that each of the
                           8 XCDs works through its own clouds (launches of a multiple
                           of 8 clouds): 0 = default, 1 = on, -1 = off */
-  int32_t nn_seed;     /* S^2 nearest-neighbour search in two passes: every 4th query
-                          row and column first, then every query starting from the
-                          nearest of its four surrounding first-pass points (an upper
-                          bound that prunes the walk from its first band; the result
-                          is the same exact nearest point): 0 or 1 = on (default),
-                          -1 = off */
+  int32_t nn_seed;     /* how the exact S^2 nearest-neighbour search starts and prunes
+                          (the result is the same exact nearest point either way):
+                          bit 0 = seeds (each query starts from a point near its
+                          coarse cell, found by jump flooding over the cell grid: an
+                          upper bound that prunes the walk from its first band),
+                          bit 1 = bands skipped on their sector occupancy bits in LDS;
+                          0 = default (neither: at 64 pairs per launch neither
+                          shortened the search), -1 = neither, 1, 2 or 3 */
   float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
   float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
   float ms_fused;      /* the three fused passes at max_batch, chosen form */

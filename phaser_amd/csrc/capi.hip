@@ -75,9 +75,6 @@ constexpr int kMaxBandwidth = 160;
 // XCD-ordered S^2 nearest-neighbour workgroups by default (measured on
 // gfx950 at B = 64, 64 clouds per launch: k_nn 1.34 -> 1.18 ms)
 constexpr bool kNnXcdDefault = true;
-// the exact NN search seeded by a first pass over every 4th query row and
-// column (the full pass then prunes from its first band)
-constexpr int kNnSeedStride = 4;
 
 struct phz_ctx {
   phz_config cfg{};
@@ -105,7 +102,7 @@ struct phz_ctx {
   bool occ_fwd = true;     // forward passes skip rows / planes without points
   bool inv_plane = true;   // inverse x + c2r as one pass over z planes in LDS
   bool nn_xcd = kNnXcdDefault;  // XCD-ordered NN workgroups
-  int nn_seed = kNnSeedStride;  // stride of the NN seeding pass (0: off)
+  int nn_seed = kNnSeedDefault;  // NN seeds (bit 0) and LDS band skipping (bit 1)
   phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
@@ -314,10 +311,8 @@ size_t context_bytes(const phz_config& cf) {
   add(M * cap, sizeof(float4));
   add(M * cap, sizeof(float2));
   add(M * cap, sizeof(int));
-  {
-    const int64_t nc = (n + kNnSeedStride - 1) / kNnSeedStride;
-    add(M * nc * nc, sizeof(int));
-  }
+  add(M * kSeedCells, sizeof(int));
+  add(M * n * ((n + 63) / 64), sizeof(unsigned long long));
   // spherical spectra and SO(3) grids
   add(G * nq, sizeof(float));
   add(G * B * n, sizeof(float2));
@@ -493,10 +488,8 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));
-  {
-    const size_t nc = (size_t)(c->n + kNnSeedStride - 1) / kNnSeedStride;
-    TRY(dalloc(c, &c->cells.seed_pos, (size_t)M * nc * nc));
-  }
+  TRY(dalloc(c, &c->cells.seed_pos, (size_t)M * kSeedCells));
+  TRY(dalloc(c, &c->cells.band_mask, (size_t)M * c->n * ((c->n + 63) / 64)));
   // stage entry points use up to 16 grids / signals; with four channels the
   // reflectivity and ambient grids of all clouds follow the range and
   // intensity grids
@@ -569,8 +562,8 @@ int init_ctx(phz_ctx* c) {
   c->tuning.inv_plane = 1;
   c->nn_xcd = kNnXcdDefault;
   c->tuning.nn_xcd = kNnXcdDefault ? 1 : -1;
-  c->nn_seed = kNnSeedStride;
-  c->tuning.nn_seed = 1;
+  c->nn_seed = kNnSeedDefault;
+  c->tuning.nn_seed = kNnSeedDefault ? kNnSeedDefault : -1;
   return 0;
 }
 
@@ -1670,8 +1663,8 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.inv_plane = c->inv_plane ? 1 : -1;
   c->nn_xcd = t.nn_xcd == 0 ? kNnXcdDefault : t.nn_xcd > 0;
   c->tuning.nn_xcd = c->nn_xcd ? 1 : -1;
-  c->nn_seed = t.nn_seed >= 0 ? kNnSeedStride : 0;
-  c->tuning.nn_seed = c->nn_seed ? 1 : -1;
+  c->nn_seed = t.nn_seed == 0 ? kNnSeedDefault : (t.nn_seed > 0 ? (t.nn_seed & 3) : 0);
+  c->tuning.nn_seed = c->nn_seed ? c->nn_seed : -1;
 }
 
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {

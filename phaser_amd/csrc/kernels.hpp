@@ -27,19 +27,34 @@ struct CellBuffers {
   float4* proj;      // nclouds * cap
   float2* val;       // nclouds * cap
   int* cell;         // nclouds * cap
-  int* seed_pos;     // nclouds * ceil(n / S)^2: sorted position of each coarse
-                     // query's nearest point (-1: none), null = no seeding
+  int* seed_pos;     // nclouds * kSeedCells: per coarse cell the sorted position
+                     // of a point near its centre (-1: none)
+  unsigned long long* band_mask;  // nclouds * nb * ceil(ns / 64): sector occupancy
 };
 
-// Seeding of the exact nearest-neighbour search: pos holds, per cloud, the
-// nearest point (sorted position) of the coarse queries at every stride-th
-// row and column; record = 1: this launch runs only those queries and
-// writes pos; record = 0 and stride > 0: every query starts from the
-// nearest of its four surrounding coarse queries' points.
+// The S^2 search's coarse seed grid: F x F cells per coarse cell, F the
+// smallest with ceil(n / F)^2 <= kSeedCells.
+constexpr int kSeedCells = 4096;
+inline int seed_factor(int n) {
+  int F = 1;
+  while (((n + F - 1) / F) * ((n + F - 1) / F) > kSeedCells) ++F;
+  return F;
+}
+
+// How the exact nearest-neighbour search starts and prunes (phz_tuning.
+// nn_seed): seeds = every query starts from its coarse cell's seed point (a
+// real point: an upper bound on its nearest distance); mask = bands whose
+// nearest occupied sectors are already out of reach are skipped on the
+// band's occupancy bits in LDS, without a memory round trip.
+// phz_tuning.nn_seed's default: neither (measured at 64 pairs per launch on
+// gfx950: k_nn 1.265 ms plain, 1.263 ms with seeds, 1.353 ms with the mask,
+// 1.290 ms with both)
+constexpr int kNnSeedDefault = 0;
 struct NnSeeds {
-  int* pos = nullptr;
-  int stride = 0;
-  int record = 0;
+  const int* pos = nullptr;
+  const unsigned long long* mask = nullptr;
+  int F = 0, cns = 0;
+  int use_seeds = 0, use_mask = 0;
 };
 
 void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
@@ -65,7 +80,7 @@ int nn_auto_lanes(int nclouds);
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
                   const float2* extra = nullptr, float* xgrids = nullptr, bool xcd = false,
-                  bool blend = false, int seed_stride = 0);
+                  bool blend = false, int seed_mode = 0);
 
 // ------------------------------------------------------------- S^2 transform
 // sig: S real grids of n*n.  F: S x B x n complex scratch.  coef: S x B*B

@@ -11,6 +11,7 @@
 // the chord distance, so the answer equals a brute-force scan (ties go to the
 // lowest point index).
 #include "kernels.hpp"
+#include "libm_f32.hpp"
 
 namespace phz {
 
@@ -19,7 +20,8 @@ namespace {
 constexpr float kPi = 3.14159265358979323846f;
 
 // spherical-projection.cc:19-41 with the reference's float/double mix and no
-// FMA contraction.
+// FMA contraction; acosf and atan2f round as the host libm rounds them, so
+// that the projected points are the reference's to the bit.
 __device__ __forceinline__ void project_point(float x, float y, float z, float* px, float* py,
                                               float* pz, float* range) {
 #pragma clang fp contract(off)
@@ -28,8 +30,8 @@ __device__ __forceinline__ void project_point(float x, float y, float z, float* 
   const double zz = (double)(z * z);
   const double dist = sqrt(dxyd * dxyd + zz);
   const float pzf = (float)((double)z / dist);
-  const double tx = (double)acosf(pzf);
-  const double ty = fmod((double)atan2f(y, x) + 2.0 * M_PI, 2.0 * M_PI);
+  const double tx = (double)libm::acosf_host_exact(pzf);
+  const double ty = fmod((double)libm::atan2f_host_exact(y, x) + 2.0 * M_PI, 2.0 * M_PI);
   const double stx = sin(tx);
   *px = (float)(stx * cos(ty));
   *py = (float)(stx * sin(ty));
@@ -175,7 +177,8 @@ constexpr int kMaskWords = (kMaxSectors + 63) / 64;
 // the occupancy of the band's n <= kMaxSectors sectors is a ballot mask of
 // kMaskWords 64-bit words.
 __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
-                              int4* __restrict__ nr, int4* __restrict__ nl) {
+                              int4* __restrict__ nr, int4* __restrict__ nl,
+                              unsigned long long* __restrict__ band_mask) {
   // n = sectors per band here
   const int b = blockIdx.x, c = blockIdx.y;
   const int lane = threadIdx.x;  // 64 lanes
@@ -192,6 +195,10 @@ __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
     m[w] = __ballot(ne);
     all |= m[w];
   }
+  // the band's occupancy words, read by the search to skip bands in LDS
+#pragma unroll
+  for (int w = 0; w < kMaskWords; ++w)
+    if (lane == w && w < nw) band_mask[((int64_t)c * nb + b) * nw + w] = m[w];
   const bool any = all != 0ull;
   for (int s = lane; s < n; s += 64) {
     int right = -1, left = -1;
@@ -258,6 +265,116 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
   return r;
 }
 
+// Seed points of the search: per coarse cell (F x F cells of the search
+// grid, cnb x cns of them), a point of the cloud near the coarse cell's
+// centre, by jump flooding (each pass takes the nearest of the candidates of
+// the 3 x 3 coarse cells at distance k, k = 2^j ... 1, then 2, 1).  Only the
+// seeds' quality depends on this: any point of the cloud bounds a query's
+// nearest distance from above, and the search stays exact.  One workgroup
+// per cloud, both candidate generations in LDS (2 x kSeedCells float4: the
+// point and its sorted position).
+__global__ void __launch_bounds__(1024) k_cell_seeds(int nb, int ns, int F, int cnb, int cns,
+                                                     const int* __restrict__ cell_start,
+                                                     const float4* __restrict__ sorted,
+                                                     int64_t cap, int* __restrict__ seed_pos) {
+  extern __shared__ float4 gen[];  // 2 * kSeedCells
+  const int c = blockIdx.x, ncc = cnb * cns;
+  const int* st = cell_start + (int64_t)c * (nb * ns + 1);
+  const float4* sp = sorted + c * cap;
+  float4* cur = gen;
+  float4* nxt = gen + kSeedCells;
+  // generation 0: the first point of the coarse cell's first occupied cell
+  for (int cc = threadIdx.x; cc < ncc; cc += blockDim.x) {
+    const int cb = cc / cns, cs = cc - cb * cns;
+    int pos = -1;
+    for (int i = 0; i < F && pos < 0; ++i) {
+      const int b = cb * F + i;
+      if (b >= nb) break;
+      for (int j = 0; j < F; ++j) {
+        const int sct = cs * F + j;
+        if (sct >= ns) break;
+        if (st[b * ns + sct + 1] > st[b * ns + sct]) {
+          pos = st[b * ns + sct];
+          break;
+        }
+      }
+    }
+    float4 v = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+    if (pos >= 0) {
+      v = sp[pos];
+      v.w = __int_as_float(pos);
+    }
+    cur[cc] = v;
+  }
+  __syncthreads();
+  // steps kmax, kmax / 2, ..., 1, then 2, 1 (two refining passes)
+  int kmax = 1, lg = 0;
+  while (2 * kmax < (cnb > cns ? cnb : cns)) {
+    kmax *= 2;
+    ++lg;
+  }
+  const int npass = lg + 1 + (kmax >= 2 ? 2 : 0);
+  for (int pass = 0; pass < npass; ++pass) {
+    const int k = pass <= lg ? kmax >> pass : (pass == lg + 1 ? 2 : 1);
+    for (int cc = threadIdx.x; cc < ncc; cc += blockDim.x) {
+      const int cb = cc / cns, cs = cc - cb * cns;
+      const float th = fminf((cb + 0.5f) * F * (kPi / nb), kPi);
+      const float ph = (cs * F + 0.5f * (F - 1)) * (2.f * kPi / ns);
+      const float4 q = make_float4(sinf(th) * cosf(ph), sinf(th) * sinf(ph), cosf(th), 0.f);
+      float4 best = cur[cc];
+      float bd = __float_as_int(best.w) >= 0 ? l2_simple(q, best) : INFINITY;
+      for (int db = -1; db <= 1; ++db) {
+        const int bb = cb + db * k;
+        if (bb < 0 || bb >= cnb) continue;
+        for (int ds = -1; ds <= 1; ++ds) {
+          int ss = cs + ds * k;
+          ss = ((ss % cns) + cns) % cns;
+          const float4 v = cur[bb * cns + ss];
+          if (__float_as_int(v.w) < 0) continue;
+          const float d = l2_simple(q, v);
+          if (d < bd) {
+            bd = d;
+            best = v;
+          }
+        }
+      }
+      nxt[cc] = best;
+    }
+    __syncthreads();
+    float4* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  for (int cc = threadIdx.x; cc < ncc; cc += blockDim.x)
+    seed_pos[(int64_t)c * kSeedCells + cc] = __float_as_int(cur[cc].w);
+}
+
+// first set bit at or after s (cyclically) of a band's occupancy words, or -1
+__device__ __forceinline__ int mask_next(const unsigned long long* m, int nw, int s) {
+  for (int k = 0; k <= nw; ++k) {
+    int w = (s >> 6) + k;
+    w = w >= nw ? w - nw : w;
+    unsigned long long mm = m[w];
+    if (k == 0) mm &= ~0ull << (s & 63);
+    if (k == nw) mm &= (s & 63) ? ((1ull << (s & 63)) - 1ull) : 0ull;
+    if (mm) return w * 64 + __builtin_ctzll(mm);
+  }
+  return -1;
+}
+
+// last set bit at or before s (cyclically), or -1
+__device__ __forceinline__ int mask_prev(const unsigned long long* m, int nw, int s) {
+  for (int k = 0; k <= nw; ++k) {
+    int w = (s >> 6) - k;
+    w = w < 0 ? w + nw : w;
+    unsigned long long mm = m[w];
+    if (k == 0) mm &= (s & 63) == 63 ? ~0ull : ((1ull << ((s & 63) + 1)) - 1ull);
+    if (k == nw) mm &= (s & 63) == 63 ? 0ull : (~0ull << ((s & 63) + 1));
+    if (mm) return w * 64 + 63 - __builtin_clzll(mm);
+  }
+  return -1;
+}
+
 // Exact nearest neighbour of every query, LANES lanes per query.  Bands
 // are visited in order of colatitude distance (own, +1, -1, +2, ...),
 // sectors of a band outward from the query's own, and a cell is skipped once
@@ -270,7 +387,13 @@ __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
 // batched so that a walk step costs about one memory round trip: the points
 // of a cell in groups of four per lane, the next jump-table entries while the
 // current cells are scanned (the walk then decides with a best that may lag
-// by one step, which only prunes less).
+// by one step, which only prunes less).  With seeds, a query starts from its
+// coarse cell's seed point (a point of the cloud: its distance bounds the
+// nearest one from above, so the walk prunes from its first band and still
+// finds the exact nearest point with the same tie rule); with the mask, a
+// band whose nearest occupied sectors on both sides are already out of reach
+// is skipped on its occupancy bits in LDS (the decision the walk would take
+// after loading its first entries).
 template <int LANES>
 __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ theta, int n,
                      int nb, int ns, float max_sq, const int* __restrict__ cell_start,
@@ -295,25 +418,25 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     qb = blockIdx.x;
   }
   const int sub = threadIdx.x & (LANES - 1);
-  const int ql = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
+  const int q = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
-  // seeds.record: this pass runs the coarse queries (every S-th row and
-  // column, nc = ceil(n / S) of each) and records their nearest points
-  const int S = seeds.stride, nc = S > 0 ? (n + S - 1) / S : 0;
-  const int nql = seeds.record ? nc * nc : nq;
-  // min sin(theta) over each band and the cloud's band table (nb <=
-  // kMaxSectors), once per workgroup
+  const int nw = (ns + 63) >> 6;
+  // min sin(theta) over each band, the cloud's band table (nb <=
+  // kMaxSectors) and the bands' occupancy words, once per workgroup
   __shared__ float band_sin_min[kMaxSectors];
   __shared__ int bs[kMaxSectors + 1];
+  extern __shared__ unsigned long long bmask[];  // nb * nw words with the mask, else none
   for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
     const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
     band_sin_min[bb] = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
   }
   for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x)
     bs[bb] = band_start[(int64_t)c * (nb + 1) + bb];
+  if (seeds.use_mask)
+    for (int i = threadIdx.x; i < nb * nw; i += blockDim.x)
+      bmask[i] = seeds.mask[(int64_t)c * nb * nw + i];
   __syncthreads();
-  if (ql >= nql) return;  // whole groups leave together
-  const int q = seeds.record ? (ql / nc) * S * n + (ql % nc) * S : ql;
+  if (q >= nq) return;  // whole groups leave together
   const int qrow = q / n, qcol = q - qrow * n;
   const float4 qv = grid[q];
   const float thq = theta[qrow];
@@ -338,28 +461,16 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // recomputed only when the best changes
   constexpr float kBoundErr = 3e-5f;
   float thr = INFINITY;
-  // a seeded pass starts from the nearest of the points the four coarse
-  // queries around this one found: real points of the cloud, so their
-  // distance bounds the exact nearest one from above and the walk below
-  // (unchanged, with the same tie rule) prunes from its first band
-  if (S > 0 && !seeds.record) {
-    const int r0 = qrow / S, c0 = qcol / S;
-    const int r1 = r0 + 1 < nc ? r0 + 1 : r0, c1 = c0 + 1 < nc ? c0 + 1 : 0;
-    const int* sd = seeds.pos + (int64_t)c * nc * nc;
-    const int cand[4] = {sd[r0 * nc + c0], sd[r0 * nc + c1], sd[r1 * nc + c0], sd[r1 * nc + c1]};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (cand[k] < 0) continue;
-      const float4 pv = sp[cand[k]];
-      const float d = l2_simple(qv, pv);
-      const int oi = __float_as_int(pv.w);
-      if (d < best || (d == best && oi < bidx)) {
-        best = d;
-        bidx = oi;
-        bpos = cand[k];
-      }
-    }
-    if (bpos >= 0) {
+  // the coarse cell's seed point: a point of the cloud, so its distance
+  // bounds the exact nearest one from above and the walk below (unchanged,
+  // with the same tie rule) prunes from its first band
+  if (seeds.use_seeds) {
+    const int pos = seeds.pos[(int64_t)c * kSeedCells + (qi / seeds.F) * seeds.cns + qj / seeds.F];
+    if (pos >= 0) {
+      const float4 pv = sp[pos];
+      best = l2_simple(qv, pv);
+      bidx = __float_as_int(pv.w);
+      bpos = pos;
       const float t = sqrtf(best) + slack;
       thr = t * t + kBoundErr;
     }
@@ -427,14 +538,6 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
     const int cb = b * ns;
-    // the own cell's range and the first entries of both walks in one round
-    const int own0 = st[cb + qj], own1 = st[cb + qj + 1];
-    int4 jr = nr[cb + wrap(qj + 1)];
-    int4 jl = nl[cb + wrap(qj - 1 + ns)];
-    scan_range(own0, own1);
-    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
-    int ur = 0, ul = 0;
-    bool rdone = false, ldone = false;
     auto bound = [&](int sec) {
       // azimuth distance from the query to the sector's interval, in
       // [0, pi], where 1 - cos is increasing: the bound grows with u
@@ -444,6 +547,29 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       const float s2 = __sinf(0.5f * dphi);
       return A + Bc * 2.f * s2 * s2;
     };
+    if (seeds.use_mask) {
+      // own cell empty and the nearest occupied sector on each side (right
+      // at distances 1..ns/2, left at 1..(ns-1)/2, as the walks below) out
+      // of reach: the band holds no point within the threshold
+      const unsigned long long* m = bmask + b * nw;
+      if (!((m[qj >> 6] >> (qj & 63)) & 1ull)) {
+        const int pr = mask_next(m, nw, wrap(qj + 1));
+        const int pl = mask_prev(m, nw, wrap(qj - 1 + ns));
+        const int dr = pr < 0 ? ns : wrap(pr - qj + ns);
+        const int dl = pl < 0 ? ns : wrap(qj - pl + ns);
+        const bool reach_r = dr >= 1 && dr <= ns / 2 && !(bound(pr) > thr);
+        const bool reach_l = dl >= 1 && dl <= (ns - 1) / 2 && !(bound(pl) > thr);
+        if (!reach_r && !reach_l) continue;
+      }
+    }
+    // the own cell's range and the first entries of both walks in one round
+    const int own0 = st[cb + qj], own1 = st[cb + qj + 1];
+    int4 jr = nr[cb + wrap(qj + 1)];
+    int4 jl = nl[cb + wrap(qj - 1 + ns)];
+    scan_range(own0, own1);
+    // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
+    int ur = 0, ul = 0;
+    bool rdone = false, ldone = false;
     while (!rdone || !ldone) {
       int4 cr = jr, cl = jl;
       bool scan_r = false, scan_l = false;
@@ -472,10 +598,6 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       if (scan_r) scan_range(cr.y, cr.z);
       if (scan_l) scan_range(cl.y, cl.z);
     }
-  }
-  if (seeds.record) {
-    if (sub == 0) seeds.pos[(int64_t)c * nc * nc + ql] = bpos;
-    return;
   }
   if (sub == 0) {
     float r = 0.f, it = 0.f;
@@ -600,7 +722,8 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
                       const float4* grid, const float* theta, float max_sq, CellBuffers cb,
                       float* grids, hipStream_t s) {
   launch_s2_cells(pts, npts, cap, nclouds, n, cb, s);
-  launch_s2_nn(cap, nclouds, n, grid, theta, max_sq, cb, grids, 4, s);
+  launch_s2_nn(cap, nclouds, n, grid, theta, max_sq, cb, grids, 4, s, nullptr, nullptr, false,
+               false, kNnSeedDefault);
 }
 
 void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int nclouds, int n,
@@ -626,20 +749,20 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj, cb.val,
                      cb.cell, cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
   hipLaunchKernelGGL(k_jump_tables, dim3(cb.nb, nclouds), dim3(64), 0, s, cb.nb, cb.ns,
-                     cb.cell_start, cb.next_right, cb.next_left);
+                     cb.cell_start, cb.next_right, cb.next_left, cb.band_mask);
 }
 
 template <int L>
 static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                         float max_sq, const CellBuffers& cb, float* grids, const float2* extra,
                         float* xgrids, bool xcd, bool blend, NnSeeds seeds, hipStream_t s) {
-  const int S = seeds.stride, nc = S > 0 ? (n + S - 1) / S : 0;
-  const int nq = seeds.record ? nc * nc : n * n, qpb = 256 / L;
+  const int nq = n * n, qpb = 256 / L;
   const int nqb = (nq + qpb - 1) / qpb;
   // the XCD-ordered grid needs whole groups of 8 clouds
   const bool by_xcd = xcd && nclouds % 8 == 0;
   const dim3 gq = by_xcd ? dim3((unsigned)(nqb * nclouds)) : dim3((unsigned)nqb, nclouds);
-  hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), 0, s, grid, theta, n, cb.nb, cb.ns, max_sq,
+  const size_t lds = seeds.use_mask ? (size_t)cb.nb * ((cb.ns + 63) / 64) * 8 : 0;
+  hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), lds, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
                      cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
                      seeds);
@@ -662,22 +785,23 @@ static void launch_nn_lanes(int lanes, int64_t cap, int nclouds, int n, const fl
 
 void launch_s2_nn(int64_t cap, int nclouds, int n, const float4* grid, const float* theta,
                   float max_sq, CellBuffers cb, float* grids, int lanes, hipStream_t s,
-                  const float2* extra, float* xgrids, bool xcd, bool blend, int seed_stride) {
+                  const float2* extra, float* xgrids, bool xcd, bool blend, int seed_mode) {
   if (lanes == 0) lanes = nn_auto_lanes(nclouds);
-  NnSeeds none{};
-  if (seed_stride > 1 && cb.seed_pos) {
-    // coarse queries first (a sixteenth of them at stride 4, more lanes
-    // each: they run unseeded), then every query seeded by them
-    NnSeeds rec{cb.seed_pos, seed_stride, 1};
-    launch_nn_lanes(lanes * 2 <= 16 ? lanes * 2 : 16, cap, nclouds, n, grid, theta, max_sq, cb,
-                    grids, extra, xgrids, xcd, blend, rec, s);
-    NnSeeds use{cb.seed_pos, seed_stride, 0};
-    launch_nn_lanes(lanes, cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd,
-                    blend, use, s);
-    return;
+  NnSeeds sd{};
+  sd.use_seeds = (seed_mode & 1) && cb.seed_pos;
+  sd.use_mask = (seed_mode & 2) && cb.band_mask;
+  sd.pos = cb.seed_pos;
+  sd.mask = cb.band_mask;
+  sd.F = seed_factor(cb.nb > cb.ns ? cb.nb : cb.ns);
+  sd.cns = (cb.ns + sd.F - 1) / sd.F;
+  if (sd.use_seeds) {
+    const int cnb = (cb.nb + sd.F - 1) / sd.F;
+    hipLaunchKernelGGL(k_cell_seeds, dim3(nclouds), dim3(1024), 2 * kSeedCells * sizeof(float4), s,
+                       cb.nb, cb.ns, sd.F, cnb, sd.cns, cb.cell_start, cb.sorted, cap,
+                       cb.seed_pos);
   }
   launch_nn_lanes(lanes, cap, nclouds, n, grid, theta, max_sq, cb, grids, extra, xgrids, xcd,
-                  blend, none, s);
+                  blend, sd, s);
 }
 
 void launch_fst(const float* sig, int S, int bw, const float* legendre, const int* leg_off_dev,

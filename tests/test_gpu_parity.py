@@ -1,7 +1,8 @@
 """GPU parity: every stage of the HIP path against the FP64 oracle.
 
 Tolerances (FP32 device arithmetic vs the FP64 oracle):
-  * S^2 samples: the nearest point agrees on >= 99.9% of queries;
+  * S^2 samples: bit-identical (the projection rounds acosf / atan2f as the
+    host libm does, the search is exact with ties to the lowest index);
   * S^2 coefficients: max |diff| <= 2e-5 * max |coef|;
   * SO(3) and spatial correlation grids: max |diff| <= 2e-4 * max |grid|;
   * argmax indices: bit-exact, except where the oracle's own top two values
@@ -58,6 +59,30 @@ def test_sample_s2_matches_oracle(ctx32, po):
         o = po.sample_cloud(32, cloud, kdtree=True)
         same = np.isclose(g[0], o[0], rtol=1e-6, atol=0) & (g[1] == o[1].astype(np.float32))
         assert same.mean() >= 0.999, same.mean()
+
+
+@pytest.mark.parametrize("B,seeds", [(32, (3, 5)), (64, (166, 217, 355, 11)), (128, (0,))])
+def test_s2_samples_are_bit_identical(po, B, seeds):
+    """The device's S^2 samples equal the oracle's exactly: every projected
+    point is the host's to the bit (acosf / atan2f restated from the host
+    libm), so every query picks the oracle's nearest point, ties included.
+    Seeds 166 and 217 of the B = 64 workload are pairs whose samples differed
+    while the device used its own acosf / atan2f."""
+    from phaser_amd import synth
+    from phaser_amd.capi import Context
+    cfg = dict(synth.CONFIG_B64) if B == 64 else dict(rings=64 if B == 32 else 128,
+                                                         azimuths=1024)
+    for seed in seeds:
+        src, tgt, _, _ = synth.pair(seed=seed, **cfg)
+        with Context(bandwidth=B, n_voxels=2 * B, max_points=max(len(src), len(tgt))) as c:
+            for name, cloud in (("source", src), ("target", tgt)):
+                g = c.sample_s2(cloud)
+                o = po.sample_cloud(B, cloud, kdtree=True)
+                diff = np.nonzero((g[0] != o[0].astype(np.float32))
+                                  | (g[1] != o[1].astype(np.float32)))[0]
+                print(f"B {B} seed {seed} {name}: {len(diff)} of {g.shape[1]} samples differ "
+                      f"{diff[:8].tolist()}", flush=True)
+                assert len(diff) == 0
 
 
 @pytest.mark.parametrize("B", [64, 128])
@@ -512,10 +537,11 @@ def test_nn_lanes_give_identical_samples(lanes):
 @pytest.mark.parametrize("B,N,rings,nb", [(32, 64, 64, 8), (64, 128, 128, 8), (75, 100, 128, 2),
                                           (150, 64, 128, 2)])
 def test_nn_seeding_gives_identical_results(B, N, rings, nb):
-    """The seeded two-pass NN search (every 4th query row and column first)
-    changes where the exact walk starts only: the rotation grid of every
-    pair is bit-identical with seeding on and off, also for 2B not a
-    multiple of 4 (150) and the largest search grid (300 bands); the
+    """The NN search's seeds (a point per coarse cell by jump flooding) and
+    its LDS band skipping change how much of the exact walk runs, never its
+    result: the rotation grid of every pair is bit-identical in all four
+    modes (neither, seeds, mask, both), also for 2B not a power of two (150,
+    coarse factor 3) and the largest search grid (300 bands, factor 5); the
     translation grid within 1e-5 of its maximum (its voxel sums are FP32
     atomics whose order varies from run to run)."""
     from phaser_amd.capi import Context
@@ -523,16 +549,21 @@ def test_nn_seeding_gives_identical_results(B, N, rings, nb):
     cap = max(max(len(s), len(t)) for s, t, _, _ in pairs)
     got = {}
     with Context(bandwidth=B, n_voxels=N, max_points=cap) as ctx:
-        for mode in (-1, 1):
+        for mode in (-1, 1, 2, 3):
             ctx.set_tuning(nn_seed=mode)
             assert ctx.tuning()["nn_seed"] == mode
             print(f"nn_seed {mode}: B={B}, {nb} pairs", flush=True)
             got[mode] = [ctx.register(t, s, grids=True) for s, t, _, _ in pairs]
-    for a, b in zip(got[-1], got[1]):
-        assert a["rot_index"] == b["rot_index"]
-        assert np.array_equal(a["rot_corr"], b["rot_corr"])
-        assert _rel(a["trans_corr"], b["trans_corr"]) < 1e-5
-        assert _argmax_ok(b["trans_index"], a["trans_corr"])
+            samples = [ctx.sample_s2(cl) for s, t, _, _ in pairs[:2] for cl in (s, t)]
+            got[mode, "samples"] = samples
+    for mode in (1, 2, 3):
+        for a, b in zip(got[-1], got[mode]):
+            assert a["rot_index"] == b["rot_index"]
+            assert np.array_equal(a["rot_corr"], b["rot_corr"])
+            assert _rel(a["trans_corr"], b["trans_corr"]) < 1e-5
+            assert _argmax_ok(b["trans_index"], a["trans_corr"])
+        for a, b in zip(got[-1, "samples"], got[mode, "samples"]):
+            assert np.array_equal(a, b)
 
 
 def test_nn_xcd_order_gives_identical_results():
