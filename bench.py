@@ -1,16 +1,19 @@
 """Registration throughput of the MI355X correlation core.
 
 Workload: seeded synthetic scan pairs with 128 rings x 1024 azimuths (about
-80k source points), the target rigidly moved by a uniform random rotation and
-a translation in +-5 m, cropped to 70% of its azimuth ring, noise 2 cm;
+80k source points), the target cast again from the sensor moved by a uniform
+random rotation and a translation in +-5 m (--target-mode 1, the default; 0
+moves the source's points rigidly instead), cropped to 70% of its azimuth
+ring, noise 2 cm;
 B = 64 (128 x 128 S^2 grid, 128^3 SO(3) bins), 128^3 voxels over +-50 m.
 One step registers `--batch` independent pairs whose clouds are already in
 HBM, split over `--streams` contexts (one phz_register_pairs_device call per
 context, both correlation grids of every pair written to HBM), so value =
 batch * steps / time.  Latency p50/p99 comes from single-pair calls.  The
 argmax-match rate registers the 1000 seeded pairs of the committed FP64
-oracle fixture (tests/golden/parity_b64_n128_mode0.npz) and compares the
-rotation bin and translation voxel of each; the FP64 oracle itself is timed
+oracle fixture (tests/golden/parity_b64_n128_mode{0,1}.npz) and compares the
+rotation bin and translation voxel of each (the fixture of the same target
+mode; the other mode's fixture is swept too, FP32 form); the FP64 oracle itself is timed
 on --cpu-sample pairs over up to --cpu-threads host threads (one pair per
 thread) as the CPU baseline.  The roofline names the kernel of the stage
 with the largest device time: its algorithmic bytes per launch over its mean
@@ -355,6 +358,8 @@ def main():
     ap.add_argument("--serve-rates", default="500,2000",
                     help="batch mode: offered rates of the short served sweep per form ('' = off)")
     ap.add_argument("--serve-duration", type=float, default=2.0)
+    ap.add_argument("--target-mode", type=int, default=1, choices=[0, 1],
+                    help="1: target re-cast from the moved sensor; 0: source points moved")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); registrations are "
@@ -369,7 +374,7 @@ def main():
     from phaser_amd.capi import Context
 
     P = args.batch
-    pairs = [synth.pair(seed=s, **WORKLOAD) for s in range(P)]
+    pairs = [synth.pair(seed=s, target_mode=args.target_mode, **WORKLOAD) for s in range(P)]
     max_pts = max(max(len(s), len(t)) for s, t, _, _ in pairs)
     P_mean = float(np.mean([len(s) + len(t) for s, t, _, _ in pairs]) / 2)
     ctx = Context(bandwidth=B, n_voxels=N, max_points=max_pts, max_batch=P,
@@ -489,6 +494,9 @@ def main():
     copy_gbs = 2 * nb / (copy_ms * 1e-3) / 1e9
     d_a.free()
     d_b.free()
+    # MFMA reference: the dense matrix-core rate per dtype (best of three)
+    mfma_peak = {name: round(max(ctx.mfma_peak(d, 4000) for _ in range(3)), 1)
+                 for d, name in ((0, "f16"), (1, "bf16"), (2, "f32"))}
 
     # oracle: argmax match on the first pairs and the CPU baseline, pairs
     # spread over host threads (the oracle is re-entrant C; ctypes releases
@@ -529,7 +537,16 @@ def main():
     # same workload), both forms; falls back to the timed sample above
     from phaser_amd import parity
     sweep_res = {}
-    fx_path = parity.fixture_path(ROOT, 0)
+    fx_path = parity.fixture_path(ROOT, args.target_mode)
+    other_mode = None
+    if args.parity_pairs and os.path.exists(parity.fixture_path(ROOT, 1 - args.target_mode)):
+        r = parity.sweep(parity.load(parity.fixture_path(ROOT, 1 - args.target_mode)),
+                         precision=args.precision, limit=args.parity_pairs)
+        other_mode = {"target_mode": 1 - args.target_mode, "form": FORM_NAMES[args.precision],
+                      "matched": r["matched"], "pairs": r["pairs"], "rate": round(r["rate"], 5),
+                      "near_ties": r["near_ties"], "mismatches": r["mismatches"][:20]}
+        print(f"argmax sweep target_mode {1 - args.target_mode}: {r['matched']}/{r['pairs']}",
+              file=sys.stderr, flush=True)
     if args.parity_pairs and os.path.exists(fx_path):
         fx = parity.load(fx_path)
         for prec in ([args.precision, 1 - args.precision] if args.compare_forms
@@ -572,13 +589,17 @@ def main():
         "dtype": "fp32" if args.precision == 0 else "fp32 (contractions split fp16)",
         "data": "synthetic (seeded ray-cast scenes, seeds 0..batch-1)",
         "config": {"workload": "B=64 S2 grid 128x128, SO(3) 128^3, voxels 128^3 over +-50 m, "
-                               "128x1024-ray scans, target 70% overlap, noise 2 cm",
+                               "128x1024-ray scans, target 70% overlap, noise 2 cm, target "
+                               + ("re-cast from the moved sensor" if args.target_mode == 1
+                                  else "points rigidly moved"),
+                   "target_mode": args.target_mode,
                    "pairs_per_step": P, "streams": args.streams, "mean_points_per_cloud": round(P_mean),
                    "form": FORM_NAMES[args.precision], "fft_column_tiles": tuning,
                    "hip_graphs": bool(args.graphs)},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "argmax_match": argmax_match,
+        "argmax_match_other_target_mode": other_mode,
         "stage_ms_per_step": stage_ms,
         "forms": None if other is None else {
             FORM_NAMES[args.precision]: {"value": round(value, 2),
@@ -600,7 +621,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": load_traffic(kname), "bytes_per_launch": int(kbytes),
                      "avg_launch_ms": round(dom_ms, 4),
-                     "copy_kernel_gbs": round(copy_gbs, 1)},
+                     "copy_kernel_gbs": round(copy_gbs, 1),
+                     "mfma_measured_tflops": mfma_peak},
         "cpu_baseline": {"value": None if cpu_value is None else round(cpu_value, 4),
                          "unit": "registrations/s", "cores": threads, "kind": "port",
                          "latency_ms_1_thread": None if cpu_lat_ms is None else round(cpu_lat_ms, 1),

@@ -420,6 +420,13 @@ int phz_synchronize(phz_ctx* ctx);
  * reference against which kernel bandwidth and traffic counters are read. */
 int phz_copy_kernel(phz_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, int repeats,
                     float* ms);
+/* The dense matrix-core rate measured on this device: 8192 waves each run
+ * `iters` rounds of four independent accumulator chains of one MFMA
+ * instruction on register data (dtype 0: v_mfma_f32_32x32x16_f16, 1: the
+ * bf16 form, 2: v_mfma_f32_32x32x2_f32); *tflops = 2 M N K flops per
+ * instruction over the launch's duration.  The MFMA reference beside
+ * phz_copy_kernel's HBM one. */
+int phz_mfma_peak(phz_ctx* ctx, int dtype, int iters, double* tflops);
 
 #ifdef __cplusplus
 }

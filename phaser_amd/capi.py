@@ -101,6 +101,7 @@ EXPORTS = {
                                         C.POINTER(C.c_int64)]),
     "phz_pass_through_filter": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_float,
                                           C.c_float, C.c_int, C.POINTER(C.c_int64)]),
+    "phz_mfma_peak": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "phz_zscore_default_params": (None, [C.c_void_p]),
     "phz_zscore_peaks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                    C.c_int64, C.POINTER(C.c_int64)]),
@@ -445,6 +446,12 @@ class Context:
         _check(lib().phz_spherical_correlate_low_pass(self.h, _p(f), _p(h), lower, upper,
                                                       _p(corr), C.byref(am)))
         return corr, int(am.value)
+
+    def mfma_peak(self, dtype=0, iters=2000):
+        """Measured dense MFMA TFLOP/s (0 f16, 1 bf16, 2 f32)."""
+        t = C.c_double()
+        _check(lib().phz_mfma_peak(self.h, dtype, iters, C.byref(t)))
+        return t.value
 
     def voxel_grid_filter(self, pts, leaf=0.25):
         """VoxelGridCmd: the voxel centroids, ordered by voxel index."""

@@ -113,6 +113,8 @@ struct phz_ctx {
   float* d_wig = nullptr;
   int64_t* d_wig_off = nullptr;
   float* d_wig_norm = nullptr;
+  uint16_t* d_wig_hl = nullptr;      // precision 1: the table in FP16 halves
+  int64_t* d_wig_hl_off = nullptr;
   float2* d_tw_n = nullptr;
   float2* d_tw_no = nullptr;     // exp(-2 pi i e / no) (d_tw_n when unpadded)
   int* d_pad_map = nullptr;      // Bo-layout coefficient -> B-layout index or -1
@@ -290,6 +292,11 @@ size_t context_bytes(const phz_config& cf) {
   add(no * Bo * (Bo + 1) * (Bo + 2) / 6, sizeof(float));
   add(Bo * (Bo + 1) / 2 + 1, sizeof(int64_t));
   add(Bo, sizeof(float));
+  if (cf.precision == 1) {
+    std::vector<int64_t> hl_off;
+    add(wig_split_offsets((int)Bo, &hl_off), sizeof(uint16_t));
+    add(Bo * (Bo + 1) / 2, sizeof(int64_t));
+  }
   add(n, sizeof(float2));
   add(N, sizeof(float2));
   add(N, sizeof(float));
@@ -439,6 +446,14 @@ int init_ctx(phz_ctx* c) {
   TRY(upload(c, &c->d_wig, so3t.wigner));
   TRY(upload(c, &c->d_wig_off, so3t.wig_off));
   TRY(upload(c, &c->d_wig_norm, so3t.wig_norm));
+  if (cf.precision == 1) {
+    std::vector<int64_t> hl_off;
+    const int64_t hl_halves = wig_split_offsets(c->Bo, &hl_off);
+    TRY(upload(c, &c->d_wig_hl_off, hl_off));
+    TRY(dalloc(c, &c->d_wig_hl, (size_t)hl_halves));
+    launch_wig_split(c->Bo, c->d_wig, c->d_wig_off, c->d_wig_hl_off, c->d_wig_hl, c->stream);
+    PHZ_HIP(hipGetLastError());
+  }
   {
     std::vector<float2> t(c->n), u(c->N);
     std::memcpy(t.data(), c->s2.tw_n.data(), sizeof(float2) * c->n);
@@ -657,7 +672,8 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     StageScope t(c, kSo3Synth, s);
     launch_so3_synth(Bo, c->d_fused_o, shared ? 0 : BBo, c->d_fused_o + nt * BBo, BBo, np,
-                     c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S, c->cfg.precision, s);
+                     c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S, c->cfg.precision, s,
+                     c->d_wig_hl, c->d_wig_hl_off);
   }
   {
     StageScope t(c, kSo3Alpha, s);
@@ -1180,7 +1196,7 @@ static int so3_from_fused(phz_ctx* c, const float2* d_f, const float2* d_h, floa
                           float* synthesis, uint32_t* argmax, hipStream_t s) {
   const int B = c->B, n = c->n;
   launch_so3_synth(B, d_f, 0, d_h, 0, 1, c->d_wig, c->d_wig_off, c->d_wig_norm, c->d_S,
-                   c->cfg.precision, s);
+                   c->cfg.precision, s, c->d_wig_hl, c->d_wig_hl_off);
   launch_so3_fft_alpha(B, 1, c->d_S, c->plan_n, c->d_tw_n, c->d_U, s, c->so3_reg);
   launch_so3_fft_gamma(B, 1, c->d_U, c->plan_n, c->d_tw_n, corr ? c->d_corr_rot : nullptr,
                        (int64_t)n * n * n, c->d_partials, c->d_keys, s, c->so3_reg);
@@ -1831,6 +1847,31 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   c->tuning.ms_fused = best.fused_reg > 0 ? fused_ms[1] : fused_ms[0];
   c->tuning.ms_so3 = best.so3_reg > 0 ? so3_ms[1] : so3_ms[0];
   if (out) *out = c->tuning;
+  return 0;
+}
+
+int phz_mfma_peak(phz_ctx* c, int dtype, int iters, double* tflops) {
+  PHZ_CHECK(c && tflops, "null argument");
+  PHZ_CHECK(dtype >= 0 && dtype <= 2, "dtype must be 0 (f16), 1 (bf16) or 2 (f32)");
+  PHZ_CHECK(iters >= 1 && iters <= (1 << 20), "iters must be in [1, 2^20]");
+  constexpr int kBlocks = 2048;  // 8192 waves: 8 per SIMD of the 256 CUs
+  CallBuffers buf;
+  float* out;
+  PHZ_HIP(buf.get(&out, (size_t)kBlocks * 256));
+  hipEvent_t a, b;
+  PHZ_HIP(hipEventCreate(&a));
+  PHZ_HIP(hipEventCreate(&b));
+  launch_mfma_peak(dtype, iters, kBlocks, out, c->stream);  // warm-up
+  PHZ_HIP(hipEventRecord(a, c->stream));
+  const double flops = launch_mfma_peak(dtype, iters, kBlocks, out, c->stream);
+  PHZ_HIP(hipEventRecord(b, c->stream));
+  PHZ_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  PHZ_HIP(hipEventElapsedTime(&ms, a, b));
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  PHZ_HIP(hipGetLastError());
+  *tflops = flops / (ms * 1e-3) / 1e12;
   return 0;
 }
 

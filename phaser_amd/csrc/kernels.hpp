@@ -108,13 +108,21 @@ void launch_reduce_keys(const unsigned long long* partials, int count, int npair
 // ------------------------------------------------------------- SO(3)
 // S buffer: [m2 in 0..B-1][m1+B-1 in 0..2B-2][k in 0..n-1] complex per pair.
 // precision 0: FP32 Wigner-d GEMV per pair; 1: split-FP16 MFMA over the batch
+// (wig_hl, hl_off: the table split into FP16 halves by launch_wig_split,
+// needed with precision 1)
 void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
                       int64_t h_stride, int npairs, const float* wigner,
                       const int64_t* wig_off_dev, const float* wig_norm, float2* S,
-                      int precision, hipStream_t s);
+                      int precision, hipStream_t s, const uint16_t* wig_hl = nullptr,
+                      const int64_t* hl_off = nullptr);
 void launch_so3_synth_mfma(int bw, const float2* f, int64_t f_stride, const float2* h,
-                           int64_t h_stride, int npairs, const float* wig, const int64_t* wig_off,
+                           int64_t h_stride, int npairs, const uint16_t* hl, const int64_t* hl_off,
                            const float* wig_norm, float2* S, hipStream_t s);
+// the Wigner table of bandwidth bw split once into FP16 hi / lo halves in the
+// synthesis GEMM's layout: offsets per block (in halves) and the total
+int64_t wig_split_offsets(int bw, std::vector<int64_t>* off);
+void launch_wig_split(int bw, const float* wig, const int64_t* wig_off, const int64_t* hl_off,
+                      uint16_t* hl, hipStream_t s);
 // FFT over m1 -> alpha: U[k][m2][alpha] per pair
 // reg: register four-step kernels (bw = 32, 64, 128; see so3_reg_ok)
 bool so3_reg_ok(int bw);
@@ -196,6 +204,9 @@ void launch_rot_pose(int bw, int npairs, const unsigned long long* keys, const f
 // voxel accumulators per cloud: sumI, sumR (float) and count (uint) of N^3
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 void launch_copy_f4(float4* dst, const float4* src, size_t n, hipStream_t s);
+// dense MFMA issue rate (dtype 0 f16, 1 bf16, 2 f32): returns the flops the
+// launch performs; out: blocks * 256 floats
+double launch_mfma_peak(int dtype, int iters, int blocks, float* out, hipStream_t s);
 // occupancy of the voxel grids, kept beside the accumulators by
 // launch_voxel_scatter (sets) and launch_voxel_clear (resets):
 // row[cloud][z*N + x] and plane[cloud][z] are 1 where some voxel of that

@@ -199,19 +199,28 @@ struct SynChunks {
   int pairs, cols;
 };
 
+// LDS of one workgroup: split A (8 rows per pair) and B (one row per beta
+// column) at the largest block's leading dimension, the combined
+// coefficients (4 cases per pair, FP32 complex) and the row scales
 static size_t syn_lds(int bw, SynChunks c) {
-  const int ld = ((bw + 15) / 16) * 16 + kKPad;
-  return (size_t)2 * (8 * c.pairs + c.cols) * ld * sizeof(_Float16) + 8 * c.pairs * sizeof(float);
+  const int kp = ((bw + 15) / 16) * 16, ld = kp + kKPad;
+  return (size_t)2 * (8 * c.pairs + c.cols) * ld * sizeof(_Float16) +
+         (size_t)4 * c.pairs * kp * sizeof(float2) + 8 * c.pairs * sizeof(float);
 }
 
+// the largest chunk whose LDS leaves room for two workgroups per CU (a
+// workgroup's phases are separated by barriers: a second one hides them)
 static SynChunks syn_chunks(int bw) {
   const int n = 2 * bw, full = ((n + 31) / 32) * 32;
   const int cols[4] = {full, 128, 64, 32};
   const int pairs[3] = {16, 8, 4};
+  SynChunks best{4, 32};
   for (int cc : cols)
     for (int p : pairs)
-      if (cc <= full && syn_lds(bw, SynChunks{p, cc}) <= kMfmaLdsMax) return SynChunks{p, cc};
-  return SynChunks{4, 32};
+      if (cc <= full && syn_lds(bw, SynChunks{p, cc}) <= kMfmaLdsMax / 3 &&
+          p * cc > best.pairs * best.cols)
+        best = SynChunks{p, cc};
+  return best;
 }
 
 // combined coefficient of case c for degree l (zero where the case is unused)
@@ -242,10 +251,37 @@ __device__ __forceinline__ float2 synth_coef(const float2* f, const float2* h, f
   return v;
 }
 
+// The Wigner table split once per context: per block (a <= b), the hi
+// halves of d~^l_{a,b}(beta_k) as rows k of ld = round16(bw - b) + kKPad
+// halves (degrees t = l - b along the row, zero past bw - b), then the lo
+// halves likewise: the layout of the GEMM's B operand in LDS.
+__global__ void k_wig_split(int bw, const float* __restrict__ wig,
+                            const int64_t* __restrict__ wig_off,
+                            const int64_t* __restrict__ hl_off, _Float16* __restrict__ hl) {
+  const int n = 2 * bw, blk = blockIdx.x;
+  int b = (int)((sqrtf(8.f * blk + 1.f) - 1.f) * 0.5f);
+  while ((b + 1) * (b + 2) / 2 <= blk) ++b;
+  while (b * (b + 1) / 2 > blk) --b;
+  const int L = bw - b, ld = round_up(L, 16) + kKPad;
+  const float* T = wig + wig_off[blk];
+  _Float16* hi = hl + hl_off[blk];
+  _Float16* lo = hi + (int64_t)n * ld;
+  for (int idx = threadIdx.x; idx < n * ld; idx += blockDim.x) {
+    const int k = idx / ld, t = idx - k * ld;
+    split_store(t < L ? T[(int64_t)t * n + k] : 0.f, &hi[idx], &lo[idx]);
+  }
+}
+
+// One workgroup per table block (a <= b), chunk of `pairs` pairs and chunk
+// of `cols` beta columns.  A rows: (pair, case, re/im) of the combined
+// coefficients over l (each computed once, kept in LDS for the row scales
+// and the split); B: the block's pre-split table rows, copied as they lie.
+// The GEMM's re/im rows sit in neighbouring accumulator registers, so each
+// lane stores whole complex values: consecutive beta in consecutive lanes.
 __global__ void __launch_bounds__(256) k_so3_synth_mfma(
     int bw, const float2* __restrict__ fc, int64_t f_stride, const float2* __restrict__ hc,
-    int64_t h_stride, int npairs, SynChunks ch, const float* __restrict__ wig,
-    const int64_t* __restrict__ wig_off, const float* __restrict__ wnorm,
+    int64_t h_stride, int npairs, SynChunks ch, const _Float16* __restrict__ hl,
+    const int64_t* __restrict__ hl_off, const float* __restrict__ wnorm,
     float2* __restrict__ S) {
   const int n = 2 * bw, w1 = 2 * bw - 1;
   const int blk = blockIdx.x, p0 = blockIdx.y * ch.pairs, k0 = blockIdx.z * ch.cols;
@@ -256,23 +292,27 @@ __global__ void __launch_bounds__(256) k_so3_synth_mfma(
   const int a = blk - b * (b + 1) / 2;
   const int L = bw - b;
   const int cs = (b == 0) ? 0 : (a == 0 ? 1 : (a == b ? 2 : 3));
-  const int Kp = round_up(L, 16), ld = round_up(bw, 16) + kKPad;
+  const int Kp = round_up(L, 16), ld = Kp + kKPad;
+  const int kpw = round_up(bw, 16), ldw = kpw + kKPad;  // the LDS carve-up's sizes
   const int Mp = round_up(np * 8, 32), Np = round_up(nk, 32);
   extern __shared__ __align__(16) unsigned char lds_raw[];
   _Float16* Ahi = (_Float16*)lds_raw;
-  _Float16* Alo = Ahi + (size_t)8 * ch.pairs * ld;
-  _Float16* Bhi = Alo + (size_t)8 * ch.pairs * ld;
-  _Float16* Blo = Bhi + (size_t)ch.cols * ld;
-  float* scale = (float*)(Blo + (size_t)ch.cols * ld);
+  _Float16* Alo = Ahi + (size_t)8 * ch.pairs * ldw;
+  _Float16* Bhi = Alo + (size_t)8 * ch.pairs * ldw;
+  _Float16* Blo = Bhi + (size_t)ch.cols * ldw;
+  float2* cf = (float2*)(Blo + (size_t)ch.cols * ldw);  // [pair*4 + case][t], stride kpw
+  float* scale = (float*)(cf + (size_t)4 * ch.pairs * kpw);
   int* scale_bits = (int*)scale;
   for (int r = threadIdx.x; r < Mp; r += blockDim.x) scale_bits[r] = 0;
   __syncthreads();
-  // 1. row maxima of the combined coefficients (non-negative floats order as ints)
+  // 1. the combined coefficients, once each, and the row maxima
+  // (non-negative floats order as ints)
   for (int idx = threadIdx.x; idx < np * 4 * L; idx += blockDim.x) {
     const int row4 = idx / L, t = idx - row4 * L;
     const int p = row4 >> 2, c = row4 & 3, l = b + t;
     const float2 v = synth_coef(fc + (p0 + p) * f_stride, hc + (p0 + p) * h_stride, wnorm[l], l,
                                 a, b, cs, c, bw);
+    cf[row4 * kpw + t] = v;
     atomicMax(&scale_bits[2 * row4], __float_as_int(fabsf(v.x)));
     atomicMax(&scale_bits[2 * row4 + 1], __float_as_int(fabsf(v.y)));
   }
@@ -280,62 +320,172 @@ __global__ void __launch_bounds__(256) k_so3_synth_mfma(
   for (int r = threadIdx.x; r < Mp; r += blockDim.x)
     scale[r] = row_scale(__int_as_float(scale_bits[r]));
   __syncthreads();
-  // 2. scaled split A; 3. the table block d~^l_{a,b}(beta_k) for this column chunk
+  // 2. scaled split A from LDS
   for (int idx = threadIdx.x; idx < Mp / 2 * Kp; idx += blockDim.x) {
     const int row4 = idx / Kp, t = idx - row4 * Kp;
-    float2 v = make_float2(0.f, 0.f);
-    if (row4 < np * 4 && t < L) {
-      const int p = row4 >> 2, c = row4 & 3, l = b + t;
-      v = synth_coef(fc + (p0 + p) * f_stride, hc + (p0 + p) * h_stride, wnorm[l], l, a, b, cs,
-                     c, bw);
-    }
+    const float2 v = (row4 < np * 4 && t < L) ? cf[row4 * kpw + t] : make_float2(0.f, 0.f);
     split_store(v.x * scale[2 * row4], &Ahi[(2 * row4) * ld + t], &Alo[(2 * row4) * ld + t]);
     split_store(v.y * scale[2 * row4 + 1], &Ahi[(2 * row4 + 1) * ld + t],
                 &Alo[(2 * row4 + 1) * ld + t]);
   }
-  const float* T = wig + wig_off[blk];
-  for (int idx = threadIdx.x; idx < Kp * Np; idx += blockDim.x) {
-    const int t = idx / Np, k = idx - t * Np;
-    const float v = (t < L && k < nk) ? T[(int64_t)t * n + k0 + k] : 0.f;
-    split_store(v, &Bhi[k * ld + t], &Blo[k * ld + t]);
+  // 3. the block's pre-split rows k0 .. k0 + Np (zero past the chunk), 16 bytes at a time
+  {
+    const _Float16* ghi = hl + hl_off[blk] + (int64_t)k0 * ld;
+    const _Float16* glo = ghi + (int64_t)n * ld;
+    const int vec = Np * ld / 8;
+    for (int v = threadIdx.x; v < vec; v += blockDim.x) {
+      const int kk = (v * 8) / ld;
+      uint4 x = make_uint4(0u, 0u, 0u, 0u), y = x;
+      if (kk < nk) {
+        x = ((const uint4*)ghi)[v];
+        y = ((const uint4*)glo)[v];
+      }
+      ((uint4*)Bhi)[v] = x;
+      ((uint4*)Blo)[v] = y;
+    }
   }
   __syncthreads();
-  // 4. GEMM; row (pair, case, re/im) and column k land in S[pair][m2][m1][beta]
-  split_gemm(Ahi, Alo, ld, Bhi, Blo, ld, Mp, Np, Kp, [&](int r, int kc, float v) {
-    if (r >= np * 8 || kc >= nk) return;
-    const int part = r & 1, c = (r >> 1) & 3, p = r >> 3, k = k0 + kc;
-    const float x = v / scale[r];
-    float* Sp = (float*)(S + (int64_t)(p0 + p) * bw * w1 * n);
-    auto put = [&](int m1, int m2, int kk, bool conj) {
-      Sp[2 * (((int64_t)m2 * w1 + (m1 + bw - 1)) * n + kk) + part] = (conj && part) ? -x : x;
-    };
-    const int kf = n - 1 - k;  // the beta-flipped cases read d(pi - beta)
-    if (c == 0) {
-      put(a, b, k, false);
-    } else if (c == 1) {
-      if (cs == 1) {
-        put(b, 0, k, false);
-        put(-b, 0, k, true);
-      } else if (cs == 3) {
-        put(b, a, k, false);
-      }
-    } else if (c == 2) {
-      if (cs == 2) put(-a, a, kf, false);
-      else if (cs == 3) put(-a, b, kf, true);
-    } else if (cs == 3) {
-      put(-b, a, kf, true);
+  // 4. GEMM; rows (pair, case, re/im), column k -> S[pair][m2][m1][beta]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  const int tn = Np / 32, tiles = (Mp / 32) * tn;
+  const int r32 = lane & 31, kh = 8 * (lane >> 5);
+  for (int tile = wave; tile < tiles; tile += nwaves) {
+    const int i0 = (tile / tn) * 32, j0 = (tile % tn) * 32;
+    floatx16 acc = {};
+    const _Float16* ah = Ahi + (i0 + r32) * ld + kh;
+    const _Float16* al = Alo + (i0 + r32) * ld + kh;
+    const _Float16* bh = Bhi + (j0 + r32) * ld + kh;
+    const _Float16* bl = Blo + (j0 + r32) * ld + kh;
+    for (int kq = 0; kq < Kp; kq += 16) {
+      const half8 a_h = *(const half8*)(ah + kq);
+      const half8 a_l = *(const half8*)(al + kq);
+      const half8 b_h = *(const half8*)(bh + kq);
+      const half8 b_l = *(const half8*)(bl + kq);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_l, b_h, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_h, b_l, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(a_h, b_h, acc, 0, 0, 0);
     }
-  });
+    const int kc = j0 + r32;
+    if (kc >= nk) continue;
+    const int k = k0 + kc, kf = n - 1 - k;  // the beta-flipped cases read d(pi - beta)
+#pragma unroll
+    for (int r = 0; r < 16; r += 2) {
+      // rows re = i0 + 8 (r / 4) + 4 (lane / 32) + r % 4 (even) and im = re + 1
+      const int row = i0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      if (row >= np * 8) continue;
+      const int c = (row >> 1) & 3, p = row >> 3;
+      const float2 z = make_float2(acc[r] / scale[row], acc[r + 1] / scale[row + 1]);
+      float2* Sp = S + (int64_t)(p0 + p) * bw * w1 * n;
+      auto put = [&](int m1, int m2, int kk, bool conj) {
+        Sp[((int64_t)m2 * w1 + (m1 + bw - 1)) * n + kk] = conj ? make_float2(z.x, -z.y) : z;
+      };
+      if (c == 0) {
+        put(a, b, k, false);
+      } else if (c == 1) {
+        if (cs == 1) {
+          put(b, 0, k, false);
+          put(-b, 0, k, true);
+        } else if (cs == 3) {
+          put(b, a, k, false);
+        }
+      } else if (c == 2) {
+        if (cs == 2) put(-a, a, kf, false);
+        else if (cs == 3) put(-a, b, kf, true);
+      } else if (cs == 3) {
+        put(-b, a, kf, true);
+      }
+    }
+  }
+}
+
+int64_t wig_split_offsets(int bw, std::vector<int64_t>* off) {
+  const int n = 2 * bw, nblk = bw * (bw + 1) / 2;
+  off->assign(nblk, 0);
+  int64_t total = 0;
+  for (int b = 0; b < bw; ++b)
+    for (int a = 0; a <= b; ++a) {
+      const int ld = ((bw - b + 15) / 16) * 16 + kKPad;
+      (*off)[b * (b + 1) / 2 + a] = total;
+      total += (int64_t)2 * n * ld;
+    }
+  return total;
+}
+
+void launch_wig_split(int bw, const float* wig, const int64_t* wig_off, const int64_t* hl_off,
+                      uint16_t* hl, hipStream_t s) {
+  hipLaunchKernelGGL(k_wig_split, dim3(bw * (bw + 1) / 2), dim3(256), 0, s, bw, wig, wig_off,
+                     hl_off, (_Float16*)hl);
+}
+
+// ------------------------------------------------------------ MFMA peak
+// Dense MFMA issue rate: every wave runs `iters` rounds of four independent
+// accumulator chains of one matrix instruction on register data (no memory
+// traffic) and stores one value per lane so that nothing is dead.
+// DT 0: v_mfma_f32_32x32x16_f16, 1: v_mfma_f32_32x32x16_bf16,
+// 2: v_mfma_f32_32x32x2_f32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int DT>
+__global__ void __launch_bounds__(256) k_mfma_peak(int iters, float* __restrict__ out) {
+  floatx16 c0 = {}, c1 = {}, c2 = {}, c3 = {};
+  const float x = 1e-3f * (float)(threadIdx.x & 63);
+  if constexpr (DT == 0) {
+    half8 a, b;
+    for (int i = 0; i < 8; ++i) {
+      a[i] = (_Float16)(x + i);
+      b[i] = (_Float16)(x - i);
+    }
+    for (int it = 0; it < iters; ++it) {
+      c0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c1, 0, 0, 0);
+      c2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c2, 0, 0, 0);
+      c3 = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c3, 0, 0, 0);
+    }
+  } else if constexpr (DT == 1) {
+    bf16x8 a, b;
+    for (int i = 0; i < 8; ++i) {
+      a[i] = (__bf16)(x + i);
+      b[i] = (__bf16)(x - i);
+    }
+    for (int it = 0; it < iters; ++it) {
+      c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c1, 0, 0, 0);
+      c2 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c2, 0, 0, 0);
+      c3 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c3, 0, 0, 0);
+    }
+  } else {
+    const float a = x + 1.f, b = x - 1.f;
+    for (int it = 0; it < iters; ++it) {
+      c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c0, 0, 0, 0);
+      c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c1, 0, 0, 0);
+      c2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c2, 0, 0, 0);
+      c3 = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c3, 0, 0, 0);
+    }
+  }
+  float s = 0.f;
+  for (int r = 0; r < 16; ++r) s += c0[r] + c1[r] + c2[r] + c3[r];
+  out[blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+double launch_mfma_peak(int dtype, int iters, int blocks, float* out, hipStream_t s) {
+  switch (dtype) {
+    case 1: hipLaunchKernelGGL(k_mfma_peak<1>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+    case 2: hipLaunchKernelGGL(k_mfma_peak<2>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+    default: hipLaunchKernelGGL(k_mfma_peak<0>, dim3(blocks), dim3(256), 0, s, iters, out); break;
+  }
+  // 2 M N K flops per instruction, four per round, four waves per block
+  const double per = dtype == 2 ? 2.0 * 32 * 32 * 2 : 2.0 * 32 * 32 * 16;
+  return per * 4.0 * iters * 4.0 * blocks;
 }
 
 void launch_so3_synth_mfma(int bw, const float2* f, int64_t f_stride, const float2* h,
-                           int64_t h_stride, int npairs, const float* wig, const int64_t* wig_off,
+                           int64_t h_stride, int npairs, const uint16_t* hl, const int64_t* hl_off,
                            const float* wig_norm, float2* S, hipStream_t s) {
   const int n = 2 * bw;
   const SynChunks ch = syn_chunks(bw);
   dim3 g(bw * (bw + 1) / 2, (npairs + ch.pairs - 1) / ch.pairs, (n + ch.cols - 1) / ch.cols);
   hipLaunchKernelGGL(k_so3_synth_mfma, g, dim3(256), syn_lds(bw, ch), s, bw, f, f_stride, h,
-                     h_stride, npairs, ch, wig, wig_off, wig_norm, S);
+                     h_stride, npairs, ch, (const _Float16*)hl, hl_off, wig_norm, S);
 }
 
 }  // namespace phz

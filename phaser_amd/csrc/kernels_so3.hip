@@ -458,9 +458,10 @@ void launch_pad_coef(const float2* in, int64_t in_stride, const int* map, int64_
 void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const float2* h_coef,
                       int64_t h_stride, int npairs, const float* wigner,
                       const int64_t* wig_off_dev, const float* wig_norm, float2* S,
-                      int precision, hipStream_t s) {
+                      int precision, hipStream_t s, const uint16_t* wig_hl,
+                      const int64_t* hl_off) {
   if (precision == 1) {
-    launch_so3_synth_mfma(bw, f_coef, f_stride, h_coef, h_stride, npairs, wigner, wig_off_dev,
+    launch_so3_synth_mfma(bw, f_coef, f_stride, h_coef, h_stride, npairs, wig_hl, hl_off,
                           wig_norm, S, s);
     return;
   }
@@ -468,7 +469,8 @@ void launch_so3_synth(int bw, const float2* f_coef, int64_t f_stride, const floa
   const int nblk = bw * (bw + 1) / 2;
   const int th = ((n + 63) / 64) * 64;
   // table columns shared by four pairs per workgroup once there are enough
-  // pairs to keep the device busy
+  // pairs to keep the device busy (eight pairs per workgroup measured 0.35
+  // against 0.22 ms at 64 pairs: their accumulators cost occupancy)
   if (npairs >= 8) {
     constexpr int NP = 4;
     hipLaunchKernelGGL(k_so3_synth<NP>, dim3(nblk, (npairs + NP - 1) / NP), dim3(th),
