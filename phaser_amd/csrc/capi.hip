@@ -314,7 +314,6 @@ size_t context_bytes(const phz_config& cf) {
   add(M * nq, sizeof(int4));
   add(M * nq, sizeof(int4));
   add(M * cap, sizeof(float4));
-  add(M * cap, sizeof(float2));
   add(M * cap, sizeof(float4));
   add(M * cap, sizeof(float2));
   add(M * cap, sizeof(int));
@@ -499,7 +498,6 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->cells.next_right, (size_t)M * nq));
   TRY(dalloc(c, &c->cells.next_left, (size_t)M * nq));
   TRY(dalloc(c, &c->cells.sorted, (size_t)M * cap));
-  TRY(dalloc(c, &c->cells.sorted_val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));

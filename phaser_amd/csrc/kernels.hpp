@@ -23,9 +23,8 @@ struct CellBuffers {
                      // its point range {sector, start, end, 0} (-1: empty band)
   int4* next_left;   // nclouds * n*n: the same to the left
   float4* sorted;    // nclouds * cap: projected x,y,z + original index bits
-  float2* sorted_val;  // nclouds * cap: (range, intensity)
   float4* proj;      // nclouds * cap
-  float2* val;       // nclouds * cap
+  float2* val;       // nclouds * cap: (range, intensity) by original index
   int* cell;         // nclouds * cap
   int* seed_pos;     // nclouds * kSeedCells: per coarse cell the sorted position
                      // of a point near its centre (-1: none)

@@ -146,10 +146,9 @@ __global__ void __launch_bounds__(1024) k_cell_scan(int nb, int ns, int* __restr
 }
 
 __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int ncell,
-                               const float4* __restrict__ proj, const float2* __restrict__ val,
+                               const float4* __restrict__ proj,
                                const int* __restrict__ cell, int* __restrict__ cursor,
-                               const int* __restrict__ cell_start, float4* __restrict__ sorted,
-                               float2* __restrict__ sorted_val) {
+                               const int* __restrict__ cell_start, float4* __restrict__ sorted) {
   const int c = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
@@ -164,7 +163,6 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
   const int pos = base + (lane - run.head_lane);
   const int dst = cell_start[(int64_t)c * (ncell + 1) + cid] + pos;
   sorted[c * cap + dst] = proj[c * cap + i];
-  sorted_val[c * cap + dst] = val[c * cap + i];
 }
 
 // The S^2 search grid has at most kMaxSectors bands and sectors (n = 2B,
@@ -233,26 +231,23 @@ __device__ __forceinline__ int quad_xor(int v) {
   return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xF, 0xF, false);
 }
 
-// the LANES lanes of a query agree on the smallest (distance, index)
+// the LANES lanes of a query agree on the smallest key (squared distance
+// bits above, original point index below: non-negative floats order as
+// their bits, so the key orders by distance, then by the lower index)
 template <int LANES>
-__device__ __forceinline__ void agree_best(float& best, int& bidx, int& bpos) {
-  auto step = [&](float ob, int oi, int op) {
-    if (ob < best || (ob == best && oi < bidx)) {
-      best = ob;
-      bidx = oi;
-      bpos = op;
-    }
+__device__ __forceinline__ void agree_best(unsigned long long& key) {
+  auto step = [&](unsigned lo, unsigned hi) {
+    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    key = o < key ? o : key;
   };
   if constexpr (LANES >= 16)
-    step(__shfl_xor(best, 8, 64), __shfl_xor(bidx, 8, 64), __shfl_xor(bpos, 8, 64));
+    step(__shfl_xor((unsigned)key, 8, 64), __shfl_xor((unsigned)(key >> 32), 8, 64));
   if constexpr (LANES >= 8)
-    step(__shfl_xor(best, 4, 64), __shfl_xor(bidx, 4, 64), __shfl_xor(bpos, 4, 64));
+    step(__shfl_xor((unsigned)key, 4, 64), __shfl_xor((unsigned)(key >> 32), 4, 64));
   if constexpr (LANES >= 4)
-    step(__int_as_float(quad_xor<0x4E>(__float_as_int(best))), quad_xor<0x4E>(bidx),
-         quad_xor<0x4E>(bpos));
+    step((unsigned)quad_xor<0x4E>((int)(unsigned)key), (unsigned)quad_xor<0x4E>((int)(key >> 32)));
   if constexpr (LANES >= 2)
-    step(__int_as_float(quad_xor<0xB1>(__float_as_int(best))), quad_xor<0xB1>(bidx),
-         quad_xor<0xB1>(bpos));
+    step((unsigned)quad_xor<0xB1>((int)(unsigned)key), (unsigned)quad_xor<0xB1>((int)(key >> 32)));
 }
 
 __device__ __forceinline__ float l2_simple(float4 q, float4 p) {
@@ -399,7 +394,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      int nb, int ns, float max_sq, const int* __restrict__ cell_start,
                      const int* __restrict__ band_start, const int4* __restrict__ next_right,
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
-                     const float2* __restrict__ sorted_val, int64_t cap,
+                     const float2* __restrict__ vals, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
                      float* __restrict__ xgrids, int xcd_blocks, int blend, NnSeeds seeds) {
   constexpr int kBatch = 4;  // point loads in flight per lane
@@ -425,10 +420,34 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // kMaxSectors) and the bands' occupancy words, once per workgroup
   __shared__ float band_sin_min[kMaxSectors];
   __shared__ int bs[kMaxSectors + 1];
+  // the azimuth term of the cell bound by sector distance u = 0 .. ns / 2:
+  // 2 sin^2(dphi / 2), dphi = the azimuth distance from a sector's centre to
+  // the nearest edge of the sector u away (queries sit on sector centres)
+  __shared__ float wsec[kMaxSectors / 2 + 1];
+  // the colatitude term of the cell bound, 4 sin^2(dth / 2), per band for
+  // each query row of this workgroup (rows r0 .. r0 + nrows - 1; at most
+  // blockDim / n + 2 rows of nb bands, n >= 16)
+  __shared__ float atab[2 * kMaxSectors + 256];
+  const int qpb = blockDim.x / LANES, q0 = qb * qpb;
+  const int r0 = q0 / n, nrows = min(q0 + qpb, nq) > q0 ? (min(q0 + qpb, nq) - 1) / n - r0 + 1 : 0;
   extern __shared__ unsigned long long bmask[];  // nb * nw words with the mask, else none
   for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
     const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
     band_sin_min[bb] = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
+  }
+  for (int i = threadIdx.x; i < nrows * nb; i += blockDim.x) {
+    const int rr = i / nb, bb = i - rr * nb;
+    const float th = theta[r0 + rr];
+    const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
+    const float dth = fmaxf(0.f, fmaxf(lo - th, th - hi));
+    const float s1 = __sinf(0.5f * dth);
+    atab[i] = 4.f * s1 * s1;
+  }
+  for (int u = threadIdx.x; u <= ns / 2; u += blockDim.x) {
+    const float dsec_ = 2.f * kPi / ns;
+    const float dphi = fmaxf(u * dsec_ - 0.5f * dsec_ - 1e-5f, 0.f);
+    const float s2 = __sinf(0.5f * dphi);
+    wsec[u] = 2.f * s2 * s2;
   }
   for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x)
     bs[bb] = band_start[(int64_t)c * (nb + 1) + bb];
@@ -452,8 +471,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   int qj = (int)floorf(phq / dsec + 0.5f);
   qj = qj >= ns ? qj - ns : qj;
   const float eps = 1e-5f, slack = 1e-5f;
-  float best = INFINITY;
-  int bidx = 0x7fffffff, bpos = -1;
+  // best so far: (squared distance bits, original index); ~0 = none
+  unsigned long long bkey = ~0ull;
   bool up = true, down = true;
   // pruning threshold: squared (distance + slack) plus kBoundErr, the
   // largest error of the bounds below as computed with the hardware sine
@@ -468,10 +487,9 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const int pos = seeds.pos[(int64_t)c * kSeedCells + (qi / seeds.F) * seeds.cns + qj / seeds.F];
     if (pos >= 0) {
       const float4 pv = sp[pos];
-      best = l2_simple(qv, pv);
-      bidx = __float_as_int(pv.w);
-      bpos = pos;
-      const float t = sqrtf(best) + slack;
+      const float d = l2_simple(qv, pv);
+      bkey = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)__float_as_int(pv.w);
+      const float t = sqrtf(d) + slack;
       thr = t * t + kBoundErr;
     }
   }
@@ -489,16 +507,14 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       for (int u = 0; u < kBatch; ++u) {
         const int p = p0 + u * LANES;
         const float d = l2_simple(qv, pv[u]);
-        const int oi = __float_as_int(pv[u].w);
-        const bool take = p < e && (d < best || (d == best && oi < bidx));
-        best = take ? d : best;
-        bidx = take ? oi : bidx;
-        bpos = take ? p : bpos;
+        const unsigned long long k =
+            ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)__float_as_int(pv[u].w);
+        bkey = (p < e && k < bkey) ? k : bkey;
       }
     }
-    agree_best<LANES>(best, bidx, bpos);
-    if (bpos >= 0) {
-      const float t = sqrtf(best) + slack;
+    agree_best<LANES>(bkey);
+    if (bkey != ~0ull) {
+      const float t = sqrtf(__uint_as_float((unsigned)(bkey >> 32))) + slack;
       thr = t * t + kBoundErr;
     }
   };
@@ -522,10 +538,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       }
     }
     if (bs[b + 1] == bs[b]) continue;  // empty band (LDS)
-    const float lo = b * dband - eps, hi = (b + 1) * dband + eps;
-    const float dth = fmaxf(0.f, fmaxf(lo - thq, thq - hi));
-    const float s1 = __sinf(0.5f * dth);
-    const float A = 4.f * s1 * s1;
+    const float A = atab[(qrow - r0) * nb + b];
     if (A > thr) {  // every farther band in this direction is farther still
       if (t == 0)
         up = down = false;
@@ -539,13 +552,12 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const float Bc = 2.f * sq * spm;
     const int cb = b * ns;
     auto bound = [&](int sec) {
-      // azimuth distance from the query to the sector's interval, in
-      // [0, pi], where 1 - cos is increasing: the bound grows with u
-      float d = fabsf(phq - sec * dsec);
-      if (d > kPi) d = 2.f * kPi - d;
-      const float dphi = fmaxf(d - 0.5f * dsec - eps, 0.f);
-      const float s2 = __sinf(0.5f * dphi);
-      return A + Bc * 2.f * s2 * s2;
+      // cyclic sector distance from the query's own sector (the query's
+      // azimuth is that sector's centre), in [0, ns / 2], where the term
+      // grows: the bound grows with it
+      int u = sec > qj ? sec - qj : qj - sec;
+      u = u > ns - u ? ns - u : u;
+      return A + Bc * wsec[u];
     };
     if (seeds.use_mask) {
       // own cell empty and the nearest occupied sector on each side (right
@@ -601,8 +613,12 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   }
   if (sub == 0) {
     float r = 0.f, it = 0.f;
-    if (bpos >= 0 && best <= max_sq) {
-      const float2 v = sorted_val[c * cap + bpos];
+    // the nearest point's values by its original index (the projection's
+    // per-point array), where the squared distance is accepted
+    const bool hit = bkey != ~0ull && __uint_as_float((unsigned)(bkey >> 32)) <= max_sq;
+    const int oi = (int)(unsigned)bkey;
+    if (hit) {
+      const float2 v = vals[c * cap + oi];
       r = v.x;
       it = v.y;
     }
@@ -614,7 +630,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     }
     if (extra) {  // the nearest point's (reflectivity, ambient), point-cloud.cc:152-155
       float2 e = make_float2(0.f, 0.f);
-      if (bpos >= 0 && best <= max_sq) e = extra[c * cap + __float_as_int(sp[bpos].w)];
+      if (hit) e = extra[c * cap + oi];
       xgrids[(int64_t)c * 2 * nq + q] = e.x;
       xgrids[(int64_t)c * 2 * nq + nq + q] = e.y;
     }
@@ -746,8 +762,8 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   else  // up to kMaxSectors^2 = 100 * 1024 cells
     hipLaunchKernelGGL(k_cell_scan<100>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
                        cb.cell_count, cb.cell_start, cb.band_start);
-  hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj, cb.val,
-                     cb.cell, cb.cell_count, cb.cell_start, cb.sorted, cb.sorted_val);
+  hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj,
+                     cb.cell, cb.cell_count, cb.cell_start, cb.sorted);
   hipLaunchKernelGGL(k_jump_tables, dim3(cb.nb, nclouds), dim3(64), 0, s, cb.nb, cb.ns,
                      cb.cell_start, cb.next_right, cb.next_left, cb.band_mask);
 }
@@ -764,7 +780,7 @@ static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, con
   const size_t lds = seeds.use_mask ? (size_t)cb.nb * ((cb.ns + 63) / 64) * 8 : 0;
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), lds, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.sorted_val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
+                     cb.val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
                      seeds);
 }
 
