@@ -313,7 +313,7 @@ size_t context_bytes(const phz_config& cf) {
   add(M * (n + 1), sizeof(int));
   add(M * nq, sizeof(int4));
   add(M * nq, sizeof(int4));
-  add(M * cap, sizeof(float4));
+  add(M * cap + kNnScanPad, sizeof(float4));
   add(M * cap, sizeof(float4));
   add(M * cap, sizeof(float2));
   add(M * cap, sizeof(int));
@@ -497,7 +497,7 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->cells.band_start, (size_t)M * (c->n + 1)));
   TRY(dalloc(c, &c->cells.next_right, (size_t)M * nq));
   TRY(dalloc(c, &c->cells.next_left, (size_t)M * nq));
-  TRY(dalloc(c, &c->cells.sorted, (size_t)M * cap));
+  TRY(dalloc(c, &c->cells.sorted, (size_t)M * cap + kNnScanPad));
   TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));

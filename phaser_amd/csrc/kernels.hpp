@@ -22,7 +22,7 @@ struct CellBuffers {
   int4* next_right;  // nclouds * n*n: nearest non-empty sector to the right and
                      // its point range {sector, start, end, 0} (-1: empty band)
   int4* next_left;   // nclouds * n*n: the same to the left
-  float4* sorted;    // nclouds * cap: projected x,y,z + original index bits
+  float4* sorted;    // nclouds * cap + kNnScanPad: projected x,y,z + original index bits
   float4* proj;      // nclouds * cap
   float2* val;       // nclouds * cap: (range, intensity) by original index
   int* cell;         // nclouds * cap
@@ -30,6 +30,10 @@ struct CellBuffers {
                      // of a point near its centre (-1: none)
   unsigned long long* band_mask;  // nclouds * nb * ceil(ns / 64): sector occupancy
 };
+
+// points of padding past the last cloud's sorted points: the search's
+// branch-free batches read up to 4 x 16 points past a cell's range
+constexpr int64_t kNnScanPad = 64;
 
 // The S^2 search's coarse seed grid: F x F cells per coarse cell, F the
 // smallest with ceil(n / F)^2 <= kSeedCells.

@@ -496,20 +496,23 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // x in [0, 2 ns) reduced mod ns without an integer division
   auto wrap = [&](int x) { return x >= ns ? x - ns : x; };
   auto scan_range = [&](int s0, int e) {
-    // branch-free batches: indices past the range load its last point
-    // (in bounds, the range being non-empty whenever the loop runs) and
-    // are masked out of the comparison
+    // branch-free batches: indices past the range still load (the sorted
+    // array carries kNnScanPad points of padding past its last cloud, so
+    // every load stays inside the allocation) and are masked out of the
+    // comparison
     for (int p0 = s0 + sub; p0 < e; p0 += kBatch * LANES) {
+      const float4* base = sp + p0;
       float4 pv[kBatch];
 #pragma unroll
-      for (int u = 0; u < kBatch; ++u) pv[u] = sp[min(p0 + u * LANES, e - 1)];
+      for (int u = 0; u < kBatch; ++u) pv[u] = base[u * LANES];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int p = p0 + u * LANES;
         const float d = l2_simple(qv, pv[u]);
         const unsigned long long k =
             ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)__float_as_int(pv[u].w);
-        bkey = (p < e && k < bkey) ? k : bkey;
+        const bool take = (p < e) & (k < bkey);
+        bkey = take ? k : bkey;
       }
     }
     agree_best<LANES>(bkey);
