@@ -522,35 +522,21 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     }
   };
   for (int t = 0; t < 2 * nb && (up || down); ++t) {
-    int b;
-    if (t == 0) {
-      b = qi;
-    } else if (t & 1) {
-      if (!up) continue;
-      b = qi + (t + 1) / 2;
-      if (b >= nb) {
-        up = false;
-        continue;
-      }
-    } else {
-      if (!down) continue;
-      b = qi - t / 2;
-      if (b < 0) {
-        down = false;
-        continue;
-      }
-    }
-    if (bs[b + 1] == bs[b]) continue;  // empty band (LDS)
+    // band of step t and whether this query still walks in its direction;
+    // the direction's flag clears past the grid or once the band's bound
+    // exceeds the threshold (every farther band that way is farther still).
+    // Decided with selects, then one branch around the band's work.
+    const bool odd = (t & 1) != 0;
+    const int h = (t + 1) >> 1;
+    const int braw = t == 0 ? qi : (odd ? qi + h : qi - h);
+    const bool inr = braw >= 0 && braw < nb;
+    const bool dir = t == 0 || (odd ? up : down);
+    const int b = inr ? braw : qi;
     const float A = atab[(qrow - r0) * nb + b];
-    if (A > thr) {  // every farther band in this direction is farther still
-      if (t == 0)
-        up = down = false;
-      else if (t & 1)
-        up = false;
-      else
-        down = false;
-      continue;
-    }
+    const bool far = !inr || A > thr;
+    up = (far && dir && (t == 0 || odd)) ? false : up;
+    down = (far && dir && (t == 0 || !odd)) ? false : down;
+    if (!(dir && !far && bs[b + 1] != bs[b])) continue;  // empty band (LDS)
     const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
     const int cb = b * ns;
@@ -585,33 +571,27 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     // right: sector distances 1..ns/2; left: 1..(ns-1)/2 (each sector once)
     int ur = 0, ul = 0;
     bool rdone = false, ldone = false;
+    // the steps are predicated rather than branched: a walk that is done
+    // keeps loading its (in-range) next entry and scans an empty range, so
+    // the lanes of a wave share one instruction stream (bound() is safe for
+    // any sector in [-1, ns))
     while (!rdone || !ldone) {
-      int4 cr = jr, cl = jl;
-      bool scan_r = false, scan_l = false;
+      const int4 cr = jr, cl = jl;
       const float th = thr;
-      if (!rdone) {
-        const int u = wrap(jr.x - qj + ns);
-        if (jr.x < 0 || u <= ur || u > ns / 2 || bound(jr.x) > th) {
-          rdone = true;
-        } else {
-          scan_r = true;
-          ur = u;
-        }
-      }
-      if (!ldone) {
-        const int u = wrap(qj - jl.x + ns);  // jl.x >= 0 when used
-        if (jl.x < 0 || u <= ul || u > (ns - 1) / 2 || bound(jl.x) > th) {
-          ldone = true;
-        } else {
-          scan_l = true;
-          ul = u;
-        }
-      }
+      const int uR = wrap(jr.x - qj + ns);
+      const int uL = wrap(qj - jl.x + ns);
+      const bool scan_r = !rdone && jr.x >= 0 && uR > ur && uR <= ns / 2 && !(bound(jr.x) > th);
+      const bool scan_l =
+          !ldone && jl.x >= 0 && uL > ul && uL <= (ns - 1) / 2 && !(bound(jl.x) > th);
+      rdone = !scan_r;
+      ldone = !scan_l;
+      ur = scan_r ? uR : ur;
+      ul = scan_l ? uL : ul;
       // next entries first, so they are in flight during the scans
-      if (!rdone) jr = nr[cb + wrap(qj + ur + 1)];
-      if (!ldone) jl = nl[cb + wrap(qj - ul - 1 + ns)];
-      if (scan_r) scan_range(cr.y, cr.z);
-      if (scan_l) scan_range(cl.y, cl.z);
+      jr = nr[cb + wrap(qj + ur + 1)];
+      jl = nl[cb + wrap(qj - ul - 1 + ns)];
+      scan_range(scan_r ? cr.y : 0, scan_r ? cr.z : 0);
+      scan_range(scan_l ? cl.y : 0, scan_l ? cl.z : 0);
     }
   }
   if (sub == 0) {
