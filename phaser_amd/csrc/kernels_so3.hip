@@ -260,13 +260,18 @@ __global__ void __launch_bounds__(256) k_so3_fft_alpha_reg(int bw, const float2*
   const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
   const float2* Sp = S + ((int64_t)pair * bw + m2) * w1 * L + k0 + c;
   const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  // every row loads unconditionally (the Nyquist row m1 = -bw reads row
+  // 0 and is zeroed afterwards), so the R1 loads are in flight together
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
-    const int M1 = n2 + R2 * n1;  // m1 mod n; the Nyquist row m1 = -bw is zero
+    const int M1 = n2 + R2 * n1;  // m1 mod n
     const int m1 = M1 < bw ? M1 : M1 - L;
-    v[n1] = (M1 == bw) ? make_float2(0.f, 0.f) : Sp[(int64_t)(m1 + bw - 1) * L];
+    v[n1] = Sp[(int64_t)(M1 == bw ? 0 : m1 + bw - 1) * L];
   }
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1)
+    if (n2 + R2 * n1 == bw) v[n1] = make_float2(0.f, 0.f);
   float2 out[R1];
   fft_four_step_reg<-1, L>(v, n2, buf + c * SS, w, out);
   __syncthreads();
@@ -297,26 +302,25 @@ __global__ void __launch_bounds__(256) k_so3_fft_gamma_reg(int bw, const float2*
   const int aa = a0 + 2 * c;  // alpha values aa and aa + 1
   const float2* Up = U + ((int64_t)pair * L + k) * bw * L + aa;
   const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  // every row loads unconditionally (the Nyquist row m = bw reads row
+  // bw - 1 and is zeroed afterwards), so the R1 loads are in flight together
+  float4 ab[R1];
+#pragma unroll
+  for (int n1 = 0; n1 < R1; ++n1) {
+    const int m = n2 + R2 * n1;
+    const int mm = m < bw ? m : L - m;
+    ab[n1] = *(const float4*)(Up + (int64_t)(mm < bw ? mm : bw - 1) * L);
+  }
   float2 v[R1];
 #pragma unroll
   for (int n1 = 0; n1 < R1; ++n1) {
     const int m = n2 + R2 * n1;
-    float2 va, vb;
-    if (m == bw) {
-      va = vb = make_float2(0.f, 0.f);
-    } else {
-      const int mm = m < bw ? m : L - m;
-      const float4 ab = *(const float4*)(Up + (int64_t)mm * L);
-      va = make_float2(ab.x, ab.y);
-      vb = make_float2(ab.z, ab.w);
-      if (m == 0) {
-        va.y = 0.f;
-        vb.y = 0.f;
-      } else if (m > bw) {
-        va = cconj(va);
-        vb = cconj(vb);
-      }
-    }
+    float2 va = make_float2(ab[n1].x, ab[n1].y), vb = make_float2(ab[n1].z, ab[n1].w);
+    // m = 0: real parts only; m > bw: the conjugate half; m = bw: zero
+    const float s = m > bw ? -1.f : (m == 0 ? 0.f : 1.f);
+    va.y *= s;
+    vb.y *= s;
+    if (m == bw) va = vb = make_float2(0.f, 0.f);
     v[n1] = make_float2(va.x - vb.y, va.y + vb.x);  // va + i vb
   }
   float2 out[R1];
