@@ -33,8 +33,10 @@ __device__ __forceinline__ void project_point(float x, float y, float z, float* 
   const double tx = (double)libm::acosf_host_exact(pzf);
   const double ty = fmod((double)libm::atan2f_host_exact(y, x) + 2.0 * M_PI, 2.0 * M_PI);
   const double stx = sin(tx);
-  *px = (float)(stx * cos(ty));
-  *py = (float)(stx * sin(ty));
+  double sty, cty;
+  sincos(ty, &sty, &cty);  // one argument reduction for both
+  *px = (float)(stx * cty);
+  *py = (float)(stx * sty);
   *pz = pzf;
   *range = (float)dist;
 }
