@@ -84,6 +84,18 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False
 
 FORM_NAMES = {0: "plain fp32", 1: "split fp16 mfma contractions"}
 
+
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
 # launches of the stage's dominant kernel per pipeline pass
 LAUNCHES = {"spatial_fft_fwd": 2, "voxel_scatter": 2}
 
@@ -626,7 +638,7 @@ def main():
         "cpu_baseline": {"value": None if cpu_value is None else round(cpu_value, 4),
                          "unit": "registrations/s", "cores": threads, "kind": "port",
                          "latency_ms_1_thread": None if cpu_lat_ms is None else round(cpu_lat_ms, 1),
-                         "host_cpus": os.cpu_count(),
+                         "host_cpus": os.cpu_count(), "host_cpu_model": host_cpu_model(),
                          "sample": f"{S} pairs of the same workload through the FP64 oracle "
                                    f"(kd-tree NN, own FFT) on {threads} host threads, one pair "
                                    f"per thread at a time"},
