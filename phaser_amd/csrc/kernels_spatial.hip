@@ -1137,7 +1137,7 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
 // Inverse FFT along x and c2r along y of one z plane of a pair, with the
 // plane in LDS (L h complex, 66.5 KB at L = 128): the x-transformed plane is
 // never stored.  Workgroup (z, pair) of 256 threads: lines of R2 threads,
-// LINES = 256 / R2 at a time (line = thread % LINES fastest, n2 = thread /
+// LINES = TPB / R2 at a time (line = thread % LINES fastest, n2 = thread /
 // LINES).  Pass x: L / 2 lines, one per column k2 = 1 .. L/2 - 1 and line 0
 // for columns 0 and L/2 together: c2r reads only the real parts of those two,
 // and Re IFFT(a) + i Re IFFT(b) = IFFT(a_h + i b_h) with the Hermitian parts
@@ -1147,12 +1147,12 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
 // hc2r), exchanging through the pair's own two rows; signed first-maximum
 // argmax over x N + y + N^2 z into partials[pair][z]; the grid is written
 // when corr is given.
-template <int L>
-__global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict__ work,
+template <int L, int TPB>
+__global__ void __launch_bounds__(TPB) k_spat_inv_plane(const float2* __restrict__ work,
                                                         const float2* __restrict__ tw,
                                                         float* __restrict__ corr,
                                                         unsigned long long* __restrict__ partials) {
-  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = 256 / R2;
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, LINES = TPB / R2;
   constexpr int h = L / 2 + 1;
   static_assert((L / 2) % LINES == 0, "whole passes");
   __shared__ __align__(16) float2 P[L * h];
@@ -1164,17 +1164,17 @@ __global__ void __launch_bounds__(256) k_spat_inv_plane(const float2* __restrict
     // LDS: a strided copy loop waits for each load before its LDS store,
     // one HBM round trip per iteration
     static_assert((L * h) % 2 == 0, "whole float4s");
-    constexpr int NV = L * h / 2, PER = (NV + 255) / 256;
+    constexpr int NV = L * h / 2, PER = (NV + TPB - 1) / TPB;
     const float4* src = (const float4*)(work + ((int64_t)pair * L + z) * L * h);
     float4 t[PER];
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int idx = threadIdx.x + 256 * i;
+      const int idx = threadIdx.x + TPB * i;
       t[i] = src[idx < NV ? idx : NV - 1];  // in bounds; the store below is predicated
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int idx = threadIdx.x + 256 * i;
+      const int idx = threadIdx.x + TPB * i;
       if (idx < NV) ((float4*)P)[idx] = t[i];
     }
   }
@@ -1263,10 +1263,13 @@ void launch_spatial_inv_plane(int N, int npairs, const float2* work, const float
                               float* corr, unsigned long long* partials,
                               unsigned long long* keys, hipStream_t s) {
   const dim3 g(N, npairs);
+  // 512 threads at N = 128: one pass over the columns and one over the row
+  // pairs, and four waves per SIMD beside the plane's 66.5 KB of LDS
   if (N == 64)
-    hipLaunchKernelGGL(k_spat_inv_plane<64>, g, dim3(256), 0, s, work, tw, corr, partials);
+    hipLaunchKernelGGL((k_spat_inv_plane<64, 256>), g, dim3(256), 0, s, work, tw, corr, partials);
   else
-    hipLaunchKernelGGL(k_spat_inv_plane<128>, g, dim3(256), 0, s, work, tw, corr, partials);
+    hipLaunchKernelGGL((k_spat_inv_plane<128, 512>), g, dim3(512), 0, s, work, tw, corr,
+                       partials);
   launch_reduce_keys(partials, N, npairs, keys, s);
 }
 
