@@ -171,7 +171,9 @@ int phz_register_device(phz_ctx* ctx, const float* d_target, int64_t Pt,
 /* n independent pairs in one pass (n <= max_batch).  If shared_target is
  * nonzero only d_targets[0]/Pt[0] is read and every source registers against
  * it; the target's spectra are then computed once.  d_rot_corr/d_trans_corr
- * (optional) hold n consecutive grids. */
+ * (optional) hold n consecutive grids.  Point buffers aligned to 16 bytes
+ * (8 for the reflectivity / ambient pairs) are gathered into the context in
+ * one launch; otherwise each cloud is copied on its own. */
 int phz_register_pairs_device(phz_ctx* ctx, int n, int shared_target,
                               const float* const* d_targets, const int64_t* Pt,
                               const float* const* d_sources, const int64_t* Ps,

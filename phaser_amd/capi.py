@@ -50,6 +50,41 @@ class Result(C.Structure):
         }
 
 
+def _struct_dtype(st):
+    """The numpy dtype with the layout of ctypes structure st (scalars and
+    one-dimensional arrays of 32-bit ints / floats and doubles)."""
+    scalar = {C.c_uint32: np.uint32, C.c_int32: np.int32, C.c_float: np.float32,
+              C.c_double: np.float64}
+    names, formats, offsets = [], [], []
+    for name, ct in st._fields_:
+        names.append(name)
+        offsets.append(getattr(st, name).offset)
+        formats.append((scalar[ct._type_], (ct._length_,)) if hasattr(ct, "_length_")
+                       else scalar[ct])
+    return np.dtype({"names": names, "formats": formats, "offsets": offsets,
+                     "itemsize": C.sizeof(st)})
+
+
+_RESULT_DT = _struct_dtype(Result)
+_RESULT_SHAPES = {"T_rot": (4, 4), "trans_cov": (3, 3), "rot_axes": (4, 4)}
+
+
+def results_as_dicts(res, n):
+    """Result.as_dict() of the first n entries of a ctypes Result array,
+    converted column by column (one copy of the block instead of a dozen
+    small arrays per result); the same keys, types and values."""
+    a = np.frombuffer(res, dtype=_RESULT_DT, count=n).copy()
+    cols = {k: a[k].reshape(n, *_RESULT_SHAPES.get(k, a[k].shape[1:]))
+            for k in ("zyz", "quat", "euler_xyz", "T_rot", "translation", "trans_cov",
+                      "rot_omega", "rot_axes")}
+    ri, ti, st = a["rot_index"].tolist(), a["trans_index"].tolist(), a["status"].tolist()
+    rv = a["rot_value"].astype(np.float64).tolist()
+    tv = a["trans_value"].astype(np.float64).tolist()
+    return [{"rot_index": ri[k], "trans_index": ti[k], "rot_value": rv[k], "trans_value": tv[k],
+             **{name: col[k] for name, col in cols.items()}, "status": st[k]}
+            for k in range(n)]
+
+
 EXPORTS = {
     "phz_default_config": (None, [C.POINTER(Config)]),
     "phz_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
@@ -291,7 +326,7 @@ class Context:
                                                C.c_void_p(d_rot or None),
                                                C.c_void_p(d_trans or None),
                                                C.c_void_p(stream or None)))
-        return [r.as_dict() for r in res]
+        return results_as_dicts(res, n)
 
     def register_pairs_channels_device(self, d_targets, d_targets_ra, Pt, d_sources,
                                        d_sources_ra, Ps, shared_target=False, d_rot=0,
@@ -309,7 +344,7 @@ class Context:
         _check(lib().phz_register_pairs_channels_device(
             self.h, n, int(shared_target), T, TR, PT, S, SR, PS, res, C.c_void_p(d_rot or None),
             C.c_void_p(d_trans or None), C.c_void_p(stream or None)))
-        return [r.as_dict() for r in res]
+        return results_as_dicts(res, n)
 
     def register_pairs(self, targets, sources, shared_target=False):
         """Host clouds; targets holds one cloud when shared_target is set."""
@@ -322,7 +357,7 @@ class Context:
         PS = (C.c_int64 * n)(*[len(s) for s in sr])
         res = (Result * n)()
         _check(lib().phz_register_pairs(self.h, n, int(shared_target), T, PT, S, PS, res))
-        return [r.as_dict() for r in res]
+        return results_as_dicts(res, n)
 
     def memory_usage(self):
         b = C.c_size_t()

@@ -155,6 +155,13 @@ struct phz_ctx {
   float* d_peak_trans = nullptr;  // caller asks for no grids (neighbours > 0 only)
   float4* d_reg = nullptr;
   int* h_npts = nullptr;  // pinned staging for per-cloud point counts
+  // the input table of a batched call on device pointers: pinned staging,
+  // its device copy, and an event after the upload (the staging is not
+  // rewritten before the previous upload has run)
+  GatherEntry* h_gather = nullptr;
+  GatherEntry* d_gather = nullptr;
+  hipEvent_t gather_done = nullptr;
+  bool gather_pending = false;
   std::vector<void*> allocs;
   size_t alloc_bytes = 0;
   // per-stage HIP event timing (phz_set_profiling)
@@ -308,6 +315,7 @@ size_t context_bytes(const phz_config& cf) {
   // clouds and the S^2 search grid
   add(M * cap, sizeof(float4));
   add(M, sizeof(int));
+  add(M, sizeof(GatherEntry));
   add(M * nq, sizeof(int));
   add(M * (nq + 1), sizeof(int));
   add(M * (n + 1), sizeof(int));
@@ -492,6 +500,7 @@ int init_ctx(phz_ctx* c) {
   c->cells.ns = c->n;
   TRY(dalloc(c, &c->d_pts, (size_t)M * cap));
   TRY(dalloc(c, &c->d_npts, (size_t)M));
+  TRY(dalloc(c, &c->d_gather, (size_t)M));
   TRY(dalloc(c, &c->cells.cell_count, (size_t)M * nq));
   TRY(dalloc(c, &c->cells.cell_start, (size_t)M * (nq + 1)));
   TRY(dalloc(c, &c->cells.band_start, (size_t)M * (c->n + 1)));
@@ -548,6 +557,8 @@ int init_ctx(phz_ctx* c) {
     TRY(dalloc(c, &c->d_spec2, (size_t)(2 * M > 16 ? 2 * M : 16) * c->H));
   }
   PHZ_HIP(hipHostMalloc((void**)&c->h_npts, sizeof(int) * M, hipHostMallocDefault));
+  PHZ_HIP(hipHostMalloc((void**)&c->h_gather, sizeof(GatherEntry) * M, hipHostMallocDefault));
+  PHZ_HIP(hipEventCreateWithFlags(&c->gather_done, hipEventDisableTiming));
   // voxel accumulators stay zero between calls: every pipeline run clears
   // exactly the voxels it touched
   launch_fill_u32((uint32_t*)c->d_sumI, (size_t)MV * c->T3, 0u, c->stream);
@@ -959,6 +970,39 @@ int load_cloud(phz_ctx* c, int slot, const float* pts, int64_t P, hipMemcpyKind 
   return 0;
 }
 
+// Stages slots [0, ncl) from device buffers: the table of pointers and
+// counts goes up in one copy and one kernel gathers every cloud (instead
+// of a copy per cloud, each a launch of its own).  Returns 1 without doing
+// anything when a buffer is not aligned to its element (16 bytes for the
+// points, 8 for the pairs): the caller then copies cloud by cloud.
+int gather_clouds(phz_ctx* c, int ncl, const float* const* pts, const float* const* extra,
+                  const int64_t* counts, hipStream_t s) {
+  PHZ_CHECK(ncl >= 0 && ncl <= c->max_clouds, "cloud count exceeds the context's slots");
+  int64_t max_n = 0;
+  for (int i = 0; i < ncl; ++i) {
+    const int64_t P = counts[i];
+    PHZ_CHECK(P >= 0 && P <= c->cap, "point count exceeds max_points");
+    PHZ_CHECK(P == 0 || pts[i] != nullptr, "null point buffer");
+    PHZ_CHECK(!(extra && extra[i]) || c->cfg.channels == 4,
+              "reflectivity and ambient need a context created with channels = 4");
+    if (((uintptr_t)pts[i] & 15u) || (extra && ((uintptr_t)extra[i] & 7u))) return 1;
+    max_n = P > max_n ? P : max_n;
+  }
+  if (c->gather_pending) PHZ_HIP(hipEventSynchronize(c->gather_done));
+  for (int i = 0; i < ncl; ++i) {
+    c->h_gather[i] = {(const float4*)pts[i], extra ? (const float2*)extra[i] : nullptr, counts[i]};
+    c->h_npts[i] = (int)counts[i];
+  }
+  PHZ_HIP(hipMemcpyAsync(c->d_gather, c->h_gather, sizeof(GatherEntry) * ncl,
+                         hipMemcpyHostToDevice, s));
+  PHZ_HIP(hipEventRecord(c->gather_done, s));
+  c->gather_pending = true;
+  launch_gather_clouds(c->d_gather, ncl, max_n, c->d_pts, c->cfg.channels == 4 ? c->d_extra : nullptr,
+                       c->cap, c->d_npts, s);
+  PHZ_HIP(hipGetLastError());
+  return 0;
+}
+
 // copies the staged counts of slots [0, ncl) to the device
 int commit_counts(phz_ctx* c, int ncl, hipStream_t s) {
   PHZ_HIP(hipMemcpyAsync(c->d_npts, c->h_npts, sizeof(int) * ncl, hipMemcpyHostToDevice, s));
@@ -1044,6 +1088,8 @@ void phz_destroy(phz_ctx* c) {
   }
   for (void* p : c->allocs) hipFree(p);
   if (c->h_npts) hipHostFree(c->h_npts);
+  if (c->h_gather) hipHostFree(c->h_gather);
+  if (c->gather_done) hipEventDestroy(c->gather_done);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1108,13 +1154,26 @@ int phz_register_pairs_channels_device(phz_ctx* c, int n, int shared_target,
   PHZ_CHECK(n >= 1 && n <= c->cfg.max_batch, "pair count exceeds max_batch");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   const int nt = shared_target ? 1 : n;
-  for (int t = 0; t < nt; ++t)
-    TRY(load_cloud(c, t, d_targets[t], Pt[t], hipMemcpyDeviceToDevice, s,
-                   d_targets_ra ? d_targets_ra[t] : nullptr));
-  for (int p = 0; p < n; ++p)
-    TRY(load_cloud(c, nt + p, d_sources[p], Ps[p], hipMemcpyDeviceToDevice, s,
-                   d_sources_ra ? d_sources_ra[p] : nullptr));
-  TRY(commit_counts(c, nt + n, s));
+  // slots: targets [0, nt), sources [nt, nt + n)
+  std::vector<const float*> pts(nt + n), extra(nt + n);
+  std::vector<int64_t> counts(nt + n);
+  for (int t = 0; t < nt; ++t) {
+    pts[t] = d_targets[t];
+    extra[t] = d_targets_ra ? d_targets_ra[t] : nullptr;
+    counts[t] = Pt[t];
+  }
+  for (int p = 0; p < n; ++p) {
+    pts[nt + p] = d_sources[p];
+    extra[nt + p] = d_sources_ra ? d_sources_ra[p] : nullptr;
+    counts[nt + p] = Ps[p];
+  }
+  const int g = gather_clouds(c, nt + n, pts.data(), extra.data(), counts.data(), s);
+  if (g < 0) return g;
+  if (g > 0) {
+    for (int i = 0; i < nt + n; ++i)
+      TRY(load_cloud(c, i, pts[i], counts[i], hipMemcpyDeviceToDevice, s, extra[i]));
+    TRY(commit_counts(c, nt + n, s));
+  }
   TRY(exec_pairs(c, n, shared_target != 0, s, d_rot_corr, d_trans_corr));
   return fetch_results(c, n, results, s);
 }
@@ -1152,9 +1211,15 @@ int phz_register_device(phz_ctx* c, const float* d_target, int64_t Pt, const flo
                         float* d_registered, void* stream) {
   PHZ_CHECK(c && result, "null argument");
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-  TRY(load_cloud(c, 0, d_target, Pt, hipMemcpyDeviceToDevice, s));
-  TRY(load_cloud(c, 1, d_source, Ps, hipMemcpyDeviceToDevice, s));
-  TRY(commit_counts(c, 2, s));
+  const float* pts[2] = {d_target, d_source};
+  const int64_t counts[2] = {Pt, Ps};
+  const int g = gather_clouds(c, 2, pts, nullptr, counts, s);
+  if (g < 0) return g;
+  if (g > 0) {
+    TRY(load_cloud(c, 0, d_target, Pt, hipMemcpyDeviceToDevice, s));
+    TRY(load_cloud(c, 1, d_source, Ps, hipMemcpyDeviceToDevice, s));
+    TRY(commit_counts(c, 2, s));
+  }
   TRY(exec_pairs(c, 1, false, s, d_rot_corr, d_trans_corr));
   if (d_registered && Ps > 0)
     launch_apply_pose(c->d_pts + c->cap, Ps, c->d_pose, (float4*)d_registered, s);

@@ -139,6 +139,19 @@ void launch_so3_fft_gamma(int bw, int npairs, const float2* U, const FftPlan& pl
                           unsigned long long* partials, unsigned long long* keys,
                           hipStream_t s, bool reg = false);
 
+// One input cloud of a batched call on device pointers: its points (x, y, z,
+// intensity), its (reflectivity, ambient) pairs or null, and its count.
+struct GatherEntry {
+  const float4* pts;
+  const float2* extra;
+  int64_t n;
+};
+// Copies cloud i < ncl of the device table tab to pts + i cap (and, when
+// extra is given, its pairs, or zeros for a null entry, to extra + i cap)
+// and its count to npts[i], all in one launch; max_n bounds every n.
+void launch_gather_clouds(const GatherEntry* tab, int ncl, int64_t max_n, float4* pts,
+                          float2* extra, int64_t cap, int* npts, hipStream_t s);
+
 // CloudPreProcessor's filters (kernels_pre.hip).  launch_bbox: min / max of
 // the finite points' x, y, z into d_out[6] (d_part: 6 x the returned block
 // count floats, at most 256 blocks).

@@ -12,6 +12,9 @@
 // indices, and one centroid per run summed in float in the sorted order
 // (each voxel's points in their input order).  The output is ordered by
 // voxel index, as the reference's.
+//
+// Also here: the gather of a batched call's input clouds from the caller's
+// device buffers into a context's cloud slots (one launch for the batch).
 #include <hipcub/hipcub.hpp>
 
 #include "kernels.hpp"
@@ -177,6 +180,58 @@ void launch_voxel_grid_centroids(const float4* pts, const uint32_t* order, const
   if (max_runs <= 0) return;
   hipLaunchKernelGGL(k_voxel_grid_centroids, dim3((max_runs + 255) / 256), dim3(256), 0, s, pts,
                      order, counts, offsets, nruns, out);
+}
+
+namespace {
+// The input clouds of a batched call, gathered from the caller's buffers
+// into the context's slots: one workgroup of 256 threads per 1024 points of
+// cloud blockIdx.y; each thread issues its four loads before its stores
+// (indices clamped into the cloud, stores predicated).
+__global__ void __launch_bounds__(256) k_gather_clouds(const GatherEntry* __restrict__ tab,
+                                                       float4* __restrict__ pts,
+                                                       float2* __restrict__ extra, int64_t cap,
+                                                       int* __restrict__ npts) {
+  const int slot = blockIdx.y;
+  const GatherEntry e = tab[slot];
+  if (blockIdx.x == 0 && threadIdx.x == 0) npts[slot] = (int)e.n;
+  float4* dp = pts + (int64_t)slot * cap;
+  const int64_t stride = (int64_t)gridDim.x * 1024;
+  for (int64_t b = (int64_t)blockIdx.x * 1024; b < e.n; b += stride) {
+    float4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = b + threadIdx.x + 256 * k;
+      v[k] = e.pts[i < e.n ? i : e.n - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = b + threadIdx.x + 256 * k;
+      if (i < e.n) dp[i] = v[k];
+    }
+    if (extra) {
+      float2* dx = extra + (int64_t)slot * cap;
+      float2 x[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t i = b + threadIdx.x + 256 * k;
+        x[k] = e.extra ? e.extra[i < e.n ? i : e.n - 1] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t i = b + threadIdx.x + 256 * k;
+        if (i < e.n) dx[i] = x[k];
+      }
+    }
+  }
+}
+}  // namespace
+
+void launch_gather_clouds(const GatherEntry* tab, int ncl, int64_t max_n, float4* pts,
+                          float2* extra, int64_t cap, int* npts, hipStream_t s) {
+  if (ncl <= 0) return;
+  const int64_t per = (max_n + 1023) / 1024;
+  const unsigned bx = (unsigned)(per < 1 ? 1 : (per > 256 ? 256 : per));
+  hipLaunchKernelGGL(k_gather_clouds, dim3(bx, ncl), dim3(256), 0, s, tab, pts, extra, cap, npts);
 }
 
 hipError_t pass_through_select(void* temp, size_t* temp_bytes, const float4* pts, int64_t P,

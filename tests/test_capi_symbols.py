@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -184,3 +185,34 @@ def test_memory_estimate_without_gpu():
     bad = capi.default_config(bandwidth=161)
     b = C.c_size_t()
     assert capi.lib().phz_estimate_memory(C.byref(bad), C.byref(b)) == -1
+
+
+def test_results_as_dicts_equals_as_dict():
+    """The column-wise conversion of a batch of results gives what
+    Result.as_dict() gives for each: the same keys, types, shapes and values."""
+    from phaser_amd import capi
+    rng = np.random.default_rng(0)
+    n = 5
+    res = (capi.Result * n)()
+    for k in range(n):
+        r = res[k]
+        r.rot_index, r.trans_index, r.status = 7 * k + 1, 11 * k + 2, k % 3
+        r.rot_value, r.trans_value = float(rng.normal()), float(rng.normal() * 1e9)
+        for name, ct in capi.Result._fields_:
+            if hasattr(ct, "_length_"):
+                arr = getattr(r, name)
+                for i in range(ct._length_):
+                    arr[i] = float(rng.normal())
+    got = capi.results_as_dicts(res, n)
+    assert len(got) == n
+    for a, b in zip(got, [r.as_dict() for r in res]):
+        assert list(a) == list(b)
+        for key, vb in b.items():
+            va = a[key]
+            if isinstance(vb, np.ndarray):
+                assert va.dtype == vb.dtype and va.shape == vb.shape, key
+                np.testing.assert_array_equal(va, vb)
+            else:
+                assert type(va) is type(vb) and va == vb, key
+    two = capi.results_as_dicts(res, 2)
+    assert [d["rot_index"] for d in two] == [d["rot_index"] for d in got[:2]]
