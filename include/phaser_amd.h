@@ -378,6 +378,14 @@ typedef struct {
                           bit 1 = bands skipped on their sector occupancy bits in LDS;
                           0 = default (neither: at 64 pairs per launch neither
                           shortened the search), -1 = neither, 1, 2 or 3 */
+  int32_t xz_slab;     /* with occ_fwd and inv_plane on (two channels, N = 64, 128):
+                          the x pass, the forward z pass, fusion, the cross
+                          spectrum and the inverse z pass in one kernel per
+                          (pair, y frequency) with the (z, x) plane in LDS, so
+                          the x-transformed spectra are never stored (exact,
+                          other rounding): 1 = on, 0 or -1 = off (default: at
+                          N = 128 it took 1.08 ms per 64 pairs against 0.85 ms
+                          for the separate x pass and cross-spectrum kernel) */
   float ms_fwd_x, ms_fwd_z, ms_inv_x;  /* per launch at max_batch (phz_autotune) */
   float ms_plane;      /* plane pass per launch, or the r2c + x passes it replaces */
   float ms_fused;      /* the three fused passes at max_batch, chosen form */

@@ -165,7 +165,7 @@ class Tuning(C.Structure):
     _fields_ = [("fwd_x", C.c_int32), ("fwd_z", C.c_int32), ("inv_x", C.c_int32),
                 ("plane_fwd", C.c_int32), ("nn_lanes", C.c_int32), ("fused_reg", C.c_int32),
                 ("so3_reg", C.c_int32), ("occ_fwd", C.c_int32), ("inv_plane", C.c_int32),
-                ("nn_xcd", C.c_int32), ("nn_seed", C.c_int32),
+                ("nn_xcd", C.c_int32), ("nn_seed", C.c_int32), ("xz_slab", C.c_int32),
                 ("ms_fwd_x", C.c_float), ("ms_fwd_z", C.c_float),
                 ("ms_inv_x", C.c_float), ("ms_plane", C.c_float), ("ms_fused", C.c_float),
                 ("ms_so3", C.c_float)]
@@ -393,9 +393,9 @@ class Context:
         return t.as_dict()
 
     def set_tuning(self, fwd_x=0, fwd_z=0, inv_x=0, plane_fwd=0, nn_lanes=0, fused_reg=0,
-                   so3_reg=0, occ_fwd=0, inv_plane=0, nn_xcd=0, nn_seed=0):
+                   so3_reg=0, occ_fwd=0, inv_plane=0, nn_xcd=0, nn_seed=0, xz_slab=0):
         t = Tuning(fwd_x, fwd_z, inv_x, plane_fwd, nn_lanes, fused_reg, so3_reg, occ_fwd,
-                   inv_plane, nn_xcd, nn_seed)
+                   inv_plane, nn_xcd, nn_seed, xz_slab)
         _check(lib().phz_set_tuning(self.h, C.byref(t)))
 
     def set_graphs(self, on=True):

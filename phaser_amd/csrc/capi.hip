@@ -103,6 +103,7 @@ struct phz_ctx {
   bool inv_plane = true;   // inverse x + c2r as one pass over z planes in LDS
   bool nn_xcd = kNnXcdDefault;  // XCD-ordered NN workgroups
   int nn_seed = kNnSeedDefault;  // NN seeds (bit 0) and LDS band skipping (bit 1)
+  bool xz_slab = false;    // x and z passes + cross spectrum per (pair, k2) slab in LDS
   phz_tuning tuning{};
   float max_sq = 0.f;
   // resident tables
@@ -588,6 +589,8 @@ int init_ctx(phz_ctx* c) {
   c->tuning.nn_xcd = kNnXcdDefault ? 1 : -1;
   c->nn_seed = kNnSeedDefault;
   c->tuning.nn_seed = kNnSeedDefault ? kNnSeedDefault : -1;
+  c->xz_slab = false;
+  c->tuning.xz_slab = -1;
   return 0;
 }
 
@@ -732,6 +735,12 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                   (four ? spatial_cross_zz4_ok(N, F4, G4) : spatial_cross_zz_ok(N, Fc, Gc));
   const bool occ = zz && !c->plane_fwd && c->tuning.fwd_x == kRegisterEngine;
   const bool plane = c->plane_fwd && !four;
+  const bool inv_plane = !padded && c->inv_plane && c->tiles.fused_reg &&
+                         c->tuning.inv_x == kRegisterEngine && spatial_inv_plane_ok(N);
+  // slab: the x pass, the forward z pass, fusion, cross spectrum and
+  // inverse z pass in one kernel per (pair, k2), written in the slab layout
+  // that only the inverse plane pass reads
+  const bool slab = occ && !four && inv_plane && c->xz_slab && spatial_cross_slab_ok(N, Fc, Gc);
   {  // y (and with the plane pass also x) first, then the remaining columns
     StageScope t(c, kSpatialR2c, s);
     if (plane)
@@ -761,9 +770,9 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     StageScope t(c, kSpatialFftFwd, s);
     for (int set = 0; set < (four ? 2 : 1); ++set) {
       float2* spec = set ? c->d_spec2 : c->d_spec;
-      if (occ)
-        launch_spatial_fft_x_occ(N, 2 * ncl, spec, occ_from(c, 0), c->d_tw_N, s);
-      else if (!plane)
+      if (occ) {
+        if (!slab) launch_spatial_fft_x_occ(N, 2 * ncl, spec, occ_from(c, 0), c->d_tw_N, s);
+      } else if (!plane)
         launch_spatial_fft_pass(N, 2 * ncl, spec, 0, c->tiles.fwd_x, c->plan_N, c->d_tw_N, s);
       if (!zz)
         launch_spatial_fft_pass(N, 2 * ncl, spec, 1, c->tiles.fwd_z, c->plan_N, c->d_tw_N, s);
@@ -772,7 +781,11 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
   {
     {
       StageScope t(c, kSpatialCrossZ, s);
-      if (padded)
+      if (slab)
+        launch_spatial_cross_slab(N, np, Fc, fs, Gc, gs, occ_from(c, 0).row, occ_from(c, 0).plane,
+                                  shared, occ_from(c, nt).row, occ_from(c, nt).plane, c->vox.runs,
+                                  c->d_tw_N, c->d_work, s);
+      else if (padded)
         launch_spatial_cross_product(N, np, four ? F4 : Fc, fs, four ? G4 : Gc, gs, c->vox.runs,
                                      true, c->d_work, s);
       else if (four && zz)
@@ -807,8 +820,6 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
                          c->d_edges, c->d_sumI + nt * c->T3, c->d_sumR + nt * c->T3,
                          c->d_cnt + nt * c->T3, occ_from(c, nt), s, extra_from(c, nt));
     }
-    const bool inv_plane = !padded && c->inv_plane && c->tiles.fused_reg &&
-                           c->tuning.inv_x == kRegisterEngine && spatial_inv_plane_ok(N);
     if (padded) {  // spatial-correlation.cc:142-161,221-230; spatial-correlation-laplace.cc:36-37
       const int Np = c->Np, tile = fft_cols_default_tile(Np);
       {
@@ -823,7 +834,7 @@ int run_pairs(phz_ctx* c, int np, bool shared, hipStream_t s, float* d_rot_corr,
     } else if (inv_plane) {  // inverse x and c2r in one pass over whole z planes
       StageScope t(c, kSpatialC2r, s);
       launch_spatial_inv_plane(N, np, c->d_work, c->d_tw_N, d_trans_corr, c->d_partials,
-                               c->d_keys + np, s);
+                               c->d_keys + np, s, slab);
     } else {
       {
         StageScope t(c, kSpatialFftInv, s);
@@ -1744,6 +1755,8 @@ static void apply_tuning(phz_ctx* c, const phz_tuning& t) {
   c->tuning.nn_xcd = c->nn_xcd ? 1 : -1;
   c->nn_seed = t.nn_seed == 0 ? kNnSeedDefault : (t.nn_seed > 0 ? (t.nn_seed & 3) : 0);
   c->tuning.nn_seed = c->nn_seed ? c->nn_seed : -1;
+  c->xz_slab = t.xz_slab > 0;
+  c->tuning.xz_slab = c->xz_slab ? 1 : -1;
 }
 
 int phz_set_tuning(phz_ctx* c, const phz_tuning* t) {
@@ -1898,6 +1911,7 @@ int phz_autotune(phz_ctx* c, int repeats, phz_tuning* out) {
   best.nn_lanes = c->tuning.nn_lanes;  // data-dependent: set with phz_set_tuning
   best.nn_xcd = c->tuning.nn_xcd;
   best.nn_seed = c->tuning.nn_seed;
+  best.xz_slab = c->tuning.xz_slab;
   best.ms_fwd_x = best_ms[0];
   best.ms_fwd_z = best_ms[1];
   best.ms_inv_x = best_ms[2];

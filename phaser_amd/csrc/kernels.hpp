@@ -319,10 +319,24 @@ int spatial_c2r_reg_blocks(int N);
 // inverse FFT along x + c2r along y + argmax in one pass over whole z planes
 // held in LDS (N = 64, 128): replaces launch_spatial_fft_inv followed by
 // launch_spatial_c2r_reg; partials: npairs * N entries
+// slab: work in the [pair][k2][z][x] layout of launch_spatial_cross_slab
 bool spatial_inv_plane_ok(int N);
 void launch_spatial_inv_plane(int N, int npairs, const float2* work, const float2* tw,
                               float* corr, unsigned long long* partials,
-                              unsigned long long* keys, hipStream_t s);
+                              unsigned long long* keys, hipStream_t s, bool slab = false);
+// the x pass, the z pass, fusion, cross spectrum and inverse z pass in one
+// launch over (pair, k2) slabs with the (z, x) plane in LDS (N = 64, 128;
+// two live channels as spatial_cross_zz_ok): F / G are the half spectra of
+// launch_spatial_r2c_rows (only occupied rows written), f_rows / f_planes
+// and g_rows / g_planes the occupancy bytes of the first target and source
+// clouds (one cloud per pair; f_shared: target 0 for all).  Output:
+// work[pair][k2][z][x], for launch_spatial_inv_plane(.., slab = true).
+bool spatial_cross_slab_ok(int N, const ChannelSet& F, const ChannelSet& G);
+void launch_spatial_cross_slab(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                               int64_t g_stride, const uint8_t* f_rows, const uint8_t* f_planes,
+                               bool f_shared, const uint8_t* g_rows, const uint8_t* g_planes,
+                               const FusionRuns& runs, const float2* tw, float2* work,
+                               hipStream_t s);
 void launch_spatial_c2r_reg(int N, int npairs, const float2* work, const float2* tw, float* corr,
                             unsigned long long* partials, unsigned long long* keys,
                             hipStream_t s);

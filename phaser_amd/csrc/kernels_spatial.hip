@@ -965,6 +965,264 @@ __global__ void __launch_bounds__(256, L <= 128 ? 3 : 2) k_spat_cross_zz_reg(
   });
 }
 
+// The forward x and z passes of both clouds' two channels, their Laplace
+// fusion, C = F conj(G) and the inverse z pass for one pair at one y
+// frequency k2, with the (z, x) plane of that frequency in LDS: the
+// x-transformed spectra never go to memory (k_fft_x_occ followed by
+// k_spat_cross_zz_reg store and reload them, two HBM round trips of every
+// occupied plane).  Workgroup (pair, k2) of L R2 threads: line = thread % L,
+// n2 = thread / L, so each wave holds 64 lines at one n2.
+// Input: the half spectra of k_spat_r2c_rows (layout [z][x][k2]; only rows
+// (z, x) that hold points were written), read with buffer loads whose
+// masked lanes address past the buffer (no memory access, zero returned);
+// the row bits come from the clouds' occupancy bytes.  x pass: over the z
+// planes that hold points, listed ascending one per line (a wave whose
+// lines are all past the list idles); the z pass reads every other plane
+// row as zero.  Each line exchanges through its own row (x pass) or
+// column (z pass) of the plane.  A thread's z-pass output k = n2 + R2 j +
+// R1 k2' is the inverse transform's input n1 = j + (R1 / R2) k2' (as in
+// k_spat_cross_zz_reg).  Output: work[pair][k2][z][x], one contiguous
+// slab per workgroup (k_spat_inv_plane<L, TPB, true> reads it).
+// f_shared: every pair reads target cloud 0.  xcd: workgroup b runs pair
+// (b mod 8) + 8 (b / 8h) at k2 = (b / 8) mod h, so that the h workgroups of
+// a pair run on one XCD and share its L2 (a 128-byte line of a spectrum
+// row holds 16 consecutive k2); needs a multiple of 8 pairs.
+// The fused F side waits in global memory (the workgroup's own output
+// slab) while the G side runs, which keeps the registers of 1024 threads
+// within 128.  Measured at L = 128, 64 pairs: 1.08 ms against 0.85 ms for
+// k_fft_x_occ + k_spat_cross_zz_reg: with 138 KB of LDS one workgroup fills
+// a CU, and its load latency, LDS phases and barriers have no other
+// workgroup to overlap with; an option (phz_tuning.xz_slab), off by default.
+template <int L>
+__global__ void __launch_bounds__(L * RegSplit<L>::R2) k_spat_cross_slab(
+    ChannelSet F, int64_t f_stride, ChannelSet G, int64_t g_stride,
+    const uint8_t* __restrict__ f_rows, const uint8_t* __restrict__ f_planes, int f_shared,
+    const uint8_t* __restrict__ g_rows, const uint8_t* __restrict__ g_planes, FusionRuns runs,
+    int xcd, const float2* __restrict__ tw, float2* __restrict__ work) {
+  constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2, S = L + 1, h = L / 2 + 1;
+  constexpr int TPB = L * R2, W = L / 32;  // W: bit words per plane row
+  constexpr int64_t inner = (int64_t)L * h, H = inner * L;
+  static_assert(32 % R2 == 0 && TPB >= 2 * L && L % 64 == 0, "thread layout");
+  static_assert(H * 8 < 0x7fffffffLL, "one buffer resource per spectrum");
+  extern __shared__ float2 P[];  // L rows (z) of S entries (x, padded)
+  __shared__ uint32_t rowbits[2][L * W];
+  __shared__ uint32_t planebits[2][W];
+  __shared__ int zlist[2][L];
+  __shared__ int nzs[2];
+  __shared__ int wcnt[2 * L / 64];
+  // n2 is uniform over a wave (L >= 64): the twiddles W_L^(n2 k1) are
+  // scalar loads, kept out of the vector registers
+  const int t = threadIdx.x, line = t % L, n2 = __builtin_amdgcn_readfirstlane(t / L);
+  int p, k2;
+  if (xcd) {
+    const int b = blockIdx.x;
+    p = (b & 7) + 8 * (b / (8 * h));
+    k2 = (b >> 3) % h;
+  } else {
+    p = blockIdx.x / h;
+    k2 = blockIdx.x - p * h;
+  }
+  const int fc = f_shared ? 0 : p;
+  // row occupancy bytes -> bits: word r of a side covers bytes 32 r .. 32 r + 31
+  for (int wi = t; wi < 2 * L * W; wi += TPB) {
+    const int side = wi / (L * W), r = wi - side * (L * W);
+    const uint4* src = (const uint4*)((side ? g_rows + (int64_t)p * L * L
+                                            : f_rows + (int64_t)fc * L * L) + 32 * r);
+    const uint4 a = src[0], b = src[1];
+    const uint32_t wd[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t bits = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bits |= (uint32_t)(((wd[k] >> (8 * j)) & 0xffu) != 0u) << (4 * k + j);
+    rowbits[side][r] = bits;
+  }
+  // occupied planes of both sides: bits and ascending lists
+  {
+    const bool in = t < 2 * L;
+    const int side = t / L, z = t % L, lane = t & 63, wv = t >> 6;
+    const bool occ = in && (side ? g_planes[(int64_t)p * L + z] : f_planes[(int64_t)fc * L + z]) != 0;
+    const unsigned long long m = __ballot(occ);
+    if (in && lane == 0) {
+      wcnt[wv] = __popcll(m);
+      planebits[side][z >> 5] = (uint32_t)m;
+      planebits[side][(z >> 5) + 1] = (uint32_t)(m >> 32);
+    }
+    __syncthreads();
+    if (in) {
+      int base = 0;
+      for (int i = side * (L / 64); i < wv; ++i) base += wcnt[i];
+      if (occ) zlist[side][base + __popcll(m & ((1ull << lane) - 1ull))] = z;
+      if (z == 0) {
+        int tot = 0;
+        for (int i = 0; i < L / 64; ++i) tot += wcnt[side * (L / 64) + i];
+        nzs[side] = tot;
+      }
+    }
+    __syncthreads();
+  }
+  // forward x and z passes of one channel of one side; out[j R2 + k2'] =
+  // X(kz = n2 + R2 j + R1 k2', kx = line)
+  auto fwd2 = [&](int side, const float2* X, float2(&out)[R1]) {
+    const int nz = nzs[side];
+    const bool wave_on = (line & ~63) < nz, act = line < nz;
+    const int z = act ? zlist[side][line] : 0;
+    float2* row = P + z * S;
+    float2 v[R1];
+    if (wave_on) {
+      // this thread's 16 row bits (x = n2 + R2 n1) in one register
+      uint32_t xm = 0;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const uint32_t rw = act ? rowbits[side][z * W + i] : 0u;
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1)
+          if (((R2 * n1) >> 5) == i) xm |= ((rw >> ((n2 + R2 * n1) & 31)) & 1u) << n1;
+      }
+      const __amdgpu_buffer_rsrc_t rs =
+          __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, (int)(H * 8), 0x00020000);
+      // lane offset of x = n2 plus a scalar offset per n1; a masked lane's
+      // offset lies past the buffer with or without the scalar part.  Two
+      // 32-bit loads per entry (merged into one 64-bit load): this
+      // compiler's 64-bit buffer-load builtin fetches one dword and returns
+      // it in both halves.
+      const uint32_t base = (uint32_t)(((z * L + n2) * h + k2) * 8);
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        const uint32_t o = ((xm >> n1) & 1u) ? base : 0x80000000u;
+        const int so = R2 * n1 * h * 8;
+        v[n1] = make_float2(
+            __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o, so, 0)),
+            __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, o + 4u, so, 0)));
+      }
+      dft_reg<R1, -1>(v);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+    }
+    __syncthreads();  // the previous readers of the plane are done
+    if (wave_on && act) {
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) row[k1 * R2 + n2] = v[k1];
+    }
+    __syncthreads();
+    float2 r[R1];
+    if (wave_on && act) {
+#pragma unroll
+      for (int j = 0; j < R1 / R2; ++j) {
+        const int k1 = n2 + R2 * j;
+        float2 u[R2];
+#pragma unroll
+        for (int m = 0; m < R2; ++m) u[m] = row[k1 * R2 + m];
+        dft_reg<R2, -1>(u);
+#pragma unroll
+        for (int kk = 0; kk < R2; ++kk) r[j * R2 + kk] = u[kk];
+      }
+    }
+    __syncthreads();  // the exchange is read before the results go back
+    if (wave_on && act) {
+#pragma unroll
+      for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+        for (int kk = 0; kk < R2; ++kk) row[(n2 + R2 * j) + R1 * kk] = r[j * R2 + kk];
+    }
+    __syncthreads();
+    // z pass along column x = line; planes without points are zero
+    uint32_t pw[W];  // uniform: scalar registers
+#pragma unroll
+    for (int i = 0; i < W; ++i) pw[i] = __builtin_amdgcn_readfirstlane(planebits[side][i]);
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) {
+      const int zz = n2 + R2 * n1;
+      const float2 e = P[zz * S + line];
+      v[n1] = ((pw[(R2 * n1) >> 5] >> (zz & 31)) & 1u) ? e : make_float2(0.f, 0.f);
+    }
+    dft_reg<R1, -1>(v);
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], tw[n2 * k1]);
+    __syncthreads();  // the columns are read before the exchanges overwrite them
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) P[(k1 * R2 + n2) * S + line] = v[k1];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < R1 / R2; ++j) {
+      const int k1 = n2 + R2 * j;
+      float2 u[R2];
+#pragma unroll
+      for (int m = 0; m < R2; ++m) u[m] = P[(k1 * R2 + m) * S + line];
+      dft_reg<R2, -1>(u);
+#pragma unroll
+      for (int kk = 0; kk < R2; ++kk) out[j * R2 + kk] = u[kk];
+    }
+  };
+  // fusion mode of this thread's frequencies (2 bits each)
+  uint32_t modes = 0;
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int kk = 0; kk < R2; ++kk) {
+      const uint64_t idx = (uint64_t)(n2 + R2 * j + R1 * kk) * inner + (uint64_t)line * h + k2;
+      uint32_t m = 0;
+      for (int r = 0; r < runs.count; ++r)
+        if (idx >= runs.start[r]) m = runs.mode[r];
+      modes |= (m & 3u) << (2 * (j * R2 + kk));
+    }
+  // the fused F side waits in this workgroup's own output slab (rows kz,
+  // column line) while the G side is transformed: every thread rereads and
+  // finally overwrites exactly the entries it parked there, so no other
+  // thread's accesses intervene
+  float2* slab = work + ((int64_t)p * h + k2) * L * L + line;
+  {
+    float2 fa[R1], fb[R1];
+    fwd2(0, F.ch[0] + p * f_stride, fa);
+    fwd2(0, F.ch[1] + p * f_stride, fb);
+#pragma unroll
+    for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+      for (int kk = 0; kk < R2; ++kk) {
+        const int i = j * R2 + kk;
+        slab[(int64_t)(n2 + R2 * j + R1 * kk) * L] =
+            fuse_pair(fa[i], fb[i], F.C, (modes >> (2 * i)) & 3u);
+      }
+  }
+  float2 ga[R1], xb[R1];
+  fwd2(1, G.ch[0] + p * g_stride, ga);
+  fwd2(1, G.ch[1] + p * g_stride, xb);
+#pragma unroll
+  for (int i = 0; i < R1; ++i) ga[i] = fuse_pair(ga[i], xb[i], G.C, (modes >> (2 * i)) & 3u);
+  // the last z pass's exchange is read before the inverse one is written;
+  // the parked F entries are loaded only after the G side has been fused
+  __syncthreads();
+  float2 v[R1];
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j)
+#pragma unroll
+    for (int kk = 0; kk < R2; ++kk) {
+      const int i = j * R2 + kk;
+      const float2 f = slab[(int64_t)(n2 + R2 * j + R1 * kk) * L], g = ga[i];
+      float2 cz;
+      cz.x = f.x * g.x - f.y * (-g.y);
+      cz.y = f.x * (-g.y) + f.y * g.x;
+      v[j + (R1 / R2) * kk] = cz;
+    }
+  // inverse z pass along column line, straight to the slab
+  dft_reg<R1, +1>(v);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cconj(tw[n2 * k1]));
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) P[(k1 * R2 + n2) * S + line] = v[k1];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < R1 / R2; ++j) {
+    const int k1 = n2 + R2 * j;
+    float2 u[R2];
+#pragma unroll
+    for (int m = 0; m < R2; ++m) u[m] = P[(k1 * R2 + m) * S + line];
+    dft_reg<R2, +1>(u);
+#pragma unroll
+    for (int kk = 0; kk < R2; ++kk) slab[(int64_t)(k1 + R1 * kk) * L] = u[kk];
+  }
+}
+
 // One side (F or G) of the four-channel cross spectrum: each channel's
 // forward FFT along z (occupied planes only), folded into the fusion state
 // in channel order as fuse_at does: mode 0 keeps the first channel of
@@ -1146,8 +1404,10 @@ __global__ void __launch_bounds__(256) k_spat_c2r_reg(const float2* __restrict__
 // (2r, 2r + 1) packed as one complex line (real parts of k = 0, L/2, FFTW
 // hc2r), exchanging through the pair's own two rows; signed first-maximum
 // argmax over x N + y + N^2 z into partials[pair][z]; the grid is written
-// when corr is given.
-template <int L, int TPB>
+// when corr is given.  kSlab: work is laid out [pair][k2][z][x] (the output
+// of k_spat_cross_slab) instead of [pair][z][x][k2]; the plane's h rows of
+// L entries are transposed into the same LDS layout as they are stored.
+template <int L, int TPB, bool kSlab>
 __global__ void __launch_bounds__(TPB) k_spat_inv_plane(const float2* __restrict__ work,
                                                         const float2* __restrict__ tw,
                                                         float* __restrict__ corr,
@@ -1159,7 +1419,28 @@ __global__ void __launch_bounds__(TPB) k_spat_inv_plane(const float2* __restrict
   const int c = threadIdx.x % LINES, n2 = threadIdx.x / LINES;
   const int z = blockIdx.x, pair = blockIdx.y;
   const Twiddles<L> w = load_twiddles<L>(n2, tw);
-  {
+  if (kSlab) {
+    // h rows (k2) of L contiguous entries (x), two per float4, all loads
+    // of a thread in flight before its LDS stores
+    constexpr int NV = h * L / 2, PER = (NV + TPB - 1) / TPB;
+    const float4* src = (const float4*)(work + (int64_t)pair * h * L * L + (int64_t)z * L);
+    float4 t[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = threadIdx.x + TPB * i, ic = idx < NV ? idx : NV - 1;
+      const int k2 = ic / (L / 2), x2 = ic - k2 * (L / 2);
+      t[i] = src[(int64_t)k2 * (L * L / 2) + x2];  // in bounds; the stores are predicated
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int idx = threadIdx.x + TPB * i;
+      if (idx < NV) {
+        const int k2 = idx / (L / 2), x = 2 * (idx - k2 * (L / 2));
+        P[x * h + k2] = make_float2(t[i].x, t[i].y);
+        P[(x + 1) * h + k2] = make_float2(t[i].z, t[i].w);
+      }
+    }
+  } else {
     // the plane (L h complex) in one batch of loads per thread, then into
     // LDS: a strided copy loop waits for each load before its LDS store,
     // one HBM round trip per iteration
@@ -1261,16 +1542,56 @@ bool spatial_inv_plane_ok(int N) { return N == 64 || N == 128; }
 
 void launch_spatial_inv_plane(int N, int npairs, const float2* work, const float2* tw,
                               float* corr, unsigned long long* partials,
-                              unsigned long long* keys, hipStream_t s) {
+                              unsigned long long* keys, hipStream_t s, bool slab) {
   const dim3 g(N, npairs);
   // 512 threads at N = 128: one pass over the columns and one over the row
   // pairs, and four waves per SIMD beside the plane's 66.5 KB of LDS
-  if (N == 64)
-    hipLaunchKernelGGL((k_spat_inv_plane<64, 256>), g, dim3(256), 0, s, work, tw, corr, partials);
-  else
-    hipLaunchKernelGGL((k_spat_inv_plane<128, 512>), g, dim3(512), 0, s, work, tw, corr,
-                       partials);
+  if (N == 64) {
+    if (slab)
+      hipLaunchKernelGGL((k_spat_inv_plane<64, 256, true>), g, dim3(256), 0, s, work, tw, corr,
+                         partials);
+    else
+      hipLaunchKernelGGL((k_spat_inv_plane<64, 256, false>), g, dim3(256), 0, s, work, tw, corr,
+                         partials);
+  } else {
+    if (slab)
+      hipLaunchKernelGGL((k_spat_inv_plane<128, 512, true>), g, dim3(512), 0, s, work, tw, corr,
+                         partials);
+    else
+      hipLaunchKernelGGL((k_spat_inv_plane<128, 512, false>), g, dim3(512), 0, s, work, tw, corr,
+                         partials);
+  }
   launch_reduce_keys(partials, N, npairs, keys, s);
+}
+
+bool spatial_cross_slab_ok(int N, const ChannelSet& F, const ChannelSet& G) {
+  return (N == 64 || N == 128) && spatial_cross_zz_ok(N, F, G);
+}
+
+template <int L>
+static void launch_cross_slab_t(int npairs, ChannelSet F, int64_t fs, ChannelSet G, int64_t gs,
+                                const uint8_t* f_rows, const uint8_t* f_planes, bool f_shared,
+                                const uint8_t* g_rows, const uint8_t* g_planes,
+                                const FusionRuns& runs, const float2* tw, float2* work,
+                                hipStream_t s) {
+  constexpr int h = L / 2 + 1, TPB = L * RegSplit<L>::R2;
+  const int xcd = npairs % 8 == 0;
+  const size_t lds = sizeof(float2) * L * (L + 1);
+  hipLaunchKernelGGL(k_spat_cross_slab<L>, dim3(npairs * h), dim3(TPB), lds, s, F, fs, G, gs,
+                     f_rows, f_planes, (int)f_shared, g_rows, g_planes, runs, xcd, tw, work);
+}
+
+void launch_spatial_cross_slab(int N, int npairs, ChannelSet F, int64_t f_stride, ChannelSet G,
+                               int64_t g_stride, const uint8_t* f_rows, const uint8_t* f_planes,
+                               bool f_shared, const uint8_t* g_rows, const uint8_t* g_planes,
+                               const FusionRuns& runs, const float2* tw, float2* work,
+                               hipStream_t s) {
+  if (N == 64)
+    launch_cross_slab_t<64>(npairs, F, f_stride, G, g_stride, f_rows, f_planes, f_shared, g_rows,
+                            g_planes, runs, tw, work, s);
+  else
+    launch_cross_slab_t<128>(npairs, F, f_stride, G, g_stride, f_rows, f_planes, f_shared,
+                             g_rows, g_planes, runs, tw, work, s);
 }
 
 bool spatial_reg_ok(int N) { return N == 64 || N == 128 || N == 256; }

@@ -702,6 +702,46 @@ def test_occupancy_skipping_batch_matches_dense():
             assert abs(a["trans_value"] - b["trans_value"]) <= 1e-6 * abs(a["trans_value"])
 
 
+@pytest.mark.parametrize("B,npairs", [(32, 8), (32, 3), (64, 1), (64, 8)])
+def test_slab_pass_matches_separate_passes(B, npairs):
+    """The x pass, z pass, fusion and cross spectrum in one kernel per (pair,
+    k2) slab (phz_tuning.xz_slab) against the separate x pass and fused z /
+    cross-spectrum kernel: same bins, translation grids within FP32
+    rounding; separate targets (8 pairs: the XCD-ordered slabs) and a shared
+    target."""
+    from phaser_amd.capi import Context
+    N = 2 * B
+    cfg = dict(rings=64, azimuths=1024) if B == 32 else dict(rings=128, azimuths=1024,
+                                                            overlap=0.7)
+    pairs = [_pair(s, cfg) for s in range(40, 40 + npairs)]
+    tg = [t for _, t, _, _ in pairs]
+    sr = [s for s, _, _, _ in pairs]
+    with Context(bandwidth=B, n_voxels=N, max_points=200000, max_batch=npairs) as ctx:
+        dt = [ctx.device_array(t) for t in tg]
+        ds = [ctx.device_array(s) for s in sr]
+        grid = ctx.device_empty(npairs * N ** 3 * 4)
+        got = {}
+        for slab in (-1, 1):
+            ctx.set_tuning(1, 1, 1, fused_reg=1, xz_slab=slab)
+            assert ctx.tuning()["xz_slab"] == slab
+            print(f"B {B}, {npairs} pairs, xz_slab {slab}", flush=True)
+            res = ctx.register_pairs_device([a.ptr for a in dt], [len(t) for t in tg],
+                                            [a.ptr for a in ds], [len(s) for s in sr],
+                                            d_trans=grid.ptr)
+            shared = ctx.register_pairs_device([dt[0].ptr], [len(tg[0])], [a.ptr for a in ds],
+                                               [len(s) for s in sr], shared_target=True)
+            got[slab] = (res, shared, grid.download((npairs, N ** 3), np.float32))
+        for a in dt + ds + [grid]:
+            a.free()
+    for la, lb in zip(got[-1][:2], got[1][:2]):
+        for a, b in zip(la, lb):
+            assert a["trans_index"] == b["trans_index"] and a["rot_index"] == b["rot_index"]
+            assert abs(a["trans_value"] - b["trans_value"]) <= 1e-5 * abs(a["trans_value"])
+    ga, gb = got[-1][2].reshape(npairs, -1), got[1][2].reshape(npairs, -1)
+    for k in range(npairs):
+        assert _rel(gb[k], ga[k]) < 1e-5
+
+
 def test_so3_register_engine_matches_oracle(po):
     """SO(3) alpha/gamma passes on the register four-step kernels."""
     from phaser_amd.capi import Context
