@@ -8,6 +8,8 @@
 // One LDS round trip per line instead of one per radix stage.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace phz {
@@ -78,6 +80,22 @@ struct RegSplit {
 template <int L>
 struct Twiddles {
   float2 w[RegSplit<L>::R1];
+  __device__ __forceinline__ float2 operator()(int k1) const { return w[k1]; }
+};
+
+// The same twiddles for a wave whose lanes 0-31 are thread n2 = a and lanes
+// 32-63 thread n2 = a + 1, with a uniform over the wave: each W_L^(n2 k1) is
+// a select between two scalar loads, so no vector registers hold the table
+// between transforms.
+template <int L>
+struct PairTwiddles {
+  const float2* __restrict__ tw;
+  int a;
+  bool hi;
+  __device__ __forceinline__ float2 operator()(int k1) const {
+    const float2 wa = tw[a * k1], wb = tw[(a + 1) * k1];
+    return hi ? wb : wa;
+  }
 };
 template <int L>
 __device__ __forceinline__ Twiddles<L> load_twiddles(int n2, const float2* __restrict__ tw) {
@@ -88,17 +106,33 @@ __device__ __forceinline__ Twiddles<L> load_twiddles(int n2, const float2* __res
   return t;
 }
 
+// A kernel whose lines are LINES = 32 per workgroup, line = thread % 32 and
+// n2 = thread / 32, has two threads n2 per wave (lanes 0-31 and 32-63): it
+// can take PairTwiddles; otherwise Twiddles.
+template <int L>
+__device__ __forceinline__ void init_twiddles(Twiddles<L>& w, int n2,
+                                              const float2* __restrict__ tw) {
+  w = load_twiddles<L>(n2, tw);
+}
+template <int L>
+__device__ __forceinline__ void init_twiddles(PairTwiddles<L>& w, int n2,
+                                              const float2* __restrict__ tw) {
+  w.tw = tw;
+  w.a = __builtin_amdgcn_readfirstlane(n2 & ~1);
+  w.hi = (n2 & 1) != 0;
+}
+
 // As fft_four_step, with the results kept in registers:
 // out[j * R2 + k2] = X[k1 + R1 k2], k1 = n2 + R2 j.
-template <int DIR, int L>
+template <int DIR, int L, class TW>
 __device__ __forceinline__ void fft_four_step_reg(float2 (&v)[RegSplit<L>::R1], int n2,
-                                                  float2* lbuf, const Twiddles<L>& tw,
+                                                  float2* lbuf, const TW& tw,
                                                   float2 (&out)[RegSplit<L>::R1]) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
   dft_reg<R1, DIR>(v);
 #pragma unroll
   for (int k1 = 1; k1 < R1; ++k1) {
-    float2 w = tw.w[k1];
+    float2 w = tw(k1);
     if (DIR > 0) w.y = -w.y;
     v[k1] = cmul(v[k1], w);
   }
@@ -153,14 +187,14 @@ __device__ __forceinline__ void fft_four_step_reg_strided(float2 (&v)[RegSplit<L
 // scratch (>= L entries).  store(k, X[k]) is called for k = k1 + R1 k2 with
 // k1 = n2 + R2 j, j < R1 / R2.  Contains one __syncthreads, so every thread
 // of the workgroup must call it.
-template <int DIR, int L, class Store>
+template <int DIR, int L, class Store, class TW>
 __device__ __forceinline__ void fft_four_step(float2 (&v)[RegSplit<L>::R1], int n2, float2* lbuf,
-                                              const Twiddles<L>& tw, Store store) {
+                                              const TW& tw, Store store) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
   dft_reg<R1, DIR>(v);
 #pragma unroll
   for (int k1 = 1; k1 < R1; ++k1) {
-    float2 w = tw.w[k1];
+    float2 w = tw(k1);
     if (DIR > 0) w.y = -w.y;
     v[k1] = cmul(v[k1], w);
   }

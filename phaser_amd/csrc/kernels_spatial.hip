@@ -884,7 +884,10 @@ __global__ void __launch_bounds__(256, L <= 128 ? 3 : 2) k_spat_cross_zz_reg(
   const int q = blockIdx.x * LINES + cl;
   const bool live = q < inner;
   const int p = blockIdx.y;
-  const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  // twiddles from scalar loads where a wave holds two n2 (L = 64, 128):
+  // 30 vector registers fewer across the transforms
+  std::conditional_t<LINES == 32, PairTwiddles<L>, Twiddles<L>> w;
+  init_twiddles<L>(w, n2, tw);
   for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
     const uint8_t* o = i < L / 4 ? f_occ : g_occ;
     const int64_t st = i < L / 4 ? f_occ_stride : g_occ_stride;
@@ -1227,10 +1230,9 @@ __global__ void __launch_bounds__(L * RegSplit<L>::R2) k_spat_cross_slab(
 // forward FFT along z (occupied planes only), folded into the fusion state
 // in channel order as fuse_at does: mode 0 keeps the first channel of
 // largest energy, 1 sums ((c0 + c1) + c2) + c3 and divides by C, 2 is zero.
-template <int L>
+template <int L, class TW>
 __device__ __forceinline__ void fused_side4(const ChannelSet& X, int64_t off, const uint8_t* occ,
-                                            int q, bool live, int n2, float2* lb,
-                                            const Twiddles<L>& w,
+                                            int q, bool live, int n2, float2* lb, const TW& w,
                                             const int (&mode)[RegSplit<L>::R1],
                                             float2 (&acc)[RegSplit<L>::R1]) {
   constexpr int R1 = RegSplit<L>::R1, R2 = RegSplit<L>::R2;
@@ -1291,7 +1293,10 @@ __global__ void __launch_bounds__(256, 2) k_spat_cross_zz4_reg(
   const int q = blockIdx.x * LINES + cl;
   const bool live = q < inner;
   const int p = blockIdx.y;
-  const Twiddles<L> w = load_twiddles<L>(n2, tw);
+  // twiddles from scalar loads where a wave holds two n2 (L = 64, 128):
+  // 30 vector registers fewer across the transforms
+  std::conditional_t<LINES == 32, PairTwiddles<L>, Twiddles<L>> w;
+  init_twiddles<L>(w, n2, tw);
   for (int i = threadIdx.x; i < 2 * L / 4; i += blockDim.x) {
     const uint8_t* o = i < L / 4 ? f_occ : g_occ;
     const int64_t st = i < L / 4 ? f_occ_stride : g_occ_stride;
