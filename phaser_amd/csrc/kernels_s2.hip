@@ -71,11 +71,14 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
 }
 
 // exclusive scan of the n*n cell counts of a cloud (one 1024-thread
-// workgroup per cloud, ncell <= 1024 * PER): every count is loaded up front,
-// chunk k holding cells 1024 k + thread (coalesced); wave scans per chunk,
-// one wave scans the PER x 16 wave totals in (chunk, wave) order, and the
-// starts are written back, also coalesced.  The counts are zeroed for reuse
-// as the scatter cursors; band_start[b] = start of cell b ns.
+// workgroup per cloud, ncell <= 1024 * PER): chunk k holds cells 1024 k +
+// thread (coalesced); wave scans per chunk, one wave scans the PER x 16 wave
+// totals in (chunk, wave) order, and the starts are written back, also
+// coalesced.  PER <= 16: every count is loaded up front and the within-wave
+// sums stay in registers; larger PER (B >= 128): 16 chunks at a time, the
+// within-wave sums parked in cell_start and the wave offsets added in a
+// second pass (registers for all of them would spill).  The counts are
+// zeroed for reuse as the scatter cursors; band_start[b] = start of cell b ns.
 template <int PER>
 __global__ void __launch_bounds__(1024) k_cell_scan(int nb, int ns, int* __restrict__ cell_count,
                                                     int* __restrict__ cell_start,
@@ -86,35 +89,55 @@ __global__ void __launch_bounds__(1024) k_cell_scan(int nb, int ns, int* __restr
   int* st = cell_start + (int64_t)c * (ncell + 1);
   int* bs = band_start + (int64_t)c * (nb + 1);
   constexpr int kW = 16;  // waves per workgroup
+  constexpr bool kHold = PER <= 16;
   __shared__ int wsum[PER * kW];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int v[PER];  // the counts, then their exclusive sums within the wave
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int i = k * 1024 + threadIdx.x;
-    v[k] = i < ncell ? cnt[i] : 0;
-  }
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int own = v[k];
+  // within-wave exclusive sum of one chunk; the wave's total into wsum
+  auto wave_scan = [&](int k, int own) {
     int y = own;
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       const int t = __shfl_up(y, off, 64);
       if (lane >= off) y += t;
     }
-    v[k] = y - own;
     if (lane == 63) wsum[k * kW + wv] = y;
+    return y - own;
+  };
+  int v[kHold ? PER : 1];  // the counts, then their exclusive sums within the wave
+  if constexpr (kHold) {
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int i = k * 1024 + threadIdx.x;
+      v[k] = i < ncell ? cnt[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) v[k] = wave_scan(k, v[k]);
+  } else {
+#pragma unroll 1
+    for (int k0 = 0; k0 < PER; k0 += 16) {
+      int u[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int i = (k0 + j) * 1024 + threadIdx.x;
+        u[j] = (k0 + j < PER && i < ncell) ? cnt[i] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        if (k0 + j >= PER) break;
+        const int i = (k0 + j) * 1024 + threadIdx.x;
+        const int e = wave_scan(k0 + j, u[j]);
+        if (i < ncell) st[i] = e;
+      }
+    }
   }
   __syncthreads();
   if (wv == 0) {
     constexpr int E = (PER * kW + 63) / 64;  // totals per lane
-    int t[E], sum = 0;
+    int sum = 0;
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int idx = lane * E + j;
-      t[j] = idx < PER * kW ? wsum[idx] : 0;
-      sum += t[j];
+      sum += idx < PER * kW ? wsum[idx] : 0;
     }
     int incl = sum;
 #pragma unroll
@@ -126,8 +149,11 @@ __global__ void __launch_bounds__(1024) k_cell_scan(int nb, int ns, int* __restr
 #pragma unroll
     for (int j = 0; j < E; ++j) {
       const int idx = lane * E + j;
-      if (idx < PER * kW) wsum[idx] = acc;
-      acc += t[j];
+      if (idx < PER * kW) {
+        const int t = wsum[idx];
+        wsum[idx] = acc;
+        acc += t;
+      }
     }
     if (lane == 63) {
       st[ncell] = incl;
@@ -135,14 +161,23 @@ __global__ void __launch_bounds__(1024) k_cell_scan(int nb, int ns, int* __restr
     }
   }
   __syncthreads();
+  auto finish = [&](int k, int i, int excl) {
+    const int start = wsum[k * kW + wv] + excl;
+    st[i] = start;
+    cnt[i] = 0;  // reused as the scatter cursor
+    if (i % ns == 0) bs[i / ns] = start;
+  };
+  if constexpr (kHold) {
 #pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    const int i = k * 1024 + threadIdx.x;
-    if (i < ncell) {
-      const int start = wsum[k * kW + wv] + v[k];
-      st[i] = start;
-      cnt[i] = 0;  // reused as the scatter cursor
-      if (i % ns == 0) bs[i / ns] = start;
+    for (int k = 0; k < PER; ++k) {
+      const int i = k * 1024 + threadIdx.x;
+      if (i < ncell) finish(k, i, v[k]);
+    }
+  } else {
+#pragma unroll 4
+    for (int k = 0; k < PER; ++k) {
+      const int i = k * 1024 + threadIdx.x;
+      if (i < ncell) finish(k, i, st[i]);  // the parked within-wave sum
     }
   }
 }
