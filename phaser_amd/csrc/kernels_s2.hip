@@ -558,22 +558,27 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       thr = t * t + kBoundErr;
     }
   };
+  // next band offsets up (increasing colatitude index) and down
+  int hu = 1, hd = 1;
   for (int t = 0; t < 2 * nb && (up || down); ++t) {
-    // band of step t and whether this query still walks in its direction;
-    // the direction's flag clears past the grid or once the band's bound
+    // band of step t: the own band, then alternately up and down (up
+    // first) while both directions walk, else the one still walking (a
+    // finished direction takes no steps; the bands visited, their order
+    // and every result are those of strict alternation with idle steps).
+    // A direction's flag clears past the grid or once the band's bound
     // exceeds the threshold (every farther band that way is farther still).
     // Decided with selects, then one branch around the band's work.
-    const bool odd = (t & 1) != 0;
-    const int h = (t + 1) >> 1;
-    const int braw = t == 0 ? qi : (odd ? qi + h : qi - h);
+    const bool goup = t != 0 && up && (!down || hu <= hd);
+    const int braw = t == 0 ? qi : (goup ? qi + hu : qi - hd);
+    hu += goup ? 1 : 0;
+    hd += (t != 0 && !goup) ? 1 : 0;
     const bool inr = braw >= 0 && braw < nb;
-    const bool dir = t == 0 || (odd ? up : down);
     const int b = inr ? braw : qi;
     const float A = atab[(qrow - r0) * nb + b];
     const bool far = !inr || A > thr;
-    up = (far && dir && (t == 0 || odd)) ? false : up;
-    down = (far && dir && (t == 0 || !odd)) ? false : down;
-    if (!(dir && !far && bs[b + 1] != bs[b])) continue;  // empty band (LDS)
+    up = (far && (t == 0 || goup)) ? false : up;
+    down = (far && (t == 0 || !goup)) ? false : down;
+    if (far || bs[b + 1] == bs[b]) continue;  // out of reach, or an empty band (LDS)
     const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
     const int cb = b * ns;
