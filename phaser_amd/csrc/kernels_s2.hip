@@ -457,6 +457,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // kMaxSectors) and the bands' occupancy words, once per workgroup
   __shared__ float band_sin_min[kMaxSectors];
   __shared__ int bs[kMaxSectors + 1];
+  // per band the nearest non-empty band at or after it (nb: none) and at or
+  // before it (-1: none), so that the walk over bands skips empty runs
+  __shared__ short ne_up[kMaxSectors], ne_dn[kMaxSectors];
+  __shared__ unsigned long long bandbits[kMaskWords];
   // the azimuth term of the cell bound by sector distance u = 0 .. ns / 2:
   // 2 sin^2(dphi / 2), dphi = the azimuth distance from a sector's centre to
   // the nearest edge of the sector u away (queries sit on sector centres)
@@ -488,6 +492,40 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   }
   for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x)
     bs[bb] = band_start[(int64_t)c * (nb + 1) + bb];
+  __syncthreads();
+  // band occupancy bits (one ballot per 64 bands), then the two tables
+  for (int w0 = threadIdx.x & ~63; w0 < nb; w0 += blockDim.x) {
+    const int bb = w0 + (threadIdx.x & 63);
+    const unsigned long long m = __ballot(bb < nb && bs[bb + 1] != bs[bb]);
+    if ((threadIdx.x & 63) == 0) bandbits[w0 >> 6] = m;
+  }
+  __syncthreads();
+  for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
+    const int nwb = (nb + 63) >> 6, w = bb >> 6, o = bb & 63;
+    int up_b = nb, dn_b = -1;
+    const unsigned long long hi = bandbits[w] & (~0ull << o);
+    if (hi) {
+      up_b = (w << 6) + __builtin_ctzll(hi);
+    } else {
+      for (int k = w + 1; k < nwb; ++k)
+        if (bandbits[k]) {
+          up_b = (k << 6) + __builtin_ctzll(bandbits[k]);
+          break;
+        }
+    }
+    const unsigned long long lo = bandbits[w] & (o == 63 ? ~0ull : ((1ull << (o + 1)) - 1ull));
+    if (lo) {
+      dn_b = (w << 6) + 63 - __builtin_clzll(lo);
+    } else {
+      for (int k = w - 1; k >= 0; --k)
+        if (bandbits[k]) {
+          dn_b = (k << 6) + 63 - __builtin_clzll(bandbits[k]);
+          break;
+        }
+    }
+    ne_up[bb] = (short)up_b;
+    ne_dn[bb] = (short)dn_b;
+  }
   if (seeds.use_mask)
     for (int i = threadIdx.x; i < nb * nw; i += blockDim.x)
       bmask[i] = seeds.mask[(int64_t)c * nb * nw + i];
@@ -558,20 +596,29 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       thr = t * t + kBoundErr;
     }
   };
-  // next band offsets up (increasing colatitude index) and down
-  int hu = 1, hd = 1;
+  // offset of the next non-empty band up (increasing colatitude index) and
+  // down at or beyond offset h; past the grid when there is none
+  auto up_off = [&](int h) { return (qi + h < nb ? (int)ne_up[qi + h] : nb) - qi; };
+  auto dn_off = [&](int h) { return qi - (qi - h >= 0 ? (int)ne_dn[qi - h] : -1); };
+  int hu = up_off(1), hd = dn_off(1);
   for (int t = 0; t < 2 * nb && (up || down); ++t) {
-    // band of step t: the own band, then alternately up and down (up
-    // first) while both directions walk, else the one still walking (a
-    // finished direction takes no steps; the bands visited, their order
-    // and every result are those of strict alternation with idle steps).
-    // A direction's flag clears past the grid or once the band's bound
-    // exceeds the threshold (every farther band that way is farther still).
-    // Decided with selects, then one branch around the band's work.
+    // band of step t: the own band, then the nearer of the next non-empty
+    // bands up and down (up on a tie) while both directions walk, else the
+    // next one of the direction still walking.  Empty bands take no steps:
+    // the bands searched, their order and every result are those of a
+    // strict up / down alternation over all bands.  A direction's flag
+    // clears past the grid or once the band's bound exceeds the threshold
+    // (the bound grows with the band offset, so every farther band that
+    // way is farther still).  Decided with selects, then one branch around
+    // the band's work.
     const bool goup = t != 0 && up && (!down || hu <= hd);
     const int braw = t == 0 ? qi : (goup ? qi + hu : qi - hd);
-    hu += goup ? 1 : 0;
-    hd += (t != 0 && !goup) ? 1 : 0;
+    if (t != 0) {
+      if (goup)
+        hu = up_off(hu + 1);
+      else
+        hd = dn_off(hd + 1);
+    }
     const bool inr = braw >= 0 && braw < nb;
     const int b = inr ? braw : qi;
     const float A = atab[(qrow - r0) * nb + b];
