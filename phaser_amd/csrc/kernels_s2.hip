@@ -46,6 +46,9 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
                           float2* __restrict__ val, int* __restrict__ cell,
                           int* __restrict__ cell_count) {
   const int c = blockIdx.y;
+  // the grid covers cap points per cloud: workgroups past the cloud's
+  // points leave at once (uniformly) instead of projecting dummy points
+  if ((int64_t)blockIdx.x * blockDim.x >= npts[c]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < npts[c];
   const float4 p = valid ? pts[c * cap + i] : make_float4(0.f, 0.f, 1.f, 0.f);
@@ -187,6 +190,7 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
                                const int* __restrict__ cell, int* __restrict__ cursor,
                                const int* __restrict__ cell_start, float4* __restrict__ sorted) {
   const int c = blockIdx.y;
+  if ((int64_t)blockIdx.x * blockDim.x >= npts[c]) return;  // past the cloud (uniform)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   const bool valid = i < npts[c];

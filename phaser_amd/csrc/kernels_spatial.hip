@@ -63,6 +63,9 @@ __global__ void k_voxel_scatter(const float4* __restrict__ pts, const int* __res
                                 VoxelOcc occ, const float2* __restrict__ extra,
                                 float* __restrict__ sumF, float* __restrict__ sumA) {
 #pragma clang fp contract(off)
+  // the grid covers cap points per cloud: workgroups past the cloud's
+  // points leave before staging the edges (uniform over the workgroup)
+  if ((int64_t)blockIdx.x * blockDim.x >= npts[blockIdx.y]) return;
   extern __shared__ float E[];
   for (int i = threadIdx.x; i < N; i += blockDim.x) E[i] = edges[i];
   __syncthreads();
