@@ -556,8 +556,16 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // pruning threshold: squared (distance + slack) plus kBoundErr, the
   // largest error of the bounds below as computed with the hardware sine
   // (v_sin_f32, absolute error below 2^-20), so pruning stays exact;
-  // recomputed only when the best changes
-  constexpr float kBoundErr = 3e-5f;
+  // recomputed only when the best changes.  The square root is the
+  // hardware's (v_sqrt_f32, within 1 ulp: below 2.4e-7 for the chord
+  // distances < 2 here) plus kSqrtErr = 1e-6, so the threshold is never below
+  // the one a correctly rounded root gives and pruning stays exact; the
+  // correctly rounded sqrtf costs about 14 VALU instructions per update.
+  constexpr float kBoundErr = 3e-5f, kSqrtErr = 1e-6f;
+  auto thr_of = [&](float d) {
+    const float t = __builtin_amdgcn_sqrtf(d) + (slack + kSqrtErr);
+    return t * t + kBoundErr;
+  };
   float thr = INFINITY;
   // the coarse cell's seed point: a point of the cloud, so its distance
   // bounds the exact nearest one from above and the walk below (unchanged,
@@ -568,8 +576,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       const float4 pv = sp[pos];
       const float d = l2_simple(qv, pv);
       bkey = ((unsigned long long)__float_as_uint(d) << 32) | (unsigned)__float_as_int(pv.w);
-      const float t = sqrtf(d) + slack;
-      thr = t * t + kBoundErr;
+      thr = thr_of(d);
     }
   }
   // x in [0, 2 ns) reduced mod ns without an integer division
@@ -595,10 +602,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       }
     }
     agree_best<LANES>(bkey);
-    if (bkey != ~0ull) {
-      const float t = sqrtf(__uint_as_float((unsigned)(bkey >> 32))) + slack;
-      thr = t * t + kBoundErr;
-    }
+    if (bkey != ~0ull) thr = thr_of(__uint_as_float((unsigned)(bkey >> 32)));
   };
   // offset of the next non-empty band up (increasing colatitude index) and
   // down at or beyond offset h; past the grid when there is none
