@@ -50,8 +50,9 @@ inline int seed_factor(int n) {
 // nearest occupied sectors are already out of reach are skipped on the
 // band's occupancy bits in LDS, without a memory round trip.
 // phz_tuning.nn_seed's default: neither (measured at 64 pairs per launch on
-// gfx950, re-cast targets, after the walk's other changes: the NN stage 0.73
-// ms plain, 0.81 ms with seeds, 0.73 ms with the mask, 0.81 ms with both)
+// gfx950, re-cast targets, with the band walk that jumps empty bands: the NN
+// stage 0.566 ms plain, 0.654 ms with seeds, 0.571 ms with the mask, 0.649 ms
+// with both)
 constexpr int kNnSeedDefault = 0;
 struct NnSeeds {
   const int* pos = nullptr;
