@@ -71,6 +71,26 @@ __device__ __forceinline__ WaveRun wave_run(long long key) {
   return WaveRun{hl, next - hl, head};
 }
 
+// Maximum of an unsigned value over the 64 lanes of a wave (every lane must
+// call), by DPP row operations instead of LDS-routed shuffles: within quads
+// ([1,0,3,2], [2,3,0,1]), half rows (row_half_mirror), rows (row_mirror),
+// then row 1 and row 3 take the last lane of the row below (row_bcast:15)
+// and rows 2 and 3 that of row 1 (row_bcast:31), so that lane 63 holds the
+// maximum; lanes a step does not write keep their value (max is idempotent).
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, dpp_u32<0xB1>(v));
+  v = max(v, dpp_u32<0x4E>(v));
+  v = max(v, dpp_u32<0x141>(v));
+  v = max(v, dpp_u32<0x140>(v));
+  v = max(v, dpp_u32<0x142, 0xA>(v));
+  v = max(v, dpp_u32<0x143, 0xC>(v));
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 // ---------------------------------------------------------------- argmax keys
 // key = (ordered value bits << 32) | (0xffffffff - index): atomicMax keeps the
 // largest value and, among equal values, the smallest index.

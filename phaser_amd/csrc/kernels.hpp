@@ -16,7 +16,11 @@ namespace phz {
 // (theta) x n sectors (phi centred on the query columns).
 struct CellBuffers {
   int nb, ns;        // colatitude bands x azimuth sectors of the search grid
-  int* cell_count;   // nclouds * n*n
+  int* cell_count;   // nclouds * n*n, then the band colatitude ranges below
+  // nclouds * nb * 2 past the counts (cleared with them): per band the
+  // complement of the smallest and the largest colatitude bits of its points
+  // (non-negative floats order as their bits; 0 = none yet)
+  unsigned* band_th;
   int* cell_start;   // nclouds * (n*n + 1)
   int* band_start;   // nclouds * (n + 1)
   int4* next_right;  // nclouds * n*n: nearest non-empty sector to the right and

@@ -41,10 +41,19 @@ __device__ __forceinline__ void project_point(float x, float y, float z, float* 
   *range = (float)dist;
 }
 
-__global__ void k_project(const float4* __restrict__ pts, const int* __restrict__ npts,
-                          int64_t cap, int nb, int ns, float4* __restrict__ proj,
-                          float2* __restrict__ val, int* __restrict__ cell,
-                          int* __restrict__ cell_count) {
+// The S^2 search grid has at most kMaxSectors bands and sectors (n = 2B,
+// B <= 160).
+constexpr int kMaxSectors = 320;
+constexpr int kMaskWords = (kMaxSectors + 63) / 64;
+
+// k_project's workgroup: the band ranges of its 16 waves are merged in LDS
+// before the memory atomics (one pair per band per workgroup)
+constexpr int kProjectThreads = 1024;
+
+__global__ void __launch_bounds__(kProjectThreads)
+k_project(const float4* __restrict__ pts, const int* __restrict__ npts, int64_t cap, int nb, int ns,
+          float4* __restrict__ proj, float2* __restrict__ val, int* __restrict__ cell,
+          int* __restrict__ cell_count, unsigned* __restrict__ band_th) {
   const int c = blockIdx.y;
   // the grid covers cap points per cloud: workgroups past the cloud's
   // points leave at once (uniformly) instead of projecting dummy points
@@ -66,11 +75,42 @@ __global__ void k_project(const float4* __restrict__ pts, const int* __restrict_
   const int cid = band * ns + sec;
   // scans arrive in ring order: one count atomic per run of equal cells
   const WaveRun run = wave_run(valid ? (long long)cid : -1ll - (threadIdx.x & 63));
-  if (!valid) return;
-  proj[c * cap + i] = make_float4(px, py, pz, __int_as_float(i));
-  val[c * cap + i] = make_float2(r, p.w);
-  cell[c * cap + i] = cid;
-  if (run.head) atomicAdd(&cell_count[(int64_t)c * nb * ns + cid], run.len);
+  if (valid) {
+    proj[c * cap + i] = make_float4(px, py, pz, __int_as_float(i));
+    val[c * cap + i] = make_float2(r, p.w);
+    cell[c * cap + i] = cid;
+    if (run.head) atomicAdd(&cell_count[(int64_t)c * nb * ns + cid], run.len);
+  }
+  // each band's colatitude range (the search bounds a band by it rather than
+  // by its edges), kept as the complement of the smallest and the largest
+  // colatitude bits of its points (th >= 0 orders as its bits; both are
+  // maxima, 0 = no point yet): per band present in a wave (a ring's points
+  // share one or two), wave-wide maxima into the workgroup's LDS table, then
+  // one memory atomic pair per band the workgroup holds
+  __shared__ unsigned sth[2 * kMaxSectors];
+  for (int k = threadIdx.x; k < 2 * nb; k += blockDim.x) sth[k] = 0u;
+  __syncthreads();
+  {
+    const unsigned tb = __float_as_uint(th);
+    unsigned long long todo = __ballot(valid);
+    while (todo) {
+      const int lead = __builtin_ctzll(todo);
+      const int bl = __builtin_amdgcn_readlane(band, lead);
+      const bool mine = valid && band == bl;
+      todo &= ~__ballot(mine);
+      const unsigned cmn = wave_max_u32(mine ? ~tb : 0u);
+      const unsigned mx = wave_max_u32(mine ? tb : 0u);
+      if ((threadIdx.x & 63) == lead) {
+        atomicMax(&sth[2 * bl], cmn);
+        atomicMax(&sth[2 * bl + 1], mx);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < 2 * nb; k += blockDim.x) {
+    const unsigned v = sth[k];
+    if (v != 0u) atomicMax(&band_th[(int64_t)c * nb * 2 + k], v);
+  }
 }
 
 // exclusive scan of the n*n cell counts of a cloud (one 1024-thread
@@ -206,10 +246,6 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
   sorted[c * cap + dst] = proj[c * cap + i];
 }
 
-// The S^2 search grid has at most kMaxSectors bands and sectors (n = 2B,
-// B <= 160).
-constexpr int kMaxSectors = 320;
-constexpr int kMaskWords = (kMaxSectors + 63) / 64;
 
 // per cell: nearest non-empty sector of the same band to the right (>= s)
 // and to the left (<= s), cyclic; -1 in an empty band.  One wave per band:
@@ -417,7 +453,11 @@ __device__ __forceinline__ int mask_prev(const unsigned long long* m, int nw, in
 // its lower bound on the squared chord distance,
 //   |q - p|^2 >= 2(1 - cos dth) + 2 sin(th_q) min sin(th_p) (1 - cos dphi),
 // exceeds the best distance found (with a float slack): points outside it are
-// provably farther.  Empty sectors are jumped over with the per-cell tables,
+// provably farther.  dth and min sin(th_p) are taken over the colatitudes of
+// the band's points (their range, from the projection), so a band whose
+// points lie far from the query's colatitude within it is skipped whole; a
+// direction's walk over bands ends on the bound from the band's edges, which
+// grows with the band offset.  Empty sectors are jumped over with the per-cell tables,
 // empty bands with the cloud's band table (in LDS).  The LANES lanes of a
 // query split each cell's points and agree on the best after it.  Loads are
 // batched so that a walk step costs about one memory round trip: the points
@@ -437,7 +477,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
                      const float2* __restrict__ vals, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
-                     float* __restrict__ xgrids, int xcd_blocks, int blend, NnSeeds seeds) {
+                     float* __restrict__ xgrids, int xcd_blocks, int blend, NnSeeds seeds,
+                     const unsigned* __restrict__ band_th) {
   constexpr int kBatch = 2;  // point loads in flight per lane (measured: 1, 2, 4 -> 0.78, 0.71, 0.75 ms at 64 pairs)
   // xcd_blocks > 0: a 1-D grid of xcd_blocks query blocks per cloud, where
   // consecutive workgroups go to the 8 XCDs in turn: XCD x takes clouds
@@ -457,7 +498,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   const int q = qb * (blockDim.x / LANES) + threadIdx.x / LANES;
   const int nq = n * n, ncell = nb * ns;
   const int nw = (ns + 63) >> 6;
-  // min sin(theta) over each band, the cloud's band table (nb <=
+  // min sin(theta) over each band's points, the cloud's band table (nb <=
   // kMaxSectors) and the bands' occupancy words, once per workgroup
   __shared__ float band_sin_min[kMaxSectors];
   __shared__ int bs[kMaxSectors + 1];
@@ -471,22 +512,45 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   __shared__ float wsec[kMaxSectors / 2 + 1];
   // the colatitude term of the cell bound, 4 sin^2(dth / 2), per band for
   // each query row of this workgroup (rows r0 .. r0 + nrows - 1; at most
-  // blockDim / n + 2 rows of nb bands, n >= 16)
+  // blockDim / n + 2 rows of nb bands, n >= 16): atab from the band's edges,
+  // which grows with the band offset and so ends a direction's walk; aact
+  // from the colatitude range of the band's points, which skips a band
   __shared__ float atab[2 * kMaxSectors + 256];
+  __shared__ float aact[2 * kMaxSectors + 256];
   const int qpb = blockDim.x / LANES, q0 = qb * qpb;
   const int r0 = q0 / n, nrows = min(q0 + qpb, nq) > q0 ? (min(q0 + qpb, nq) - 1) / n - r0 + 1 : 0;
   extern __shared__ unsigned long long bmask[];  // nb * nw words with the mask, else none
+  // band bb's colatitude range: its edges, or within them the range of its
+  // points' colatitudes, each widened by 1e-5 (the error of a point's
+  // computed colatitude, and of the band it is sorted into)
+  auto band_edges = [&](int bb, float& lo, float& hi) {
+    lo = bb * (kPi / nb) - 1e-5f;
+    hi = (bb + 1) * (kPi / nb) + 1e-5f;
+  };
+  auto band_range = [&](int bb, float& lo, float& hi) {
+    band_edges(bb, lo, hi);
+    const unsigned* bt = band_th + ((int64_t)c * nb + bb) * 2;
+    const unsigned mnc = bt[0], mx = bt[1];
+    if (mnc != 0u) {
+      lo = fmaxf(lo, __uint_as_float(~mnc) - 1e-5f);
+      hi = fminf(hi, __uint_as_float(mx) + 1e-5f);
+    }
+  };
   for (int bb = threadIdx.x; bb < nb; bb += blockDim.x) {
-    const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
+    float lo, hi;
+    band_range(bb, lo, hi);
     band_sin_min[bb] = fmaxf(0.f, fminf(sinf(fmaxf(lo, 0.f)), sinf(fminf(hi, kPi))));
   }
   for (int i = threadIdx.x; i < nrows * nb; i += blockDim.x) {
     const int rr = i / nb, bb = i - rr * nb;
     const float th = theta[r0 + rr];
-    const float lo = bb * (kPi / nb) - 1e-5f, hi = (bb + 1) * (kPi / nb) + 1e-5f;
-    const float dth = fmaxf(0.f, fmaxf(lo - th, th - hi));
-    const float s1 = __sinf(0.5f * dth);
+    float lo, hi;
+    band_edges(bb, lo, hi);
+    const float s1 = __sinf(0.5f * fmaxf(0.f, fmaxf(lo - th, th - hi)));
     atab[i] = 4.f * s1 * s1;
+    band_range(bb, lo, hi);
+    const float s2 = __sinf(0.5f * fmaxf(0.f, fmaxf(lo - th, th - hi)));
+    aact[i] = 4.f * s2 * s2;
   }
   for (int u = threadIdx.x; u <= ns / 2; u += blockDim.x) {
     const float dsec_ = 2.f * kPi / ns;
@@ -629,11 +693,16 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     }
     const bool inr = braw >= 0 && braw < nb;
     const int b = inr ? braw : qi;
-    const float A = atab[(qrow - r0) * nb + b];
-    const bool far = !inr || A > thr;
+    const float Ae = atab[(qrow - r0) * nb + b];
+    const bool far = !inr || Ae > thr;
     up = (far && (t == 0 || goup)) ? false : up;
     down = (far && (t == 0 || !goup)) ? false : down;
     if (far || bs[b + 1] == bs[b]) continue;  // out of reach, or an empty band (LDS)
+    // the band's points out of reach (their colatitudes, not its edges):
+    // skipped, while the walk goes on (a farther band's points may lie
+    // nearer the band edge)
+    const float A = aact[(qrow - r0) * nb + b];
+    if (A > thr) continue;
     const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
     const int cb = b * ns;
@@ -827,9 +896,11 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   (void)n;
   const int ncell = cb.nb * cb.ns;
   launch_fill_u32((uint32_t*)cb.cell_count, (size_t)nclouds * ncell, 0u, s);
+  launch_fill_u32(cb.band_th, (size_t)nclouds * cb.nb * 2, 0u, s);
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
-  hipLaunchKernelGGL(k_project, g, dim3(256), 0, s, pts, npts, cap, cb.nb, cb.ns, cb.proj, cb.val,
-                     cb.cell, cb.cell_count);
+  dim3 gp((unsigned)((cap + kProjectThreads - 1) / kProjectThreads), nclouds);
+  hipLaunchKernelGGL(k_project, gp, dim3(kProjectThreads), 0, s, pts, npts, cap, cb.nb, cb.ns,
+                     cb.proj, cb.val, cb.cell, cb.cell_count, cb.band_th);
   if (ncell <= 4 * 1024)
     hipLaunchKernelGGL(k_cell_scan<4>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
                        cb.cell_count, cb.cell_start, cb.band_start);
@@ -861,7 +932,7 @@ static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, con
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), lds, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
                      cb.val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
-                     seeds);
+                     seeds, cb.band_th);
 }
 
 int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
