@@ -42,18 +42,24 @@ WORKLOAD = dict(rings=128, azimuths=1024, noise_sigma=0.02, overlap=0.7)
 
 
 def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False,
-                 lines_fwd=False, lines_inv=False, nn_lanes=4):
+                 lines_fwd=False, lines_inv=False, nn_lanes=4, occ=False, inv_plane=False,
+                 so3_reg=False):
     """Algorithmic (compulsory) HBM bytes of ONE launch of the kernel that
     dominates each stage, for a batch of `pairs` registrations: inputs read
     once, outputs written once, FP32 / complex64 elements, C = 2 populated
     channels (range/intensity), H = N^2 (N/2+1) half-spectrum entries.
-    Returns {stage: (kernel name, bytes per launch)}."""
+    Returns {stage: (kernel name, bytes per launch)}.  The names follow the
+    tuned pipeline: with occ (occupancy passes, fused register kernels) the
+    forward x pass is k_fft_x_occ, the r2c k_spat_r2c_rows and the cross
+    spectrum k_spat_cross_zz_reg, which read only the voxel rows / z planes
+    that hold points, so their measured traffic is below these dense bytes;
+    with inv_plane the inverse x + c2r is k_spat_inv_plane."""
     n = 2 * B
     C = 2
     H = N * N * (N // 2 + 1)
     nt = 1 if shared else pairs
     clouds = nt + pairs
-    return {
+    out = {
         # S^2 sampling per cloud: its points read once (16 B each) and its C
         # grids written once (the query grid and the kernel's own sorted
         # layout are not counted)
@@ -80,6 +86,17 @@ def kernel_bytes(B, N, P_mean, pairs, shared=False, plane=False, fused_reg=False
         "spatial_c2r": ("k_spat_c2r_reg" if fused_reg else "k_spat_c2r",
                         pairs * (8 * H + 4 * N ** 3)),
     }
+    rename = {}
+    if so3_reg:
+        rename.update(so3_fft_alpha="k_so3_fft_alpha_reg", so3_fft_gamma="k_so3_fft_gamma_reg")
+    if occ and fused_reg:
+        rename.update(spatial_r2c="k_spat_r2c_rows", spatial_fft_fwd="k_fft_x_occ",
+                      spatial_cross_z="k_spat_cross_zz_reg")
+    if inv_plane and fused_reg:
+        rename.update(spatial_c2r="k_spat_inv_plane")
+    for stage, name in rename.items():
+        out[stage] = (name, out[stage][1])
+    return out
 
 
 FORM_NAMES = {0: "plain fp32", 1: "split fp16 mfma contractions"}
@@ -451,7 +468,8 @@ def main():
     lanes = tuning["nn_lanes"] or (2 if 2 * P >= 64 else 4)
     kb = kernel_bytes(B, N, P_mean, P, plane=plane, fused_reg=tuning["fused_reg"] == 1,
                       lines_fwd=tuning["fwd_z"] == 1, lines_inv=tuning["inv_x"] == 1,
-                      nn_lanes=lanes)
+                      nn_lanes=lanes, occ=tuning["occ_fwd"] == 1 and not plane,
+                      inv_plane=tuning["inv_plane"] == 1, so3_reg=tuning["so3_reg"] == 1)
     dom = max((k for k in kb if st.get(k, (0, 0))[1] > 0), key=lambda k: st[k][0])
     per_call = dict(LAUNCHES, spatial_fft_fwd=1 if plane else 2)
     launches = st[dom][1] * per_call.get(dom, 1)
