@@ -513,11 +513,11 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   __shared__ float wsec[kMaxSectors / 2 + 1];
   // the colatitude term of the cell bound, 4 sin^2(dth / 2), per band for
   // each query row of this workgroup (rows r0 .. r0 + nrows - 1; at most
-  // blockDim / n + 2 rows of nb bands, n >= 16): atab from the band's edges,
-  // which grows with the band offset and so ends a direction's walk; aact
-  // from the colatitude range of the band's points, which skips a band
-  __shared__ float atab[2 * kMaxSectors + 256];
-  __shared__ float aact[2 * kMaxSectors + 256];
+  // blockDim / n + 2 rows of nb bands, n >= 16): .x from the band's edges,
+  // which grows with the band offset and so ends a direction's walk; .y
+  // from the colatitude range of the band's points, which skips a band (one
+  // LDS read for both)
+  __shared__ float2 atab[2 * kMaxSectors + 256];
   const int qpb = blockDim.x / LANES, q0 = qb * qpb;
   const int r0 = q0 / n, nrows = min(q0 + qpb, nq) > q0 ? (min(q0 + qpb, nq) - 1) / n - r0 + 1 : 0;
   extern __shared__ unsigned long long bmask[];  // nb * nw words with the mask, else none
@@ -548,10 +548,9 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     float lo, hi;
     band_edges(bb, lo, hi);
     const float s1 = __sinf(0.5f * fmaxf(0.f, fmaxf(lo - th, th - hi)));
-    atab[i] = 4.f * s1 * s1;
     band_range(bb, lo, hi);
     const float s2 = __sinf(0.5f * fmaxf(0.f, fmaxf(lo - th, th - hi)));
-    aact[i] = 4.f * s2 * s2;
+    atab[i] = make_float2(4.f * s1 * s1, 4.f * s2 * s2);
   }
   for (int u = threadIdx.x; u <= ns / 2; u += blockDim.x) {
     const float dsec_ = 2.f * kPi / ns;
@@ -694,16 +693,17 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     }
     const bool inr = braw >= 0 && braw < nb;
     const int b = inr ? braw : qi;
-    const float Ae = atab[(qrow - r0) * nb + b];
-    const bool far = !inr || Ae > thr;
+    // the band's table entries in one round of LDS reads
+    const float2 Ab = atab[(qrow - r0) * nb + b];
+    const int bs0 = bs[b], bs1 = bs[b + 1];
+    const bool far = !inr || Ab.x > thr;
     up = (far && (t == 0 || goup)) ? false : up;
     down = (far && (t == 0 || !goup)) ? false : down;
-    if (far || bs[b + 1] == bs[b]) continue;  // out of reach, or an empty band (LDS)
-    // the band's points out of reach (their colatitudes, not its edges):
-    // skipped, while the walk goes on (a farther band's points may lie
-    // nearer the band edge)
-    const float A = aact[(qrow - r0) * nb + b];
-    if (A > thr) continue;
+    // out of reach, an empty band, or the band's points out of reach (their
+    // colatitudes, not its edges: skipped while the walk goes on, as a
+    // farther band's points may lie nearer its edge)
+    if (far || bs1 == bs0 || Ab.y > thr) continue;
+    const float A = Ab.y;
     const float spm = band_sin_min[b];
     const float Bc = 2.f * sq * spm;
     const int cb = b * ns;
