@@ -317,7 +317,7 @@ size_t context_bytes(const phz_config& cf) {
   add(M * cap, sizeof(float4));
   add(M, sizeof(int));
   add(M, sizeof(GatherEntry));
-  add(M * (nq + 2 * n), sizeof(int));
+  add(M * (nq + 3 * n), sizeof(int));
   add(M * (nq + 1), sizeof(int));
   add(M * (n + 1), sizeof(int));
   add(M * nq, sizeof(int4));
@@ -502,8 +502,9 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->d_pts, (size_t)M * cap));
   TRY(dalloc(c, &c->d_npts, (size_t)M));
   TRY(dalloc(c, &c->d_gather, (size_t)M));
-  TRY(dalloc(c, &c->cells.cell_count, (size_t)M * (nq + 2 * c->n)));
+  TRY(dalloc(c, &c->cells.cell_count, (size_t)M * (nq + 3 * c->n)));
   c->cells.band_th = (unsigned*)(c->cells.cell_count + (size_t)M * nq);
+  c->cells.band_gap = c->cells.band_th + (size_t)M * 2 * c->n;
   TRY(dalloc(c, &c->cells.cell_start, (size_t)M * (nq + 1)));
   TRY(dalloc(c, &c->cells.band_start, (size_t)M * (c->n + 1)));
   TRY(dalloc(c, &c->cells.next_right, (size_t)M * nq));

@@ -21,6 +21,9 @@ struct CellBuffers {
   // complement of the smallest and the largest colatitude bits of its points
   // (non-negative floats order as their bits; 0 = none yet)
   unsigned* band_th;
+  // nclouds * nb past the band ranges: per band its longest cyclic run of
+  // empty sectors, (length << 16) | first sector (rewritten by every call)
+  unsigned* band_gap;
   int* cell_start;   // nclouds * (n*n + 1)
   int* band_start;   // nclouds * (n + 1)
   int4* next_right;  // nclouds * n*n: nearest non-empty sector to the right and

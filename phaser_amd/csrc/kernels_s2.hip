@@ -251,10 +251,13 @@ __global__ void k_cell_scatter(const int* __restrict__ npts, int64_t cap, int nc
 // per cell: nearest non-empty sector of the same band to the right (>= s)
 // and to the left (<= s), cyclic; -1 in an empty band.  One wave per band:
 // the occupancy of the band's n <= kMaxSectors sectors is a ballot mask of
-// kMaskWords 64-bit words.
+// kMaskWords 64-bit words.  Per band also its longest cyclic run of empty
+// sectors, band_gap = (length << 16) | first sector (0: none, or an empty
+// band), from which the search bounds a query inside the run.
 __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
                               int4* __restrict__ nr, int4* __restrict__ nl,
-                              unsigned long long* __restrict__ band_mask) {
+                              unsigned long long* __restrict__ band_mask,
+                              unsigned* __restrict__ band_gap) {
   // n = sectors per band here
   const int b = blockIdx.x, c = blockIdx.y;
   const int lane = threadIdx.x;  // 64 lanes
@@ -276,16 +279,28 @@ __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
   for (int w = 0; w < kMaskWords; ++w)
     if (lane == w && w < nw) band_mask[((int64_t)c * nb + b) * nw + w] = m[w];
   const bool any = all != 0ull;
+  // first set bit at or after x, cyclically (-1: none)
+  auto next_set = [&](int x) {
+    int r = -1;
+    for (int k = 0; k <= nw && r < 0; ++k) {
+      const int w = (x / 64 + k) % nw;
+      unsigned long long mm = m[w];
+      if (k == 0) mm &= (~0ull) << (x % 64);
+      if (k == nw) mm = m[w] & ((x % 64) ? ((1ull << (x % 64)) - 1ull) : 0ull);
+      if (mm) r = w * 64 + __builtin_ctzll(mm);
+    }
+    return r;
+  };
+  unsigned gap = 0u;
   for (int s = lane; s < n; s += 64) {
     int right = -1, left = -1;
     if (any) {
-      // first set bit at or after s, cyclically
-      for (int k = 0; k <= nw && right < 0; ++k) {
-        const int w = (s / 64 + k) % nw;
-        unsigned long long mm = m[w];
-        if (k == 0) mm &= (~0ull) << (s % 64);
-        if (k == nw) mm = m[w] & ((s % 64) ? ((1ull << (s % 64)) - 1ull) : 0ull);
-        if (mm) right = w * 64 + __builtin_ctzll(mm);
+      right = next_set(s);
+      // the empty run after an occupied sector: up to the next occupied one
+      if (right == s) {
+        const int x = s + 1 < n ? s + 1 : 0, r1 = next_set(x);
+        const int len = (r1 - s - 1 + n) % n;
+        gap = max(gap, ((unsigned)len << 16) | (unsigned)x);
       }
       // last set bit at or before s, cyclically
       for (int k = 0; k <= nw && left < 0; ++k) {
@@ -300,6 +315,8 @@ __global__ void k_jump_tables(int nb, int n, const int* __restrict__ cell_start,
     r[s] = right < 0 ? make_int4(-1, 0, 0, 0) : make_int4(right, st[right], st[right + 1], 0);
     l[s] = left < 0 ? make_int4(-1, 0, 0, 0) : make_int4(left, st[left], st[left + 1], 0);
   }
+  gap = wave_max_u32(gap);
+  if (lane == 0) band_gap[(int64_t)c * nb + b] = gap;
 }
 
 // lane exchange within aligned groups of four: DPP quad permutes
@@ -458,7 +475,9 @@ __device__ __forceinline__ int mask_prev(const unsigned long long* m, int nw, in
 // the band's points (their range, from the projection), so a band whose
 // points lie far from the query's colatitude within it is skipped whole; a
 // direction's walk over bands ends on the bound from the band's edges, which
-// grows with the band offset.  Empty sectors are jumped over with the per-cell tables,
+// grows with the band offset.  A query whose sector lies in the band's
+// longest run of empty sectors skips the band when the run's nearer end is
+// out of reach.  Empty sectors are jumped over with the per-cell tables,
 // empty bands with the cloud's band table (in LDS).  The LANES lanes of a
 // query split each cell's points and agree on the best after it.  Loads are
 // batched so that a walk step costs about one memory round trip: the points
@@ -479,7 +498,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      const float2* __restrict__ vals, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
                      float* __restrict__ xgrids, int xcd_blocks, int blend, NnSeeds seeds,
-                     const unsigned* __restrict__ band_th) {
+                     const unsigned* __restrict__ band_th,
+                     const unsigned* __restrict__ band_gap) {
   constexpr int kBatch = 2;  // point loads in flight per lane (measured: 1, 2, 4 -> 0.78, 0.71, 0.75 ms at 64 pairs)
   // xcd_blocks > 0: a 1-D grid of xcd_blocks query blocks per cloud, where
   // consecutive workgroups go to the 8 XCDs in turn: XCD x takes clouds
@@ -503,6 +523,8 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   // kMaxSectors) and the bands' occupancy words, once per workgroup
   __shared__ float band_sin_min[kMaxSectors];
   __shared__ int bs[kMaxSectors + 1];
+  // per band its longest cyclic run of empty sectors, (length << 16) | first
+  __shared__ unsigned bgap[kMaxSectors];
   // per band the nearest non-empty band at or after it (nb: none) and at or
   // before it (-1: none), so that the walk over bands skips empty runs
   __shared__ short ne_up[kMaxSectors], ne_dn[kMaxSectors];
@@ -558,8 +580,10 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
     const float s2 = __sinf(0.5f * dphi);
     wsec[u] = 2.f * s2 * s2;
   }
-  for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x)
+  for (int bb = threadIdx.x; bb <= nb; bb += blockDim.x) {
     bs[bb] = band_start[(int64_t)c * (nb + 1) + bb];
+    if (bb < nb) bgap[bb] = band_gap[(int64_t)c * nb + bb];
+  }
   __syncthreads();
   // band occupancy bits (one ballot per 64 bands), then the two tables
   for (int w0 = threadIdx.x & ~63; w0 < nb; w0 += blockDim.x) {
@@ -715,6 +739,18 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
       u = u > ns - u ? ns - u : u;
       return A + Bc * wsec[u];
     };
+    {
+      // the query's sector inside the band's longest run of empty sectors:
+      // every occupied sector is at least as far (cyclically) as the
+      // nearer end of the run, u <= ns / 2, so the bound there covers every
+      // point of the band (read in LDS, without the walk's round trip)
+      const unsigned gp = bgap[b];
+      const int glen = (int)(gp >> 16);
+      int off = qj - (int)(gp & 0xffffu);
+      off += off < 0 ? ns : 0;
+      const int u = min(off + 1, glen - off);
+      if (off < glen && A + Bc * wsec[u] > thr) continue;
+    }
     if (seeds.use_mask) {
       // own cell empty and the nearest occupied sector on each side (right
       // at distances 1..ns/2, left at 1..(ns-1)/2, as the walks below) out
@@ -917,7 +953,7 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   hipLaunchKernelGGL(k_cell_scatter, g, dim3(256), 0, s, npts, cap, ncell, cb.proj,
                      cb.cell, cb.cell_count, cb.cell_start, cb.sorted);
   hipLaunchKernelGGL(k_jump_tables, dim3(cb.nb, nclouds), dim3(64), 0, s, cb.nb, cb.ns,
-                     cb.cell_start, cb.next_right, cb.next_left, cb.band_mask);
+                     cb.cell_start, cb.next_right, cb.next_left, cb.band_mask, cb.band_gap);
 }
 
 template <int L>
@@ -933,7 +969,7 @@ static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, con
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), lds, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
                      cb.val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
-                     seeds, cb.band_th);
+                     seeds, cb.band_th, cb.band_gap);
 }
 
 int nn_auto_lanes(int nclouds) { return nclouds >= 64 ? 2 : 4; }
