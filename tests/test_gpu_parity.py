@@ -130,6 +130,75 @@ def test_s2_nearest_is_exact(po, B):
             assert bad == 0
 
 
+def _degenerate_cloud(case, B):
+    """Clouds whose bands and sectors are mostly empty, for the search's band
+    skips: a band's colatitude range far narrower than the band, long runs of
+    empty sectors, points at the poles and rings just either side of a band
+    edge.  Ranges and intensities are distinct, so a sample names its point."""
+    rng = np.random.default_rng(7 + B)
+
+    def dirs(th, ph, r):
+        return np.stack([r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph), r * np.cos(th)], 1)
+
+    if case == "arc_and_poles":
+        k = 500
+        th = np.full(k, np.deg2rad(80.0))
+        ph = rng.uniform(0, np.deg2rad(100.0), k)
+        p = dirs(th, ph, rng.uniform(10, 20, k))
+        poles = dirs(np.array([0.004, 0.01, 0.02, np.pi - 0.004, np.pi - 0.01, np.pi - 0.03]),
+                     rng.uniform(0, 2 * np.pi, 6), rng.uniform(5, 15, 6))
+        p = np.concatenate([p, poles])
+    elif case == "cone":
+        k = 300
+        th = np.deg2rad(120.0) + rng.uniform(-0.02, 0.02, k)
+        ph = np.deg2rad(200.0) + rng.uniform(-0.02, 0.02, k)
+        p = dirs(th, ph, rng.uniform(8, 30, k))
+    elif case == "five_points":
+        p = dirs(rng.uniform(0.1, np.pi - 0.1, 5), rng.uniform(0, 2 * np.pi, 5),
+                 rng.uniform(5, 30, 5))
+    else:  # "band_edge": two partial rings 1e-6 either side of band edge 20 of 2B bands
+        k = 400
+        edge = 20 * np.pi / (2 * B)
+        th = np.where(np.arange(k) % 2 == 0, edge - 1e-6, edge + 1e-6)
+        ph = rng.uniform(np.deg2rad(30.0), np.deg2rad(250.0), k)
+        p = dirs(th, ph, rng.uniform(10, 20, k))
+    inten = rng.permutation(len(p)) / len(p) + 0.001
+    return np.concatenate([p, inten[:, None]], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["arc_and_poles", "cone", "five_points", "band_edge"])
+@pytest.mark.parametrize("B", [32, 64])
+def test_s2_nearest_is_exact_on_sparse_clouds(po, B, case):
+    """The exact-minimum check above on clouds that leave most bands and
+    sectors empty: the search skips bands on their points' colatitude range
+    and on their longest run of empty sectors, and every query must still
+    sample a point at the minimum chord distance (up to the projections'
+    last-bit near-ties, as above)."""
+    from phaser_amd.capi import Context
+    cloud = _degenerate_cloud(case, B)
+    n = 2 * B
+    grid = np.asarray(po.sampler_grid(B), np.float32).reshape(n * n, -1)[:, :3]
+    with Context(bandwidth=B, n_voxels=2 * B, max_points=4096) as ctx:
+        g = ctx.sample_s2(cloud)
+    proj, rng_ = po.project(cloud)
+    near = po.nn(proj, grid, kdtree=True)
+    dmin = np.sum((grid.astype(np.float64) - proj[near].astype(np.float64)) ** 2, axis=1)
+    keys = {}
+    for i, k in enumerate(zip(rng_.astype(np.float32).tolist(), cloud[:, 3].tolist())):
+        keys.setdefault(k, []).append(i)
+    delta = 2.4e-7
+    bad = 0
+    for q in range(n * n):
+        cand = keys.get((float(np.float32(g[0][q])), float(np.float32(g[1][q]))), [])
+        d = min((float(np.sum((grid[q].astype(np.float64) - proj[i]) ** 2)) for i in cand),
+                default=np.inf)
+        tol = 2 * (2 * np.sqrt(3 * dmin[q]) * delta + 3 * delta * delta)
+        bad += not (d <= dmin[q] + tol)
+    print(f"{case} B {B}: {len(cloud)} points, {bad} of {n * n} queries off the minimum",
+          flush=True)
+    assert bad == 0
+
+
 def test_s2_transform_matches_oracle(ctx32, po):
     rng = np.random.default_rng(0)
     grids = rng.normal(size=(3, 64 * 64))
