@@ -57,9 +57,10 @@ inline int seed_factor(int n) {
 // nearest occupied sectors are already out of reach are skipped on the
 // band's occupancy bits in LDS, without a memory round trip.
 // phz_tuning.nn_seed's default: neither (measured at 64 pairs per launch on
-// gfx950, re-cast targets, with the band walk that jumps empty bands: the NN
-// stage 0.566 ms plain, 0.654 ms with seeds, 0.571 ms with the mask, 0.649 ms
-// with both)
+// gfx950, re-cast targets, with bands bounded by their points' colatitude
+// range: the NN stage 0.454 ms plain, 0.540 ms with seeds, 0.474 ms with the
+// mask; the search now also skips a band from its longest run of empty
+// sectors, the mask's main case, without the mask's bit scans)
 constexpr int kNnSeedDefault = 0;
 struct NnSeeds {
   const int* pos = nullptr;
