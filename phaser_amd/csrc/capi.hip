@@ -324,7 +324,6 @@ size_t context_bytes(const phz_config& cf) {
   add(M * nq, sizeof(int4));
   add(M * cap + kNnScanPad, sizeof(float4));
   add(M * cap, sizeof(float4));
-  add(M * cap, sizeof(float2));
   add(M * cap, sizeof(int));
   add(M * kSeedCells, sizeof(int));
   add(M * n * ((n + 63) / 64), sizeof(unsigned long long));
@@ -500,6 +499,7 @@ int init_ctx(phz_ctx* c) {
   c->cells.nb = c->n;
   c->cells.ns = c->n;
   TRY(dalloc(c, &c->d_pts, (size_t)M * cap));
+  c->cells.pts = c->d_pts;
   TRY(dalloc(c, &c->d_npts, (size_t)M));
   TRY(dalloc(c, &c->d_gather, (size_t)M));
   TRY(dalloc(c, &c->cells.cell_count, (size_t)M * (nq + 3 * c->n)));
@@ -511,7 +511,6 @@ int init_ctx(phz_ctx* c) {
   TRY(dalloc(c, &c->cells.next_left, (size_t)M * nq));
   TRY(dalloc(c, &c->cells.sorted, (size_t)M * cap + kNnScanPad));
   TRY(dalloc(c, &c->cells.proj, (size_t)M * cap));
-  TRY(dalloc(c, &c->cells.val, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.cell, (size_t)M * cap));
   TRY(dalloc(c, &c->cells.seed_pos, (size_t)M * kSeedCells));
   TRY(dalloc(c, &c->cells.band_mask, (size_t)M * c->n * ((c->n + 63) / 64)));

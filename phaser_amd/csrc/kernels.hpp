@@ -31,7 +31,8 @@ struct CellBuffers {
   int4* next_left;   // nclouds * n*n: the same to the left
   float4* sorted;    // nclouds * cap + kNnScanPad: projected x,y,z + original index bits
   float4* proj;      // nclouds * cap
-  float2* val;       // nclouds * cap: (range, intensity) by original index
+  const float4* pts;  // the clouds the cells were built from (cloud c at pts + c cap): the
+                      // search reads its samples' range and intensity there
   int* cell;         // nclouds * cap
   int* seed_pos;     // nclouds * kSeedCells: per coarse cell the sorted position
                      // of a point near its centre (-1: none)

@@ -19,16 +19,24 @@ namespace {
 
 constexpr float kPi = 3.14159265358979323846f;
 
-// spherical-projection.cc:19-41 with the reference's float/double mix and no
-// FMA contraction; acosf and atan2f round as the host libm rounds them, so
-// that the projected points are the reference's to the bit.
-__device__ __forceinline__ void project_point(float x, float y, float z, float* px, float* py,
-                                              float* pz, float* range) {
+// a point's distance from the sensor as spherical-projection.cc:22-24 forms
+// it (float xy norm, double sum, no FMA contraction); the search recomputes a
+// sample's range with this same function, so it equals the projection's
+__device__ __forceinline__ double point_dist(float x, float y, float z) {
 #pragma clang fp contract(off)
   const float dxy = sqrtf(x * x + y * y);
   const double dxyd = (double)dxy;
   const double zz = (double)(z * z);
-  const double dist = sqrt(dxyd * dxyd + zz);
+  return sqrt(dxyd * dxyd + zz);
+}
+
+// spherical-projection.cc:19-41 with the reference's float/double mix and no
+// FMA contraction; acosf and atan2f round as the host libm rounds them, so
+// that the projected points are the reference's to the bit.
+__device__ __forceinline__ void project_point(float x, float y, float z, float* px, float* py,
+                                              float* pz) {
+#pragma clang fp contract(off)
+  const double dist = point_dist(x, y, z);
   const float pzf = (float)((double)z / dist);
   const double tx = (double)libm::acosf_host_exact(pzf);
   const double ty = fmod((double)libm::atan2f_host_exact(y, x) + 2.0 * M_PI, 2.0 * M_PI);
@@ -38,7 +46,6 @@ __device__ __forceinline__ void project_point(float x, float y, float z, float* 
   *px = (float)(stx * cty);
   *py = (float)(stx * sty);
   *pz = pzf;
-  *range = (float)dist;
 }
 
 // The S^2 search grid has at most kMaxSectors bands and sectors (n = 2B,
@@ -53,7 +60,7 @@ constexpr int kProjectThreads = 256;
 
 __global__ void __launch_bounds__(kProjectThreads)
 k_project(const float4* __restrict__ pts, const int* __restrict__ npts, int64_t cap, int nb, int ns,
-          float4* __restrict__ proj, float2* __restrict__ val, int* __restrict__ cell,
+          float4* __restrict__ proj, int* __restrict__ cell,
           int* __restrict__ cell_count, unsigned* __restrict__ band_th) {
   const int c = blockIdx.y;
   // the grid covers cap points per cloud: workgroups past the cloud's
@@ -62,8 +69,8 @@ k_project(const float4* __restrict__ pts, const int* __restrict__ npts, int64_t 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < npts[c];
   const float4 p = valid ? pts[c * cap + i] : make_float4(0.f, 0.f, 1.f, 0.f);
-  float px, py, pz, r;
-  project_point(p.x, p.y, p.z, &px, &py, &pz, &r);
+  float px, py, pz;
+  project_point(p.x, p.y, p.z, &px, &py, &pz);
   // cell of the stored float point: nb colatitude bands, ns azimuth sectors
   // centred on phi_s = 2 pi s / ns
   const float th = atan2f(sqrtf(px * px + py * py), pz);
@@ -78,7 +85,6 @@ k_project(const float4* __restrict__ pts, const int* __restrict__ npts, int64_t 
   const WaveRun run = wave_run(valid ? (long long)cid : -1ll - (threadIdx.x & 63));
   if (valid) {
     proj[c * cap + i] = make_float4(px, py, pz, __int_as_float(i));
-    val[c * cap + i] = make_float2(r, p.w);
     cell[c * cap + i] = cid;
     if (run.head) atomicAdd(&cell_count[(int64_t)c * nb * ns + cid], run.len);
   }
@@ -495,7 +501,7 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
                      int nb, int ns, float max_sq, const int* __restrict__ cell_start,
                      const int* __restrict__ band_start, const int4* __restrict__ next_right,
                      const int4* __restrict__ next_left, const float4* __restrict__ sorted,
-                     const float2* __restrict__ vals, int64_t cap,
+                     const float4* __restrict__ pts, int64_t cap,
                      float* __restrict__ grids, const float2* __restrict__ extra,
                      float* __restrict__ xgrids, int xcd_blocks, int blend, NnSeeds seeds,
                      const unsigned* __restrict__ band_th,
@@ -799,14 +805,15 @@ __global__ void k_nn(const float4* __restrict__ grid, const float* __restrict__ 
   }
   if (sub == 0) {
     float r = 0.f, it = 0.f;
-    // the nearest point's values by its original index (the projection's
-    // per-point array), where the squared distance is accepted
+    // the nearest point's values from the cloud by its original index, where
+    // the squared distance is accepted: its range as the projection forms it
+    // (the same function) and its intensity
     const bool hit = bkey != ~0ull && __uint_as_float((unsigned)(bkey >> 32)) <= max_sq;
     const int oi = (int)(unsigned)bkey;
     if (hit) {
-      const float2 v = vals[c * cap + oi];
-      r = v.x;
-      it = v.y;
+      const float4 p = pts[c * cap + oi];
+      r = (float)point_dist(p.x, p.y, p.z);
+      it = p.w;
     }
     if (blend) {  // SphRegistration's one channel, spherical-correlation.cc:103-112
       grids[(int64_t)c * nq + q] = 0.6f * it + 0.4f * r;
@@ -924,6 +931,7 @@ void launch_sample_s2(const float4* pts, const int* npts, int64_t cap, int nclou
                       const float4* grid, const float* theta, float max_sq, CellBuffers cb,
                       float* grids, hipStream_t s) {
   launch_s2_cells(pts, npts, cap, nclouds, n, cb, s);
+  cb.pts = pts;
   launch_s2_nn(cap, nclouds, n, grid, theta, max_sq, cb, grids, 4, s, nullptr, nullptr, false,
                false, kNnSeedDefault);
 }
@@ -937,7 +945,7 @@ void launch_s2_cells(const float4* pts, const int* npts, int64_t cap, int ncloud
   dim3 g((unsigned)((cap + 255) / 256), nclouds);
   dim3 gp((unsigned)((cap + kProjectThreads - 1) / kProjectThreads), nclouds);
   hipLaunchKernelGGL(k_project, gp, dim3(kProjectThreads), 0, s, pts, npts, cap, cb.nb, cb.ns,
-                     cb.proj, cb.val, cb.cell, cb.cell_count, cb.band_th);
+                     cb.proj, cb.cell, cb.cell_count, cb.band_th);
   if (ncell <= 4 * 1024)
     hipLaunchKernelGGL(k_cell_scan<4>, dim3(nclouds), dim3(1024), 0, s, cb.nb, cb.ns,
                        cb.cell_count, cb.cell_start, cb.band_start);
@@ -968,7 +976,7 @@ static void launch_nn_t(int64_t cap, int nclouds, int n, const float4* grid, con
   const size_t lds = seeds.use_mask ? (size_t)cb.nb * ((cb.ns + 63) / 64) * 8 : 0;
   hipLaunchKernelGGL(k_nn<L>, gq, dim3(256), lds, s, grid, theta, n, cb.nb, cb.ns, max_sq,
                      cb.cell_start, cb.band_start, cb.next_right, cb.next_left, cb.sorted,
-                     cb.val, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
+                     cb.pts, cap, grids, extra, xgrids, by_xcd ? nqb : 0, (int)blend,
                      seeds, cb.band_th, cb.band_gap);
 }
 
